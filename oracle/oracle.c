@@ -1,0 +1,1818 @@
+/*
+ * oracle.c — plain-C restatement of quoin's block path (Rust reference).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker that GPU results are compared against.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it;
+ * the product library never does. Each function cites the reference lines it
+ * restates (paths relative to the reference repository root).
+ *
+ * Pinning (what fixes each part independently of this restatement):
+ *  - bitpack / for_bitpack / delta_bitpack / alp / float_mult / alp_rd / dict /
+ *    rle / raw / const / stride / xorz: round trips at every width and the
+ *    closed-form payload sizes (tests/test_oracle_pins.py), the reference's own
+ *    unit-test shapes and size bounds restated there.
+ *  - ALP (e,f) search and ALP-RD cut: brute-force enumeration in numpy on small
+ *    inputs (tests/test_oracle_pins.py).
+ *  - frame/header: hand-derived known-answer streams (empty, constant, ramp,
+ *    single value) and the header dtype bytes (tests/typed.rs:170-192).
+ *  - competition: ratio floors at Level::Fastest from
+ *    tests/ratio_regression.rs:40-56 (size within ~3%).
+ *  - rANS: inverse on all 1-3 symbol alphabets; normalize's tie rule against a
+ *    brute-force restatement.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* growable byte buffer                                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint8_t *p;
+    size_t len, cap;
+    int oom;
+} buf_t;
+
+static void buf_reserve(buf_t *b, size_t extra) {
+    if (b->oom) return;
+    if (b->len + extra <= b->cap) return;
+    size_t nc = b->cap ? b->cap * 2 : 256;
+    while (nc < b->len + extra) nc *= 2;
+    uint8_t *np = (uint8_t *)realloc(b->p, nc);
+    if (!np) {
+        b->oom = 1;
+        return;
+    }
+    b->p = np;
+    b->cap = nc;
+}
+static void buf_push(buf_t *b, uint8_t v) {
+    buf_reserve(b, 1);
+    if (b->oom) return;
+    b->p[b->len++] = v;
+}
+static void buf_append(buf_t *b, const void *src, size_t n) {
+    if (n == 0) return;
+    buf_reserve(b, n);
+    if (b->oom) return;
+    memcpy(b->p + b->len, src, n);
+    b->len += n;
+}
+static void buf_u64le(buf_t *b, uint64_t v) {
+    uint8_t t[8];
+    for (int i = 0; i < 8; i++) t[i] = (uint8_t)(v >> (8 * i));
+    buf_append(b, t, 8);
+}
+static void buf_u32le(buf_t *b, uint32_t v) {
+    uint8_t t[4];
+    for (int i = 0; i < 4; i++) t[i] = (uint8_t)(v >> (8 * i));
+    buf_append(b, t, 4);
+}
+static void buf_u16le(buf_t *b, uint16_t v) {
+    buf_push(b, (uint8_t)v);
+    buf_push(b, (uint8_t)(v >> 8));
+}
+static void buf_free(buf_t *b) {
+    free(b->p);
+    b->p = NULL;
+    b->len = b->cap = 0;
+}
+static uint64_t rd_u64le(const uint8_t *p) {
+    uint64_t v = 0;
+    for (int i = 0; i < 8; i++) v |= (uint64_t)p[i] << (8 * i);
+    return v;
+}
+static uint32_t rd_u32le(const uint8_t *p) {
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
+           ((uint32_t)p[3] << 24);
+}
+
+/* varint.rs:9-21 */
+static void varint_write(buf_t *b, uint64_t v) {
+    for (;;) {
+        uint8_t byte = (uint8_t)(v & 0x7f);
+        v >>= 7;
+        if (v != 0) {
+            buf_push(b, byte | 0x80);
+        } else {
+            buf_push(b, byte);
+            return;
+        }
+    }
+}
+static size_t varint_len(uint64_t v) {
+    size_t n = 1;
+    while (v >= 0x80) {
+        v >>= 7;
+        n++;
+    }
+    return n;
+}
+/* varint.rs:24-39 */
+static int varint_read(const uint8_t *in, size_t len, size_t *pos, uint64_t *out) {
+    uint64_t result = 0;
+    uint32_t shift = 0;
+    for (;;) {
+        if (*pos >= len) return ORC_TRUNCATED;
+        uint8_t byte = in[(*pos)++];
+        result |= (uint64_t)(byte & 0x7f) << shift;
+        if ((byte & 0x80) == 0) {
+            *out = result;
+            return ORC_OK;
+        }
+        shift += 7;
+        if (shift >= 64) return ORC_CORRUPT;
+    }
+}
+
+static uint32_t lz64(uint64_t x) { return x ? (uint32_t)__builtin_clzll(x) : 64u; }
+static uint32_t width_of(uint64_t range) { return range == 0 ? 0 : 64 - lz64(range); }
+
+/* Rust `f64 as i64`: truncation toward zero, saturating, NaN -> 0. */
+static int64_t f64_to_i64_sat(double x) {
+    if (x != x) return 0;
+    if (x >= 9223372036854775808.0) return INT64_MAX;
+    if (x < -9223372036854775808.0) return INT64_MIN;
+    return (int64_t)x;
+}
+static double bits_f64(uint64_t b) {
+    double d;
+    memcpy(&d, &b, 8);
+    return d;
+}
+static uint64_t f64_bits(double d) {
+    uint64_t b;
+    memcpy(&b, &d, 8);
+    return b;
+}
+
+/* ------------------------------------------------------------------------ */
+/* modes, dtypes, levels                                                     */
+/* ------------------------------------------------------------------------ */
+enum {
+    M_PRED = 0, M_CONST = 1, M_STRIDE = 2, M_XORZ = 3, M_LZ = 4, M_RAW = 5,
+    M_ORDERED_DELTA = 7, M_ALP = 9, M_BYTE_TRANSPOSE = 11, M_FLOAT_MULT = 17,
+    M_DICT = 19, M_DELTA2 = 20, M_PRED2 = 25, M_PRED_RC = 29, M_DELTA_DP = 45,
+    M_FOR_BITPACK = 50, M_DELTA_BITPACK = 51, M_ALP_RD = 52, M_RLE = 53, M_PCO = 54
+};
+
+/* dtype.rs:95-124 */
+static int dtype_is_float(int dt) { return dt == ORC_F64 || dt == ORC_F32; }
+static size_t dtype_lane_bytes(int dt) {
+    return (dt == ORC_I32 || dt == ORC_U32 || dt == ORC_F32) ? 4 : 8;
+}
+static int dtype_signed(int dt) { return dt == ORC_I32 || dt == ORC_I64; }
+static size_t dtype_width(int dt) { return dtype_lane_bytes(dt); }
+
+/* encoder.rs:79-89 */
+static uint64_t decode_weight(int mode) {
+    switch (mode) {
+    case M_RAW: case M_CONST: case M_STRIDE: return 0;
+    case M_XORZ: case M_FOR_BITPACK: case M_DELTA_BITPACK: case M_RLE: return 1;
+    case M_ALP: case M_ALP_RD: case M_DICT: return 2;
+    case M_PRED: return 3;
+    case M_BYTE_TRANSPOSE: case M_FLOAT_MULT: return 6;
+    case M_ORDERED_DELTA: case M_DELTA2: case M_DELTA_DP: case M_LZ: case M_PCO: return 7;
+    default: return 10; /* PredRc, Pred2 */
+    }
+}
+/* lib.rs:127-135 */
+static uint64_t level_lambda(int level) {
+    switch (level) {
+    case ORC_FASTEST: return 16;
+    case ORC_FAST: return 4;
+    case ORC_BALANCED: return 2;
+    default: return 0;
+    }
+}
+static uint64_t sat_mul(uint64_t a, uint64_t b) {
+    unsigned __int128 p = (unsigned __int128)a * b;
+    return p > UINT64_MAX ? UINT64_MAX : (uint64_t)p;
+}
+/* encoder.rs:94-97 */
+static size_t penalty(int mode, uint64_t lambda, size_t decoded_bytes) {
+    return (size_t)(sat_mul(sat_mul(lambda, decode_weight(mode)), decoded_bytes) >> 8);
+}
+
+static uint64_t g_wins[64];
+void orc_mode_wins(uint64_t out[64]) { memcpy(out, g_wins, sizeof g_wins); }
+void orc_reset_mode_wins(void) { memset(g_wins, 0, sizeof g_wins); }
+
+/* ------------------------------------------------------------------------ */
+/* FastLanes bit-packing (bitpack.rs:47-161)                                  */
+/* ------------------------------------------------------------------------ */
+#define BLOCK 1024
+
+void orc_pack32(const uint32_t *values, uint32_t width, uint32_t *out) {
+    memset(out, 0, sizeof(uint32_t) * 32 * width);
+    if (width == 0) return;
+    uint32_t mask = width >= 32 ? 0xFFFFFFFFu : ((1u << width) - 1);
+    for (uint32_t r = 0; r < 32; r++) {
+        uint32_t bit_off = r * width, word = bit_off / 32, shift = bit_off % 32;
+        for (uint32_t l = 0; l < 32; l++) {
+            uint32_t v = values[r * 32 + l] & mask;
+            out[word * 32 + l] |= v << shift;
+            if (shift + width > 32) out[(word + 1) * 32 + l] |= v >> (32 - shift);
+        }
+    }
+}
+void orc_unpack32(const uint32_t *packed, uint32_t width, uint32_t *out) {
+    if (width == 0) {
+        memset(out, 0, 4 * BLOCK);
+        return;
+    }
+    uint32_t mask = width >= 32 ? 0xFFFFFFFFu : ((1u << width) - 1);
+    for (uint32_t r = 0; r < 32; r++) {
+        uint32_t bit_off = r * width, word = bit_off / 32, shift = bit_off % 32;
+        for (uint32_t l = 0; l < 32; l++) {
+            uint32_t lo = packed[word * 32 + l];
+            if (shift + width > 32) {
+                uint32_t hi = packed[(word + 1) * 32 + l];
+                out[r * 32 + l] = ((lo >> shift) | (hi << (32 - shift))) & mask;
+            } else {
+                out[r * 32 + l] = (lo >> shift) & mask;
+            }
+        }
+    }
+}
+void orc_pack64(const uint64_t *values, uint32_t width, uint64_t *out) {
+    memset(out, 0, sizeof(uint64_t) * 16 * width);
+    if (width == 0) return;
+    uint64_t mask = width >= 64 ? ~0ull : ((1ull << width) - 1);
+    for (uint32_t r = 0; r < 64; r++) {
+        uint32_t bit_off = r * width, word = bit_off / 64, shift = bit_off % 64;
+        for (uint32_t l = 0; l < 16; l++) {
+            uint64_t v = values[r * 16 + l] & mask;
+            out[word * 16 + l] |= v << shift;
+            if (shift + width > 64) out[(word + 1) * 16 + l] |= v >> (64 - shift);
+        }
+    }
+}
+void orc_unpack64(const uint64_t *packed, uint32_t width, uint64_t *out) {
+    if (width == 0) {
+        memset(out, 0, 8 * BLOCK);
+        return;
+    }
+    uint64_t mask = width >= 64 ? ~0ull : ((1ull << width) - 1);
+    for (uint32_t r = 0; r < 64; r++) {
+        uint32_t bit_off = r * width, word = bit_off / 64, shift = bit_off % 64;
+        for (uint32_t l = 0; l < 16; l++) {
+            uint64_t lo = packed[word * 16 + l];
+            if (shift + width > 64) {
+                uint64_t hi = packed[(word + 1) * 16 + l];
+                out[r * 16 + l] = ((lo >> shift) | (hi << (64 - shift))) & mask;
+            } else {
+                out[r * 16 + l] = (lo >> shift) & mask;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* FOR_BITPACK (codecs/for_bitpack.rs:25-157)                                */
+/* ------------------------------------------------------------------------ */
+static void for_encode_subblock(const uint64_t *sub, size_t cnt, int sgn, buf_t *out) {
+    uint64_t mn, range;
+    if (sgn) {
+        int64_t a = INT64_MAX, b = INT64_MIN;
+        for (size_t k = 0; k < cnt; k++) {
+            int64_t s = (int64_t)sub[k];
+            if (s < a) a = s;
+            if (s > b) b = s;
+        }
+        mn = (uint64_t)a;
+        range = (uint64_t)b - (uint64_t)a;
+    } else {
+        uint64_t a = UINT64_MAX, b = 0;
+        for (size_t k = 0; k < cnt; k++) {
+            if (sub[k] < a) a = sub[k];
+            if (sub[k] > b) b = sub[k];
+        }
+        mn = a;
+        range = b - a;
+    }
+    uint32_t width = width_of(range);
+    buf_push(out, (uint8_t)width);
+    buf_u64le(out, mn);
+    if (width == 0) return;
+    if (width <= 32) {
+        uint32_t res[BLOCK] = {0}, packed[32 * 32];
+        for (size_t k = 0; k < cnt; k++) res[k] = (uint32_t)(sub[k] - mn);
+        orc_pack32(res, width, packed);
+        for (uint32_t i = 0; i < 32 * width; i++) buf_u32le(out, packed[i]);
+    } else {
+        uint64_t res[BLOCK] = {0}, packed[16 * 64];
+        for (size_t k = 0; k < cnt; k++) res[k] = sub[k] - mn;
+        orc_pack64(res, width, packed);
+        for (uint32_t i = 0; i < 16 * width; i++) buf_u64le(out, packed[i]);
+    }
+}
+static void for_encode(const uint64_t *vals, size_t n, int sgn, buf_t *out) {
+    varint_write(out, n);
+    for (size_t i = 0; i < n; i += BLOCK) {
+        size_t end = i + BLOCK < n ? i + BLOCK : n;
+        for_encode_subblock(vals + i, end - i, sgn, out);
+    }
+}
+static int for_decode(const uint8_t *p, size_t len, size_t n_values, uint64_t *out) {
+    size_t pos = 0;
+    uint64_t n;
+    int rc = varint_read(p, len, &pos, &n);
+    if (rc) return rc;
+    if (n != n_values) return ORC_CORRUPT;
+    size_t done = 0;
+    while (done < n) {
+        if (pos >= len) return ORC_TRUNCATED;
+        uint32_t width = p[pos++];
+        if (width > 64) return ORC_CORRUPT;
+        size_t count = n - done < BLOCK ? n - done : BLOCK;
+        if (len - pos < 8) return ORC_TRUNCATED;
+        uint64_t mn = rd_u64le(p + pos);
+        pos += 8;
+        if (width == 0) {
+            for (size_t k = 0; k < count; k++) out[done + k] = mn;
+        } else if (width <= 32) {
+            size_t nb = 128 * (size_t)width;
+            if (len - pos < nb) return ORC_TRUNCATED;
+            uint32_t packed[32 * 32], res[BLOCK];
+            for (size_t k = 0; k < 32 * (size_t)width; k++) packed[k] = rd_u32le(p + pos + 4 * k);
+            pos += nb;
+            orc_unpack32(packed, width, res);
+            /* unresidual: signed and unsigned are the same wrapping add */
+            for (size_t k = 0; k < count; k++) out[done + k] = mn + (uint64_t)res[k];
+        } else {
+            size_t nb = 128 * (size_t)width;
+            if (len - pos < nb) return ORC_TRUNCATED;
+            uint64_t packed[16 * 64], res[BLOCK];
+            for (size_t k = 0; k < 16 * (size_t)width; k++) packed[k] = rd_u64le(p + pos + 8 * k);
+            pos += nb;
+            orc_unpack64(packed, width, res);
+            for (size_t k = 0; k < count; k++) out[done + k] = mn + res[k];
+        }
+        done += count;
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* DELTA_BITPACK (codecs/delta_bitpack.rs:23-64)                             */
+/* ------------------------------------------------------------------------ */
+static uint64_t zigzag(uint64_t n) { return (n << 1) ^ (uint64_t)((int64_t)n >> 63); }
+static uint64_t unzigzag(uint64_t z) { return (z >> 1) ^ (0 - (z & 1)); }
+
+static void delta_encode(const uint64_t *vals, size_t n, buf_t *out) {
+    uint64_t base = n ? vals[0] : 0;
+    buf_u64le(out, base);
+    uint64_t *d = (uint64_t *)malloc(sizeof(uint64_t) * (n ? n : 1));
+    uint64_t prev = base;
+    for (size_t i = 0; i < n; i++) {
+        d[i] = zigzag(vals[i] - prev);
+        prev = vals[i];
+    }
+    for_encode(d, n, 0, out);
+    free(d);
+}
+static int delta_decode(const uint8_t *p, size_t len, size_t n, uint64_t *out) {
+    if (len < 8) return ORC_TRUNCATED;
+    uint64_t base = rd_u64le(p);
+    int rc = for_decode(p + 8, len - 8, n, out);
+    if (rc) return rc;
+    uint64_t prev = base;
+    for (size_t i = 0; i < n; i++) {
+        prev += unzigzag(out[i]);
+        out[i] = prev;
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* RAW / CONST / STRIDE / XORZ                                                */
+/* ------------------------------------------------------------------------ */
+/* raw.rs:12-27 */
+static void raw_encode(const uint64_t *vals, size_t n, int dt, buf_t *out) {
+    if (dt == ORC_F32) {
+        for (size_t i = 0; i < n; i++) {
+            float f = (float)bits_f64(vals[i]);
+            uint32_t b;
+            memcpy(&b, &f, 4);
+            buf_u32le(out, b);
+        }
+        return;
+    }
+    size_t lb = dtype_lane_bytes(dt);
+    for (size_t i = 0; i < n; i++) {
+        uint8_t t[8];
+        for (int k = 0; k < 8; k++) t[k] = (uint8_t)(vals[i] >> (8 * k));
+        buf_append(out, t, lb);
+    }
+}
+/* raw.rs:29-48 */
+static int raw_decode(const uint8_t *p, size_t len, size_t n, int dt, uint64_t *out) {
+    size_t lb = dtype_lane_bytes(dt);
+    if (len != n * lb) return ORC_CORRUPT;
+    for (size_t i = 0; i < n; i++) {
+        if (dt == ORC_F32) {
+            uint32_t b = rd_u32le(p + 4 * i);
+            float f;
+            memcpy(&f, &b, 4);
+            out[i] = f64_bits((double)f);
+        } else if (lb == 4) {
+            out[i] = rd_u32le(p + 4 * i);
+        } else {
+            out[i] = rd_u64le(p + 8 * i);
+        }
+    }
+    return ORC_OK;
+}
+/* const_block.rs:5-12 */
+static int const_encode(const uint64_t *v, size_t n, buf_t *out) {
+    if (n == 0) return 0;
+    for (size_t i = 1; i < n; i++)
+        if (v[i] != v[0]) return 0;
+    buf_u64le(out, v[0]);
+    return 1;
+}
+/* stride.rs:6-22 */
+static int stride_encode(const uint64_t *v, size_t n, buf_t *out) {
+    if (n < 2) return 0;
+    uint64_t first = v[0], stride = v[1] - v[0];
+    for (size_t i = 0; i < n; i++)
+        if (v[i] != first + (uint64_t)i * stride) return 0;
+    buf_u64le(out, first);
+    buf_u64le(out, stride);
+    return 1;
+}
+/* xorz.rs:8-16 */
+static void xorz_encode(const uint64_t *v, size_t n, buf_t *out) {
+    uint64_t prev = 0;
+    for (size_t i = 0; i < n; i++) {
+        varint_write(out, v[i] ^ prev);
+        prev = v[i];
+    }
+}
+/* xorz.rs:18-31 */
+static int xorz_decode(const uint8_t *p, size_t len, size_t n, uint64_t *out) {
+    size_t pos = 0;
+    uint64_t prev = 0;
+    for (size_t i = 0; i < n; i++) {
+        uint64_t x;
+        int rc = varint_read(p, len, &pos, &x);
+        if (rc) return rc;
+        prev ^= x;
+        out[i] = prev;
+    }
+    return pos == len ? ORC_OK : ORC_CORRUPT;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ALP (codecs/alp.rs:18-248) — FP64 with no contraction (-ffp-contract=off)  */
+/* ------------------------------------------------------------------------ */
+#define ALP_MAX_EXP 18
+static const double EXP10[19] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,  1e8, 1e9,
+                                 1e10, 1e11, 1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18};
+static const double FRAC10[19] = {1e0,   1e-1,  1e-2,  1e-3,  1e-4,  1e-5,  1e-6,
+                                  1e-7,  1e-8,  1e-9,  1e-10, 1e-11, 1e-12, 1e-13,
+                                  1e-14, 1e-15, 1e-16, 1e-17, 1e-18};
+static const double ALP_MAGIC = 6755399441055744.0;
+static const double ALP_UPPER = 9.223372036854776e18;
+
+/* alp.rs:33-40; returns 0 when the value has no digit */
+static int alp_encode_value(double v, int e, int f, int64_t *d) {
+    volatile double t1 = v * EXP10[e];
+    volatile double tmp = t1 * FRAC10[f];
+    if (!isfinite(tmp) || tmp >= ALP_UPPER || tmp <= -ALP_UPPER) return 0;
+    volatile double r1 = tmp + ALP_MAGIC;
+    volatile double r2 = r1 - ALP_MAGIC;
+    *d = f64_to_i64_sat(r2);
+    return 1;
+}
+/* alp.rs:42-45 */
+static double alp_decode_value(int64_t d, int e, int f) {
+    volatile double a = (double)d * EXP10[f];
+    volatile double b = a * FRAC10[e];
+    return b;
+}
+/* alp.rs:49-91 */
+static int alp_find_ef(const uint64_t *sub, size_t n, int *be, int *bf) {
+    size_t stride = n / 32 > 1 ? n / 32 : 1;
+    double sample[32];
+    size_t sn = 0;
+    for (size_t i = 0; i < n && sn < 32; i += stride) sample[sn++] = bits_f64(sub[i]);
+    size_t best_cost = SIZE_MAX, best_exc = sn;
+    int e0 = 0, f0 = 0;
+    for (int e = 0; e <= ALP_MAX_EXP; e++) {
+        for (int f = 0; f <= e; f++) {
+            size_t exc = 0;
+            int64_t lo = INT64_MAX, hi = INT64_MIN;
+            for (size_t k = 0; k < sn; k++) {
+                int64_t d;
+                if (alp_encode_value(sample[k], e, f, &d) &&
+                    f64_bits(alp_decode_value(d, e, f)) == f64_bits(sample[k])) {
+                    if (d < lo) lo = d;
+                    if (d > hi) hi = d;
+                } else {
+                    exc++;
+                }
+            }
+            size_t width;
+            if (exc == sn || hi <= lo)
+                width = exc == sn ? 64 : 0;
+            else
+                width = width_of((uint64_t)hi - (uint64_t)lo);
+            size_t cost = width * sn + exc * 80;
+            if (cost < best_cost) {
+                best_cost = cost;
+                e0 = e;
+                f0 = f;
+                best_exc = exc;
+            }
+        }
+    }
+    if (best_exc * 2 > sn) return 0;
+    *be = e0;
+    *bf = f0;
+    return 1;
+}
+/* alp.rs:108-166 */
+static int alp_encode_subblock(const uint64_t *sub, size_t cnt, buf_t *out) {
+    int e, f;
+    if (!alp_find_ef(sub, cnt, &e, &f)) return 0;
+    int64_t digits[BLOCK];
+    uint16_t exc_pos[BLOCK];
+    size_t nexc = 0;
+    int have_first = 0;
+    int64_t first_valid = 0;
+    for (size_t k = 0; k < cnt; k++) {
+        double v = bits_f64(sub[k]);
+        int64_t d;
+        if (alp_encode_value(v, e, f, &d) && f64_bits(alp_decode_value(d, e, f)) == sub[k]) {
+            digits[k] = d;
+            if (!have_first) {
+                have_first = 1;
+                first_valid = d;
+            }
+        } else {
+            exc_pos[nexc++] = (uint16_t)k;
+        }
+    }
+    int64_t filler = have_first ? first_valid : 0;
+    for (size_t j = 0; j < nexc; j++) digits[exc_pos[j]] = filler;
+    for (size_t k = cnt; k < BLOCK; k++) digits[k] = filler;
+    int64_t mn = digits[0], mx = digits[0];
+    for (size_t k = 1; k < BLOCK; k++) {
+        if (digits[k] < mn) mn = digits[k];
+        if (digits[k] > mx) mx = digits[k];
+    }
+    uint32_t width = width_of((uint64_t)mx - (uint64_t)mn);
+    if (width > 32) return 0;
+    buf_push(out, (uint8_t)e);
+    buf_push(out, (uint8_t)f);
+    buf_u16le(out, (uint16_t)nexc);
+    buf_u64le(out, (uint64_t)mn);
+    buf_push(out, (uint8_t)width);
+    if (width > 0) {
+        uint32_t res[BLOCK], packed[32 * 32];
+        for (size_t k = 0; k < BLOCK; k++) res[k] = (uint32_t)((uint64_t)digits[k] - (uint64_t)mn);
+        orc_pack32(res, width, packed);
+        for (uint32_t i = 0; i < 32 * width; i++) buf_u32le(out, packed[i]);
+    }
+    for (size_t j = 0; j < nexc; j++) {
+        buf_u16le(out, exc_pos[j]);
+        buf_u64le(out, sub[exc_pos[j]]);
+    }
+    return 1;
+}
+/* alp.rs:93-106 */
+static int alp_encode(const uint64_t *v, size_t n, buf_t *out) {
+    if (n == 0) return 0;
+    varint_write(out, n);
+    for (size_t i = 0; i < n; i += BLOCK) {
+        size_t end = i + BLOCK < n ? i + BLOCK : n;
+        if (!alp_encode_subblock(v + i, end - i, out)) return 0;
+    }
+    return 1;
+}
+/* alp.rs:168-248 */
+static int alp_decode(const uint8_t *p, size_t len, size_t n_values, uint64_t *out) {
+    size_t pos = 0;
+    uint64_t n;
+    int rc = varint_read(p, len, &pos, &n);
+    if (rc) return rc;
+    if (n != n_values) return ORC_CORRUPT;
+    size_t done = 0;
+    while (done < n) {
+        if (len < 13 || pos > len - 13) {
+            /* mirror the reference's per-field truncation checks */
+            if (pos >= len || pos + 1 >= len) return ORC_TRUNCATED;
+            int e = p[pos], f = p[pos + 1];
+            if (e > ALP_MAX_EXP || f > e) return ORC_CORRUPT;
+            return ORC_TRUNCATED;
+        }
+        int e = p[pos], f = p[pos + 1];
+        if (e > ALP_MAX_EXP || f > e) return ORC_CORRUPT;
+        size_t exc = (size_t)p[pos + 2] | ((size_t)p[pos + 3] << 8);
+        int64_t mn = (int64_t)rd_u64le(p + pos + 4);
+        uint32_t width = p[pos + 12];
+        pos += 13;
+        if (width > 32) return ORC_CORRUPT;
+        size_t count = n - done < BLOCK ? n - done : BLOCK;
+        uint32_t res[BLOCK];
+        if (width > 0) {
+            size_t nb = 128 * (size_t)width;
+            if (len - pos < nb) return ORC_TRUNCATED;
+            uint32_t packed[32 * 32];
+            for (size_t k = 0; k < 32 * (size_t)width; k++) packed[k] = rd_u32le(p + pos + 4 * k);
+            pos += nb;
+            orc_unpack32(packed, width, res);
+        } else {
+            memset(res, 0, sizeof res);
+        }
+        for (size_t k = 0; k < count; k++) {
+            int64_t d = (int64_t)((uint64_t)mn + (uint64_t)res[k]);
+            out[done + k] = f64_bits(alp_decode_value(d, e, f));
+        }
+        for (size_t j = 0; j < exc; j++) {
+            if (len - pos < 10) return ORC_TRUNCATED;
+            size_t ep = (size_t)p[pos] | ((size_t)p[pos + 1] << 8);
+            uint64_t bits = rd_u64le(p + pos + 2);
+            pos += 10;
+            if (ep >= count) return ORC_CORRUPT;
+            out[done + ep] = bits;
+        }
+        done += count;
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FLOAT_MULT, bit-pack tag only at the non-entropy levels                    */
+/* (codecs/float_mult.rs:24-98)                                               */
+/* ------------------------------------------------------------------------ */
+static const double FM_SCALES[7] = {10.0, 100.0, 1000.0, 10000.0, 100000.0, 1000000.0, 10000000.0};
+
+/* float_mult.rs:45-64 */
+static int fm_find_scale(const uint64_t *v, size_t n, int *idx, int64_t *ks) {
+    for (int s = 0; s < 7; s++) {
+        double scale = FM_SCALES[s];
+        int ok = 1;
+        for (size_t i = 0; i < n; i++) {
+            double x = bits_f64(v[i]);
+            volatile double prod = x * scale;
+            double k = round(prod);
+            if (!isfinite(k) || fabs(k) >= 9.0e18) {
+                ok = 0;
+                break;
+            }
+            int64_t ki = f64_to_i64_sat(k);
+            volatile double back = (double)ki / scale;
+            if (f64_bits(back) != v[i]) {
+                ok = 0;
+                break;
+            }
+            ks[i] = ki;
+        }
+        if (ok) {
+            *idx = s;
+            return 1;
+        }
+    }
+    return 0;
+}
+static int fm_encode(const uint64_t *v, size_t n, buf_t *out) {
+    if (n == 0) return 0;
+    int64_t *ks = (int64_t *)malloc(sizeof(int64_t) * n);
+    int idx;
+    if (!fm_find_scale(v, n, &idx, ks)) {
+        free(ks);
+        return 0;
+    }
+    buf_push(out, 0); /* FM_BITPACK */
+    buf_push(out, (uint8_t)idx);
+    for_encode((const uint64_t *)ks, n, 1, out);
+    free(ks);
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* rANS (entropy/rans.rs:17-175)                                             */
+/* ------------------------------------------------------------------------ */
+#define RANS_M 4096u
+#define RANS_L (1u << 23)
+
+/* rans.rs:25-50. Rust max_by_key returns the LAST maximum on ties. */
+static void rans_normalize(const uint32_t counts[256], uint64_t total, uint32_t freq[256]) {
+    uint32_t sum = 0;
+    for (int s = 0; s < 256; s++) {
+        freq[s] = 0;
+        if (counts[s] > 0) {
+            uint32_t f = (uint32_t)(((uint64_t)counts[s] * RANS_M) / total);
+            freq[s] = f > 1 ? f : 1;
+            sum += freq[s];
+        }
+    }
+    while (sum > RANS_M) {
+        int best = -1;
+        for (int s = 0; s < 256; s++)
+            if (freq[s] > 1 && (best < 0 || freq[s] >= freq[best])) best = s;
+        freq[best]--;
+        sum--;
+    }
+    if (sum < RANS_M) {
+        int best = 0;
+        for (int s = 0; s < 256; s++)
+            if (freq[s] >= freq[best]) best = s;
+        freq[best] += RANS_M - sum;
+    }
+}
+uint8_t *orc_rans_compress(const uint8_t *data, size_t n, size_t *len) {
+    *len = 0;
+    if (n == 0) return NULL;
+    uint32_t counts[256] = {0}, freq[256], cum[257];
+    for (size_t i = 0; i < n; i++) counts[data[i]]++;
+    rans_normalize(counts, n, freq);
+    cum[0] = 0;
+    for (int s = 0; s < 256; s++) cum[s + 1] = cum[s] + freq[s];
+    size_t cap = 2 * n + 16 + 64;
+    uint8_t *scratch = (uint8_t *)malloc(cap);
+    size_t p = cap;
+    uint32_t x[4] = {RANS_L, RANS_L, RANS_L, RANS_L};
+    for (size_t i = n; i-- > 0;) {
+        uint32_t s = data[i], f = freq[s], c = cum[s];
+        size_t idx = i & 3;
+        uint32_t xi = x[idx];
+        uint32_t x_max = ((RANS_L >> 12) << 8) * f;
+        while (xi >= x_max) {
+            scratch[--p] = (uint8_t)xi;
+            xi >>= 8;
+        }
+        x[idx] = ((xi / f) << 12) + (xi % f) + c;
+    }
+    for (int idx = 3; idx >= 0; idx--) {
+        p -= 4;
+        for (int k = 0; k < 4; k++) scratch[p + k] = (uint8_t)(x[idx] >> (8 * k));
+    }
+    buf_t out = {0};
+    varint_write(&out, n);
+    size_t present = 0;
+    for (int s = 0; s < 256; s++) present += freq[s] > 0;
+    varint_write(&out, present);
+    for (int s = 0; s < 256; s++)
+        if (freq[s] > 0) {
+            buf_push(&out, (uint8_t)s);
+            varint_write(&out, freq[s]);
+        }
+    buf_append(&out, scratch + p, cap - p);
+    free(scratch);
+    *len = out.len;
+    return out.p;
+}
+int orc_rans_decompress(const uint8_t *pl, size_t len, size_t max_len, uint8_t **outp,
+                        size_t *out_len) {
+    size_t pos = 0;
+    uint64_t n, nsym;
+    int rc = varint_read(pl, len, &pos, &n);
+    if (rc) return rc;
+    if (n > max_len) return ORC_CORRUPT;
+    rc = varint_read(pl, len, &pos, &nsym);
+    if (rc) return rc;
+    if (nsym == 0 || nsym > 256) return ORC_CORRUPT;
+    uint32_t freq[256] = {0}, cum[257];
+    uint32_t sum = 0;
+    for (uint64_t k = 0; k < nsym; k++) {
+        if (pos >= len) return ORC_TRUNCATED;
+        uint8_t s = pl[pos++];
+        uint64_t f;
+        rc = varint_read(pl, len, &pos, &f);
+        if (rc) return rc;
+        if (f == 0 || f > RANS_M) return ORC_CORRUPT;
+        freq[s] = (uint32_t)f;
+        sum += (uint32_t)f;
+    }
+    if (sum != RANS_M) return ORC_CORRUPT;
+    cum[0] = 0;
+    for (int s = 0; s < 256; s++) cum[s + 1] = cum[s] + freq[s];
+    static __thread uint8_t slot_to_sym[RANS_M];
+    for (int s = 0; s < 256; s++)
+        for (uint32_t sl = cum[s]; sl < cum[s + 1]; sl++) slot_to_sym[sl] = (uint8_t)s;
+    uint32_t x[4];
+    for (int i = 0; i < 4; i++) {
+        if (len - pos < 4 || pos > len) return ORC_TRUNCATED;
+        x[i] = rd_u32le(pl + pos);
+        pos += 4;
+    }
+    uint8_t *out = (uint8_t *)malloc(n ? n : 1);
+    for (size_t i = 0; i < n; i++) {
+        size_t idx = i & 3;
+        uint32_t xi = x[idx], slot = xi & (RANS_M - 1);
+        uint8_t s = slot_to_sym[slot];
+        out[i] = s;
+        uint32_t xn = freq[s] * (xi >> 12) + slot - cum[s];
+        while (xn < RANS_L) {
+            if (pos >= len) {
+                free(out);
+                return ORC_TRUNCATED;
+            }
+            xn = (xn << 8) | pl[pos++];
+        }
+        x[idx] = xn;
+    }
+    *outp = out;
+    *out_len = n;
+    return ORC_OK;
+}
+
+/* Tag layer decode (entropy/mod.rs:153-178) restricted to TAG_RANS; range
+ * coder, tANS and the LZ cascade streams are out of this oracle's scope. */
+static int decode_residuals(const uint8_t *p, size_t len, size_t max_len, uint8_t **out,
+                            size_t *olen) {
+    if (len == 0) return ORC_TRUNCATED;
+    if (p[0] == 2) return orc_rans_decompress(p + 1, len - 1, max_len, out, olen);
+    if (p[0] <= 3) return ORC_UNSUPPORTED;
+    return ORC_CORRUPT;
+}
+
+static int fm_decode(const uint8_t *p, size_t len, size_t n, uint64_t *out) {
+    if (len < 1) return ORC_TRUNCATED;
+    if (len < 2) return ORC_TRUNCATED;
+    uint8_t tag = p[0], idx = p[1];
+    if (idx >= 7) return ORC_CORRUPT;
+    double scale = FM_SCALES[idx];
+    if (tag == 0) {
+        int rc = for_decode(p + 2, len - 2, n, out);
+        if (rc) return rc;
+    } else if (tag == 1) {
+        uint8_t *delta;
+        size_t dl;
+        int rc = decode_residuals(p + 2, len - 2, n * 10 + 16, &delta, &dl);
+        if (rc) return rc;
+        size_t pos = 0;
+        int64_t prev = 0;
+        for (size_t i = 0; i < n; i++) {
+            uint64_t z;
+            rc = varint_read(delta, dl, &pos, &z);
+            if (rc) {
+                free(delta);
+                return rc;
+            }
+            prev = (int64_t)((uint64_t)prev + unzigzag(z));
+            out[i] = (uint64_t)prev;
+        }
+        free(delta);
+        if (pos != dl) return ORC_CORRUPT;
+    } else {
+        return ORC_CORRUPT;
+    }
+    for (size_t i = 0; i < n; i++) {
+        volatile double q = (double)(int64_t)out[i] / scale;
+        out[i] = f64_bits(q);
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ALP-RD (codecs/alp_rd.rs:20-285), bit-pack tags at the non-entropy levels  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint64_t key;
+    uint32_t cnt;
+} kc_t;
+static int cmp_u64(const void *a, const void *b) {
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : x > y;
+}
+/* (count desc, key asc): alp_rd.rs:58-61 */
+static int cmp_kc(const void *a, const void *b) {
+    const kc_t *x = (const kc_t *)a, *y = (const kc_t *)b;
+    if (x->cnt != y->cnt) return x->cnt > y->cnt ? -1 : 1;
+    return x->key < y->key ? -1 : x->key > y->key;
+}
+/* left_freq + top_dict (alp_rd.rs:48-69) via a sort: returns dict (<=8) sorted
+ * by (count desc, key asc) and the covered count. */
+static size_t top_dict(const uint64_t *vals, size_t n, size_t step, uint32_t right_bw,
+                       uint64_t dict[8], uint64_t *covered) {
+    size_t m = (n + step - 1) / step;
+    uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * (m ? m : 1));
+    size_t j = 0;
+    for (size_t i = 0; i < n; i += step) keys[j++] = right_bw >= 64 ? 0 : vals[i] >> right_bw;
+    qsort(keys, m, 8, cmp_u64);
+    kc_t *kc = (kc_t *)malloc(sizeof(kc_t) * (m ? m : 1));
+    size_t nk = 0;
+    for (size_t i = 0; i < m;) {
+        size_t k = i;
+        while (k < m && keys[k] == keys[i]) k++;
+        kc[nk].key = keys[i];
+        kc[nk].cnt = (uint32_t)(k - i);
+        nk++;
+        i = k;
+    }
+    qsort(kc, nk, sizeof(kc_t), cmp_kc);
+    size_t dl = nk < 8 ? nk : 8;
+    uint64_t cov = 0;
+    for (size_t i = 0; i < dl; i++) {
+        dict[i] = kc[i].key;
+        cov += kc[i].cnt;
+    }
+    *covered = cov;
+    free(keys);
+    free(kc);
+    return dl;
+}
+/* alp_rd.rs:72-78 */
+static uint32_t rd_code_bits(size_t dict_len) {
+    if (dict_len <= 1) return 1;
+    uint32_t b = 64 - lz64((uint64_t)(dict_len - 1));
+    return b > 1 ? b : 1;
+}
+/* alp_rd.rs:80-107 */
+static int rd_choose_cut(const uint64_t *v, size_t n, uint32_t *cut) {
+    size_t step = n / 2048 > 1 ? n / 2048 : 1;
+    uint64_t sn = (n + step - 1) / step;
+    uint32_t best_cut = 0;
+    double best_bits = INFINITY;
+    for (uint32_t left_bw = 1; left_bw <= 24; left_bw++) {
+        uint32_t right_bw = 64 - left_bw;
+        uint64_t dict[8], covered;
+        size_t dl = top_dict(v, n, step, right_bw, dict, &covered);
+        uint64_t exc = sn - covered;
+        double per_value =
+            (double)(sn * (uint64_t)(right_bw + rd_code_bits(dl)) + exc * 96) / (double)sn;
+        if (per_value < best_bits) {
+            best_bits = per_value;
+            best_cut = left_bw;
+        }
+    }
+    if (best_cut == 0 || best_bits >= 63.0) return 0;
+    *cut = best_cut;
+    return 1;
+}
+/* alp_rd.rs:132-206 with entropy = false */
+static int alprd_encode(const uint64_t *v, size_t n, buf_t *out) {
+    if (n == 0) return 0;
+    uint32_t left_bw;
+    if (!rd_choose_cut(v, n, &left_bw)) return 0;
+    uint32_t right_bw = 64 - left_bw;
+    uint64_t mask = (1ull << right_bw) - 1;
+    uint64_t dict[8], cov;
+    size_t dl = top_dict(v, n, 1, right_bw, dict, &cov);
+    uint64_t *codes = (uint64_t *)malloc(8 * n), *rights = (uint64_t *)malloc(8 * n);
+    size_t *exc_pos = (size_t *)malloc(sizeof(size_t) * n);
+    size_t nexc = 0;
+    for (size_t i = 0; i < n; i++) {
+        uint64_t left = v[i] >> right_bw;
+        rights[i] = v[i] & mask;
+        size_t c = 0;
+        int found = 0;
+        for (; c < dl; c++)
+            if (dict[c] == left) {
+                found = 1;
+                break;
+            }
+        if (found) {
+            codes[i] = c;
+        } else {
+            codes[i] = 0;
+            exc_pos[nexc++] = i;
+        }
+    }
+    int ok = 0;
+    if (nexc * 4 <= n) {
+        ok = 1;
+        buf_push(out, (uint8_t)left_bw);
+        buf_push(out, (uint8_t)dl);
+        for (size_t i = 0; i < dl; i++) buf_u64le(out, dict[i]);
+        varint_write(out, nexc);
+        for (size_t j = 0; j < nexc; j++) {
+            varint_write(out, exc_pos[j]);
+            buf_u64le(out, v[exc_pos[j]] >> right_bw);
+        }
+        buf_t cb = {0};
+        for_encode(codes, n, 0, &cb);
+        buf_push(out, 0); /* T_BITPACK */
+        varint_write(out, cb.len);
+        buf_append(out, cb.p, cb.len);
+        buf_free(&cb);
+        buf_push(out, 0); /* T_BITPACK */
+        for_encode(rights, n, 0, out);
+    }
+    free(codes);
+    free(rights);
+    free(exc_pos);
+    return ok;
+}
+/* alp_rd.rs:208-285 */
+static int alprd_decode(const uint8_t *p, size_t len, size_t n, uint64_t *out) {
+    size_t pos = 0;
+    if (pos >= len) return ORC_TRUNCATED;
+    uint32_t left_bw = p[pos++];
+    if (left_bw < 1 || left_bw > 63) return ORC_CORRUPT;
+    uint32_t right_bw = 64 - left_bw;
+    if (pos >= len) return ORC_TRUNCATED;
+    size_t dsz = p[pos++];
+    if (dsz > 8) return ORC_CORRUPT;
+    uint64_t dict[8];
+    for (size_t i = 0; i < dsz; i++) {
+        if (len - pos < 8) return ORC_TRUNCATED;
+        dict[i] = rd_u64le(p + pos);
+        pos += 8;
+    }
+    uint64_t nexc;
+    int rc = varint_read(p, len, &pos, &nexc);
+    if (rc) return rc;
+    if (nexc > n) return ORC_CORRUPT;
+    uint64_t *ep = (uint64_t *)malloc(8 * (nexc ? nexc : 1)), *el = (uint64_t *)malloc(8 * (nexc ? nexc : 1));
+    uint64_t *codes = NULL, *rights = NULL;
+    uint8_t *tmp = NULL;
+    for (uint64_t j = 0; j < nexc; j++) {
+        rc = varint_read(p, len, &pos, &ep[j]);
+        if (rc) goto done;
+        if (len - pos < 8) {
+            rc = ORC_TRUNCATED;
+            goto done;
+        }
+        el[j] = rd_u64le(p + pos);
+        pos += 8;
+        if (ep[j] >= n) {
+            rc = ORC_CORRUPT;
+            goto done;
+        }
+    }
+    if (pos >= len) {
+        rc = ORC_TRUNCATED;
+        goto done;
+    }
+    uint8_t ctag = p[pos++];
+    uint64_t clen;
+    rc = varint_read(p, len, &pos, &clen);
+    if (rc) goto done;
+    if (clen > len - pos) {
+        rc = ORC_TRUNCATED;
+        goto done;
+    }
+    const uint8_t *cblob = p + pos;
+    pos += clen;
+    if (pos >= len) {
+        rc = ORC_TRUNCATED;
+        goto done;
+    }
+    uint8_t rtag = p[pos++];
+    codes = (uint64_t *)malloc(8 * (n ? n : 1));
+    rights = (uint64_t *)malloc(8 * (n ? n : 1));
+    if (ctag == 0) {
+        rc = for_decode(cblob, clen, n, codes);
+        if (rc) goto done;
+    } else if (ctag == 1) {
+        size_t tl;
+        rc = decode_residuals(cblob, clen, n, &tmp, &tl);
+        if (rc) goto done;
+        if (tl != n) {
+            rc = ORC_CORRUPT;
+            goto done;
+        }
+        for (size_t i = 0; i < n; i++) codes[i] = tmp[i];
+        free(tmp);
+        tmp = NULL;
+    } else {
+        rc = ORC_CORRUPT;
+        goto done;
+    }
+    if (rtag == 0) {
+        rc = for_decode(p + pos, len - pos, n, rights);
+        if (rc) goto done;
+    } else if (rtag == 1) {
+        size_t tl;
+        rc = decode_residuals(p + pos, len - pos, n * 8 + 64, &tmp, &tl);
+        if (rc) goto done;
+        rc = for_decode(tmp, tl, n, rights);
+        if (rc) goto done;
+    } else {
+        rc = ORC_CORRUPT;
+        goto done;
+    }
+    for (size_t i = 0; i < n; i++) {
+        if (codes[i] >= dsz) {
+            rc = ORC_CORRUPT;
+            goto done;
+        }
+        out[i] = (dict[codes[i]] << right_bw) | rights[i];
+    }
+    for (uint64_t j = 0; j < nexc; j++) out[ep[j]] = (el[j] << right_bw) | rights[ep[j]];
+    rc = ORC_OK;
+done:
+    free(ep);
+    free(el);
+    free(codes);
+    free(rights);
+    free(tmp);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* DICT (codecs/dict.rs:24-207), non-entropy forms                           */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint64_t *keys;
+    uint32_t *vals;
+    uint8_t *used;
+    size_t cap;
+} hmap_t;
+static uint64_t hmix(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    return x;
+}
+/* dict.rs:87-152 with entropy = false. First-occurrence codes are remapped to
+ * sorted order, so the final codes are the value ranks. */
+static int dict_encode(const uint64_t *v, size_t n, buf_t *out) {
+    if (n == 0) return 0;
+    size_t max_card = n / 2 + 1;
+    hmap_t h;
+    h.cap = 1;
+    while (h.cap < 2 * (max_card + 2)) h.cap <<= 1;
+    h.keys = (uint64_t *)malloc(8 * h.cap);
+    h.vals = (uint32_t *)malloc(4 * h.cap);
+    h.used = (uint8_t *)calloc(h.cap, 1);
+    uint64_t *dict = (uint64_t *)malloc(8 * (max_card + 2));
+    uint64_t *codes = (uint64_t *)malloc(8 * n);
+    size_t card = 0;
+    int ok = 1;
+    for (size_t i = 0; i < n; i++) {
+        size_t s = hmix(v[i]) & (h.cap - 1);
+        while (h.used[s] && h.keys[s] != v[i]) s = (s + 1) & (h.cap - 1);
+        if (!h.used[s]) {
+            h.used[s] = 1;
+            h.keys[s] = v[i];
+            h.vals[s] = (uint32_t)card;
+            dict[card++] = v[i];
+        }
+        codes[i] = h.vals[s];
+        if (card > max_card) {
+            ok = 0;
+            break;
+        }
+    }
+    if (ok) {
+        /* sort + remap */
+        uint64_t *sorted = (uint64_t *)malloc(8 * card);
+        memcpy(sorted, dict, 8 * card);
+        qsort(sorted, card, 8, cmp_u64);
+        uint32_t *remap = (uint32_t *)malloc(4 * card);
+        for (size_t c = 0; c < card; c++) {
+            size_t lo = 0, hi = card;
+            while (lo < hi) {
+                size_t mid = (lo + hi) / 2;
+                if (sorted[mid] < dict[c]) lo = mid + 1; else hi = mid;
+            }
+            remap[c] = (uint32_t)lo;
+        }
+        for (size_t i = 0; i < n; i++) codes[i] = remap[codes[i]];
+        /* encode_values (dict.rs:40-61), entropy = false */
+        buf_t delta = {0};
+        delta_encode(sorted, card, &delta);
+        varint_write(out, card);
+        if (delta.len < card * 8) {
+            buf_push(out, 1); /* VAL_DELTA */
+            varint_write(out, delta.len);
+            buf_append(out, delta.p, delta.len);
+        } else {
+            buf_push(out, 0); /* VAL_RAW */
+            varint_write(out, card * 8);
+            for (size_t c = 0; c < card; c++) buf_u64le(out, sorted[c]);
+        }
+        buf_free(&delta);
+        buf_push(out, 0); /* CODES_BITPACK */
+        for_encode(codes, n, 0, out);
+        free(sorted);
+        free(remap);
+    }
+    free(h.keys);
+    free(h.vals);
+    free(h.used);
+    free(dict);
+    free(codes);
+    return ok;
+}
+/* dict.rs:154-207 */
+static int dict_decode(const uint8_t *p, size_t len, size_t n, uint64_t *out) {
+    size_t pos = 0;
+    uint64_t card, vlen;
+    int rc = varint_read(p, len, &pos, &card);
+    if (rc) return rc;
+    if (card > n) return ORC_CORRUPT;
+    if (pos >= len) return ORC_TRUNCATED;
+    uint8_t vtag = p[pos++];
+    rc = varint_read(p, len, &pos, &vlen);
+    if (rc) return rc;
+    if (vlen > len - pos) return ORC_TRUNCATED;
+    const uint8_t *vb = p + pos;
+    pos += vlen;
+    uint64_t *dict = (uint64_t *)malloc(8 * (card ? card : 1));
+    uint64_t *codes = (uint64_t *)malloc(8 * (n ? n : 1));
+    uint8_t *tmp = NULL;
+    if (vtag == 0) {
+        if (vlen != card * 8) {
+            rc = ORC_CORRUPT;
+            goto done;
+        }
+        for (size_t c = 0; c < card; c++) dict[c] = rd_u64le(vb + 8 * c);
+    } else if (vtag == 1) {
+        rc = delta_decode(vb, vlen, card, dict);
+        if (rc) goto done;
+    } else if (vtag == 2) {
+        rc = ORC_UNSUPPORTED; /* transpose->entropy needs the range coder path */
+        goto done;
+    } else {
+        rc = ORC_CORRUPT;
+        goto done;
+    }
+    if (pos >= len) {
+        rc = ORC_TRUNCATED;
+        goto done;
+    }
+    uint8_t tag = p[pos++];
+    if (tag == 0) {
+        rc = for_decode(p + pos, len - pos, n, codes);
+        if (rc) goto done;
+    } else if (tag == 1) {
+        if (pos >= len) {
+            rc = ORC_TRUNCATED;
+            goto done;
+        }
+        size_t nb = p[pos++];
+        if (nb < 1 || nb > 2) {
+            rc = ORC_CORRUPT;
+            goto done;
+        }
+        memset(codes, 0, 8 * n);
+        for (size_t pl = 0; pl < nb; pl++) {
+            uint64_t bl;
+            rc = varint_read(p, len, &pos, &bl);
+            if (rc) goto done;
+            if (bl > len - pos) {
+                rc = ORC_TRUNCATED;
+                goto done;
+            }
+            size_t tl;
+            rc = decode_residuals(p + pos, bl, n, &tmp, &tl);
+            if (rc) goto done;
+            pos += bl;
+            if (tl != n) {
+                rc = ORC_CORRUPT;
+                goto done;
+            }
+            for (size_t i = 0; i < n; i++) codes[i] |= (uint64_t)tmp[i] << (8 * pl);
+            free(tmp);
+            tmp = NULL;
+        }
+    } else {
+        rc = ORC_CORRUPT;
+        goto done;
+    }
+    for (size_t i = 0; i < n; i++) {
+        if (codes[i] >= card) {
+            rc = ORC_CORRUPT;
+            goto done;
+        }
+        out[i] = dict[codes[i]];
+    }
+    rc = ORC_OK;
+done:
+    free(dict);
+    free(codes);
+    free(tmp);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* RLE (codecs/rle.rs:10-75)                                                  */
+/* ------------------------------------------------------------------------ */
+static int rle_encode(const uint64_t *v, size_t n, buf_t *out) {
+    if (n == 0) return 0;
+    size_t max_runs = n / 2 + 1;
+    uint64_t *rv = (uint64_t *)malloc(8 * (max_runs + 2)), *rl = (uint64_t *)malloc(8 * (max_runs + 2));
+    size_t runs = 0;
+    uint64_t prev = v[0], len = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (v[i] == prev) {
+            len++;
+        } else {
+            rv[runs] = prev;
+            rl[runs] = len;
+            runs++;
+            prev = v[i];
+            len = 1;
+            if (runs > max_runs) {
+                free(rv);
+                free(rl);
+                return 0;
+            }
+        }
+    }
+    rv[runs] = prev;
+    rl[runs] = len;
+    runs++;
+    varint_write(out, runs);
+    for (size_t r = 0; r < runs; r++) buf_u64le(out, rv[r]);
+    for_encode(rl, runs, 0, out);
+    free(rv);
+    free(rl);
+    return 1;
+}
+static int rle_decode(const uint8_t *p, size_t len, size_t n, uint64_t *out) {
+    size_t pos = 0;
+    uint64_t runs;
+    int rc = varint_read(p, len, &pos, &runs);
+    if (rc) return rc;
+    if (runs > n) return ORC_CORRUPT;
+    if ((len - pos) / 8 < runs) return ORC_TRUNCATED;
+    const uint8_t *vals = p + pos;
+    pos += 8 * runs;
+    uint64_t *lens = (uint64_t *)malloc(8 * (runs ? runs : 1));
+    rc = for_decode(p + pos, len - pos, runs, lens);
+    if (rc) {
+        free(lens);
+        return rc;
+    }
+    size_t o = 0;
+    for (size_t r = 0; r < runs; r++) {
+        uint64_t l = lens[r];
+        if (l > n - o) {
+            free(lens);
+            return ORC_CORRUPT;
+        }
+        uint64_t val = rd_u64le(vals + 8 * r);
+        for (uint64_t k = 0; k < l; k++) out[o++] = val;
+    }
+    free(lens);
+    return o == n ? ORC_OK : ORC_LENGTH_MISMATCH;
+}
+
+/* ------------------------------------------------------------------------ */
+/* probe / planning (encoder.rs:142-178, 644-750)                             */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint32_t exp_range;
+    int distinct_low, repeats;
+} feats_t;
+
+static feats_t probe(const uint64_t *b, size_t n) {
+    feats_t f = {0, 1, 0};
+    if (n == 0) return f;
+    uint32_t mn = UINT32_MAX, mx = 0, consec = 0;
+    uint64_t prev = b[0];
+    for (size_t i = 0; i < n; i++) {
+        uint32_t e = (uint32_t)((b[i] >> 52) & 0x7FF);
+        if (e < mn) mn = e;
+        if (e > mx) mx = e;
+        consec += b[i] == prev;
+        prev = b[i];
+    }
+    uint64_t seen[256] = {0};
+    uint32_t distinct = 0;
+    size_t stride = n / 4096 > 1 ? n / 4096 : 1;
+    for (size_t k = 0; k < n; k += stride) {
+        uint64_t h = b[k] * 0x9E3779B97F4A7C15ull;
+        uint32_t bucket = (uint32_t)((h >> 50) & 0x3FFF);
+        uint64_t bit = 1ull << (bucket & 63);
+        if (!(seen[bucket >> 6] & bit)) {
+            seen[bucket >> 6] |= bit;
+            distinct++;
+        }
+    }
+    f.exp_range = mx - mn;
+    f.distinct_low = distinct < 2048;
+    /* consec_eq.saturating_mul(2) > n as u32 */
+    uint64_t c2 = (uint64_t)consec * 2;
+    if (c2 > UINT32_MAX) c2 = UINT32_MAX;
+    f.repeats = c2 > (uint32_t)n;
+    return f;
+}
+/* encoder.rs:655-683 */
+static int looks_incompressible(const uint64_t *b, size_t n, feats_t f, size_t lane_bytes) {
+    if (f.distinct_low || f.repeats || n < 256) return 0;
+    uint32_t thr = (uint32_t)(lane_bytes * 8) - 4;
+    size_t step = n / 2048 > 1 ? n / 2048 : 1;
+    uint64_t vmin = b[0], vmax = b[0], dmax = 0;
+    for (size_t i = step; i < n; i += step) {
+        uint64_t v = b[i];
+        if (v < vmin) vmin = v;
+        if (v > vmax) vmax = v;
+        uint64_t zz = zigzag(v - b[i - 1]);
+        if (zz > dmax) dmax = zz;
+    }
+    uint32_t vw = 64 - lz64(vmax - vmin);
+    if (vmax - vmin == 0) vw = 0;
+    uint32_t dw = dmax == 0 ? 0 : 64 - lz64(dmax);
+    return vw >= thr && dw >= thr;
+}
+#define BASE_Q 32768
+#define MAX_Q 131072
+size_t orc_plan_blocks(const uint64_t *v, size_t n, size_t bs, uint64_t *ranges) {
+    size_t nb = 0, start = 0;
+    if (bs) {
+        if (bs > MAX_Q) bs = MAX_Q;
+        while (start < n) {
+            size_t end = start + bs < n ? start + bs : n;
+            ranges[2 * nb] = start;
+            ranges[2 * nb + 1] = end;
+            nb++;
+            start = end;
+        }
+        return nb;
+    }
+    while (start < n) {
+        size_t be = start + BASE_Q < n ? start + BASE_Q : n;
+        feats_t f = probe(v + start, be - start);
+        size_t end = (f.distinct_low || f.repeats) ? (start + MAX_Q < n ? start + MAX_Q : n) : be;
+        ranges[2 * nb] = start;
+        ranges[2 * nb + 1] = end;
+        nb++;
+        start = end;
+    }
+    return nb;
+}
+
+/* ------------------------------------------------------------------------ */
+/* the competition (encoder.rs:275-469) at Fastest/Fast                       */
+/* ------------------------------------------------------------------------ */
+static int looks_compressible(size_t res, size_t raw) {
+    return sat_mul(res, 20) < sat_mul(raw, 19);
+}
+
+typedef struct {
+    int mode;
+    buf_t payload;
+    size_t score;
+    uint64_t lambda;
+    size_t decoded_bytes;
+} best_t;
+
+static void best_consider(best_t *b, int mode, buf_t *p) {
+    size_t score = p->len + penalty(mode, b->lambda, b->decoded_bytes);
+    if (score < b->score) {
+        buf_free(&b->payload);
+        b->payload = *p;
+        b->mode = mode;
+        b->score = score;
+    } else {
+        buf_free(p);
+    }
+    memset(p, 0, sizeof *p);
+}
+
+static void encode_block(const uint64_t *b, size_t n, int dt, int level, buf_t *out) {
+    int isf = dtype_is_float(dt);
+    int reduced = level == ORC_FASTEST;
+    uint64_t lambda = level_lambda(level);
+    size_t decoded = n * dtype_lane_bytes(dt), raw_bytes = n * 8;
+    feats_t f = probe(b, n);
+    best_t best;
+    memset(&best, 0, sizeof best);
+    buf_t p = {0};
+    int mode = M_RAW;
+    if (!isf && looks_incompressible(b, n, f, dtype_lane_bytes(dt))) {
+        raw_encode(b, n, dt, &best.payload);
+        best.mode = M_RAW;
+        goto emit;
+    }
+    raw_encode(b, n, dt, &best.payload);
+    best.mode = M_RAW;
+    best.lambda = lambda;
+    best.decoded_bytes = decoded;
+    best.score = best.payload.len + penalty(M_RAW, lambda, decoded);
+    if (const_encode(b, n, &p)) best_consider(&best, M_CONST, &p); else buf_free(&p);
+    if (stride_encode(b, n, &p)) best_consider(&best, M_STRIDE, &p); else buf_free(&p);
+    if (!reduced) {
+        xorz_encode(b, n, &p);
+        best_consider(&best, M_XORZ, &p);
+    }
+    for_encode(b, n, dtype_signed(dt), &p);
+    best_consider(&best, M_FOR_BITPACK, &p);
+    delta_encode(b, n, &p);
+    best_consider(&best, M_DELTA_BITPACK, &p);
+    if (isf && !reduced) {
+        if (fm_encode(b, n, &p)) best_consider(&best, M_FLOAT_MULT, &p); else buf_free(&p);
+        if (alp_encode(b, n, &p)) best_consider(&best, M_ALP, &p); else buf_free(&p);
+        if (alprd_encode(b, n, &p)) best_consider(&best, M_ALP_RD, &p); else buf_free(&p);
+    }
+    if (looks_compressible(best.payload.len, raw_bytes) && !reduced) {
+        if (dict_encode(b, n, &p)) best_consider(&best, M_DICT, &p); else buf_free(&p);
+        if (rle_encode(b, n, &p)) best_consider(&best, M_RLE, &p); else buf_free(&p);
+    }
+emit:
+    mode = best.mode;
+    g_wins[mode & 63]++;
+    buf_push(out, (uint8_t)mode);
+    buf_u32le(out, (uint32_t)n);
+    buf_u32le(out, (uint32_t)best.payload.len);
+    buf_append(out, best.payload.p, best.payload.len);
+    buf_free(&best.payload);
+}
+
+/* validity.rs:40-70 */
+static void validity_encode(const uint8_t *bm, size_t n, buf_t *out) {
+    buf_t rle = {0};
+    uint8_t cur = 0;
+    uint64_t run = 0;
+    for (size_t i = 0; i < n; i++) {
+        uint8_t bit = (bm[i >> 3] >> (i & 7)) & 1;
+        if (bit == cur) {
+            run++;
+        } else {
+            varint_write(&rle, run);
+            cur ^= 1;
+            run = 1;
+        }
+    }
+    varint_write(&rle, run);
+    size_t raw_len = (n + 7) / 8;
+    if (rle.len < raw_len) {
+        buf_push(out, 1);
+        buf_append(out, rle.p, rle.len);
+    } else {
+        buf_push(out, 0);
+        buf_append(out, bm, raw_len);
+    }
+    buf_free(&rle);
+}
+/* validity.rs:73-119 */
+static int validity_decode(const uint8_t *blob, size_t blen, size_t n, uint8_t **bmp) {
+    size_t raw_len = (n + 7) / 8;
+    if (blen == 0) return ORC_TRUNCATED;
+    uint8_t tag = blob[0];
+    const uint8_t *rest = blob + 1;
+    size_t rl = blen - 1;
+    if (tag == 0) {
+        if (rl < raw_len) return ORC_TRUNCATED;
+    } else if (tag != 1) {
+        return ORC_CORRUPT;
+    }
+    uint8_t *bm = (uint8_t *)calloc(raw_len ? raw_len : 1, 1);
+    if (!bm) return ORC_CORRUPT;
+    if (tag == 0) {
+        memcpy(bm, rest, raw_len);
+        if (n & 7) bm[raw_len - 1] &= (uint8_t)((1u << (n & 7)) - 1);
+    } else {
+        size_t pos = 0, i = 0;
+        int cur = 0;
+        while (i < n) {
+            uint64_t run;
+            int rc = varint_read(rest, rl, &pos, &run);
+            if (rc) {
+                free(bm);
+                return rc;
+            }
+            if (run > n - i) {
+                free(bm);
+                return ORC_CORRUPT;
+            }
+            if (cur)
+                for (size_t k = i; k < i + run; k++) bm[k >> 3] |= (uint8_t)(1u << (k & 7));
+            i += run;
+            cur ^= 1;
+        }
+    }
+    *bmp = bm;
+    return ORC_OK;
+}
+static size_t count_valid(const uint8_t *bm, size_t n) {
+    size_t c = 0;
+    for (size_t i = 0; i < n; i++) c += (bm[i >> 3] >> (i & 7)) & 1;
+    return c;
+}
+
+/* lib.rs:389-403 (to_lane) */
+static uint64_t to_lane(int dt, const uint8_t *p, size_t i) {
+    switch (dt) {
+    case ORC_I32: {
+        int32_t v;
+        memcpy(&v, p + 4 * i, 4);
+        return (uint64_t)(int64_t)v;
+    }
+    case ORC_U32: {
+        uint32_t v;
+        memcpy(&v, p + 4 * i, 4);
+        return v;
+    }
+    case ORC_F32: {
+        float v;
+        memcpy(&v, p + 4 * i, 4);
+        return f64_bits((double)v);
+    }
+    default: {
+        uint64_t v;
+        memcpy(&v, p + 8 * i, 8);
+        return v;
+    }
+    }
+}
+
+/* lib.rs:429-455 + encoder.rs:184-214 */
+int orc_compress_column(int dtype, size_t n, const void *values, const uint8_t *validity,
+                        int level, size_t block_size, uint8_t **outp, size_t *out_len) {
+    if (dtype < 0 || dtype > 5) return ORC_BAD_DTYPE;
+    if (level > ORC_FAST) return ORC_UNSUPPORTED;
+    const uint8_t *src = (const uint8_t *)values;
+    int has_nulls = validity && count_valid(validity, n) < n;
+    size_t nv = has_nulls ? count_valid(validity, n) : n;
+    uint64_t *lane = (uint64_t *)malloc(8 * (nv ? nv : 1));
+    size_t j = 0;
+    for (size_t i = 0; i < n; i++)
+        if (!has_nulls || ((validity[i >> 3] >> (i & 7)) & 1)) lane[j++] = to_lane(dtype, src, i);
+    buf_t out = {0};
+    buf_append(&out, "FCR1", 4);
+    buf_push(&out, 2);
+    buf_push(&out, has_nulls ? 1 : 0);
+    buf_push(&out, 16);
+    buf_push(&out, (uint8_t)dtype);
+    buf_u64le(&out, n);
+    if (has_nulls) {
+        buf_t vb = {0};
+        validity_encode(validity, n, &vb);
+        varint_write(&out, vb.len);
+        buf_append(&out, vb.p, vb.len);
+        buf_free(&vb);
+    }
+    uint64_t *ranges = (uint64_t *)malloc(16 * (nv + 1));
+    size_t nb = orc_plan_blocks(lane, nv, block_size, ranges);
+    for (size_t b = 0; b < nb; b++)
+        encode_block(lane + ranges[2 * b], ranges[2 * b + 1] - ranges[2 * b], dtype, level, &out);
+    free(ranges);
+    free(lane);
+    if (out.oom) {
+        buf_free(&out);
+        return ORC_NOMEM;
+    }
+    *outp = out.p;
+    *out_len = out.len;
+    return ORC_OK;
+}
+
+static int decode_frame(int mode, const uint8_t *p, size_t len, size_t n, int dt, uint64_t *out) {
+    switch (mode) {
+    case M_RAW: return raw_decode(p, len, n, dt, out);
+    case M_CONST:
+        if (len != 8) return ORC_CORRUPT;
+        for (size_t i = 0; i < n; i++) out[i] = rd_u64le(p);
+        return ORC_OK;
+    case M_STRIDE: {
+        if (len != 16) return ORC_CORRUPT;
+        uint64_t a = rd_u64le(p), s = rd_u64le(p + 8);
+        for (size_t i = 0; i < n; i++) out[i] = a + (uint64_t)i * s;
+        return ORC_OK;
+    }
+    case M_XORZ: return xorz_decode(p, len, n, out);
+    case M_FOR_BITPACK: return for_decode(p, len, n, out);
+    case M_DELTA_BITPACK: return delta_decode(p, len, n, out);
+    case M_ALP: return alp_decode(p, len, n, out);
+    case M_FLOAT_MULT: return fm_decode(p, len, n, out);
+    case M_ALP_RD: return alprd_decode(p, len, n, out);
+    case M_DICT: return dict_decode(p, len, n, out);
+    case M_RLE: return rle_decode(p, len, n, out);
+    case M_PRED: case M_LZ: case M_ORDERED_DELTA: case M_BYTE_TRANSPOSE: case M_DELTA2:
+    case M_PRED2: case M_PRED_RC: case M_DELTA_DP: case M_PCO:
+        return ORC_UNSUPPORTED;
+    default: return ORC_UNKNOWN_MODE;
+    }
+}
+
+/* decoder.rs:43-118 */
+int orc_decompress_column(const uint8_t *src, size_t len, int *dtype, size_t *nout,
+                          void **values, uint8_t **validity) {
+    if (len < 16) return ORC_TRUNCATED;
+    if (memcmp(src, "FCR1", 4) != 0) return ORC_BAD_MAGIC;
+    if (src[4] != 2) return ORC_BAD_VERSION;
+    if (src[5] & ~1u) return ORC_CORRUPT;
+    int has_v = src[5] & 1;
+    if (src[6] < 10 || src[6] > 16) return ORC_CORRUPT;
+    int dt = src[7];
+    if (dt > 7) return ORC_BAD_DTYPE;
+    if (dt > 5) return ORC_CORRUPT; /* decimal lane without container */
+    uint64_t n = rd_u64le(src + 8);
+    size_t pos = 16;
+    uint8_t *bm = NULL;
+    size_t nt = (size_t)n;
+    if (has_v) {
+        uint64_t vlen;
+        int rc = varint_read(src, len, &pos, &vlen);
+        if (rc) return rc;
+        if (vlen > len - pos) return ORC_TRUNCATED;
+        if (n > ((uint64_t)1 << 40)) return ORC_CORRUPT;
+        rc = validity_decode(src + pos, vlen, n, &bm);
+        if (rc) return rc;
+        pos += vlen;
+        nt = count_valid(bm, n);
+    } else if (n > ((uint64_t)1 << 40)) {
+        return ORC_CORRUPT;
+    }
+    /* frames hold compact values */
+    uint64_t *lane = (uint64_t *)malloc(8 * (nt ? nt : 1));
+    if (!lane) {
+        free(bm);
+        return ORC_CORRUPT;
+    }
+    size_t counted = 0;
+    int rc = ORC_OK;
+    /* Phase 1 (scan) validation precedes phase 2 (decode) in the reference;
+     * errors from either phase surface as an error code here. */
+    size_t scan = pos, cnt = 0;
+    while (cnt < nt) {
+        if (len - scan < 9) {
+            rc = ORC_TRUNCATED;
+            break;
+        }
+        int mode = src[scan];
+        if (!(mode == 0 || mode == 1 || mode == 2 || mode == 3 || mode == 4 || mode == 5 ||
+              mode == 7 || mode == 9 || mode == 11 || mode == 17 || mode == 19 || mode == 20 ||
+              mode == 25 || mode == 29 || mode == 45 || (mode >= 50 && mode <= 54))) {
+            rc = ORC_UNKNOWN_MODE;
+            break;
+        }
+        size_t fn = rd_u32le(src + scan + 1), pl = rd_u32le(src + scan + 5);
+        if (fn > MAX_Q) {
+            rc = ORC_CORRUPT;
+            break;
+        }
+        scan += 9;
+        if (pl > len - scan) {
+            rc = ORC_TRUNCATED;
+            break;
+        }
+        scan += pl;
+        cnt += fn;
+        if (cnt > nt) {
+            rc = ORC_CORRUPT;
+            break;
+        }
+    }
+    if (rc == ORC_OK && cnt != nt) rc = ORC_LENGTH_MISMATCH;
+    while (rc == ORC_OK && counted < nt) {
+        int mode = src[pos];
+        size_t fn = rd_u32le(src + pos + 1), pl = rd_u32le(src + pos + 5);
+        pos += 9;
+        rc = decode_frame(mode, src + pos, pl, fn, dt, lane + counted);
+        pos += pl;
+        counted += fn;
+    }
+    if (rc != ORC_OK) {
+        free(lane);
+        free(bm);
+        return rc;
+    }
+    size_t w = dtype_width(dt);
+    uint8_t *vals = (uint8_t *)calloc(n ? n : 1, w);
+    size_t j = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (bm && !((bm[i >> 3] >> (i & 7)) & 1)) continue;
+        uint64_t x = lane[j++];
+        if (dt == ORC_F32) {
+            float f = (float)bits_f64(x);
+            memcpy(vals + 4 * i, &f, 4);
+        } else if (w == 4) {
+            uint32_t u = (uint32_t)x;
+            memcpy(vals + 4 * i, &u, 4);
+        } else {
+            memcpy(vals + 8 * i, &x, 8);
+        }
+    }
+    free(lane);
+    *dtype = dt;
+    *nout = (size_t)n;
+    *values = vals;
+    *validity = bm;
+    return ORC_OK;
+}
+
+void orc_free(void *p) { free(p); }
+
+uint8_t *orc_encode_mode(int mode, const uint64_t *v, size_t n, int dt, size_t *len) {
+    buf_t p = {0};
+    int ok = 1;
+    switch (mode) {
+    case M_RAW: raw_encode(v, n, dt, &p); break;
+    case M_CONST: ok = const_encode(v, n, &p); break;
+    case M_STRIDE: ok = stride_encode(v, n, &p); break;
+    case M_XORZ: xorz_encode(v, n, &p); break;
+    case M_FOR_BITPACK: for_encode(v, n, dtype_signed(dt), &p); break;
+    case M_DELTA_BITPACK: delta_encode(v, n, &p); break;
+    case M_FLOAT_MULT: ok = fm_encode(v, n, &p); break;
+    case M_ALP: ok = alp_encode(v, n, &p); break;
+    case M_ALP_RD: ok = alprd_encode(v, n, &p); break;
+    case M_DICT: ok = dict_encode(v, n, &p); break;
+    case M_RLE: ok = rle_encode(v, n, &p); break;
+    default: ok = 0;
+    }
+    if (!ok) {
+        buf_free(&p);
+        *len = 0;
+        return NULL;
+    }
+    *len = p.len;
+    if (!p.p) p.p = (uint8_t *)malloc(1);
+    return p.p;
+}
+int orc_decode_mode(int mode, const uint8_t *payload, size_t len, size_t n, int dt, uint64_t *out) {
+    return decode_frame(mode, payload, len, n, dt, out);
+}

@@ -1,0 +1,179 @@
+"""ctypes bindings for the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure: the oracle is the checker the HIP path is compared
+against; nothing in the product imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+F64, I64, U64, I32, U32, F32 = 0, 1, 2, 3, 4, 5
+FASTEST, FAST, BALANCED, HIGH, MAX = 0, 1, 2, 3, 4
+NP_DTYPE = {F64: np.float64, I64: np.int64, U64: np.uint64, I32: np.int32, U32: np.uint32,
+            F32: np.float32}
+
+MODE = dict(PRED=0, CONST=1, STRIDE=2, XORZ=3, LZ=4, RAW=5, ORDERED_DELTA=7, ALP=9,
+            BYTE_TRANSPOSE=11, FLOAT_MULT=17, DICT=19, DELTA2=20, PRED2=25, PRED_RC=29,
+            DELTA_DP=45, FOR_BITPACK=50, DELTA_BITPACK=51, ALP_RD=52, RLE=53, PCO=54)
+
+
+def _load():
+    if not os.path.exists(_LIB):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    lib = ctypes.CDLL(_LIB)
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    lib.orc_compress_column.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t,
+                                        ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_decompress_column.argtypes = [ctypes.c_void_p, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_size_t),
+                                          ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(u8p)]
+    lib.orc_free.argtypes = [ctypes.c_void_p]
+    lib.orc_encode_mode.restype = u8p
+    lib.orc_encode_mode.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_decode_mode.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                    ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    lib.orc_rans_compress.restype = u8p
+    lib.orc_rans_compress.argtypes = [ctypes.c_void_p, ctypes.c_size_t,
+                                      ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_rans_decompress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                                        ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_plan_blocks.restype = ctypes.c_size_t
+    lib.orc_plan_blocks.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                                    ctypes.c_void_p]
+    for f in ("orc_pack32", "orc_unpack32", "orc_pack64", "orc_unpack64"):
+        getattr(lib, f).argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+    lib.orc_mode_wins.argtypes = [ctypes.c_void_p]
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def compress(values, dtype, validity=None, level=FAST, block_size=0):
+    values = np.ascontiguousarray(values, dtype=NP_DTYPE[dtype])
+    if validity is not None:
+        validity = np.ascontiguousarray(validity, dtype=np.uint8)
+    out = ctypes.POINTER(ctypes.c_uint8)()
+    n = ctypes.c_size_t()
+    rc = lib.orc_compress_column(dtype, len(values), _ptr(values), _ptr(validity), level,
+                                 block_size, ctypes.byref(out), ctypes.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"oracle compress rc={rc}")
+    b = ctypes.string_at(out, n.value)
+    lib.orc_free(out)
+    return b
+
+
+def decompress(stream):
+    """Returns (dtype, values ndarray, validity ndarray|None) or raises."""
+    buf = np.frombuffer(stream, dtype=np.uint8)
+    dt = ctypes.c_int()
+    n = ctypes.c_size_t()
+    vals = ctypes.c_void_p()
+    vb = ctypes.POINTER(ctypes.c_uint8)()
+    rc = lib.orc_decompress_column(_ptr(buf) if len(buf) else None, len(buf), ctypes.byref(dt),
+                                   ctypes.byref(n), ctypes.byref(vals), ctypes.byref(vb))
+    if rc != 0:
+        raise ValueError(f"oracle decompress rc={rc}")
+    npdt = NP_DTYPE[dt.value]
+    nbytes = n.value * np.dtype(npdt).itemsize
+    arr = np.frombuffer(ctypes.string_at(vals, nbytes), dtype=npdt).copy()
+    lib.orc_free(vals)
+    validity = None
+    if vb:
+        validity = np.frombuffer(ctypes.string_at(vb, (n.value + 7) // 8), dtype=np.uint8).copy()
+        lib.orc_free(vb)
+    return dt.value, arr, validity
+
+
+def encode_mode(mode, lane, dtype=U64):
+    lane = np.ascontiguousarray(lane, dtype=np.uint64)
+    n = ctypes.c_size_t()
+    p = lib.orc_encode_mode(MODE[mode] if isinstance(mode, str) else mode, _ptr(lane), len(lane),
+                            dtype, ctypes.byref(n))
+    if not p:
+        return None
+    b = ctypes.string_at(p, n.value)
+    lib.orc_free(p)
+    return b
+
+
+def decode_mode(mode, payload, n, dtype=U64):
+    buf = np.frombuffer(payload, dtype=np.uint8)
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    rc = lib.orc_decode_mode(MODE[mode] if isinstance(mode, str) else mode,
+                             _ptr(buf) if len(buf) else None, len(buf), n, dtype, _ptr(out))
+    if rc != 0:
+        raise ValueError(f"decode rc={rc}")
+    return out[:n]
+
+
+def rans_compress(data):
+    data = np.ascontiguousarray(np.frombuffer(bytes(data), dtype=np.uint8))
+    n = ctypes.c_size_t()
+    p = lib.orc_rans_compress(_ptr(data) if len(data) else None, len(data), ctypes.byref(n))
+    if not p:
+        return None
+    b = ctypes.string_at(p, n.value)
+    lib.orc_free(p)
+    return b
+
+
+def rans_decompress(payload, max_len):
+    buf = np.frombuffer(payload, dtype=np.uint8)
+    out = ctypes.POINTER(ctypes.c_uint8)()
+    n = ctypes.c_size_t()
+    rc = lib.orc_rans_decompress(_ptr(buf), len(buf), max_len, ctypes.byref(out), ctypes.byref(n))
+    if rc != 0:
+        raise ValueError(f"rans rc={rc}")
+    b = ctypes.string_at(out, n.value)
+    lib.orc_free(out)
+    return b
+
+
+def plan_blocks(lane, block_size=0):
+    lane = np.ascontiguousarray(lane, dtype=np.uint64)
+    ranges = np.zeros(2 * (len(lane) + 1), dtype=np.uint64)
+    nb = lib.orc_plan_blocks(_ptr(lane), len(lane), block_size, _ptr(ranges))
+    return [(int(ranges[2 * i]), int(ranges[2 * i + 1])) for i in range(nb)]
+
+
+def mode_wins():
+    out = np.zeros(64, dtype=np.uint64)
+    lib.orc_mode_wins(_ptr(out))
+    return out
+
+
+def frames(stream):
+    """Parse (mode, n, payload) frames of a stream without validity."""
+    import struct
+    pos = 16
+    if stream[5] & 1:
+        v, shift = 0, 0
+        while True:
+            b = stream[pos]
+            pos += 1
+            v |= (b & 0x7F) << shift
+            shift += 7
+            if not b & 0x80:
+                break
+        pos += v
+    out = []
+    while pos < len(stream):
+        mode = stream[pos]
+        n, plen = struct.unpack_from("<II", stream, pos + 1)
+        out.append((mode, n, stream[pos + 9: pos + 9 + plen]))
+        pos += 9 + plen
+    return out
