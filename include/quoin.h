@@ -1,0 +1,203 @@
+/*
+ * quoin.h — C ABI of the MI355X-native quoin block path (libquoin_amd.so).
+ *
+ * The first block of declarations keeps the reference C ABI symbol for symbol
+ * (reference: capi/include/quoin.h:28-142): same names, signatures, error codes
+ * and buffer ownership. The second block adds the device-resident batched and
+ * served entry points the GPU build exists for.
+ *
+ * Levels. The reference's Config::default is Level::Max, whose competition also
+ * runs pco, the order-1 range coder and the LZ cascade. Those host coders are
+ * not part of this build yet, so the reference-named compress entry points run
+ * at QUOIN_DEFAULT_LEVEL (Level::Fast: the largest level whose whole codec pool
+ * runs on the GPU). Their streams are valid quoin streams, bit-identical to the
+ * reference encoder at Level::Fast, not at Level::Max. Levels above Fast return
+ * QUOIN_ERR_UNSUPPORTED from the *_cfg entry points. Decoding accepts every
+ * frame the Fast and Fastest pools emit; a stream holding a frame of a host-only
+ * mode (pco, range coder, LZ, predictors) returns QUOIN_ERR_UNSUPPORTED.
+ *
+ * Threading: the library owns one HIP device context per process (device 0
+ * unless quoin_gpu_create picks another). Calls are serialized per handle.
+ */
+#ifndef QUOIN_H
+#define QUOIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QUOIN_OK                    0
+#define QUOIN_ERR_NULL             (-1) /* a required pointer argument was NULL   */
+#define QUOIN_ERR_BUFFER_TOO_SMALL (-2) /* dst too small; needed size in out_*    */
+#define QUOIN_ERR_CORRUPT          (-3) /* malformed / truncated input stream     */
+#define QUOIN_ERR_PANIC            (-4) /* internal error caught at the boundary  */
+/* Additions of this build (no reference counterpart). */
+#define QUOIN_ERR_UNSUPPORTED      (-5) /* level / frame mode not built (see top) */
+#define QUOIN_ERR_DEVICE           (-6) /* HIP runtime error or no usable GPU     */
+
+/* ---- reference C ABI (capi/include/quoin.h) ---------------------------- */
+
+/* capi/include/quoin.h:37 — static NUL-terminated version string. */
+const char *quoin_version(void);
+
+/* capi/include/quoin.h:40 — 9*n + 1024. */
+size_t quoin_compress_bound(size_t n_values);
+
+/* capi/include/quoin.h:48-49 — compress doubles (at QUOIN_DEFAULT_LEVEL). */
+int quoin_compress(const double *src, size_t n_values, uint8_t *dst, size_t dst_cap,
+                   size_t *out_len);
+
+/* capi/include/quoin.h:57-58 */
+int quoin_decompress(const uint8_t *src, size_t src_len, double *dst, size_t dst_cap_values,
+                     size_t *out_values);
+
+/* capi/include/quoin.h:65 */
+ptrdiff_t quoin_decompressed_value_count(const uint8_t *src, size_t src_len);
+
+/* capi/include/quoin.h:68-76 — a context; `threads` sizes the host worker
+ * pool used for host-side frame scans (0 = one per core). */
+typedef struct QuoinCtx QuoinCtx;
+QuoinCtx *quoin_ctx_create(size_t threads);
+void quoin_ctx_free(QuoinCtx *ctx);
+
+/* capi/include/quoin.h:80-83 */
+int quoin_compress_ctx(const QuoinCtx *ctx, const double *src, size_t n_values, uint8_t *dst,
+                       size_t dst_cap, size_t *out_len);
+int quoin_decompress_ctx(const QuoinCtx *ctx, const uint8_t *src, size_t src_len, double *dst,
+                         size_t dst_cap_values, size_t *out_values);
+
+/* capi/include/quoin.h:97-115 — Arrow-native dtype tags. */
+typedef enum {
+    QUOIN_DTYPE_BOOL = 0,
+    QUOIN_DTYPE_I8 = 1,
+    QUOIN_DTYPE_I16 = 2,
+    QUOIN_DTYPE_I32 = 3,
+    QUOIN_DTYPE_I64 = 4,
+    QUOIN_DTYPE_U8 = 5,
+    QUOIN_DTYPE_U16 = 6,
+    QUOIN_DTYPE_U32 = 7,
+    QUOIN_DTYPE_U64 = 8,
+    QUOIN_DTYPE_F32 = 9,
+    QUOIN_DTYPE_F64 = 10,
+    QUOIN_DTYPE_DECIMAL32 = 11,
+    QUOIN_DTYPE_DECIMAL64 = 12,
+    QUOIN_DTYPE_DECIMAL128 = 13,
+    QUOIN_DTYPE_DECIMAL256 = 14
+} QuoinDType;
+
+/* capi/include/quoin.h:119 */
+size_t quoin_dtype_width(int32_t dtype);
+/* capi/include/quoin.h:122 */
+size_t quoin_typed_compress_bound(int32_t dtype, size_t n);
+/* capi/include/quoin.h:128-129 — returns bytes written, 0 on any failure. */
+size_t quoin_typed_compress(int32_t dtype, size_t n, const void *values, const uint8_t *validity,
+                            uint8_t *dst, size_t dst_cap);
+/* capi/include/quoin.h:136-138 — decimals are outside this build's path:
+ * always returns 0 (the documented failure value). */
+size_t quoin_typed_compress_decimal(int32_t dtype, size_t n, const void *values,
+                                    const uint8_t *validity, int8_t scale, uint8_t precision,
+                                    uint8_t *dst, size_t dst_cap);
+/* capi/include/quoin.h:144-145 */
+int quoin_typed_decompress(int32_t dtype, size_t n, const uint8_t *src, size_t src_len,
+                           void *values_out, uint8_t *validity_out);
+
+/* ---- additions: explicit configuration (src/lib.rs:97-245) -------------- */
+
+/* Level (src/lib.rs:97-118). */
+#define QUOIN_LEVEL_FASTEST  0
+#define QUOIN_LEVEL_FAST     1
+#define QUOIN_LEVEL_BALANCED 2
+#define QUOIN_LEVEL_HIGH     3
+#define QUOIN_LEVEL_MAX      4
+#define QUOIN_DEFAULT_LEVEL  QUOIN_LEVEL_FAST
+
+/* Config (src/lib.rs:194-220): level and fixed block size in values
+ * (0 = adaptive; clamped to [1, 131072]). */
+typedef struct {
+    int32_t level;
+    uint32_t block_size;
+} QuoinConfig;
+
+/* As quoin_typed_compress with an explicit config; NULL cfg = defaults. */
+size_t quoin_typed_compress_cfg(const QuoinConfig *cfg, int32_t dtype, size_t n,
+                                const void *values, const uint8_t *validity, uint8_t *dst,
+                                size_t dst_cap);
+/* As quoin_compress with an explicit config. */
+int quoin_compress_cfg(const QuoinConfig *cfg, const double *src, size_t n_values, uint8_t *dst,
+                       size_t dst_cap, size_t *out_len);
+
+/* Error code behind the last 0 returned by quoin_typed_compress* on this
+ * thread (QUOIN_OK after a success). */
+int quoin_last_error(void);
+
+/* Per-mode win counters since the last reset (src/lib.rs:540-547), indexed by
+ * mode id. */
+void quoin_mode_win_counts(uint64_t out[64]);
+void quoin_reset_mode_win_counts(void);
+
+/* ---- additions: device-resident batched path --------------------------- */
+
+/* A GPU handle: device, streams, scratch arenas sized by an HBM budget. */
+typedef struct QuoinGpu QuoinGpu;
+
+/* device < 0 picks device 0. hbm_budget_bytes bounds the scratch the handle may
+ * allocate (0 = 64 GiB). Returns NULL when no usable GPU is present. */
+QuoinGpu *quoin_gpu_create(int device, uint64_t hbm_budget_bytes);
+void quoin_gpu_destroy(QuoinGpu *g);
+
+/* One column in device memory. dtype is a QuoinDType among
+ * I32, I64, U32, U64, F32, F64. */
+typedef struct {
+    int32_t dtype;
+    uint32_t reserved;
+    uint64_t n;              /* logical value count (nulls included)        */
+    const void *values;      /* device, n * width bytes                     */
+    const uint8_t *validity; /* device Arrow bitmap (ceil(n/8) B) or NULL   */
+    uint8_t *dst;            /* device output stream                        */
+    uint64_t dst_cap;        /* bytes; quoin_typed_compress_bound() is safe */
+} QuoinDeviceColumn;
+
+/* Compress a batch of device columns with one launch sequence covering every
+ * block of every column. out_lens (host, ncols entries) receives each stream's
+ * length. `stream` is a hipStream_t (NULL = the handle's stream). Synchronous:
+ * returns once out_lens is valid. */
+int quoin_gpu_compress(QuoinGpu *g, const QuoinConfig *cfg, const QuoinDeviceColumn *cols,
+                       size_t ncols, uint64_t *out_lens, void *stream);
+
+/* One stream to decode. The frame index is built on the host from host_src
+ * (the same bytes as src), the way the reference decoder's sequential frame
+ * scan reads the 9-byte frame headers (src/decoder.rs:65-94). */
+typedef struct {
+    int32_t dtype;            /* expected QuoinDType                          */
+    uint32_t reserved;
+    uint64_t n;               /* expected logical value count                 */
+    const uint8_t *src;       /* device copy of the stream                    */
+    const uint8_t *host_src;  /* host copy of the stream (frame scan)          */
+    uint64_t src_len;
+    void *values_out;         /* device, n * width bytes                      */
+    uint8_t *validity_out;    /* device, ceil(n/8) bytes, or NULL             */
+} QuoinDeviceStream;
+
+/* Decode a batch of device streams with one launch sequence. Returns QUOIN_OK
+ * or the first error. */
+int quoin_gpu_decompress(QuoinGpu *g, const QuoinDeviceStream *streams, size_t nstreams,
+                         void *stream);
+
+/* Launch form of the two calls above: 0 = batched (default: every block of every
+ * column in one launch per kernel stage), 1 = plain (one launch sequence per
+ * block, the per-block-launch baseline). */
+int quoin_gpu_set_launch_form(QuoinGpu *g, int form);
+
+/* Device time (ms, HIP events on the handle's stream) of the last compress /
+ * decompress call, and of its dominant kernel with its name. */
+int quoin_gpu_last_timing(const QuoinGpu *g, double *total_ms, double *kernel_ms,
+                          const char **kernel_name);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* QUOIN_H */

@@ -1,0 +1,266 @@
+"""quoin_amd — Python binding of the MI355X quoin block path (libquoin_amd.so).
+
+Mirrors the reference's public API shape (src/lib.rs:97-547): ``Level``,
+``Config``, ``compress``/``decompress`` for f64, ``compress_column`` /
+``decompress_column`` for typed columns with an Arrow validity bitmap, and
+``mode_win_counts``. ``Gpu`` exposes the device-resident batched entry points
+(device pointers, e.g. from torch tensors) that the GPU build adds.
+
+Everything runs through the HIP library; when it is missing or no GPU is
+usable, calls raise instead of falling back to any CPU path.
+"""
+import ctypes
+import enum
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libquoin_amd.so")
+
+OK, ERR_NULL, ERR_BUFFER_TOO_SMALL, ERR_CORRUPT, ERR_PANIC = 0, -1, -2, -3, -4
+ERR_UNSUPPORTED, ERR_DEVICE = -5, -6
+
+
+class QuoinError(RuntimeError):
+    def __init__(self, code, what=""):
+        super().__init__(f"quoin error {code}{': ' + what if what else ''}")
+        self.code = code
+
+
+class Level(enum.IntEnum):
+    """src/lib.rs:97-118"""
+    FASTEST = 0
+    FAST = 1
+    BALANCED = 2
+    HIGH = 3
+    MAX = 4
+
+
+# Arrow-native dtype tags (capi/include/quoin.h:97-115)
+class DType(enum.IntEnum):
+    BOOL = 0
+    I8 = 1
+    I16 = 2
+    I32 = 3
+    I64 = 4
+    U8 = 5
+    U16 = 6
+    U32 = 7
+    U64 = 8
+    F32 = 9
+    F64 = 10
+
+
+NP = {DType.BOOL: np.uint8, DType.I8: np.int8, DType.I16: np.int16, DType.I32: np.int32,
+      DType.I64: np.int64, DType.U8: np.uint8, DType.U16: np.uint16, DType.U32: np.uint32,
+      DType.U64: np.uint64, DType.F32: np.float32, DType.F64: np.float64}
+FROM_NP = {np.dtype(np.int8): DType.I8, np.dtype(np.int16): DType.I16,
+           np.dtype(np.int32): DType.I32, np.dtype(np.int64): DType.I64,
+           np.dtype(np.uint8): DType.U8, np.dtype(np.uint16): DType.U16,
+           np.dtype(np.uint32): DType.U32, np.dtype(np.uint64): DType.U64,
+           np.dtype(np.float32): DType.F32, np.dtype(np.float64): DType.F64}
+
+
+@dataclass
+class Config:
+    """src/lib.rs:194-245 (threads/selection/predictor_log2 have no GPU meaning:
+    the GPU path is Selection::Full with predictor_log2 = 16)."""
+    level: Level = Level.FAST
+    block_size: int = 0  # 0 = adaptive
+
+
+class _CConfig(ctypes.Structure):
+    _fields_ = [("level", ctypes.c_int32), ("block_size", ctypes.c_uint32)]
+
+
+class DeviceColumn(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int32), ("reserved", ctypes.c_uint32), ("n", ctypes.c_uint64),
+                ("values", ctypes.c_void_p), ("validity", ctypes.c_void_p),
+                ("dst", ctypes.c_void_p), ("dst_cap", ctypes.c_uint64)]
+
+
+class DeviceStream(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int32), ("reserved", ctypes.c_uint32), ("n", ctypes.c_uint64),
+                ("src", ctypes.c_void_p), ("host_src", ctypes.c_void_p),
+                ("src_len", ctypes.c_uint64), ("values_out", ctypes.c_void_p),
+                ("validity_out", ctypes.c_void_p)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run python quoin_amd/build.py")
+    lib = ctypes.CDLL(LIB_PATH)
+    sz, vp, u8p = ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p
+    lib.quoin_version.restype = ctypes.c_char_p
+    lib.quoin_compress_bound.restype = sz
+    lib.quoin_compress_bound.argtypes = [sz]
+    lib.quoin_typed_compress_bound.restype = sz
+    lib.quoin_typed_compress_bound.argtypes = [ctypes.c_int32, sz]
+    lib.quoin_dtype_width.restype = sz
+    lib.quoin_dtype_width.argtypes = [ctypes.c_int32]
+    lib.quoin_typed_compress_cfg.restype = sz
+    lib.quoin_typed_compress_cfg.argtypes = [ctypes.POINTER(_CConfig), ctypes.c_int32, sz, vp,
+                                             u8p, u8p, sz]
+    lib.quoin_typed_compress.restype = sz
+    lib.quoin_typed_compress.argtypes = [ctypes.c_int32, sz, vp, u8p, u8p, sz]
+    lib.quoin_typed_decompress.argtypes = [ctypes.c_int32, sz, u8p, sz, vp, u8p]
+    lib.quoin_compress.argtypes = [vp, sz, u8p, sz, ctypes.POINTER(sz)]
+    lib.quoin_compress_cfg.argtypes = [ctypes.POINTER(_CConfig), vp, sz, u8p, sz,
+                                       ctypes.POINTER(sz)]
+    lib.quoin_decompress.argtypes = [u8p, sz, vp, sz, ctypes.POINTER(sz)]
+    lib.quoin_decompressed_value_count.restype = ctypes.c_ssize_t
+    lib.quoin_decompressed_value_count.argtypes = [u8p, sz]
+    lib.quoin_mode_win_counts.argtypes = [vp]
+    lib.quoin_gpu_create.restype = vp
+    lib.quoin_gpu_create.argtypes = [ctypes.c_int, ctypes.c_uint64]
+    lib.quoin_gpu_destroy.argtypes = [vp]
+    lib.quoin_gpu_compress.argtypes = [vp, ctypes.POINTER(_CConfig), ctypes.POINTER(DeviceColumn),
+                                       sz, ctypes.POINTER(ctypes.c_uint64), vp]
+    lib.quoin_gpu_decompress.argtypes = [vp, ctypes.POINTER(DeviceStream), sz, vp]
+    lib.quoin_gpu_set_launch_form.argtypes = [vp, ctypes.c_int]
+    lib.quoin_gpu_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_char_p)]
+    return lib
+
+
+lib = _load()
+
+# every symbol include/quoin.h declares (checked by the CPU test suite)
+EXPORTS = [
+    "quoin_version", "quoin_compress_bound", "quoin_compress", "quoin_decompress",
+    "quoin_decompressed_value_count", "quoin_ctx_create", "quoin_ctx_free", "quoin_compress_ctx",
+    "quoin_decompress_ctx", "quoin_dtype_width", "quoin_typed_compress_bound",
+    "quoin_typed_compress", "quoin_typed_compress_decimal", "quoin_typed_decompress",
+    "quoin_typed_compress_cfg", "quoin_compress_cfg", "quoin_mode_win_counts",
+    "quoin_reset_mode_win_counts", "quoin_last_error", "quoin_gpu_create", "quoin_gpu_destroy", "quoin_gpu_compress",
+    "quoin_gpu_decompress", "quoin_gpu_set_launch_form", "quoin_gpu_last_timing",
+]
+
+
+def version():
+    return lib.quoin_version().decode()
+
+
+def _cfg(config):
+    config = config or Config()
+    return _CConfig(int(config.level), int(config.block_size))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+def compress_column(values, validity=None, config=None, dtype=None):
+    """Typed column -> stream bytes (src/lib.rs:429-455)."""
+    values = np.ascontiguousarray(values)
+    dtype = DType(dtype) if dtype is not None else FROM_NP[values.dtype]
+    values = values.astype(NP[dtype], copy=False)
+    if validity is not None:
+        validity = np.ascontiguousarray(validity, dtype=np.uint8)
+    n = len(values)
+    cap = lib.quoin_typed_compress_bound(int(dtype), n)
+    bs = (config or Config()).block_size
+    if bs:
+        cap += -(-n // bs) * 9  # 9 framing bytes per fixed-size block
+    dst = np.empty(cap, dtype=np.uint8)
+    cfg = _cfg(config)
+    got = lib.quoin_typed_compress_cfg(ctypes.byref(cfg), int(dtype), n,
+                                       _ptr(values) or ctypes.c_void_p(dst.ctypes.data),
+                                       _ptr(validity), ctypes.c_void_p(dst.ctypes.data), cap)
+    if got == 0:
+        raise QuoinError(lib.quoin_last_error(), "compress failed")
+    return dst[:got].tobytes()
+
+
+def decompress_column(stream, dtype, n, with_validity=False):
+    """Stream -> (values, validity|None) (src/lib.rs:462-508)."""
+    dtype = DType(dtype)
+    src = np.frombuffer(stream, dtype=np.uint8)
+    out = np.empty(max(n, 1), dtype=NP[dtype])
+    vout = np.empty(max((n + 7) // 8, 1), dtype=np.uint8) if with_validity else None
+    rc = lib.quoin_typed_decompress(int(dtype), n, ctypes.c_void_p(src.ctypes.data), len(src),
+                                    ctypes.c_void_p(out.ctypes.data),
+                                    ctypes.c_void_p(vout.ctypes.data) if vout is not None else None)
+    if rc != OK:
+        raise QuoinError(rc)
+    return out[:n], (vout[:(n + 7) // 8] if vout is not None else None)
+
+
+def compress(values, config=None):
+    """f64 convenience (src/lib.rs:514-516)."""
+    return compress_column(np.asarray(values, dtype=np.float64), None, config)
+
+
+def decompress(stream):
+    """f64 convenience (src/lib.rs:522-527)."""
+    src = np.frombuffer(stream, dtype=np.uint8)
+    n = lib.quoin_decompressed_value_count(ctypes.c_void_p(src.ctypes.data), len(src))
+    if n < 0:
+        raise QuoinError(int(n))
+    out = np.empty(max(n, 1), dtype=np.float64)
+    got = ctypes.c_size_t()
+    rc = lib.quoin_decompress(ctypes.c_void_p(src.ctypes.data), len(src),
+                              ctypes.c_void_p(out.ctypes.data), n, ctypes.byref(got))
+    if rc != OK:
+        raise QuoinError(rc)
+    return out[:n]
+
+
+def mode_win_counts():
+    out = np.zeros(64, dtype=np.uint64)
+    lib.quoin_mode_win_counts(ctypes.c_void_p(out.ctypes.data))
+    return out
+
+
+def reset_mode_win_counts():
+    lib.quoin_reset_mode_win_counts()
+
+
+class Gpu:
+    """Handle over the device-resident batched API (include/quoin.h)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self.h = lib.quoin_gpu_create(device, hbm_budget)
+        if not self.h:
+            raise QuoinError(ERR_DEVICE, "no usable GPU")
+
+    def close(self):
+        if self.h:
+            lib.quoin_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_launch_form(self, form):
+        rc = lib.quoin_gpu_set_launch_form(self.h, form)
+        if rc != OK:
+            raise QuoinError(rc)
+
+    def compress(self, cols, config=None, stream=None):
+        """cols: list of DeviceColumn; returns stream lengths."""
+        arr = (DeviceColumn * len(cols))(*cols)
+        lens = (ctypes.c_uint64 * len(cols))()
+        cfg = _cfg(config)
+        rc = lib.quoin_gpu_compress(self.h, ctypes.byref(cfg), arr, len(cols), lens, stream)
+        if rc != OK:
+            raise QuoinError(rc)
+        return [int(x) for x in lens]
+
+    def decompress(self, streams, stream=None):
+        arr = (DeviceStream * len(streams))(*streams)
+        rc = lib.quoin_gpu_decompress(self.h, arr, len(streams), stream)
+        if rc != OK:
+            raise QuoinError(rc)
+
+    def last_timing(self):
+        t, k = ctypes.c_double(), ctypes.c_double()
+        name = ctypes.c_char_p()
+        lib.quoin_gpu_last_timing(self.h, ctypes.byref(t), ctypes.byref(k), ctypes.byref(name))
+        return t.value, k.value, (name.value or b"").decode()

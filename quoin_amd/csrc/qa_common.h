@@ -1,0 +1,203 @@
+// qa_common.h — structures shared by the host engine and the gfx950 kernels of
+// the quoin block path, plus small device helpers.
+//
+// Layout in HBM (one batch = many columns):
+//   lane[]   : every column lowered to its u64 lane (src/lib.rs:389-403) and,
+//              with nulls, compacted to its valid values (src/validity.rs:122),
+//              each column starting at a 1024-aligned offset.
+//   regions  : per column, ceil(n / R) region slots of R values (R = 32768 for
+//              adaptive planning, the fixed block size otherwise). A block
+//              starts at a region slot (blocks are always region-aligned:
+//              src/encoder.rs:142-178), so per-block state is indexed by the
+//              slot where the block starts.
+//   tiles    : per region, T = ceil(R / 1024) tile slots of 1024 values — the
+//              FastLanes sub-block (src/bitpack.rs:22). A 128 Ki block spans
+//              4 regions = 128 consecutive tile slots.
+#pragma once
+#include <stdint.h>
+
+#define QA_TILE 1024
+#define QA_BASE_Q 32768
+#define QA_MAX_Q 131072
+
+// engine dtype wire ids (src/dtype.rs:70-80)
+enum { QA_F64 = 0, QA_I64 = 1, QA_U64 = 2, QA_I32 = 3, QA_U32 = 4, QA_F32 = 5 };
+enum { QA_FASTEST = 0, QA_FAST = 1 };
+
+// mode ids (src/mode.rs:9-85)
+enum {
+    QA_M_CONST = 1, QA_M_STRIDE = 2, QA_M_XORZ = 3, QA_M_RAW = 5, QA_M_ALP = 9,
+    QA_M_FLOAT_MULT = 17, QA_M_DICT = 19, QA_M_FOR = 50, QA_M_DELTA = 51, QA_M_ALP_RD = 52,
+    QA_M_RLE = 53
+};
+
+// error flags raised by kernels (host maps them to QUOIN_ERR_*)
+enum { QA_ERR_CORRUPT = 1, QA_ERR_UNSUPPORTED = 2, QA_ERR_OVERFLOW = 4 };
+
+struct DCol {
+    const void *values;       // typed input, device
+    const uint8_t *validity;  // device bitmap or null
+    uint8_t *dst;             // device stream output
+    uint64_t dst_cap;
+    uint64_t n;               // logical values
+    uint64_t lane_off;        // lane[] offset (values)
+    uint64_t tile_base;       // first global tile slot
+    uint64_t vword_base;      // first validity 64-bit word slot (global)
+    uint64_t val_base;        // first logical value (global, columns concatenated)
+    uint32_t nvwords;         // ceil(n / 64) when a bitmap is given, else 0
+    uint32_t dtype;           // engine wire id
+    uint32_t in_width;        // 4 or 8
+    uint32_t region_size;     // R
+    uint32_t tpr;             // tiles per region
+    uint32_t region_base;     // first global region slot
+    uint32_t nregions;        // region slots (upper bound from n)
+    uint32_t adaptive;
+    uint32_t level;
+    uint32_t in_kind;         // QA_IN_*: input element type when narrower than the lane
+};
+
+// narrow Arrow inputs widened on load (capi/src/typed.rs:147-153)
+enum { QA_IN_NATIVE = 0, QA_IN_I8 = 1, QA_IN_I16 = 2, QA_IN_U8 = 3, QA_IN_U16 = 4 };
+
+struct DColState {
+    uint64_t nv;          // compacted (valid) value count
+    uint64_t vblob_len;   // validity blob bytes (0 without nulls)
+    uint64_t vblob_rle;   // RLE form bytes (without tag)
+    uint64_t frames_off;  // first frame offset within the stream
+    uint64_t stream_len;  // total stream bytes
+    uint32_t has_nulls;
+    uint32_t nblocks;
+    uint32_t err;
+    uint32_t pad;
+};
+
+// candidate slots of the competition, in the reference's consider order
+enum { C_RAW = 0, C_CONST, C_STRIDE, C_XORZ, C_FOR, C_DELTA, C_FM, C_ALP, C_ALPRD, C_DICT, C_RLE,
+       C_N };
+
+struct DBlock {
+    uint64_t start;        // value offset in the column's lane
+    uint64_t payload_len;  // winner payload bytes
+    uint64_t frame_off;    // frame offset within the column stream
+    uint64_t v0, v1;       // first two values (CONST / STRIDE / DELTA base)
+    uint64_t sz[C_N];      // candidate payload sizes (UINT64_MAX = bailed)
+    uint32_t len;          // values; 0 = no block starts at this slot
+    uint32_t col;
+    uint32_t ntiles;
+    uint32_t mode;
+    uint32_t exp_range;
+    uint8_t distinct_low, repeats, incompressible, compressible;
+};
+
+struct DTile {
+    uint64_t umin, umax;
+    int64_t smin, smax;
+    uint64_t dzmin, dzmax;   // zig-zag deltas (src/codecs/delta_bitpack.rs:32-46)
+    uint64_t xorz_bytes;     // sum of varint lengths (src/codecs/xorz.rs:8-16)
+    uint64_t off;            // this sub-block's offset inside the winner payload
+    uint64_t tail;           // carried value (delta decode prefix)
+    uint32_t const_bad, stride_bad, changes, cnt;
+    uint32_t w, active, pad0, pad1;
+};
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+
+__device__ __forceinline__ uint64_t qa_zigzag(uint64_t n) {
+    return (n << 1) ^ (uint64_t)((int64_t)n >> 63);
+}
+__device__ __forceinline__ uint64_t qa_unzigzag(uint64_t z) { return (z >> 1) ^ (0ull - (z & 1)); }
+__device__ __forceinline__ uint32_t qa_width(uint64_t range) {
+    return range == 0 ? 0u : 64u - (uint32_t)__clzll((long long)range);
+}
+__device__ __forceinline__ uint32_t qa_varint_len(uint64_t v) {
+    // 1 + floor(bits/7) for v > 0
+    uint32_t bits = v ? 64u - (uint32_t)__clzll((long long)v) : 1u;
+    return (bits + 6) / 7;
+}
+
+// Unaligned little-endian loads from a byte stream. The stream buffer always
+// carries >= 8 bytes of tail padding so the second aligned word is in bounds.
+__device__ __forceinline__ uint32_t qa_ld32u(const uint8_t *p) {
+    uintptr_t a = (uintptr_t)p;
+    const uint32_t *w = (const uint32_t *)(a & ~(uintptr_t)3);
+    uint32_t s = (uint32_t)(a & 3);
+    uint32_t lo = w[0];
+    if (s == 0) return lo;
+    uint32_t hi = w[1];
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * s));
+}
+__device__ __forceinline__ uint64_t qa_ld64u(const uint8_t *p) {
+    return (uint64_t)qa_ld32u(p) | ((uint64_t)qa_ld32u(p + 4) << 32);
+}
+__device__ __forceinline__ void qa_st8(uint8_t *p, uint8_t v) { *p = v; }
+__device__ __forceinline__ void qa_st32u(uint8_t *p, uint32_t v) {
+    p[0] = (uint8_t)v;
+    p[1] = (uint8_t)(v >> 8);
+    p[2] = (uint8_t)(v >> 16);
+    p[3] = (uint8_t)(v >> 24);
+}
+__device__ __forceinline__ void qa_st64u(uint8_t *p, uint64_t v) {
+    qa_st32u(p, (uint32_t)v);
+    qa_st32u(p + 4, (uint32_t)(v >> 32));
+}
+// write a LEB128 varint (src/varint.rs:9-21); returns bytes written
+__device__ __forceinline__ uint32_t qa_put_varint(uint8_t *p, uint64_t v) {
+    uint32_t k = 0;
+    for (;;) {
+        uint8_t b = (uint8_t)(v & 0x7f);
+        v >>= 7;
+        if (v) {
+            p[k++] = b | 0x80;
+        } else {
+            p[k++] = b;
+            return k;
+        }
+    }
+}
+
+// f32 <-> f64 lane conversions as integer operations with x86 semantics
+// (src/lib.rs:398 `f as f64`, src/codecs/raw.rs:16): exact widening, signaling
+// NaNs are quieted, subnormals are preserved (no FTZ on any path).
+__device__ __forceinline__ uint64_t qa_f32_to_lane(uint32_t b) {
+    uint64_t sign = (uint64_t)(b >> 31) << 63;
+    uint32_t e = (b >> 23) & 0xFF, m = b & 0x7FFFFF;
+    if (e == 0xFF) {
+        if (m) m |= 0x400000;  // quiet
+        return sign | (0x7FFull << 52) | ((uint64_t)m << 29);
+    }
+    if (e == 0) {
+        if (m == 0) return sign;
+        // subnormal f32: value = m * 2^-149; normalize
+        // leading one at bit p = 23 - lz; value = 1.f * 2^(p - 149)
+        uint32_t lz = (uint32_t)__clz((int)m) - 8;
+        uint64_t mm = (uint64_t)((m << lz) & 0x7FFFFFu);
+        uint64_t ee = 897u - lz;
+        return sign | (ee << 52) | (mm << 29);
+    }
+    return sign | ((uint64_t)(e - 127 + 1023) << 52) | ((uint64_t)m << 29);
+}
+// narrowing of a lane that came from an f32 (exact)
+__device__ __forceinline__ uint32_t qa_lane_to_f32(uint64_t x) {
+    uint32_t sign = (uint32_t)(x >> 63) << 31;
+    uint32_t e = (uint32_t)(x >> 52) & 0x7FF;
+    uint64_t m = x & 0xFFFFFFFFFFFFFull;
+    if (e == 0x7FF) {
+        if (m) return sign | 0x7F800000u | 0x400000u | (uint32_t)(m >> 29);
+        return sign | 0x7F800000u;
+    }
+    if (e == 0) return sign;  // f64 subnormals are far below the f32 range
+    int32_t ee = (int32_t)e - 1023 + 127;
+    if (ee >= 0xFF) return sign | 0x7F800000u;
+    if (ee <= 0) {
+        // f32 subnormal: exact when the lane came from an f32
+        uint64_t full = m | (1ull << 52);
+        uint32_t sh = (uint32_t)(1 - ee) + 29;
+        return sign | (uint32_t)(full >> sh);
+    }
+    return sign | ((uint32_t)ee << 23) | (uint32_t)(m >> 29);
+}
+#endif

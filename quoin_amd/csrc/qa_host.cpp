@@ -1,0 +1,743 @@
+// qa_host.cpp — host side of the MI355X quoin block path: batch layout, device
+// arenas, the frame scan, and every exported C-ABI symbol of include/quoin.h.
+//
+// The host never computes codec output: it lays out descriptors, launches the
+// gfx950 kernels and copies results. A missing or unusable GPU is an error
+// (QUOIN_ERR_DEVICE), never a silent CPU path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/quoin.h"
+#include "qa_kernels.h"
+
+namespace {
+
+constexpr uint64_t kDefaultBudget = 64ull << 30;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+// grow-only device buffer
+bool ensure(DevBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    if (b.cap >= bytes) return true;
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 4 + 256;
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        b.p = nullptr;
+        return false;
+    }
+    b.cap = want;
+    return true;
+}
+
+template <typename T>
+T *dp(DevBuf &b) { return static_cast<T *>(b.p); }
+
+std::atomic<uint64_t> g_wins[64];
+thread_local int t_last_err = 0;
+
+}  // namespace
+
+struct QuoinGpu {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = kDefaultBudget;
+    int form = 0;
+    std::mutex mu;
+    // encode arenas
+    DevBuf cols, cst, lane, vpre, tpre, tpos, rsz, colbase, scan_tmp, rfeat, owner, blocks, tiles,
+        wins;
+    // decode arenas
+    DevBuf dcols, frames, dtiles, dlane, dvpre, derr, dbitmaps, dscan;
+    // host-API staging
+    DevBuf st_in, st_valid, st_out, st_vout;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_total_ms = 0, last_kernel_ms = 0;
+    const char *last_kernel = "";
+};
+
+namespace {
+
+__global__ void k_zero_u64(uint64_t *p, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+void zero_async(void *p, size_t bytes, hipStream_t s) {
+    uint64_t n = (bytes + 7) / 8;
+    if (n) hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                              (uint64_t *)p, n);
+}
+
+// typed-API tag -> (engine dtype, narrow input kind, input width)
+bool map_dtype(int32_t t, uint32_t *dt, uint32_t *kind, uint32_t *width) {
+    *kind = QA_IN_NATIVE;
+    switch (t) {
+    case QUOIN_DTYPE_F64: *dt = QA_F64; *width = 8; return true;
+    case QUOIN_DTYPE_F32: *dt = QA_F32; *width = 4; return true;
+    case QUOIN_DTYPE_I64: *dt = QA_I64; *width = 8; return true;
+    case QUOIN_DTYPE_U64: *dt = QA_U64; *width = 8; return true;
+    case QUOIN_DTYPE_I32: *dt = QA_I32; *width = 4; return true;
+    case QUOIN_DTYPE_U32: *dt = QA_U32; *width = 4; return true;
+    case QUOIN_DTYPE_I16: *dt = QA_I32; *kind = QA_IN_I16; *width = 2; return true;
+    case QUOIN_DTYPE_I8: *dt = QA_I32; *kind = QA_IN_I8; *width = 1; return true;
+    case QUOIN_DTYPE_U16: *dt = QA_U32; *kind = QA_IN_U16; *width = 2; return true;
+    case QUOIN_DTYPE_U8:
+    case QUOIN_DTYPE_BOOL: *dt = QA_U32; *kind = QA_IN_U8; *width = 1; return true;
+    default: return false;
+    }
+}
+
+uint64_t rd64(const uint8_t *p) {
+    uint64_t v;
+    memcpy(&v, p, 8);
+    return v;
+}
+uint32_t rd32(const uint8_t *p) {
+    uint32_t v;
+    memcpy(&v, p, 4);
+    return v;
+}
+bool rd_varint(const uint8_t *p, size_t len, size_t *pos, uint64_t *out) {
+    uint64_t r = 0;
+    uint32_t sh = 0;
+    for (;;) {
+        if (*pos >= len) return false;
+        uint8_t b = p[(*pos)++];
+        r |= (uint64_t)(b & 0x7f) << sh;
+        if (!(b & 0x80)) {
+            *out = r;
+            return true;
+        }
+        sh += 7;
+        if (sh >= 64) return false;
+    }
+}
+bool known_mode(int m) {
+    // src/mode.rs:45-69
+    switch (m) {
+    case 0: case 1: case 2: case 3: case 4: case 5: case 7: case 9: case 11: case 17: case 19:
+    case 20: case 25: case 29: case 45: case 50: case 51: case 52: case 53: case 54: return true;
+    default: return false;
+    }
+}
+bool gpu_mode(int m, int) {
+    return m == QA_M_RAW || m == QA_M_CONST || m == QA_M_STRIDE || m == QA_M_FOR || m == QA_M_DELTA;
+}
+
+int level_supported(int level) { return level == QUOIN_LEVEL_FASTEST; }
+
+std::mutex g_default_mu;
+QuoinGpu *g_default = nullptr;
+
+QuoinGpu *default_gpu() {
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    if (!g_default) g_default = quoin_gpu_create(-1, 0);
+    return g_default;
+}
+
+// Validity blob decode on the host from the stream bytes (src/validity.rs:73-119)
+// into a word-padded bitmap; returns false on a malformed blob.
+bool host_validity(const uint8_t *blob, size_t blen, uint64_t n, std::vector<uint64_t> &words) {
+    uint64_t raw_len = (n + 7) / 8;
+    words.assign((size_t)((n + 63) / 64) + 1, 0);
+    if (blen == 0) return false;
+    uint8_t tag = blob[0];
+    const uint8_t *rest = blob + 1;
+    size_t rl = blen - 1;
+    if (tag == 0) {
+        if (rl < raw_len) return false;
+        memcpy(words.data(), rest, raw_len);
+        if (n & 7) ((uint8_t *)words.data())[raw_len - 1] &= (uint8_t)((1u << (n & 7)) - 1);
+        return true;
+    }
+    if (tag != 1) return false;
+    size_t pos = 0;
+    uint64_t i = 0;
+    int cur = 0;
+    while (i < n) {
+        uint64_t run;
+        if (!rd_varint(rest, rl, &pos, &run)) return false;
+        if (run > n - i) return false;
+        if (cur) {
+            uint64_t a = i, b = i + run;
+            while (a < b) {
+                uint64_t w = a >> 6, off = a & 63;
+                uint64_t take = std::min<uint64_t>(64 - off, b - a);
+                uint64_t m = take == 64 ? ~0ull : (((1ull << take) - 1) << off);
+                words[w] |= m;
+                a += take;
+            }
+        }
+        i += run;
+        cur ^= 1;
+    }
+    return true;
+}
+
+struct ParsedStream {
+    uint32_t dtype = 0;
+    uint64_t n = 0, nv = 0;
+    bool has_validity = false;
+    std::vector<uint64_t> bitmap;
+    std::vector<DFrame> frames;
+};
+
+// header + validity + sequential frame scan (src/format.rs:65-94,
+// src/decoder.rs:43-94); returns a QUOIN_ERR code
+int parse_stream(const uint8_t *src, size_t len, ParsedStream &ps) {
+    if (len < 16) return QUOIN_ERR_CORRUPT;
+    if (memcmp(src, "FCR1", 4) != 0 || src[4] != 2) return QUOIN_ERR_CORRUPT;
+    if (src[5] & 0x02) return QUOIN_ERR_UNSUPPORTED;  // decimal container
+    if (src[5] & ~0x01) return QUOIN_ERR_CORRUPT;
+    if (src[6] < 10 || src[6] > 16) return QUOIN_ERR_CORRUPT;
+    if (src[7] > 7) return QUOIN_ERR_CORRUPT;
+    if (src[7] > 5) return QUOIN_ERR_CORRUPT;
+    ps.dtype = src[7];
+    ps.n = rd64(src + 8);
+    ps.has_validity = src[5] & 1;
+    if (ps.n > (1ull << 40)) return QUOIN_ERR_CORRUPT;
+    size_t pos = 16;
+    ps.nv = ps.n;
+    if (ps.has_validity) {
+        uint64_t vlen;
+        if (!rd_varint(src, len, &pos, &vlen)) return QUOIN_ERR_CORRUPT;
+        if (vlen > len - pos) return QUOIN_ERR_CORRUPT;
+        if (!host_validity(src + pos, (size_t)vlen, ps.n, ps.bitmap)) return QUOIN_ERR_CORRUPT;
+        pos += vlen;
+        uint64_t c = 0;
+        for (uint64_t w : ps.bitmap) c += (uint64_t)__builtin_popcountll(w);
+        ps.nv = c;
+    }
+    uint64_t counted = 0;
+    bool unsupported = false;
+    while (counted < ps.nv) {
+        if (len - pos < 9) return QUOIN_ERR_CORRUPT;
+        int mode = src[pos];
+        if (!known_mode(mode)) return QUOIN_ERR_CORRUPT;
+        uint64_t fn = rd32(src + pos + 1), pl = rd32(src + pos + 5);
+        if (fn > QA_MAX_Q) return QUOIN_ERR_CORRUPT;
+        pos += 9;
+        if (pl > len - pos) return QUOIN_ERR_CORRUPT;
+        if (!gpu_mode(mode, 0)) unsupported = true;
+        DFrame f{};
+        f.payload_off = pos;
+        f.out_off = counted;
+        f.n = (uint32_t)fn;
+        f.plen = (uint32_t)pl;
+        f.mode = (uint32_t)mode;
+        f.ntiles = (uint32_t)((fn + QA_TILE - 1) / QA_TILE);
+        ps.frames.push_back(f);
+        pos += pl;
+        counted += fn;
+        if (counted > ps.nv) return QUOIN_ERR_CORRUPT;
+    }
+    if (unsupported) return QUOIN_ERR_UNSUPPORTED;
+    return QUOIN_OK;
+}
+
+inline unsigned blocks_of(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// device-resident batched API
+// ---------------------------------------------------------------------------
+extern "C" QuoinGpu *quoin_gpu_create(int device, uint64_t hbm_budget_bytes) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return nullptr;
+    if (device < 0) device = 0;
+    if (device >= count) return nullptr;
+    if (hipSetDevice(device) != hipSuccess) return nullptr;
+    QuoinGpu *g = new (std::nothrow) QuoinGpu();
+    if (!g) return nullptr;
+    g->device = device;
+    g->budget = hbm_budget_bytes ? hbm_budget_bytes : kDefaultBudget;
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
+        delete g;
+        return nullptr;
+    }
+    return g;
+}
+
+extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
+    if (!g) return;
+    hipSetDevice(g->device);
+    hipStreamSynchronize(g->stream);
+    DevBuf *all[] = {&g->cols,  &g->cst,      &g->lane,   &g->vpre,    &g->tpre,    &g->tpos,
+                     &g->rsz,   &g->colbase,  &g->scan_tmp, &g->rfeat, &g->owner,   &g->blocks,
+                     &g->tiles, &g->wins,     &g->dcols,  &g->frames,  &g->dtiles,  &g->dlane,
+                     &g->dvpre, &g->derr,     &g->dbitmaps, &g->dscan, &g->st_in,   &g->st_valid,
+                     &g->st_out, &g->st_vout};
+    for (DevBuf *b : all)
+        if (b->p) hipFree(b->p);
+    hipEventDestroy(g->ev0);
+    hipEventDestroy(g->ev1);
+    hipStreamDestroy(g->stream);
+    delete g;
+}
+
+extern "C" int quoin_gpu_set_launch_form(QuoinGpu *g, int form) {
+    if (!g) return QUOIN_ERR_NULL;
+    if (form != 0 && form != 1) return QUOIN_ERR_UNSUPPORTED;
+    g->form = form;
+    return QUOIN_OK;
+}
+
+extern "C" int quoin_gpu_last_timing(const QuoinGpu *g, double *total_ms, double *kernel_ms,
+                                     const char **kernel_name) {
+    if (!g) return QUOIN_ERR_NULL;
+    if (total_ms) *total_ms = g->last_total_ms;
+    if (kernel_ms) *kernel_ms = g->last_kernel_ms;
+    if (kernel_name) *kernel_name = g->last_kernel;
+    return QUOIN_OK;
+}
+
+static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinDeviceColumn *in,
+                               size_t ncols, uint64_t *out_lens, hipStream_t s) {
+    int level = cfg ? cfg->level : QUOIN_DEFAULT_LEVEL;
+    if (level < 0 || level > QUOIN_LEVEL_MAX) return QUOIN_ERR_UNSUPPORTED;
+    if (!level_supported(level)) return QUOIN_ERR_UNSUPPORTED;
+    uint32_t bs = cfg ? cfg->block_size : 0;
+    if (bs > QA_MAX_Q) bs = QA_MAX_Q;
+    if (ncols == 0) return QUOIN_OK;
+    std::vector<DCol> cols(ncols);
+    uint64_t lane_off = 0, tile_base = 0, vword_base = 0, val_base = 0, ntrans = 0;
+    uint32_t region_base = 0;
+    for (size_t c = 0; c < ncols; c++) {
+        const QuoinDeviceColumn &q = in[c];
+        DCol &d = cols[c];
+        memset(&d, 0, sizeof d);
+        uint32_t dt, kind, width;
+        if (!map_dtype(q.dtype, &dt, &kind, &width)) return QUOIN_ERR_UNSUPPORTED;
+        if ((q.n && !q.values) || !q.dst) return QUOIN_ERR_NULL;
+        d.values = q.values;
+        d.validity = q.n ? q.validity : nullptr;
+        d.dst = q.dst;
+        d.dst_cap = q.dst_cap;
+        d.n = q.n;
+        d.dtype = dt;
+        d.in_kind = kind;
+        d.in_width = width;
+        d.level = (uint32_t)level;
+        d.adaptive = bs == 0;
+        d.region_size = bs ? bs : QA_BASE_Q;
+        d.tpr = (d.region_size + QA_TILE - 1) / QA_TILE;
+        d.nregions = (uint32_t)((q.n + d.region_size - 1) / d.region_size);
+        d.region_base = region_base;
+        d.tile_base = tile_base;
+        d.lane_off = lane_off;
+        d.val_base = val_base;
+        d.vword_base = vword_base;
+        d.nvwords = d.validity ? (uint32_t)((q.n + 63) / 64) : 0;
+        region_base += d.nregions;
+        tile_base += (uint64_t)d.nregions * d.tpr;
+        lane_off += (q.n + QA_TILE - 1) / QA_TILE * QA_TILE;
+        val_base += q.n;
+        vword_base += d.nvwords;
+        if (d.validity) ntrans += q.n;
+    }
+    uint64_t nruns_max = ntrans + ncols;
+    size_t scan_words = qa_scan_tmp_words(std::max<uint64_t>(vword_base, nruns_max) + 1);
+    uint64_t need = lane_off * 8 + (vword_base + 1) * 16 + ntrans * 8 + (nruns_max + 1) * 8 +
+                    (uint64_t)region_base * (sizeof(DBlock) + 8) + tile_base * sizeof(DTile);
+    if (need > g->budget) return QUOIN_ERR_BUFFER_TOO_SMALL;
+    if (!ensure(g->cols, ncols * sizeof(DCol)) || !ensure(g->cst, ncols * sizeof(DColState)) ||
+        !ensure(g->lane, lane_off * 8) || !ensure(g->vpre, (vword_base + 1) * 8) ||
+        !ensure(g->tpre, (vword_base + 1) * 8) || !ensure(g->tpos, ntrans * 8) ||
+        !ensure(g->rsz, (nruns_max + 1) * 8) || !ensure(g->colbase, (ncols + 2) * 8) ||
+        !ensure(g->scan_tmp, scan_words * 8) || !ensure(g->rfeat, region_base * 4ull) ||
+        !ensure(g->owner, region_base * 4ull) || !ensure(g->blocks, region_base * sizeof(DBlock)) ||
+        !ensure(g->tiles, tile_base * sizeof(DTile)) || !ensure(g->wins, 64 * 8))
+        return QUOIN_ERR_DEVICE;
+    EncArgs a{};
+    a.cols = dp<DCol>(g->cols);
+    a.cst = dp<DColState>(g->cst);
+    a.ncols = (uint32_t)ncols;
+    a.nregions = region_base;
+    a.ntiles = tile_base;
+    a.nvals = val_base;
+    a.nvwords = vword_base;
+    a.lane = dp<uint64_t>(g->lane);
+    a.vpre = dp<uint64_t>(g->vpre);
+    a.tpre = dp<uint64_t>(g->tpre);
+    a.tpos = dp<uint64_t>(g->tpos);
+    a.rsz = dp<uint64_t>(g->rsz);
+    a.colbase = dp<uint64_t>(g->colbase);
+    a.scan_tmp = dp<uint64_t>(g->scan_tmp);
+    a.rfeat = dp<uint32_t>(g->rfeat);
+    a.owner = dp<uint32_t>(g->owner);
+    a.blocks = dp<DBlock>(g->blocks);
+    a.tiles = dp<DTile>(g->tiles);
+    a.ntrans_max = ntrans;
+    a.wins = dp<unsigned long long>(g->wins);
+    if (hipMemcpyAsync(g->cols.p, cols.data(), ncols * sizeof(DCol), hipMemcpyHostToDevice, s) !=
+        hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    hipEventRecord(g->ev0, s);
+    zero_async(g->cst.p, ncols * sizeof(DColState), s);
+    zero_async(g->wins.p, 64 * 8, s);
+    qa_encode_launch(a, s, level);
+    hipEventRecord(g->ev1, s);
+    std::vector<DColState> st(ncols);
+    uint64_t wins[64];
+    hipMemcpyAsync(st.data(), g->cst.p, ncols * sizeof(DColState), hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(wins, g->wins.p, sizeof wins, hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) return QUOIN_ERR_DEVICE;
+    if (hipGetLastError() != hipSuccess) return QUOIN_ERR_DEVICE;
+    float ms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    g->last_total_ms = ms;
+    g->last_kernel_ms = 0;
+    g->last_kernel = "";
+    for (int m = 0; m < 64; m++) g_wins[m] += wins[m];
+    int rc = QUOIN_OK;
+    for (size_t c = 0; c < ncols; c++) {
+        out_lens[c] = st[c].stream_len;
+        if (st[c].err & QA_ERR_OVERFLOW) rc = QUOIN_ERR_BUFFER_TOO_SMALL;
+        else if (st[c].err) rc = QUOIN_ERR_PANIC;
+    }
+    return rc;
+}
+
+extern "C" int quoin_gpu_compress(QuoinGpu *g, const QuoinConfig *cfg, const QuoinDeviceColumn *cols,
+                                  size_t ncols, uint64_t *out_lens, void *stream) {
+    if (!g || (ncols && (!cols || !out_lens))) return QUOIN_ERR_NULL;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    return gpu_compress_locked(g, cfg, cols, ncols, out_lens, s);
+}
+
+static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_t ns,
+                                 hipStream_t s) {
+    if (ns == 0) return QUOIN_OK;
+    std::vector<ParsedStream> ps(ns);
+    std::vector<DDecCol> cols(ns);
+    std::vector<DFrame> frames;
+    uint64_t tile_base = 0, lane_off = 0, vword_base = 0, val_base = 0, bm_words = 0;
+    for (size_t c = 0; c < ns; c++) {
+        const QuoinDeviceStream &q = in[c];
+        if (!q.src || !q.host_src || (q.n && !q.values_out)) return QUOIN_ERR_NULL;
+        uint32_t dt, kind, width;
+        if (!map_dtype(q.dtype, &dt, &kind, &width)) return QUOIN_ERR_UNSUPPORTED;
+        int rc = parse_stream(q.host_src, (size_t)q.src_len, ps[c]);
+        if (rc) return rc;
+        if (ps[c].dtype != dt || ps[c].n != q.n) return QUOIN_ERR_CORRUPT;
+        DDecCol &d = cols[c];
+        memset(&d, 0, sizeof d);
+        d.src = q.src;
+        d.out = q.values_out;
+        d.validity_out = q.validity_out;
+        d.src_len = q.src_len;
+        d.n = q.n;
+        d.nv = ps[c].nv;
+        d.dtype = dt;
+        d.out_kind = kind;
+        d.lane_off = lane_off;
+        d.val_base = val_base;
+        d.vword_base = vword_base;
+        if (ps[c].has_validity) {
+            d.nvwords = (uint32_t)((q.n + 63) / 64);
+            lane_off += (ps[c].nv + QA_TILE - 1) / QA_TILE * QA_TILE;
+            bm_words += ps[c].bitmap.size();
+        }
+        vword_base += d.nvwords;
+        val_base += q.n;
+        for (DFrame f : ps[c].frames) {
+            f.col = (uint32_t)c;
+            f.tile_base = tile_base;
+            tile_base += f.ntiles;
+            frames.push_back(f);
+        }
+    }
+    if (!ensure(g->dcols, ns * sizeof(DDecCol)) || !ensure(g->frames, frames.size() * sizeof(DFrame)) ||
+        !ensure(g->dtiles, tile_base * sizeof(DTile)) || !ensure(g->dlane, lane_off * 8) ||
+        !ensure(g->dvpre, (vword_base + 1) * 8) || !ensure(g->derr, 8) ||
+        !ensure(g->dbitmaps, bm_words * 8) ||
+        !ensure(g->dscan, qa_scan_tmp_words(vword_base + 1) * 8))
+        return QUOIN_ERR_DEVICE;
+    // bitmaps
+    uint64_t bw = 0;
+    for (size_t c = 0; c < ns; c++) {
+        if (!ps[c].has_validity) continue;
+        uint64_t *dst = dp<uint64_t>(g->dbitmaps) + bw;
+        hipMemcpyAsync(dst, ps[c].bitmap.data(), ps[c].bitmap.size() * 8, hipMemcpyHostToDevice, s);
+        cols[c].bitmap = (const uint8_t *)dst;
+        bw += ps[c].bitmap.size();
+    }
+    hipMemcpyAsync(g->dcols.p, cols.data(), ns * sizeof(DDecCol), hipMemcpyHostToDevice, s);
+    if (!frames.empty())
+        hipMemcpyAsync(g->frames.p, frames.data(), frames.size() * sizeof(DFrame),
+                       hipMemcpyHostToDevice, s);
+    DecArgs a{};
+    a.cols = dp<DDecCol>(g->dcols);
+    a.ncols = (uint32_t)ns;
+    a.frames = dp<DFrame>(g->frames);
+    a.nframes = (uint32_t)frames.size();
+    a.ntiles = tile_base;
+    a.nvals = val_base;
+    a.nvwords = vword_base;
+    a.lane = dp<uint64_t>(g->dlane);
+    a.vpre = dp<uint64_t>(g->dvpre);
+    a.tiles = dp<DTile>(g->dtiles);
+    a.err = dp<uint32_t>(g->derr);
+    a.scan_tmp = dp<uint64_t>(g->dscan);
+    hipEventRecord(g->ev0, s);
+    zero_async(g->derr.p, 8, s);
+    qa_decode_launch(a, s);
+    hipEventRecord(g->ev1, s);
+    uint32_t err = 0;
+    hipMemcpyAsync(&err, g->derr.p, 4, hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) return QUOIN_ERR_DEVICE;
+    if (hipGetLastError() != hipSuccess) return QUOIN_ERR_DEVICE;
+    float ms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    g->last_total_ms = ms;
+    if (err & QA_ERR_UNSUPPORTED) return QUOIN_ERR_UNSUPPORTED;
+    if (err) return QUOIN_ERR_CORRUPT;
+    return QUOIN_OK;
+}
+
+extern "C" int quoin_gpu_decompress(QuoinGpu *g, const QuoinDeviceStream *streams, size_t ns,
+                                    void *stream) {
+    if (!g || (ns && !streams)) return QUOIN_ERR_NULL;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    return gpu_decompress_locked(g, streams, ns, s);
+}
+
+// ---------------------------------------------------------------------------
+// host-pointer C ABI (capi/src/lib.rs, capi/src/typed.rs)
+// ---------------------------------------------------------------------------
+extern "C" const char *quoin_version(void) { return "quoin-amd 0.1.0 (gfx950)"; }
+
+extern "C" size_t quoin_compress_bound(size_t n) {
+    // capi/src/lib.rs:37-40: 9n + 1024, saturating
+    if (n > (SIZE_MAX - 1024) / 9) return SIZE_MAX;
+    return n * 9 + 1024;
+}
+
+extern "C" size_t quoin_dtype_width(int32_t dtype) {
+    switch (dtype) {  // capi/src/typed.rs:51-62
+    case QUOIN_DTYPE_BOOL: case QUOIN_DTYPE_I8: case QUOIN_DTYPE_U8: return 1;
+    case QUOIN_DTYPE_I16: case QUOIN_DTYPE_U16: return 2;
+    case QUOIN_DTYPE_I32: case QUOIN_DTYPE_U32: case QUOIN_DTYPE_F32: case QUOIN_DTYPE_DECIMAL32:
+        return 4;
+    case QUOIN_DTYPE_I64: case QUOIN_DTYPE_U64: case QUOIN_DTYPE_F64: case QUOIN_DTYPE_DECIMAL64:
+        return 8;
+    case QUOIN_DTYPE_DECIMAL128: return 16;
+    case QUOIN_DTYPE_DECIMAL256: return 32;
+    default: return 0;
+    }
+}
+
+extern "C" size_t quoin_typed_compress_bound(int32_t dtype, size_t n) {
+    size_t w = std::max<size_t>(quoin_dtype_width(dtype), 8);  // capi/src/typed.rs:66-69
+    if (n > (SIZE_MAX - 1024) / (w + 1)) return SIZE_MAX;
+    return n * (w + 1) + 1024;
+}
+
+// compress a host column through the default GPU handle; returns bytes or a
+// negative QUOIN_ERR code
+static long long host_compress(const QuoinConfig *cfg, int32_t dtype, size_t n,
+                               const void *values, const uint8_t *validity, uint8_t *dst,
+                               size_t dst_cap) {
+    uint32_t dt, kind, width;
+    if (!map_dtype(dtype, &dt, &kind, &width)) return QUOIN_ERR_UNSUPPORTED;
+    QuoinGpu *g = default_gpu();
+    if (!g) return QUOIN_ERR_DEVICE;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    hipStream_t s = g->stream;
+    // staging holds the stream even when it exceeds the caller's capacity, so
+    // the needed size can be reported; tiny fixed blocks cost 9 B of framing each
+    size_t bound = quoin_typed_compress_bound(dtype, n);
+    uint32_t bs = cfg ? cfg->block_size : 0;
+    if (bs) bound += ((n + bs - 1) / bs) * 9;
+    bound = std::max(bound, dst_cap);
+    size_t vbytes = (n + 7) / 8;
+    if (!ensure(g->st_in, n * width) || !ensure(g->st_out, bound) ||
+        (validity && !ensure(g->st_valid, vbytes)))
+        return QUOIN_ERR_DEVICE;
+    if (n) hipMemcpyAsync(g->st_in.p, values, n * width, hipMemcpyHostToDevice, s);
+    if (validity && vbytes) hipMemcpyAsync(g->st_valid.p, validity, vbytes, hipMemcpyHostToDevice, s);
+    QuoinDeviceColumn col{};
+    col.dtype = dtype;
+    col.n = n;
+    col.values = g->st_in.p;
+    col.validity = validity ? (const uint8_t *)g->st_valid.p : nullptr;
+    col.dst = (uint8_t *)g->st_out.p;
+    col.dst_cap = bound;
+    uint64_t len = 0;
+    int rc = gpu_compress_locked(g, cfg, &col, 1, &len, s);
+    if (rc) return rc;
+    if (len > dst_cap) return -(long long)1000 - (long long)len;  // too small, report need
+    if (hipMemcpy(dst, g->st_out.p, len, hipMemcpyDeviceToHost) != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    return (long long)len;
+}
+
+static int host_decompress(int32_t dtype, size_t n, const uint8_t *src, size_t src_len,
+                           void *values_out, uint8_t *validity_out) {
+    uint32_t dt, kind, width;
+    if (!map_dtype(dtype, &dt, &kind, &width)) return QUOIN_ERR_UNSUPPORTED;
+    QuoinGpu *g = default_gpu();
+    if (!g) return QUOIN_ERR_DEVICE;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    hipStream_t s = g->stream;
+    size_t vbytes = (n + 7) / 8;
+    if (!ensure(g->st_in, src_len) || !ensure(g->st_out, n * width) ||
+        (validity_out && !ensure(g->st_vout, vbytes)))
+        return QUOIN_ERR_DEVICE;
+    if (src_len) hipMemcpyAsync(g->st_in.p, src, src_len, hipMemcpyHostToDevice, s);
+    QuoinDeviceStream q{};
+    q.dtype = dtype;
+    q.n = n;
+    q.src = (const uint8_t *)g->st_in.p;
+    q.host_src = src;
+    q.src_len = src_len;
+    q.values_out = g->st_out.p;
+    q.validity_out = validity_out ? (uint8_t *)g->st_vout.p : nullptr;
+    int rc = gpu_decompress_locked(g, &q, 1, s);
+    if (rc) return rc;
+    if (n && hipMemcpy(values_out, g->st_out.p, n * width, hipMemcpyDeviceToHost) != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    if (validity_out && vbytes &&
+        hipMemcpy(validity_out, g->st_vout.p, vbytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    return QUOIN_OK;
+}
+
+extern "C" size_t quoin_typed_compress_cfg(const QuoinConfig *cfg, int32_t dtype, size_t n,
+                                           const void *values, const uint8_t *validity,
+                                           uint8_t *dst, size_t dst_cap) {
+    if (!values || !dst) return 0;
+    try {
+        long long r = host_compress(cfg, dtype, n, values, validity, dst, dst_cap);
+        t_last_err = r > 0 ? QUOIN_OK : r <= -1000 ? QUOIN_ERR_BUFFER_TOO_SMALL : (int)r;
+        return r > 0 ? (size_t)r : 0;
+    } catch (...) {
+        t_last_err = QUOIN_ERR_PANIC;
+        return 0;
+    }
+}
+
+extern "C" size_t quoin_typed_compress(int32_t dtype, size_t n, const void *values,
+                                       const uint8_t *validity, uint8_t *dst, size_t dst_cap) {
+    return quoin_typed_compress_cfg(nullptr, dtype, n, values, validity, dst, dst_cap);
+}
+
+extern "C" size_t quoin_typed_compress_decimal(int32_t, size_t, const void *, const uint8_t *,
+                                               int8_t, uint8_t, uint8_t *, size_t) {
+    return 0;  // the decimal container is outside this build's path
+}
+
+extern "C" int quoin_typed_decompress(int32_t dtype, size_t n, const uint8_t *src, size_t src_len,
+                                      void *values_out, uint8_t *validity_out) {
+    if (!src || !values_out) return QUOIN_ERR_NULL;
+    try {
+        return host_decompress(dtype, n, src, src_len, values_out, validity_out);
+    } catch (...) {
+        return QUOIN_ERR_PANIC;
+    }
+}
+
+extern "C" int quoin_compress_cfg(const QuoinConfig *cfg, const double *src, size_t n, uint8_t *dst,
+                                  size_t dst_cap, size_t *out_len) {
+    if (!src || !dst || !out_len) return QUOIN_ERR_NULL;
+    try {
+        long long r = host_compress(cfg, QUOIN_DTYPE_F64, n, src, nullptr, dst, dst_cap);
+        if (r <= -1000) {
+            *out_len = (size_t)(-(r + 1000));
+            return QUOIN_ERR_BUFFER_TOO_SMALL;
+        }
+        if (r < 0) return (int)r;
+        *out_len = (size_t)r;
+        return QUOIN_OK;
+    } catch (...) {
+        return QUOIN_ERR_PANIC;
+    }
+}
+
+extern "C" int quoin_compress(const double *src, size_t n, uint8_t *dst, size_t dst_cap,
+                              size_t *out_len) {
+    return quoin_compress_cfg(nullptr, src, n, dst, dst_cap, out_len);
+}
+
+extern "C" ptrdiff_t quoin_decompressed_value_count(const uint8_t *src, size_t src_len) {
+    if (!src) return QUOIN_ERR_NULL;
+    // src/lib.rs:531-534: the header alone (src/format.rs:65-94)
+    if (src_len < 16 || memcmp(src, "FCR1", 4) != 0 || src[4] != 2 || (src[5] & ~0x01) ||
+        src[6] < 10 || src[6] > 16 || src[7] > 7)
+        return QUOIN_ERR_CORRUPT;
+    uint64_t n = rd64(src + 8);
+    if (n > (uint64_t)PTRDIFF_MAX) return QUOIN_ERR_CORRUPT;
+    return (ptrdiff_t)n;
+}
+
+extern "C" int quoin_decompress(const uint8_t *src, size_t src_len, double *dst,
+                                size_t dst_cap_values, size_t *out_values) {
+    if (!src || !dst || !out_values) return QUOIN_ERR_NULL;
+    try {
+        ParsedStream ps;
+        if (src_len < 16) return QUOIN_ERR_CORRUPT;
+        ptrdiff_t n = quoin_decompressed_value_count(src, src_len);
+        if (n < 0) return QUOIN_ERR_CORRUPT;
+        if (src[7] != QA_F64) return QUOIN_ERR_CORRUPT;  // DTypeMismatch -> corrupt at the ABI
+        *out_values = (size_t)n;
+        if ((size_t)n > dst_cap_values) {
+            // the reference decodes first, then reports the size; validate likewise
+            int rc = parse_stream(src, src_len, ps);
+            if (rc) return rc == QUOIN_ERR_UNSUPPORTED ? rc : QUOIN_ERR_CORRUPT;
+            return QUOIN_ERR_BUFFER_TOO_SMALL;
+        }
+        int rc = host_decompress(QUOIN_DTYPE_F64, (size_t)n, src, src_len, dst, nullptr);
+        return rc;
+    } catch (...) {
+        return QUOIN_ERR_PANIC;
+    }
+}
+
+struct QuoinCtx {
+    size_t threads;
+};
+
+extern "C" QuoinCtx *quoin_ctx_create(size_t threads) {
+    QuoinCtx *c = new (std::nothrow) QuoinCtx();
+    if (c) c->threads = threads;
+    return c;
+}
+extern "C" void quoin_ctx_free(QuoinCtx *ctx) { delete ctx; }
+extern "C" int quoin_compress_ctx(const QuoinCtx *, const double *src, size_t n, uint8_t *dst,
+                                  size_t dst_cap, size_t *out_len) {
+    return quoin_compress(src, n, dst, dst_cap, out_len);
+}
+extern "C" int quoin_decompress_ctx(const QuoinCtx *, const uint8_t *src, size_t src_len, double *dst,
+                                    size_t dst_cap_values, size_t *out_values) {
+    return quoin_decompress(src, src_len, dst, dst_cap_values, out_values);
+}
+
+extern "C" int quoin_last_error(void) { return t_last_err; }
+
+extern "C" void quoin_mode_win_counts(uint64_t out[64]) {
+    for (int m = 0; m < 64; m++) out[m] = g_wins[m].load();
+}
+extern "C" void quoin_reset_mode_win_counts(void) {
+    for (int m = 0; m < 64; m++) g_wins[m] = 0;
+}

@@ -1,0 +1,79 @@
+// qa_kernels.h — launch-side declarations shared by the kernel files and the
+// host engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "qa_common.h"
+
+size_t qa_scan_tmp_words(uint64_t n);
+void qa_exclusive_scan(const uint64_t *in, uint64_t n, uint64_t *out, uint64_t *tmp,
+                       hipStream_t s);
+void qa_exclusive_scan_dn(const uint64_t *in, uint64_t nmax, const uint64_t *ndev, uint64_t *out,
+                          uint64_t *tmp, hipStream_t s);
+
+// Device pointers of one encode batch (all arrays sized by the host from upper
+// bounds; counts that depend on the data live in DColState).
+struct EncArgs {
+    const DCol *cols;
+    DColState *cst;
+    uint32_t ncols;
+    uint32_t nregions;     // total region slots
+    uint64_t ntiles;       // total tile slots
+    uint64_t nvals;        // total logical values
+    uint64_t nvwords;      // total validity words
+    uint64_t *lane;
+    uint64_t *vpre;        // nvwords+1: popcount, then exclusive prefix
+    uint64_t *tpre;        // nvwords+1: transition counts, then prefix
+    uint64_t *tpos;        // transition positions (<= total bits)
+    uint64_t *rsz;         // run varint sizes (+ prefix), ntrans_max + ncols + 1
+    uint64_t *colbase;     // per column: transition base (ncols+1), then [ncols+1] = run count
+    uint64_t *scan_tmp;
+    uint32_t *rfeat;       // per region slot features
+    uint32_t *owner;       // per region slot: global slot of the owning block
+    DBlock *blocks;        // per region slot
+    DTile *tiles;          // per tile slot
+    uint64_t ntrans_max;
+    unsigned long long *wins;  // 64 per-mode win counters
+};
+
+void qa_encode_launch(const EncArgs &a, hipStream_t s, int level_max);
+
+struct DFrame {
+    uint64_t payload_off;  // byte offset of the payload in the stream
+    uint64_t out_off;      // first compact value of this frame (within column)
+    uint64_t tile_base;    // first global decode tile slot
+    uint32_t n, plen;
+    uint32_t mode, col;
+    uint32_t ntiles, pad;
+};
+
+struct DDecCol {
+    const uint8_t *src;     // device stream
+    void *out;              // typed output
+    uint8_t *validity_out;  // device bitmap out or null
+    const uint8_t *bitmap;  // device bitmap decoded from the stream, or null
+    uint64_t src_len;
+    uint64_t n;             // logical values
+    uint64_t nv;            // valid values
+    uint64_t lane_off;      // scratch lane offset (when nulls)
+    uint64_t vword_base;    // rank prefix slots
+    uint64_t val_base;
+    uint32_t dtype, nvwords;
+    uint32_t out_kind, pad;   // QA_IN_* narrow output element type
+};
+
+struct DecArgs {
+    const DDecCol *cols;
+    uint32_t ncols;
+    const DFrame *frames;
+    uint32_t nframes;
+    uint64_t ntiles;
+    uint64_t nvals;
+    uint64_t nvwords;
+    uint64_t *lane;        // compact scratch for columns with nulls
+    uint64_t *vpre;        // rank prefix (nvwords + 1)
+    DTile *tiles;          // per decode tile
+    uint32_t *err;         // error flags
+    uint64_t *scan_tmp;
+};
+
+void qa_decode_launch(const DecArgs &a, hipStream_t s);

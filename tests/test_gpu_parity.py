@@ -1,0 +1,92 @@
+"""GPU stream parity: the HIP encoder's bytes equal the oracle's, and the HIP
+decoder round-trips its own and the oracle's streams bit for bit."""
+import numpy as np
+import pytest
+
+import _data
+import _oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+ENGINE = {"F64": orc.F64, "I64": orc.I64, "U64": orc.U64, "I32": orc.I32, "U32": orc.U32,
+          "F32": orc.F32}
+
+
+@pytest.fixture(scope="module")
+def qa():
+    import quoin_amd
+    return quoin_amd
+
+
+def engine_dtype(arr):
+    return {np.dtype(np.float64): orc.F64, np.dtype(np.int64): orc.I64,
+            np.dtype(np.uint64): orc.U64, np.dtype(np.int32): orc.I32,
+            np.dtype(np.uint32): orc.U32, np.dtype(np.float32): orc.F32}[arr.dtype]
+
+
+def check(qa, name, vals, validity=None, level=0, block_size=0):
+    print(f"case {name} n={len(vals)} dtype={vals.dtype} nulls={validity is not None} "
+          f"level={level} bs={block_size}", flush=True)
+    cfg = qa.Config(level=qa.Level(level), block_size=block_size)
+    got = qa.compress_column(vals, validity, cfg)
+    want = orc.compress(vals, engine_dtype(vals), validity, level, block_size)
+    assert got == want, f"{name}: stream differs (gpu {len(got)} B, oracle {len(want)} B)"
+    out, vout = qa.decompress_column(got, qa.FROM_NP[vals.dtype], len(vals),
+                                     with_validity=validity is not None)
+    # the oracle's decode is the reference behaviour (f32 signaling NaNs come
+    # back quieted: src/dtype.rs:40-48); everything else is the input itself
+    _, want_vals, _ = orc.decompress(want)
+    assert out.view(np.uint8).tobytes() == want_vals.view(np.uint8).tobytes(), f"{name}: decode"
+    if vals.dtype != np.float32:
+        assert out.view(np.uint8).tobytes() == vals.view(np.uint8).tobytes(), f"{name}: round trip"
+    if validity is not None:
+        n = len(vals)
+        exp = validity.copy()
+        if n % 8:
+            exp[-1] &= (1 << (n % 8)) - 1
+        assert vout.tobytes() == exp.tobytes()
+
+
+SIZES = [0, 1, 2, 15, 16, 1023, 1024, 1025, 32767, 32768, 32769, 70000, 131072 + 5]
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_int_shapes_fastest(qa, n):
+    for name, v in _data.int_columns(n, seed=n).items():
+        check(qa, name, v)
+
+
+@pytest.mark.parametrize("n", [1, 1000, 40000, 140000])
+def test_float_shapes_fastest(qa, n):
+    for name, v in _data.float_columns(n, seed=n).items():
+        check(qa, name, v)
+
+
+def test_nulls_fastest(qa):
+    for density in (0.0, 0.01, 0.5, 1.0):
+        for name, v in _data.int_columns(50000, seed=3).items():
+            vv, bm = _data.with_nulls(v, density, seed=7)
+            check(qa, f"{name}+nulls{density}", vv, bm)
+
+
+def test_fixed_block_sizes(qa):
+    for bs in (1, 100, 1000, 1024, 5000, 131072, 200000):
+        v = _data.int_columns(20000, seed=bs)["ts_i64"]
+        check(qa, f"bs{bs}", v, block_size=bs)
+
+
+def test_special_f32(qa):
+    check(qa, "special_f32", _data.special_f32())
+
+
+def test_config1_shapes(qa):
+    """Integer lanes at a reduced size (the bench runs 16 Mi per column)."""
+    n = 1 << 20
+    cols = _data.int_columns(n, seed=0)
+    for name in ("ts_i64", "u32_k3", "u32_k11", "u32_k23"):
+        v = cols[name]
+        if name.startswith("u32"):
+            v, bm = _data.with_nulls(v, 0.01, seed=11)
+            check(qa, name, v, bm)
+        else:
+            check(qa, name, v)
