@@ -59,7 +59,15 @@ struct DCol {
 // narrow Arrow inputs widened on load (capi/src/typed.rs:147-153)
 enum { QA_IN_NATIVE = 0, QA_IN_I8 = 1, QA_IN_I16 = 2, QA_IN_U8 = 3, QA_IN_U16 = 4 };
 
+// How a column's values are read as u64 lane words (src/lib.rs:389-403): the
+// kernels widen on load, so a column without nulls is read in place and a
+// column with nulls is compacted in its own element width.
+enum { QA_LK_U64 = 0, QA_LK_I32, QA_LK_U32, QA_LK_F32, QA_LK_I8, QA_LK_I16, QA_LK_U8, QA_LK_U16 };
+
 struct DColState {
+    const void *lanep;    // lane base: the input itself, or the compacted copy
+    uint32_t lkind;       // QA_LK_*
+    uint32_t lwidth;      // bytes per element
     uint64_t nv;          // compacted (valid) value count
     uint64_t vblob_len;   // validity blob bytes (0 without nulls)
     uint64_t vblob_rle;   // RLE form bytes (without tag)
@@ -89,15 +97,24 @@ struct DBlock {
     uint8_t distinct_low, repeats, incompressible, compressible;
 };
 
+// Per-tile statistics. They are computed before block planning, so every
+// quantity that depends on the block boundary is kept split: the tile's first
+// element is recorded on its own and the block decision re-applies the block
+// start rules (delta base, XORZ predecessor 0, CONST/STRIDE anchors).
 struct DTile {
     uint64_t umin, umax;
     int64_t smin, smax;
-    uint64_t dzmin, dzmax;   // zig-zag deltas (src/codecs/delta_bitpack.rs:32-46)
-    uint64_t xorz_bytes;     // sum of varint lengths (src/codecs/xorz.rs:8-16)
-    uint64_t off;            // this sub-block's offset inside the winner payload
-    uint64_t tail;           // carried value (delta decode prefix)
-    uint32_t const_bad, stride_bad, changes, cnt;
-    uint32_t w, active, pad0, pad1;
+    uint64_t dzmin, dzmax;   // zig-zag deltas of elements 1..cnt-1 (delta_bitpack.rs:32-46)
+    uint64_t dz_first;       // zig-zag delta of element 0 against the column predecessor
+    uint64_t v_first, v_last, step;
+    uint64_t xorz_rest;      // varint bytes of v_i ^ v_{i-1}, i = 1..cnt-1 (xorz.rs:8-16)
+    uint64_t off;            // sub-block offset (encode: in the payload; decode: in the stream)
+    uint64_t tail;           // decode: delta prefix carried into the tile
+    uint32_t chg_rest;       // i in 1..cnt-1 with v_i != v_{i-1}
+    uint32_t exp_min, exp_max;
+    uint32_t cnt;
+    uint32_t w;
+    uint8_t arith, allsame, active, pad;
 };
 
 // ---------------------------------------------------------------------------
@@ -180,6 +197,20 @@ __device__ __forceinline__ uint64_t qa_f32_to_lane(uint32_t b) {
     }
     return sign | ((uint64_t)(e - 127 + 1023) << 52) | ((uint64_t)m << 29);
 }
+// lane word i of a column stored as kind k
+__device__ __forceinline__ uint64_t qa_lane_get(const void *p, uint32_t k, uint64_t i) {
+    switch (k) {
+    case QA_LK_I32: return (uint64_t)(int64_t)((const int32_t *)p)[i];
+    case QA_LK_U32: return (uint64_t)((const uint32_t *)p)[i];
+    case QA_LK_F32: return qa_f32_to_lane(((const uint32_t *)p)[i]);
+    case QA_LK_I8: return (uint64_t)(int64_t)((const int8_t *)p)[i];
+    case QA_LK_I16: return (uint64_t)(int64_t)((const int16_t *)p)[i];
+    case QA_LK_U8: return (uint64_t)((const uint8_t *)p)[i];
+    case QA_LK_U16: return (uint64_t)((const uint16_t *)p)[i];
+    default: return ((const uint64_t *)p)[i];
+    }
+}
+
 // narrowing of a lane that came from an f32 (exact)
 __device__ __forceinline__ uint32_t qa_lane_to_f32(uint64_t x) {
     uint32_t sign = (uint32_t)(x >> 63) << 31;

@@ -1,13 +1,18 @@
 // qa_decode.hip — gfx950 kernels decoding a batch of quoin streams.
 //
-// The host scans the 9-byte frame headers (src/decoder.rs:65-94) and hands the
-// frame table over; everything sized by the data runs here:
-//   1 index: walk each frame's sub-block headers (widths), validate bounds
-//   2 DeltaBitpack: per-tile delta sums, then a per-frame carry scan
+// The host reads only the 16-byte header and the 9-byte frame headers
+// (src/format.rs:65-94, src/decoder.rs:65-94); every data-sized step runs here:
+//   1 validity blob -> bitmap (src/validity.rs:73-119): RLE varints are split
+//     in parallel (a byte ends a varint iff its top bit is clear), run lengths
+//     are scanned, each bitmap word ORs the valid runs that meet it
+//   2 valid count and the frames that cover it (the reference scans frames
+//     until it has counted the valid values)
+//   3 index: each frame's sub-block headers (widths, offsets), bounds checked
+//   4 DeltaBitpack: per-tile delta sums, then a per-frame carry scan
 //     (src/codecs/delta_bitpack.rs:48-64: a wrapping prefix sum)
-//   3 per-tile unpack (FastLanes, src/bitpack.rs:78-161) + frame-of-reference
-//   4 scatter compact values to logical positions for streams with nulls
-//     (src/validity.rs:130-146) and narrow to the column type (src/lib.rs:492-506)
+//   5 per-tile unpack from LDS (FastLanes, src/bitpack.rs:78-161) + FoR
+//   6 scatter compact values to logical positions for streams with nulls
+//     (src/validity.rs:130-146), narrowing to the column type (src/lib.rs:492-506)
 #include "qa_kernels.h"
 
 #define WAVE 64
@@ -16,38 +21,176 @@ __device__ __forceinline__ uint64_t lane_bytes_d(uint32_t dt) {
     return (dt == QA_I32 || dt == QA_U32 || dt == QA_F32) ? 4 : 8;
 }
 
-// value k of a FastLanes block packed at byte address p with width w
-__device__ __forceinline__ uint64_t unpack_one(const uint8_t *p, uint32_t w, uint32_t k) {
-    if (w == 0) return 0;
-    if (w <= 32) {
-        uint32_t r = k >> 5, l = k & 31;
-        uint32_t bit = r * w, word = bit >> 5, sh = bit & 31;
-        uint32_t lo = qa_ld32u(p + 4ull * (word * 32 + l));
-        uint32_t mask = w >= 32 ? 0xFFFFFFFFu : ((1u << w) - 1);
-        uint32_t v = lo >> sh;
-        if (sh + w > 32) v |= qa_ld32u(p + 4ull * ((word + 1) * 32 + l)) << (32 - sh);
-        return v & mask;
-    }
-    uint32_t r = k >> 4, l = k & 15;
-    uint32_t bit = r * w, word = bit >> 6, sh = bit & 63;
-    uint64_t lo = qa_ld64u(p + 8ull * (word * 16 + l));
-    uint64_t mask = w >= 64 ? ~0ull : ((1ull << w) - 1);
-    uint64_t v = lo >> sh;
-    if (sh + w > 64) v |= qa_ld64u(p + 8ull * ((word + 1) * 16 + l)) << (64 - sh);
-    return v & mask;
-}
+#define QA_SEARCH(cnt, keyexpr, key)                              \
+    int lo = 0, hi = (int)(cnt) - 1;                              \
+    while (lo < hi) {                                             \
+        int mid = (lo + hi + 1) >> 1;                             \
+        if ((keyexpr) <= (key)) lo = mid; else hi = mid - 1;      \
+    }                                                             \
+    return lo;
 
 __device__ __forceinline__ int frame_by_tile(const DecArgs &a, uint64_t t) {
-    int lo = 0, hi = (int)a.nframes - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (a.frames[mid].tile_base <= t) lo = mid; else hi = mid - 1;
-    }
-    return lo;
+    QA_SEARCH(a.nframes, a.frames[mid].tile_base, t)
+}
+__device__ __forceinline__ int dcol_by_vword(const DecArgs &a, uint64_t j) {
+    QA_SEARCH(a.ncols, a.cols[mid].vword_base, j)
+}
+__device__ __forceinline__ int dcol_by_vbyte(const DecArgs &a, uint64_t g) {
+    QA_SEARCH(a.ncols, a.cols[mid].vbyte_base, g)
+}
+__device__ __forceinline__ int dcol_by_val(const DecArgs &a, uint64_t g) {
+    QA_SEARCH(a.ncols, a.cols[mid].val_base, g)
+}
+__device__ __forceinline__ bool frame_used(const DecArgs &a, uint32_t f) {
+    const DFrame &fr = a.frames[f];
+    return f < a.dst[fr.col].fused_end;
 }
 
 // ---------------------------------------------------------------------------
-// stage 1: sub-block index (one thread per frame)
+// stage 1: validity blob -> bitmap
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint8_t vtag(const DDecCol &c) {
+    return c.vblob_len ? c.src[c.vblob_off] : 0xFF;
+}
+
+__global__ void k_vb_ends(DecArgs a) {
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.nvbytes) return;
+    const DDecCol &c = a.cols[dcol_by_vbyte(a, g)];
+    uint64_t k = g - c.vbyte_base;
+    uint8_t b = c.src[c.vblob_off + 1 + k];
+    a.vbpre[g] = (vtag(c) == 1 && !(b & 0x80)) ? 1 : 0;
+}
+
+// each varint-ending byte assembles its run length (src/varint.rs:24-39)
+__global__ void k_vb_runs(DecArgs a) {
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.nvbytes) return;
+    const DDecCol &c = a.cols[dcol_by_vbyte(a, g)];
+    if (vtag(c) != 1) return;
+    const uint8_t *p = c.src + c.vblob_off + 1;
+    uint64_t k = g - c.vbyte_base;
+    if (p[k] & 0x80) return;
+    uint64_t j = k;
+    while (j > 0 && (p[j - 1] & 0x80)) j--;
+    uint64_t len = k - j + 1, v = 0;
+    if (len > 10) {
+        v = c.n + 1;  // overflowing varint: an error if the decoder reaches it
+    } else {
+        for (uint64_t q = 0; q < len; q++) v |= (uint64_t)(p[j + q] & 0x7f) << (7 * q);
+        if (v > c.n) v = c.n + 1;  // keeps the prefix sums from wrapping
+    }
+    a.vruns[a.vbpre[g]] = v;
+}
+
+// per column: how many runs the reference loop reads, and blob validation
+__global__ void k_vb_colfin(DecArgs a) {
+    int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= (int)a.ncols) return;
+    const DDecCol &c = a.cols[ci];
+    DDecState &s = a.dst[ci];
+    if (!c.nvwords) return;
+    uint8_t tag = vtag(c);
+    if (c.vblob_len == 0 || (tag != 0 && tag != 1)) {
+        atomicOr(a.err, QA_ERR_CORRUPT);
+        return;
+    }
+    if (tag == 0) {
+        if (c.vblob_len - 1 < (c.n + 7) / 8) atomicOr(a.err, QA_ERR_CORRUPT);
+        return;
+    }
+    uint64_t r0 = a.vbpre[c.vbyte_base], r1 = a.vbpre[c.vbyte_base + c.vblob_len - 1];
+    uint64_t base = a.vruns[r0];  // prefix at the column's first run
+    if (c.n == 0) {
+        s.nruns = 0;
+        return;
+    }
+    // smallest m = K+1 in [1, r1-r0] with P[m] >= n, P[m] = vruns[r0 + m] - base
+    uint64_t hi = r1 - r0;
+    if (hi == 0 || a.vruns[r0 + hi] - base < c.n) {
+        atomicOr(a.err, QA_ERR_CORRUPT);  // the runs end before n values
+        return;
+    }
+    uint64_t lo = 1;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (a.vruns[r0 + mid] - base >= c.n) hi = mid; else lo = mid + 1;
+    }
+    if (a.vruns[r0 + lo] - base != c.n) atomicOr(a.err, QA_ERR_CORRUPT);  // a run overruns
+    s.nruns = lo;
+}
+
+__global__ void k_vb_words(DecArgs a) {
+    if (*a.err) return;
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.nvwords) return;
+    int ci = dcol_by_vword(a, j);
+    const DDecCol &c = a.cols[ci];
+    uint64_t w = j - c.vword_base, bit0 = w * 64, bit1 = bit0 + 64;
+    uint64_t out = 0;
+    if (vtag(c) == 0) {
+        const uint8_t *p = c.src + c.vblob_off + 1;
+        uint64_t nbytes = (c.n + 7) / 8;
+        for (int k = 0; k < 8; k++)
+            if (w * 8 + k < nbytes) out |= (uint64_t)p[w * 8 + k] << (8 * k);
+    } else {
+        uint64_t r0 = a.vbpre[c.vbyte_base], nr = a.dst[ci].nruns;
+        uint64_t base = a.vruns[r0];
+        // last run k with start P[k] <= bit0
+        uint64_t lo = 0, hi = nr ? nr - 1 : 0;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi + 1) >> 1;
+            if (a.vruns[r0 + mid] - base <= bit0) lo = mid; else hi = mid - 1;
+        }
+        for (uint64_t k = lo; k < nr; k++) {
+            uint64_t st = a.vruns[r0 + k] - base;
+            if (st >= bit1) break;
+            if (k & 1) {  // runs alternate starting from a null run
+                uint64_t en = a.vruns[r0 + k + 1] - base;
+                uint64_t s0 = st > bit0 ? st : bit0, e0 = en < bit1 ? en : bit1;
+                if (e0 > s0) {
+                    uint64_t cntb = e0 - s0, sh = s0 - bit0;
+                    out |= (cntb == 64 ? ~0ull : ((1ull << cntb) - 1)) << sh;
+                }
+            }
+        }
+    }
+    uint64_t bits = c.n - bit0;
+    if (bits < 64) out &= (1ull << bits) - 1;
+    c.bitmap[w] = out;
+}
+
+__global__ void k_dec_vcount(DecArgs a) {
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.nvwords) return;
+    const DDecCol &c = a.cols[dcol_by_vword(a, j)];
+    a.vpre[j] = *a.err ? 0 : (uint64_t)__popcll(c.bitmap[j - c.vword_base]);
+}
+
+// valid count and the frames the reference decoder would read for it
+__global__ void k_dec_colfin(DecArgs a) {
+    int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= (int)a.ncols) return;
+    const DDecCol &c = a.cols[ci];
+    DDecState &s = a.dst[ci];
+    uint64_t nv = c.n;
+    if (c.nvwords) nv = a.vpre[c.vword_base + c.nvwords] - a.vpre[c.vword_base];
+    s.nv = nv;
+    // frames are read while fewer than nv values are counted: the first frame
+    // starting at or past nv ends the read (out_off is non-decreasing)
+    uint32_t lo = c.fbegin, hi = c.fend;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a.frames[mid].out_off < nv) lo = mid + 1; else hi = mid;
+    }
+    uint32_t fe = lo;
+    s.fused_end = fe;
+    uint64_t got = fe > c.fbegin ? a.frames[fe - 1].out_off + a.frames[fe - 1].n : 0;
+    if (got != nv) atomicOr(a.err, QA_ERR_CORRUPT);
+}
+
+// ---------------------------------------------------------------------------
+// stage 3: sub-block index (one thread per frame)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool rd_varint(const uint8_t *p, uint64_t end, uint64_t *pos,
                                           uint64_t *out) {
@@ -68,12 +211,16 @@ __device__ __forceinline__ bool rd_varint(const uint8_t *p, uint64_t end, uint64
 
 __global__ void k_dec_index(DecArgs a) {
     uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= a.nframes) return;
+    if (f >= a.nframes || *a.err || !frame_used(a, f)) return;
     const DFrame &fr = a.frames[f];
     const DDecCol &col = a.cols[fr.col];
     const uint8_t *s = col.src;
     uint64_t pos = fr.payload_off, end = fr.payload_off + fr.plen;
     uint32_t err = 0;
+    if (!fr.gpu_ok) {
+        atomicOr(a.err, QA_ERR_UNSUPPORTED);
+        return;
+    }
     switch (fr.mode) {
     case QA_M_RAW:
         if (fr.plen != fr.n * lane_bytes_d(col.dtype)) err = QA_ERR_CORRUPT;
@@ -128,7 +275,60 @@ __global__ void k_dec_index(DecArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// stage 2: DeltaBitpack carries
+// LDS staging of packed bytes: nwords u32 words starting at an arbitrary byte
+// address, realigned. The aligned word holding the last needed byte is inside
+// the allocation (aligned words never cross it).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void stage_words(const uint8_t *src, uint32_t nbytes, uint32_t *lds,
+                                            int lane) {
+    uintptr_t ad = (uintptr_t)src;
+    const uint32_t *al = (const uint32_t *)(ad & ~(uintptr_t)3);
+    uint32_t m = (uint32_t)(ad & 3);
+    uint32_t nwords = (nbytes + 3) / 4;
+    // aligned words that hold bytes [src, src + nbytes)
+    uint32_t nal = (m + nbytes + 3) / 4;
+    if (m == 0) {
+        for (uint32_t i = lane; i < nwords; i += WAVE) lds[i] = al[i];
+    } else {
+        for (uint32_t i = lane; i < nwords; i += WAVE) {
+            uint32_t lo = al[i];
+            uint32_t hi = i + 1 < nal ? al[i + 1] : 0;
+            lds[i] = (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * m));
+        }
+    }
+}
+__device__ __forceinline__ uint64_t unpack_lds(const uint32_t *wd, uint32_t w, uint32_t k) {
+    if (w == 0) return 0;
+    if (w <= 32) {
+        uint32_t r = k >> 5, l = k & 31;
+        uint32_t bit = r * w, word = bit >> 5, sh = bit & 31;
+        uint32_t mask = w >= 32 ? 0xFFFFFFFFu : ((1u << w) - 1);
+        uint32_t v = wd[word * 32 + l] >> sh;
+        if (sh + w > 32) v |= wd[(word + 1) * 32 + l] << (32 - sh);
+        return v & mask;
+    }
+    uint32_t r = k >> 4, l = k & 15;
+    uint32_t bit = r * w, word = bit >> 6, sh = bit & 63;
+    uint64_t mask = w >= 64 ? ~0ull : ((1ull << w) - 1);
+    uint32_t i0 = 2 * (word * 16 + l);
+    uint64_t lo = ((uint64_t)wd[i0 + 1] << 32) | wd[i0];
+    uint64_t v = lo >> sh;
+    if (sh + w > 64) {
+        uint32_t i1 = 2 * ((word + 1) * 16 + l);
+        uint64_t hi = ((uint64_t)wd[i1 + 1] << 32) | wd[i1];
+        v |= hi << (64 - sh);
+    }
+    return v & mask;
+}
+// LDS written by one wavefront and read back by its other lanes
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// stage 4: DeltaBitpack carries
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T wsum(T v) {
@@ -138,41 +338,53 @@ __device__ __forceinline__ T wsum(T v) {
 }
 
 __global__ void __launch_bounds__(256) k_dec_delta_sums(DecArgs a) {
+    __shared__ uint32_t wbuf[4][2 * QA_TILE + 4];
     if (*a.err) return;
-    const int lane = threadIdx.x & 63;
-    uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t t = (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.ntiles) return;
     int f = frame_by_tile(a, t);
     const DFrame &fr = a.frames[f];
-    if (fr.mode != QA_M_DELTA) return;
+    if (fr.mode != QA_M_DELTA || !frame_used(a, f)) return;
     const DDecCol &col = a.cols[fr.col];
     DTile &ts = a.tiles[t];
     uint32_t lt = (uint32_t)(t - fr.tile_base);
     uint32_t cnt = min((uint32_t)QA_TILE, fr.n - lt * QA_TILE);
     const uint8_t *h = col.src + ts.off;
     uint64_t mn = qa_ld64u(h + 1);
+    uint32_t *wd = wbuf[wv];
+    stage_words(h + 9, 128 * ts.w, wd, lane);
+    wave_lds_sync();
     uint64_t s = 0;
-    for (uint32_t k = lane; k < cnt; k += WAVE) s += qa_unzigzag(mn + unpack_one(h + 9, ts.w, k));
+    for (uint32_t k = lane; k < cnt; k += WAVE) s += qa_unzigzag(mn + unpack_lds(wd, ts.w, k));
     s = wsum(s);
     if (lane == 0) ts.tail = s;
 }
 
-__global__ void k_dec_delta_scan(DecArgs a) {
-    uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per frame: exclusive scan of the tile sums, seeded with the base
+__global__ void __launch_bounds__(256) k_dec_delta_scan(DecArgs a) {
+    const int lane = threadIdx.x & 63;
+    uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (f >= a.nframes || *a.err) return;
     const DFrame &fr = a.frames[f];
-    if (fr.mode != QA_M_DELTA) return;
-    uint64_t acc = qa_ld64u(a.cols[fr.col].src + fr.payload_off);  // base
-    for (uint32_t t = 0; t < fr.ntiles; t++) {
-        DTile &ts = a.tiles[fr.tile_base + t];
-        uint64_t s = ts.tail;
-        ts.tail = acc;
-        acc += s;
+    if (fr.mode != QA_M_DELTA || !frame_used(a, f)) return;
+    uint64_t carry = qa_ld64u(a.cols[fr.col].src + fr.payload_off);  // base
+    for (uint32_t t0 = 0; t0 < fr.ntiles; t0 += WAVE) {
+        uint32_t t = t0 + lane;
+        uint64_t s = t < fr.ntiles ? a.tiles[fr.tile_base + t].tail : 0;
+        uint64_t inc = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint64_t y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        if (t < fr.ntiles) a.tiles[fr.tile_base + t].tail = carry + inc - s;
+        carry += __shfl(inc, 63, 64);
     }
 }
 
 // ---------------------------------------------------------------------------
-// stage 3: per-tile decode
+// stage 5: per-tile decode (one wavefront per 1024-value tile)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void store_typed(const DDecCol &col, uint64_t idx, uint64_t x) {
     switch (col.out_kind) {
@@ -190,82 +402,87 @@ __device__ __forceinline__ void store_typed(const DDecCol &col, uint64_t idx, ui
     }
 }
 
-__device__ __forceinline__ void put_value(const DDecCol &col, uint64_t *lane_scratch,
+__device__ __forceinline__ void put_value(const DDecCol &col, bool nulls, uint64_t *lane_scratch,
                                           uint64_t idx, uint64_t x) {
-    if (col.nv != col.n) {  // nulls: compact scratch, scattered later
+    if (nulls) {  // compact scratch, scattered to logical positions later
         lane_scratch[col.lane_off + idx] = x;
         return;
     }
     store_typed(col, idx, x);
 }
 
-__device__ __forceinline__ uint64_t raw_value(const uint8_t *p, uint32_t dt, uint64_t i) {
-    if (dt == QA_F32) return qa_f32_to_lane(qa_ld32u(p + 4 * i));
-    if (lane_bytes_d(dt) == 4) return qa_ld32u(p + 4 * i);
-    return qa_ld64u(p + 8 * i);
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint64_t y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    return v;
 }
 
 __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
+    __shared__ uint32_t wbuf[4][2 * QA_TILE + 4];
     if (*a.err) return;
-    const int lane = threadIdx.x & 63;
-    uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t t = (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.ntiles) return;
     int f = frame_by_tile(a, t);
+    if (!frame_used(a, f)) return;
     const DFrame &fr = a.frames[f];
     const DDecCol &col = a.cols[fr.col];
+    const bool nulls = a.dst[fr.col].nv != col.n;
     const DTile &ts = a.tiles[t];
     uint32_t lt = (uint32_t)(t - fr.tile_base);
     uint32_t cnt = min((uint32_t)QA_TILE, fr.n - lt * QA_TILE);
     uint64_t base = fr.out_off + (uint64_t)lt * QA_TILE;
     const uint8_t *pl = col.src + fr.payload_off;
+    uint32_t *wd = wbuf[wv];
     switch (fr.mode) {
-    case QA_M_RAW:
-        for (uint32_t k = lane; k < cnt; k += WAVE)
-            put_value(col, a.lane, base + k, raw_value(pl, col.dtype, (uint64_t)lt * QA_TILE + k));
+    case QA_M_RAW: {
+        uint32_t lb = (uint32_t)lane_bytes_d(col.dtype);
+        stage_words(pl + (uint64_t)lt * QA_TILE * lb, cnt * lb, wd, lane);
+        wave_lds_sync();
+        for (uint32_t k = lane; k < cnt; k += WAVE) {
+            uint64_t x;
+            if (lb == 4) {
+                x = wd[k];
+                if (col.dtype == QA_F32) x = qa_f32_to_lane((uint32_t)x);
+            } else {
+                x = ((uint64_t)wd[2 * k + 1] << 32) | wd[2 * k];
+            }
+            put_value(col, nulls, a.lane, base + k, x);
+        }
         break;
+    }
     case QA_M_CONST: {
         uint64_t v = qa_ld64u(pl);
-        for (uint32_t k = lane; k < cnt; k += WAVE) put_value(col, a.lane, base + k, v);
+        for (uint32_t k = lane; k < cnt; k += WAVE) put_value(col, nulls, a.lane, base + k, v);
         break;
     }
     case QA_M_STRIDE: {
         uint64_t v0 = qa_ld64u(pl), st = qa_ld64u(pl + 8);
         for (uint32_t k = lane; k < cnt; k += WAVE)
-            put_value(col, a.lane, base + k, v0 + ((uint64_t)lt * QA_TILE + k) * st);
+            put_value(col, nulls, a.lane, base + k, v0 + ((uint64_t)lt * QA_TILE + k) * st);
         break;
     }
-    case QA_M_FOR: {
-        const uint8_t *h = col.src + ts.off;
-        uint64_t mn = qa_ld64u(h + 1);
-        for (uint32_t k = lane; k < cnt; k += WAVE)
-            put_value(col, a.lane, base + k, mn + unpack_one(h + 9, ts.w, k));
-        break;
-    }
+    case QA_M_FOR:
     case QA_M_DELTA: {
-        // each lane owns 16 consecutive values: local sums, wave scan, replay
         const uint8_t *h = col.src + ts.off;
         uint64_t mn = qa_ld64u(h + 1);
-        uint32_t k0 = lane * 16;
-        uint64_t d[16];
-        uint64_t s = 0;
-#pragma unroll
-        for (int j = 0; j < 16; j++) {
-            uint32_t k = k0 + j;
-            d[j] = k < cnt ? qa_unzigzag(mn + unpack_one(h + 9, ts.w, k)) : 0;
-            s += d[j];
-        }
-        uint64_t inc = s;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            uint64_t y = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += y;
-        }
-        uint64_t acc = ts.tail + inc - s;
-#pragma unroll
-        for (int j = 0; j < 16; j++) {
-            uint32_t k = k0 + j;
-            acc += d[j];
-            if (k < cnt) put_value(col, a.lane, base + k, acc);
+        stage_words(h + 9, 128 * ts.w, wd, lane);
+        wave_lds_sync();
+        if (fr.mode == QA_M_FOR) {
+            for (uint32_t k = lane; k < cnt; k += WAVE)
+                put_value(col, nulls, a.lane, base + k, mn + unpack_lds(wd, ts.w, k));
+        } else {
+            uint64_t carry = ts.tail;
+            for (uint32_t k0 = 0; k0 < cnt; k0 += WAVE) {
+                uint32_t k = k0 + lane;
+                uint64_t d = k < cnt ? qa_unzigzag(mn + unpack_lds(wd, ts.w, k)) : 0;
+                uint64_t inc = wave_incl_scan(d, lane);
+                if (k < cnt) put_value(col, nulls, a.lane, base + k, carry + inc);
+                carry += __shfl(inc, 63, 64);
+            }
         }
         break;
     }
@@ -274,66 +491,54 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// stage 4: nulls — rank prefix over the bitmap, scatter + narrow
+// stage 6: nulls — scatter + narrow, validity out
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int dcol_by_vword(const DecArgs &a, uint64_t j) {
-    int lo = 0, hi = (int)a.ncols - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (a.cols[mid].vword_base <= j) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-__device__ __forceinline__ uint64_t dbm_word(const uint8_t *bm, uint64_t n, uint64_t j) {
-    // decoded bitmaps are library-owned and padded to whole 64-bit words
-    uint64_t w = ((const uint64_t *)bm)[j];
-    uint64_t bits = n - j * 64;
-    if (bits < 64) w &= (1ull << bits) - 1;
-    return w;
-}
-
-__global__ void k_dec_vcount(DecArgs a) {
-    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= a.nvwords) return;
-    int c = dcol_by_vword(a, j);
-    const DDecCol &col = a.cols[c];
-    a.vpre[j] = (uint64_t)__popcll(dbm_word(col.bitmap, col.n, j - col.vword_base));
-}
-
-__global__ void k_dec_scatter(DecArgs a) {
+__global__ void __launch_bounds__(256) k_dec_scatter(DecArgs a) {
+    // one workgroup per 16 bitmap words of columns that carry a bitmap
+    __shared__ uint64_t words[16], wpre[16];
+    __shared__ int wcol[16];
     if (*a.err) return;
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.nvals) return;
-    int lo = 0, hi = (int)a.ncols - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (a.cols[mid].val_base <= g) lo = mid; else hi = mid - 1;
-    }
-    const DDecCol &col = a.cols[lo];
-    uint64_t i = g - col.val_base;
-    if (i >= col.n) return;
-    if (col.validity_out && (i & 7) == 0) {
-        uint8_t byte;
-        if (col.bitmap) {
-            byte = col.bitmap[i >> 3];
+    uint64_t w0 = (uint64_t)blockIdx.x * 16;
+    if (threadIdx.x < 16) {
+        uint64_t j = w0 + threadIdx.x;
+        if (j < a.nvwords) {
+            int c = dcol_by_vword(a, j);
+            const DDecCol &col = a.cols[c];
+            wcol[threadIdx.x] = c;
+            words[threadIdx.x] = col.bitmap[j - col.vword_base];
+            wpre[threadIdx.x] = a.vpre[j] - a.vpre[col.vword_base];
         } else {
-            byte = 0xFF;
+            wcol[threadIdx.x] = -1;
         }
-        uint64_t rem = col.n - i;
-        if (rem < 8) byte &= (uint8_t)((1u << rem) - 1);
-        col.validity_out[i >> 3] = byte;
     }
-    if (col.nv == col.n) return;  // decoded in place
-    uint64_t j = i >> 6;
-    uint64_t w = dbm_word(col.bitmap, col.n, j);
-    uint32_t b = (uint32_t)(i & 63);
-    uint64_t x = 0;
-    if ((w >> b) & 1) {
-        uint64_t rank = a.vpre[col.vword_base + j] - a.vpre[col.vword_base] +
-                        (uint64_t)__popcll(w & ((1ull << b) - 1));
-        x = a.lane[col.lane_off + rank];
+    __syncthreads();
+    for (int k = threadIdx.x; k < 16 * 64; k += 256) {
+        int wi = k >> 6, c = wcol[wi];
+        if (c < 0) continue;
+        const DDecCol &col = a.cols[c];
+        uint64_t lj = w0 + wi - col.vword_base;
+        uint64_t i = lj * 64 + (k & 63);
+        if (i >= col.n) continue;
+        uint64_t w = words[wi];
+        uint32_t b = k & 63;
+        if (col.validity_out && (b & 7) == 0) col.validity_out[i >> 3] = (uint8_t)(w >> b);
+        if (a.dst[c].nv == col.n) continue;  // decoded in place
+        uint64_t x = 0;
+        if ((w >> b) & 1)
+            x = a.lane[col.lane_off + wpre[wi] + (uint64_t)__popcll(w & ((1ull << b) - 1))];
+        store_typed(col, i, x);
     }
-    store_typed(col, i, x);
+}
+
+// validity_out of streams without a bitmap: all valid, trailing bits cleared
+__global__ void k_dec_allvalid(DecArgs a) {
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.nvals || *a.err) return;
+    const DDecCol &col = a.cols[dcol_by_val(a, g)];
+    uint64_t i = g - col.val_base;
+    if (i >= col.n || col.nvwords || !col.validity_out || (i & 7)) return;
+    uint64_t rem = col.n - i;
+    col.validity_out[i >> 3] = rem >= 8 ? 0xFF : (uint8_t)((1u << rem) - 1);
 }
 
 static inline unsigned nbk(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
@@ -344,15 +549,25 @@ static inline unsigned nbk(uint64_t n, unsigned b) { return (unsigned)((n + b - 
     } while (0)
 
 void qa_decode_launch(const DecArgs &a, hipStream_t s) {
-    if (a.nframes) {
-        QA_LAUNCH(k_dec_index, dim3(nbk(a.nframes, 64)), dim3(64), 0, s, a);
-        QA_LAUNCH(k_dec_delta_sums, dim3(nbk(a.ntiles, 4)), dim3(256), 0, s, a);
-        QA_LAUNCH(k_dec_delta_scan, dim3(nbk(a.nframes, 64)), dim3(64), 0, s, a);
-        QA_LAUNCH(k_dec_tiles, dim3(nbk(a.ntiles, 4)), dim3(256), 0, s, a);
+    if (a.nvbytes) {
+        QA_LAUNCH(k_vb_ends, dim3(nbk(a.nvbytes, 256)), dim3(256), 0, s, a);
+        qa_exclusive_scan(a.vbpre, a.nvbytes, a.vbpre, a.scan_tmp, s);
+        QA_LAUNCH(k_vb_runs, dim3(nbk(a.nvbytes, 256)), dim3(256), 0, s, a);
+        qa_exclusive_scan_dn(a.vruns, a.nvbytes, a.vbpre + a.nvbytes, a.vruns, a.scan_tmp, s);
     }
+    QA_LAUNCH(k_vb_colfin, dim3(nbk(a.ncols, 64)), dim3(64), 0, s, a);
     if (a.nvwords) {
+        QA_LAUNCH(k_vb_words, dim3(nbk(a.nvwords, 256)), dim3(256), 0, s, a);
         QA_LAUNCH(k_dec_vcount, dim3(nbk(a.nvwords, 256)), dim3(256), 0, s, a);
         qa_exclusive_scan(a.vpre, a.nvwords, a.vpre, a.scan_tmp, s);
     }
-    if (a.nvals) QA_LAUNCH(k_dec_scatter, dim3(nbk(a.nvals, 256)), dim3(256), 0, s, a);
+    QA_LAUNCH(k_dec_colfin, dim3(nbk(a.ncols, 64)), dim3(64), 0, s, a);
+    if (a.nframes) {
+        QA_LAUNCH(k_dec_index, dim3(nbk(a.nframes, 64)), dim3(64), 0, s, a);
+        QA_LAUNCH(k_dec_delta_sums, dim3(nbk(a.ntiles, 4)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_dec_delta_scan, dim3(nbk(a.nframes, 4)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_dec_tiles, dim3(nbk(a.ntiles, 4)), dim3(256), 0, s, a);
+    }
+    if (a.nvwords) QA_LAUNCH(k_dec_scatter, dim3(nbk(a.nvwords, 16)), dim3(256), 0, s, a);
+    QA_LAUNCH(k_dec_allvalid, dim3(nbk(a.nvals, 256)), dim3(256), 0, s, a);
 }

@@ -1,76 +1,85 @@
 // qa_encode.hip — gfx950 kernels of the batched block competition (encode).
 //
-// Stages (one launch each, every block of every column of the batch at once):
-//   1 validity: popcount words -> scan -> compact+lower to the u64 lane
-//   2 probe every 32 Ki region (src/encoder.rs:705-750)
-//   3 plan blocks per column (src/encoder.rs:154-178)
-//   4 full-block features + looks_incompressible (src/encoder.rs:655-683)
-//   5 per-tile statistics (one wavefront per 1024-value FastLanes sub-block)
-//   6 per-block candidate sizes + the ordered Best replay (src/encoder.rs:99-140,
-//     275-469)
-//   7 validity blob (src/validity.rs:40-70) and frame layout per column
-//   8 emit headers and payloads
+// Two passes over the values:
+//   pass 1 (k_region_stats): one workgroup per region slot computes every
+//     block-independent statistic of its 1024-value tiles and the region's
+//     probe features (src/encoder.rs:696-750) and incompressibility samples
+//     (src/encoder.rs:655-683).
+//   pass 2 (k_emit_tiles): one wavefront per tile writes the winner's payload.
+// Between them: block planning (src/encoder.rs:154-178), features of grown
+// blocks, the per-block candidate sizes and the ordered Best::consider replay
+// (src/encoder.rs:99-140, 275-469), the validity blob (src/validity.rs:40-70)
+// and the frame layout (src/encoder.rs:184-214, 752-759).
 #include "qa_kernels.h"
 
 #define WAVE 64
 #define NONE32 0xFFFFFFFFu
+#define RF_DLOW 1u
+#define RF_REP 2u
+#define RF_INC 4u
 
 // ---------------------------------------------------------------------------
-// column lookups (columns are sorted by every *_base field)
+// lookups and lane access
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int col_by_region(const DCol *c, int nc, uint32_t r) {
-    int lo = 0, hi = nc - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (c[mid].region_base <= r) lo = mid; else hi = mid - 1;
-    }
+#define QA_COL_SEARCH(field, key)                                   \
+    int lo = 0, hi = nc - 1;                                        \
+    while (lo < hi) {                                               \
+        int mid = (lo + hi + 1) >> 1;                               \
+        if (c[mid].field <= (key)) lo = mid; else hi = mid - 1;     \
+    }                                                               \
     return lo;
+
+__device__ __forceinline__ int col_by_region(const DCol *c, int nc, uint32_t r) {
+    QA_COL_SEARCH(region_base, r)
 }
 __device__ __forceinline__ int col_by_tile(const DCol *c, int nc, uint64_t t) {
-    int lo = 0, hi = nc - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (c[mid].tile_base <= t) lo = mid; else hi = mid - 1;
-    }
-    return lo;
+    QA_COL_SEARCH(tile_base, t)
 }
-__device__ __forceinline__ int col_by_val(const DCol *c, int nc, uint64_t v) {
-    int lo = 0, hi = nc - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (c[mid].val_base <= v) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-// among columns WITH a bitmap (nvwords > 0): column owning validity word j
+// zero-word columns share the next column's base and lose the tie
 __device__ __forceinline__ int col_by_vword(const DCol *c, int nc, uint64_t j) {
-    int lo = 0, hi = nc - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (c[mid].vword_base <= j) lo = mid; else hi = mid - 1;
-    }
-    return lo;  // zero-word columns share the next column's base and lose ties
+    QA_COL_SEARCH(vword_base, j)
 }
 __device__ __forceinline__ uint64_t lane_bytes_of(uint32_t dt) {
     return (dt == QA_I32 || dt == QA_U32 || dt == QA_F32) ? 4 : 8;
 }
 __device__ __forceinline__ bool is_float(uint32_t dt) { return dt == QA_F64 || dt == QA_F32; }
 __device__ __forceinline__ bool is_signed(uint32_t dt) { return dt == QA_I32 || dt == QA_I64; }
+__device__ __forceinline__ uint64_t lane_at(const DColState &st, uint64_t i) {
+    return qa_lane_get(st.lanep, st.lkind, i);
+}
 
 // bitmap word j of a column (bits >= n cleared); byte loads: the caller's
 // bitmap is ceil(n/8) bytes with no padding or alignment promise
 __device__ __forceinline__ uint64_t bm_word(const uint8_t *bm, uint64_t n, uint64_t j) {
     uint64_t nbytes = (n + 7) >> 3, b0 = j * 8;
     uint64_t w = 0;
-    for (int k = 0; k < 8; k++)
-        if (b0 + k < nbytes) w |= (uint64_t)bm[b0 + k] << (8 * k);
+    if (b0 + 8 <= nbytes) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) w |= (uint64_t)bm[b0 + k] << (8 * k);
+    } else {
+        for (int k = 0; k < 8; k++)
+            if (b0 + k < nbytes) w |= (uint64_t)bm[b0 + k] << (8 * k);
+    }
     uint64_t bits = n - j * 64;
     if (bits < 64) w &= (1ull << bits) - 1;
     return w;
 }
 
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_reduce(T v, Op op) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = op(v, (T)__shfl_xor(v, d, 64));
+    return v;
+}
+struct OpMinU { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a < b ? a : b; } };
+struct OpMaxU { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a > b ? a : b; } };
+struct OpMinS { __device__ int64_t operator()(int64_t a, int64_t b) const { return a < b ? a : b; } };
+struct OpMaxS { __device__ int64_t operator()(int64_t a, int64_t b) const { return a > b ? a : b; } };
+struct OpAdd { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a + b; } };
+struct OpAnd { __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a & b; } };
+
 // ---------------------------------------------------------------------------
-// stage 1: validity popcounts, compaction and lowering (src/lib.rs:429-455)
+// stage 1: validity popcounts / transitions, lane setup, compaction
 // ---------------------------------------------------------------------------
 __global__ void k_vcount(EncArgs a) {
     uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,8 +89,8 @@ __global__ void k_vcount(EncArgs a) {
     uint64_t lj = j - col.vword_base;
     uint64_t w = bm_word(col.validity, col.n, lj);
     a.vpre[j] = (uint64_t)__popcll(w);
-    // transitions for the validity RLE (src/validity.rs:40-60): runs alternate
-    // starting from a 0-run, so bit -1 counts as 0
+    // transitions for the validity RLE: runs alternate starting from a 0-run,
+    // so bit -1 counts as 0 (src/validity.rs:40-60)
     uint64_t prev_top = lj ? (bm_word(col.validity, col.n, lj - 1) >> 63) : 0;
     uint64_t t = w ^ ((w << 1) | prev_top);
     uint64_t bits = col.n - lj * 64;
@@ -89,7 +98,22 @@ __global__ void k_vcount(EncArgs a) {
     a.tpre[j] = (uint64_t)__popcll(t);
 }
 
-// per-column counts from the global scans
+__device__ __forceinline__ uint32_t lkind_of(const DCol &col) {
+    switch (col.in_kind) {
+    case QA_IN_I8: return QA_LK_I8;
+    case QA_IN_I16: return QA_LK_I16;
+    case QA_IN_U8: return QA_LK_U8;
+    case QA_IN_U16: return QA_LK_U16;
+    default: break;
+    }
+    switch (col.dtype) {
+    case QA_I32: return QA_LK_I32;
+    case QA_U32: return QA_LK_U32;
+    case QA_F32: return QA_LK_F32;
+    default: return QA_LK_U64;
+    }
+}
+
 __global__ void k_col_counts(EncArgs a) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (int)a.ncols) return;
@@ -105,6 +129,11 @@ __global__ void k_col_counts(EncArgs a) {
         st.has_nulls = 0;
         a.colbase[c] = a.nvwords ? a.tpre[col.vword_base] : 0;
     }
+    st.lkind = lkind_of(col);
+    st.lwidth = col.in_width;
+    // without nulls the kernels read the caller's values in place
+    st.lanep = st.has_nulls ? (const void *)((const uint8_t *)a.lane + col.lane_off * 8)
+                            : col.values;
     if (c == (int)a.ncols - 1) {
         uint64_t T = a.nvwords ? a.tpre[a.nvwords] : 0;
         a.colbase[a.ncols] = T;
@@ -112,310 +141,367 @@ __global__ void k_col_counts(EncArgs a) {
     }
 }
 
-__device__ __forceinline__ uint64_t load_lane(const DCol &col, uint64_t i) {
-    switch (col.in_kind) {
-    case QA_IN_I8: return (uint64_t)(int64_t)((const int8_t *)col.values)[i];
-    case QA_IN_I16: return (uint64_t)(int64_t)((const int16_t *)col.values)[i];
-    case QA_IN_U8: return (uint64_t)((const uint8_t *)col.values)[i];
-    case QA_IN_U16: return (uint64_t)((const uint16_t *)col.values)[i];
-    default: break;
-    }
-    switch (col.dtype) {
-    case QA_I32: return (uint64_t)(int64_t)((const int32_t *)col.values)[i];
-    case QA_U32: return (uint64_t)((const uint32_t *)col.values)[i];
-    case QA_F32: return qa_f32_to_lane(((const uint32_t *)col.values)[i]);
-    default: return ((const uint64_t *)col.values)[i];
-    }
-}
-
-// one thread per logical value: lower to the lane, compacting out nulls
-__global__ void k_to_lane(EncArgs a) {
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.nvals) return;
-    int c = col_by_val(a.cols, a.ncols, g);
-    const DCol &col = a.cols[c];
-    uint64_t i = g - col.val_base;
-    if (i >= col.n) return;
-    uint64_t v = load_lane(col, i);
-    if (col.nvwords && a.cst[c].has_nulls) {
-        uint64_t j = i >> 6;
-        uint64_t w = bm_word(col.validity, col.n, j);
-        uint32_t b = (uint32_t)(i & 63);
-        if (!((w >> b) & 1)) return;
-        uint64_t rank = a.vpre[col.vword_base + j] - a.vpre[col.vword_base] +
-                        (uint64_t)__popcll(w & ((1ull << b) - 1));
-        a.lane[col.lane_off + rank] = v;
-    } else {
-        a.lane[col.lane_off + i] = v;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// stage 2: region probe (src/encoder.rs:705-750)
-// ---------------------------------------------------------------------------
-#define RF_DLOW 1u
-#define RF_REP 2u
-
-// workgroup-wide features of lane[base .. base+len)
-__device__ void probe_range(const uint64_t *v, uint64_t len, uint32_t *exp_range, bool *dlow,
-                            bool *rep) {
-    __shared__ uint64_t seen[256];
-    __shared__ uint32_t red[3];
-    for (int k = threadIdx.x; k < 256; k += blockDim.x) seen[k] = 0;
-    if (threadIdx.x == 0) {
-        red[0] = 0xFFFFFFFFu;
-        red[1] = 0;
-        red[2] = 0;
+// compaction of the valid values, in the input's own element width
+// (src/validity.rs:122-127); one workgroup per 16 validity words
+__global__ void __launch_bounds__(256) k_compact(EncArgs a) {
+    __shared__ uint64_t words[16], wpre[16];
+    __shared__ int wcol[16];
+    uint64_t w0 = (uint64_t)blockIdx.x * 16;
+    if (threadIdx.x < 16) {
+        uint64_t j = w0 + threadIdx.x;
+        if (j < a.nvwords) {
+            int c = col_by_vword(a.cols, a.ncols, j);
+            const DCol &col = a.cols[c];
+            wcol[threadIdx.x] = c;
+            words[threadIdx.x] = bm_word(col.validity, col.n, j - col.vword_base);
+            wpre[threadIdx.x] = a.vpre[j] - a.vpre[col.vword_base];
+        } else {
+            wcol[threadIdx.x] = -1;
+        }
     }
     __syncthreads();
-    uint32_t mn = 0xFFFFFFFFu, mx = 0, eq = 0;
-    for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) {
-        uint64_t x = v[i];
-        uint32_t e = (uint32_t)((x >> 52) & 0x7FF);
-        mn = min(mn, e);
-        mx = max(mx, e);
-        if (i == 0 || v[i - 1] == x) eq++;
+    for (int k = threadIdx.x; k < 16 * 64; k += 256) {
+        int wi = k >> 6;
+        int c = wcol[wi];
+        if (c < 0) continue;
+        const DCol &col = a.cols[c];
+        if (!a.cst[c].has_nulls) continue;
+        uint64_t lj = w0 + wi - col.vword_base;
+        uint64_t i = lj * 64 + (k & 63);
+        if (i >= col.n) continue;
+        uint64_t w = words[wi];
+        uint32_t b = k & 63;
+        if (!((w >> b) & 1)) continue;
+        uint64_t rank = wpre[wi] + (uint64_t)__popcll(w & ((1ull << b) - 1));
+        uint8_t *dst = (uint8_t *)a.lane + col.lane_off * 8;
+        const uint8_t *src = (const uint8_t *)col.values;
+        switch (col.in_width) {
+        case 1: dst[rank] = src[i]; break;
+        case 2: ((uint16_t *)dst)[rank] = ((const uint16_t *)src)[i]; break;
+        case 4: ((uint32_t *)dst)[rank] = ((const uint32_t *)src)[i]; break;
+        default: ((uint64_t *)dst)[rank] = ((const uint64_t *)src)[i];
+        }
     }
-    uint64_t stride = len / 4096 > 1 ? len / 4096 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// stage 2: fused tile statistics + region probe, one workgroup per region
+// ---------------------------------------------------------------------------
+struct RegionAcc {
+    uint32_t exp_min, exp_max, eq, dsum;
+    uint64_t smin, smax, sdz;
+};
+
+__global__ void __launch_bounds__(1024) k_region_stats(EncArgs a) {
+    __shared__ uint64_t seen[256];
+    __shared__ RegionAcc acc;
+    const uint32_t r = blockIdx.x;
+    const int c = col_by_region(a.cols, a.ncols, r);
+    const DCol &col = a.cols[c];
+    const DColState &st = a.cst[c];
+    const uint64_t nv = st.nv;
+    const uint64_t rs = (uint64_t)(r - col.region_base) * col.region_size;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    DTile *tl = a.tiles + col.tile_base + (uint64_t)(r - col.region_base) * col.tpr;
+    if (rs >= nv) {
+        for (uint32_t t = threadIdx.x; t < col.tpr; t += blockDim.x) tl[t].active = 0;
+        return;
+    }
+    const uint64_t len = nv - rs < col.region_size ? nv - rs : col.region_size;
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) seen[k] = 0;
+    if (threadIdx.x == 0) {
+        acc.exp_min = 0xFFFFFFFFu;
+        acc.exp_max = 0;
+        acc.eq = 0;
+        acc.dsum = 0;
+        acc.smin = ~0ull;
+        acc.smax = 0;
+        acc.sdz = 0;
+    }
+    __syncthreads();
+    const uint32_t ntl = (uint32_t)((len + QA_TILE - 1) / QA_TILE);
+    uint32_t eq_region = 0;  // consecutive-equal pairs inside the region (lane 0 sums)
+    for (uint32_t t = wv; t < col.tpr; t += nwv) {
+        DTile &ts = tl[t];
+        if (t >= ntl) {
+            if (lane == 0) ts.active = 0;
+            continue;
+        }
+        const uint64_t k0 = rs + (uint64_t)t * QA_TILE;  // column index of the tile start
+        const uint32_t cnt = (uint32_t)min((uint64_t)QA_TILE, rs + len - k0);
+        uint64_t umin = ~0ull, umax = 0, dzmin = ~0ull, dzmax = 0, xb = 0;
+        int64_t smin = INT64_MAX, smax = INT64_MIN;
+        uint32_t emin = 0xFFFFFFFFu, emax = 0, chg = 0, eqp = 0, arith = 1, same = 1;
+        const uint64_t vf = lane_at(st, k0);
+        const uint64_t step = cnt > 1 ? lane_at(st, k0 + 1) - vf : 0;
+        for (uint32_t k = lane; k < cnt; k += WAVE) {
+            uint64_t x = lane_at(st, k0 + k);
+            umin = x < umin ? x : umin;
+            umax = x > umax ? x : umax;
+            smin = (int64_t)x < smin ? (int64_t)x : smin;
+            smax = (int64_t)x > smax ? (int64_t)x : smax;
+            uint32_t e = (uint32_t)((x >> 52) & 0x7FF);
+            emin = min(emin, e);
+            emax = max(emax, e);
+            same &= x == vf;
+            if (k > 0) {
+                uint64_t p = lane_at(st, k0 + k - 1);
+                uint64_t z = qa_zigzag(x - p);
+                dzmin = z < dzmin ? z : dzmin;
+                dzmax = z > dzmax ? z : dzmax;
+                xb += qa_varint_len(x ^ p);
+                chg += x != p;
+                arith &= (x - p) == step;
+                eqp += x == p;
+            } else if (k0 > rs) {
+                eqp += x == lane_at(st, k0 - 1);  // pair across tiles of this region
+            }
+        }
+        umin = wave_reduce(umin, OpMinU());
+        umax = wave_reduce(umax, OpMaxU());
+        smin = wave_reduce(smin, OpMinS());
+        smax = wave_reduce(smax, OpMaxS());
+        dzmin = wave_reduce(dzmin, OpMinU());
+        dzmax = wave_reduce(dzmax, OpMaxU());
+        xb = wave_reduce(xb, OpAdd());
+        uint64_t chg64 = wave_reduce((uint64_t)chg, OpAdd());
+        uint64_t eq64 = wave_reduce((uint64_t)eqp, OpAdd());
+        emin = (uint32_t)wave_reduce((uint64_t)emin, OpMinU());
+        emax = (uint32_t)wave_reduce((uint64_t)emax, OpMaxU());
+        arith = wave_reduce(arith, OpAnd());
+        same = wave_reduce(same, OpAnd());
+        if (lane == 0) {
+            ts.umin = umin;
+            ts.umax = umax;
+            ts.smin = smin;
+            ts.smax = smax;
+            ts.dzmin = dzmin;
+            ts.dzmax = dzmax;
+            ts.xorz_rest = xb;
+            ts.chg_rest = (uint32_t)chg64;
+            ts.exp_min = emin;
+            ts.exp_max = emax;
+            ts.v_first = vf;
+            ts.v_last = lane_at(st, k0 + cnt - 1);
+            ts.step = step;
+            ts.dz_first = k0 ? qa_zigzag(vf - lane_at(st, k0 - 1)) : 0;
+            ts.cnt = cnt;
+            ts.arith = (uint8_t)arith;
+            ts.allsame = (uint8_t)same;
+            ts.active = 1;
+            eq_region += (uint32_t)eq64;
+            atomicMin(&acc.exp_min, emin);
+            atomicMax(&acc.exp_max, emax);
+        }
+    }
+    if (lane == 0) atomicAdd(&acc.eq, eq_region);
+    // distinct estimate over a strided sample (src/encoder.rs:728-743)
+    const uint64_t stride = len / 4096 > 1 ? len / 4096 : 1;
     for (uint64_t k = (uint64_t)threadIdx.x * stride; k < len; k += (uint64_t)blockDim.x * stride) {
-        uint64_t h = v[k] * 0x9E3779B97F4A7C15ull;
+        uint64_t h = lane_at(st, rs + k) * 0x9E3779B97F4A7C15ull;
         uint32_t bucket = (uint32_t)((h >> 50) & 0x3FFF);
         atomicOr((unsigned long long *)&seen[bucket >> 6], 1ull << (bucket & 63));
     }
-    atomicMin(&red[0], mn);
-    atomicMax(&red[1], mx);
-    atomicAdd(&red[2], eq);
+    // incompressibility samples for the region taken as a block
+    const uint64_t step = len / 2048 > 1 ? len / 2048 : 1;
+    const uint64_t v0 = lane_at(st, rs);
+    uint64_t mn = v0, mx = v0, dz = 0;
+    for (uint64_t i = step * (1 + threadIdx.x); i < len; i += step * blockDim.x) {
+        uint64_t x = lane_at(st, rs + i);
+        mn = x < mn ? x : mn;
+        mx = x > mx ? x : mx;
+        uint64_t z = qa_zigzag(x - lane_at(st, rs + i - 1));
+        dz = z > dz ? z : dz;
+    }
+    mn = wave_reduce(mn, OpMinU());
+    mx = wave_reduce(mx, OpMaxU());
+    dz = wave_reduce(dz, OpMaxU());
+    if (lane == 0) {
+        atomicMin((unsigned long long *)&acc.smin, mn);
+        atomicMax((unsigned long long *)&acc.smax, mx);
+        atomicMax((unsigned long long *)&acc.sdz, dz);
+    }
     __syncthreads();
     uint32_t d = 0;
     for (int k = threadIdx.x; k < 256; k += blockDim.x) d += (uint32_t)__popcll(seen[k]);
-    __shared__ uint32_t dsum;
-    if (threadIdx.x == 0) dsum = 0;
+    d = (uint32_t)wave_reduce((uint64_t)d, OpAdd());
+    if (lane == 0) atomicAdd(&acc.dsum, d);
     __syncthreads();
-    atomicAdd(&dsum, d);
-    __syncthreads();
-    if (len == 0) {
-        *exp_range = 0;
-        *dlow = true;
-        *rep = false;
-    } else {
-        *exp_range = red[1] - red[0];
-        *dlow = dsum < 2048;
-        uint64_t c2 = (uint64_t)red[2] * 2;
+    if (threadIdx.x == 0) {
+        bool dl = acc.dsum < 2048;
+        uint64_t c2 = (uint64_t)(acc.eq + 1) * 2;  // consec_eq counts element 0 against itself
         if (c2 > 0xFFFFFFFFull) c2 = 0xFFFFFFFFull;
-        *rep = c2 > (uint32_t)len;
+        bool rp = c2 > (uint32_t)len;
+        bool inc = false;
+        if (!is_float(col.dtype) && !dl && !rp && len >= 256) {
+            uint32_t thr = (uint32_t)(lane_bytes_of(col.dtype) * 8) - 4;
+            inc = qa_width(acc.smax - acc.smin) >= thr && qa_width(acc.sdz) >= thr;
+        }
+        a.rfeat[r] = (dl ? RF_DLOW : 0) | (rp ? RF_REP : 0) | (inc ? RF_INC : 0) |
+                     ((acc.exp_max - acc.exp_min) << 8);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stage 3: block planning (src/encoder.rs:154-178), one workgroup per column
+// ---------------------------------------------------------------------------
+#define PLAN_CHUNK 4096
+
+__global__ void __launch_bounds__(256) k_plan(EncArgs a) {
+    __shared__ uint8_t feat[PLAN_CHUNK];
+    __shared__ uint32_t own[PLAN_CHUNK];
+    __shared__ uint32_t next_s, nb_s, cur_owner;
+    const int c = blockIdx.x;
+    const DCol &col = a.cols[c];
+    const uint64_t nv = a.cst[c].nv;
+    const uint32_t R = col.region_size;
+    const uint32_t nreg = (uint32_t)((nv + R - 1) / R);  // regions holding values
+    if (threadIdx.x == 0) {
+        next_s = 0;
+        nb_s = 0;
+        cur_owner = NONE32;
+    }
+    for (uint32_t base = 0; base < col.nregions; base += PLAN_CHUNK) {
+        uint32_t cn = min((uint32_t)PLAN_CHUNK, col.nregions - base);
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < cn; k += blockDim.x)
+            feat[k] = base + k < nreg ? (uint8_t)(a.rfeat[col.region_base + base + k] & 3) : 0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // the sequential walk, in LDS: a region starts a block iff it is
+            // the region after the previous block's end
+            uint32_t ns = next_s, nb = nb_s, ow = cur_owner;
+            for (uint32_t k = 0; k < cn; k++) {
+                uint32_t lr = base + k;
+                if (lr >= nreg) {
+                    own[k] = NONE32;
+                    continue;
+                }
+                if (lr == ns) {
+                    uint64_t rs = (uint64_t)lr * R;
+                    uint64_t end = (col.adaptive && feat[k]) ? min(rs + (uint64_t)QA_MAX_Q, nv)
+                                                             : min(rs + (uint64_t)R, nv);
+                    ns = (uint32_t)((end + R - 1) / R);
+                    nb++;
+                    ow = col.region_base + lr;
+                }
+                own[k] = ow;
+            }
+            next_s = ns;
+            nb_s = nb;
+            cur_owner = ow;
+        }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < cn; k += blockDim.x) {
+            uint32_t lr = base + k, slot = col.region_base + lr;
+            DBlock &b = a.blocks[slot];
+            uint32_t o = own[k];
+            a.owner[slot] = o;
+            if (o == slot) {
+                uint64_t rs = (uint64_t)lr * R;
+                uint64_t end = (col.adaptive && feat[k]) ? min(rs + (uint64_t)QA_MAX_Q, nv)
+                                                         : min(rs + (uint64_t)R, nv);
+                b.start = rs;
+                b.len = (uint32_t)(end - rs);
+                b.col = (uint32_t)c;
+                b.ntiles = (uint32_t)((end - rs + QA_TILE - 1) / QA_TILE);
+            } else {
+                b.len = 0;
+            }
+        }
     }
     __syncthreads();
-}
-
-__global__ void __launch_bounds__(256) k_probe_regions(EncArgs a) {
-    uint32_t r = blockIdx.x;
-    int c = col_by_region(a.cols, a.ncols, r);
-    const DCol &col = a.cols[c];
-    uint64_t nv = a.cst[c].nv;
-    uint64_t start = (uint64_t)(r - col.region_base) * col.region_size;
-    if (!col.adaptive || start >= nv) return;
-    uint64_t len = nv - start < col.region_size ? nv - start : col.region_size;
-    uint32_t er;
-    bool dl, rp;
-    probe_range(a.lane + col.lane_off + start, len, &er, &dl, &rp);
-    if (threadIdx.x == 0) a.rfeat[r] = (dl ? RF_DLOW : 0) | (rp ? RF_REP : 0);
+    if (threadIdx.x == 0) a.cst[c].nblocks = nb_s;
 }
 
 // ---------------------------------------------------------------------------
-// stage 3: block planning, one thread per column (src/encoder.rs:154-178)
+// stage 4: features of blocks grown past one region (131072 values)
 // ---------------------------------------------------------------------------
-__global__ void k_plan(EncArgs a) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (int)a.ncols) return;
-    const DCol &col = a.cols[c];
-    uint64_t nv = a.cst[c].nv;
-    uint32_t nb = 0;
-    uint64_t start = 0;
-    uint32_t R = col.region_size;
-    for (uint32_t lr = 0; lr < col.nregions; lr++) {
-        uint32_t slot = col.region_base + lr;
-        DBlock &b = a.blocks[slot];
-        uint64_t rs = (uint64_t)lr * R;
-        if (rs >= nv) {
-            b.len = 0;
-            a.owner[slot] = NONE32;
-            continue;
-        }
-        if (rs < start) {  // covered by a grown block
-            b.len = 0;
-            continue;
-        }
-        uint64_t end;
-        if (col.adaptive) {
-            uint32_t f = a.rfeat[slot];
-            uint64_t be = rs + R < nv ? rs + R : nv;
-            end = (f & (RF_DLOW | RF_REP)) ? (rs + QA_MAX_Q < nv ? rs + QA_MAX_Q : nv) : be;
-        } else {
-            end = rs + R < nv ? rs + R : nv;
-        }
-        b.start = rs;
-        b.len = (uint32_t)(end - rs);
-        b.col = (uint32_t)c;
-        b.ntiles = (uint32_t)((end - rs + QA_TILE - 1) / QA_TILE);
-        for (uint32_t k = lr; (uint64_t)k * R < end && k < col.nregions; k++)
-            a.owner[col.region_base + k] = slot;
-        start = end;
-        nb++;
-    }
-    a.cst[c].nblocks = nb;
-}
-
-// ---------------------------------------------------------------------------
-// stage 4: features of the whole block + incompressibility probe
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_block_feats(EncArgs a) {
-    uint32_t slot = blockIdx.x;
+__global__ void __launch_bounds__(256) k_grown_feats(EncArgs a) {
+    __shared__ uint64_t seen[256];
+    __shared__ uint64_t smin, smax, sdz;
+    __shared__ uint32_t dsum, emin_s, emax_s, eq_s;
+    const uint32_t slot = blockIdx.x;
     DBlock &b = a.blocks[slot];
     if (b.len == 0) return;
     const DCol &col = a.cols[b.col];
-    const uint64_t *v = a.lane + col.lane_off + b.start;
-    uint64_t n = b.len;
-    uint32_t er;
-    bool dl, rp;
-    probe_range(v, n, &er, &dl, &rp);
-    // looks_incompressible (src/encoder.rs:655-683): sampled value range and
-    // zig-zag delta range both within 4 bits of the lane width
-    __shared__ uint64_t smin, smax, sdz;
+    const DColState &st = a.cst[b.col];
+    const uint32_t f = a.rfeat[slot];
+    if (b.len <= col.region_size) {  // a one-region block: its region's probe
+        if (threadIdx.x == 0) {
+            b.distinct_low = (f & RF_DLOW) != 0;
+            b.repeats = (f & RF_REP) != 0;
+            b.incompressible = (f & RF_INC) != 0;
+            b.exp_range = f >> 8;
+        }
+        return;
+    }
+    const uint64_t n = b.len, s0 = b.start;
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) seen[k] = 0;
     if (threadIdx.x == 0) {
-        smin = v[0];
-        smax = v[0];
+        uint64_t v0 = lane_at(st, s0);
+        smin = v0;
+        smax = v0;
         sdz = 0;
+        dsum = 0;
+        emin_s = 0xFFFFFFFFu;
+        emax_s = 0;
+        eq_s = 0;
     }
     __syncthreads();
-    uint64_t step = n / 2048 > 1 ? n / 2048 : 1;
-    uint64_t mn = v[0], mx = v[0], dz = 0;
+    const uint64_t stride = n / 4096 > 1 ? n / 4096 : 1;
+    for (uint64_t k = (uint64_t)threadIdx.x * stride; k < n; k += (uint64_t)blockDim.x * stride) {
+        uint64_t h = lane_at(st, s0 + k) * 0x9E3779B97F4A7C15ull;
+        uint32_t bucket = (uint32_t)((h >> 50) & 0x3FFF);
+        atomicOr((unsigned long long *)&seen[bucket >> 6], 1ull << (bucket & 63));
+    }
+    const uint64_t step = n / 2048 > 1 ? n / 2048 : 1;
+    uint64_t mn = ~0ull, mx = 0, dz = 0;
     for (uint64_t i = step * (1 + threadIdx.x); i < n; i += step * blockDim.x) {
-        uint64_t x = v[i];
+        uint64_t x = lane_at(st, s0 + i);
         mn = x < mn ? x : mn;
         mx = x > mx ? x : mx;
-        uint64_t z = qa_zigzag(x - v[i - 1]);
+        uint64_t z = qa_zigzag(x - lane_at(st, s0 + i - 1));
         dz = z > dz ? z : dz;
     }
     atomicMin((unsigned long long *)&smin, mn);
     atomicMax((unsigned long long *)&smax, mx);
     atomicMax((unsigned long long *)&sdz, dz);
+    // exp range and consecutive-equal count combine from the tiles
+    const DTile *tl = a.tiles + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
+    for (uint32_t t = threadIdx.x; t < b.ntiles; t += blockDim.x) {
+        const DTile &ts = tl[t];
+        atomicMin(&emin_s, ts.exp_min);
+        atomicMax(&emax_s, ts.exp_max);
+        uint32_t e = ts.cnt - 1 - ts.chg_rest;
+        if (t > 0) e += ts.v_first == tl[t - 1].v_last;
+        atomicAdd(&eq_s, e);
+    }
+    __syncthreads();
+    uint32_t d = 0;
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) d += (uint32_t)__popcll(seen[k]);
+    atomicAdd(&dsum, d);
     __syncthreads();
     if (threadIdx.x == 0) {
-        b.exp_range = er;
-        b.distinct_low = dl;
-        b.repeats = rp;
+        bool dl = dsum < 2048;
+        uint64_t c2 = (uint64_t)(eq_s + 1) * 2;
+        if (c2 > 0xFFFFFFFFull) c2 = 0xFFFFFFFFull;
+        bool rp = c2 > (uint32_t)n;
         bool inc = false;
         if (!is_float(col.dtype) && !dl && !rp && n >= 256) {
             uint32_t thr = (uint32_t)(lane_bytes_of(col.dtype) * 8) - 4;
-            uint32_t vw = qa_width(smax - smin), dw = qa_width(sdz);
-            inc = vw >= thr && dw >= thr;
+            inc = qa_width(smax - smin) >= thr && qa_width(sdz) >= thr;
         }
+        b.distinct_low = dl;
+        b.repeats = rp;
         b.incompressible = inc;
-        b.v0 = v[0];
-        b.v1 = n > 1 ? v[1] : 0;
+        b.exp_range = emax_s - emin_s;
     }
 }
 
 // ---------------------------------------------------------------------------
-// stage 5: per-tile statistics, one wavefront per 1024-value sub-block
-// ---------------------------------------------------------------------------
-template <typename T, typename Op>
-__device__ __forceinline__ T wave_reduce(T v, Op op) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = op(v, (T)__shfl_xor(v, d, 64));
-    return v;
-}
-struct OpMinU { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a < b ? a : b; } };
-struct OpMaxU { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a > b ? a : b; } };
-struct OpMinS { __device__ int64_t operator()(int64_t a, int64_t b) const { return a < b ? a : b; } };
-struct OpMaxS { __device__ int64_t operator()(int64_t a, int64_t b) const { return a > b ? a : b; } };
-struct OpAdd { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a + b; } };
-
-// resolve a global tile slot to (column, block slot, tile index within block);
-// returns false for an inactive slot
-__device__ __forceinline__ bool tile_locate(const EncArgs &a, uint64_t t, int *cp, uint32_t *slotp,
-                                            uint32_t *btp) {
-    int c = col_by_tile(a.cols, a.ncols, t);
-    const DCol &col = a.cols[c];
-    uint64_t lt = t - col.tile_base;
-    uint32_t lr = (uint32_t)(lt / col.tpr), tin = (uint32_t)(lt % col.tpr);
-    if (lr >= col.nregions) return false;
-    uint32_t o = a.owner[col.region_base + lr];
-    if (o == NONE32) return false;
-    const DBlock &b = a.blocks[o];
-    uint32_t bt = (col.region_base + lr - o) * col.tpr + tin;
-    if ((uint64_t)bt * QA_TILE >= b.len) return false;
-    *cp = c;
-    *slotp = o;
-    *btp = bt;
-    return true;
-}
-
-__global__ void __launch_bounds__(256) k_tile_stats(EncArgs a) {
-    const int lane = threadIdx.x & 63;
-    uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (t >= a.ntiles) return;
-    int c;
-    uint32_t slot, bt;
-    DTile &ts = a.tiles[t];
-    if (!tile_locate(a, t, &c, &slot, &bt)) {
-        if (lane == 0) ts.active = 0;
-        return;
-    }
-    const DCol &col = a.cols[c];
-    const DBlock &b = a.blocks[slot];
-    const uint64_t *v = a.lane + col.lane_off + b.start;
-    uint64_t k0 = (uint64_t)bt * QA_TILE;
-    uint32_t cnt = (uint32_t)min((uint64_t)QA_TILE, (uint64_t)b.len - k0);
-    uint64_t umin = ~0ull, umax = 0, dzmin = ~0ull, dzmax = 0, xb = 0;
-    int64_t smin = INT64_MAX, smax = INT64_MIN;
-    uint32_t cbad = 0, sbad = 0, chg = 0;
-    uint64_t stride = b.v1 - b.v0;
-    for (uint32_t k = lane; k < cnt; k += WAVE) {
-        uint64_t i = k0 + k;
-        uint64_t x = v[i];
-        uint64_t p = i ? v[i - 1] : b.v0;   // delta predecessor (base = v0)
-        uint64_t px = i ? v[i - 1] : 0;     // xorz predecessor
-        umin = x < umin ? x : umin;
-        umax = x > umax ? x : umax;
-        smin = (int64_t)x < smin ? (int64_t)x : smin;
-        smax = (int64_t)x > smax ? (int64_t)x : smax;
-        uint64_t z = qa_zigzag(x - p);
-        dzmin = z < dzmin ? z : dzmin;
-        dzmax = z > dzmax ? z : dzmax;
-        xb += qa_varint_len(x ^ px);
-        cbad += x != b.v0;
-        sbad += x != b.v0 + i * stride;
-        chg += (i > 0 && x != p);
-    }
-    umin = wave_reduce(umin, OpMinU());
-    umax = wave_reduce(umax, OpMaxU());
-    smin = wave_reduce(smin, OpMinS());
-    smax = wave_reduce(smax, OpMaxS());
-    dzmin = wave_reduce(dzmin, OpMinU());
-    dzmax = wave_reduce(dzmax, OpMaxU());
-    xb = wave_reduce(xb, OpAdd());
-    uint64_t cb = wave_reduce((uint64_t)cbad, OpAdd());
-    uint64_t sb = wave_reduce((uint64_t)sbad, OpAdd());
-    uint64_t ch = wave_reduce((uint64_t)chg, OpAdd());
-    if (lane == 0) {
-        ts.umin = umin;
-        ts.umax = umax;
-        ts.smin = smin;
-        ts.smax = smax;
-        ts.dzmin = dzmin;
-        ts.dzmax = dzmax;
-        ts.xorz_bytes = xb;
-        ts.const_bad = (uint32_t)cb;
-        ts.stride_bad = (uint32_t)sb;
-        ts.changes = (uint32_t)ch;
-        ts.cnt = cnt;
-        ts.active = 1;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// stage 6: candidate sizes and the ordered replay of Best::consider
+// stage 5: candidate sizes and the ordered replay of Best::consider, one
+// wavefront per block
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t weight_of(uint32_t mode) {
     switch (mode) {  // src/encoder.rs:79-89
@@ -429,78 +515,120 @@ __device__ __forceinline__ uint64_t penalty_of(uint32_t mode, uint64_t lambda, u
     return (lambda * weight_of(mode) * dec) >> 8;  // src/encoder.rs:94-97
 }
 
-__global__ void k_block_decide(EncArgs a) {
-    uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ uint64_t tile_for_range(const DTile &s, bool sg) {
+    return sg ? (uint64_t)s.smax - (uint64_t)s.smin : s.umax - s.umin;
+}
+// zig-zag delta range of a tile inside a block (element 0 of the block has
+// delta 0 against the base v0)
+__device__ __forceinline__ void tile_dz(const DTile &s, bool block_first, uint64_t *mn,
+                                        uint64_t *mx) {
+    uint64_t f = block_first ? 0 : s.dz_first;
+    *mn = s.cnt > 1 ? (s.dzmin < f ? s.dzmin : f) : f;
+    *mx = s.cnt > 1 ? (s.dzmax > f ? s.dzmax : f) : f;
+}
+
+__global__ void __launch_bounds__(256) k_block_decide(EncArgs a) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t slot = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (slot >= a.nregions) return;
     DBlock &b = a.blocks[slot];
     if (b.len == 0) return;
     const DCol &col = a.cols[b.col];
-    const DTile *tl = a.tiles + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
-    uint64_t n = b.len;
-    bool fl = is_float(col.dtype), sg = is_signed(col.dtype);
-    bool reduced = col.level == QA_FASTEST;
-    uint64_t lambda = reduced ? 16 : 4;
-    uint64_t dec = n * lane_bytes_of(col.dtype);
-    uint64_t raw = n * lane_bytes_of(col.dtype);
-    for (int k = 0; k < C_N; k++) b.sz[k] = ~0ull;
-    b.sz[C_RAW] = raw;
-    uint32_t mode = QA_M_RAW;
-    uint64_t best_len = raw, best_score = raw;
-    if (!fl && b.incompressible) {
-        b.mode = mode;
-        b.payload_len = raw;
-        atomicAdd(&a.wins[mode & 63], 1ull);
-        return;
-    }
-    bool cst = true, str = n >= 2;
-    uint64_t xz = 0, fsz = qa_varint_len(n), dsz = 8 + qa_varint_len(n);
-    for (uint32_t t = 0; t < b.ntiles; t++) {
+    DTile *tl = a.tiles + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
+    const uint64_t n = b.len;
+    const bool fl = is_float(col.dtype), sg = is_signed(col.dtype);
+    const bool reduced = col.level == QA_FASTEST;
+    const uint64_t lambda = reduced ? 16 : 4;
+    const uint64_t lb = lane_bytes_of(col.dtype);
+    const uint64_t dec = n * lb, raw = n * lb;
+    const uint64_t v0 = tl[0].v_first;
+    const uint64_t v1 = n > 1 ? v0 + tl[0].step : 0;
+    const uint64_t stride = v1 - v0;
+    uint32_t cst = 1, str = 1;
+    uint64_t xz = 0, fsz = 0, dsz = 0;
+    for (uint32_t t = lane; t < b.ntiles; t += WAVE) {
         const DTile &s = tl[t];
-        cst &= s.const_bad == 0;
-        str &= s.stride_bad == 0;
-        xz += s.xorz_bytes;
-        uint64_t range = sg ? (uint64_t)s.smax - (uint64_t)s.smin : s.umax - s.umin;
-        fsz += 9 + 128ull * qa_width(range);
-        dsz += 9 + 128ull * qa_width(s.dzmax - s.dzmin);
+        cst &= s.allsame && s.v_first == v0;
+        uint64_t k0 = (uint64_t)t * QA_TILE;
+        str &= (s.cnt == 1 || (s.arith && s.step == stride)) && s.v_first == v0 + k0 * stride;
+        xz += s.xorz_rest + qa_varint_len(s.v_first ^ (t ? tl[t - 1].v_last : 0));
+        fsz += 9 + 128ull * qa_width(tile_for_range(s, sg));
+        uint64_t dmn, dmx;
+        tile_dz(s, t == 0, &dmn, &dmx);
+        dsz += 9 + 128ull * qa_width(dmx - dmn);
     }
-    if (cst) b.sz[C_CONST] = 8;
-    if (str) b.sz[C_STRIDE] = 16;
-    if (!reduced) b.sz[C_XORZ] = xz;
-    b.sz[C_FOR] = fsz;
-    b.sz[C_DELTA] = dsz;
-    const uint32_t order[6] = {QA_M_RAW, QA_M_CONST, QA_M_STRIDE, QA_M_XORZ, QA_M_FOR, QA_M_DELTA};
-    for (int k = 1; k < 6; k++) {
-        uint64_t sz = b.sz[k];
-        if (sz == ~0ull) continue;
-        uint64_t sc = sz + penalty_of(order[k], lambda, dec);
-        if (sc < best_score) {
-            best_score = sc;
-            best_len = sz;
-            mode = order[k];
+    cst = wave_reduce(cst, OpAnd());
+    str = wave_reduce(str, OpAnd());
+    xz = wave_reduce(xz, OpAdd());
+    fsz = wave_reduce(fsz, OpAdd()) + qa_varint_len(n);
+    dsz = wave_reduce(dsz, OpAdd()) + 8 + qa_varint_len(n);
+    if (lane == 0) {
+        b.v0 = v0;
+        b.v1 = v1;
+        for (int k = 0; k < C_N; k++) b.sz[k] = ~0ull;
+        b.sz[C_RAW] = raw;
+    }
+    uint32_t mode = QA_M_RAW;
+    uint64_t best_len = raw;
+    if (!(!fl && b.incompressible)) {
+        uint64_t sz[6] = {raw, cst ? 8ull : ~0ull, (n >= 2 && str) ? 16ull : ~0ull,
+                          reduced ? ~0ull : xz, fsz, dsz};
+        const uint32_t order[6] = {QA_M_RAW, QA_M_CONST, QA_M_STRIDE, QA_M_XORZ, QA_M_FOR,
+                                   QA_M_DELTA};
+        uint64_t best_score = raw;
+        for (int k = 1; k < 6; k++) {
+            if (sz[k] == ~0ull) continue;
+            uint64_t sc = sz[k] + penalty_of(order[k], lambda, dec);
+            if (sc < best_score) {
+                best_score = sc;
+                best_len = sz[k];
+                mode = order[k];
+            }
         }
+        if (lane == 0)
+            for (int k = 1; k < 6; k++) b.sz[k] = sz[k];
     }
-    b.compressible = best_len * 20 < (n * 8) * 19;
-    b.mode = mode;
-    atomicAdd(&a.wins[mode & 63], 1ull);
-    b.payload_len = best_len;
-    // sub-block offsets inside the winner payload
+    if (lane == 0) {
+        b.compressible = best_len * 20 < (n * 8) * 19;
+        b.mode = mode;
+        b.payload_len = best_len;
+        atomicAdd(&a.wins[mode & 63], 1ull);
+    }
+    // sub-block widths and offsets inside the winner payload: a wave-wide
+    // exclusive scan of the sub-block sizes
     if (mode == QA_M_FOR || mode == QA_M_DELTA) {
-        uint64_t off = mode == QA_M_FOR ? qa_varint_len(n) : 8 + qa_varint_len(n);
-        for (uint32_t t = 0; t < b.ntiles; t++) {
-            DTile &s = a.tiles[col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr + t];
-            uint64_t range = mode == QA_M_DELTA ? s.dzmax - s.dzmin
-                             : sg ? (uint64_t)s.smax - (uint64_t)s.smin : s.umax - s.umin;
-            s.w = qa_width(range);
-            s.off = off;
-            off += 9 + 128ull * s.w;
+        uint64_t carry = mode == QA_M_FOR ? qa_varint_len(n) : 8 + qa_varint_len(n);
+        for (uint32_t base = 0; base < b.ntiles; base += WAVE) {
+            uint32_t t = base + lane;
+            uint32_t w = 0;
+            if (t < b.ntiles) {
+                const DTile &s = tl[t];
+                if (mode == QA_M_DELTA) {
+                    uint64_t dmn, dmx;
+                    tile_dz(s, t == 0, &dmn, &dmx);
+                    w = qa_width(dmx - dmn);
+                } else {
+                    w = qa_width(tile_for_range(s, sg));
+                }
+            }
+            uint64_t sz = t < b.ntiles ? 9 + 128ull * w : 0, inc = sz;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                uint64_t y = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += y;
+            }
+            if (t < b.ntiles) {
+                tl[t].w = w;
+                tl[t].off = carry + inc - sz;
+            }
+            carry += __shfl(inc, 63, 64);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// stage 7: validity blob + frame layout
+// stage 6: validity blob + frame layout
 // ---------------------------------------------------------------------------
-// transition positions (column-local bit index) into tpos[global transition]
 __global__ void k_vtrans_pos(EncArgs a) {
     uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= a.nvwords) return;
@@ -520,8 +648,8 @@ __global__ void k_vtrans_pos(EncArgs a) {
     }
 }
 
-// run slot s = global transition index + column index; run k of column c ends
-// at transition k (or at n for the last run)
+// run slot s = global transition index + column index; run k of a column ends
+// at its transition k (or at n for the last run)
 __device__ __forceinline__ int col_by_run(const EncArgs &a, uint64_t s) {
     int lo = 0, hi = (int)a.ncols - 1;
     while (lo < hi) {
@@ -550,10 +678,11 @@ __global__ void k_vrun_sizes(EncArgs a) {
 
 __global__ void __launch_bounds__(256) k_col_layout(EncArgs a) {
     // one workgroup per column: validity blob choice, then frame offsets
-    int c = blockIdx.x;
+    const int c = blockIdx.x;
     const DCol &col = a.cols[c];
     DColState &st = a.cst[c];
-    __shared__ uint64_t vlen_s, frames_off_s;
+    __shared__ uint64_t frames_off_s;
+    __shared__ uint64_t buf[256];
     if (threadIdx.x == 0) {
         uint64_t vlen = 0;
         if (st.has_nulls) {
@@ -568,7 +697,6 @@ __global__ void __launch_bounds__(256) k_col_layout(EncArgs a) {
         st.frames_off = frames_off_s;
     }
     __syncthreads();
-    // frame offsets: exclusive scan of (9 + payload) over the column's slots
     uint64_t carry = frames_off_s;
     for (uint32_t base = 0; base < col.nregions; base += blockDim.x) {
         uint32_t lr = base + threadIdx.x;
@@ -577,8 +705,6 @@ __global__ void __launch_bounds__(256) k_col_layout(EncArgs a) {
             const DBlock &b = a.blocks[col.region_base + lr];
             if (b.len) v = 9 + b.payload_len;
         }
-        // simple workgroup scan through LDS
-        __shared__ uint64_t buf[256];
         buf[threadIdx.x] = v;
         __syncthreads();
         for (int d = 1; d < 256; d <<= 1) {
@@ -600,7 +726,7 @@ __global__ void __launch_bounds__(256) k_col_layout(EncArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// stage 8: emission
+// stage 7: emission
 // ---------------------------------------------------------------------------
 __global__ void k_emit_header(EncArgs a) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -623,7 +749,6 @@ __global__ void k_emit_header(EncArgs a) {
     }
 }
 
-// RLE validity blob: one thread per run
 __global__ void k_emit_vruns(EncArgs a) {
     uint64_t total = a.colbase[a.ncols + 1];
     for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total;
@@ -643,7 +768,7 @@ __global__ void k_emit_vruns(EncArgs a) {
     }
 }
 
-// raw validity blob: copy the bitmap bytes verbatim (src/validity.rs:63-68)
+// raw validity blob: the bitmap bytes verbatim (src/validity.rs:63-68)
 __global__ void k_emit_vraw(EncArgs a) {
     uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.nvwords * 8) return;
@@ -686,10 +811,33 @@ __global__ void k_emit_frames(EncArgs a) {
     }
 }
 
-// FastLanes pack of 1024 residuals held in LDS into the stream (bitpack.rs:47-133):
-// output word (k, l) gathers the rows whose bit range meets word k of lane l.
-__device__ __forceinline__ void pack_store(const uint64_t *res, uint32_t w, uint8_t *dst, int lane) {
-    if (w == 0) return;
+// Copy nbytes held in LDS (word-aligned at 0) to an arbitrary global byte
+// address: aligned dword stores inside, byte stores at the two edges only, so
+// neighbouring writers' bytes are never touched. lds must hold one extra word.
+__device__ __forceinline__ void wave_store_bytes(uint8_t *dst, const uint32_t *lds, uint32_t nbytes,
+                                                 int lane) {
+    const uint8_t *sb = (const uint8_t *)lds;
+    uint32_t m = (uint32_t)((uintptr_t)dst & 3);
+    uint32_t head = (4 - m) & 3;
+    if (head > nbytes) head = nbytes;
+    if ((uint32_t)lane < head) dst[lane] = sb[lane];
+    uint32_t rem = nbytes - head, nw = rem >> 2;
+    uint32_t *dw = (uint32_t *)(dst + head);
+    if (head == 0) {
+        for (uint32_t i = lane; i < nw; i += WAVE) dw[i] = lds[i];
+    } else {
+        for (uint32_t i = lane; i < nw; i += WAVE) {
+            uint64_t pair = ((uint64_t)lds[i + 1] << 32) | lds[i];
+            dw[i] = (uint32_t)(pair >> (8 * head));
+        }
+    }
+    uint32_t tb = head + 4 * nw;
+    if (tb + lane < nbytes) dst[tb + lane] = sb[tb + lane];
+}
+
+// FastLanes pack (src/bitpack.rs:47-133) from residuals in LDS into LDS words:
+// output word (k, l) gathers the rows whose bit range meets word k of lane l
+__device__ __forceinline__ void pack_lds(const uint64_t *res, uint32_t w, uint32_t *out, int lane) {
     if (w <= 32) {
         for (uint32_t idx = lane; idx < 32 * w; idx += WAVE) {
             uint32_t k = idx >> 5, l = idx & 31;
@@ -701,7 +849,7 @@ __device__ __forceinline__ void pack_store(const uint64_t *res, uint32_t w, uint
                 int off = (int)(r * w) - (int)lo_bit;
                 word |= off >= 0 ? (off < 32 ? v << off : 0u) : v >> (-off);
             }
-            qa_st32u(dst + 4 * (uint64_t)idx, word);
+            out[idx] = word;
         }
     } else {
         for (uint32_t idx = lane; idx < 16 * w; idx += WAVE) {
@@ -714,50 +862,63 @@ __device__ __forceinline__ void pack_store(const uint64_t *res, uint32_t w, uint
                 int off = (int)(r * w) - (int)lo_bit;
                 word |= off >= 0 ? (off < 64 ? v << off : 0ull) : v >> (-off);
             }
-            qa_st64u(dst + 8 * (uint64_t)idx, word);
+            out[2 * idx] = (uint32_t)word;
+            out[2 * idx + 1] = (uint32_t)(word >> 32);
         }
     }
 }
 
 __global__ void __launch_bounds__(64) k_emit_tiles(EncArgs a) {
     // one wavefront per workgroup: every early exit below is wave-uniform, so
-    // the barrier before packing is reached by the whole group
+    // the barriers are reached by the whole group
     __shared__ uint64_t res[QA_TILE];
+    __shared__ uint32_t outw[2 * QA_TILE + 4];
     const int lane = threadIdx.x;
-    uint64_t t = blockIdx.x;
-    if (t >= a.ntiles) return;
+    const uint64_t t = blockIdx.x;
     const DTile &ts = a.tiles[t];
     if (!ts.active) return;
-    int c;
-    uint32_t slot, bt;
-    if (!tile_locate(a, t, &c, &slot, &bt)) return;
+    const int c = col_by_tile(a.cols, a.ncols, t);
     const DCol &col = a.cols[c];
-    if (a.cst[c].err) return;
-    const DBlock &b = a.blocks[slot];
-    const uint64_t *v = a.lane + col.lane_off + b.start;
-    uint64_t k0 = (uint64_t)bt * QA_TILE;
-    uint32_t cnt = ts.cnt;
+    const DColState &st = a.cst[c];
+    if (st.err) return;
+    const uint64_t lt = t - col.tile_base;
+    const uint32_t lr = (uint32_t)(lt / col.tpr);
+    const uint32_t o = a.owner[col.region_base + lr];
+    const DBlock &b = a.blocks[o];
+    const uint32_t bt = (col.region_base + lr - o) * col.tpr + (uint32_t)(lt % col.tpr);
+    const uint64_t k0 = b.start + (uint64_t)bt * QA_TILE;
+    const uint32_t cnt = ts.cnt;
     uint8_t *p = col.dst + b.frame_off + 9;
     if (b.mode == QA_M_RAW) {
-        uint32_t lb = (uint32_t)lane_bytes_of(col.dtype);
+        const uint32_t lb = (uint32_t)lane_bytes_of(col.dtype);
         for (uint32_t k = lane; k < cnt; k += WAVE) {
-            uint64_t x = v[k0 + k];
-            uint8_t *q = p + (k0 + k) * lb;
-            if (col.dtype == QA_F32) qa_st32u(q, qa_lane_to_f32(x));
-            else if (lb == 4) qa_st32u(q, (uint32_t)x);
-            else qa_st64u(q, x);
+            uint64_t x = lane_at(st, k0 + k);
+            if (col.dtype == QA_F32) outw[k] = qa_lane_to_f32(x);
+            else if (lb == 4) outw[k] = (uint32_t)x;
+            else {
+                outw[2 * k] = (uint32_t)x;
+                outw[2 * k + 1] = (uint32_t)(x >> 32);
+            }
         }
+        __syncthreads();
+        wave_store_bytes(p + (uint64_t)bt * QA_TILE * lb, outw, cnt * lb, lane);
         return;
     }
     if (b.mode != QA_M_FOR && b.mode != QA_M_DELTA) return;
-    bool sg = is_signed(col.dtype);
-    uint64_t mn = b.mode == QA_M_DELTA ? ts.dzmin : sg ? (uint64_t)ts.smin : ts.umin;
+    const bool sg = is_signed(col.dtype);
+    uint64_t mn;
+    if (b.mode == QA_M_DELTA) {
+        uint64_t dmx;
+        tile_dz(ts, bt == 0, &mn, &dmx);
+    } else {
+        mn = sg ? (uint64_t)ts.smin : ts.umin;
+    }
     for (uint32_t k = lane; k < QA_TILE; k += WAVE) {
         uint64_t r = 0;
         if (k < cnt) {
-            uint64_t i = k0 + k, x = v[i];
+            uint64_t x = lane_at(st, k0 + k);
             if (b.mode == QA_M_DELTA) {
-                uint64_t pv = i ? v[i - 1] : b.v0;
+                uint64_t pv = (bt == 0 && k == 0) ? x : lane_at(st, k0 + k - 1);
                 r = qa_zigzag(x - pv) - mn;
             } else {
                 r = x - mn;
@@ -767,15 +928,16 @@ __global__ void __launch_bounds__(64) k_emit_tiles(EncArgs a) {
     }
     __syncthreads();
     uint8_t *q = p + ts.off;
-    if (lane == 0) {
-        q[0] = (uint8_t)ts.w;
-        qa_st64u(q + 1, mn);
-    }
-    pack_store(res, ts.w, q + 9, lane);
+    if (lane == 0) q[0] = (uint8_t)ts.w;
+    if (lane >= 1 && lane < 9) q[lane] = (uint8_t)(mn >> (8 * (lane - 1)));
+    if (ts.w == 0) return;
+    pack_lds(res, ts.w, outw, lane);
+    __syncthreads();
+    wave_store_bytes(q + 9, outw, 128 * ts.w, lane);
 }
 
 // ---------------------------------------------------------------------------
-// host-side launcher
+// launcher
 // ---------------------------------------------------------------------------
 static inline unsigned nb(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 // launch unless the grid is empty (a zero-size grid is a launch error)
@@ -792,12 +954,11 @@ void qa_encode_launch(const EncArgs &a, hipStream_t s, int) {
         qa_exclusive_scan(a.tpre, a.nvwords, a.tpre, a.scan_tmp, s);
     }
     QA_LAUNCH(k_col_counts, dim3(nb(a.ncols, 64)), dim3(64), 0, s, a);
-    QA_LAUNCH(k_to_lane, dim3(nb(a.nvals, 256)), dim3(256), 0, s, a);
-    QA_LAUNCH(k_probe_regions, dim3(a.nregions), dim3(256), 0, s, a);
-    QA_LAUNCH(k_plan, dim3(nb(a.ncols, 64)), dim3(64), 0, s, a);
-    QA_LAUNCH(k_block_feats, dim3(a.nregions), dim3(256), 0, s, a);
-    QA_LAUNCH(k_tile_stats, dim3(nb(a.ntiles, 4)), dim3(256), 0, s, a);
-    QA_LAUNCH(k_block_decide, dim3(nb(a.nregions, 64)), dim3(64), 0, s, a);
+    if (a.nvwords) QA_LAUNCH(k_compact, dim3(nb(a.nvwords, 16)), dim3(256), 0, s, a);
+    QA_LAUNCH(k_region_stats, dim3(a.nregions), dim3(1024), 0, s, a);
+    QA_LAUNCH(k_plan, dim3(a.ncols), dim3(256), 0, s, a);
+    QA_LAUNCH(k_grown_feats, dim3(a.nregions), dim3(256), 0, s, a);
+    QA_LAUNCH(k_block_decide, dim3(nb(a.nregions, 4)), dim3(256), 0, s, a);
     uint64_t nruns_max = a.ntrans_max + a.ncols;
     if (a.nvwords) {
         QA_LAUNCH(k_vtrans_pos, dim3(nb(a.nvwords, 256)), dim3(256), 0, s, a);

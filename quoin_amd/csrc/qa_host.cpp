@@ -61,7 +61,7 @@ struct QuoinGpu {
     DevBuf cols, cst, lane, vpre, tpre, tpos, rsz, colbase, scan_tmp, rfeat, owner, blocks, tiles,
         wins;
     // decode arenas
-    DevBuf dcols, frames, dtiles, dlane, dvpre, derr, dbitmaps, dscan;
+    DevBuf dcols, dstate, frames, dtiles, dlane, dvpre, derr, dbitmaps, dvbpre, dvruns, dscan;
     // host-API staging
     DevBuf st_in, st_valid, st_out, st_vout;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -148,101 +148,73 @@ QuoinGpu *default_gpu() {
     return g_default;
 }
 
-// Validity blob decode on the host from the stream bytes (src/validity.rs:73-119)
-// into a word-padded bitmap; returns false on a malformed blob.
-bool host_validity(const uint8_t *blob, size_t blen, uint64_t n, std::vector<uint64_t> &words) {
-    uint64_t raw_len = (n + 7) / 8;
-    words.assign((size_t)((n + 63) / 64) + 1, 0);
-    if (blen == 0) return false;
-    uint8_t tag = blob[0];
-    const uint8_t *rest = blob + 1;
-    size_t rl = blen - 1;
-    if (tag == 0) {
-        if (rl < raw_len) return false;
-        memcpy(words.data(), rest, raw_len);
-        if (n & 7) ((uint8_t *)words.data())[raw_len - 1] &= (uint8_t)((1u << (n & 7)) - 1);
-        return true;
-    }
-    if (tag != 1) return false;
-    size_t pos = 0;
-    uint64_t i = 0;
-    int cur = 0;
-    while (i < n) {
-        uint64_t run;
-        if (!rd_varint(rest, rl, &pos, &run)) return false;
-        if (run > n - i) return false;
-        if (cur) {
-            uint64_t a = i, b = i + run;
-            while (a < b) {
-                uint64_t w = a >> 6, off = a & 63;
-                uint64_t take = std::min<uint64_t>(64 - off, b - a);
-                uint64_t m = take == 64 ? ~0ull : (((1ull << take) - 1) << off);
-                words[w] |= m;
-                a += take;
-            }
-        }
-        i += run;
-        cur ^= 1;
-    }
-    return true;
-}
-
 struct ParsedStream {
     uint32_t dtype = 0;
-    uint64_t n = 0, nv = 0;
+    uint64_t n = 0;
     bool has_validity = false;
-    std::vector<uint64_t> bitmap;
+    uint64_t vblob_off = 0, vblob_len = 0;
     std::vector<DFrame> frames;
 };
 
-// header + validity + sequential frame scan (src/format.rs:65-94,
-// src/decoder.rs:43-94); returns a QUOIN_ERR code
+// Header and frame scan (src/format.rs:65-94, src/decoder.rs:43-94). Without a
+// bitmap this is the reference's scan: frames are read until n values are
+// counted. With a bitmap the valid count comes out of the blob, which the
+// device expands; the host then reads frame headers until the stream ends or a
+// header stops parsing, and the device keeps the frames the reference would
+// have read (k_dec_colfin), reporting the same failures.
 int parse_stream(const uint8_t *src, size_t len, ParsedStream &ps) {
     if (len < 16) return QUOIN_ERR_CORRUPT;
     if (memcmp(src, "FCR1", 4) != 0 || src[4] != 2) return QUOIN_ERR_CORRUPT;
     if (src[5] & 0x02) return QUOIN_ERR_UNSUPPORTED;  // decimal container
     if (src[5] & ~0x01) return QUOIN_ERR_CORRUPT;
     if (src[6] < 10 || src[6] > 16) return QUOIN_ERR_CORRUPT;
-    if (src[7] > 7) return QUOIN_ERR_CORRUPT;
     if (src[7] > 5) return QUOIN_ERR_CORRUPT;
     ps.dtype = src[7];
     ps.n = rd64(src + 8);
     ps.has_validity = src[5] & 1;
     if (ps.n > (1ull << 40)) return QUOIN_ERR_CORRUPT;
     size_t pos = 16;
-    ps.nv = ps.n;
-    if (ps.has_validity) {
-        uint64_t vlen;
-        if (!rd_varint(src, len, &pos, &vlen)) return QUOIN_ERR_CORRUPT;
-        if (vlen > len - pos) return QUOIN_ERR_CORRUPT;
-        if (!host_validity(src + pos, (size_t)vlen, ps.n, ps.bitmap)) return QUOIN_ERR_CORRUPT;
-        pos += vlen;
-        uint64_t c = 0;
-        for (uint64_t w : ps.bitmap) c += (uint64_t)__builtin_popcountll(w);
-        ps.nv = c;
-    }
     uint64_t counted = 0;
-    bool unsupported = false;
-    while (counted < ps.nv) {
-        if (len - pos < 9) return QUOIN_ERR_CORRUPT;
-        int mode = src[pos];
-        if (!known_mode(mode)) return QUOIN_ERR_CORRUPT;
-        uint64_t fn = rd32(src + pos + 1), pl = rd32(src + pos + 5);
-        if (fn > QA_MAX_Q) return QUOIN_ERR_CORRUPT;
-        pos += 9;
-        if (pl > len - pos) return QUOIN_ERR_CORRUPT;
-        if (!gpu_mode(mode, 0)) unsupported = true;
+    auto push = [&](int mode, uint64_t fn, uint64_t pl) {
         DFrame f{};
-        f.payload_off = pos;
+        f.payload_off = pos + 9;
         f.out_off = counted;
         f.n = (uint32_t)fn;
         f.plen = (uint32_t)pl;
         f.mode = (uint32_t)mode;
         f.ntiles = (uint32_t)((fn + QA_TILE - 1) / QA_TILE);
+        f.gpu_ok = gpu_mode(mode, 0);
         ps.frames.push_back(f);
-        pos += pl;
+        pos += 9 + pl;
         counted += fn;
-        if (counted > ps.nv) return QUOIN_ERR_CORRUPT;
+    };
+    if (ps.has_validity) {
+        uint64_t vlen;
+        if (!rd_varint(src, len, &pos, &vlen)) return QUOIN_ERR_CORRUPT;
+        if (vlen > len - pos) return QUOIN_ERR_CORRUPT;
+        ps.vblob_off = pos;
+        ps.vblob_len = vlen;
+        pos += vlen;
+        while (len - pos >= 9 && counted <= ps.n) {
+            int mode = src[pos];
+            if (!known_mode(mode)) break;
+            uint64_t fn = rd32(src + pos + 1), pl = rd32(src + pos + 5);
+            if (fn > QA_MAX_Q || pl > len - pos - 9) break;
+            push(mode, fn, pl);
+        }
+        return QUOIN_OK;
+    }
+    bool unsupported = false;
+    while (counted < ps.n) {
+        if (len - pos < 9) return QUOIN_ERR_CORRUPT;
+        int mode = src[pos];
+        if (!known_mode(mode)) return QUOIN_ERR_CORRUPT;
+        uint64_t fn = rd32(src + pos + 1), pl = rd32(src + pos + 5);
+        if (fn > QA_MAX_Q) return QUOIN_ERR_CORRUPT;
+        if (pl > len - pos - 9) return QUOIN_ERR_CORRUPT;
+        if (!gpu_mode(mode, 0)) unsupported = true;
+        push(mode, fn, pl);
+        if (counted > ps.n) return QUOIN_ERR_CORRUPT;
     }
     if (unsupported) return QUOIN_ERR_UNSUPPORTED;
     return QUOIN_OK;
@@ -281,6 +253,7 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
                      &g->rsz,   &g->colbase,  &g->scan_tmp, &g->rfeat, &g->owner,   &g->blocks,
                      &g->tiles, &g->wins,     &g->dcols,  &g->frames,  &g->dtiles,  &g->dlane,
                      &g->dvpre, &g->derr,     &g->dbitmaps, &g->dscan, &g->st_in,   &g->st_valid,
+                     &g->dstate, &g->dvbpre, &g->dvruns,
                      &g->st_out, &g->st_vout};
     for (DevBuf *b : all)
         if (b->p) hipFree(b->p);
@@ -428,7 +401,7 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     std::vector<ParsedStream> ps(ns);
     std::vector<DDecCol> cols(ns);
     std::vector<DFrame> frames;
-    uint64_t tile_base = 0, lane_off = 0, vword_base = 0, val_base = 0, bm_words = 0;
+    uint64_t tile_base = 0, lane_off = 0, vword_base = 0, val_base = 0, vbyte_base = 0;
     for (size_t c = 0; c < ns; c++) {
         const QuoinDeviceStream &q = in[c];
         if (!q.src || !q.host_src || (q.n && !q.values_out)) return QUOIN_ERR_NULL;
@@ -444,60 +417,65 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         d.validity_out = q.validity_out;
         d.src_len = q.src_len;
         d.n = q.n;
-        d.nv = ps[c].nv;
         d.dtype = dt;
         d.out_kind = kind;
         d.lane_off = lane_off;
         d.val_base = val_base;
         d.vword_base = vword_base;
+        d.vbyte_base = vbyte_base;
         if (ps[c].has_validity) {
             d.nvwords = (uint32_t)((q.n + 63) / 64);
-            lane_off += (ps[c].nv + QA_TILE - 1) / QA_TILE * QA_TILE;
-            bm_words += ps[c].bitmap.size();
+            d.vblob_off = ps[c].vblob_off;
+            d.vblob_len = ps[c].vblob_len;
+            lane_off += (q.n + QA_TILE - 1) / QA_TILE * QA_TILE;
+            if (d.vblob_len > 1) vbyte_base += d.vblob_len - 1;
         }
         vword_base += d.nvwords;
         val_base += q.n;
+        d.fbegin = (uint32_t)frames.size();
         for (DFrame f : ps[c].frames) {
             f.col = (uint32_t)c;
             f.tile_base = tile_base;
             tile_base += f.ntiles;
             frames.push_back(f);
         }
+        d.fend = (uint32_t)frames.size();
     }
-    if (!ensure(g->dcols, ns * sizeof(DDecCol)) || !ensure(g->frames, frames.size() * sizeof(DFrame)) ||
+    uint64_t nscan = std::max<uint64_t>(vword_base, vbyte_base) + 1;
+    if (!ensure(g->dcols, ns * sizeof(DDecCol)) || !ensure(g->dstate, ns * sizeof(DDecState)) ||
+        !ensure(g->frames, frames.size() * sizeof(DFrame)) ||
         !ensure(g->dtiles, tile_base * sizeof(DTile)) || !ensure(g->dlane, lane_off * 8) ||
         !ensure(g->dvpre, (vword_base + 1) * 8) || !ensure(g->derr, 8) ||
-        !ensure(g->dbitmaps, bm_words * 8) ||
-        !ensure(g->dscan, qa_scan_tmp_words(vword_base + 1) * 8))
+        !ensure(g->dbitmaps, (vword_base + 1) * 8) || !ensure(g->dvbpre, (vbyte_base + 1) * 8) ||
+        !ensure(g->dvruns, (vbyte_base + 1) * 8) ||
+        !ensure(g->dscan, qa_scan_tmp_words(nscan) * 8))
         return QUOIN_ERR_DEVICE;
-    // bitmaps
-    uint64_t bw = 0;
-    for (size_t c = 0; c < ns; c++) {
-        if (!ps[c].has_validity) continue;
-        uint64_t *dst = dp<uint64_t>(g->dbitmaps) + bw;
-        hipMemcpyAsync(dst, ps[c].bitmap.data(), ps[c].bitmap.size() * 8, hipMemcpyHostToDevice, s);
-        cols[c].bitmap = (const uint8_t *)dst;
-        bw += ps[c].bitmap.size();
-    }
+    for (size_t c = 0; c < ns; c++)
+        if (cols[c].nvwords) cols[c].bitmap = dp<uint64_t>(g->dbitmaps) + cols[c].vword_base;
     hipMemcpyAsync(g->dcols.p, cols.data(), ns * sizeof(DDecCol), hipMemcpyHostToDevice, s);
     if (!frames.empty())
         hipMemcpyAsync(g->frames.p, frames.data(), frames.size() * sizeof(DFrame),
                        hipMemcpyHostToDevice, s);
     DecArgs a{};
     a.cols = dp<DDecCol>(g->dcols);
+    a.dst = dp<DDecState>(g->dstate);
     a.ncols = (uint32_t)ns;
     a.frames = dp<DFrame>(g->frames);
     a.nframes = (uint32_t)frames.size();
     a.ntiles = tile_base;
     a.nvals = val_base;
     a.nvwords = vword_base;
+    a.nvbytes = vbyte_base;
     a.lane = dp<uint64_t>(g->dlane);
     a.vpre = dp<uint64_t>(g->dvpre);
+    a.vbpre = dp<uint64_t>(g->dvbpre);
+    a.vruns = dp<uint64_t>(g->dvruns);
     a.tiles = dp<DTile>(g->dtiles);
     a.err = dp<uint32_t>(g->derr);
     a.scan_tmp = dp<uint64_t>(g->dscan);
     hipEventRecord(g->ev0, s);
     zero_async(g->derr.p, 8, s);
+    zero_async(g->dstate.p, ns * sizeof(DDecState), s);
     qa_decode_launch(a, s);
     hipEventRecord(g->ev1, s);
     uint32_t err = 0;

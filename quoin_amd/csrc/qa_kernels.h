@@ -20,7 +20,7 @@ struct EncArgs {
     uint64_t ntiles;       // total tile slots
     uint64_t nvals;        // total logical values
     uint64_t nvwords;      // total validity words
-    uint64_t *lane;
+    uint64_t *lane;        // compacted copies of columns with nulls
     uint64_t *vpre;        // nvwords+1: popcount, then exclusive prefix
     uint64_t *tpre;        // nvwords+1: transition counts, then prefix
     uint64_t *tpos;        // transition positions (<= total bits)
@@ -43,34 +43,50 @@ struct DFrame {
     uint64_t tile_base;    // first global decode tile slot
     uint32_t n, plen;
     uint32_t mode, col;
-    uint32_t ntiles, pad;
+    uint32_t ntiles, gpu_ok;  // gpu_ok: the mode has a GPU decoder
 };
 
 struct DDecCol {
     const uint8_t *src;     // device stream
     void *out;              // typed output
     uint8_t *validity_out;  // device bitmap out or null
-    const uint8_t *bitmap;  // device bitmap decoded from the stream, or null
+    uint64_t *bitmap;       // library-owned decoded bitmap (words), or null
     uint64_t src_len;
     uint64_t n;             // logical values
-    uint64_t nv;            // valid values
     uint64_t lane_off;      // scratch lane offset (when nulls)
     uint64_t vword_base;    // rank prefix slots
     uint64_t val_base;
+    uint64_t vblob_off;     // stream offset of the validity blob (its tag byte)
+    uint64_t vblob_len;     // blob bytes including the tag
+    uint64_t vbyte_base;    // global slot of the blob's first byte after the tag
     uint32_t dtype, nvwords;
-    uint32_t out_kind, pad;   // QA_IN_* narrow output element type
+    uint32_t out_kind;      // QA_IN_* narrow output element type
+    uint32_t fbegin, fend;  // this column's frames
+    uint32_t pad;
+};
+
+// device-computed per-column decode state
+struct DDecState {
+    uint64_t nv;            // valid values
+    uint64_t nruns;         // RLE runs used
+    uint32_t fused_end;     // frames [fbegin, fused_end) are decoded
+    uint32_t err;
 };
 
 struct DecArgs {
     const DDecCol *cols;
+    DDecState *dst;
     uint32_t ncols;
     const DFrame *frames;
     uint32_t nframes;
     uint64_t ntiles;
     uint64_t nvals;
     uint64_t nvwords;
+    uint64_t nvbytes;      // total validity blob bytes (after tags)
     uint64_t *lane;        // compact scratch for columns with nulls
     uint64_t *vpre;        // rank prefix (nvwords + 1)
+    uint64_t *vbpre;       // varint-end prefix over blob bytes (nvbytes + 1)
+    uint64_t *vruns;       // run lengths, then their prefix (nvbytes + 1)
     DTile *tiles;          // per decode tile
     uint32_t *err;         // error flags
     uint64_t *scan_tmp;

@@ -90,3 +90,32 @@ def test_config1_shapes(qa):
             check(qa, name, v, bm)
         else:
             check(qa, name, v)
+
+
+def test_corrupt_streams_report_errors(qa):
+    """Mutated and truncated streams never fault the decoder: they decode or
+    return an error code (tests/robustness.rs:40-52, 77-100)."""
+    r = np.random.default_rng(3)
+    v = _data.int_columns(5000, seed=5)["u32_k11"]
+    vv, bm = _data.with_nulls(v, 0.05, seed=2)
+    seeds = [(qa.compress_column(vv, bm, qa.Config(qa.Level.FASTEST)), qa.DType.U32, len(vv)),
+             (qa.compress_column(_data.int_columns(5000, 1)["ts_i64"], None,
+                                 qa.Config(qa.Level.FASTEST)), qa.DType.I64, 5000)]
+    for s, dt, n in seeds:
+        for it in range(150):
+            b = bytearray(s)
+            k = int(r.integers(0, len(b)))
+            if it % 3 == 0:
+                b = b[:k]
+            else:
+                b[k] ^= 1 << int(r.integers(0, 8))
+            try:
+                qa.decompress_column(bytes(b), dt, n, True)
+            except qa.QuoinError as e:
+                assert e.code in (qa.ERR_CORRUPT, qa.ERR_UNSUPPORTED), e.code
+    for it in range(200):
+        b = bytes(r.integers(0, 256, int(r.integers(0, 80)), dtype=np.uint8))
+        try:
+            qa.decompress_column(b, qa.DType.F64, 10)
+        except qa.QuoinError as e:
+            assert e.code == qa.ERR_CORRUPT
