@@ -89,6 +89,7 @@ struct DBlock {
     uint64_t frame_off;    // frame offset within the column stream
     uint64_t v0, v1;       // first two values (CONST / STRIDE / DELTA base)
     uint64_t sz[C_N];      // candidate payload sizes (UINT64_MAX = bailed)
+    uint64_t score;        // score of the current winner (size + decode penalty)
     uint32_t len;          // values; 0 = no block starts at this slot
     uint32_t col;
     uint32_t ntiles;
@@ -115,6 +116,45 @@ struct DTile {
     uint32_t cnt;
     uint32_t w;
     uint8_t arith, allsame, active, pad;
+};
+
+// Per-tile statistics of the float-value codecs (Float family, Level::Fast).
+struct DTileF {
+    int64_t kmin[7], kmax[7];  // FloatMult k = round(x * s) per scale (float_mult.rs:45-64)
+    int64_t dmin;              // ALP digit minimum (filler included)
+    uint64_t rd_cmin, rd_cmax; // ALP-RD codes range
+    uint64_t rd_rmin, rd_rmax; // ALP-RD rights range
+    uint32_t rd_exc;           // ALP-RD exceptions in the tile
+    uint32_t rd_exc_bytes;     // their varint position bytes + 8 each
+    uint32_t alp_exc;
+    uint8_t fm_pass;           // bit s: every value round-trips at scale s
+    uint8_t alp_e, alp_f, alp_w;
+    uint8_t alp_bail, pad0, pad1, pad2;
+    uint64_t alp_off, rd_exc_off, rd_coff, rd_roff;  // emit offsets
+    uint64_t x_off;            // XORZ byte offset of the tile
+    uint64_t rle_base;         // runs starting before this tile inside its block
+    // chunk slots (chunk c of a block lives in the block's tile slot c):
+    uint64_t ch_min, ch_off;   // RLE length chunk / Dict value chunk: FoR min and offset
+    uint64_t dv_min, dv_off;
+    uint32_t ch_w, dv_w;
+    uint32_t rd_wc, rd_wr;     // ALP-RD codes / rights widths
+    uint32_t dict_w, fm_w;
+    uint64_t dict_min;         // smallest code (rank) in the tile
+};
+
+// Per-block state of the Level::Fast codecs.
+struct DBlockX {
+    uint64_t rd_dict[8];
+    uint64_t rd_nexc, rd_excb, rd_clen, rd_rlen;
+    uint64_t card;            // Dict cardinality (or > max when bailed)
+    uint64_t dict_vlen;       // Dict values blob bytes
+    uint64_t dict_clen;       // Dict codes blob bytes
+    uint64_t nruns, rle_llen; // RLE runs and lengths blob bytes
+    uint32_t rd_left, rd_dl, rd_bail;
+    uint32_t fm_scale;        // first scale passing for every value (7 = none)
+    uint32_t dict_state;      // 0 exact in LDS, 1 bailed, 2 exact in global scratch
+    uint32_t dict_vtag;       // 0 raw values, 1 delta-bitpacked
+    uint32_t rle_bail, pad;
 };
 
 // ---------------------------------------------------------------------------

@@ -10,73 +10,8 @@
 // blocks, the per-block candidate sizes and the ordered Best::consider replay
 // (src/encoder.rs:99-140, 275-469), the validity blob (src/validity.rs:40-70)
 // and the frame layout (src/encoder.rs:184-214, 752-759).
-#include "qa_kernels.h"
+#include "qa_dev.h"
 
-#define WAVE 64
-#define NONE32 0xFFFFFFFFu
-#define RF_DLOW 1u
-#define RF_REP 2u
-#define RF_INC 4u
-
-// ---------------------------------------------------------------------------
-// lookups and lane access
-// ---------------------------------------------------------------------------
-#define QA_COL_SEARCH(field, key)                                   \
-    int lo = 0, hi = nc - 1;                                        \
-    while (lo < hi) {                                               \
-        int mid = (lo + hi + 1) >> 1;                               \
-        if (c[mid].field <= (key)) lo = mid; else hi = mid - 1;     \
-    }                                                               \
-    return lo;
-
-__device__ __forceinline__ int col_by_region(const DCol *c, int nc, uint32_t r) {
-    QA_COL_SEARCH(region_base, r)
-}
-__device__ __forceinline__ int col_by_tile(const DCol *c, int nc, uint64_t t) {
-    QA_COL_SEARCH(tile_base, t)
-}
-// zero-word columns share the next column's base and lose the tie
-__device__ __forceinline__ int col_by_vword(const DCol *c, int nc, uint64_t j) {
-    QA_COL_SEARCH(vword_base, j)
-}
-__device__ __forceinline__ uint64_t lane_bytes_of(uint32_t dt) {
-    return (dt == QA_I32 || dt == QA_U32 || dt == QA_F32) ? 4 : 8;
-}
-__device__ __forceinline__ bool is_float(uint32_t dt) { return dt == QA_F64 || dt == QA_F32; }
-__device__ __forceinline__ bool is_signed(uint32_t dt) { return dt == QA_I32 || dt == QA_I64; }
-__device__ __forceinline__ uint64_t lane_at(const DColState &st, uint64_t i) {
-    return qa_lane_get(st.lanep, st.lkind, i);
-}
-
-// bitmap word j of a column (bits >= n cleared); byte loads: the caller's
-// bitmap is ceil(n/8) bytes with no padding or alignment promise
-__device__ __forceinline__ uint64_t bm_word(const uint8_t *bm, uint64_t n, uint64_t j) {
-    uint64_t nbytes = (n + 7) >> 3, b0 = j * 8;
-    uint64_t w = 0;
-    if (b0 + 8 <= nbytes) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) w |= (uint64_t)bm[b0 + k] << (8 * k);
-    } else {
-        for (int k = 0; k < 8; k++)
-            if (b0 + k < nbytes) w |= (uint64_t)bm[b0 + k] << (8 * k);
-    }
-    uint64_t bits = n - j * 64;
-    if (bits < 64) w &= (1ull << bits) - 1;
-    return w;
-}
-
-template <typename T, typename Op>
-__device__ __forceinline__ T wave_reduce(T v, Op op) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = op(v, (T)__shfl_xor(v, d, 64));
-    return v;
-}
-struct OpMinU { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a < b ? a : b; } };
-struct OpMaxU { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a > b ? a : b; } };
-struct OpMinS { __device__ int64_t operator()(int64_t a, int64_t b) const { return a < b ? a : b; } };
-struct OpMaxS { __device__ int64_t operator()(int64_t a, int64_t b) const { return a > b ? a : b; } };
-struct OpAdd { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a + b; } };
-struct OpAnd { __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a & b; } };
 
 // ---------------------------------------------------------------------------
 // stage 1: validity popcounts / transitions, lane setup, compaction
@@ -503,129 +438,6 @@ __global__ void __launch_bounds__(256) k_grown_feats(EncArgs a) {
 // stage 5: candidate sizes and the ordered replay of Best::consider, one
 // wavefront per block
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t weight_of(uint32_t mode) {
-    switch (mode) {  // src/encoder.rs:79-89
-    case QA_M_RAW: case QA_M_CONST: case QA_M_STRIDE: return 0;
-    case QA_M_XORZ: case QA_M_FOR: case QA_M_DELTA: case QA_M_RLE: return 1;
-    case QA_M_ALP: case QA_M_ALP_RD: case QA_M_DICT: return 2;
-    default: return 6;  // FLOAT_MULT
-    }
-}
-__device__ __forceinline__ uint64_t penalty_of(uint32_t mode, uint64_t lambda, uint64_t dec) {
-    return (lambda * weight_of(mode) * dec) >> 8;  // src/encoder.rs:94-97
-}
-
-__device__ __forceinline__ uint64_t tile_for_range(const DTile &s, bool sg) {
-    return sg ? (uint64_t)s.smax - (uint64_t)s.smin : s.umax - s.umin;
-}
-// zig-zag delta range of a tile inside a block (element 0 of the block has
-// delta 0 against the base v0)
-__device__ __forceinline__ void tile_dz(const DTile &s, bool block_first, uint64_t *mn,
-                                        uint64_t *mx) {
-    uint64_t f = block_first ? 0 : s.dz_first;
-    *mn = s.cnt > 1 ? (s.dzmin < f ? s.dzmin : f) : f;
-    *mx = s.cnt > 1 ? (s.dzmax > f ? s.dzmax : f) : f;
-}
-
-__global__ void __launch_bounds__(256) k_block_decide(EncArgs a) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t slot = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (slot >= a.nregions) return;
-    DBlock &b = a.blocks[slot];
-    if (b.len == 0) return;
-    const DCol &col = a.cols[b.col];
-    DTile *tl = a.tiles + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
-    const uint64_t n = b.len;
-    const bool fl = is_float(col.dtype), sg = is_signed(col.dtype);
-    const bool reduced = col.level == QA_FASTEST;
-    const uint64_t lambda = reduced ? 16 : 4;
-    const uint64_t lb = lane_bytes_of(col.dtype);
-    const uint64_t dec = n * lb, raw = n * lb;
-    const uint64_t v0 = tl[0].v_first;
-    const uint64_t v1 = n > 1 ? v0 + tl[0].step : 0;
-    const uint64_t stride = v1 - v0;
-    uint32_t cst = 1, str = 1;
-    uint64_t xz = 0, fsz = 0, dsz = 0;
-    for (uint32_t t = lane; t < b.ntiles; t += WAVE) {
-        const DTile &s = tl[t];
-        cst &= s.allsame && s.v_first == v0;
-        uint64_t k0 = (uint64_t)t * QA_TILE;
-        str &= (s.cnt == 1 || (s.arith && s.step == stride)) && s.v_first == v0 + k0 * stride;
-        xz += s.xorz_rest + qa_varint_len(s.v_first ^ (t ? tl[t - 1].v_last : 0));
-        fsz += 9 + 128ull * qa_width(tile_for_range(s, sg));
-        uint64_t dmn, dmx;
-        tile_dz(s, t == 0, &dmn, &dmx);
-        dsz += 9 + 128ull * qa_width(dmx - dmn);
-    }
-    cst = wave_reduce(cst, OpAnd());
-    str = wave_reduce(str, OpAnd());
-    xz = wave_reduce(xz, OpAdd());
-    fsz = wave_reduce(fsz, OpAdd()) + qa_varint_len(n);
-    dsz = wave_reduce(dsz, OpAdd()) + 8 + qa_varint_len(n);
-    if (lane == 0) {
-        b.v0 = v0;
-        b.v1 = v1;
-        for (int k = 0; k < C_N; k++) b.sz[k] = ~0ull;
-        b.sz[C_RAW] = raw;
-    }
-    uint32_t mode = QA_M_RAW;
-    uint64_t best_len = raw;
-    if (!(!fl && b.incompressible)) {
-        uint64_t sz[6] = {raw, cst ? 8ull : ~0ull, (n >= 2 && str) ? 16ull : ~0ull,
-                          reduced ? ~0ull : xz, fsz, dsz};
-        const uint32_t order[6] = {QA_M_RAW, QA_M_CONST, QA_M_STRIDE, QA_M_XORZ, QA_M_FOR,
-                                   QA_M_DELTA};
-        uint64_t best_score = raw;
-        for (int k = 1; k < 6; k++) {
-            if (sz[k] == ~0ull) continue;
-            uint64_t sc = sz[k] + penalty_of(order[k], lambda, dec);
-            if (sc < best_score) {
-                best_score = sc;
-                best_len = sz[k];
-                mode = order[k];
-            }
-        }
-        if (lane == 0)
-            for (int k = 1; k < 6; k++) b.sz[k] = sz[k];
-    }
-    if (lane == 0) {
-        b.compressible = best_len * 20 < (n * 8) * 19;
-        b.mode = mode;
-        b.payload_len = best_len;
-        atomicAdd(&a.wins[mode & 63], 1ull);
-    }
-    // sub-block widths and offsets inside the winner payload: a wave-wide
-    // exclusive scan of the sub-block sizes
-    if (mode == QA_M_FOR || mode == QA_M_DELTA) {
-        uint64_t carry = mode == QA_M_FOR ? qa_varint_len(n) : 8 + qa_varint_len(n);
-        for (uint32_t base = 0; base < b.ntiles; base += WAVE) {
-            uint32_t t = base + lane;
-            uint32_t w = 0;
-            if (t < b.ntiles) {
-                const DTile &s = tl[t];
-                if (mode == QA_M_DELTA) {
-                    uint64_t dmn, dmx;
-                    tile_dz(s, t == 0, &dmn, &dmx);
-                    w = qa_width(dmx - dmn);
-                } else {
-                    w = qa_width(tile_for_range(s, sg));
-                }
-            }
-            uint64_t sz = t < b.ntiles ? 9 + 128ull * w : 0, inc = sz;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                uint64_t y = __shfl_up(inc, o, 64);
-                if (lane >= o) inc += y;
-            }
-            if (t < b.ntiles) {
-                tl[t].w = w;
-                tl[t].off = carry + inc - sz;
-            }
-            carry += __shfl(inc, 63, 64);
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // stage 6: validity blob + frame layout
 // ---------------------------------------------------------------------------
@@ -811,63 +623,6 @@ __global__ void k_emit_frames(EncArgs a) {
     }
 }
 
-// Copy nbytes held in LDS (word-aligned at 0) to an arbitrary global byte
-// address: aligned dword stores inside, byte stores at the two edges only, so
-// neighbouring writers' bytes are never touched. lds must hold one extra word.
-__device__ __forceinline__ void wave_store_bytes(uint8_t *dst, const uint32_t *lds, uint32_t nbytes,
-                                                 int lane) {
-    const uint8_t *sb = (const uint8_t *)lds;
-    uint32_t m = (uint32_t)((uintptr_t)dst & 3);
-    uint32_t head = (4 - m) & 3;
-    if (head > nbytes) head = nbytes;
-    if ((uint32_t)lane < head) dst[lane] = sb[lane];
-    uint32_t rem = nbytes - head, nw = rem >> 2;
-    uint32_t *dw = (uint32_t *)(dst + head);
-    if (head == 0) {
-        for (uint32_t i = lane; i < nw; i += WAVE) dw[i] = lds[i];
-    } else {
-        for (uint32_t i = lane; i < nw; i += WAVE) {
-            uint64_t pair = ((uint64_t)lds[i + 1] << 32) | lds[i];
-            dw[i] = (uint32_t)(pair >> (8 * head));
-        }
-    }
-    uint32_t tb = head + 4 * nw;
-    if (tb + lane < nbytes) dst[tb + lane] = sb[tb + lane];
-}
-
-// FastLanes pack (src/bitpack.rs:47-133) from residuals in LDS into LDS words:
-// output word (k, l) gathers the rows whose bit range meets word k of lane l
-__device__ __forceinline__ void pack_lds(const uint64_t *res, uint32_t w, uint32_t *out, int lane) {
-    if (w <= 32) {
-        for (uint32_t idx = lane; idx < 32 * w; idx += WAVE) {
-            uint32_t k = idx >> 5, l = idx & 31;
-            uint32_t lo_bit = 32 * k;
-            uint32_t r0 = lo_bit / w, r1 = min(31u, (lo_bit + 31) / w);
-            uint32_t word = 0;
-            for (uint32_t r = r0; r <= r1; r++) {
-                uint32_t v = (uint32_t)res[r * 32 + l];
-                int off = (int)(r * w) - (int)lo_bit;
-                word |= off >= 0 ? (off < 32 ? v << off : 0u) : v >> (-off);
-            }
-            out[idx] = word;
-        }
-    } else {
-        for (uint32_t idx = lane; idx < 16 * w; idx += WAVE) {
-            uint32_t k = idx >> 4, l = idx & 15;
-            uint32_t lo_bit = 64 * k;
-            uint32_t r0 = lo_bit / w, r1 = min(63u, (lo_bit + 63) / w);
-            uint64_t word = 0;
-            for (uint32_t r = r0; r <= r1; r++) {
-                uint64_t v = res[r * 16 + l];
-                int off = (int)(r * w) - (int)lo_bit;
-                word |= off >= 0 ? (off < 64 ? v << off : 0ull) : v >> (-off);
-            }
-            out[2 * idx] = (uint32_t)word;
-            out[2 * idx + 1] = (uint32_t)(word >> 32);
-        }
-    }
-}
-
 __global__ void __launch_bounds__(64) k_emit_tiles(EncArgs a) {
     // one wavefront per workgroup: every early exit below is wave-uniform, so
     // the barriers are reached by the whole group
@@ -958,7 +713,8 @@ void qa_encode_launch(const EncArgs &a, hipStream_t s, int) {
     QA_LAUNCH(k_region_stats, dim3(a.nregions), dim3(1024), 0, s, a);
     QA_LAUNCH(k_plan, dim3(a.ncols), dim3(256), 0, s, a);
     QA_LAUNCH(k_grown_feats, dim3(a.nregions), dim3(256), 0, s, a);
-    QA_LAUNCH(k_block_decide, dim3(nb(a.nregions, 4)), dim3(256), 0, s, a);
+    qa_fast_sizes_launch(a, s);
+    qa_fast_final_launch(a, s);
     uint64_t nruns_max = a.ntrans_max + a.ncols;
     if (a.nvwords) {
         QA_LAUNCH(k_vtrans_pos, dim3(nb(a.nvwords, 256)), dim3(256), 0, s, a);
@@ -975,4 +731,5 @@ void qa_encode_launch(const EncArgs &a, hipStream_t s, int) {
     }
     QA_LAUNCH(k_emit_frames, dim3(nb(a.nregions, 64)), dim3(64), 0, s, a);
     QA_LAUNCH(k_emit_tiles, dim3((unsigned)a.ntiles), dim3(64), 0, s, a);
+    qa_fast_emit_launch(a, s);
 }

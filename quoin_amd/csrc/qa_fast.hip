@@ -1,0 +1,1127 @@
+// qa_fast.hip — candidate sizing for the Level::Fast pool and the block
+// decisions (src/encoder.rs:275-469 with entropy off).
+//
+//   k_float_tiles   FloatMult per-scale checks (src/codecs/float_mult.rs:45-64)
+//                   and ALP's (e, f) search + digit range per sub-block
+//                   (src/codecs/alp.rs:49-166); one wavefront per tile, FP64
+//                   with no contraction
+//   k_alprd_cut     ALP-RD cut search on the block sample (alp_rd.rs:80-107):
+//                   sorted once in LDS, every cut's histogram is a run scan
+//   k_alprd_hist    full-block left-part histogram and its top 8
+//                   (alp_rd.rs:48-69): LDS hash, exact global-hash fallback
+//   k_alprd_tiles   codes / rights ranges and exceptions per tile
+//   k_block_decide  sizes RAW..ALP-RD and the first part of the ordered replay
+//   k_dict_set      Dict cardinality (dict.rs:87-107): exact LDS hash set, a
+//                   bucket-count lower bound that certifies the bail, exact
+//                   global-hash fallback; sorted dictionary + values blob size
+//   k_dict_tiles    code (rank) range per tile
+//   k_rle_starts / k_rle_chunks   run starts and length chunks (rle.rs:10-43)
+//   k_block_final   Dict / RLE consideration and the winner's layout
+#include "qa_dev.h"
+
+__constant__ double c_exp10[19] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,  1e8, 1e9,
+                                   1e10, 1e11, 1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18};
+__constant__ double c_frac10[19] = {1e0,   1e-1,  1e-2,  1e-3,  1e-4,  1e-5,  1e-6,
+                                    1e-7,  1e-8,  1e-9,  1e-10, 1e-11, 1e-12, 1e-13,
+                                    1e-14, 1e-15, 1e-16, 1e-17, 1e-18};
+__constant__ double c_fm_scales[7] = {10.0, 100.0, 1000.0, 10000.0, 100000.0, 1000000.0,
+                                      10000000.0};
+#define ALP_MAGIC 6755399441055744.0
+#define ALP_UPPER 9.223372036854776e18
+
+__device__ __forceinline__ uint64_t f2b(double d) { return (uint64_t)__double_as_longlong(d); }
+__device__ __forceinline__ double b2f(uint64_t b) { return __longlong_as_double((long long)b); }
+
+// Rust `f64 as i64` for the finite values reaching it: truncate, saturate
+__device__ __forceinline__ int64_t sat_i64(double x) {
+    if (x >= 9223372036854775808.0) return INT64_MAX;
+    if (x < -9223372036854775808.0) return INT64_MIN;
+    return (int64_t)x;
+}
+// alp.rs:33-45
+__device__ __forceinline__ bool alp_digit(uint64_t bits, uint32_t e, uint32_t f, int64_t *d) {
+    double v = b2f(bits);
+    double t1 = __dmul_rn(v, c_exp10[e]);
+    double tmp = __dmul_rn(t1, c_frac10[f]);
+    if (!isfinite(tmp) || tmp >= ALP_UPPER || tmp <= -ALP_UPPER) return false;
+    double r = __dsub_rn(__dadd_rn(tmp, ALP_MAGIC), ALP_MAGIC);
+    int64_t di = sat_i64(r);
+    double back = __dmul_rn(__dmul_rn((double)di, c_exp10[f]), c_frac10[e]);
+    if (f2b(back) != bits) return false;
+    *d = di;
+    return true;
+}
+// float_mult.rs:50-58
+__device__ __forceinline__ bool fm_k(uint64_t bits, int s, int64_t *k) {
+    double x = b2f(bits);
+    double sc = c_fm_scales[s];
+    double kk = round(__dmul_rn(x, sc));
+    if (!isfinite(kk) || fabs(kk) >= 9.0e18) return false;
+    int64_t ki = (int64_t)kk;
+    if (f2b(__ddiv_rn((double)ki, sc)) != bits) return false;
+    *k = ki;
+    return true;
+}
+
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
+
+// locate a tile slot: column, owner block slot, tile index inside the block
+__device__ __forceinline__ bool tile_owner(const EncArgs &a, uint64_t t, int *cp, uint32_t *op,
+                                           uint32_t *btp) {
+    int c = col_by_tile(a.cols, a.ncols, t);
+    const DCol &col = a.cols[c];
+    uint64_t lt = t - col.tile_base;
+    uint32_t lr = (uint32_t)(lt / col.tpr);
+    if (lr >= col.nregions) return false;
+    uint32_t o = a.owner[col.region_base + lr];
+    if (o == NONE32) return false;
+    *cp = c;
+    *op = o;
+    *btp = (col.region_base + lr - o) * col.tpr + (uint32_t)(lt % col.tpr);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// FloatMult + ALP per tile
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_float_tiles(EncArgs a) {
+    __shared__ uint64_t samp[4][32];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t t = (uint64_t)blockIdx.x * 4 + wv;
+    if (t >= a.ntiles) return;
+    const DTile &ts = a.tiles[t];
+    if (!ts.active) return;
+    int c;
+    uint32_t o, bt;
+    if (!tile_owner(a, t, &c, &o, &bt)) return;
+    const DCol &col = a.cols[c];
+    if (!is_float(col.dtype) || col.level == QA_FASTEST) return;
+    const DColState &st = a.cst[c];
+    const DBlock &b = a.blocks[o];
+    const uint64_t k0 = b.start + (uint64_t)bt * QA_TILE;
+    const uint32_t cnt = ts.cnt;
+    DTileF &tf = a.tilesf[t];
+    // FloatMult: every scale, with a wave-wide early exit on the first failure
+    uint8_t pass = 0;
+    for (int s = 0; s < 7; s++) {
+        int64_t kmn = INT64_MAX, kmx = INT64_MIN;
+        bool ok = true;
+        for (uint32_t kb = 0; kb < cnt && ok; kb += WAVE) {
+            uint32_t k = kb + lane;
+            bool bad = false;
+            if (k < cnt) {
+                int64_t kk;
+                if (fm_k(lane_at(st, k0 + k), s, &kk)) {
+                    kmn = kk < kmn ? kk : kmn;
+                    kmx = kk > kmx ? kk : kmx;
+                } else {
+                    bad = true;
+                }
+            }
+            if (wave_any(bad)) ok = false;
+        }
+        if (ok) {
+            pass |= (uint8_t)(1u << s);
+            kmn = wave_reduce(kmn, OpMinS());
+            kmx = wave_reduce(kmx, OpMaxS());
+            if (lane == 0) {
+                tf.kmin[s] = kmn;
+                tf.kmax[s] = kmx;
+            }
+        }
+    }
+    // ALP: (e, f) on a 32-value sample, 190 pairs spread over the lanes
+    const uint32_t stride = cnt / 32 > 1 ? cnt / 32 : 1;
+    uint32_t sn = 0;
+    for (uint32_t i = 0; i < cnt && sn < 32; i += stride) sn++;
+    if (lane < (int)sn) samp[wv][lane] = lane_at(st, k0 + (uint64_t)lane * stride);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t best_key = 0xFFFFFFFFu;
+    for (uint32_t combo = lane; combo < 190; combo += WAVE) {
+        uint32_t e = 0;
+        while ((e + 1) * (e + 2) / 2 <= combo) e++;
+        uint32_t f = combo - e * (e + 1) / 2;
+        uint32_t exc = 0;
+        int64_t lo = INT64_MAX, hi = INT64_MIN;
+        for (uint32_t j = 0; j < sn; j++) {
+            int64_t d;
+            if (alp_digit(samp[wv][j], e, f, &d)) {
+                lo = d < lo ? d : lo;
+                hi = d > hi ? d : hi;
+            } else {
+                exc++;
+            }
+        }
+        uint32_t width;
+        if (exc == sn || hi <= lo)
+            width = exc == sn ? 64 : 0;
+        else
+            width = qa_width((uint64_t)hi - (uint64_t)lo);
+        uint32_t cost = width * sn + exc * 80;
+        uint32_t key = (cost << 16) | (combo << 8) | exc;  // first minimum in (e, f) order
+        best_key = key < best_key ? key : best_key;
+    }
+    best_key = (uint32_t)wave_reduce((uint64_t)best_key, OpMinU());
+    const uint32_t bcombo = (best_key >> 8) & 0xFF, bexc = best_key & 0xFF;
+    uint32_t be = 0;
+    while ((be + 1) * (be + 2) / 2 <= bcombo) be++;
+    const uint32_t bf = bcombo - be * (be + 1) / 2;
+    bool bail = bexc * 2 > sn;
+    uint32_t nexc = 0, w = 0;
+    int64_t dmin = 0;
+    if (!bail) {
+        int64_t lo = INT64_MAX, hi = INT64_MIN;
+        uint32_t first_valid = 0xFFFFFFFFu;
+        int64_t fv_digit = 0;
+        for (uint32_t kb = 0; kb < cnt; kb += WAVE) {
+            uint32_t k = kb + lane;
+            bool valid = false;
+            int64_t d = 0;
+            if (k < cnt) valid = alp_digit(lane_at(st, k0 + k), be, bf, &d);
+            if (k < cnt && !valid) nexc++;
+            if (valid) {
+                lo = d < lo ? d : lo;
+                hi = d > hi ? d : hi;
+            }
+            uint64_t bal = __ballot(valid);
+            if (first_valid == 0xFFFFFFFFu && bal) {
+                int src = __ffsll((long long)bal) - 1;
+                first_valid = kb + src;
+                fv_digit = __shfl(d, src, 64);
+            }
+        }
+        nexc = (uint32_t)wave_reduce((uint64_t)nexc, OpAdd());
+        lo = wave_reduce(lo, OpMinS());
+        hi = wave_reduce(hi, OpMaxS());
+        int64_t filler = first_valid == 0xFFFFFFFFu ? 0 : fv_digit;
+        if (nexc > 0 || cnt < QA_TILE) {  // exceptions and padding take the filler
+            lo = filler < lo ? filler : lo;
+            hi = filler > hi ? filler : hi;
+        }
+        dmin = lo;
+        w = qa_width((uint64_t)hi - (uint64_t)lo);
+        if (w > 32) bail = true;
+    }
+    if (lane == 0) {
+        tf.fm_pass = pass;
+        tf.alp_e = (uint8_t)be;
+        tf.alp_f = (uint8_t)bf;
+        tf.alp_w = (uint8_t)w;
+        tf.alp_exc = nexc;
+        tf.dmin = dmin;
+        tf.alp_bail = bail;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ALP-RD: cut search on the sorted sample
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t rd_code_bits(uint32_t dl) {  // alp_rd.rs:72-78
+    if (dl <= 1) return 1;
+    uint32_t b = 32 - __clz((int)(dl - 1));
+    return b > 1 ? b : 1;
+}
+
+__device__ void lds_bitonic_sort(uint64_t *s, uint32_t n_pow2) {
+    for (uint32_t k = 2; k <= n_pow2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n_pow2; i += blockDim.x) {
+                uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    bool up = (i & k) == 0;
+                    uint64_t x = s[i], y = s[ixj];
+                    if ((x > y) == up) {
+                        s[i] = y;
+                        s[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ bool fast_float_block(const EncArgs &a, const DBlock &b) {
+    const DCol &col = a.cols[b.col];
+    return b.len && is_float(col.dtype) && col.level != QA_FASTEST;
+}
+
+__global__ void __launch_bounds__(1024) k_alprd_cut(EncArgs a) {
+    __shared__ uint64_t s[4096];
+    __shared__ double bits_of[24];
+    const uint32_t slot = blockIdx.x;
+    const DBlock &b = a.blocks[slot];
+    if (!fast_float_block(a, b)) return;
+    DBlockX &x = a.blocksx[slot];
+    const DColState &st = a.cst[b.col];
+    const uint64_t n = b.len;
+    const uint64_t step = n / 2048 > 1 ? n / 2048 : 1;
+    const uint32_t sn = (uint32_t)((n + step - 1) / step);
+    for (uint32_t i = threadIdx.x; i < 4096; i += blockDim.x)
+        s[i] = i < sn ? lane_at(st, b.start + i * step) : ~0ull;
+    __syncthreads();
+    uint32_t np2 = 1;
+    while (np2 < sn) np2 <<= 1;
+    lds_bitonic_sort(s, np2 < 2 ? 2 : np2);
+    if (threadIdx.x < 24) {
+        const uint32_t left = threadIdx.x + 1, rbw = 64 - left;
+        uint32_t top[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t distinct = 0, run = 0;
+        uint64_t prev = 0;
+        for (uint32_t i = 0; i <= sn; i++) {
+            bool end = i == sn;
+            uint64_t key = end ? 0 : s[i] >> rbw;
+            if (i > 0 && (end || key != prev)) {
+                uint32_t v = run;  // insert the finished run into the top-8
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    if (v > top[q]) {
+                        uint32_t tmp = top[q];
+                        top[q] = v;
+                        v = tmp;
+                    }
+                }
+                distinct++;
+                run = 0;
+            }
+            if (!end) {
+                run++;
+                prev = key;
+            }
+        }
+        uint64_t covered = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) covered += top[q];
+        uint32_t dl = distinct < 8 ? distinct : 8;
+        uint64_t exc = sn - covered;
+        uint64_t num = (uint64_t)sn * (rbw + rd_code_bits(dl)) + exc * 96;
+        bits_of[threadIdx.x] = __ddiv_rn((double)num, (double)sn);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t best = 0;
+        double bb = __builtin_inf();
+        for (uint32_t c = 0; c < 24; c++)
+            if (bits_of[c] < bb) {
+                bb = bits_of[c];
+                best = c + 1;
+            }
+        x.rd_bail = (best == 0 || bb >= 63.0) ? 1 : 0;
+        x.rd_left = best;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ALP-RD: full-block histogram of the left parts and its top 8
+// ---------------------------------------------------------------------------
+#define RDH_CAP 4096
+#define RDH_LIMIT 3072
+#define HEMPTY (~0ull)
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {  // SplitMix64 finalizer
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
+// top-8 by (count desc, key asc) over a (keys, counts) table; keys < 2^24
+__device__ void top8_select(uint64_t *keys, uint32_t *cnt, uint32_t cap, uint64_t *out,
+                            uint32_t *outn) {
+    __shared__ unsigned long long best;
+    uint32_t got = 0;
+    for (int r = 0; r < 8; r++) {
+        if (threadIdx.x == 0) best = 0;
+        __syncthreads();
+        unsigned long long my = 0;
+        for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
+            if (keys[i] == HEMPTY || cnt[i] == 0) continue;
+            unsigned long long p = ((unsigned long long)cnt[i] << 24) | (0xFFFFFFull - keys[i]);
+            my = p > my ? p : my;
+        }
+        my = wave_reduce((uint64_t)my, OpMaxU());
+        if ((threadIdx.x & 63) == 0) atomicMax(&best, my);
+        __syncthreads();
+        unsigned long long w = best;
+        if (w == 0) break;
+        uint64_t key = 0xFFFFFFull - (w & 0xFFFFFFull);
+        for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x)
+            if (keys[i] == key) cnt[i] = 0;
+        if (threadIdx.x == 0) out[got] = key;
+        got++;
+        __syncthreads();
+    }
+    *outn = got;
+}
+
+__global__ void __launch_bounds__(1024) k_alprd_hist(EncArgs a) {
+    __shared__ uint64_t keys[RDH_CAP];
+    __shared__ uint32_t cnt[RDH_CAP];
+    __shared__ uint32_t ndist, overflow;
+    const uint32_t slot = blockIdx.x;
+    const DBlock &b = a.blocks[slot];
+    if (!fast_float_block(a, b)) return;
+    DBlockX &x = a.blocksx[slot];
+    if (x.rd_bail) return;
+    const DCol &col = a.cols[b.col];
+    const DColState &st = a.cst[b.col];
+    const uint64_t n = b.len;
+    const uint32_t rbw = 64 - x.rd_left;
+    for (uint32_t i = threadIdx.x; i < RDH_CAP; i += blockDim.x) {
+        keys[i] = HEMPTY;
+        cnt[i] = 0;
+    }
+    if (threadIdx.x == 0) {
+        ndist = 0;
+        overflow = 0;
+    }
+    __syncthreads();
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        if (overflow) break;
+        uint64_t key = lane_at(st, b.start + i) >> rbw;
+        uint32_t h = (uint32_t)(mix64(key) >> 52);
+        for (uint32_t probe = 0; probe < RDH_CAP; probe++) {
+            uint64_t cur = keys[h];
+            if (cur == key) {
+                atomicAdd(&cnt[h], 1u);
+                break;
+            }
+            if (cur == HEMPTY) {
+                if (atomicAdd(&ndist, 1u) >= RDH_LIMIT) {
+                    overflow = 1;
+                    break;
+                }
+                unsigned long long old =
+                    atomicCAS((unsigned long long *)&keys[h], HEMPTY, (unsigned long long)key);
+                if (old == HEMPTY || old == key) {
+                    atomicAdd(&cnt[h], 1u);
+                    if (old == key) atomicSub(&ndist, 1u);
+                    break;
+                }
+                atomicSub(&ndist, 1u);
+            }
+            h = (h + 1) & (RDH_CAP - 1);
+        }
+    }
+    __syncthreads();
+    uint32_t dl = 0;
+    if (!overflow) {
+        top8_select(keys, cnt, RDH_CAP, x.rd_dict, &dl);
+    } else {
+        // exact recount in the block's global table (next_pow2(2n) <= 4n slots)
+        uint32_t cap = 1;
+        while (cap < 2 * n) cap <<= 1;
+        uint64_t *gk = a.hkeys + 4 * (col.lane_off + b.start);
+        uint32_t *gc = a.hcnt + 4 * (col.lane_off + b.start);
+        for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
+            gk[i] = HEMPTY;
+            gc[i] = 0;
+        }
+        __threadfence_block();
+        __syncthreads();
+        for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            uint64_t key = lane_at(st, b.start + i) >> rbw;
+            uint32_t h = (uint32_t)(mix64(key) & (cap - 1));
+            for (;;) {
+                unsigned long long old =
+                    atomicCAS((unsigned long long *)&gk[h], HEMPTY, (unsigned long long)key);
+                if (old == HEMPTY || old == key) {
+                    atomicAdd(&gc[h], 1u);
+                    break;
+                }
+                h = (h + 1) & (cap - 1);
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        top8_select(gk, gc, cap, x.rd_dict, &dl);
+    }
+    if (threadIdx.x == 0) x.rd_dl = dl;
+}
+
+// per tile: codes / rights ranges and the exceptions (alp_rd.rs:145-163)
+__global__ void __launch_bounds__(256) k_alprd_tiles(EncArgs a) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= a.ntiles) return;
+    const DTile &ts = a.tiles[t];
+    if (!ts.active) return;
+    int c;
+    uint32_t o, bt;
+    if (!tile_owner(a, t, &c, &o, &bt)) return;
+    const DBlock &b = a.blocks[o];
+    if (!fast_float_block(a, b)) return;
+    const DBlockX &x = a.blocksx[o];
+    if (x.rd_bail) return;
+    const DColState &st = a.cst[c];
+    const uint32_t rbw = 64 - x.rd_left;
+    const uint64_t mask = (1ull << rbw) - 1;
+    const uint64_t p0 = (uint64_t)bt * QA_TILE;  // block-relative position of the tile
+    uint64_t cmn = ~0ull, cmx = 0, rmn = ~0ull, rmx = 0, eb = 0, ne = 0;
+    for (uint32_t k = lane; k < ts.cnt; k += WAVE) {
+        uint64_t v = lane_at(st, b.start + p0 + k);
+        uint64_t left = v >> rbw, code = 0;
+        bool found = false;
+        for (uint32_t q = 0; q < x.rd_dl; q++)
+            if (!found && x.rd_dict[q] == left) {
+                code = q;
+                found = true;
+            }
+        if (!found) {
+            ne++;
+            eb += qa_varint_len(p0 + k) + 8;
+        }
+        cmn = code < cmn ? code : cmn;
+        cmx = code > cmx ? code : cmx;
+        uint64_t rr = v & mask;
+        rmn = rr < rmn ? rr : rmn;
+        rmx = rr > rmx ? rr : rmx;
+    }
+    cmn = wave_reduce(cmn, OpMinU());
+    cmx = wave_reduce(cmx, OpMaxU());
+    rmn = wave_reduce(rmn, OpMinU());
+    rmx = wave_reduce(rmx, OpMaxU());
+    eb = wave_reduce(eb, OpAdd());
+    ne = wave_reduce(ne, OpAdd());
+    if (lane == 0) {
+        DTileF &tf = a.tilesf[t];
+        tf.rd_cmin = cmn;
+        tf.rd_cmax = cmx;
+        tf.rd_rmin = rmn;
+        tf.rd_rmax = rmx;
+        tf.rd_exc = (uint32_t)ne;
+        tf.rd_exc_bytes = (uint32_t)eb;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// block decision, part 1: RAW .. ALP-RD in the reference's order
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool consider(uint64_t sz, uint32_t m, uint64_t lambda, uint64_t dec,
+                                         uint64_t *best_score, uint64_t *best_len,
+                                         uint32_t *mode) {
+    if (sz == ~0ull) return false;
+    uint64_t sc = sz + penalty_of(m, lambda, dec);
+    if (sc < *best_score) {  // strict: earlier candidates win ties (encoder.rs:126-139)
+        *best_score = sc;
+        *best_len = sz;
+        *mode = m;
+        return true;
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(256) k_block_decide(EncArgs a) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (slot >= a.nregions) return;
+    DBlock &b = a.blocks[slot];
+    if (b.len == 0) return;
+    const DCol &col = a.cols[b.col];
+    DTile *tl = a.tiles + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
+    DTileF *tf = a.tilesf ? a.tilesf + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr
+                          : nullptr;
+    const uint64_t n = b.len;
+    const bool fl = is_float(col.dtype), sg = is_signed(col.dtype);
+    const bool reduced = col.level == QA_FASTEST;
+    const uint64_t lambda = reduced ? 16 : 4;
+    const uint64_t lb = lane_bytes_of(col.dtype);
+    const uint64_t dec = n * lb, raw = n * lb;
+    const uint64_t v0 = tl[0].v_first;
+    const uint64_t v1 = n > 1 ? v0 + tl[0].step : 0;
+    const uint64_t stride = v1 - v0;
+    uint32_t cst = 1, str = 1;
+    uint64_t xz = 0, fsz = 0, dsz = 0, chg = 0;
+    for (uint32_t t = lane; t < b.ntiles; t += WAVE) {
+        const DTile &s = tl[t];
+        cst &= s.allsame && s.v_first == v0;
+        uint64_t k0 = (uint64_t)t * QA_TILE;
+        str &= (s.cnt == 1 || (s.arith && s.step == stride)) && s.v_first == v0 + k0 * stride;
+        xz += s.xorz_rest + qa_varint_len(s.v_first ^ (t ? tl[t - 1].v_last : 0));
+        fsz += 9 + 128ull * qa_width(tile_for_range(s, sg));
+        uint64_t dmn, dmx;
+        tile_dz(s, t == 0, &dmn, &dmx);
+        dsz += 9 + 128ull * qa_width(dmx - dmn);
+        chg += s.chg_rest + (t ? (s.v_first != tl[t - 1].v_last) : 0);
+    }
+    cst = wave_reduce(cst, OpAnd());
+    str = wave_reduce(str, OpAnd());
+    xz = wave_reduce(xz, OpAdd());
+    fsz = wave_reduce(fsz, OpAdd()) + qa_varint_len(n);
+    dsz = wave_reduce(dsz, OpAdd()) + 8 + qa_varint_len(n);
+    chg = wave_reduce(chg, OpAdd());
+    // float-value codecs
+    uint64_t fmsz = ~0ull, alpsz = ~0ull, rdsz = ~0ull;
+    uint32_t fm_scale = 7;
+    if (fl && !reduced && tf) {
+        uint32_t pass = 0x7F, abail = 0;
+        uint64_t asz = 0, nexc = 0, excb = 0, cl = 0, rl = 0;
+        for (uint32_t t = lane; t < b.ntiles; t += WAVE) {
+            const DTileF &f = tf[t];
+            pass &= f.fm_pass;
+            abail |= f.alp_bail;
+            asz += 13 + 128ull * f.alp_w + 10ull * f.alp_exc;
+            nexc += f.rd_exc;
+            excb += f.rd_exc_bytes;
+            cl += 9 + 128ull * qa_width(f.rd_cmax - f.rd_cmin);
+            rl += 9 + 128ull * qa_width(f.rd_rmax - f.rd_rmin);
+        }
+        pass = wave_reduce(pass, OpAnd());
+        abail = (uint32_t)wave_reduce((uint64_t)abail, OpMaxU());
+        asz = wave_reduce(asz, OpAdd());
+        nexc = wave_reduce(nexc, OpAdd());
+        excb = wave_reduce(excb, OpAdd());
+        cl = wave_reduce(cl, OpAdd()) + qa_varint_len(n);
+        rl = wave_reduce(rl, OpAdd()) + qa_varint_len(n);
+        if (pass) {
+            fm_scale = __ffs((int)pass) - 1;
+            uint64_t s = 0;
+            for (uint32_t t = lane; t < b.ntiles; t += WAVE)
+                s += 9 + 128ull * qa_width((uint64_t)tf[t].kmax[fm_scale] - (uint64_t)tf[t].kmin[fm_scale]);
+            fmsz = 2 + qa_varint_len(n) + wave_reduce(s, OpAdd());
+        }
+        if (!abail) alpsz = qa_varint_len(n) + asz;
+        const DBlockX &x = a.blocksx[slot];
+        if (!x.rd_bail && nexc * 4 <= n)
+            rdsz = 2 + 8ull * x.rd_dl + qa_varint_len(nexc) + excb + 1 + qa_varint_len(cl) + cl + 1 +
+                   rl;
+        if (lane == 0) {
+            a.blocksx[slot].fm_scale = fm_scale;
+            a.blocksx[slot].rd_nexc = nexc;
+            a.blocksx[slot].rd_excb = excb;
+            a.blocksx[slot].rd_clen = cl;
+            a.blocksx[slot].rd_rlen = rl;
+        }
+    }
+    uint32_t mode = QA_M_RAW;
+    uint64_t best_len = raw, best_score = raw;
+    uint64_t sz[C_N];
+    for (int k = 0; k < C_N; k++) sz[k] = ~0ull;
+    sz[C_RAW] = raw;
+    if (!(!fl && b.incompressible)) {
+        sz[C_CONST] = cst ? 8 : ~0ull;
+        sz[C_STRIDE] = (n >= 2 && str) ? 16 : ~0ull;
+        sz[C_XORZ] = reduced ? ~0ull : xz;
+        sz[C_FOR] = fsz;
+        sz[C_DELTA] = dsz;
+        sz[C_FM] = fmsz;
+        sz[C_ALP] = alpsz;
+        sz[C_ALPRD] = rdsz;
+        const uint32_t order[9] = {QA_M_RAW, QA_M_CONST, QA_M_STRIDE, QA_M_XORZ, QA_M_FOR,
+                                   QA_M_DELTA, QA_M_FLOAT_MULT, QA_M_ALP, QA_M_ALP_RD};
+        for (int k = 1; k < 9; k++) consider(sz[k], order[k], lambda, dec, &best_score, &best_len, &mode);
+    }
+    if (lane == 0) {
+        b.v0 = v0;
+        b.v1 = v1;
+        for (int k = 0; k < C_N; k++) b.sz[k] = sz[k];
+        b.compressible = best_len * 20 < (n * 8) * 19 && !(!fl && b.incompressible);
+        b.mode = mode;
+        b.payload_len = best_len;
+        b.score = best_score;
+        if (!reduced) {
+            DBlockX &x = a.blocksx[slot];
+            x.rle_bail = chg > n / 2 + 1;
+            x.nruns = chg + 1;
+            x.dict_state = 1;  // set by k_dict_set when it runs
+        }
+    }
+    // RLE: runs starting before each tile (a wave-wide scan)
+    if (!reduced && b.compressible && chg <= n / 2 + 1) {
+        uint64_t carry = 0;
+        for (uint32_t base = 0; base < b.ntiles; base += WAVE) {
+            uint32_t t = base + lane;
+            uint64_t starts = 0;
+            if (t < b.ntiles)
+                starts = tl[t].chg_rest + (t ? (tl[t].v_first != tl[t - 1].v_last) : 1);
+            uint64_t inc = starts;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                uint64_t y = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += y;
+            }
+            if (t < b.ntiles) tf[t].rle_base = carry + inc - starts;
+            carry += __shfl(inc, 63, 64);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dict: cardinality, sorted dictionary, values blob size
+// ---------------------------------------------------------------------------
+#define DH_CAP 8192
+#define DH_LIMIT 6144
+
+__device__ __forceinline__ bool dict_candidate(const EncArgs &a, const DBlock &b) {
+    return b.len && b.compressible && a.cols[b.col].level != QA_FASTEST;
+}
+
+// delta_bitpack size of the sorted dictionary (dict.rs:40-61); chunk c's
+// width and minimum go to the block's tile slot c
+__device__ void dict_values_size(const EncArgs &a, uint32_t slot, const uint64_t *d, uint64_t card,
+                                 DTileF *tf, uint64_t *vlen, uint32_t *vtag) {
+    __shared__ unsigned long long tot;
+    if (threadIdx.x == 0) tot = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    uint64_t nch = (card + QA_TILE - 1) / QA_TILE;
+    for (uint64_t ch = wv; ch < nch; ch += nwv) {
+        uint64_t mn = ~0ull, mx = 0;
+        uint64_t i0 = ch * QA_TILE, i1 = min(card, i0 + QA_TILE);
+        for (uint64_t i = i0 + lane; i < i1; i += WAVE) {
+            uint64_t z = i ? qa_zigzag(d[i] - d[i - 1]) : 0;
+            mn = z < mn ? z : mn;
+            mx = z > mx ? z : mx;
+        }
+        mn = wave_reduce(mn, OpMinU());
+        mx = wave_reduce(mx, OpMaxU());
+        if (lane == 0) {
+            uint32_t w = qa_width(mx - mn);
+            tf[ch].dv_w = w;
+            tf[ch].dv_min = mn;
+            atomicAdd(&tot, 9ull + 128ull * w);
+        }
+    }
+    __syncthreads();
+    uint64_t delta = 8 + qa_varint_len(card) + tot;
+    uint64_t raw = 8 * card;
+    *vtag = delta < raw ? 1 : 0;
+    *vlen = delta < raw ? delta : raw;
+}
+
+__global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
+    __shared__ uint64_t keys[DH_CAP];
+    __shared__ uint32_t card_l, overflow, has_max;
+    const uint32_t slot = blockIdx.x;
+    const DBlock &b = a.blocks[slot];
+    if (!dict_candidate(a, b)) return;
+    DBlockX &x = a.blocksx[slot];
+    const DCol &col = a.cols[b.col];
+    const DColState &st = a.cst[b.col];
+    const uint64_t n = b.len, max_card = n / 2 + 1;
+    DTileF *tf = a.tilesf + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
+    uint64_t *sorted = a.dsorted + col.lane_off + b.start;
+    for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x) keys[i] = HEMPTY;
+    if (threadIdx.x == 0) {
+        card_l = 0;
+        overflow = 0;
+        has_max = 0;
+    }
+    __syncthreads();
+    // phase 1: exact set in LDS (the all-ones value is tracked by a flag)
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        if (overflow) break;
+        uint64_t v = lane_at(st, b.start + i);
+        if (v == HEMPTY) {
+            has_max = 1;
+            continue;
+        }
+        uint32_t h = (uint32_t)(mix64(v) >> 51);
+        for (uint32_t probe = 0; probe < DH_CAP; probe++) {
+            uint64_t cur = keys[h];
+            if (cur == v) break;
+            if (cur == HEMPTY) {
+                unsigned long long old =
+                    atomicCAS((unsigned long long *)&keys[h], HEMPTY, (unsigned long long)v);
+                if (old == HEMPTY) {
+                    uint32_t c = atomicAdd(&card_l, 1u) + 1;
+                    if (c >= DH_LIMIT || c > max_card) overflow = 1;
+                    break;
+                }
+                if (old == v) break;
+            }
+            h = (h + 1) & (DH_CAP - 1);
+        }
+    }
+    __syncthreads();
+    uint64_t card = 0;
+    bool exact = false, bail = false;
+    if (!overflow) {
+        card = card_l + has_max;
+        exact = true;
+        bail = card > max_card;
+    } else if (card_l + has_max > max_card) {
+        bail = true;  // the LDS set already holds more distinct values than allowed
+    } else {
+        // phase 2: a lower bound on the cardinality from 2^19 hash buckets
+        for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x) keys[i] = 0;
+        __syncthreads();
+        for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            uint32_t bk = (uint32_t)(mix64(lane_at(st, b.start + i)) >> 45);
+            atomicOr((unsigned long long *)&keys[bk >> 6], 1ull << (bk & 63));
+        }
+        __syncthreads();
+        uint64_t pc = 0;
+        for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x) pc += __popcll(keys[i]);
+        pc = wave_reduce(pc, OpAdd());
+        __shared__ unsigned long long lb;
+        if (threadIdx.x == 0) lb = 0;
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) atomicAdd(&lb, pc);
+        __syncthreads();
+        if (lb > max_card) {
+            bail = true;
+        } else {
+            // phase 3: exact set in the block's global table
+            uint32_t cap = 1;
+            while (cap < n + 4) cap <<= 1;
+            uint64_t *gk = a.hkeys + 4 * (col.lane_off + b.start);
+            for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) gk[i] = HEMPTY;
+            if (threadIdx.x == 0) {
+                card_l = 0;
+                overflow = 0;
+            }
+            __threadfence_block();
+            __syncthreads();
+            for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+                if (overflow) break;
+                uint64_t v = lane_at(st, b.start + i);
+                if (v == HEMPTY) continue;
+                uint32_t h = (uint32_t)(mix64(v) & (cap - 1));
+                for (;;) {
+                    unsigned long long old =
+                        atomicCAS((unsigned long long *)&gk[h], HEMPTY, (unsigned long long)v);
+                    if (old == HEMPTY) {
+                        if (atomicAdd(&card_l, 1u) + 1 + has_max > max_card) overflow = 1;
+                        break;
+                    }
+                    if (old == v) break;
+                    h = (h + 1) & (cap - 1);
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+            card = card_l + has_max;
+            bail = overflow || card > max_card;
+            if (!bail) {
+                // compact, pad to a power of two with the all-ones value, sort
+                // in global memory (this path only runs for cardinalities
+                // between the LDS limit and n/2 + 1)
+                __shared__ unsigned int wpos;
+                if (threadIdx.x == 0) wpos = 0;
+                __syncthreads();
+                uint64_t *buf = a.hkeys + 4 * (col.lane_off + b.start) + cap;
+                uint32_t np2 = 1;
+                while (np2 < card) np2 <<= 1;
+                for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x)
+                    if (gk[i] != HEMPTY) buf[atomicAdd(&wpos, 1u)] = gk[i];
+                __syncthreads();
+                for (uint32_t i = threadIdx.x + wpos; i < np2; i += blockDim.x) buf[i] = HEMPTY;
+                __threadfence_block();
+                __syncthreads();
+                for (uint32_t k = 2; k <= np2; k <<= 1) {
+                    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                        for (uint32_t i = threadIdx.x; i < np2; i += blockDim.x) {
+                            uint32_t ixj = i ^ j;
+                            if (ixj > i) {
+                                bool up = (i & k) == 0;
+                                uint64_t p = buf[i], q = buf[ixj];
+                                if ((p > q) == up) {
+                                    buf[i] = q;
+                                    buf[ixj] = p;
+                                }
+                            }
+                        }
+                        __threadfence_block();
+                        __syncthreads();
+                    }
+                }
+                for (uint32_t i = threadIdx.x; i < card; i += blockDim.x) sorted[i] = buf[i];
+                __threadfence_block();
+                __syncthreads();
+            }
+        }
+    }
+    if (exact && !bail) {
+        lds_bitonic_sort(keys, DH_CAP);
+        for (uint32_t i = threadIdx.x; i < card; i += blockDim.x) sorted[i] = keys[i];
+        __threadfence_block();
+        __syncthreads();
+    }
+    uint64_t vlen = 0;
+    uint32_t vtag = 0;
+    if (!bail) dict_values_size(a, slot, sorted, card, tf, &vlen, &vtag);
+    if (threadIdx.x == 0) {
+        x.dict_state = bail ? 1 : 0;
+        x.card = card;
+        x.dict_vlen = vlen;
+        x.dict_vtag = vtag;
+    }
+}
+
+__device__ __forceinline__ uint64_t rank_of(const uint64_t *d, uint64_t card, uint64_t v) {
+    uint64_t lo = 0, hi = card;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (d[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// code range per tile: ranks of the tile's smallest and largest value
+__global__ void k_dict_tiles(EncArgs a) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntiles) return;
+    const DTile &ts = a.tiles[t];
+    if (!ts.active) return;
+    int c;
+    uint32_t o, bt;
+    if (!tile_owner(a, t, &c, &o, &bt)) return;
+    const DBlock &b = a.blocks[o];
+    if (!dict_candidate(a, b) || a.blocksx[o].dict_state != 0) return;
+    const DCol &col = a.cols[c];
+    const uint64_t *d = a.dsorted + col.lane_off + b.start;
+    uint64_t card = a.blocksx[o].card;
+    uint64_t r0 = rank_of(d, card, ts.umin), r1 = rank_of(d, card, ts.umax);
+    a.tilesf[t].dict_w = qa_width(r1 - r0);
+    a.tilesf[t].dict_min = r0;
+}
+
+// ---------------------------------------------------------------------------
+// RLE: run starts, then min / max length per chunk of 1024 runs
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool rle_candidate(const EncArgs &a, const DBlock &b, uint32_t o) {
+    return b.len && b.compressible && a.cols[b.col].level != QA_FASTEST && !a.blocksx[o].rle_bail;
+}
+
+__global__ void __launch_bounds__(256) k_rle_starts(EncArgs a) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= a.ntiles) return;
+    const DTile &ts = a.tiles[t];
+    if (!ts.active) return;
+    int c;
+    uint32_t o, bt;
+    if (!tile_owner(a, t, &c, &o, &bt)) return;
+    const DBlock &b = a.blocks[o];
+    if (!rle_candidate(a, b, o)) return;
+    const DCol &col = a.cols[c];
+    const DColState &st = a.cst[c];
+    const uint64_t p0 = (uint64_t)bt * QA_TILE;
+    uint32_t *rs = a.rstart + col.lane_off + b.start;
+    uint64_t base = a.tilesf[t].rle_base;
+    for (uint32_t kb = 0; kb < ts.cnt; kb += WAVE) {
+        uint32_t k = kb + lane;
+        bool start = false;
+        if (k < ts.cnt) {
+            uint64_t p = p0 + k;
+            start = p == 0 || lane_at(st, b.start + p) != lane_at(st, b.start + p - 1);
+        }
+        uint64_t bal = __ballot(start);
+        if (start) rs[base + __popcll(bal & ((1ull << lane) - 1))] = (uint32_t)(p0 + k);
+        base += __popcll(bal);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_rle_chunks(EncArgs a) {
+    __shared__ unsigned long long tot;
+    const uint32_t slot = blockIdx.x;
+    const DBlock &b = a.blocks[slot];
+    if (!rle_candidate(a, b, slot)) return;
+    DBlockX &x = a.blocksx[slot];
+    const DCol &col = a.cols[b.col];
+    const uint32_t *rs = a.rstart + col.lane_off + b.start;
+    DTileF *tf = a.tilesf + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t R = x.nruns;
+    if (threadIdx.x == 0) tot = 0;
+    __syncthreads();
+    for (uint64_t ch = wv; ch * QA_TILE < R; ch += 4) {
+        uint64_t mn = ~0ull, mx = 0;
+        for (uint64_t k = ch * QA_TILE + lane; k < min(R, (ch + 1) * QA_TILE); k += WAVE) {
+            uint64_t len = (k + 1 < R ? rs[k + 1] : b.len) - rs[k];
+            mn = len < mn ? len : mn;
+            mx = len > mx ? len : mx;
+        }
+        mn = wave_reduce(mn, OpMinU());
+        mx = wave_reduce(mx, OpMaxU());
+        if (lane == 0) {
+            uint32_t w = qa_width(mx - mn);
+            tf[ch].ch_w = w;
+            tf[ch].ch_min = mn;
+            atomicAdd(&tot, 9ull + 128ull * w);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) x.rle_llen = qa_varint_len(R) + tot;
+}
+
+// ---------------------------------------------------------------------------
+// block decision, part 2: Dict and RLE, then the winner's layout
+// ---------------------------------------------------------------------------
+template <typename F>
+__device__ __forceinline__ void wave_offsets(uint32_t count, uint64_t carry, int lane, F size_of,
+                                             uint64_t *out_total, DTileF *tf, int which,
+                                             DTile *tl) {
+    for (uint32_t base = 0; base < count; base += WAVE) {
+        uint32_t t = base + lane;
+        uint64_t sz = t < count ? size_of(t) : 0, inc = sz;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint64_t y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        if (t < count) {
+            uint64_t off = carry + inc - sz;
+            switch (which) {
+            case 0: tl[t].off = off; break;
+            case 1: tf[t].x_off = off; break;
+            case 2: tf[t].alp_off = off; break;
+            case 3: tf[t].rd_exc_off = off; break;
+            case 4: tf[t].rd_coff = off; break;
+            case 5: tf[t].rd_roff = off; break;
+            case 6: tf[t].dv_off = off; break;
+            case 7: tf[t].ch_off = off; break;
+            }
+        }
+        carry += __shfl(inc, 63, 64);
+    }
+    *out_total = carry;
+}
+
+__global__ void __launch_bounds__(256) k_block_final(EncArgs a) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (slot >= a.nregions) return;
+    DBlock &b = a.blocks[slot];
+    if (b.len == 0) return;
+    const DCol &col = a.cols[b.col];
+    DTile *tl = a.tiles + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
+    DTileF *tf = a.tilesf ? a.tilesf + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr
+                          : nullptr;
+    const uint64_t n = b.len;
+    const bool sg = is_signed(col.dtype), reduced = col.level == QA_FASTEST;
+    const uint64_t lambda = reduced ? 16 : 4, dec = n * lane_bytes_of(col.dtype);
+    uint32_t mode = b.mode;
+    uint64_t best_len = b.payload_len, best_score = b.score;
+    if (!reduced && b.compressible) {
+        DBlockX &x = a.blocksx[slot];
+        uint64_t dsz = ~0ull, rsz = ~0ull;
+        if (x.dict_state == 0) {
+            uint64_t cl = 0;
+            for (uint32_t t = lane; t < b.ntiles; t += WAVE) cl += 9 + 128ull * tf[t].dict_w;
+            cl = wave_reduce(cl, OpAdd()) + qa_varint_len(n);
+            x.dict_clen = cl;
+            dsz = qa_varint_len(x.card) + 1 + qa_varint_len(x.dict_vlen) + x.dict_vlen + 1 + cl;
+        }
+        if (!x.rle_bail) rsz = qa_varint_len(x.nruns) + 8 * x.nruns + x.rle_llen;
+        consider(dsz, QA_M_DICT, lambda, dec, &best_score, &best_len, &mode);
+        consider(rsz, QA_M_RLE, lambda, dec, &best_score, &best_len, &mode);
+        if (lane == 0) {
+            b.sz[C_DICT] = dsz;
+            b.sz[C_RLE] = rsz;
+        }
+    }
+    if (lane == 0) {
+        b.mode = mode;
+        b.payload_len = best_len;
+        atomicAdd(&a.wins[mode & 63], 1ull);
+    }
+    uint64_t tot;
+    const uint64_t vn = qa_varint_len(n);
+    switch (mode) {
+    case QA_M_FOR:
+    case QA_M_DELTA: {
+        bool dl = mode == QA_M_DELTA;
+        for (uint32_t t = lane; t < b.ntiles; t += WAVE) {
+            uint64_t mn, mx;
+            if (dl) tile_dz(tl[t], t == 0, &mn, &mx);
+            tl[t].w = qa_width(dl ? mx - mn : tile_for_range(tl[t], sg));
+        }
+        wave_offsets(b.ntiles, dl ? 8 + vn : vn, lane,
+                     [&](uint32_t t) { return 9 + 128ull * tl[t].w; }, &tot, tf, 0, tl);
+        break;
+    }
+    case QA_M_XORZ:
+        wave_offsets(b.ntiles, 0, lane, [&](uint32_t t) {
+            return tl[t].xorz_rest + qa_varint_len(tl[t].v_first ^ (t ? tl[t - 1].v_last : 0));
+        }, &tot, tf, 1, tl);
+        break;
+    case QA_M_FLOAT_MULT: {
+        uint32_t s = a.blocksx[slot].fm_scale;
+        for (uint32_t t = lane; t < b.ntiles; t += WAVE)
+            tf[t].fm_w = qa_width((uint64_t)tf[t].kmax[s] - (uint64_t)tf[t].kmin[s]);
+        wave_offsets(b.ntiles, 2 + vn, lane, [&](uint32_t t) { return 9 + 128ull * tf[t].fm_w; },
+                     &tot, tf, 0, tl);
+        break;
+    }
+    case QA_M_ALP:
+        wave_offsets(b.ntiles, vn, lane, [&](uint32_t t) {
+            return 13 + 128ull * tf[t].alp_w + 10ull * tf[t].alp_exc;
+        }, &tot, tf, 2, tl);
+        break;
+    case QA_M_ALP_RD: {
+        const DBlockX &x = a.blocksx[slot];
+        uint64_t e0 = 2 + 8ull * x.rd_dl + qa_varint_len(x.rd_nexc);
+        wave_offsets(b.ntiles, e0, lane, [&](uint32_t t) { return (uint64_t)tf[t].rd_exc_bytes; },
+                     &tot, tf, 3, tl);
+        uint64_t cs = e0 + x.rd_excb + 1 + qa_varint_len(x.rd_clen);
+        for (uint32_t t = lane; t < b.ntiles; t += WAVE) {
+            tf[t].rd_wc = qa_width(tf[t].rd_cmax - tf[t].rd_cmin);
+            tf[t].rd_wr = qa_width(tf[t].rd_rmax - tf[t].rd_rmin);
+        }
+        wave_offsets(b.ntiles, cs + vn, lane, [&](uint32_t t) { return 9 + 128ull * tf[t].rd_wc; },
+                     &tot, tf, 4, tl);
+        uint64_t rs0 = cs + x.rd_clen + 1;
+        wave_offsets(b.ntiles, rs0 + vn, lane, [&](uint32_t t) { return 9 + 128ull * tf[t].rd_wr; },
+                     &tot, tf, 5, tl);
+        break;
+    }
+    case QA_M_DICT: {
+        const DBlockX &x = a.blocksx[slot];
+        uint64_t v0 = qa_varint_len(x.card) + 1 + qa_varint_len(x.dict_vlen);
+        if (x.dict_vtag == 1) {
+            uint32_t nch = (uint32_t)((x.card + QA_TILE - 1) / QA_TILE);
+            wave_offsets(nch, v0 + 8 + qa_varint_len(x.card), lane,
+                         [&](uint32_t c) { return 9 + 128ull * tf[c].dv_w; }, &tot, tf, 6, tl);
+        }
+        uint64_t cs = v0 + x.dict_vlen + 1;
+        wave_offsets(b.ntiles, cs + vn, lane, [&](uint32_t t) { return 9 + 128ull * tf[t].dict_w; },
+                     &tot, tf, 0, tl);
+        break;
+    }
+    case QA_M_RLE: {
+        const DBlockX &x = a.blocksx[slot];
+        uint32_t nch = (uint32_t)((x.nruns + QA_TILE - 1) / QA_TILE);
+        uint64_t ls = qa_varint_len(x.nruns) + 8 * x.nruns + qa_varint_len(x.nruns);
+        wave_offsets(nch, ls, lane, [&](uint32_t c) { return 9 + 128ull * tf[c].ch_w; }, &tot, tf,
+                     7, tl);
+        break;
+    }
+    default: break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+static inline unsigned nbf(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+#define QA_LAUNCH(k, grid, ...)                                   \
+    do {                                                          \
+        dim3 g_ = (grid);                                         \
+        if (g_.x) hipLaunchKernelGGL(k, g_, __VA_ARGS__);         \
+    } while (0)
+
+// before k_block_decide: the float-value codecs' statistics; then the decision
+void qa_fast_sizes_launch(const EncArgs &a, hipStream_t s) {
+    if (a.fast) {
+        QA_LAUNCH(k_float_tiles, dim3(nbf(a.ntiles, 4)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_alprd_cut, dim3(a.nregions), dim3(1024), 0, s, a);
+        QA_LAUNCH(k_alprd_hist, dim3(a.nregions), dim3(1024), 0, s, a);
+        QA_LAUNCH(k_alprd_tiles, dim3(nbf(a.ntiles, 4)), dim3(256), 0, s, a);
+    }
+    QA_LAUNCH(k_block_decide, dim3(nbf(a.nregions, 4)), dim3(256), 0, s, a);
+}
+
+// Dict / RLE sizing, then the final decision and layout
+void qa_fast_final_launch(const EncArgs &a, hipStream_t s) {
+    if (a.fast) {
+        QA_LAUNCH(k_dict_set, dim3(a.nregions), dim3(1024), 0, s, a);
+        QA_LAUNCH(k_dict_tiles, dim3(nbf(a.ntiles, 256)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_rle_starts, dim3(nbf(a.ntiles, 4)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_rle_chunks, dim3(a.nregions), dim3(256), 0, s, a);
+    }
+    QA_LAUNCH(k_block_final, dim3(nbf(a.nregions, 4)), dim3(256), 0, s, a);
+}

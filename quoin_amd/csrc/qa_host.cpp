@@ -59,9 +59,10 @@ struct QuoinGpu {
     std::mutex mu;
     // encode arenas
     DevBuf cols, cst, lane, vpre, tpre, tpos, rsz, colbase, scan_tmp, rfeat, owner, blocks, tiles,
-        wins;
+        wins, tilesf, blocksx, dsorted, rstart, hkeys, hcnt;
     // decode arenas
-    DevBuf dcols, dstate, frames, dtiles, dlane, dvpre, derr, dbitmaps, dvbpre, dvruns, dscan;
+    DevBuf dcols, dstate, frames, dtiles, dlane, dvpre, derr, dbitmaps, dvbpre, dvruns, dscan,
+        dfx, dscr, dscr2;
     // host-API staging
     DevBuf st_in, st_valid, st_out, st_vout;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -134,10 +135,16 @@ bool known_mode(int m) {
     }
 }
 bool gpu_mode(int m, int) {
-    return m == QA_M_RAW || m == QA_M_CONST || m == QA_M_STRIDE || m == QA_M_FOR || m == QA_M_DELTA;
+    switch (m) {
+    case QA_M_RAW: case QA_M_CONST: case QA_M_STRIDE: case QA_M_FOR: case QA_M_DELTA:
+    case QA_M_XORZ: case QA_M_FLOAT_MULT: case QA_M_ALP: case QA_M_ALP_RD: case QA_M_DICT:
+    case QA_M_RLE:
+        return true;
+    default: return false;
+    }
 }
 
-int level_supported(int level) { return level == QUOIN_LEVEL_FASTEST; }
+int level_supported(int level) { return level == QUOIN_LEVEL_FASTEST || level == QUOIN_LEVEL_FAST; }
 
 std::mutex g_default_mu;
 QuoinGpu *g_default = nullptr;
@@ -253,7 +260,8 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
                      &g->rsz,   &g->colbase,  &g->scan_tmp, &g->rfeat, &g->owner,   &g->blocks,
                      &g->tiles, &g->wins,     &g->dcols,  &g->frames,  &g->dtiles,  &g->dlane,
                      &g->dvpre, &g->derr,     &g->dbitmaps, &g->dscan, &g->st_in,   &g->st_valid,
-                     &g->dstate, &g->dvbpre, &g->dvruns,
+                     &g->dstate, &g->dvbpre, &g->dvruns, &g->tilesf, &g->blocksx, &g->dsorted,
+                     &g->rstart, &g->hkeys, &g->hcnt, &g->dfx, &g->dscr, &g->dscr2,
                      &g->st_out, &g->st_vout};
     for (DevBuf *b : all)
         if (b->p) hipFree(b->p);
@@ -357,6 +365,22 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     a.tiles = dp<DTile>(g->tiles);
     a.ntrans_max = ntrans;
     a.wins = dp<unsigned long long>(g->wins);
+    if (level == QUOIN_LEVEL_FAST) {
+        // Level::Fast scratch: per-tile / per-block codec state and the
+        // value-slot arrays of Dict / RLE / the hash fallbacks
+        if (!ensure(g->tilesf, tile_base * sizeof(DTileF)) ||
+            !ensure(g->blocksx, region_base * sizeof(DBlockX)) || !ensure(g->dsorted, lane_off * 8) ||
+            !ensure(g->rstart, lane_off * 4) || !ensure(g->hkeys, lane_off * 32) ||
+            !ensure(g->hcnt, lane_off * 16))
+            return QUOIN_ERR_DEVICE;
+        a.fast = 1;
+        a.tilesf = dp<DTileF>(g->tilesf);
+        a.blocksx = dp<DBlockX>(g->blocksx);
+        a.dsorted = dp<uint64_t>(g->dsorted);
+        a.rstart = dp<uint32_t>(g->rstart);
+        a.hkeys = dp<uint64_t>(g->hkeys);
+        a.hcnt = dp<uint32_t>(g->hcnt);
+    }
     if (hipMemcpyAsync(g->cols.p, cols.data(), ncols * sizeof(DCol), hipMemcpyHostToDevice, s) !=
         hipSuccess)
         return QUOIN_ERR_DEVICE;
@@ -402,6 +426,8 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     std::vector<DDecCol> cols(ns);
     std::vector<DFrame> frames;
     uint64_t tile_base = 0, lane_off = 0, vword_base = 0, val_base = 0, vbyte_base = 0;
+    uint64_t scr_base = 0;
+    uint32_t any_fast = 0;
     for (size_t c = 0; c < ns; c++) {
         const QuoinDeviceStream &q = in[c];
         if (!q.src || !q.host_src || (q.n && !q.values_out)) return QUOIN_ERR_NULL;
@@ -436,7 +462,12 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         for (DFrame f : ps[c].frames) {
             f.col = (uint32_t)c;
             f.tile_base = tile_base;
+            f.scr = scr_base;
             tile_base += f.ntiles;
+            scr_base += f.n;
+            if (f.mode != QA_M_RAW && f.mode != QA_M_CONST && f.mode != QA_M_STRIDE &&
+                f.mode != QA_M_FOR && f.mode != QA_M_DELTA)
+                any_fast = 1;
             frames.push_back(f);
         }
         d.fend = (uint32_t)frames.size();
@@ -448,7 +479,9 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         !ensure(g->dvpre, (vword_base + 1) * 8) || !ensure(g->derr, 8) ||
         !ensure(g->dbitmaps, (vword_base + 1) * 8) || !ensure(g->dvbpre, (vbyte_base + 1) * 8) ||
         !ensure(g->dvruns, (vbyte_base + 1) * 8) ||
-        !ensure(g->dscan, qa_scan_tmp_words(nscan) * 8))
+        !ensure(g->dscan, qa_scan_tmp_words(nscan) * 8) ||
+        (any_fast && (!ensure(g->dfx, frames.size() * sizeof(DFrameX)) ||
+                      !ensure(g->dscr, scr_base * 8) || !ensure(g->dscr2, scr_base * 8))))
         return QUOIN_ERR_DEVICE;
     for (size_t c = 0; c < ns; c++)
         if (cols[c].nvwords) cols[c].bitmap = dp<uint64_t>(g->dbitmaps) + cols[c].vword_base;
@@ -473,6 +506,12 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     a.tiles = dp<DTile>(g->dtiles);
     a.err = dp<uint32_t>(g->derr);
     a.scan_tmp = dp<uint64_t>(g->dscan);
+    if (any_fast) {
+        a.fast = 1;
+        a.fx = dp<DFrameX>(g->dfx);
+        a.scr = dp<uint64_t>(g->dscr);
+        a.scr2 = dp<uint64_t>(g->dscr2);
+    }
     hipEventRecord(g->ev0, s);
     zero_async(g->derr.p, 8, s);
     zero_async(g->dstate.p, ns * sizeof(DDecState), s);

@@ -33,7 +33,20 @@ struct EncArgs {
     DTile *tiles;          // per tile slot
     uint64_t ntrans_max;
     unsigned long long *wins;  // 64 per-mode win counters
+    // Level::Fast codecs
+    uint32_t fast;         // some column runs at Level::Fast
+    uint32_t pad;
+    DTileF *tilesf;        // per tile slot
+    DBlockX *blocksx;      // per region slot
+    uint64_t *dsorted;     // per lane value slot: sorted dictionaries
+    uint32_t *rstart;      // per lane value slot (+1 per block): RLE run starts
+    uint64_t *hkeys;       // 4 per lane value slot: global hash fallback keys
+    uint32_t *hcnt;        // 4 per lane value slot: their counts
 };
+
+void qa_fast_sizes_launch(const EncArgs &a, hipStream_t s);
+void qa_fast_final_launch(const EncArgs &a, hipStream_t s);
+void qa_fast_emit_launch(const EncArgs &a, hipStream_t s);
 
 void qa_encode_launch(const EncArgs &a, hipStream_t s, int level_max);
 
@@ -44,6 +57,20 @@ struct DFrame {
     uint32_t n, plen;
     uint32_t mode, col;
     uint32_t ntiles, gpu_ok;  // gpu_ok: the mode has a GPU decoder
+    uint64_t scr;             // base of this frame's n slots in the decode scratch
+};
+
+// device-parsed frame state of the Level::Fast modes
+struct DFrameX {
+    uint64_t dict[8];      // ALP-RD dictionary
+    uint64_t exc_off;      // ALP-RD: stream offset of the exceptions list
+    uint64_t nexc;         // ALP-RD exceptions
+    uint64_t aux_off;      // Dict: values blob offset; RLE: run values offset
+    uint64_t card;         // Dict cardinality / RLE runs
+    uint32_t left, dl;     // ALP-RD cut and dictionary size
+    uint32_t sorted;       // ALP-RD exception positions strictly increasing
+    uint32_t vtag;         // Dict values tag
+    uint32_t scale, pad;   // FloatMult scale index
 };
 
 struct DDecCol {
@@ -90,6 +117,12 @@ struct DecArgs {
     DTile *tiles;          // per decode tile
     uint32_t *err;         // error flags
     uint64_t *scan_tmp;
+    DFrameX *fx;           // per frame
+    uint64_t *scr, *scr2;  // per frame value slots (Dict values, RLE starts, XORZ, ALP-RD exceptions)
+    uint32_t fast;         // some frame needs the Level::Fast decoders
+    uint32_t pad;
 };
 
 void qa_decode_launch(const DecArgs &a, hipStream_t s);
+void qa_fast_dec_index_launch(const DecArgs &a, hipStream_t s);
+void qa_fast_dec_tiles_launch(const DecArgs &a, hipStream_t s);

@@ -1,0 +1,108 @@
+"""Level::Fast stream parity: the HIP encoder's bytes equal the oracle's for
+every codec of the Fast pool, and the HIP decoder round-trips them."""
+import numpy as np
+import pytest
+
+import _data
+import _oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def qa():
+    import quoin_amd
+    return quoin_amd
+
+
+ODT = {np.dtype(np.float64): orc.F64, np.dtype(np.int64): orc.I64,
+       np.dtype(np.uint64): orc.U64, np.dtype(np.int32): orc.I32,
+       np.dtype(np.uint32): orc.U32, np.dtype(np.float32): orc.F32}
+MODES = {v: k for k, v in orc.MODE.items()}
+
+
+def enc(qa, name, vals, validity=None, block_size=0, decode=True):
+    print(f"case {name} n={len(vals)} dtype={vals.dtype} nulls={validity is not None} "
+          f"bs={block_size}", flush=True)
+    cfg = qa.Config(level=qa.Level.FAST, block_size=block_size)
+    got = qa.compress_column(vals, validity, cfg)
+    want = orc.compress(vals, ODT[vals.dtype], validity, orc.FAST, block_size)
+    if got != want:
+        gf, wf = orc.frames(got) if got[:4] == b"FCR1" else [], orc.frames(want)
+        for i, (g, w) in enumerate(zip(gf, wf)):
+            if g != w:
+                raise AssertionError(
+                    f"{name}: frame {i} gpu mode {MODES.get(g[0])} len {len(g[2])} vs oracle "
+                    f"{MODES.get(w[0])} len {len(w[2])}")
+        raise AssertionError(f"{name}: stream differs ({len(got)} vs {len(want)} B)")
+    if decode:
+        out, _ = qa.decompress_column(got, qa.FROM_NP[vals.dtype], len(vals),
+                                      with_validity=validity is not None)
+        _, ov, _ = orc.decompress(want)
+        assert out.view(np.uint8).tobytes() == ov.view(np.uint8).tobytes(), f"{name}: decode"
+    return got
+
+
+def modes_of(stream):
+    return {MODES.get(m) for m, _, _ in orc.frames(stream)}
+
+
+DECODE = True
+
+
+@pytest.mark.parametrize("n", [1, 15, 1000, 40000, 140000])
+def test_int_shapes_fast(qa, n):
+    for name, v in _data.int_columns(n, seed=n).items():
+        enc(qa, name, v, decode=DECODE)
+
+
+@pytest.mark.parametrize("n", [1, 3000, 40000, 140000])
+def test_float_shapes_fast(qa, n):
+    for name, v in _data.float_columns(n, seed=n).items():
+        enc(qa, name, v, decode=DECODE)
+
+
+def test_codec_coverage_fast(qa):
+    """Columns built so that each Fast codec wins somewhere."""
+    r = np.random.default_rng(7)
+    n = 70000
+    seen = set()
+    cases = {
+        "dict_scattered": r.choice(r.integers(0, 1 << 60, 40).astype(np.uint64), n),
+        "rle_runs": np.repeat(r.integers(0, 1 << 60, n // 50 + 1).astype(np.uint64), 50)[:n],
+        "geo_runs_dec": np.repeat(r.integers(0, 10**6, n).astype(np.float64) / 1000,
+                                  r.geometric(1 / 8, n))[:n],
+        "xorz_small": np.arange(10, dtype=np.int64) * 3,
+        "decimal2": np.round(r.normal(1000, 50, n), 2),
+        "decimal3_wide": r.integers(0, 10**9, n) / 1000.0,
+        "fm_smooth": 1000.0 + np.arange(n) / 1000.0,
+        "alprd_unif": r.integers(0, 2**53, n, dtype=np.uint64) * 2.0**-53,
+        "alprd_coords": (40.0 + r.random(n) * 1e-3),
+        "f32_dec": (r.integers(0, 1000, n) / 10).astype(np.float32),
+        "specials": np.array([0.0, -0.0, np.nan, np.inf, -np.inf, 1e308, 5e-324] * 100),
+        "dict_card_mid": r.integers(0, 3000, n).astype(np.int64) * 1_000_003,
+        "dict_card_hi": r.integers(0, 20000, 131072).astype(np.int64) * 7,
+        "alprd_normal": r.normal(0, 1, n),
+        "alprd_normal_exc": np.concatenate([r.normal(0, 1, n - 50), r.random(50) * 1e300]),
+    }
+    for name, v in cases.items():
+        s = enc(qa, name, v, decode=DECODE)
+        seen |= modes_of(s)
+    print("modes seen:", sorted(x for x in seen if x))
+    for m in ("DICT", "RLE", "ALP", "ALP_RD", "FOR_BITPACK", "DELTA_BITPACK"):
+        assert m in seen, m
+
+
+def test_nulls_fast(qa):
+    r = np.random.default_rng(8)
+    for density in (0.01, 0.3):
+        for name, v in list(_data.float_columns(30000, seed=4).items()) + \
+                list(_data.int_columns(30000, seed=4).items()):
+            vv, bm = _data.with_nulls(v, density, seed=9)
+            enc(qa, f"{name}+nulls{density}", vv, bm, decode=DECODE)
+
+
+def test_fixed_block_sizes_fast(qa):
+    v = _data.float_columns(20000, seed=2)["walk_f64"]
+    for bs in (1, 100, 1000, 1024, 5000, 131072):
+        enc(qa, f"walk bs{bs}", v, block_size=bs, decode=DECODE)
