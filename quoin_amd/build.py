@@ -11,34 +11,39 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libquoin_amd.so")
 SOURCES = ["qa_scan.hip", "qa_encode.hip", "qa_fast.hip", "qa_fast_emit.hip", "qa_decode.hip",
-           "qa_fast_dec.hip", "qa_host.cpp"]
+           "qa_fast_dec.hip", "qa_dec_tiles.hip", "qa_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
-    "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fno-fast-math", "-fno-strict-aliasing", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-value",
     "-Wno-unused-result",
 ]
 
 
+def _compile(src, bdir, verbose):
+    obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
+    lang = ["-x", "hip"] if src.endswith(".cpp") else []
+    cmd = [HIPCC, *FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
+                    if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src}:\n{msg}")
+    if verbose:
+        print(" ".join(cmd), flush=True)
+        if msg:
+            print(msg)
+    return obj
+
+
 def build(verbose=False):
-    objs = []
+    from concurrent.futures import ThreadPoolExecutor
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
-    for src in SOURCES:
-        obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
-        lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmd = [HIPCC, *FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
-                        if "packed-fp32-ops" not in l and l.strip())
-        if r.returncode != 0:
-            raise RuntimeError(f"hipcc failed on {src}:\n{msg}")
-        if msg and verbose:
-            print(msg)
-        objs.append(obj)
+    jobs = min(len(SOURCES), os.cpu_count() or 1, 8)
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(lambda src: _compile(src, bdir, verbose), SOURCES))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

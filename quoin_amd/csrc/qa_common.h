@@ -237,6 +237,13 @@ __device__ __forceinline__ uint64_t qa_f32_to_lane(uint32_t b) {
     }
     return sign | ((uint64_t)(e - 127 + 1023) << 52) | ((uint64_t)m << 29);
 }
+// LDS written by one wavefront and read back by its other lanes
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // lane word i of a column stored as kind k
 __device__ __forceinline__ uint64_t qa_lane_get(const void *p, uint32_t k, uint64_t i) {
     switch (k) {

@@ -21,7 +21,7 @@ __device__ __forceinline__ uint64_t lane_bytes_d(uint32_t dt) {
     return lo;
 
 __device__ __forceinline__ int frame_by_tile(const DecArgs &a, uint64_t t) {
-    QA_SEARCH(a.nframes, a.frames[mid].tile_base, t)
+    return (int)a.tiles[t].frame;
 }
 __device__ __forceinline__ int dcol_by_vword(const DecArgs &a, uint64_t j) {
     QA_SEARCH(a.ncols, a.cols[mid].vword_base, j)
@@ -29,8 +29,8 @@ __device__ __forceinline__ int dcol_by_vword(const DecArgs &a, uint64_t j) {
 __device__ __forceinline__ int dcol_by_vbyte(const DecArgs &a, uint64_t g) {
     QA_SEARCH(a.ncols, a.cols[mid].vbyte_base, g)
 }
-__device__ __forceinline__ int dcol_by_val(const DecArgs &a, uint64_t g) {
-    QA_SEARCH(a.ncols, a.cols[mid].val_base, g)
+__device__ __forceinline__ int dcol_by_avbyte(const DecArgs &a, uint64_t g) {
+    QA_SEARCH(a.ncols, a.cols[mid].avbyte_base, g)
 }
 __device__ __forceinline__ bool frame_used(const DecArgs &a, uint32_t f) {
     const DFrame &fr = a.frames[f];
@@ -63,17 +63,24 @@ __device__ __forceinline__ void stage_words(const uint8_t *src, uint32_t nbytes,
                                             int lane) {
     uintptr_t ad = (uintptr_t)src;
     const uint32_t *al = (const uint32_t *)(ad & ~(uintptr_t)3);
-    uint32_t m = (uint32_t)(ad & 3);
-    uint32_t nwords = (nbytes + 3) / 4;
+    const uint32_t m = (uint32_t)(ad & 3);
+    const uint32_t nwords = (nbytes + 3) / 4;
     // aligned words that hold bytes [src, src + nbytes)
-    uint32_t nal = (m + nbytes + 3) / 4;
-    if (m == 0) {
-        for (uint32_t i = lane; i < nwords; i += WAVE) lds[i] = al[i];
-    } else {
-        for (uint32_t i = lane; i < nwords; i += WAVE) {
-            uint32_t lo = al[i];
-            uint32_t hi = i + 1 < nal ? al[i + 1] : 0;
-            lds[i] = (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * m));
+    const uint32_t nal = (m + nbytes + 3) / 4;
+    // eight loads in flight per lane before the first LDS store
+    for (uint32_t i0 = 0; i0 < nwords; i0 += 8 * WAVE) {
+        uint32_t lo[8], hi[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t i = i0 + u * WAVE + lane;
+            lo[u] = i < nwords ? al[i] : 0;
+            hi[u] = (m && i + 1 < nal) ? al[i + 1] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t i = i0 + u * WAVE + lane;
+            if (i < nwords)
+                lds[i] = m ? (uint32_t)((((uint64_t)hi[u] << 32) | lo[u]) >> (8 * m)) : lo[u];
         }
     }
 }
@@ -100,12 +107,6 @@ __device__ __forceinline__ uint64_t unpack_lds(const uint32_t *wd, uint32_t w, u
     }
     return v & mask;
 }
-// LDS written by one wavefront and read back by its other lanes
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __device__ __forceinline__ void store_typed(const DDecCol &col, uint64_t idx, uint64_t x) {
     switch (col.out_kind) {
@@ -123,13 +124,116 @@ __device__ __forceinline__ void store_typed(const DDecCol &col, uint64_t idx, ui
     }
 }
 
-__device__ __forceinline__ void put_value(const DDecCol &col, bool nulls, uint64_t *lane_scratch,
-                                          uint64_t idx, uint64_t x) {
-    if (nulls) {  // compact scratch, scattered to logical positions later
-        lane_scratch[col.lane_off + idx] = x;
-        return;
+// position of the r-th set bit (0-based) of w; r < popcount(w)
+__device__ __forceinline__ uint32_t select64(uint64_t w, uint32_t r) {
+    uint32_t pos = 0, c = (uint32_t)__popc((uint32_t)w);
+    if (r >= c) { r -= c; w >>= 32; pos += 32; }
+    c = (uint32_t)__popc((uint32_t)w & 0xFFFFu);
+    if (r >= c) { r -= c; w >>= 16; pos += 16; }
+    c = (uint32_t)__popc((uint32_t)w & 0xFFu);
+    if (r >= c) { r -= c; w >>= 8; pos += 8; }
+    for (int k = 0; k < 8; k++, pos++, w >>= 1)
+        if (w & 1) {
+            if (r == 0) return pos;
+            r--;
+        }
+    return pos;
+}
+
+// Output element of the column (its narrow Arrow type, src/lib.rs:492-506).
+enum { QA_OUT_U8 = 0, QA_OUT_U16, QA_OUT_U32, QA_OUT_F32, QA_OUT_U64 };
+__device__ __forceinline__ int out_form(const DDecCol &col) {
+    switch (col.out_kind) {
+    case QA_IN_I8:
+    case QA_IN_U8: return QA_OUT_U8;
+    case QA_IN_I16:
+    case QA_IN_U16: return QA_OUT_U16;
+    default: break;
     }
-    store_typed(col, idx, x);
+    if (col.dtype == QA_F32) return QA_OUT_F32;
+    if (col.dtype == QA_I32 || col.dtype == QA_U32) return QA_OUT_U32;
+    return QA_OUT_U64;
+}
+template <int F>
+__device__ __forceinline__ void st_out(void *out, uint64_t i, uint64_t x) {
+    if (F == QA_OUT_U8) ((uint8_t *)out)[i] = (uint8_t)x;
+    else if (F == QA_OUT_U16) ((uint16_t *)out)[i] = (uint16_t)x;
+    else if (F == QA_OUT_U32) ((uint32_t *)out)[i] = (uint32_t)x;
+    else if (F == QA_OUT_F32) ((uint32_t *)out)[i] = qa_lane_to_f32(x);
+    else ((uint64_t *)out)[i] = x;
+}
+
+// Tile values x[q] (value k = lane + 64 q) of a stream without nulls go to
+// out[base + k].
+template <int F>
+__device__ __forceinline__ void emit_dense_f(void *out, uint64_t base, uint32_t cnt,
+                                             const uint64_t (&x)[QA_TILE / WAVE], int lane) {
+#pragma unroll
+    for (int q = 0; q < QA_TILE / WAVE; q++) {
+        uint32_t k = lane + WAVE * q;
+        if (k < cnt) st_out<F>(out, base + k, x[q]);
+    }
+}
+__device__ __forceinline__ void emit_dense(const DDecCol &col, uint64_t base, uint32_t cnt,
+                                           const uint64_t (&x)[QA_TILE / WAVE], int lane) {
+    switch (out_form(col)) {
+    case QA_OUT_U8: emit_dense_f<QA_OUT_U8>(col.out, base, cnt, x, lane); break;
+    case QA_OUT_U16: emit_dense_f<QA_OUT_U16>(col.out, base, cnt, x, lane); break;
+    case QA_OUT_U32: emit_dense_f<QA_OUT_U32>(col.out, base, cnt, x, lane); break;
+    case QA_OUT_F32: emit_dense_f<QA_OUT_F32>(col.out, base, cnt, x, lane); break;
+    default: emit_dense_f<QA_OUT_U64>(col.out, base, cnt, x, lane); break;
+    }
+}
+
+// Streams with nulls (src/validity.rs:130-146): the tile's cnt compact values
+// vv[0..cnt) fill the valid slots of logical positions [L0, L1), null slots get
+// zero. Lanes load up to 64 bitmap words of the range at once and scan their
+// popcounts; the wavefront then stores one word's 64 positions per step.
+template <int F>
+__device__ __forceinline__ void emit_gather_f(void *out, const uint64_t *bm, uint64_t L0,
+                                              uint64_t L1, const uint64_t *vv, uint32_t cnt,
+                                              int lane) {
+    if (L1 <= L0) return;
+    const uint64_t jb = L0 >> 6, je = (L1 - 1) >> 6;  // inclusive word range
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint64_t r = 0;  // valid values before the chunk
+    for (uint64_t c0 = jb; c0 <= je; c0 += WAVE) {
+        const uint64_t j = c0 + lane;
+        uint64_t m = 0;
+        if (j <= je) {
+            m = bm[j];
+            const uint64_t p = j << 6;
+            if (L0 > p) m &= ~0ull << (L0 - p);
+            if (L1 < p + 64) m &= (1ull << (L1 - p)) - 1;
+        }
+        const uint64_t pc = (uint64_t)__popcll(m);
+        uint64_t inc = pc;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint64_t y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        const uint64_t ex = inc - pc;
+        const uint32_t nw = (uint32_t)(je - c0 + 1 < (uint64_t)WAVE ? je - c0 + 1 : (uint64_t)WAVE);
+        for (uint32_t i = 0; i < nw; i++) {
+            const uint64_t mi = __shfl(m, (int)i, 64);
+            const uint64_t idx = r + __shfl(ex, (int)i, 64) + (uint64_t)__popcll(mi & below);
+            const uint64_t pos = ((c0 + i) << 6) + lane;
+            const uint64_t x = ((mi >> lane) & 1) && idx < cnt ? vv[idx] : 0;
+            if (pos >= L0 && pos < L1) st_out<F>(out, pos, x);
+        }
+        r += __shfl(inc, 63, 64);
+    }
+}
+__device__ __forceinline__ void emit_gather(const DDecCol &col, uint64_t L0, uint64_t L1,
+                                            const uint64_t *vv, uint32_t cnt, int lane) {
+    switch (out_form(col)) {
+    case QA_OUT_U8: emit_gather_f<QA_OUT_U8>(col.out, col.bitmap, L0, L1, vv, cnt, lane); break;
+    case QA_OUT_U16: emit_gather_f<QA_OUT_U16>(col.out, col.bitmap, L0, L1, vv, cnt, lane); break;
+    case QA_OUT_U32: emit_gather_f<QA_OUT_U32>(col.out, col.bitmap, L0, L1, vv, cnt, lane); break;
+    case QA_OUT_F32: emit_gather_f<QA_OUT_F32>(col.out, col.bitmap, L0, L1, vv, cnt, lane); break;
+    default: emit_gather_f<QA_OUT_U64>(col.out, col.bitmap, L0, L1, vv, cnt, lane); break;
+    }
 }
 
 __device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, int lane) {
@@ -143,3 +247,83 @@ __device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, int lane) {
 
 
 __device__ __forceinline__ bool wave_any_d(bool p) { return __ballot(p) != 0; }
+
+// ---------------------------------------------------------------------------
+// Sub-block walks by one wavefront. Each sub-block header gives the position
+// of the next one, so the format's walk is sequential. Every round the lanes
+// test whether the next 64 sub-blocks repeat the current header's stride key
+// (the width, and for ALP the exception count): the first lane whose header
+// differs ends the confirmed run, and the walk resumes there. A run of equal
+// widths (the usual case) costs one round trip per 64 sub-blocks. All lanes
+// return the same value.
+// ---------------------------------------------------------------------------
+
+// for_bitpack payload [pos, end) that must hold n values (for_bitpack.rs:108-157).
+// Records sub-block offsets (which = 0: .off and .w, which = 1: .tail). Returns
+// the position after the walk or ~0 on a malformed payload.
+static __device__ uint64_t for_walk_wave(const uint8_t *s, uint64_t pos, uint64_t end, uint64_t n,
+                                         DTileD *tl, int which, int lane) {
+    uint64_t m;
+    if (!rd_varint(s, end, &pos, &m) || m != n) return ~0ull;
+    const uint64_t nsub = (n + QA_TILE - 1) / QA_TILE;
+    for (uint64_t t = 0; t < nsub;) {
+        if (pos + 9 > end) return ~0ull;
+        const uint32_t w0 = s[pos];
+        if (w0 > 64) return ~0ull;
+        const uint64_t stride = 9 + 128ull * w0;
+        if (stride > end - pos) return ~0ull;
+        const uint64_t nin = nsub - t < (uint64_t)WAVE ? nsub - t : (uint64_t)WAVE;
+        const uint64_t pj = pos + (uint64_t)lane * stride;
+        bool ok = false;
+        if ((uint64_t)lane < nin && pj + stride <= end) ok = s[pj] == w0;
+        const uint64_t inmask = nin >= 64 ? ~0ull : ((1ull << nin) - 1);
+        const uint64_t fail = __ballot(!ok) & inmask;
+        const uint64_t cnf = fail ? (uint64_t)__builtin_ctzll(fail) : nin;  // >= 1: lane 0 passes
+        if ((uint64_t)lane < cnf) {
+            if (which == 0) {
+                tl[t + lane].off = pj;
+                tl[t + lane].w = w0;
+            } else {
+                tl[t + lane].tail = pj;
+            }
+        }
+        t += cnf;
+        pos += cnf * stride;
+    }
+    return pos;
+}
+
+// ALP sub-blocks [e][f][exc u16][min i64][w u8][128 w B][exc x 10 B] of a
+// payload holding ntiles of them (alp.rs:168-248). Returns the position after
+// the walk or ~0.
+__device__ __forceinline__ bool alp_hdr_ok(const uint8_t *h) {
+    uint32_t e = h[0], f = h[1], w = h[12];
+    return e <= 18 && f <= e && w <= 32;
+}
+static __device__ uint64_t alp_walk_wave(const uint8_t *s, uint64_t pos, uint64_t end,
+                                         uint64_t ntiles, DTileD *tl, int lane) {
+    for (uint64_t t = 0; t < ntiles;) {
+        if (pos + 13 > end || !alp_hdr_ok(s + pos)) return ~0ull;
+        const uint32_t w0 = s[pos + 12];
+        const uint32_t x0 = s[pos + 2] | ((uint32_t)s[pos + 3] << 8);
+        const uint64_t stride = 13 + 128ull * w0 + 10ull * x0;
+        if (stride > end - pos) return ~0ull;
+        const uint64_t nin = ntiles - t < (uint64_t)WAVE ? ntiles - t : (uint64_t)WAVE;
+        const uint64_t pj = pos + (uint64_t)lane * stride;
+        bool ok = false;
+        if ((uint64_t)lane < nin && pj + stride <= end) {
+            const uint8_t *h = s + pj;
+            ok = h[12] == w0 && (h[2] | ((uint32_t)h[3] << 8)) == x0 && alp_hdr_ok(h);
+        }
+        const uint64_t inmask = nin >= 64 ? ~0ull : ((1ull << nin) - 1);
+        const uint64_t fail = __ballot(!ok) & inmask;
+        const uint64_t cnf = fail ? (uint64_t)__builtin_ctzll(fail) : nin;
+        if ((uint64_t)lane < cnf) {
+            tl[t + lane].off = pj;
+            tl[t + lane].w = w0;
+        }
+        t += cnf;
+        pos += cnf * stride;
+    }
+    return pos;
+}

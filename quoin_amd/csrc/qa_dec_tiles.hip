@@ -1,0 +1,318 @@
+// qa_dec_tiles.hip — per-tile decoders of every GPU mode, one wavefront per
+// 1024-value tile, writing straight into the typed output.
+//
+// Each lane holds the tile's values k = lane + 64 q (q < 16) in registers:
+//   RAW        (raw.rs:30-49)             staged through LDS, f32 widened
+//   CONST      (const_block.rs:13-20)
+//   STRIDE     (stride.rs:22-35)          v0 + i * step, wrapping
+//   FOR        (for_bitpack.rs:108-157)   FastLanes unpack from LDS + min
+//   DELTA      (delta_bitpack.rs:48-64)   unpack, unzigzag, wavefront scan
+//                                         seeded with the tile carry
+//   FLOAT_MULT (float_mult.rs:100-133)    k / scale in IEEE f64
+//   ALP        (alp.rs:168-248)           (d * 10^f) * 10^-e in IEEE f64,
+//                                         exceptions patched in list order
+//   ALP_RD     (alp_rd.rs:208-285)        (dict[code] << right_bw) | right,
+//                                         exceptions patched in list order
+//   DICT       (dict.rs:154-207)          code -> sorted dictionary value
+//   RLE        (rle.rs:45-75)             run search over the run starts
+//   XORZ       (xorz.rs:18-31)            values prefix-XORed per frame by
+//                                         k_dec_frame_wg
+// Streams without nulls store the values at their positions. Streams with
+// nulls put them in LDS and the wavefront walks the bitmap words of the tile's
+// logical range (src/validity.rs:130-146).
+#include "qa_dec.h"
+
+__constant__ double d_exp10[19] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,  1e8, 1e9,
+                                   1e10, 1e11, 1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18};
+__constant__ double d_frac10[19] = {1e0,   1e-1,  1e-2,  1e-3,  1e-4,  1e-5,  1e-6,
+                                    1e-7,  1e-8,  1e-9,  1e-10, 1e-11, 1e-12, 1e-13,
+                                    1e-14, 1e-15, 1e-16, 1e-17, 1e-18};
+__constant__ double d_fm_scales[7] = {10.0, 100.0, 1000.0, 10000.0, 100000.0, 1000000.0,
+                                      10000000.0};
+
+#define QV (QA_TILE / WAVE)
+
+// FoR sub-block value k at stream sub-block header h (direct global reads;
+// sub-block offsets are validated by the index walk)
+__device__ __forceinline__ uint64_t for_value_direct(const uint8_t *h, uint32_t k) {
+    uint32_t w = h[0];
+    uint64_t mn = qa_ld64u(h + 1);
+    if (w == 0) return mn;
+    if (w <= 32) {
+        uint32_t r = k >> 5, l = k & 31, bit = r * w, word = bit >> 5, sh = bit & 31;
+        uint32_t v = qa_ld32u(h + 9 + 4ull * (word * 32 + l)) >> sh;
+        if (sh + w > 32) v |= qa_ld32u(h + 9 + 4ull * ((word + 1) * 32 + l)) << (32 - sh);
+        return mn + (v & (w >= 32 ? 0xFFFFFFFFu : ((1u << w) - 1)));
+    }
+    uint32_t r = k >> 4, l = k & 15, bit = r * w, word = bit >> 6, sh = bit & 63;
+    uint64_t v = qa_ld64u(h + 9 + 8ull * (word * 16 + l)) >> sh;
+    if (sh + w > 64) v |= qa_ld64u(h + 9 + 8ull * ((word + 1) * 16 + l)) << (64 - sh);
+    return mn + (v & (w >= 64 ? ~0ull : ((1ull << w) - 1)));
+}
+
+// Patch value v at tile index k (uniform across the wavefront) into the lane
+// that holds it. Static register indexing keeps x[] in registers.
+__device__ __forceinline__ void patch_reg(uint64_t (&x)[QV], uint32_t k, uint64_t v, int lane) {
+    const uint32_t q = k >> 6;
+    const bool hit = (int)(k & 63) == lane;
+#pragma unroll
+    for (int i = 0; i < QV; i++) {
+        const bool sel = hit & ((uint32_t)i == q);
+        x[i] = sel ? v : x[i];  // a select, not an indexed store
+    }
+}
+
+__global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
+    __shared__ uint64_t lds[4][QA_TILE];
+    if (*a.err) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t t = (uint64_t)blockIdx.x * 4 + wv;
+    if (t >= a.ntiles) return;
+    const DTileD ts = a.tiles[t];
+    const uint32_t f = ts.frame;
+    const DFrame &fr = a.frames[f];
+    const DDecCol &col = a.cols[fr.col];
+    const DDecState &st = a.dst[fr.col];
+    if (f >= st.fused_end) return;
+    const bool nulls = st.nv != col.n;
+    const uint32_t lt = (uint32_t)(t - fr.tile_base);
+    const uint32_t cnt = min((uint32_t)QA_TILE, fr.n - lt * QA_TILE);
+    const uint64_t p0 = (uint64_t)lt * QA_TILE;  // first value within the frame
+    const uint8_t *s = col.src;
+    uint32_t *wd = (uint32_t *)lds[wv];
+    uint64_t *vv = lds[wv];
+    uint64_t x[QV];
+    bool bad = false;
+    // streams with nulls: the tile covers logical positions [ts.lpos, L1), up
+    // to the next tile's first value in the same stream (or the stream's end)
+    uint64_t L1 = col.n;
+    if (nulls && t + 1 < a.ntiles) {
+        const DTileD &nx = a.tiles[t + 1];
+        if (nx.frame < st.fused_end && a.frames[nx.frame].col == fr.col) L1 = nx.lpos;
+    }
+
+    switch (fr.mode) {
+    case QA_M_RAW: {
+        const uint32_t lb = (uint32_t)lane_bytes_d(col.dtype);
+        stage_words(s + fr.payload_off + p0 * lb, cnt * lb, wd, lane);
+        wave_lds_sync();
+#pragma unroll
+        for (int q = 0; q < QV; q++) {
+            uint32_t k = lane + WAVE * q;
+            uint64_t v = 0;
+            if (k < cnt) {
+                if (lb == 4) {
+                    v = wd[k];
+                    if (col.dtype == QA_F32) v = qa_f32_to_lane((uint32_t)v);
+                } else {
+                    v = ((uint64_t)wd[2 * k + 1] << 32) | wd[2 * k];
+                }
+            }
+            x[q] = v;
+        }
+        break;
+    }
+    case QA_M_CONST: {
+        const uint64_t v = qa_ld64u(s + fr.payload_off);
+#pragma unroll
+        for (int q = 0; q < QV; q++) x[q] = v;
+        break;
+    }
+    case QA_M_STRIDE: {
+        const uint64_t v0 = qa_ld64u(s + fr.payload_off), step = qa_ld64u(s + fr.payload_off + 8);
+#pragma unroll
+        for (int q = 0; q < QV; q++) x[q] = v0 + (p0 + lane + WAVE * q) * step;
+        break;
+    }
+    case QA_M_FOR:
+    case QA_M_DELTA:
+    case QA_M_FLOAT_MULT:
+    case QA_M_DICT: {
+        const uint8_t *h = s + ts.off;
+        const uint32_t w = ts.w;
+        const uint64_t mn = qa_ld64u(h + 1);
+        stage_words(h + 9, 128 * w, wd, lane);
+        wave_lds_sync();
+#pragma unroll
+        for (int q = 0; q < QV; q++) {
+            uint32_t k = lane + WAVE * q;
+            x[q] = k < cnt ? mn + unpack_lds(wd, w, k) : 0;
+        }
+        if (fr.mode == QA_M_DELTA) {
+            uint64_t carry = ts.tail;
+#pragma unroll
+            for (int q = 0; q < QV; q++) {
+                uint32_t k = lane + WAVE * q;
+                uint64_t d = k < cnt ? qa_unzigzag(x[q]) : 0;
+                uint64_t inc = wave_incl_scan(d, lane);
+                x[q] = carry + inc;
+                carry += __shfl(inc, 63, 64);
+            }
+        } else if (fr.mode == QA_M_FLOAT_MULT) {
+            const double sc = d_fm_scales[a.fx[f].scale];
+#pragma unroll
+            for (int q = 0; q < QV; q++)
+                x[q] = (uint64_t)__double_as_longlong(__ddiv_rn((double)(int64_t)x[q], sc));
+        } else if (fr.mode == QA_M_DICT) {
+            const uint64_t card = a.fx[f].card;
+            const uint64_t *dv = a.scr + fr.scr;
+#pragma unroll
+            for (int q = 0; q < QV; q++) {
+                uint32_t k = lane + WAVE * q;
+                uint64_t code = x[q], v = 0;
+                if (k < cnt) {
+                    if (code >= card) bad = true; else v = dv[code];
+                }
+                x[q] = v;
+            }
+        }
+        break;
+    }
+    case QA_M_ALP: {
+        const uint8_t *h = s + ts.off;
+        const uint32_t w = ts.w, e = h[0], ff = h[1], exc = h[2] | ((uint32_t)h[3] << 8);
+        const uint64_t mn = qa_ld64u(h + 4);
+        stage_words(h + 13, 128 * w, wd, lane);
+        wave_lds_sync();
+        const double m1 = d_exp10[ff], m2 = d_frac10[e];
+#pragma unroll
+        for (int q = 0; q < QV; q++) {
+            uint32_t k = lane + WAVE * q;
+            int64_t d = k < cnt ? (int64_t)(mn + unpack_lds(wd, w, k)) : 0;
+            x[q] = (uint64_t)__double_as_longlong(__dmul_rn(__dmul_rn((double)d, m1), m2));
+        }
+        // exceptions in list order (alp.rs:224-244): chunks of 64 read in
+        // parallel, applied one by one so a repeated position keeps the last
+        const uint8_t *ep = h + 13 + 128ull * w;
+        for (uint32_t j0 = 0; j0 < exc; j0 += WAVE) {
+            uint32_t j = j0 + lane, pp = 0;
+            uint64_t v = 0;
+            if (j < exc) {
+                pp = ep[10 * j] | ((uint32_t)ep[10 * j + 1] << 8);
+                v = qa_ld64u(ep + 10 * j + 2);
+            }
+            uint32_t m = min((uint32_t)WAVE, exc - j0);
+            for (uint32_t i = 0; i < m; i++) {
+                uint32_t pk = (uint32_t)__shfl((int)pp, (int)i, 64);
+                uint64_t pv = __shfl(v, (int)i, 64);
+                if (pk >= cnt) {
+                    bad = true;
+                    break;
+                }
+                patch_reg(x, pk, pv, lane);
+            }
+            if (__ballot(bad)) break;
+        }
+        break;
+    }
+    case QA_M_ALP_RD: {
+        const DFrameX &fx = a.fx[f];
+        const uint32_t rbw = (64 - fx.left) & 63, dl = fx.dl;
+        const uint8_t *hc = s + ts.off, *hr = s + ts.tail;
+        const uint32_t wc = hc[0], wr = hr[0];
+        const uint64_t mc = qa_ld64u(hc + 1), mr = qa_ld64u(hr + 1);
+        stage_words(hr + 9, 128 * wr, wd, lane);
+        wave_lds_sync();
+#pragma unroll
+        for (int q = 0; q < QV; q++) {
+            uint32_t k = lane + WAVE * q;
+            uint64_t v = 0;
+            if (k < cnt) {
+                uint64_t code = wc ? for_value_direct(hc, k) : mc;
+                uint64_t right = mr + unpack_lds(wd, wr, k);
+                if (code >= dl) {
+                    bad = true;
+                    code = 0;
+                }
+                v = dl ? ((fx.dict[code] << rbw) | right) : right;
+            }
+            x[q] = v;
+        }
+        // exceptions: out = (left << right_bw) | right, in list order
+        const uint64_t *epos = a.scr + fr.scr, *eleft = a.scr2 + fr.scr;
+        uint64_t jb = 0, je = fx.nexc;
+        if (fx.sorted) {  // only this tile's slice
+            uint64_t lo = 0, hi = je;
+            while (lo < hi) {
+                uint64_t mid = (lo + hi) >> 1;
+                if (epos[mid] < p0) lo = mid + 1; else hi = mid;
+            }
+            jb = lo;
+            lo = jb, hi = je;
+            while (lo < hi) {
+                uint64_t mid = (lo + hi) >> 1;
+                if (epos[mid] < p0 + cnt) lo = mid + 1; else hi = mid;
+            }
+            je = lo;
+        }
+        for (uint64_t j0 = jb; j0 < je; j0 += WAVE) {
+            uint64_t j = j0 + lane, pp = ~0ull, v = 0;
+            if (j < je) {
+                pp = epos[j];
+                if (pp >= p0 && pp < p0 + cnt)
+                    v = (eleft[j] << rbw) | (mr + unpack_lds(wd, wr, (uint32_t)(pp - p0)));
+            }
+            uint32_t m = (uint32_t)min((uint64_t)WAVE, je - j0);
+            for (uint32_t i = 0; i < m; i++) {
+                uint64_t pk = __shfl(pp, (int)i, 64);
+                uint64_t pv = __shfl(v, (int)i, 64);
+                if (pk >= p0 && pk < p0 + cnt) patch_reg(x, (uint32_t)(pk - p0), pv, lane);
+            }
+        }
+        break;
+    }
+    case QA_M_RLE: {
+        const uint64_t R = a.fx[f].card, aux = a.fx[f].aux_off;
+        const uint64_t *rs = a.scr + fr.scr;
+#pragma unroll
+        for (int q = 0; q < QV; q++) {
+            uint32_t k = lane + WAVE * q;
+            uint64_t v = 0;
+            if (k < cnt && R) {
+                uint64_t p = p0 + k, lo = 0, hi = R;  // last run with start <= p
+                while (hi - lo > 1) {
+                    uint64_t mid = (lo + hi) >> 1;
+                    if (rs[mid] <= p) lo = mid; else hi = mid;
+                }
+                v = qa_ld64u(s + aux + 8 * lo);
+            }
+            x[q] = v;
+        }
+        break;
+    }
+    case QA_M_XORZ: {
+        const uint64_t *xv = a.scr + fr.scr + p0;
+#pragma unroll
+        for (int q = 0; q < QV; q++) {
+            uint32_t k = lane + WAVE * q;
+            x[q] = k < cnt ? xv[k] : 0;
+        }
+        break;
+    }
+    default: return;
+    }
+    if (__ballot(bad)) {
+        if (lane == 0) atomicOr(a.err, QA_ERR_CORRUPT);
+        return;
+    }
+
+    if (!nulls) {
+        emit_dense(col, fr.out_off + p0, cnt, x, lane);
+        return;
+    }
+    wave_lds_sync();  // every lane is done with the staged words
+#pragma unroll
+    for (int q = 0; q < QV; q++) {
+        uint32_t k = lane + WAVE * q;
+        if (k < cnt) vv[k] = x[q];
+    }
+    wave_lds_sync();
+    emit_gather(col, ts.lpos, L1, vv, cnt, lane);
+}
+
+static inline unsigned nbt(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
+    if (!a.ntiles) return;
+    hipLaunchKernelGGL(k_dec_tiles, dim3(nbt(a.ntiles, 4)), dim3(256), 0, s, a);
+}

@@ -2,17 +2,21 @@
 //
 // The host reads only the 16-byte header and the 9-byte frame headers
 // (src/format.rs:65-94, src/decoder.rs:65-94); every data-sized step runs here:
-//   1 validity blob -> bitmap (src/validity.rs:73-119): RLE varints are split
-//     in parallel (a byte ends a varint iff its top bit is clear), run lengths
-//     are scanned, each bitmap word ORs the valid runs that meet it
+//   1 validity blob -> bitmap + validity_out (src/validity.rs:73-119): RLE
+//     varints are split in parallel (a byte ends a varint iff its top bit is
+//     clear), run lengths are scanned, each bitmap word ORs the valid runs that
+//     meet it
 //   2 valid count and the frames that cover it (the reference scans frames
 //     until it has counted the valid values)
-//   3 index: each frame's sub-block headers (widths, offsets), bounds checked
-//   4 DeltaBitpack: per-tile delta sums, then a per-frame carry scan
+//   3 tile map and sub-block index: each frame's sub-block headers (widths,
+//     offsets), bounds checked; one wavefront per frame confirms runs of
+//     equal-width sub-blocks per round trip
+//   4 streams with nulls: the logical position of each tile's first value
+//     (select over the bitmap)
+//   5 DeltaBitpack: per-tile delta sums, then a per-frame carry scan
 //     (src/codecs/delta_bitpack.rs:48-64: a wrapping prefix sum)
-//   5 per-tile unpack from LDS (FastLanes, src/bitpack.rs:78-161) + FoR
-//   6 scatter compact values to logical positions for streams with nulls
-//     (src/validity.rs:130-146), narrowing to the column type (src/lib.rs:492-506)
+//   6 per-tile decode straight into the typed output (qa_dec_tiles.hip)
+//   7 all-null streams and validity_out of streams without a bitmap
 #include "qa_dec.h"
 
 // ---------------------------------------------------------------------------
@@ -127,6 +131,16 @@ __global__ void k_vb_words(DecArgs a) {
     uint64_t bits = c.n - bit0;
     if (bits < 64) out &= (1ull << bits) - 1;
     c.bitmap[w] = out;
+    // validity_out: the decoded bitmap, trailing bits cleared (src/lib.rs:503-506)
+    if (c.validity_out) {
+        uint64_t nbytes = (c.n + 7) / 8, b0 = w * 8;
+        uint8_t *vo = c.validity_out + b0;
+        if (b0 + 8 <= nbytes && ((uintptr_t)vo & 7) == 0) {
+            *(uint64_t *)vo = out;
+        } else {
+            for (int k = 0; k < 8 && b0 + k < nbytes; k++) vo[k] = (uint8_t)(out >> (8 * k));
+        }
+    }
 }
 
 __global__ void k_dec_vcount(DecArgs a) {
@@ -159,10 +173,27 @@ __global__ void k_dec_colfin(DecArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// stage 3: sub-block index (one thread per frame)
+// stage 3: tile map (one wavefront per frame), sub-block index
 // ---------------------------------------------------------------------------
-__global__ void k_dec_index(DecArgs a) {
-    uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_dec_tilemap(DecArgs a) {
+    const int lane = threadIdx.x & 63;
+    uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (f >= a.nframes) return;
+    const DFrame &fr = a.frames[f];
+    for (uint32_t t = lane; t < fr.ntiles; t += WAVE) {
+        DTileD d;
+        d.off = 0;
+        d.tail = 0;
+        d.lpos = 0;
+        d.frame = f;
+        d.w = 0;
+        a.tiles[fr.tile_base + t] = d;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_dec_index(DecArgs a) {
+    const int lane = threadIdx.x & 63;
+    uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (f >= a.nframes || *a.err || !frame_used(a, f)) return;
     const DFrame &fr = a.frames[f];
     const DDecCol &col = a.cols[fr.col];
@@ -170,7 +201,7 @@ __global__ void k_dec_index(DecArgs a) {
     uint64_t pos = fr.payload_off, end = fr.payload_off + fr.plen;
     uint32_t err = 0;
     if (!fr.gpu_ok) {
-        atomicOr(a.err, QA_ERR_UNSUPPORTED);
+        if (lane == 0) atomicOr(a.err, QA_ERR_UNSUPPORTED);
         return;
     }
     switch (fr.mode) {
@@ -184,7 +215,7 @@ __global__ void k_dec_index(DecArgs a) {
         if (fr.plen != 16) err = QA_ERR_CORRUPT;
         break;
     case QA_M_FOR:
-    case QA_M_DELTA: {
+    case QA_M_DELTA:
         if (fr.mode == QA_M_DELTA) {
             if (fr.plen < 8) {
                 err = QA_ERR_CORRUPT;
@@ -192,43 +223,51 @@ __global__ void k_dec_index(DecArgs a) {
             }
             pos += 8;
         }
-        uint64_t n;
-        if (!rd_varint(s, end, &pos, &n) || n != fr.n) {
+        if (for_walk_wave(s, pos, end, fr.n, a.tiles + fr.tile_base, 0, lane) == ~0ull)
             err = QA_ERR_CORRUPT;
-            break;
-        }
-        for (uint32_t t = 0; t < fr.ntiles; t++) {
-            if (pos + 9 > end) {
-                err = QA_ERR_CORRUPT;
-                break;
-            }
-            uint32_t w = s[pos];
-            if (w > 64) {
-                err = QA_ERR_CORRUPT;
-                break;
-            }
-            DTile &ts = a.tiles[fr.tile_base + t];
-            ts.w = w;
-            ts.off = pos;
-            pos += 9;
-            if (w) {
-                if (pos + 128ull * w > end) {
-                    err = QA_ERR_CORRUPT;
-                    break;
-                }
-                pos += 128ull * w;
-            }
-        }
         break;
-    }
     default:
         if (!fast_mode(fr.mode)) err = QA_ERR_UNSUPPORTED;  // fast modes: k_dec_index_fast
     }
-    if (err) atomicOr(a.err, err);
+    if (err && lane == 0) atomicOr(a.err, err);
 }
 
 // ---------------------------------------------------------------------------
-// stage 4: DeltaBitpack carries
+// stage 4: logical start of each tile of a stream with nulls
+// ---------------------------------------------------------------------------
+__global__ void k_dec_lpos(DecArgs a) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntiles || *a.err) return;
+    DTileD &d = a.tiles[t];
+    const uint32_t f = d.frame;
+    if (!frame_used(a, f)) return;
+    const DFrame &fr = a.frames[f];
+    const DDecCol &col = a.cols[fr.col];
+    if (!col.nvwords || a.dst[fr.col].nv == col.n) return;
+    uint64_t c = fr.out_off + (t - fr.tile_base) * QA_TILE;  // compact index of the first value
+    if (c == 0) {
+        d.lpos = 0;  // the first tile also takes the leading nulls
+        return;
+    }
+    // last bitmap word whose valid-count prefix is <= c; c < nv, so the word
+    // holds the c-th valid value
+    const uint64_t *vp = a.vpre + col.vword_base;
+    const uint64_t p0 = vp[0];
+    uint64_t lo = 0, hi = col.nvwords - 1;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi + 1) >> 1;
+        if (vp[mid] - p0 <= c) lo = mid; else hi = mid - 1;
+    }
+    uint64_t r = c - (vp[lo] - p0), w = col.bitmap[lo];
+    if (r >= (uint64_t)__popcll(w)) {
+        atomicOr(a.err, QA_ERR_CORRUPT);
+        return;
+    }
+    d.lpos = lo * 64 + select64(w, (uint32_t)r);
+}
+
+// ---------------------------------------------------------------------------
+// stage 5: DeltaBitpack carries
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T wsum(T v) {
@@ -243,20 +282,21 @@ __global__ void __launch_bounds__(256) k_dec_delta_sums(DecArgs a) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint64_t t = (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.ntiles) return;
-    int f = frame_by_tile(a, t);
+    DTileD &ts = a.tiles[t];
+    const uint32_t f = ts.frame;
     const DFrame &fr = a.frames[f];
     if (fr.mode != QA_M_DELTA || !frame_used(a, f)) return;
     const DDecCol &col = a.cols[fr.col];
-    DTile &ts = a.tiles[t];
     uint32_t lt = (uint32_t)(t - fr.tile_base);
     uint32_t cnt = min((uint32_t)QA_TILE, fr.n - lt * QA_TILE);
     const uint8_t *h = col.src + ts.off;
+    const uint32_t w = ts.w;
     uint64_t mn = qa_ld64u(h + 1);
     uint32_t *wd = wbuf[wv];
-    stage_words(h + 9, 128 * ts.w, wd, lane);
+    stage_words(h + 9, 128 * w, wd, lane);
     wave_lds_sync();
     uint64_t s = 0;
-    for (uint32_t k = lane; k < cnt; k += WAVE) s += qa_unzigzag(mn + unpack_lds(wd, ts.w, k));
+    for (uint32_t k = lane; k < cnt; k += WAVE) s += qa_unzigzag(mn + unpack_lds(wd, w, k));
     s = wsum(s);
     if (lane == 0) ts.tail = s;
 }
@@ -284,127 +324,25 @@ __global__ void __launch_bounds__(256) k_dec_delta_scan(DecArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// stage 5: per-tile decode (one wavefront per 1024-value tile)
+// stage 7: all-null streams (no tile writes them), validity_out without bitmap
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
-    __shared__ uint32_t wbuf[4][2 * QA_TILE + 4];
-    if (*a.err) return;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    uint64_t t = (uint64_t)blockIdx.x * 4 + wv;
-    if (t >= a.ntiles) return;
-    int f = frame_by_tile(a, t);
-    if (!frame_used(a, f)) return;
-    const DFrame &fr = a.frames[f];
-    const DDecCol &col = a.cols[fr.col];
-    const bool nulls = a.dst[fr.col].nv != col.n;
-    const DTile &ts = a.tiles[t];
-    uint32_t lt = (uint32_t)(t - fr.tile_base);
-    uint32_t cnt = min((uint32_t)QA_TILE, fr.n - lt * QA_TILE);
-    uint64_t base = fr.out_off + (uint64_t)lt * QA_TILE;
-    const uint8_t *pl = col.src + fr.payload_off;
-    uint32_t *wd = wbuf[wv];
-    switch (fr.mode) {
-    case QA_M_RAW: {
-        uint32_t lb = (uint32_t)lane_bytes_d(col.dtype);
-        stage_words(pl + (uint64_t)lt * QA_TILE * lb, cnt * lb, wd, lane);
-        wave_lds_sync();
-        for (uint32_t k = lane; k < cnt; k += WAVE) {
-            uint64_t x;
-            if (lb == 4) {
-                x = wd[k];
-                if (col.dtype == QA_F32) x = qa_f32_to_lane((uint32_t)x);
-            } else {
-                x = ((uint64_t)wd[2 * k + 1] << 32) | wd[2 * k];
-            }
-            put_value(col, nulls, a.lane, base + k, x);
-        }
-        break;
-    }
-    case QA_M_CONST: {
-        uint64_t v = qa_ld64u(pl);
-        for (uint32_t k = lane; k < cnt; k += WAVE) put_value(col, nulls, a.lane, base + k, v);
-        break;
-    }
-    case QA_M_STRIDE: {
-        uint64_t v0 = qa_ld64u(pl), st = qa_ld64u(pl + 8);
-        for (uint32_t k = lane; k < cnt; k += WAVE)
-            put_value(col, nulls, a.lane, base + k, v0 + ((uint64_t)lt * QA_TILE + k) * st);
-        break;
-    }
-    case QA_M_FOR:
-    case QA_M_DELTA: {
-        const uint8_t *h = col.src + ts.off;
-        uint64_t mn = qa_ld64u(h + 1);
-        stage_words(h + 9, 128 * ts.w, wd, lane);
-        wave_lds_sync();
-        if (fr.mode == QA_M_FOR) {
-            for (uint32_t k = lane; k < cnt; k += WAVE)
-                put_value(col, nulls, a.lane, base + k, mn + unpack_lds(wd, ts.w, k));
-        } else {
-            uint64_t carry = ts.tail;
-            for (uint32_t k0 = 0; k0 < cnt; k0 += WAVE) {
-                uint32_t k = k0 + lane;
-                uint64_t d = k < cnt ? qa_unzigzag(mn + unpack_lds(wd, ts.w, k)) : 0;
-                uint64_t inc = wave_incl_scan(d, lane);
-                if (k < cnt) put_value(col, nulls, a.lane, base + k, carry + inc);
-                carry += __shfl(inc, 63, 64);
-            }
-        }
-        break;
-    }
-    default: break;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// stage 6: nulls — scatter + narrow, validity out
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_dec_scatter(DecArgs a) {
-    // one workgroup per 16 bitmap words of columns that carry a bitmap
-    __shared__ uint64_t words[16], wpre[16];
-    __shared__ int wcol[16];
-    if (*a.err) return;
-    uint64_t w0 = (uint64_t)blockIdx.x * 16;
-    if (threadIdx.x < 16) {
-        uint64_t j = w0 + threadIdx.x;
-        if (j < a.nvwords) {
-            int c = dcol_by_vword(a, j);
-            const DDecCol &col = a.cols[c];
-            wcol[threadIdx.x] = c;
-            words[threadIdx.x] = col.bitmap[j - col.vword_base];
-            wpre[threadIdx.x] = a.vpre[j] - a.vpre[col.vword_base];
-        } else {
-            wcol[threadIdx.x] = -1;
-        }
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < 16 * 64; k += 256) {
-        int wi = k >> 6, c = wcol[wi];
-        if (c < 0) continue;
-        const DDecCol &col = a.cols[c];
-        uint64_t lj = w0 + wi - col.vword_base;
-        uint64_t i = lj * 64 + (k & 63);
-        if (i >= col.n) continue;
-        uint64_t w = words[wi];
-        uint32_t b = k & 63;
-        if (col.validity_out && (b & 7) == 0) col.validity_out[i >> 3] = (uint8_t)(w >> b);
-        if (a.dst[c].nv == col.n) continue;  // decoded in place
-        uint64_t x = 0;
-        if ((w >> b) & 1)
-            x = a.lane[col.lane_off + wpre[wi] + (uint64_t)__popcll(w & ((1ull << b) - 1))];
-        store_typed(col, i, x);
-    }
+__global__ void k_dec_nullfill(DecArgs a) {
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.nvwords || *a.err) return;
+    const int ci = dcol_by_vword(a, j);
+    const DDecCol &c = a.cols[ci];
+    if (a.dst[ci].nv != 0) return;
+    uint64_t i0 = (j - c.vword_base) * 64;
+    for (uint64_t i = i0; i < i0 + 64 && i < c.n; i++) store_typed(c, i, 0);
 }
 
 // validity_out of streams without a bitmap: all valid, trailing bits cleared
 __global__ void k_dec_allvalid(DecArgs a) {
     uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.nvals || *a.err) return;
-    const DDecCol &col = a.cols[dcol_by_val(a, g)];
-    uint64_t i = g - col.val_base;
-    if (i >= col.n || col.nvwords || !col.validity_out || (i & 7)) return;
-    uint64_t rem = col.n - i;
-    col.validity_out[i >> 3] = rem >= 8 ? 0xFF : (uint8_t)((1u << rem) - 1);
+    if (g >= a.navbytes || *a.err) return;
+    const DDecCol &col = a.cols[dcol_by_avbyte(a, g)];
+    uint64_t b = g - col.avbyte_base, rem = col.n - b * 8;
+    col.validity_out[b] = rem >= 8 ? 0xFF : (uint8_t)((1u << rem) - 1);
 }
 
 static inline unsigned nbk(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
@@ -429,13 +367,14 @@ void qa_decode_launch(const DecArgs &a, hipStream_t s) {
     }
     QA_LAUNCH(k_dec_colfin, dim3(nbk(a.ncols, 64)), dim3(64), 0, s, a);
     if (a.nframes) {
-        QA_LAUNCH(k_dec_index, dim3(nbk(a.nframes, 64)), dim3(64), 0, s, a);
+        QA_LAUNCH(k_dec_tilemap, dim3(nbk(a.nframes, 4)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_dec_index, dim3(nbk(a.nframes, 4)), dim3(256), 0, s, a);
         qa_fast_dec_index_launch(a, s);
+        if (a.nvwords) QA_LAUNCH(k_dec_lpos, dim3(nbk(a.ntiles, 256)), dim3(256), 0, s, a);
         QA_LAUNCH(k_dec_delta_sums, dim3(nbk(a.ntiles, 4)), dim3(256), 0, s, a);
         QA_LAUNCH(k_dec_delta_scan, dim3(nbk(a.nframes, 4)), dim3(256), 0, s, a);
-        QA_LAUNCH(k_dec_tiles, dim3(nbk(a.ntiles, 4)), dim3(256), 0, s, a);
-        qa_fast_dec_tiles_launch(a, s);
+        qa_dec_tiles_launch(a, s);
     }
-    if (a.nvwords) QA_LAUNCH(k_dec_scatter, dim3(nbk(a.nvwords, 16)), dim3(256), 0, s, a);
-    QA_LAUNCH(k_dec_allvalid, dim3(nbk(a.nvals, 256)), dim3(256), 0, s, a);
+    if (a.nvwords) QA_LAUNCH(k_dec_nullfill, dim3(nbk(a.nvwords, 256)), dim3(256), 0, s, a);
+    QA_LAUNCH(k_dec_allvalid, dim3(nbk(a.navbytes, 256)), dim3(256), 0, s, a);
 }

@@ -39,6 +39,16 @@ __device__ __forceinline__ bool is_signed(uint32_t dt) { return dt == QA_I32 || 
 __device__ __forceinline__ uint64_t lane_at(const DColState &st, uint64_t i) {
     return qa_lane_get(st.lanep, st.lkind, i);
 }
+// lane word i with the storage kind fixed at compile time (QA_LK_ANY: by lk)
+#define QA_LK_ANY 99
+template <int LK>
+__device__ __forceinline__ uint64_t lane_get_t(const void *p, uint32_t lk, uint64_t i) {
+    if (LK == QA_LK_U64) return ((const uint64_t *)p)[i];
+    if (LK == QA_LK_U32) return ((const uint32_t *)p)[i];
+    if (LK == QA_LK_I32) return (uint64_t)(int64_t)((const int32_t *)p)[i];
+    if (LK == QA_LK_F32) return qa_f32_to_lane(((const uint32_t *)p)[i]);
+    return qa_lane_get(p, lk, i);
+}
 
 // bitmap word j of a column (bits >= n cleared); byte loads: the caller's
 // bitmap is ceil(n/8) bytes with no padding or alignment promise

@@ -77,85 +77,88 @@ __global__ void k_col_counts(EncArgs a) {
 }
 
 // compaction of the valid values, in the input's own element width
-// (src/validity.rs:122-127); one workgroup per 16 validity words
+// (src/validity.rs:122-127). One wavefront per 64 bitmap words: lane i reads
+// word i, its column and its rank base; the wavefront then moves the valid
+// slots of 8 words per step (8 independent loads in flight per lane).
+__device__ __forceinline__ uint64_t ld_w(const uint8_t *p, uint64_t i, uint32_t w) {
+    switch (w) {
+    case 1: return p[i];
+    case 2: return ((const uint16_t *)p)[i];
+    case 4: return ((const uint32_t *)p)[i];
+    default: return ((const uint64_t *)p)[i];
+    }
+}
+__device__ __forceinline__ void st_w(uint8_t *p, uint64_t i, uint32_t w, uint64_t v) {
+    switch (w) {
+    case 1: p[i] = (uint8_t)v; break;
+    case 2: ((uint16_t *)p)[i] = (uint16_t)v; break;
+    case 4: ((uint32_t *)p)[i] = (uint32_t)v; break;
+    default: ((uint64_t *)p)[i] = v;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_compact(EncArgs a) {
-    __shared__ uint64_t words[16], wpre[16];
-    __shared__ int wcol[16];
-    uint64_t w0 = (uint64_t)blockIdx.x * 16;
-    if (threadIdx.x < 16) {
-        uint64_t j = w0 + threadIdx.x;
-        if (j < a.nvwords) {
-            int c = col_by_vword(a.cols, a.ncols, j);
-            const DCol &col = a.cols[c];
-            wcol[threadIdx.x] = c;
-            words[threadIdx.x] = bm_word(col.validity, col.n, j - col.vword_base);
-            wpre[threadIdx.x] = a.vpre[j] - a.vpre[col.vword_base];
-        } else {
-            wcol[threadIdx.x] = -1;
+    const int lane = threadIdx.x & 63;
+    const uint64_t j0 = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * WAVE;
+    if (j0 >= a.nvwords) return;
+    const uint64_t j = j0 + lane;
+    uint64_t w = 0, rank0 = 0, pos0 = 0;
+    uint64_t src = 0, dst = 0;
+    uint32_t width = 0;
+    if (j < a.nvwords) {
+        const int c = col_by_vword(a.cols, a.ncols, j);
+        const DCol &col = a.cols[c];
+        if (a.cst[c].has_nulls) {
+            w = bm_word(col.validity, col.n, j - col.vword_base);
+            rank0 = a.vpre[j] - a.vpre[col.vword_base];
+            pos0 = (j - col.vword_base) * 64;
+            src = (uint64_t)col.values;
+            dst = (uint64_t)((uint8_t *)a.lane + col.lane_off * 8);
+            width = col.in_width;
         }
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < 16 * 64; k += 256) {
-        int wi = k >> 6;
-        int c = wcol[wi];
-        if (c < 0) continue;
-        const DCol &col = a.cols[c];
-        if (!a.cst[c].has_nulls) continue;
-        uint64_t lj = w0 + wi - col.vword_base;
-        uint64_t i = lj * 64 + (k & 63);
-        if (i >= col.n) continue;
-        uint64_t w = words[wi];
-        uint32_t b = k & 63;
-        if (!((w >> b) & 1)) continue;
-        uint64_t rank = wpre[wi] + (uint64_t)__popcll(w & ((1ull << b) - 1));
-        uint8_t *dst = (uint8_t *)a.lane + col.lane_off * 8;
-        const uint8_t *src = (const uint8_t *)col.values;
-        switch (col.in_width) {
-        case 1: dst[rank] = src[i]; break;
-        case 2: ((uint16_t *)dst)[rank] = ((const uint16_t *)src)[i]; break;
-        case 4: ((uint32_t *)dst)[rank] = ((const uint32_t *)src)[i]; break;
-        default: ((uint64_t *)dst)[rank] = ((const uint64_t *)src)[i];
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int i0 = 0; i0 < WAVE; i0 += 8) {
+        uint64_t v[8], rk[8], dp[8];
+        uint32_t wd[8];
+        bool on[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int i = i0 + u;
+            const uint64_t wi = __shfl(w, i, 64);
+            wd[u] = (uint32_t)__shfl((int)width, i, 64);
+            dp[u] = __shfl(dst, i, 64);
+            on[u] = (wi >> lane) & 1;  // bits past the column end are clear
+            rk[u] = __shfl(rank0, i, 64) + (uint64_t)__popcll(wi & below);
+            const uint64_t pos = __shfl(pos0, i, 64) + lane;
+            const uint8_t *sp = (const uint8_t *)__shfl(src, i, 64);
+            v[u] = on[u] ? ld_w(sp, pos, wd[u]) : 0;
         }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (on[u]) st_w((uint8_t *)dp[u], rk[u], wd[u], v[u]);
     }
 }
 
 // ---------------------------------------------------------------------------
 // stage 2: fused tile statistics + region probe, one workgroup per region
 // ---------------------------------------------------------------------------
-struct RegionAcc {
-    uint32_t exp_min, exp_max, eq, dsum;
-    uint64_t smin, smax, sdz;
-};
-
-__global__ void __launch_bounds__(1024) k_region_stats(EncArgs a) {
-    __shared__ uint64_t seen[256];
-    __shared__ RegionAcc acc;
-    const uint32_t r = blockIdx.x;
-    const int c = col_by_region(a.cols, a.ncols, r);
-    const DCol &col = a.cols[c];
-    const DColState &st = a.cst[c];
-    const uint64_t nv = st.nv;
-    const uint64_t rs = (uint64_t)(r - col.region_base) * col.region_size;
+// Each wavefront takes tiles of the region. Per tile it keeps what the
+// candidates of the Fastest/Fast pools need:
+//   min/max (unsigned; signed for I32/I64, dtype.rs:121-123)    FoR, Dict
+//   zig-zag delta min/max of elements 1..cnt-1                   DeltaBitpack
+//   first/last value, step, all-equal, arithmetic                CONST, STRIDE
+//   changes between neighbours                                   RLE, repeats probe
+//   XORZ varint bytes (Fast only)                                XORZ
+// The probe's exponent range only gates the entropy levels' byte transpose
+// (encoder.rs:414-417) and is not computed. The previous element of each lane
+// comes from its neighbour lane.
+template <int LK, bool SG, bool FASTL>
+__device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp, uint32_t lk,
+                                                 uint64_t rs, uint64_t len, DTile *tl) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    DTile *tl = a.tiles + col.tile_base + (uint64_t)(r - col.region_base) * col.tpr;
-    if (rs >= nv) {
-        for (uint32_t t = threadIdx.x; t < col.tpr; t += blockDim.x) tl[t].active = 0;
-        return;
-    }
-    const uint64_t len = nv - rs < col.region_size ? nv - rs : col.region_size;
-    for (int k = threadIdx.x; k < 256; k += blockDim.x) seen[k] = 0;
-    if (threadIdx.x == 0) {
-        acc.exp_min = 0xFFFFFFFFu;
-        acc.exp_max = 0;
-        acc.eq = 0;
-        acc.dsum = 0;
-        acc.smin = ~0ull;
-        acc.smax = 0;
-        acc.sdz = 0;
-    }
-    __syncthreads();
     const uint32_t ntl = (uint32_t)((len + QA_TILE - 1) / QA_TILE);
-    uint32_t eq_region = 0;  // consecutive-equal pairs inside the region (lane 0 sums)
+    uint32_t eq_region = 0;
     for (uint32_t t = wv; t < col.tpr; t += nwv) {
         DTile &ts = tl[t];
         if (t >= ntl) {
@@ -164,47 +167,60 @@ __global__ void __launch_bounds__(1024) k_region_stats(EncArgs a) {
         }
         const uint64_t k0 = rs + (uint64_t)t * QA_TILE;  // column index of the tile start
         const uint32_t cnt = (uint32_t)min((uint64_t)QA_TILE, rs + len - k0);
+        const uint64_t before = k0 ? lane_get_t<LK>(lp, lk, k0 - 1) : 0;
+        const uint64_t vf = lane_get_t<LK>(lp, lk, k0);
+        const uint64_t vl = lane_get_t<LK>(lp, lk, k0 + cnt - 1);
+        const uint64_t step = cnt > 1 ? lane_get_t<LK>(lp, lk, k0 + 1) - vf : 0;
         uint64_t umin = ~0ull, umax = 0, dzmin = ~0ull, dzmax = 0, xb = 0;
         int64_t smin = INT64_MAX, smax = INT64_MIN;
-        uint32_t emin = 0xFFFFFFFFu, emax = 0, chg = 0, eqp = 0, arith = 1, same = 1;
-        const uint64_t vf = lane_at(st, k0);
-        const uint64_t step = cnt > 1 ? lane_at(st, k0 + 1) - vf : 0;
-        for (uint32_t k = lane; k < cnt; k += WAVE) {
-            uint64_t x = lane_at(st, k0 + k);
-            umin = x < umin ? x : umin;
+        uint32_t chg = 0, eqp = 0;
+        bool diff = false, nonarith = false;
+        // every load of the tile in flight before the first use
+        uint64_t xs[QA_TILE / WAVE];
+#pragma unroll
+        for (int q = 0; q < QA_TILE / WAVE; q++) {
+            const uint32_t k = q * WAVE + lane;
+            xs[q] = k < cnt ? lane_get_t<LK>(lp, lk, k0 + k) : vf;
+        }
+        uint64_t carry = before;
+#pragma unroll
+        for (int q = 0; q < QA_TILE / WAVE; q++) {
+            const uint32_t k = q * WAVE + lane;
+            const bool in = k < cnt;
+            const uint64_t x = xs[q];  // out-of-range lanes hold vf
+            uint64_t p = __shfl_up(x, 1, 64);
+            if (lane == 0) p = carry;
+            carry = __shfl(x, 63, 64);
+            umin = x < umin ? x : umin;  // out-of-range lanes hold vf
             umax = x > umax ? x : umax;
-            smin = (int64_t)x < smin ? (int64_t)x : smin;
-            smax = (int64_t)x > smax ? (int64_t)x : smax;
-            uint32_t e = (uint32_t)((x >> 52) & 0x7FF);
-            emin = min(emin, e);
-            emax = max(emax, e);
-            same &= x == vf;
-            if (k > 0) {
-                uint64_t p = lane_at(st, k0 + k - 1);
-                uint64_t z = qa_zigzag(x - p);
+            if (SG) {
+                smin = (int64_t)x < smin ? (int64_t)x : smin;
+                smax = (int64_t)x > smax ? (int64_t)x : smax;
+            }
+            const bool pair = in && k > 0;  // neighbours inside the tile
+            if (pair) {
+                const uint64_t z = qa_zigzag(x - p);
                 dzmin = z < dzmin ? z : dzmin;
                 dzmax = z > dzmax ? z : dzmax;
-                xb += qa_varint_len(x ^ p);
-                chg += x != p;
-                arith &= (x - p) == step;
-                eqp += x == p;
-            } else if (k0 > rs) {
-                eqp += x == lane_at(st, k0 - 1);  // pair across tiles of this region
+                if (FASTL) xb += qa_varint_len(x ^ p);
+                nonarith |= (x - p) != step;
             }
+            diff |= x != vf;
+            chg += (uint32_t)__popcll(__ballot(pair && x != p));
+            // equal neighbours of the region: inside the tile, and across
+            // tile boundaries inside the region
+            eqp += (uint32_t)__popcll(__ballot((pair || (in && k == 0 && k0 > rs)) && x == p));
         }
         umin = wave_reduce(umin, OpMinU());
         umax = wave_reduce(umax, OpMaxU());
-        smin = wave_reduce(smin, OpMinS());
-        smax = wave_reduce(smax, OpMaxS());
+        if (SG) {
+            smin = wave_reduce(smin, OpMinS());
+            smax = wave_reduce(smax, OpMaxS());
+        }
         dzmin = wave_reduce(dzmin, OpMinU());
         dzmax = wave_reduce(dzmax, OpMaxU());
-        xb = wave_reduce(xb, OpAdd());
-        uint64_t chg64 = wave_reduce((uint64_t)chg, OpAdd());
-        uint64_t eq64 = wave_reduce((uint64_t)eqp, OpAdd());
-        emin = (uint32_t)wave_reduce((uint64_t)emin, OpMinU());
-        emax = (uint32_t)wave_reduce((uint64_t)emax, OpMaxU());
-        arith = wave_reduce(arith, OpAnd());
-        same = wave_reduce(same, OpAnd());
+        if (FASTL) xb = wave_reduce(xb, OpAdd());
+        const bool same = __ballot(diff) == 0, arith = __ballot(nonarith) == 0;
         if (lane == 0) {
             ts.umin = umin;
             ts.umax = umax;
@@ -213,67 +229,172 @@ __global__ void __launch_bounds__(1024) k_region_stats(EncArgs a) {
             ts.dzmin = dzmin;
             ts.dzmax = dzmax;
             ts.xorz_rest = xb;
-            ts.chg_rest = (uint32_t)chg64;
-            ts.exp_min = emin;
-            ts.exp_max = emax;
+            ts.chg_rest = chg;
+            ts.exp_min = 0;
+            ts.exp_max = 0;
             ts.v_first = vf;
-            ts.v_last = lane_at(st, k0 + cnt - 1);
+            ts.v_last = vl;
             ts.step = step;
-            ts.dz_first = k0 ? qa_zigzag(vf - lane_at(st, k0 - 1)) : 0;
+            ts.dz_first = k0 ? qa_zigzag(vf - before) : 0;
             ts.cnt = cnt;
             ts.arith = (uint8_t)arith;
             ts.allsame = (uint8_t)same;
             ts.active = 1;
-            eq_region += (uint32_t)eq64;
-            atomicMin(&acc.exp_min, emin);
-            atomicMax(&acc.exp_max, emax);
+        }
+        eq_region += eqp;
+    }
+    return eq_region;
+}
+
+// distinct estimate (encoder.rs:728-743) and incompressibility samples
+// (encoder.rs:655-683) of the values [s0, s0 + n): seen is a 16384-bit LDS set
+template <int LK>
+__device__ __forceinline__ void probe_samples(const void *lp, uint32_t lk, uint64_t s0, uint64_t n,
+                                              uint64_t *seen, uint64_t *mn, uint64_t *mx,
+                                              uint64_t *dz) {
+    // loads are issued in groups so several are in flight per lane
+    const uint64_t stride = n / 4096 > 1 ? n / 4096 : 1, bd = blockDim.x;
+    for (uint64_t k0 = (uint64_t)threadIdx.x * stride; k0 < n; k0 += 8 * bd * stride) {
+        uint64_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint64_t k = k0 + u * bd * stride;
+            v[u] = k < n ? lane_get_t<LK>(lp, lk, s0 + k) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (k0 + u * bd * stride >= n) break;
+            uint64_t h = v[u] * 0x9E3779B97F4A7C15ull;
+            uint32_t bucket = (uint32_t)((h >> 50) & 0x3FFF);
+            atomicOr((unsigned long long *)&seen[bucket >> 6], 1ull << (bucket & 63));
         }
     }
-    if (lane == 0) atomicAdd(&acc.eq, eq_region);
-    // distinct estimate over a strided sample (src/encoder.rs:728-743)
-    const uint64_t stride = len / 4096 > 1 ? len / 4096 : 1;
-    for (uint64_t k = (uint64_t)threadIdx.x * stride; k < len; k += (uint64_t)blockDim.x * stride) {
-        uint64_t h = lane_at(st, rs + k) * 0x9E3779B97F4A7C15ull;
-        uint32_t bucket = (uint32_t)((h >> 50) & 0x3FFF);
-        atomicOr((unsigned long long *)&seen[bucket >> 6], 1ull << (bucket & 63));
+    const uint64_t step = n / 2048 > 1 ? n / 2048 : 1;
+    uint64_t a = *mn, b = *mx, d = *dz;
+    for (uint64_t i0 = step * (1 + threadIdx.x); i0 < n; i0 += 4 * step * bd) {
+        uint64_t v[4], pv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint64_t i = i0 + u * step * bd;
+            v[u] = i < n ? lane_get_t<LK>(lp, lk, s0 + i) : 0;
+            pv[u] = i < n ? lane_get_t<LK>(lp, lk, s0 + i - 1) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (i0 + u * step * bd >= n) break;
+            const uint64_t x = v[u];
+            a = x < a ? x : a;
+            b = x > b ? x : b;
+            const uint64_t z = qa_zigzag(x - pv[u]);
+            d = z > d ? z : d;
+        }
     }
-    // incompressibility samples for the region taken as a block
-    const uint64_t step = len / 2048 > 1 ? len / 2048 : 1;
-    const uint64_t v0 = lane_at(st, rs);
+    *mn = a;
+    *mx = b;
+    *dz = d;
+}
+
+template <int LK, bool SG, bool FASTL>
+__device__ __forceinline__ void region_stats_body(const EncArgs &a, uint32_t r, const DCol &col,
+                                                  const void *lp, uint32_t lk, uint64_t rs,
+                                                  uint64_t len, DTile *tl, uint64_t *seen,
+                                                  uint32_t *eq_s, uint64_t *smin_s,
+                                                  uint64_t *smax_s, uint64_t *sdz_s) {
+    const int lane = threadIdx.x & 63;
+    uint32_t eq = region_tiles<LK, SG, FASTL>(col, lp, lk, rs, len, tl);
+    if (lane == 0) atomicAdd(eq_s, eq);
+    const uint64_t v0 = lane_get_t<LK>(lp, lk, rs);
     uint64_t mn = v0, mx = v0, dz = 0;
-    for (uint64_t i = step * (1 + threadIdx.x); i < len; i += step * blockDim.x) {
-        uint64_t x = lane_at(st, rs + i);
-        mn = x < mn ? x : mn;
-        mx = x > mx ? x : mx;
-        uint64_t z = qa_zigzag(x - lane_at(st, rs + i - 1));
-        dz = z > dz ? z : dz;
-    }
+    probe_samples<LK>(lp, lk, rs, len, seen, &mn, &mx, &dz);
     mn = wave_reduce(mn, OpMinU());
     mx = wave_reduce(mx, OpMaxU());
     dz = wave_reduce(dz, OpMaxU());
     if (lane == 0) {
-        atomicMin((unsigned long long *)&acc.smin, mn);
-        atomicMax((unsigned long long *)&acc.smax, mx);
-        atomicMax((unsigned long long *)&acc.sdz, dz);
+        atomicMin((unsigned long long *)smin_s, mn);
+        atomicMax((unsigned long long *)smax_s, mx);
+        atomicMax((unsigned long long *)sdz_s, dz);
     }
+}
+
+#define QA_LANE_DISPATCH(F, lk, sg, fastl, ...)                                     \
+    do {                                                                            \
+        if (fastl) {                                                                \
+            switch (lk) {                                                           \
+            case QA_LK_U64:                                                         \
+                if (sg) F<QA_LK_U64, true, true>(__VA_ARGS__);                      \
+                else F<QA_LK_U64, false, true>(__VA_ARGS__);                        \
+                break;                                                              \
+            case QA_LK_U32: F<QA_LK_U32, false, true>(__VA_ARGS__); break;          \
+            case QA_LK_I32: F<QA_LK_I32, true, true>(__VA_ARGS__); break;           \
+            case QA_LK_F32: F<QA_LK_F32, false, true>(__VA_ARGS__); break;          \
+            default:                                                                \
+                if (sg) F<QA_LK_ANY, true, true>(__VA_ARGS__);                      \
+                else F<QA_LK_ANY, false, true>(__VA_ARGS__);                        \
+            }                                                                       \
+        } else {                                                                    \
+            switch (lk) {                                                           \
+            case QA_LK_U64:                                                         \
+                if (sg) F<QA_LK_U64, true, false>(__VA_ARGS__);                     \
+                else F<QA_LK_U64, false, false>(__VA_ARGS__);                       \
+                break;                                                              \
+            case QA_LK_U32: F<QA_LK_U32, false, false>(__VA_ARGS__); break;         \
+            case QA_LK_I32: F<QA_LK_I32, true, false>(__VA_ARGS__); break;          \
+            case QA_LK_F32: F<QA_LK_F32, false, false>(__VA_ARGS__); break;         \
+            default:                                                                \
+                if (sg) F<QA_LK_ANY, true, false>(__VA_ARGS__);                     \
+                else F<QA_LK_ANY, false, false>(__VA_ARGS__);                       \
+            }                                                                       \
+        }                                                                           \
+    } while (0)
+
+__global__ void __launch_bounds__(1024) k_region_stats(EncArgs a) {
+    __shared__ uint64_t seen[256];
+    __shared__ uint64_t smin_s, smax_s, sdz_s;
+    __shared__ uint32_t eq_s, dsum_s;
+    const uint32_t r = blockIdx.x;
+    const int c = col_by_region(a.cols, a.ncols, r);
+    const DCol &col = a.cols[c];
+    const DColState &st = a.cst[c];
+    const uint64_t nv = st.nv;
+    const void *lp = st.lanep;
+    const uint32_t lk = st.lkind;
+    const uint64_t rs = (uint64_t)(r - col.region_base) * col.region_size;
+    const int lane = threadIdx.x & 63;
+    DTile *tl = a.tiles + col.tile_base + (uint64_t)(r - col.region_base) * col.tpr;
+    if (rs >= nv) {
+        for (uint32_t t = threadIdx.x; t < col.tpr; t += blockDim.x) tl[t].active = 0;
+        return;
+    }
+    const uint64_t len = nv - rs < col.region_size ? nv - rs : col.region_size;
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) seen[k] = 0;
+    if (threadIdx.x == 0) {
+        eq_s = 0;
+        dsum_s = 0;
+        smin_s = ~0ull;
+        smax_s = 0;
+        sdz_s = 0;
+    }
+    __syncthreads();
+    const bool sg = is_signed(col.dtype), fastl = col.level != QA_FASTEST;
+    QA_LANE_DISPATCH(region_stats_body, lk, sg, fastl, a, r, col, lp, lk, rs, len, tl, seen, &eq_s,
+                     &smin_s, &smax_s, &sdz_s);
     __syncthreads();
     uint32_t d = 0;
     for (int k = threadIdx.x; k < 256; k += blockDim.x) d += (uint32_t)__popcll(seen[k]);
     d = (uint32_t)wave_reduce((uint64_t)d, OpAdd());
-    if (lane == 0) atomicAdd(&acc.dsum, d);
+    if (lane == 0) atomicAdd(&dsum_s, d);
     __syncthreads();
     if (threadIdx.x == 0) {
-        bool dl = acc.dsum < 2048;
-        uint64_t c2 = (uint64_t)(acc.eq + 1) * 2;  // consec_eq counts element 0 against itself
+        bool dl = dsum_s < 2048;
+        uint64_t c2 = (uint64_t)(eq_s + 1) * 2;  // consec_eq counts element 0 against itself
         if (c2 > 0xFFFFFFFFull) c2 = 0xFFFFFFFFull;
         bool rp = c2 > (uint32_t)len;
         bool inc = false;
         if (!is_float(col.dtype) && !dl && !rp && len >= 256) {
             uint32_t thr = (uint32_t)(lane_bytes_of(col.dtype) * 8) - 4;
-            inc = qa_width(acc.smax - acc.smin) >= thr && qa_width(acc.sdz) >= thr;
+            inc = qa_width(smax_s - smin_s) >= thr && qa_width(sdz_s) >= thr;
         }
-        a.rfeat[r] = (dl ? RF_DLOW : 0) | (rp ? RF_REP : 0) | (inc ? RF_INC : 0) |
-                     ((acc.exp_max - acc.exp_min) << 8);
+        a.rfeat[r] = (dl ? RF_DLOW : 0) | (rp ? RF_REP : 0) | (inc ? RF_INC : 0);
     }
 }
 
@@ -282,60 +403,99 @@ __global__ void __launch_bounds__(1024) k_region_stats(EncArgs a) {
 // ---------------------------------------------------------------------------
 #define PLAN_CHUNK 4096
 
+// A block that starts at region lr ends at min(start + 131072, nv) when its
+// probe says distinct_low || repeats and the column is adaptive (regions are
+// 32768 values there), else at min(start + R, nv); the next block starts at
+// region min(lr + 4 or 1, nreg). The walk runs on one thread over bitmask
+// words of the region features held in registers; owners are found by a
+// backward scan of at most three regions (a block spans at most four).
 __global__ void __launch_bounds__(256) k_plan(EncArgs a) {
-    __shared__ uint8_t feat[PLAN_CHUNK];
-    __shared__ uint32_t own[PLAN_CHUNK];
-    __shared__ uint32_t next_s, nb_s, cur_owner;
+    __shared__ uint64_t featw[PLAN_CHUNK / 64], startw[PLAN_CHUNK / 64];
+    __shared__ uint32_t next_s, nb_s, last_s;
     const int c = blockIdx.x;
     const DCol &col = a.cols[c];
     const uint64_t nv = a.cst[c].nv;
     const uint32_t R = col.region_size;
     const uint32_t nreg = (uint32_t)((nv + R - 1) / R);  // regions holding values
+    const bool adaptive = col.adaptive;
+    const uint32_t grow = QA_MAX_Q / QA_BASE_Q;
     if (threadIdx.x == 0) {
         next_s = 0;
         nb_s = 0;
-        cur_owner = NONE32;
+        last_s = NONE32;
     }
     for (uint32_t base = 0; base < col.nregions; base += PLAN_CHUNK) {
-        uint32_t cn = min((uint32_t)PLAN_CHUNK, col.nregions - base);
+        const uint32_t cn = min((uint32_t)PLAN_CHUNK, col.nregions - base);
+        const uint32_t cw = (cn + 63) / 64;
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < cn; k += blockDim.x)
-            feat[k] = base + k < nreg ? (uint8_t)(a.rfeat[col.region_base + base + k] & 3) : 0;
+        // the last block start before this chunk (thread 0 updates it below,
+        // after the next barrier)
+        const uint32_t prev_last = last_s;
+        // feature and start words (the loop bound is a multiple of 64, so each
+        // wavefront's lanes cover one aligned word per step)
+        for (uint32_t k = threadIdx.x; k < cw * 64; k += blockDim.x) {
+            const uint32_t lr = base + k;
+            const bool f = k < cn && lr < nreg && (a.rfeat[col.region_base + lr] & 3);
+            const bool st = k < cn && lr < nreg && !adaptive;  // fixed size: every region
+            const uint64_t fm = __ballot(f), sm = __ballot(st);
+            if ((threadIdx.x & 63) == 0) {
+                featw[k >> 6] = fm;
+                startw[k >> 6] = sm;
+            }
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
-            // the sequential walk, in LDS: a region starts a block iff it is
-            // the region after the previous block's end
-            uint32_t ns = next_s, nb = nb_s, ow = cur_owner;
-            for (uint32_t k = 0; k < cn; k++) {
-                uint32_t lr = base + k;
-                if (lr >= nreg) {
-                    own[k] = NONE32;
-                    continue;
+            const uint32_t lim = min(base + cn, nreg);
+            uint32_t ns = next_s, nb = nb_s, last = last_s;
+            if (!adaptive) {
+                if (ns < lim) {
+                    nb += lim - ns;
+                    last = lim - 1;
+                    ns = lim;
                 }
-                if (lr == ns) {
-                    uint64_t rs = (uint64_t)lr * R;
-                    uint64_t end = (col.adaptive && feat[k]) ? min(rs + (uint64_t)QA_MAX_Q, nv)
-                                                             : min(rs + (uint64_t)R, nv);
-                    ns = (uint32_t)((end + R - 1) / R);
+            } else {
+                uint32_t cur = NONE32;
+                uint64_t fw = 0, sw = 0;
+                while (ns < lim) {
+                    const uint32_t k = ns - base, wi = k >> 6;
+                    if (wi != cur) {
+                        if (cur != NONE32) startw[cur] = sw;
+                        cur = wi;
+                        fw = featw[wi];
+                        sw = 0;
+                    }
+                    sw |= 1ull << (k & 63);
+                    last = ns;
                     nb++;
-                    ow = col.region_base + lr;
+                    ns = min(ns + (((fw >> (k & 63)) & 1) ? grow : 1u), nreg);
                 }
-                own[k] = ow;
+                if (cur != NONE32) startw[cur] = sw;
             }
             next_s = ns;
             nb_s = nb;
-            cur_owner = ow;
+            last_s = last;
         }
         __syncthreads();
         for (uint32_t k = threadIdx.x; k < cn; k += blockDim.x) {
-            uint32_t lr = base + k, slot = col.region_base + lr;
+            const uint32_t lr = base + k, slot = col.region_base + lr;
             DBlock &b = a.blocks[slot];
-            uint32_t o = own[k];
+            if (lr >= nreg) {
+                a.owner[slot] = NONE32;
+                b.len = 0;
+                continue;
+            }
+            uint32_t o = NONE32;
+            for (uint32_t d = 0; d < grow && o == NONE32; d++) {
+                if (d > k) break;
+                const uint32_t kk = k - d;
+                if ((startw[kk >> 6] >> (kk & 63)) & 1) o = col.region_base + base + kk;
+            }
+            if (o == NONE32) o = col.region_base + prev_last;  // block begun in an earlier chunk
             a.owner[slot] = o;
             if (o == slot) {
-                uint64_t rs = (uint64_t)lr * R;
-                uint64_t end = (col.adaptive && feat[k]) ? min(rs + (uint64_t)QA_MAX_Q, nv)
-                                                         : min(rs + (uint64_t)R, nv);
+                const uint64_t rs = (uint64_t)lr * R;
+                const bool g = adaptive && ((featw[k >> 6] >> (k & 63)) & 1);
+                const uint64_t end = g ? min(rs + (uint64_t)QA_MAX_Q, nv) : min(rs + (uint64_t)R, nv);
                 b.start = rs;
                 b.len = (uint32_t)(end - rs);
                 b.col = (uint32_t)c;
@@ -352,66 +512,62 @@ __global__ void __launch_bounds__(256) k_plan(EncArgs a) {
 // ---------------------------------------------------------------------------
 // stage 4: features of blocks grown past one region (131072 values)
 // ---------------------------------------------------------------------------
+template <int LK, bool SG, bool FASTL>
+__device__ __forceinline__ void grown_samples(const void *lp, uint32_t lk, uint64_t s0, uint64_t n,
+                                              uint64_t *seen, uint64_t *mn, uint64_t *mx,
+                                              uint64_t *dz) {
+    probe_samples<LK>(lp, lk, s0, n, seen, mn, mx, dz);
+}
+
 __global__ void __launch_bounds__(256) k_grown_feats(EncArgs a) {
     __shared__ uint64_t seen[256];
     __shared__ uint64_t smin, smax, sdz;
-    __shared__ uint32_t dsum, emin_s, emax_s, eq_s;
+    __shared__ uint32_t dsum, eq_s;
     const uint32_t slot = blockIdx.x;
     DBlock &b = a.blocks[slot];
-    if (b.len == 0) return;
-    const DCol &col = a.cols[b.col];
-    const DColState &st = a.cst[b.col];
+    const uint32_t blen = b.len;
+    if (blen == 0) return;
+    const uint32_t bc = b.col;
+    const DCol &col = a.cols[bc];
     const uint32_t f = a.rfeat[slot];
-    if (b.len <= col.region_size) {  // a one-region block: its region's probe
+    if (blen <= col.region_size) {  // a one-region block: its region's probe
         if (threadIdx.x == 0) {
             b.distinct_low = (f & RF_DLOW) != 0;
             b.repeats = (f & RF_REP) != 0;
             b.incompressible = (f & RF_INC) != 0;
-            b.exp_range = f >> 8;
+            b.exp_range = 0;
         }
         return;
     }
-    const uint64_t n = b.len, s0 = b.start;
+    const void *lp = a.cst[bc].lanep;
+    const uint32_t lk = a.cst[bc].lkind;
+    const uint64_t n = blen, s0 = b.start;
+    const uint32_t bnt = b.ntiles;
     for (int k = threadIdx.x; k < 256; k += blockDim.x) seen[k] = 0;
     if (threadIdx.x == 0) {
-        uint64_t v0 = lane_at(st, s0);
+        uint64_t v0 = qa_lane_get(lp, lk, s0);
         smin = v0;
         smax = v0;
         sdz = 0;
         dsum = 0;
-        emin_s = 0xFFFFFFFFu;
-        emax_s = 0;
         eq_s = 0;
     }
     __syncthreads();
-    const uint64_t stride = n / 4096 > 1 ? n / 4096 : 1;
-    for (uint64_t k = (uint64_t)threadIdx.x * stride; k < n; k += (uint64_t)blockDim.x * stride) {
-        uint64_t h = lane_at(st, s0 + k) * 0x9E3779B97F4A7C15ull;
-        uint32_t bucket = (uint32_t)((h >> 50) & 0x3FFF);
-        atomicOr((unsigned long long *)&seen[bucket >> 6], 1ull << (bucket & 63));
-    }
-    const uint64_t step = n / 2048 > 1 ? n / 2048 : 1;
     uint64_t mn = ~0ull, mx = 0, dz = 0;
-    for (uint64_t i = step * (1 + threadIdx.x); i < n; i += step * blockDim.x) {
-        uint64_t x = lane_at(st, s0 + i);
-        mn = x < mn ? x : mn;
-        mx = x > mx ? x : mx;
-        uint64_t z = qa_zigzag(x - lane_at(st, s0 + i - 1));
-        dz = z > dz ? z : dz;
-    }
+    QA_LANE_DISPATCH(grown_samples, lk, false, false, lp, lk, s0, n, seen, &mn, &mx, &dz);
     atomicMin((unsigned long long *)&smin, mn);
     atomicMax((unsigned long long *)&smax, mx);
     atomicMax((unsigned long long *)&sdz, dz);
-    // exp range and consecutive-equal count combine from the tiles
+    // the consecutive-equal count combines from the tiles
     const DTile *tl = a.tiles + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
-    for (uint32_t t = threadIdx.x; t < b.ntiles; t += blockDim.x) {
+    uint32_t eq = 0;
+    for (uint32_t t = threadIdx.x; t < bnt; t += blockDim.x) {
         const DTile &ts = tl[t];
-        atomicMin(&emin_s, ts.exp_min);
-        atomicMax(&emax_s, ts.exp_max);
-        uint32_t e = ts.cnt - 1 - ts.chg_rest;
-        if (t > 0) e += ts.v_first == tl[t - 1].v_last;
-        atomicAdd(&eq_s, e);
+        eq += ts.cnt - 1 - ts.chg_rest;
+        if (t > 0) eq += ts.v_first == tl[t - 1].v_last;
     }
+    eq = (uint32_t)wave_reduce((uint64_t)eq, OpAdd());
+    if ((threadIdx.x & 63) == 0) atomicAdd(&eq_s, eq);
     __syncthreads();
     uint32_t d = 0;
     for (int k = threadIdx.x; k < 256; k += blockDim.x) d += (uint32_t)__popcll(seen[k]);
@@ -430,7 +586,7 @@ __global__ void __launch_bounds__(256) k_grown_feats(EncArgs a) {
         b.distinct_low = dl;
         b.repeats = rp;
         b.incompressible = inc;
-        b.exp_range = emax_s - emin_s;
+        b.exp_range = 0;
     }
 }
 
@@ -623,19 +779,25 @@ __global__ void k_emit_frames(EncArgs a) {
     }
 }
 
-__global__ void __launch_bounds__(64) k_emit_tiles(EncArgs a) {
-    // one wavefront per workgroup: every early exit below is wave-uniform, so
-    // the barriers are reached by the whole group
-    __shared__ uint64_t res[QA_TILE];
-    __shared__ uint32_t outw[2 * QA_TILE + 4];
-    const int lane = threadIdx.x;
-    const uint64_t t = blockIdx.x;
+// One wavefront per tile, four independent tiles per workgroup (LDS is
+// per-wavefront, so only wavefront-level synchronization is used). FastLanes
+// pack (src/bitpack.rs:47-133) straight from the lane: in the u32 form lane l
+// of the 32 packs rows [16 h, 16 h + 16) (h = half of the wavefront), in the
+// u64 form lane l of the 16 packs rows [16 q, 16 q + 16) (q = quarter); each
+// residual is ORed into its one or two LDS words.
+__global__ void __launch_bounds__(256) k_emit_tiles(EncArgs a) {
+    __shared__ uint64_t outs[4][QA_TILE + 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t t = (uint64_t)blockIdx.x * 4 + wv;
+    if (t >= a.ntiles) return;
     const DTile &ts = a.tiles[t];
     if (!ts.active) return;
     const int c = col_by_tile(a.cols, a.ncols, t);
     const DCol &col = a.cols[c];
     const DColState &st = a.cst[c];
     if (st.err) return;
+    const void *lp = st.lanep;
+    const uint32_t lk = st.lkind;
     const uint64_t lt = t - col.tile_base;
     const uint32_t lr = (uint32_t)(lt / col.tpr);
     const uint32_t o = a.owner[col.region_base + lr];
@@ -643,11 +805,22 @@ __global__ void __launch_bounds__(64) k_emit_tiles(EncArgs a) {
     const uint32_t bt = (col.region_base + lr - o) * col.tpr + (uint32_t)(lt % col.tpr);
     const uint64_t k0 = b.start + (uint64_t)bt * QA_TILE;
     const uint32_t cnt = ts.cnt;
+    const uint32_t mode = b.mode;
     uint8_t *p = col.dst + b.frame_off + 9;
-    if (b.mode == QA_M_RAW) {
+    uint32_t *outw = (uint32_t *)outs[wv];
+    if (mode == QA_M_RAW) {
         const uint32_t lb = (uint32_t)lane_bytes_of(col.dtype);
-        for (uint32_t k = lane; k < cnt; k += WAVE) {
-            uint64_t x = lane_at(st, k0 + k);
+        uint64_t xs[QA_TILE / WAVE];
+#pragma unroll
+        for (int i = 0; i < QA_TILE / WAVE; i++) {
+            const uint32_t k = i * WAVE + lane;
+            xs[i] = k < cnt ? qa_lane_get(lp, lk, k0 + k) : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < QA_TILE / WAVE; i++) {
+            const uint32_t k = i * WAVE + lane;
+            if (k >= cnt) break;
+            const uint64_t x = xs[i];
             if (col.dtype == QA_F32) outw[k] = qa_lane_to_f32(x);
             else if (lb == 4) outw[k] = (uint32_t)x;
             else {
@@ -655,40 +828,73 @@ __global__ void __launch_bounds__(64) k_emit_tiles(EncArgs a) {
                 outw[2 * k + 1] = (uint32_t)(x >> 32);
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         wave_store_bytes(p + (uint64_t)bt * QA_TILE * lb, outw, cnt * lb, lane);
         return;
     }
-    if (b.mode != QA_M_FOR && b.mode != QA_M_DELTA) return;
-    const bool sg = is_signed(col.dtype);
+    if (mode != QA_M_FOR && mode != QA_M_DELTA) return;
+    const bool sg = is_signed(col.dtype), delta = mode == QA_M_DELTA;
     uint64_t mn;
-    if (b.mode == QA_M_DELTA) {
+    if (delta) {
         uint64_t dmx;
         tile_dz(ts, bt == 0, &mn, &dmx);
     } else {
         mn = sg ? (uint64_t)ts.smin : ts.umin;
     }
-    for (uint32_t k = lane; k < QA_TILE; k += WAVE) {
-        uint64_t r = 0;
-        if (k < cnt) {
-            uint64_t x = lane_at(st, k0 + k);
-            if (b.mode == QA_M_DELTA) {
-                uint64_t pv = (bt == 0 && k == 0) ? x : lane_at(st, k0 + k - 1);
-                r = qa_zigzag(x - pv) - mn;
-            } else {
-                r = x - mn;
+    const uint32_t w = ts.w;
+    uint8_t *q = p + ts.off;
+    if (lane == 0) q[0] = (uint8_t)w;
+    if (lane >= 1 && lane < 9) q[lane] = (uint8_t)(mn >> (8 * (lane - 1)));
+    if (w == 0) return;
+    for (uint32_t i = lane; i < 32 * w; i += WAVE) outw[i] = 0;
+    wave_lds_sync();
+    // residual of tile value k: FoR x - min; Delta zigzag(x - prev) - min with
+    // the block's first value as its own predecessor (delta_bitpack.rs:23-46);
+    // zero past cnt (padding to 1024, for_bitpack.rs:60-72). The lane's 16
+    // values (and predecessors) are loaded before any is used.
+    const bool u32form = w <= 32;
+    const uint32_t l = u32form ? (lane & 31) : (lane & 15);
+    const uint32_t r0 = u32form ? 16 * (lane >> 5) : 16 * (lane >> 4);
+    const uint32_t L = u32form ? 32 : 16;
+    uint64_t xs[16], ps[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t k = (r0 + i) * L + l;
+        xs[i] = k < cnt ? qa_lane_get(lp, lk, k0 + k) : 0;
+        ps[i] = (delta && k < cnt && (bt != 0 || k != 0)) ? qa_lane_get(lp, lk, k0 + k - 1) : xs[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t k = (r0 + i) * L + l;
+        xs[i] = k >= cnt ? 0 : (delta ? qa_zigzag(xs[i] - ps[i]) - mn : xs[i] - mn);
+    }
+    if (u32form) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t r = r0 + i;
+            const uint32_t v = (uint32_t)xs[i];
+            const uint32_t bit = r * w, word = bit >> 5, sh = bit & 31;
+            if (v) {
+                atomicOr(&outw[word * 32 + l], v << sh);
+                if (sh + w > 32) atomicOr(&outw[(word + 1) * 32 + l], v >> (32 - sh));
             }
         }
-        res[k] = r;
+    } else {
+        unsigned long long *o64 = (unsigned long long *)outs[wv];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t r = r0 + i;
+            const uint64_t v = xs[i];
+            const uint32_t bit = r * w, word = bit >> 6, sh = bit & 63;
+            if (v) {
+                atomicOr(&o64[word * 16 + l], (unsigned long long)(v << sh));
+                if (sh + w > 64)
+                    atomicOr(&o64[(word + 1) * 16 + l], (unsigned long long)(v >> (64 - sh)));
+            }
+        }
     }
-    __syncthreads();
-    uint8_t *q = p + ts.off;
-    if (lane == 0) q[0] = (uint8_t)ts.w;
-    if (lane >= 1 && lane < 9) q[lane] = (uint8_t)(mn >> (8 * (lane - 1)));
-    if (ts.w == 0) return;
-    pack_lds(res, ts.w, outw, lane);
-    __syncthreads();
-    wave_store_bytes(q + 9, outw, 128 * ts.w, lane);
+    wave_lds_sync();
+    wave_store_bytes(q + 9, outw, 128 * w, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -709,7 +915,7 @@ void qa_encode_launch(const EncArgs &a, hipStream_t s, int) {
         qa_exclusive_scan(a.tpre, a.nvwords, a.tpre, a.scan_tmp, s);
     }
     QA_LAUNCH(k_col_counts, dim3(nb(a.ncols, 64)), dim3(64), 0, s, a);
-    if (a.nvwords) QA_LAUNCH(k_compact, dim3(nb(a.nvwords, 16)), dim3(256), 0, s, a);
+    if (a.nvwords) QA_LAUNCH(k_compact, dim3(nb(a.nvwords, 4 * WAVE)), dim3(256), 0, s, a);
     QA_LAUNCH(k_region_stats, dim3(a.nregions), dim3(1024), 0, s, a);
     QA_LAUNCH(k_plan, dim3(a.ncols), dim3(256), 0, s, a);
     QA_LAUNCH(k_grown_feats, dim3(a.nregions), dim3(256), 0, s, a);
@@ -730,6 +936,6 @@ void qa_encode_launch(const EncArgs &a, hipStream_t s, int) {
         QA_LAUNCH(k_emit_vraw, dim3(nb(a.nvwords * 8, 256)), dim3(256), 0, s, a);
     }
     QA_LAUNCH(k_emit_frames, dim3(nb(a.nregions, 64)), dim3(64), 0, s, a);
-    QA_LAUNCH(k_emit_tiles, dim3((unsigned)a.ntiles), dim3(64), 0, s, a);
+    QA_LAUNCH(k_emit_tiles, dim3(nb(a.ntiles, 4)), dim3(256), 0, s, a);
     qa_fast_emit_launch(a, s);
 }

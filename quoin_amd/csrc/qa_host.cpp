@@ -425,7 +425,7 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     std::vector<ParsedStream> ps(ns);
     std::vector<DDecCol> cols(ns);
     std::vector<DFrame> frames;
-    uint64_t tile_base = 0, lane_off = 0, vword_base = 0, val_base = 0, vbyte_base = 0;
+    uint64_t tile_base = 0, vword_base = 0, val_base = 0, vbyte_base = 0, avbyte_base = 0;
     uint64_t scr_base = 0;
     uint32_t any_fast = 0;
     for (size_t c = 0; c < ns; c++) {
@@ -445,16 +445,17 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         d.n = q.n;
         d.dtype = dt;
         d.out_kind = kind;
-        d.lane_off = lane_off;
         d.val_base = val_base;
         d.vword_base = vword_base;
         d.vbyte_base = vbyte_base;
+        d.avbyte_base = avbyte_base;
         if (ps[c].has_validity) {
             d.nvwords = (uint32_t)((q.n + 63) / 64);
             d.vblob_off = ps[c].vblob_off;
             d.vblob_len = ps[c].vblob_len;
-            lane_off += (q.n + QA_TILE - 1) / QA_TILE * QA_TILE;
             if (d.vblob_len > 1) vbyte_base += d.vblob_len - 1;
+        } else if (q.validity_out) {
+            avbyte_base += (q.n + 7) / 8;  // all valid (src/lib.rs:503-506)
         }
         vword_base += d.nvwords;
         val_base += q.n;
@@ -475,7 +476,7 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     uint64_t nscan = std::max<uint64_t>(vword_base, vbyte_base) + 1;
     if (!ensure(g->dcols, ns * sizeof(DDecCol)) || !ensure(g->dstate, ns * sizeof(DDecState)) ||
         !ensure(g->frames, frames.size() * sizeof(DFrame)) ||
-        !ensure(g->dtiles, tile_base * sizeof(DTile)) || !ensure(g->dlane, lane_off * 8) ||
+        !ensure(g->dtiles, tile_base * sizeof(DTileD)) ||
         !ensure(g->dvpre, (vword_base + 1) * 8) || !ensure(g->derr, 8) ||
         !ensure(g->dbitmaps, (vword_base + 1) * 8) || !ensure(g->dvbpre, (vbyte_base + 1) * 8) ||
         !ensure(g->dvruns, (vbyte_base + 1) * 8) ||
@@ -499,11 +500,11 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     a.nvals = val_base;
     a.nvwords = vword_base;
     a.nvbytes = vbyte_base;
-    a.lane = dp<uint64_t>(g->dlane);
+    a.navbytes = avbyte_base;
     a.vpre = dp<uint64_t>(g->dvpre);
     a.vbpre = dp<uint64_t>(g->dvbpre);
     a.vruns = dp<uint64_t>(g->dvruns);
-    a.tiles = dp<DTile>(g->dtiles);
+    a.tiles = dp<DTileD>(g->dtiles);
     a.err = dp<uint32_t>(g->derr);
     a.scan_tmp = dp<uint64_t>(g->dscan);
     if (any_fast) {

@@ -73,6 +73,16 @@ struct DFrameX {
     uint32_t scale, pad;   // FloatMult scale index
 };
 
+// per 1024-value decode tile
+struct DTileD {
+    uint64_t off;    // stream offset of the tile's sub-block header (FoR-family modes, ALP)
+    uint64_t tail;   // DeltaBitpack: tile sum, then carry; ALP-RD rights / Dict values /
+                     // RLE lengths: sub-block header of the second for_bitpack walk
+    uint64_t lpos;   // streams with nulls: logical position of the tile's first value
+    uint32_t frame;  // owning frame
+    uint32_t w;      // FastLanes width of the sub-block at off
+};
+
 struct DDecCol {
     const uint8_t *src;     // device stream
     void *out;              // typed output
@@ -80,12 +90,12 @@ struct DDecCol {
     uint64_t *bitmap;       // library-owned decoded bitmap (words), or null
     uint64_t src_len;
     uint64_t n;             // logical values
-    uint64_t lane_off;      // scratch lane offset (when nulls)
     uint64_t vword_base;    // rank prefix slots
     uint64_t val_base;
     uint64_t vblob_off;     // stream offset of the validity blob (its tag byte)
     uint64_t vblob_len;     // blob bytes including the tag
     uint64_t vbyte_base;    // global slot of the blob's first byte after the tag
+    uint64_t avbyte_base;   // all-valid validity_out bytes before this column
     uint32_t dtype, nvwords;
     uint32_t out_kind;      // QA_IN_* narrow output element type
     uint32_t fbegin, fend;  // this column's frames
@@ -110,11 +120,11 @@ struct DecArgs {
     uint64_t nvals;
     uint64_t nvwords;
     uint64_t nvbytes;      // total validity blob bytes (after tags)
-    uint64_t *lane;        // compact scratch for columns with nulls
+    uint64_t navbytes;     // validity_out bytes of streams without a bitmap
     uint64_t *vpre;        // rank prefix (nvwords + 1)
     uint64_t *vbpre;       // varint-end prefix over blob bytes (nvbytes + 1)
     uint64_t *vruns;       // run lengths, then their prefix (nvbytes + 1)
-    DTile *tiles;          // per decode tile
+    DTileD *tiles;         // per decode tile
     uint32_t *err;         // error flags
     uint64_t *scan_tmp;
     DFrameX *fx;           // per frame
@@ -125,4 +135,4 @@ struct DecArgs {
 
 void qa_decode_launch(const DecArgs &a, hipStream_t s);
 void qa_fast_dec_index_launch(const DecArgs &a, hipStream_t s);
-void qa_fast_dec_tiles_launch(const DecArgs &a, hipStream_t s);
+void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s);
