@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--level", default="fastest")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="steps of the plain form (one launch sequence per block); 0 skips it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU in this build; --gpus {args.gpus} is not supported "
@@ -139,6 +141,22 @@ def main():
         td += t2 - t1
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
+
+    # the plain form on the same columns: one launch sequence per block
+    plain = None
+    if args.plain_steps > 0:
+        gpu.set_launch_form(1)
+        gpu.compress(dcols, cfg)
+        gpu.decompress(dsts)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for _ in range(args.plain_steps):
+            gpu.compress(dcols, cfg)
+            gpu.decompress(dsts)
+        torch.cuda.synchronize()
+        plain_wall = time.perf_counter() - tp
+        gpu.set_launch_form(0)
+        plain = raw_bytes * args.plain_steps / plain_wall / 1e9
     ms_step = wall / args.steps * 1e3
     value = raw_bytes * args.steps / wall / 1e9  # GB/s of uncompressed data, round trip
     c_gbs = raw_bytes * args.steps / tc / 1e9
@@ -172,7 +190,9 @@ def main():
                    "uncompressed_bytes": raw_bytes, "compressed_bytes": comp_bytes,
                    "compress_gbs": round(c_gbs, 2), "decompress_gbs": round(d_gbs, 2),
                    "device_ms_compress": round(dev_c_ms, 3),
-                   "device_ms_decompress": round(dev_d_ms, 3)},
+                   "device_ms_decompress": round(dev_d_ms, 3),
+                   "plain_form_gbs": round(plain, 3) if plain else None,
+                   "batched_over_plain": round(value / plain, 2) if plain else None},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

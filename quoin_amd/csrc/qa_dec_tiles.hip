@@ -66,8 +66,8 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
     __shared__ uint64_t lds[4][QA_TILE];
     if (*a.err) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint64_t t = (uint64_t)blockIdx.x * 4 + wv;
-    if (t >= a.ntiles) return;
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
+    if (t >= a.t1) return;
     const DTileD ts = a.tiles[t];
     const uint32_t f = ts.frame;
     const DFrame &fr = a.frames[f];
@@ -313,6 +313,6 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
 static inline unsigned nbt(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
-    if (!a.ntiles) return;
-    hipLaunchKernelGGL(k_dec_tiles, dim3(nbt(a.ntiles, 4)), dim3(256), 0, s, a);
+    if (a.t1 <= a.t0) return;
+    hipLaunchKernelGGL(k_dec_tiles, dim3(nbt(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
 }

@@ -177,8 +177,8 @@ __global__ void k_dec_colfin(DecArgs a) {
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_dec_tilemap(DecArgs a) {
     const int lane = threadIdx.x & 63;
-    uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (f >= a.nframes) return;
+    uint32_t f = a.f0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (f >= a.f1) return;
     const DFrame &fr = a.frames[f];
     for (uint32_t t = lane; t < fr.ntiles; t += WAVE) {
         DTileD d;
@@ -193,8 +193,8 @@ __global__ void __launch_bounds__(256) k_dec_tilemap(DecArgs a) {
 
 __global__ void __launch_bounds__(256) k_dec_index(DecArgs a) {
     const int lane = threadIdx.x & 63;
-    uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (f >= a.nframes || *a.err || !frame_used(a, f)) return;
+    uint32_t f = a.f0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (f >= a.f1 || *a.err || !frame_used(a, f)) return;
     const DFrame &fr = a.frames[f];
     const DDecCol &col = a.cols[fr.col];
     const uint8_t *s = col.src;
@@ -236,8 +236,8 @@ __global__ void __launch_bounds__(256) k_dec_index(DecArgs a) {
 // stage 4: logical start of each tile of a stream with nulls
 // ---------------------------------------------------------------------------
 __global__ void k_dec_lpos(DecArgs a) {
-    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.ntiles || *a.err) return;
+    uint64_t t = a.t0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.t1 || *a.err) return;
     DTileD &d = a.tiles[t];
     const uint32_t f = d.frame;
     if (!frame_used(a, f)) return;
@@ -280,8 +280,8 @@ __global__ void __launch_bounds__(256) k_dec_delta_sums(DecArgs a) {
     __shared__ uint32_t wbuf[4][2 * QA_TILE + 4];
     if (*a.err) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    uint64_t t = (uint64_t)blockIdx.x * 4 + wv;
-    if (t >= a.ntiles) return;
+    uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
+    if (t >= a.t1) return;
     DTileD &ts = a.tiles[t];
     const uint32_t f = ts.frame;
     const DFrame &fr = a.frames[f];
@@ -304,8 +304,8 @@ __global__ void __launch_bounds__(256) k_dec_delta_sums(DecArgs a) {
 // one wavefront per frame: exclusive scan of the tile sums, seeded with the base
 __global__ void __launch_bounds__(256) k_dec_delta_scan(DecArgs a) {
     const int lane = threadIdx.x & 63;
-    uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (f >= a.nframes || *a.err) return;
+    uint32_t f = a.f0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (f >= a.f1 || *a.err) return;
     const DFrame &fr = a.frames[f];
     if (fr.mode != QA_M_DELTA || !frame_used(a, f)) return;
     uint64_t carry = qa_ld64u(a.cols[fr.col].src + fr.payload_off);  // base
@@ -352,7 +352,9 @@ static inline unsigned nbk(uint64_t n, unsigned b) { return (unsigned)((n + b - 
         if (g_.x) hipLaunchKernelGGL(k, g_, __VA_ARGS__);         \
     } while (0)
 
-void qa_decode_launch(const DecArgs &a, hipStream_t s) {
+// column-level: validity blob -> bitmap (+ validity_out), valid counts, the
+// frames each stream's decoder reads
+void qa_decode_prep_launch(const DecArgs &a, hipStream_t s) {
     if (a.nvbytes) {
         QA_LAUNCH(k_vb_ends, dim3(nbk(a.nvbytes, 256)), dim3(256), 0, s, a);
         qa_exclusive_scan(a.vbpre, a.nvbytes, a.vbpre, a.scan_tmp, s);
@@ -366,15 +368,41 @@ void qa_decode_launch(const DecArgs &a, hipStream_t s) {
         qa_exclusive_scan(a.vpre, a.nvwords, a.vpre, a.scan_tmp, s);
     }
     QA_LAUNCH(k_dec_colfin, dim3(nbk(a.ncols, 64)), dim3(64), 0, s, a);
-    if (a.nframes) {
-        QA_LAUNCH(k_dec_tilemap, dim3(nbk(a.nframes, 4)), dim3(256), 0, s, a);
-        QA_LAUNCH(k_dec_index, dim3(nbk(a.nframes, 4)), dim3(256), 0, s, a);
-        qa_fast_dec_index_launch(a, s);
-        if (a.nvwords) QA_LAUNCH(k_dec_lpos, dim3(nbk(a.ntiles, 256)), dim3(256), 0, s, a);
-        QA_LAUNCH(k_dec_delta_sums, dim3(nbk(a.ntiles, 4)), dim3(256), 0, s, a);
-        QA_LAUNCH(k_dec_delta_scan, dim3(nbk(a.nframes, 4)), dim3(256), 0, s, a);
-        qa_dec_tiles_launch(a, s);
-    }
+}
+
+// per frame [f0, f1): tile map, sub-block index, Fast-pool frame stages
+void qa_decode_index_launch(const DecArgs &a, hipStream_t s) {
+    if (a.f1 <= a.f0) return;
+    QA_LAUNCH(k_dec_tilemap, dim3(nbk(a.f1 - a.f0, 4)), dim3(256), 0, s, a);
+    QA_LAUNCH(k_dec_index, dim3(nbk(a.f1 - a.f0, 4)), dim3(256), 0, s, a);
+    qa_fast_dec_index_launch(a, s);
+}
+
+// tiles [t0, t1) of streams with nulls: logical start of each tile
+void qa_decode_lpos_launch(const DecArgs &a, hipStream_t s) {
+    if (a.nvwords && a.t1 > a.t0)
+        QA_LAUNCH(k_dec_lpos, dim3(nbk(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
+}
+
+// per frame [f0, f1) / tile [t0, t1): Delta carries, tile decode
+void qa_decode_values_launch(const DecArgs &a, hipStream_t s) {
+    if (a.t1 <= a.t0) return;
+    QA_LAUNCH(k_dec_delta_sums, dim3(nbk(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
+    QA_LAUNCH(k_dec_delta_scan, dim3(nbk(a.f1 - a.f0, 4)), dim3(256), 0, s, a);
+    qa_dec_tiles_launch(a, s);
+}
+
+// all-null streams, validity_out of streams without a bitmap
+void qa_decode_final_launch(const DecArgs &a, hipStream_t s) {
     if (a.nvwords) QA_LAUNCH(k_dec_nullfill, dim3(nbk(a.nvwords, 256)), dim3(256), 0, s, a);
     QA_LAUNCH(k_dec_allvalid, dim3(nbk(a.navbytes, 256)), dim3(256), 0, s, a);
+}
+
+// the batched form: every frame of every stream in one launch per stage
+void qa_decode_launch(const DecArgs &a, hipStream_t s) {
+    qa_decode_prep_launch(a, s);
+    qa_decode_index_launch(a, s);
+    qa_decode_lpos_launch(a, s);
+    qa_decode_values_launch(a, s);
+    qa_decode_final_launch(a, s);
 }

@@ -523,7 +523,8 @@ __global__ void __launch_bounds__(256) k_grown_feats(EncArgs a) {
     __shared__ uint64_t seen[256];
     __shared__ uint64_t smin, smax, sdz;
     __shared__ uint32_t dsum, eq_s;
-    const uint32_t slot = blockIdx.x;
+    const uint32_t slot = a.r0 + blockIdx.x;
+    if (slot >= a.r1) return;
     DBlock &b = a.blocks[slot];
     const uint32_t blen = b.len;
     if (blen == 0) return;
@@ -753,8 +754,8 @@ __global__ void k_emit_vraw(EncArgs a) {
 }
 
 __global__ void k_emit_frames(EncArgs a) {
-    uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= a.nregions) return;
+    uint32_t slot = a.r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= a.r1) return;
     const DBlock &b = a.blocks[slot];
     if (b.len == 0) return;
     const DCol &col = a.cols[b.col];
@@ -788,8 +789,8 @@ __global__ void k_emit_frames(EncArgs a) {
 __global__ void __launch_bounds__(256) k_emit_tiles(EncArgs a) {
     __shared__ uint64_t outs[4][QA_TILE + 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint64_t t = (uint64_t)blockIdx.x * 4 + wv;
-    if (t >= a.ntiles) return;
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
+    if (t >= a.t1) return;
     const DTile &ts = a.tiles[t];
     if (!ts.active) return;
     const int c = col_by_tile(a.cols, a.ncols, t);
@@ -908,7 +909,8 @@ static inline unsigned nb(uint64_t n, unsigned b) { return (unsigned)((n + b - 1
         if (g_.x) hipLaunchKernelGGL(k, g_, __VA_ARGS__);         \
     } while (0)
 
-void qa_encode_launch(const EncArgs &a, hipStream_t s, int) {
+// column-level stages: validity counts, compaction, region statistics, plan
+void qa_encode_plan_launch(const EncArgs &a, hipStream_t s) {
     if (a.nvwords) {
         QA_LAUNCH(k_vcount, dim3(nb(a.nvwords, 256)), dim3(256), 0, s, a);
         qa_exclusive_scan(a.vpre, a.nvwords, a.vpre, a.scan_tmp, s);
@@ -918,9 +920,18 @@ void qa_encode_launch(const EncArgs &a, hipStream_t s, int) {
     if (a.nvwords) QA_LAUNCH(k_compact, dim3(nb(a.nvwords, 4 * WAVE)), dim3(256), 0, s, a);
     QA_LAUNCH(k_region_stats, dim3(a.nregions), dim3(1024), 0, s, a);
     QA_LAUNCH(k_plan, dim3(a.ncols), dim3(256), 0, s, a);
-    QA_LAUNCH(k_grown_feats, dim3(a.nregions), dim3(256), 0, s, a);
+}
+
+// per-block stages over region slots [r0, r1) and tiles [t0, t1): features of
+// grown blocks, candidate sizes, the competition
+void qa_encode_blocks_launch(const EncArgs &a, hipStream_t s) {
+    QA_LAUNCH(k_grown_feats, dim3(a.r1 - a.r0), dim3(256), 0, s, a);
     qa_fast_sizes_launch(a, s);
     qa_fast_final_launch(a, s);
+}
+
+// column-level stages: validity blob, frame layout, header
+void qa_encode_layout_launch(const EncArgs &a, hipStream_t s) {
     uint64_t nruns_max = a.ntrans_max + a.ncols;
     if (a.nvwords) {
         QA_LAUNCH(k_vtrans_pos, dim3(nb(a.nvwords, 256)), dim3(256), 0, s, a);
@@ -935,7 +946,19 @@ void qa_encode_launch(const EncArgs &a, hipStream_t s, int) {
         QA_LAUNCH(k_emit_vruns, dim3(g), dim3(256), 0, s, a);
         QA_LAUNCH(k_emit_vraw, dim3(nb(a.nvwords * 8, 256)), dim3(256), 0, s, a);
     }
-    QA_LAUNCH(k_emit_frames, dim3(nb(a.nregions, 64)), dim3(64), 0, s, a);
-    QA_LAUNCH(k_emit_tiles, dim3(nb(a.ntiles, 4)), dim3(256), 0, s, a);
+}
+
+// per-block stages over [r0, r1) / [t0, t1): frame headers and payloads
+void qa_encode_emit_launch(const EncArgs &a, hipStream_t s) {
+    QA_LAUNCH(k_emit_frames, dim3(nb(a.r1 - a.r0, 64)), dim3(64), 0, s, a);
+    QA_LAUNCH(k_emit_tiles, dim3(nb(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
     qa_fast_emit_launch(a, s);
+}
+
+// the batched form: every block of every column in one launch per stage
+void qa_encode_launch(const EncArgs &a, hipStream_t s, int) {
+    qa_encode_plan_launch(a, s);
+    qa_encode_blocks_launch(a, s);
+    qa_encode_layout_launch(a, s);
+    qa_encode_emit_launch(a, s);
 }

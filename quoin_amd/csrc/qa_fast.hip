@@ -87,8 +87,8 @@ __device__ __forceinline__ bool tile_owner(const EncArgs &a, uint64_t t, int *cp
 __global__ void __launch_bounds__(256) k_float_tiles(EncArgs a) {
     __shared__ uint64_t samp[4][32];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint64_t t = (uint64_t)blockIdx.x * 4 + wv;
-    if (t >= a.ntiles) return;
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
+    if (t >= a.t1) return;
     const DTile &ts = a.tiles[t];
     if (!ts.active) return;
     int c;
@@ -251,7 +251,8 @@ __device__ __forceinline__ bool fast_float_block(const EncArgs &a, const DBlock 
 __global__ void __launch_bounds__(1024) k_alprd_cut(EncArgs a) {
     __shared__ uint64_t s[4096];
     __shared__ double bits_of[24];
-    const uint32_t slot = blockIdx.x;
+    const uint32_t slot = a.r0 + blockIdx.x;
+    if (slot >= a.r1) return;
     const DBlock &b = a.blocks[slot];
     if (!fast_float_block(a, b)) return;
     DBlockX &x = a.blocksx[slot];
@@ -362,7 +363,8 @@ __global__ void __launch_bounds__(1024) k_alprd_hist(EncArgs a) {
     __shared__ uint64_t keys[RDH_CAP];
     __shared__ uint32_t cnt[RDH_CAP];
     __shared__ uint32_t ndist, overflow;
-    const uint32_t slot = blockIdx.x;
+    const uint32_t slot = a.r0 + blockIdx.x;
+    if (slot >= a.r1) return;
     const DBlock &b = a.blocks[slot];
     if (!fast_float_block(a, b)) return;
     DBlockX &x = a.blocksx[slot];
@@ -446,8 +448,8 @@ __global__ void __launch_bounds__(1024) k_alprd_hist(EncArgs a) {
 // per tile: codes / rights ranges and the exceptions (alp_rd.rs:145-163)
 __global__ void __launch_bounds__(256) k_alprd_tiles(EncArgs a) {
     const int lane = threadIdx.x & 63;
-    const uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (t >= a.ntiles) return;
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= a.t1) return;
     const DTile &ts = a.tiles[t];
     if (!ts.active) return;
     int c;
@@ -517,8 +519,8 @@ __device__ __forceinline__ bool consider(uint64_t sz, uint32_t m, uint64_t lambd
 
 __global__ void __launch_bounds__(256) k_block_decide(EncArgs a) {
     const int lane = threadIdx.x & 63;
-    const uint32_t slot = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (slot >= a.nregions) return;
+    const uint32_t slot = a.r0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (slot >= a.r1) return;
     DBlock &b = a.blocks[slot];
     if (b.len == 0) return;
     const DCol &col = a.cols[b.col];
@@ -696,7 +698,8 @@ __device__ void dict_values_size(const EncArgs &a, uint32_t slot, const uint64_t
 __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
     __shared__ uint64_t keys[DH_CAP];
     __shared__ uint32_t card_l, overflow, has_max;
-    const uint32_t slot = blockIdx.x;
+    const uint32_t slot = a.r0 + blockIdx.x;
+    if (slot >= a.r1) return;
     const DBlock &b = a.blocks[slot];
     if (!dict_candidate(a, b)) return;
     DBlockX &x = a.blocksx[slot];
@@ -864,8 +867,8 @@ __device__ __forceinline__ uint64_t rank_of(const uint64_t *d, uint64_t card, ui
 
 // code range per tile: ranks of the tile's smallest and largest value
 __global__ void k_dict_tiles(EncArgs a) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.ntiles) return;
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.t1) return;
     const DTile &ts = a.tiles[t];
     if (!ts.active) return;
     int c;
@@ -890,8 +893,8 @@ __device__ __forceinline__ bool rle_candidate(const EncArgs &a, const DBlock &b,
 
 __global__ void __launch_bounds__(256) k_rle_starts(EncArgs a) {
     const int lane = threadIdx.x & 63;
-    const uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (t >= a.ntiles) return;
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= a.t1) return;
     const DTile &ts = a.tiles[t];
     if (!ts.active) return;
     int c;
@@ -919,7 +922,8 @@ __global__ void __launch_bounds__(256) k_rle_starts(EncArgs a) {
 
 __global__ void __launch_bounds__(256) k_rle_chunks(EncArgs a) {
     __shared__ unsigned long long tot;
-    const uint32_t slot = blockIdx.x;
+    const uint32_t slot = a.r0 + blockIdx.x;
+    if (slot >= a.r1) return;
     const DBlock &b = a.blocks[slot];
     if (!rle_candidate(a, b, slot)) return;
     DBlockX &x = a.blocksx[slot];
@@ -985,8 +989,8 @@ __device__ __forceinline__ void wave_offsets(uint32_t count, uint64_t carry, int
 
 __global__ void __launch_bounds__(256) k_block_final(EncArgs a) {
     const int lane = threadIdx.x & 63;
-    const uint32_t slot = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (slot >= a.nregions) return;
+    const uint32_t slot = a.r0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (slot >= a.r1) return;
     DBlock &b = a.blocks[slot];
     if (b.len == 0) return;
     const DCol &col = a.cols[b.col];
@@ -1107,21 +1111,21 @@ static inline unsigned nbf(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 // before k_block_decide: the float-value codecs' statistics; then the decision
 void qa_fast_sizes_launch(const EncArgs &a, hipStream_t s) {
     if (a.fast) {
-        QA_LAUNCH(k_float_tiles, dim3(nbf(a.ntiles, 4)), dim3(256), 0, s, a);
-        QA_LAUNCH(k_alprd_cut, dim3(a.nregions), dim3(1024), 0, s, a);
-        QA_LAUNCH(k_alprd_hist, dim3(a.nregions), dim3(1024), 0, s, a);
-        QA_LAUNCH(k_alprd_tiles, dim3(nbf(a.ntiles, 4)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_float_tiles, dim3(nbf(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_alprd_cut, dim3(a.r1 - a.r0), dim3(1024), 0, s, a);
+        QA_LAUNCH(k_alprd_hist, dim3(a.r1 - a.r0), dim3(1024), 0, s, a);
+        QA_LAUNCH(k_alprd_tiles, dim3(nbf(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
     }
-    QA_LAUNCH(k_block_decide, dim3(nbf(a.nregions, 4)), dim3(256), 0, s, a);
+    QA_LAUNCH(k_block_decide, dim3(nbf(a.r1 - a.r0, 4)), dim3(256), 0, s, a);
 }
 
 // Dict / RLE sizing, then the final decision and layout
 void qa_fast_final_launch(const EncArgs &a, hipStream_t s) {
     if (a.fast) {
-        QA_LAUNCH(k_dict_set, dim3(a.nregions), dim3(1024), 0, s, a);
-        QA_LAUNCH(k_dict_tiles, dim3(nbf(a.ntiles, 256)), dim3(256), 0, s, a);
-        QA_LAUNCH(k_rle_starts, dim3(nbf(a.ntiles, 4)), dim3(256), 0, s, a);
-        QA_LAUNCH(k_rle_chunks, dim3(a.nregions), dim3(256), 0, s, a);
+        QA_LAUNCH(k_dict_set, dim3(a.r1 - a.r0), dim3(1024), 0, s, a);
+        QA_LAUNCH(k_dict_tiles, dim3(nbf(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_rle_starts, dim3(nbf(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_rle_chunks, dim3(a.r1 - a.r0), dim3(256), 0, s, a);
     }
-    QA_LAUNCH(k_block_final, dim3(nbf(a.nregions, 4)), dim3(256), 0, s, a);
+    QA_LAUNCH(k_block_final, dim3(nbf(a.r1 - a.r0, 4)), dim3(256), 0, s, a);
 }

@@ -16,8 +16,8 @@
 // control flow); the sub-block walks use the whole wavefront.
 __global__ void __launch_bounds__(256) k_dec_index_fast(DecArgs a) {
     const int lane = threadIdx.x & 63;
-    uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (f >= a.nframes || *a.err || !frame_used(a, f)) return;
+    uint32_t f = a.f0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (f >= a.f1 || *a.err || !frame_used(a, f)) return;
     const DFrame &fr = a.frames[f];
     if (!fast_mode(fr.mode) || fr.mode == QA_M_XORZ) return;
     const DDecCol &col = a.cols[fr.col];
@@ -183,8 +183,8 @@ __device__ __forceinline__ uint64_t for_value_at(const uint8_t *h, uint32_t k) {
 }
 
 __global__ void __launch_bounds__(1024) k_dec_frame_wg(DecArgs a) {
-    const uint32_t f = blockIdx.x;
-    if (*a.err || f >= a.nframes || !frame_used(a, f)) return;
+    const uint32_t f = a.f0 + blockIdx.x;
+    if (*a.err || f >= a.f1 || !frame_used(a, f)) return;
     const DFrame &fr = a.frames[f];
     if (fr.mode != QA_M_XORZ && fr.mode != QA_M_DICT && fr.mode != QA_M_RLE) return;
     const DDecCol &col = a.cols[fr.col];
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(1024) k_dec_frame_wg(DecArgs a) {
 static inline unsigned nbd(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 void qa_fast_dec_index_launch(const DecArgs &a, hipStream_t s) {
-    if (!a.fast || !a.nframes) return;
-    hipLaunchKernelGGL(k_dec_index_fast, dim3(nbd(a.nframes, 4)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_dec_frame_wg, dim3(a.nframes), dim3(1024), 0, s, a);
+    if (!a.fast || a.f1 <= a.f0) return;
+    hipLaunchKernelGGL(k_dec_index_fast, dim3(nbd(a.f1 - a.f0, 4)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_dec_frame_wg, dim3(a.f1 - a.f0), dim3(1024), 0, s, a);
 }

@@ -71,8 +71,8 @@ __device__ __forceinline__ void emit_subblock(uint8_t *q, uint64_t mn, uint32_t 
 
 // ---------------------------------------------------------------------------
 __global__ void k_emit_fast_frames(EncArgs a) {
-    uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= a.nregions) return;
+    uint32_t slot = a.r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= a.r1) return;
     const DBlock &b = a.blocks[slot];
     if (b.len == 0 || a.cst[b.col].err) return;
     const DCol &col = a.cols[b.col];
@@ -132,7 +132,8 @@ __global__ void __launch_bounds__(64) k_emit_fast_tiles(EncArgs a) {
     __shared__ uint32_t outw[2 * QA_TILE + 4];
     __shared__ uint32_t bytes[(11 * QA_TILE + 16) / 4 + 4];
     const int lane = threadIdx.x;
-    const uint64_t t = blockIdx.x;
+    const uint64_t t = a.t0 + blockIdx.x;
+    if (t >= a.t1) return;
     const DTile &ts = a.tiles[t];
     if (!ts.active) return;
     int c;
@@ -323,7 +324,8 @@ __global__ void __launch_bounds__(64) k_emit_fast_chunks(EncArgs a) {
     __shared__ uint64_t res[QA_TILE];
     __shared__ uint32_t outw[2 * QA_TILE + 4];
     const int lane = threadIdx.x;
-    const uint64_t t = blockIdx.x;
+    const uint64_t t = a.t0 + blockIdx.x;
+    if (t >= a.t1) return;
     int c;
     uint32_t o, bt;
     if (!owner_of(a, t, &c, &o, &bt)) return;
@@ -385,10 +387,10 @@ static inline unsigned nbe(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 
 void qa_fast_emit_launch(const EncArgs &a, hipStream_t s) {
     if (!a.fast) return;
-    if (a.nregions)
-        hipLaunchKernelGGL(k_emit_fast_frames, dim3(nbe(a.nregions, 64)), dim3(64), 0, s, a);
-    if (a.ntiles) {
-        hipLaunchKernelGGL(k_emit_fast_tiles, dim3((unsigned)a.ntiles), dim3(64), 0, s, a);
-        hipLaunchKernelGGL(k_emit_fast_chunks, dim3((unsigned)a.ntiles), dim3(64), 0, s, a);
+    if (a.r1 > a.r0)
+        hipLaunchKernelGGL(k_emit_fast_frames, dim3(nbe(a.r1 - a.r0, 64)), dim3(64), 0, s, a);
+    if (a.t1 > a.t0) {
+        hipLaunchKernelGGL(k_emit_fast_tiles, dim3((unsigned)(a.t1 - a.t0)), dim3(64), 0, s, a);
+        hipLaunchKernelGGL(k_emit_fast_chunks, dim3((unsigned)(a.t1 - a.t0)), dim3(64), 0, s, a);
     }
 }

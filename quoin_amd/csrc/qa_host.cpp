@@ -365,6 +365,10 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     a.tiles = dp<DTile>(g->tiles);
     a.ntrans_max = ntrans;
     a.wins = dp<unsigned long long>(g->wins);
+    a.r0 = 0;
+    a.r1 = region_base;
+    a.t0 = 0;
+    a.t1 = tile_base;
     if (level == QUOIN_LEVEL_FAST) {
         // Level::Fast scratch: per-tile / per-block codec state and the
         // value-slot arrays of Dict / RLE / the hash fallbacks
@@ -387,7 +391,42 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     hipEventRecord(g->ev0, s);
     zero_async(g->cst.p, ncols * sizeof(DColState), s);
     zero_async(g->wins.p, 64 * 8, s);
-    qa_encode_launch(a, s, level);
+    if (g->form == 0) {
+        qa_encode_launch(a, s, level);
+    } else {
+        // plain form: column-level planning, then one launch sequence per
+        // block for its competition and again for its payload
+        qa_encode_plan_launch(a, s);
+        std::vector<uint32_t> owner(region_base);
+        if (region_base &&
+            hipMemcpyAsync(owner.data(), g->owner.p, region_base * 4ull, hipMemcpyDeviceToHost, s) !=
+                hipSuccess)
+            return QUOIN_ERR_DEVICE;
+        if (hipStreamSynchronize(s) != hipSuccess) return QUOIN_ERR_DEVICE;
+        std::vector<EncArgs> blocks;
+        for (size_t c = 0; c < ncols; c++) {
+            const DCol &d = cols[c];
+            const uint32_t rb = d.region_base, re = d.region_base + d.nregions;
+            for (uint32_t r = rb; r < re;) {
+                if (owner[r] != r) {
+                    r++;
+                    continue;
+                }
+                uint32_t r1 = r + 1;
+                while (r1 < re && owner[r1] == r) r1++;
+                EncArgs b = a;
+                b.r0 = r;
+                b.r1 = r1;
+                b.t0 = d.tile_base + (uint64_t)(r - rb) * d.tpr;
+                b.t1 = d.tile_base + (uint64_t)(r1 - rb) * d.tpr;
+                blocks.push_back(b);
+                r = r1;
+            }
+        }
+        for (const EncArgs &b : blocks) qa_encode_blocks_launch(b, s);
+        qa_encode_layout_launch(a, s);
+        for (const EncArgs &b : blocks) qa_encode_emit_launch(b, s);
+    }
     hipEventRecord(g->ev1, s);
     std::vector<DColState> st(ncols);
     uint64_t wins[64];
@@ -513,10 +552,32 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         a.scr = dp<uint64_t>(g->dscr);
         a.scr2 = dp<uint64_t>(g->dscr2);
     }
+    a.f0 = 0;
+    a.f1 = (uint32_t)frames.size();
+    a.t0 = 0;
+    a.t1 = tile_base;
     hipEventRecord(g->ev0, s);
     zero_async(g->derr.p, 8, s);
     zero_async(g->dstate.p, ns * sizeof(DDecState), s);
-    qa_decode_launch(a, s);
+    if (g->form == 0) {
+        qa_decode_launch(a, s);
+    } else {
+        // plain form: one launch sequence per frame (the tile starts of
+        // streams with nulls are a column-level select, run once)
+        qa_decode_prep_launch(a, s);
+        auto one = [&](size_t f) {
+            DecArgs b = a;
+            b.f0 = (uint32_t)f;
+            b.f1 = (uint32_t)f + 1;
+            b.t0 = frames[f].tile_base;
+            b.t1 = frames[f].tile_base + frames[f].ntiles;
+            return b;
+        };
+        for (size_t f = 0; f < frames.size(); f++) qa_decode_index_launch(one(f), s);
+        qa_decode_lpos_launch(a, s);
+        for (size_t f = 0; f < frames.size(); f++) qa_decode_values_launch(one(f), s);
+        qa_decode_final_launch(a, s);
+    }
     hipEventRecord(g->ev1, s);
     uint32_t err = 0;
     hipMemcpyAsync(&err, g->derr.p, 4, hipMemcpyDeviceToHost, s);

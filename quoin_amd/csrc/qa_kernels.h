@@ -42,6 +42,10 @@ struct EncArgs {
     uint32_t *rstart;      // per lane value slot (+1 per block): RLE run starts
     uint64_t *hkeys;       // 4 per lane value slot: global hash fallback keys
     uint32_t *hcnt;        // 4 per lane value slot: their counts
+    // launch range of the per-block stages: region slots [r0, r1), tile slots
+    // [t0, t1) (the whole batch, or one block in the plain form)
+    uint32_t r0, r1;
+    uint64_t t0, t1;
 };
 
 void qa_fast_sizes_launch(const EncArgs &a, hipStream_t s);
@@ -49,6 +53,10 @@ void qa_fast_final_launch(const EncArgs &a, hipStream_t s);
 void qa_fast_emit_launch(const EncArgs &a, hipStream_t s);
 
 void qa_encode_launch(const EncArgs &a, hipStream_t s, int level_max);
+void qa_encode_plan_launch(const EncArgs &a, hipStream_t s);
+void qa_encode_blocks_launch(const EncArgs &a, hipStream_t s);
+void qa_encode_layout_launch(const EncArgs &a, hipStream_t s);
+void qa_encode_emit_launch(const EncArgs &a, hipStream_t s);
 
 struct DFrame {
     uint64_t payload_off;  // byte offset of the payload in the stream
@@ -131,8 +139,17 @@ struct DecArgs {
     uint64_t *scr, *scr2;  // per frame value slots (Dict values, RLE starts, XORZ, ALP-RD exceptions)
     uint32_t fast;         // some frame needs the Level::Fast decoders
     uint32_t pad;
+    // launch range of the per-frame stages: frames [f0, f1), tiles [t0, t1)
+    // (the whole batch, or one frame in the plain form)
+    uint32_t f0, f1;
+    uint64_t t0, t1;
 };
 
 void qa_decode_launch(const DecArgs &a, hipStream_t s);
+void qa_decode_prep_launch(const DecArgs &a, hipStream_t s);
+void qa_decode_index_launch(const DecArgs &a, hipStream_t s);
+void qa_decode_lpos_launch(const DecArgs &a, hipStream_t s);
+void qa_decode_values_launch(const DecArgs &a, hipStream_t s);
+void qa_decode_final_launch(const DecArgs &a, hipStream_t s);
 void qa_fast_dec_index_launch(const DecArgs &a, hipStream_t s);
 void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s);

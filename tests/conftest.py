@@ -10,3 +10,15 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library)")
+
+
+def pytest_collection_modifyitems(config, items):
+    # PyTorch-ROCm carries its own HIP runtime; device tensors shared with the
+    # library work when torch's runtime initializes the device first
+    if any(item.get_closest_marker("gpu") for item in items):
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except ImportError:
+            pass

@@ -1,0 +1,79 @@
+"""The plain form (one launch sequence per block, quoin_gpu_set_launch_form 1)
+and the batched form (one launch per stage for the whole batch) produce the
+same bytes and the same decode; both equal the oracle."""
+import numpy as np
+import pytest
+
+import _data
+import _gpu
+import _oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+ODT = {np.dtype(np.float64): orc.F64, np.dtype(np.int64): orc.I64,
+       np.dtype(np.uint64): orc.U64, np.dtype(np.int32): orc.I32,
+       np.dtype(np.uint32): orc.U32, np.dtype(np.float32): orc.F32}
+
+
+@pytest.fixture(scope="module")
+def qa():
+    import quoin_amd
+    return quoin_amd
+
+
+def batch(n, seed):
+    ic = _data.int_columns(n, seed)
+    fc = _data.float_columns(n, seed + 1)
+    cols = [(ic["ts_i64"], None), _data.with_nulls(ic["u32_k3"], 0.01, seed),
+            _data.with_nulls(ic["u32_k11"], 0.3, seed + 2), (ic["i64_runs"], None),
+            (fc["walk_f64"], None), (fc["unif_f64"], None), (fc["unif_f32"], None),
+            (np.repeat(np.arange(37, dtype=np.float64) * 1.25, n // 37 + 1)[:n], None)]
+    return cols
+
+
+@pytest.mark.parametrize("level", [0, 1])
+@pytest.mark.parametrize("n", [1000, 70000, 300000])
+def test_plain_equals_batched(qa, level, n):
+    cols = batch(n, seed=n + level)
+    cfg = qa.Config(level=qa.Level(level))
+    gpu = qa.Gpu(0)
+    try:
+        print(f"case forms n={n} level={level} batched", flush=True)
+        gpu.set_launch_form(0)
+        batched = _gpu.device_compress(qa, gpu, cols, cfg)
+        print(f"case forms n={n} level={level} plain", flush=True)
+        gpu.set_launch_form(1)
+        plain = _gpu.device_compress(qa, gpu, cols, cfg)
+        for i, ((v, bm), a, b) in enumerate(zip(cols, batched, plain)):
+            assert a == b, f"column {i}: plain stream differs"
+            if n <= 70000:
+                assert a == orc.compress(v, ODT[v.dtype], bm, level), f"column {i}: oracle"
+        dts = [qa.FROM_NP[v.dtype] for v, _ in cols]
+        ns = [len(v) for v, _ in cols]
+        for form in (0, 1):
+            print(f"case forms n={n} level={level} decode form {form}", flush=True)
+            gpu.set_launch_form(form)
+            outs = _gpu.device_decompress(qa, gpu, plain, dts, ns, True)
+            for i, ((v, bm), (o, vo)) in enumerate(zip(cols, outs)):
+                _, want, _ = orc.decompress(batched[i])
+                assert o.view(np.uint8).tobytes() == want.view(np.uint8).tobytes(), \
+                    f"column {i}: decode form {form}"
+                if bm is not None:
+                    exp = bm.copy()
+                    if n % 8:
+                        exp[-1] &= (1 << (n % 8)) - 1
+                    assert vo == exp.tobytes()
+                else:
+                    full = np.packbits(np.ones(n, np.uint8), bitorder="little")
+                    assert vo == full.tobytes()
+    finally:
+        gpu.close()
+
+
+def test_launch_form_rejects_unknown(qa):
+    gpu = qa.Gpu(0)
+    try:
+        with pytest.raises(qa.QuoinError):
+            gpu.set_launch_form(2)
+    finally:
+        gpu.close()
