@@ -56,6 +56,144 @@ def cpu_baseline(cols, level, sample_n):
     return raw / dt / 1e9, raw
 
 
+def serve_pool(qa, gpu, cfg, dev, count, seed=1):
+    """Requests of the serving workload: uncompressed sizes log-uniform in
+    [4 KB, 64 MB], types f64/f32/i64/i32 in turn with the generators of the
+    other workloads (f64: 2-decimal walk, uniform doubles, 256-value runs;
+    f32 uniform [-1, 1); i64 timestamps; i32 uniform k-bit), half compress and
+    half decompress (of a stream made beforehand). Device buffers per request."""
+    import torch
+    r = np.random.default_rng(seed)
+    pool = []
+    for i in range(count):
+        nbytes = int(np.exp(r.uniform(np.log(4 << 10), np.log(64 << 20))))
+        kind = ["f64", "f32", "i64", "i32"][i % 4]
+        w = 4 if kind in ("f32", "i32") else 8
+        n = max(nbytes // w, 1)
+        if kind == "f64":
+            g = i // 4 % 3
+            if g == 0:
+                v = np.round(100 * (1000 + np.cumsum(r.normal(size=n)))) / 100
+            elif g == 1:
+                v = r.integers(0, 2**53, n, dtype=np.uint64) * 2.0**-53
+            else:
+                runs = r.geometric(1 / 8, n // 4 + 2)
+                v = np.repeat(r.integers(0, 256, len(runs)) * 0.5, runs)[:n]
+                if len(v) < n:
+                    v = np.resize(v, n)
+        elif kind == "f32":
+            v = ((r.integers(0, 2**24, n) * 2.0**-23) - 1).astype(np.float32)
+        elif kind == "i64":
+            v = (1_700_000_000_000_000_000 + np.cumsum(r.integers(1, 1001, n))).astype(np.int64)
+        else:
+            k = [3, 11, 23][i // 4 % 3]
+            v = (r.integers(0, 2**32, n, dtype=np.uint64) >> np.uint64(32 - k)).astype(np.int32)
+        v = np.ascontiguousarray(v)
+        dt = qa.FROM_NP[v.dtype]
+        t = torch.from_numpy(v.view(np.uint8).copy()).to(dev)
+        cap = qa.lib.quoin_typed_compress_bound(int(dt), n)
+        s = torch.empty(cap, dtype=torch.uint8, device=dev)
+        col = qa.DeviceColumn(int(dt), 0, n, t.data_ptr(), None, s.data_ptr(), cap)
+        req = {"kind": "c" if i % 2 == 0 else "d", "bytes": v.nbytes, "t": t, "s": s,
+               "col": col, "busy": False}
+        if req["kind"] == "d":
+            ln = gpu.compress([col], cfg)[0]
+            host = s[:ln].cpu().numpy().copy()
+            src = torch.from_numpy(host.copy()).to(dev)
+            out = torch.empty(v.nbytes, dtype=torch.uint8, device=dev)
+            req.update(host=host, src=src, out=out,
+                       st=qa.DeviceStream(int(dt), 0, n, src.data_ptr(), host.ctypes.data, ln,
+                                          out.data_ptr(), None))
+        pool.append(req)
+    torch.cuda.synchronize()
+    return pool
+
+
+def serve_run(qa, srv, pool, cfg, rate, count, seed=2):
+    """Open loop at `rate` requests/s (Poisson; rate None = all at once).
+    Returns (GB/s, latencies in ms, wall seconds)."""
+    import threading
+    import queue
+    r = np.random.default_rng(seed)
+    q = queue.Queue()
+    lat = []
+
+    def collect():
+        while True:
+            item = q.get()
+            if item is None:
+                return
+            req, t = item
+            rc, _, ms = srv.wait(t)
+            if rc != qa.OK:
+                raise RuntimeError(f"served job failed: {rc}")
+            lat.append(ms)
+            req["busy"] = False
+
+    th = threading.Thread(target=collect)
+    th.start()
+    done_bytes = 0
+    t0 = time.perf_counter()
+    next_t = t0
+    k = 0
+    for _ in range(count):
+        if rate:
+            next_t += r.exponential(1.0 / rate)
+            d = next_t - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+        # the next request whose buffers are free
+        for _ in range(len(pool)):
+            req = pool[k % len(pool)]
+            k += 1
+            if not req["busy"]:
+                break
+        else:
+            while req["busy"]:
+                time.sleep(1e-4)
+        req["busy"] = True
+        if req["kind"] == "c":
+            t = srv.submit_compress(req["col"], cfg)
+        else:
+            t = srv.submit_decompress(req["st"])
+        done_bytes += req["bytes"]
+        q.put((req, t))
+    q.put(None)
+    th.join()
+    wall = time.perf_counter() - t0
+    return done_bytes / wall / 1e9, lat, wall
+
+
+def serve_bench(qa, dev, budget_gb=64, slots=2, pool_size=48, per_point=800):
+    cfg = qa.Config(level=qa.Level.FAST)
+    gpu = qa.Gpu(0)
+    pool = serve_pool(qa, gpu, cfg, dev, pool_size)
+    gpu.close()
+    srv = qa.Server(0, budget_gb << 30, slots)
+    try:
+        serve_run(qa, srv, pool, cfg, None, len(pool))  # warm the slots' scratch
+        sat_gbs, lat, wall = serve_run(qa, srv, pool, cfg, None, per_point)
+        sat_rps = per_point / wall
+        points = []
+        for frac in (0.25, 0.5, 0.75, 0.9):
+            j0, b0 = srv.stats()
+            gbs, lat, wall = serve_run(qa, srv, pool, cfg, frac * sat_rps, per_point, seed=int(frac * 100))
+            j1, b1 = srv.stats()
+            points.append({"offered_rps": round(frac * sat_rps, 1), "achieved_gbs": round(gbs, 3),
+                           "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                           "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                           "jobs": j1 - j0, "batches": b1 - b0})
+    finally:
+        srv.close()
+    hi = points[-1]
+    return {"workload": f"{pool_size} requests, 4 KB-64 MB log-uniform, f64/f32/i64/i32, "
+                        f"half compress half decompress, Level::Fast, Poisson arrivals",
+            "hbm_budget_gb": budget_gb, "slots": slots,
+            "saturation_gbs": round(sat_gbs, 3), "saturation_rps": round(sat_rps, 1),
+            "p50_ms": hi["p50_ms"], "p99_ms": hi["p99_ms"], "at_offered_rps": hi["offered_rps"],
+            "points": points}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +205,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="steps of the plain form (one launch sequence per block); 0 skips it")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU in this build; --gpus {args.gpus} is not supported "
@@ -171,6 +310,10 @@ def main():
                 "achieved": round(dec_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(dec_gbs / HBM_PEAK_GBS, 4), "traffic": None}
 
+    serving = None
+    if not args.no_serve:
+        serving = serve_bench(qa, dev)
+
     cpu = None
     if not args.no_cpu:
         cpu_gbs, sample_raw = cpu_baseline(cols, int(level), args.cpu_sample)
@@ -194,6 +337,7 @@ def main():
                    "plain_form_gbs": round(plain, 3) if plain else None,
                    "batched_over_plain": round(value / plain, 2) if plain else None},
         "roofline": roofline,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

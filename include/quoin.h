@@ -196,6 +196,46 @@ int quoin_gpu_set_launch_form(QuoinGpu *g, int form);
 int quoin_gpu_last_timing(const QuoinGpu *g, double *total_ms, double *kernel_ms,
                           const char **kernel_name);
 
+/* ---- additions: serving loop (continuous batching) --------------------- */
+
+/* No reference counterpart: the reference has no request loop. Jobs name
+ * device buffers (as quoin_gpu_compress / quoin_gpu_decompress take them) that
+ * stay valid until quoin_server_wait returns for the job. */
+#define QUOIN_JOB_COMPRESS   0
+#define QUOIN_JOB_DECOMPRESS 1
+typedef struct {
+    int32_t kind;              /* QUOIN_JOB_*                              */
+    uint32_t reserved;
+    QuoinConfig cfg;           /* compress: level / block size             */
+    QuoinDeviceColumn column;  /* compress: input column, output stream    */
+    QuoinDeviceStream stream;  /* decompress: stream, typed output         */
+} QuoinJob;
+
+typedef struct QuoinServer QuoinServer;
+
+/* A request loop on one device. `slots` worker slots (each with its own HIP
+ * stream and scratch) take queued jobs as they free up: a slot admits the
+ * oldest waiting job and every other waiting job of the same kind (and, for
+ * compress, the same config) while the bytes of all admitted, unfinished jobs
+ * stay within hbm_budget_bytes (0 = 64 GiB), up to max_batch jobs (0 = 256),
+ * and runs them as one batched launch sequence. Jobs that arrive while others
+ * are on the device join the next batch; batches of different slots overlap
+ * on the device. A job's result does not depend on the jobs it is batched
+ * with. Returns NULL when no usable GPU is present. */
+QuoinServer *quoin_server_create(int device, uint64_t hbm_budget_bytes, uint32_t slots,
+                                 uint32_t max_batch);
+/* Queue a job; *ticket identifies it. QUOIN_ERR_BUFFER_TOO_SMALL when the job
+ * alone exceeds the admission budget. Thread-safe. */
+int quoin_server_submit(QuoinServer *s, const QuoinJob *job, uint64_t *ticket);
+/* Block until the job is done: its return code (as the quoin_gpu_* call
+ * would give for it alone), the stream length for compress, and the latency
+ * from submit to completion in milliseconds. Each ticket is waited once. */
+int quoin_server_wait(QuoinServer *s, uint64_t ticket, uint64_t *out_len, double *latency_ms);
+/* Jobs completed and batched launches run since creation. */
+void quoin_server_stats(const QuoinServer *s, uint64_t *jobs, uint64_t *batches);
+/* Finishes every queued job, then frees the server. */
+void quoin_server_destroy(QuoinServer *s);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -88,6 +88,15 @@ class DeviceStream(ctypes.Structure):
                 ("validity_out", ctypes.c_void_p)]
 
 
+class Job(ctypes.Structure):
+    """QuoinJob (include/quoin.h): kind 0 compress (cfg, column), 1 decompress (stream)."""
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_uint32), ("cfg", _CConfig),
+                ("column", DeviceColumn), ("stream", DeviceStream)]
+
+
+JOB_COMPRESS, JOB_DECOMPRESS = 0, 1
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run python quoin_amd/build.py")
@@ -123,6 +132,15 @@ def _load():
     lib.quoin_gpu_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_char_p)]
+    lib.quoin_server_create.restype = vp
+    lib.quoin_server_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+                                        ctypes.c_uint32]
+    lib.quoin_server_submit.argtypes = [vp, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_uint64)]
+    lib.quoin_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_double)]
+    lib.quoin_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64),
+                                       ctypes.POINTER(ctypes.c_uint64)]
+    lib.quoin_server_destroy.argtypes = [vp]
     return lib
 
 
@@ -137,6 +155,8 @@ EXPORTS = [
     "quoin_typed_compress_cfg", "quoin_compress_cfg", "quoin_mode_win_counts",
     "quoin_reset_mode_win_counts", "quoin_last_error", "quoin_gpu_create", "quoin_gpu_destroy", "quoin_gpu_compress",
     "quoin_gpu_decompress", "quoin_gpu_set_launch_form", "quoin_gpu_last_timing",
+    "quoin_server_create", "quoin_server_submit", "quoin_server_wait", "quoin_server_stats",
+    "quoin_server_destroy",
 ]
 
 
@@ -264,3 +284,50 @@ class Gpu:
         name = ctypes.c_char_p()
         lib.quoin_gpu_last_timing(self.h, ctypes.byref(t), ctypes.byref(k), ctypes.byref(name))
         return t.value, k.value, (name.value or b"").decode()
+
+
+class Server:
+    """Serving loop over one device (include/quoin.h, quoin_server_*): jobs are
+    admitted by an HBM byte budget and continuously batched per free slot."""
+
+    def __init__(self, device=0, hbm_budget=0, slots=2, max_batch=0):
+        self.h = lib.quoin_server_create(device, hbm_budget, slots, max_batch)
+        if not self.h:
+            raise QuoinError(ERR_DEVICE, "no usable GPU")
+
+    def submit_compress(self, column, config=None):
+        job = Job(JOB_COMPRESS, 0, _cfg(config), column, DeviceStream())
+        return self._submit(job)
+
+    def submit_decompress(self, stream):
+        job = Job(JOB_DECOMPRESS, 0, _CConfig(0, 0), DeviceColumn(), stream)
+        return self._submit(job)
+
+    def _submit(self, job):
+        t = ctypes.c_uint64()
+        rc = lib.quoin_server_submit(self.h, ctypes.byref(job), ctypes.byref(t))
+        if rc != OK:
+            raise QuoinError(rc)
+        return t.value
+
+    def wait(self, ticket):
+        """-> (return code, stream length for compress, latency in ms)"""
+        n, ms = ctypes.c_uint64(), ctypes.c_double()
+        rc = lib.quoin_server_wait(self.h, ticket, ctypes.byref(n), ctypes.byref(ms))
+        return rc, n.value, ms.value
+
+    def stats(self):
+        j, b = ctypes.c_uint64(), ctypes.c_uint64()
+        lib.quoin_server_stats(self.h, ctypes.byref(j), ctypes.byref(b))
+        return j.value, b.value
+
+    def close(self):
+        if self.h:
+            lib.quoin_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
