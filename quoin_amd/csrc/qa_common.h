@@ -237,6 +237,59 @@ __device__ __forceinline__ uint64_t qa_f32_to_lane(uint32_t b) {
     }
     return sign | ((uint64_t)(e - 127 + 1023) << 52) | ((uint64_t)m << 29);
 }
+// ---------------------------------------------------------------------------
+// Cross-lane operations on DPP (data-parallel primitives of the GFX9 ISA):
+// quad_perm / row_half_mirror / row_mirror combine inside a 16-lane row,
+// row_bcast15 / row_bcast31 carry a row's last lane into the next row(s).
+// Lanes a step does not update keep `old` (the operation's identity). Every
+// lane of the wavefront must be active.
+// ---------------------------------------------------------------------------
+#define QA_DPP_QP_1032 0xb1  // quad_perm [1,0,3,2]
+#define QA_DPP_QP_2301 0x4e  // quad_perm [2,3,0,1]
+#define QA_DPP_ROW_SHR(n) (0x110 + (n))
+#define QA_DPP_ROW_MIRROR 0x140
+#define QA_DPP_ROW_HALF_MIRROR 0x141
+#define QA_DPP_ROW_BCAST15 0x142
+#define QA_DPP_ROW_BCAST31 0x143
+#define QA_DPP_WAVE_SHR1 0x138  // lane i takes lane i - 1 (lane 0 keeps old)
+
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ uint32_t qa_dpp32(uint32_t old, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWMASK, 0xF, false);
+}
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ uint64_t qa_dpp64(uint64_t old, uint64_t v) {
+    const uint32_t lo = qa_dpp32<CTRL, ROWMASK>((uint32_t)old, (uint32_t)v);
+    const uint32_t hi = qa_dpp32<CTRL, ROWMASK>((uint32_t)(old >> 32), (uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t qa_readlane64(uint64_t v, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+// reduction over the wavefront, the result in every lane (read from lane 63)
+template <typename Op>
+__device__ __forceinline__ uint64_t qa_wave_reduce64(uint64_t v, Op op, uint64_t id) {
+    v = op(v, qa_dpp64<QA_DPP_QP_1032>(id, v));
+    v = op(v, qa_dpp64<QA_DPP_QP_2301>(id, v));
+    v = op(v, qa_dpp64<QA_DPP_ROW_HALF_MIRROR>(id, v));
+    v = op(v, qa_dpp64<QA_DPP_ROW_MIRROR>(id, v));
+    v = op(v, qa_dpp64<QA_DPP_ROW_BCAST15, 0xA>(id, v));
+    v = op(v, qa_dpp64<QA_DPP_ROW_BCAST31, 0xC>(id, v));
+    return qa_readlane64(v, 63);
+}
+// inclusive prefix sum over the wavefront (lane order)
+__device__ __forceinline__ uint64_t qa_wave_scan_add64(uint64_t v) {
+    v += qa_dpp64<QA_DPP_ROW_SHR(1)>(0, v);
+    v += qa_dpp64<QA_DPP_ROW_SHR(2)>(0, v);
+    v += qa_dpp64<QA_DPP_ROW_SHR(4)>(0, v);
+    v += qa_dpp64<QA_DPP_ROW_SHR(8)>(0, v);
+    v += qa_dpp64<QA_DPP_ROW_BCAST15, 0xA>(0, v);
+    v += qa_dpp64<QA_DPP_ROW_BCAST31, 0xC>(0, v);
+    return v;
+}
+
 // LDS written by one wavefront and read back by its other lanes
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -207,12 +207,7 @@ __device__ __forceinline__ void emit_gather_f(void *out, const uint64_t *bm, uin
             if (L1 < p + 64) m &= (1ull << (L1 - p)) - 1;
         }
         const uint64_t pc = (uint64_t)__popcll(m);
-        uint64_t inc = pc;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            uint64_t y = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += y;
-        }
+        const uint64_t inc = qa_wave_scan_add64(pc);
         const uint64_t ex = inc - pc;
         const uint32_t nw = (uint32_t)(je - c0 + 1 < (uint64_t)WAVE ? je - c0 + 1 : (uint64_t)WAVE);
         for (uint32_t i = 0; i < nw; i++) {
@@ -236,14 +231,7 @@ __device__ __forceinline__ void emit_gather(const DDecCol &col, uint64_t L0, uin
     }
 }
 
-__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, int lane) {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint64_t y = __shfl_up(v, o, 64);
-        if (lane >= o) v += y;
-    }
-    return v;
-}
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, int) { return qa_wave_scan_add64(v); }
 
 
 __device__ __forceinline__ bool wave_any_d(bool p) { return __ballot(p) != 0; }

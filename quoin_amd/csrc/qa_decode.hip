@@ -312,12 +312,7 @@ __global__ void __launch_bounds__(256) k_dec_delta_scan(DecArgs a) {
     for (uint32_t t0 = 0; t0 < fr.ntiles; t0 += WAVE) {
         uint32_t t = t0 + lane;
         uint64_t s = t < fr.ntiles ? a.tiles[fr.tile_base + t].tail : 0;
-        uint64_t inc = s;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            uint64_t y = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += y;
-        }
+        const uint64_t inc = qa_wave_scan_add64(s);
         if (t < fr.ntiles) a.tiles[fr.tile_base + t].tail = carry + inc - s;
         carry += __shfl(inc, 63, 64);
     }

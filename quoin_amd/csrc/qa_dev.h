@@ -67,18 +67,39 @@ __device__ __forceinline__ uint64_t bm_word(const uint8_t *bm, uint64_t n, uint6
     return w;
 }
 
+// wavefront reductions (DPP, result in every lane); every lane must be active
+struct OpMinU {
+    __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a < b ? a : b; }
+    static constexpr uint64_t id = ~0ull;
+};
+struct OpMaxU {
+    __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a > b ? a : b; }
+    static constexpr uint64_t id = 0;
+};
+struct OpMinS {
+    __device__ uint64_t operator()(uint64_t a, uint64_t b) const {
+        return (int64_t)a < (int64_t)b ? a : b;
+    }
+    static constexpr uint64_t id = 0x7FFFFFFFFFFFFFFFull;
+};
+struct OpMaxS {
+    __device__ uint64_t operator()(uint64_t a, uint64_t b) const {
+        return (int64_t)a > (int64_t)b ? a : b;
+    }
+    static constexpr uint64_t id = 0x8000000000000000ull;
+};
+struct OpAdd {
+    __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a + b; }
+    static constexpr uint64_t id = 0;
+};
+struct OpAnd {
+    __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a & b; }
+    static constexpr uint64_t id = ~0ull;
+};
 template <typename T, typename Op>
 __device__ __forceinline__ T wave_reduce(T v, Op op) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = op(v, (T)__shfl_xor(v, d, 64));
-    return v;
+    return (T)qa_wave_reduce64((uint64_t)(int64_t)v, op, Op::id);
 }
-struct OpMinU { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a < b ? a : b; } };
-struct OpMaxU { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a > b ? a : b; } };
-struct OpMinS { __device__ int64_t operator()(int64_t a, int64_t b) const { return a < b ? a : b; } };
-struct OpMaxS { __device__ int64_t operator()(int64_t a, int64_t b) const { return a > b ? a : b; } };
-struct OpAdd { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a + b; } };
-struct OpAnd { __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a & b; } };
 
 // Copy nbytes held in LDS (word-aligned at 0) to an arbitrary global byte
 // address: aligned dword stores inside, byte stores at the two edges only, so

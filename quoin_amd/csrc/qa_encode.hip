@@ -124,14 +124,14 @@ __global__ void __launch_bounds__(256) k_compact(EncArgs a) {
         bool on[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-            const int i = i0 + u;
-            const uint64_t wi = __shfl(w, i, 64);
-            wd[u] = (uint32_t)__shfl((int)width, i, 64);
-            dp[u] = __shfl(dst, i, 64);
+            const int i = i0 + u;  // word i of the 64: its fields are uniform (scalar reads)
+            const uint64_t wi = qa_readlane64(w, i);
+            wd[u] = (uint32_t)__builtin_amdgcn_readlane((int)width, i);
+            dp[u] = qa_readlane64(dst, i);
             on[u] = (wi >> lane) & 1;  // bits past the column end are clear
-            rk[u] = __shfl(rank0, i, 64) + (uint64_t)__popcll(wi & below);
-            const uint64_t pos = __shfl(pos0, i, 64) + lane;
-            const uint8_t *sp = (const uint8_t *)__shfl(src, i, 64);
+            rk[u] = qa_readlane64(rank0, i) + (uint64_t)__popcll(wi & below);
+            const uint64_t pos = qa_readlane64(pos0, i) + lane;
+            const uint8_t *sp = (const uint8_t *)qa_readlane64(src, i);
             v[u] = on[u] ? ld_w(sp, pos, wd[u]) : 0;
         }
 #pragma unroll
@@ -173,8 +173,7 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
         const uint64_t step = cnt > 1 ? lane_get_t<LK>(lp, lk, k0 + 1) - vf : 0;
         uint64_t umin = ~0ull, umax = 0, dzmin = ~0ull, dzmax = 0, xb = 0;
         int64_t smin = INT64_MAX, smax = INT64_MIN;
-        uint32_t chg = 0, eqp = 0;
-        bool diff = false, nonarith = false;
+        uint32_t eqin = 0, eqp = 0;
         // every load of the tile in flight before the first use
         uint64_t xs[QA_TILE / WAVE];
 #pragma unroll
@@ -188,10 +187,10 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
             const uint32_t k = q * WAVE + lane;
             const bool in = k < cnt;
             const uint64_t x = xs[q];  // out-of-range lanes hold vf
-            uint64_t p = __shfl_up(x, 1, 64);
-            if (lane == 0) p = carry;
-            carry = __shfl(x, 63, 64);
-            umin = x < umin ? x : umin;  // out-of-range lanes hold vf
+            // previous element: the lane below, or the carry for lane 0
+            const uint64_t p = qa_dpp64<QA_DPP_WAVE_SHR1>(carry, x);
+            carry = qa_readlane64(x, 63);
+            umin = x < umin ? x : umin;
             umax = x > umax ? x : umax;
             if (SG) {
                 smin = (int64_t)x < smin ? (int64_t)x : smin;
@@ -203,14 +202,15 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
                 dzmin = z < dzmin ? z : dzmin;
                 dzmax = z > dzmax ? z : dzmax;
                 if (FASTL) xb += qa_varint_len(x ^ p);
-                nonarith |= (x - p) != step;
             }
-            diff |= x != vf;
-            chg += (uint32_t)__popcll(__ballot(pair && x != p));
+            const uint64_t eqm = __ballot(x == p);
+            eqin += (uint32_t)__popcll(eqm & __ballot(pair));
             // equal neighbours of the region: inside the tile, and across
             // tile boundaries inside the region
-            eqp += (uint32_t)__popcll(__ballot((pair || (in && k == 0 && k0 > rs)) && x == p));
+            if (q == 0 && k0 > rs) eqp += (uint32_t)(eqm & 1);
         }
+        eqp += eqin;
+        const uint32_t chg = cnt ? cnt - 1 - eqin : 0;  // changes between neighbours in the tile
         umin = wave_reduce(umin, OpMinU());
         umax = wave_reduce(umax, OpMaxU());
         if (SG) {
@@ -220,7 +220,8 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
         dzmin = wave_reduce(dzmin, OpMinU());
         dzmax = wave_reduce(dzmax, OpMaxU());
         if (FASTL) xb = wave_reduce(xb, OpAdd());
-        const bool same = __ballot(diff) == 0, arith = __ballot(nonarith) == 0;
+        // all equal <=> min == max; arithmetic <=> every zig-zag delta equal
+        const bool same = umin == umax, arith = cnt <= 1 || dzmin == dzmax;
         if (lane == 0) {
             ts.umin = umin;
             ts.umax = umax;
@@ -785,44 +786,28 @@ __global__ void k_emit_frames(EncArgs a) {
 // pack (src/bitpack.rs:47-133) straight from the lane: in the u32 form lane l
 // of the 32 packs rows [16 h, 16 h + 16) (h = half of the wavefront), in the
 // u64 form lane l of the 16 packs rows [16 q, 16 q + 16) (q = quarter); each
-// residual is ORed into its one or two LDS words.
-__global__ void __launch_bounds__(256) k_emit_tiles(EncArgs a) {
-    __shared__ uint64_t outs[4][QA_TILE + 1];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
-    if (t >= a.t1) return;
-    const DTile &ts = a.tiles[t];
-    if (!ts.active) return;
-    const int c = col_by_tile(a.cols, a.ncols, t);
-    const DCol &col = a.cols[c];
-    const DColState &st = a.cst[c];
-    if (st.err) return;
-    const void *lp = st.lanep;
-    const uint32_t lk = st.lkind;
-    const uint64_t lt = t - col.tile_base;
-    const uint32_t lr = (uint32_t)(lt / col.tpr);
-    const uint32_t o = a.owner[col.region_base + lr];
-    const DBlock &b = a.blocks[o];
-    const uint32_t bt = (col.region_base + lr - o) * col.tpr + (uint32_t)(lt % col.tpr);
-    const uint64_t k0 = b.start + (uint64_t)bt * QA_TILE;
-    const uint32_t cnt = ts.cnt;
-    const uint32_t mode = b.mode;
-    uint8_t *p = col.dst + b.frame_off + 9;
-    uint32_t *outw = (uint32_t *)outs[wv];
+// residual is ORed into its one or two LDS words. The body is instantiated
+// per lane storage kind so the loads are straight-line code.
+template <int LK>
+__device__ __forceinline__ void emit_tile_t(const void *lp, uint32_t lk, uint32_t dtype,
+                                            uint32_t mode, const DTile &ts, uint32_t bt,
+                                            uint64_t k0, uint32_t cnt, uint8_t *p,
+                                            uint64_t *o64w, int lane) {
+    uint32_t *outw = (uint32_t *)o64w;
     if (mode == QA_M_RAW) {
-        const uint32_t lb = (uint32_t)lane_bytes_of(col.dtype);
+        const uint32_t lb = (uint32_t)lane_bytes_of(dtype);
         uint64_t xs[QA_TILE / WAVE];
 #pragma unroll
         for (int i = 0; i < QA_TILE / WAVE; i++) {
             const uint32_t k = i * WAVE + lane;
-            xs[i] = k < cnt ? qa_lane_get(lp, lk, k0 + k) : 0;
+            xs[i] = k < cnt ? lane_get_t<LK>(lp, lk, k0 + k) : 0;
         }
 #pragma unroll
         for (int i = 0; i < QA_TILE / WAVE; i++) {
             const uint32_t k = i * WAVE + lane;
             if (k >= cnt) break;
             const uint64_t x = xs[i];
-            if (col.dtype == QA_F32) outw[k] = qa_lane_to_f32(x);
+            if (dtype == QA_F32) outw[k] = qa_lane_to_f32(x);
             else if (lb == 4) outw[k] = (uint32_t)x;
             else {
                 outw[2 * k] = (uint32_t)x;
@@ -834,7 +819,7 @@ __global__ void __launch_bounds__(256) k_emit_tiles(EncArgs a) {
         return;
     }
     if (mode != QA_M_FOR && mode != QA_M_DELTA) return;
-    const bool sg = is_signed(col.dtype), delta = mode == QA_M_DELTA;
+    const bool sg = is_signed(dtype), delta = mode == QA_M_DELTA;
     uint64_t mn;
     if (delta) {
         uint64_t dmx;
@@ -861,8 +846,9 @@ __global__ void __launch_bounds__(256) k_emit_tiles(EncArgs a) {
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         const uint32_t k = (r0 + i) * L + l;
-        xs[i] = k < cnt ? qa_lane_get(lp, lk, k0 + k) : 0;
-        ps[i] = (delta && k < cnt && (bt != 0 || k != 0)) ? qa_lane_get(lp, lk, k0 + k - 1) : xs[i];
+        xs[i] = k < cnt ? lane_get_t<LK>(lp, lk, k0 + k) : 0;
+        ps[i] = (delta && k < cnt && (bt != 0 || k != 0)) ? lane_get_t<LK>(lp, lk, k0 + k - 1)
+                                                          : xs[i];
     }
 #pragma unroll
     for (int i = 0; i < 16; i++) {
@@ -875,27 +861,56 @@ __global__ void __launch_bounds__(256) k_emit_tiles(EncArgs a) {
             const uint32_t r = r0 + i;
             const uint32_t v = (uint32_t)xs[i];
             const uint32_t bit = r * w, word = bit >> 5, sh = bit & 31;
-            if (v) {
-                atomicOr(&outw[word * 32 + l], v << sh);
-                if (sh + w > 32) atomicOr(&outw[(word + 1) * 32 + l], v >> (32 - sh));
-            }
+            atomicOr(&outw[word * 32 + l], v << sh);
+            if (sh + w > 32) atomicOr(&outw[(word + 1) * 32 + l], v >> (32 - sh));
         }
     } else {
-        unsigned long long *o64 = (unsigned long long *)outs[wv];
+        unsigned long long *o64 = (unsigned long long *)o64w;
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const uint32_t r = r0 + i;
             const uint64_t v = xs[i];
             const uint32_t bit = r * w, word = bit >> 6, sh = bit & 63;
-            if (v) {
-                atomicOr(&o64[word * 16 + l], (unsigned long long)(v << sh));
-                if (sh + w > 64)
-                    atomicOr(&o64[(word + 1) * 16 + l], (unsigned long long)(v >> (64 - sh)));
-            }
+            atomicOr(&o64[word * 16 + l], (unsigned long long)(v << sh));
+            if (sh + w > 64)
+                atomicOr(&o64[(word + 1) * 16 + l], (unsigned long long)(v >> (64 - sh)));
         }
     }
     wave_lds_sync();
     wave_store_bytes(q + 9, outw, 128 * w, lane);
+}
+
+__global__ void __launch_bounds__(256) k_emit_tiles(EncArgs a) {
+    __shared__ uint64_t outs[4][QA_TILE + 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
+    if (t >= a.t1) return;
+    const DTile &ts = a.tiles[t];
+    if (!ts.active) return;
+    const int c = col_by_tile(a.cols, a.ncols, t);
+    const DCol &col = a.cols[c];
+    const DColState &st = a.cst[c];
+    if (st.err) return;
+    const void *lp = st.lanep;
+    const uint32_t lk = st.lkind;
+    const uint64_t lt = t - col.tile_base;
+    const uint32_t lr = (uint32_t)(lt / col.tpr);
+    const uint32_t o = a.owner[col.region_base + lr];
+    const DBlock &b = a.blocks[o];
+    const uint32_t bt = (col.region_base + lr - o) * col.tpr + (uint32_t)(lt % col.tpr);
+    const uint64_t k0 = b.start + (uint64_t)bt * QA_TILE;
+    const uint32_t cnt = ts.cnt;
+    const uint32_t mode = b.mode;
+    if (mode != QA_M_RAW && mode != QA_M_FOR && mode != QA_M_DELTA) return;
+    uint8_t *p = col.dst + b.frame_off + 9;
+    const uint32_t dt = col.dtype;
+    switch (lk) {
+    case QA_LK_U64: emit_tile_t<QA_LK_U64>(lp, lk, dt, mode, ts, bt, k0, cnt, p, outs[wv], lane); break;
+    case QA_LK_U32: emit_tile_t<QA_LK_U32>(lp, lk, dt, mode, ts, bt, k0, cnt, p, outs[wv], lane); break;
+    case QA_LK_I32: emit_tile_t<QA_LK_I32>(lp, lk, dt, mode, ts, bt, k0, cnt, p, outs[wv], lane); break;
+    case QA_LK_F32: emit_tile_t<QA_LK_F32>(lp, lk, dt, mode, ts, bt, k0, cnt, p, outs[wv], lane); break;
+    default: emit_tile_t<QA_LK_ANY>(lp, lk, dt, mode, ts, bt, k0, cnt, p, outs[wv], lane);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -640,12 +640,7 @@ __global__ void __launch_bounds__(256) k_block_decide(EncArgs a) {
             uint64_t starts = 0;
             if (t < b.ntiles)
                 starts = tl[t].chg_rest + (t ? (tl[t].v_first != tl[t - 1].v_last) : 1);
-            uint64_t inc = starts;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                uint64_t y = __shfl_up(inc, o, 64);
-                if (lane >= o) inc += y;
-            }
+            const uint64_t inc = qa_wave_scan_add64(starts);
             if (t < b.ntiles) tf[t].rle_base = carry + inc - starts;
             carry += __shfl(inc, 63, 64);
         }

@@ -8,15 +8,7 @@
 #define SCAN_PER 8
 #define SCAN_CHUNK (SCAN_WG * SCAN_PER)
 
-__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint64_t o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
-}
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) { return qa_wave_scan_add64(v); }
 
 // block-wide exclusive scan of one value per thread; returns exclusive prefix,
 // *total receives the block sum.
