@@ -142,6 +142,16 @@ struct DTileF {
     uint64_t dict_min;         // smallest code (rank) in the tile
 };
 
+// Probe partials of a full 32768-value region of an adaptive column, for the
+// 131072-value block it may open or join: the hash buckets of every 32nd value
+// (a 16384-bit set per region slot, stored separately), and over every 64th
+// value after the region's first: min, max and the largest zig-zag step from
+// the value before. v0 / dz0: the region's first value and its step.
+struct DRegionG {
+    uint64_t gmn, gmx, gdz, v0, dz0;
+    uint32_t full, pad;
+};
+
 // Per-block state of the Level::Fast codecs.
 struct DBlockX {
     uint64_t rd_dict[8];

@@ -174,14 +174,14 @@ __device__ __forceinline__ void emit_dense_f(void *out, uint64_t base, uint32_t 
         if (k < cnt) st_out<F>(out, base + k, x[q]);
     }
 }
-__device__ __forceinline__ void emit_dense(const DDecCol &col, uint64_t base, uint32_t cnt,
+__device__ __forceinline__ void emit_dense(void *out, uint32_t oform, uint64_t base, uint32_t cnt,
                                            const uint64_t (&x)[QA_TILE / WAVE], int lane) {
-    switch (out_form(col)) {
-    case QA_OUT_U8: emit_dense_f<QA_OUT_U8>(col.out, base, cnt, x, lane); break;
-    case QA_OUT_U16: emit_dense_f<QA_OUT_U16>(col.out, base, cnt, x, lane); break;
-    case QA_OUT_U32: emit_dense_f<QA_OUT_U32>(col.out, base, cnt, x, lane); break;
-    case QA_OUT_F32: emit_dense_f<QA_OUT_F32>(col.out, base, cnt, x, lane); break;
-    default: emit_dense_f<QA_OUT_U64>(col.out, base, cnt, x, lane); break;
+    switch (oform) {
+    case QA_OUT_U8: emit_dense_f<QA_OUT_U8>(out, base, cnt, x, lane); break;
+    case QA_OUT_U16: emit_dense_f<QA_OUT_U16>(out, base, cnt, x, lane); break;
+    case QA_OUT_U32: emit_dense_f<QA_OUT_U32>(out, base, cnt, x, lane); break;
+    case QA_OUT_F32: emit_dense_f<QA_OUT_F32>(out, base, cnt, x, lane); break;
+    default: emit_dense_f<QA_OUT_U64>(out, base, cnt, x, lane); break;
     }
 }
 
@@ -220,14 +220,15 @@ __device__ __forceinline__ void emit_gather_f(void *out, const uint64_t *bm, uin
         r += __shfl(inc, 63, 64);
     }
 }
-__device__ __forceinline__ void emit_gather(const DDecCol &col, uint64_t L0, uint64_t L1,
-                                            const uint64_t *vv, uint32_t cnt, int lane) {
-    switch (out_form(col)) {
-    case QA_OUT_U8: emit_gather_f<QA_OUT_U8>(col.out, col.bitmap, L0, L1, vv, cnt, lane); break;
-    case QA_OUT_U16: emit_gather_f<QA_OUT_U16>(col.out, col.bitmap, L0, L1, vv, cnt, lane); break;
-    case QA_OUT_U32: emit_gather_f<QA_OUT_U32>(col.out, col.bitmap, L0, L1, vv, cnt, lane); break;
-    case QA_OUT_F32: emit_gather_f<QA_OUT_F32>(col.out, col.bitmap, L0, L1, vv, cnt, lane); break;
-    default: emit_gather_f<QA_OUT_U64>(col.out, col.bitmap, L0, L1, vv, cnt, lane); break;
+__device__ __forceinline__ void emit_gather(void *out, uint32_t oform, const uint64_t *bm,
+                                            uint64_t L0, uint64_t L1, const uint64_t *vv,
+                                            uint32_t cnt, int lane) {
+    switch (oform) {
+    case QA_OUT_U8: emit_gather_f<QA_OUT_U8>(out, bm, L0, L1, vv, cnt, lane); break;
+    case QA_OUT_U16: emit_gather_f<QA_OUT_U16>(out, bm, L0, L1, vv, cnt, lane); break;
+    case QA_OUT_U32: emit_gather_f<QA_OUT_U32>(out, bm, L0, L1, vv, cnt, lane); break;
+    case QA_OUT_F32: emit_gather_f<QA_OUT_F32>(out, bm, L0, L1, vv, cnt, lane); break;
+    default: emit_gather_f<QA_OUT_U64>(out, bm, L0, L1, vv, cnt, lane); break;
     }
 }
 

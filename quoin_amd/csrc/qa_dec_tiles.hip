@@ -62,39 +62,71 @@ __device__ __forceinline__ void patch_reg(uint64_t (&x)[QV], uint32_t k, uint64_
     }
 }
 
+// one thread per tile: its decode descriptor
+__global__ void k_dec_desc(DecArgs a) {
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.t1) return;
+    TileDec d{};
+    const DTileD &ts = a.tiles[t];
+    const uint32_t f = ts.frame;
+    const DFrame &fr = a.frames[f];
+    const DDecCol &col = a.cols[fr.col];
+    const DDecState &st = a.dst[fr.col];
+    if (f < st.fused_end && !*a.err) {
+        const uint32_t lt = (uint32_t)(t - fr.tile_base);
+        d.src = col.src;
+        d.out = col.out;
+        d.bitmap = col.bitmap;
+        d.off = ts.off;
+        d.tail = ts.tail;
+        d.payload = fr.payload_off;
+        d.p0 = (uint64_t)lt * QA_TILE;
+        d.base = fr.out_off + d.p0;
+        d.frame = f;
+        d.cnt = min((uint32_t)QA_TILE, fr.n - lt * QA_TILE);
+        d.w = ts.w;
+        d.mode = fr.mode;
+        d.dtype = col.dtype;
+        d.oform = (uint32_t)out_form(col);
+        d.nulls = st.nv != col.n;
+        if (d.nulls) {
+            // logical positions [lpos, next tile's lpos in the same stream, or n)
+            d.L0 = ts.lpos;
+            d.L1 = col.n;
+            if (t + 1 < a.ntiles) {
+                const DTileD &nx = a.tiles[t + 1];
+                if (nx.frame < st.fused_end && a.frames[nx.frame].col == fr.col) d.L1 = nx.lpos;
+            }
+        }
+    }
+    a.tdec[t] = d;
+}
+
 __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
     __shared__ uint64_t lds[4][QA_TILE];
     if (*a.err) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.t1) return;
-    const DTileD ts = a.tiles[t];
-    const uint32_t f = ts.frame;
-    const DFrame &fr = a.frames[f];
-    const DDecCol &col = a.cols[fr.col];
-    const DDecState &st = a.dst[fr.col];
-    if (f >= st.fused_end) return;
-    const bool nulls = st.nv != col.n;
-    const uint32_t lt = (uint32_t)(t - fr.tile_base);
-    const uint32_t cnt = min((uint32_t)QA_TILE, fr.n - lt * QA_TILE);
-    const uint64_t p0 = (uint64_t)lt * QA_TILE;  // first value within the frame
-    const uint8_t *s = col.src;
+    const TileDec td = a.tdec[t];
+    if (!td.mode) return;
+    const uint32_t f = td.frame;
+    const bool nulls = td.nulls;
+    const uint32_t cnt = td.cnt;
+    const uint64_t p0 = td.p0;
+    const uint8_t *s = td.src;
     uint32_t *wd = (uint32_t *)lds[wv];
     uint64_t *vv = lds[wv];
     uint64_t x[QV];
     bool bad = false;
-    // streams with nulls: the tile covers logical positions [ts.lpos, L1), up
-    // to the next tile's first value in the same stream (or the stream's end)
-    uint64_t L1 = col.n;
-    if (nulls && t + 1 < a.ntiles) {
-        const DTileD &nx = a.tiles[t + 1];
-        if (nx.frame < st.fused_end && a.frames[nx.frame].col == fr.col) L1 = nx.lpos;
-    }
+    struct {
+        uint64_t off, tail, w;
+    } ts{td.off, td.tail, td.w};
 
-    switch (fr.mode) {
+    switch (td.mode) {
     case QA_M_RAW: {
-        const uint32_t lb = (uint32_t)lane_bytes_d(col.dtype);
-        stage_words(s + fr.payload_off + p0 * lb, cnt * lb, wd, lane);
+        const uint32_t lb = (uint32_t)lane_bytes_d(td.dtype);
+        stage_words(s + td.payload + p0 * lb, cnt * lb, wd, lane);
         wave_lds_sync();
 #pragma unroll
         for (int q = 0; q < QV; q++) {
@@ -103,7 +135,7 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
             if (k < cnt) {
                 if (lb == 4) {
                     v = wd[k];
-                    if (col.dtype == QA_F32) v = qa_f32_to_lane((uint32_t)v);
+                    if (td.dtype == QA_F32) v = qa_f32_to_lane((uint32_t)v);
                 } else {
                     v = ((uint64_t)wd[2 * k + 1] << 32) | wd[2 * k];
                 }
@@ -113,13 +145,13 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
         break;
     }
     case QA_M_CONST: {
-        const uint64_t v = qa_ld64u(s + fr.payload_off);
+        const uint64_t v = qa_ld64u(s + td.payload);
 #pragma unroll
         for (int q = 0; q < QV; q++) x[q] = v;
         break;
     }
     case QA_M_STRIDE: {
-        const uint64_t v0 = qa_ld64u(s + fr.payload_off), step = qa_ld64u(s + fr.payload_off + 8);
+        const uint64_t v0 = qa_ld64u(s + td.payload), step = qa_ld64u(s + td.payload + 8);
 #pragma unroll
         for (int q = 0; q < QV; q++) x[q] = v0 + (p0 + lane + WAVE * q) * step;
         break;
@@ -138,7 +170,7 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
             uint32_t k = lane + WAVE * q;
             x[q] = k < cnt ? mn + unpack_lds(wd, w, k) : 0;
         }
-        if (fr.mode == QA_M_DELTA) {
+        if (td.mode == QA_M_DELTA) {
             uint64_t carry = ts.tail;
 #pragma unroll
             for (int q = 0; q < QV; q++) {
@@ -148,14 +180,14 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
                 x[q] = carry + inc;
                 carry += __shfl(inc, 63, 64);
             }
-        } else if (fr.mode == QA_M_FLOAT_MULT) {
+        } else if (td.mode == QA_M_FLOAT_MULT) {
             const double sc = d_fm_scales[a.fx[f].scale];
 #pragma unroll
             for (int q = 0; q < QV; q++)
                 x[q] = (uint64_t)__double_as_longlong(__ddiv_rn((double)(int64_t)x[q], sc));
-        } else if (fr.mode == QA_M_DICT) {
+        } else if (td.mode == QA_M_DICT) {
             const uint64_t card = a.fx[f].card;
-            const uint64_t *dv = a.scr + fr.scr;
+            const uint64_t *dv = a.scr + a.frames[f].scr;
 #pragma unroll
             for (int q = 0; q < QV; q++) {
                 uint32_t k = lane + WAVE * q;
@@ -229,7 +261,8 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
             x[q] = v;
         }
         // exceptions: out = (left << right_bw) | right, in list order
-        const uint64_t *epos = a.scr + fr.scr, *eleft = a.scr2 + fr.scr;
+        const uint64_t fscr = a.frames[f].scr;
+        const uint64_t *epos = a.scr + fscr, *eleft = a.scr2 + fscr;
         uint64_t jb = 0, je = fx.nexc;
         if (fx.sorted) {  // only this tile's slice
             uint64_t lo = 0, hi = je;
@@ -263,7 +296,7 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
     }
     case QA_M_RLE: {
         const uint64_t R = a.fx[f].card, aux = a.fx[f].aux_off;
-        const uint64_t *rs = a.scr + fr.scr;
+        const uint64_t *rs = a.scr + a.frames[f].scr;
 #pragma unroll
         for (int q = 0; q < QV; q++) {
             uint32_t k = lane + WAVE * q;
@@ -281,7 +314,7 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
         break;
     }
     case QA_M_XORZ: {
-        const uint64_t *xv = a.scr + fr.scr + p0;
+        const uint64_t *xv = a.scr + a.frames[f].scr + p0;
 #pragma unroll
         for (int q = 0; q < QV; q++) {
             uint32_t k = lane + WAVE * q;
@@ -297,7 +330,7 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
     }
 
     if (!nulls) {
-        emit_dense(col, fr.out_off + p0, cnt, x, lane);
+        emit_dense(td.out, td.oform, td.base, cnt, x, lane);
         return;
     }
     wave_lds_sync();  // every lane is done with the staged words
@@ -307,12 +340,13 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
         if (k < cnt) vv[k] = x[q];
     }
     wave_lds_sync();
-    emit_gather(col, ts.lpos, L1, vv, cnt, lane);
+    emit_gather(td.out, td.oform, td.bitmap, td.L0, td.L1, vv, cnt, lane);
 }
 
 static inline unsigned nbt(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
     if (a.t1 <= a.t0) return;
+    hipLaunchKernelGGL(k_dec_desc, dim3(nbt(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_dec_tiles, dim3(nbt(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
 }

@@ -97,6 +97,35 @@ __device__ __forceinline__ void st_w(uint8_t *p, uint64_t i, uint32_t w, uint64_
     }
 }
 
+// the 64 words of one wavefront when every word's column has element type T:
+// 32 words' loads in flight before their stores (on this ISA loads and stores
+// share one completion counter, so batching them is what keeps loads queued)
+template <typename T>
+__device__ __forceinline__ void compact_words(uint64_t w, uint64_t rank0, uint64_t pos0,
+                                              uint64_t src, uint64_t dst, int lane) {
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int i0 = 0; i0 < WAVE; i0 += 32) {
+        T v[32];
+#pragma unroll
+        for (int u = 0; u < 32; u++) {
+            const int i = i0 + u;
+            const uint64_t wi = qa_readlane64(w, i);
+            const T *sp = (const T *)qa_readlane64(src, i);
+            const uint64_t pos = qa_readlane64(pos0, i) + lane;
+            v[u] = ((wi >> lane) & 1) ? sp[pos] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 32; u++) {
+            const int i = i0 + u;
+            const uint64_t wi = qa_readlane64(w, i);
+            if ((wi >> lane) & 1) {
+                T *dp = (T *)qa_readlane64(dst, i);
+                dp[qa_readlane64(rank0, i) + (uint64_t)__popcll(wi & below)] = v[u];
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_compact(EncArgs a) {
     const int lane = threadIdx.x & 63;
     const uint64_t j0 = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * WAVE;
@@ -115,6 +144,18 @@ __global__ void __launch_bounds__(256) k_compact(EncArgs a) {
             src = (uint64_t)col.values;
             dst = (uint64_t)((uint8_t *)a.lane + col.lane_off * 8);
             width = col.in_width;
+        }
+    }
+    // one element width across the wavefront's words (0 = nothing to move)
+    const uint64_t nz = __ballot(width != 0);
+    if (!nz) return;
+    const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)width, __builtin_ctzll(nz));
+    if (__ballot(width != 0 && width != w0) == 0) {
+        switch (w0) {
+        case 1: compact_words<uint8_t>(w, rank0, pos0, src, dst, lane); return;
+        case 2: compact_words<uint16_t>(w, rank0, pos0, src, dst, lane); return;
+        case 4: compact_words<uint32_t>(w, rank0, pos0, src, dst, lane); return;
+        default: compact_words<uint64_t>(w, rank0, pos0, src, dst, lane); return;
         }
     }
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -153,9 +194,18 @@ __global__ void __launch_bounds__(256) k_compact(EncArgs a) {
 // The probe's exponent range only gates the entropy levels' byte transpose
 // (encoder.rs:414-417) and is not computed. The previous element of each lane
 // comes from its neighbour lane.
-template <int LK, bool SG, bool FASTL>
+// sample accumulators of a full region, taken from the tile registers:
+// p* every 16th value after the first (the region probe), g* every 64th value
+// after the first (the grown-block probe partials)
+struct ProbeAcc {
+    uint64_t pmn, pmx, pdz, gmn, gmx, gdz;
+};
+
+template <int LK, bool SG, bool FASTL, bool REG>
 __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp, uint32_t lk,
-                                                 uint64_t rs, uint64_t len, DTile *tl) {
+                                                 uint64_t rs, uint64_t len, DTile *tl,
+                                                 uint64_t *seen8, uint64_t *seen32,
+                                                 ProbeAcc &pa) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const uint32_t ntl = (uint32_t)((len + QA_TILE - 1) / QA_TILE);
     uint32_t eq_region = 0;
@@ -202,6 +252,27 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
                 dzmin = z < dzmin ? z : dzmin;
                 dzmax = z > dzmax ? z : dzmax;
                 if (FASTL) xb += qa_varint_len(x ^ p);
+            }
+            if (REG) {  // every tile of a full region is full
+                const uint32_t rpos = t * QA_TILE + q * WAVE + lane;  // region position
+                if ((lane & 7) == 0) {
+                    const uint64_t h = x * 0x9E3779B97F4A7C15ull;
+                    const uint32_t bucket = (uint32_t)((h >> 50) & 0x3FFF);
+                    atomicOr((unsigned long long *)&seen8[bucket >> 6], 1ull << (bucket & 63));
+                    if ((lane & 31) == 0)
+                        atomicOr((unsigned long long *)&seen32[bucket >> 6], 1ull << (bucket & 63));
+                }
+                if ((lane & 15) == 0 && rpos != 0) {
+                    const uint64_t z = qa_zigzag(x - p);
+                    pa.pmn = x < pa.pmn ? x : pa.pmn;
+                    pa.pmx = x > pa.pmx ? x : pa.pmx;
+                    pa.pdz = z > pa.pdz ? z : pa.pdz;
+                    if (lane == 0) {
+                        pa.gmn = x < pa.gmn ? x : pa.gmn;
+                        pa.gmx = x > pa.gmx ? x : pa.gmx;
+                        pa.gdz = z > pa.gdz ? z : pa.gdz;
+                    }
+                }
             }
             const uint64_t eqm = __ballot(x == p);
             eqin += (uint32_t)__popcll(eqm & __ballot(pair));
@@ -295,25 +366,38 @@ __device__ __forceinline__ void probe_samples(const void *lp, uint32_t lk, uint6
     *dz = d;
 }
 
+struct RegionShared {
+    uint64_t smin, smax, sdz, gmn, gmx, gdz;
+    uint32_t eq, dsum;
+};
+
 template <int LK, bool SG, bool FASTL>
 __device__ __forceinline__ void region_stats_body(const EncArgs &a, uint32_t r, const DCol &col,
                                                   const void *lp, uint32_t lk, uint64_t rs,
                                                   uint64_t len, DTile *tl, uint64_t *seen,
-                                                  uint32_t *eq_s, uint64_t *smin_s,
-                                                  uint64_t *smax_s, uint64_t *sdz_s) {
+                                                  uint64_t *seen32, RegionShared *sh, bool reg) {
     const int lane = threadIdx.x & 63;
-    uint32_t eq = region_tiles<LK, SG, FASTL>(col, lp, lk, rs, len, tl);
-    if (lane == 0) atomicAdd(eq_s, eq);
-    const uint64_t v0 = lane_get_t<LK>(lp, lk, rs);
-    uint64_t mn = v0, mx = v0, dz = 0;
-    probe_samples<LK>(lp, lk, rs, len, seen, &mn, &mx, &dz);
-    mn = wave_reduce(mn, OpMinU());
-    mx = wave_reduce(mx, OpMaxU());
-    dz = wave_reduce(dz, OpMaxU());
+    ProbeAcc pa{~0ull, 0, 0, ~0ull, 0, 0};
+    uint32_t eq;
+    if (reg) {
+        eq = region_tiles<LK, SG, FASTL, true>(col, lp, lk, rs, len, tl, seen, seen32, pa);
+    } else {
+        eq = region_tiles<LK, SG, FASTL, false>(col, lp, lk, rs, len, tl, seen, seen32, pa);
+        probe_samples<LK>(lp, lk, rs, len, seen, &pa.pmn, &pa.pmx, &pa.pdz);
+    }
+    if (lane == 0) atomicAdd(&sh->eq, eq);
+    pa.pmn = wave_reduce(pa.pmn, OpMinU());
+    pa.pmx = wave_reduce(pa.pmx, OpMaxU());
+    pa.pdz = wave_reduce(pa.pdz, OpMaxU());
     if (lane == 0) {
-        atomicMin((unsigned long long *)smin_s, mn);
-        atomicMax((unsigned long long *)smax_s, mx);
-        atomicMax((unsigned long long *)sdz_s, dz);
+        atomicMin((unsigned long long *)&sh->smin, pa.pmn);
+        atomicMax((unsigned long long *)&sh->smax, pa.pmx);
+        atomicMax((unsigned long long *)&sh->sdz, pa.pdz);
+        if (reg) {  // only lane 0 took the every-64th samples
+            atomicMin((unsigned long long *)&sh->gmn, pa.gmn);
+            atomicMax((unsigned long long *)&sh->gmx, pa.gmx);
+            atomicMax((unsigned long long *)&sh->gdz, pa.gdz);
+        }
     }
 }
 
@@ -349,9 +433,8 @@ __device__ __forceinline__ void region_stats_body(const EncArgs &a, uint32_t r, 
     } while (0)
 
 __global__ void __launch_bounds__(1024) k_region_stats(EncArgs a) {
-    __shared__ uint64_t seen[256];
-    __shared__ uint64_t smin_s, smax_s, sdz_s;
-    __shared__ uint32_t eq_s, dsum_s;
+    __shared__ uint64_t seen[256], seen32[256];
+    __shared__ RegionShared sh;
     const uint32_t r = blockIdx.x;
     const int c = col_by_region(a.cols, a.ncols, r);
     const DCol &col = a.cols[c];
@@ -364,38 +447,60 @@ __global__ void __launch_bounds__(1024) k_region_stats(EncArgs a) {
     DTile *tl = a.tiles + col.tile_base + (uint64_t)(r - col.region_base) * col.tpr;
     if (rs >= nv) {
         for (uint32_t t = threadIdx.x; t < col.tpr; t += blockDim.x) tl[t].active = 0;
+        if (threadIdx.x == 0) a.rgrown[r].full = 0;
         return;
     }
     const uint64_t len = nv - rs < col.region_size ? nv - rs : col.region_size;
-    for (int k = threadIdx.x; k < 256; k += blockDim.x) seen[k] = 0;
+    // a full adaptive region samples from the tile registers (every 8th value
+    // is the region probe's stride, 16th its step) and records the partials
+    // of the 131072-value block probe (every 32nd / 64th value)
+    const bool reg = col.adaptive && len == QA_BASE_Q;
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+        seen[k] = 0;
+        seen32[k] = 0;
+    }
     if (threadIdx.x == 0) {
-        eq_s = 0;
-        dsum_s = 0;
-        smin_s = ~0ull;
-        smax_s = 0;
-        sdz_s = 0;
+        sh.eq = 0;
+        sh.dsum = 0;
+        sh.smin = ~0ull;
+        sh.smax = 0;
+        sh.sdz = 0;
+        sh.gmn = ~0ull;
+        sh.gmx = 0;
+        sh.gdz = 0;
     }
     __syncthreads();
     const bool sg = is_signed(col.dtype), fastl = col.level != QA_FASTEST;
-    QA_LANE_DISPATCH(region_stats_body, lk, sg, fastl, a, r, col, lp, lk, rs, len, tl, seen, &eq_s,
-                     &smin_s, &smax_s, &sdz_s);
+    QA_LANE_DISPATCH(region_stats_body, lk, sg, fastl, a, r, col, lp, lk, rs, len, tl, seen, seen32,
+                     &sh, reg);
     __syncthreads();
     uint32_t d = 0;
     for (int k = threadIdx.x; k < 256; k += blockDim.x) d += (uint32_t)__popcll(seen[k]);
     d = (uint32_t)wave_reduce((uint64_t)d, OpAdd());
-    if (lane == 0) atomicAdd(&dsum_s, d);
+    if (lane == 0) atomicAdd(&sh.dsum, d);
+    if (reg)
+        for (int k = threadIdx.x; k < 256; k += blockDim.x) a.gsets[(uint64_t)r * 256 + k] = seen32[k];
     __syncthreads();
     if (threadIdx.x == 0) {
-        bool dl = dsum_s < 2048;
-        uint64_t c2 = (uint64_t)(eq_s + 1) * 2;  // consec_eq counts element 0 against itself
+        const uint64_t v0 = qa_lane_get(lp, lk, rs);
+        const uint64_t mn = sh.smin < v0 ? sh.smin : v0, mx = sh.smax > v0 ? sh.smax : v0;
+        bool dl = sh.dsum < 2048;
+        uint64_t c2 = (uint64_t)(sh.eq + 1) * 2;  // consec_eq counts element 0 against itself
         if (c2 > 0xFFFFFFFFull) c2 = 0xFFFFFFFFull;
         bool rp = c2 > (uint32_t)len;
         bool inc = false;
         if (!is_float(col.dtype) && !dl && !rp && len >= 256) {
             uint32_t thr = (uint32_t)(lane_bytes_of(col.dtype) * 8) - 4;
-            inc = qa_width(smax_s - smin_s) >= thr && qa_width(sdz_s) >= thr;
+            inc = qa_width(mx - mn) >= thr && qa_width(sh.sdz) >= thr;
         }
         a.rfeat[r] = (dl ? RF_DLOW : 0) | (rp ? RF_REP : 0) | (inc ? RF_INC : 0);
+        DRegionG &g = a.rgrown[r];
+        g.full = reg;
+        g.gmn = sh.gmn;
+        g.gmx = sh.gmx;
+        g.gdz = sh.gdz;
+        g.v0 = v0;
+        g.dz0 = rs ? qa_zigzag(v0 - qa_lane_get(lp, lk, rs - 1)) : 0;
     }
 }
 
@@ -545,6 +650,10 @@ __global__ void __launch_bounds__(256) k_grown_feats(EncArgs a) {
     const uint32_t lk = a.cst[bc].lkind;
     const uint64_t n = blen, s0 = b.start;
     const uint32_t bnt = b.ntiles;
+    // a block of four full regions combines their recorded probe partials
+    // (its stride 32 and step 64 are multiples of the regions' own)
+    bool comb = n == QA_MAX_Q && col.region_size == QA_BASE_Q && col.adaptive;
+    for (int j = 0; j < 4 && comb; j++) comb = a.rgrown[slot + j].full != 0;
     for (int k = threadIdx.x; k < 256; k += blockDim.x) seen[k] = 0;
     if (threadIdx.x == 0) {
         uint64_t v0 = qa_lane_get(lp, lk, s0);
@@ -556,7 +665,25 @@ __global__ void __launch_bounds__(256) k_grown_feats(EncArgs a) {
     }
     __syncthreads();
     uint64_t mn = ~0ull, mx = 0, dz = 0;
-    QA_LANE_DISPATCH(grown_samples, lk, false, false, lp, lk, s0, n, seen, &mn, &mx, &dz);
+    if (comb) {
+        for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+            const uint64_t *gs = a.gsets + (uint64_t)slot * 256 + k;
+            seen[k] = gs[0] | gs[256] | gs[512] | gs[768];
+        }
+        if (threadIdx.x < 4) {
+            const DRegionG &g = a.rgrown[slot + threadIdx.x];
+            mn = g.gmn;
+            mx = g.gmx;
+            dz = g.gdz;
+            if (threadIdx.x > 0) {  // a later region's first value is a sample too
+                mn = g.v0 < mn ? g.v0 : mn;
+                mx = g.v0 > mx ? g.v0 : mx;
+                dz = g.dz0 > dz ? g.dz0 : dz;
+            }
+        }
+    } else {
+        QA_LANE_DISPATCH(grown_samples, lk, false, false, lp, lk, s0, n, seen, &mn, &mx, &dz);
+    }
     atomicMin((unsigned long long *)&smin, mn);
     atomicMax((unsigned long long *)&smax, mx);
     atomicMax((unsigned long long *)&sdz, dz);
@@ -789,13 +916,13 @@ __global__ void k_emit_frames(EncArgs a) {
 // residual is ORed into its one or two LDS words. The body is instantiated
 // per lane storage kind so the loads are straight-line code.
 template <int LK>
-__device__ __forceinline__ void emit_tile_t(const void *lp, uint32_t lk, uint32_t dtype,
-                                            uint32_t mode, const DTile &ts, uint32_t bt,
-                                            uint64_t k0, uint32_t cnt, uint8_t *p,
-                                            uint64_t *o64w, int lane) {
+__device__ __forceinline__ void emit_tile_t(const TileEmit &e, uint64_t *o64w, int lane) {
     uint32_t *outw = (uint32_t *)o64w;
-    if (mode == QA_M_RAW) {
-        const uint32_t lb = (uint32_t)lane_bytes_of(dtype);
+    const void *lp = e.lp;
+    const uint32_t lk = e.lk, cnt = e.cnt;
+    const uint64_t k0 = e.k0;
+    if (e.mode == QA_M_RAW) {
+        const uint32_t lb = (uint32_t)lane_bytes_of(e.dtype);
         uint64_t xs[QA_TILE / WAVE];
 #pragma unroll
         for (int i = 0; i < QA_TILE / WAVE; i++) {
@@ -807,7 +934,7 @@ __device__ __forceinline__ void emit_tile_t(const void *lp, uint32_t lk, uint32_
             const uint32_t k = i * WAVE + lane;
             if (k >= cnt) break;
             const uint64_t x = xs[i];
-            if (dtype == QA_F32) outw[k] = qa_lane_to_f32(x);
+            if (e.dtype == QA_F32) outw[k] = qa_lane_to_f32(x);
             else if (lb == 4) outw[k] = (uint32_t)x;
             else {
                 outw[2 * k] = (uint32_t)x;
@@ -815,20 +942,13 @@ __device__ __forceinline__ void emit_tile_t(const void *lp, uint32_t lk, uint32_
             }
         }
         wave_lds_sync();
-        wave_store_bytes(p + (uint64_t)bt * QA_TILE * lb, outw, cnt * lb, lane);
+        wave_store_bytes(e.dst, outw, cnt * lb, lane);
         return;
     }
-    if (mode != QA_M_FOR && mode != QA_M_DELTA) return;
-    const bool sg = is_signed(dtype), delta = mode == QA_M_DELTA;
-    uint64_t mn;
-    if (delta) {
-        uint64_t dmx;
-        tile_dz(ts, bt == 0, &mn, &dmx);
-    } else {
-        mn = sg ? (uint64_t)ts.smin : ts.umin;
-    }
-    const uint32_t w = ts.w;
-    uint8_t *q = p + ts.off;
+    const bool delta = e.mode == QA_M_DELTA;
+    const uint64_t mn = e.mn;
+    const uint32_t w = e.w;
+    uint8_t *q = e.dst;
     if (lane == 0) q[0] = (uint8_t)w;
     if (lane >= 1 && lane < 9) q[lane] = (uint8_t)(mn >> (8 * (lane - 1)));
     if (w == 0) return;
@@ -847,8 +967,8 @@ __device__ __forceinline__ void emit_tile_t(const void *lp, uint32_t lk, uint32_
     for (int i = 0; i < 16; i++) {
         const uint32_t k = (r0 + i) * L + l;
         xs[i] = k < cnt ? lane_get_t<LK>(lp, lk, k0 + k) : 0;
-        ps[i] = (delta && k < cnt && (bt != 0 || k != 0)) ? lane_get_t<LK>(lp, lk, k0 + k - 1)
-                                                          : xs[i];
+        ps[i] = (delta && k < cnt && (!e.first || k != 0)) ? lane_get_t<LK>(lp, lk, k0 + k - 1)
+                                                           : xs[i];
     }
 #pragma unroll
     for (int i = 0; i < 16; i++) {
@@ -880,36 +1000,67 @@ __device__ __forceinline__ void emit_tile_t(const void *lp, uint32_t lk, uint32_
     wave_store_bytes(q + 9, outw, 128 * w, lane);
 }
 
+// one thread per tile: the emit descriptor (block, frame offset, minimum)
+__global__ void k_tile_desc(EncArgs a) {
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.t1) return;
+    TileEmit e{};
+    const DTile &ts = a.tiles[t];
+    const int c = col_by_tile(a.cols, a.ncols, t);
+    const DCol &col = a.cols[c];
+    const DColState &st = a.cst[c];
+    if (ts.active && !st.err) {
+        const uint64_t lt = t - col.tile_base;
+        const uint32_t lr = (uint32_t)(lt / col.tpr);
+        const uint32_t o = a.owner[col.region_base + lr];
+        const DBlock &b = a.blocks[o];
+        const uint32_t bt = (col.region_base + lr - o) * col.tpr + (uint32_t)(lt % col.tpr);
+        const uint32_t mode = b.mode;
+        uint8_t *p = col.dst + b.frame_off + 9;
+        e.lp = st.lanep;
+        e.k0 = b.start + (uint64_t)bt * QA_TILE;
+        e.cnt = ts.cnt;
+        e.lk = (uint8_t)st.lkind;
+        e.dtype = col.dtype;
+        e.first = bt == 0;
+        if (mode == QA_M_RAW) {
+            e.mode = QA_M_RAW;
+            e.dst = p + (uint64_t)bt * QA_TILE * lane_bytes_of(col.dtype);
+        } else if (mode == QA_M_FOR || mode == QA_M_DELTA) {
+            e.mode = (uint8_t)mode;
+            e.w = (uint8_t)ts.w;
+            e.dst = p + ts.off;
+            if (mode == QA_M_DELTA) {
+                uint64_t dmx;
+                tile_dz(ts, bt == 0, &e.mn, &dmx);
+            } else {
+                e.mn = is_signed(col.dtype) ? (uint64_t)ts.smin : ts.umin;
+            }
+        }
+    }
+    a.temit[t] = e;
+}
+
+// One wavefront per tile, four independent tiles per workgroup (LDS is
+// per-wavefront, so only wavefront-level synchronization is used). FastLanes
+// pack (src/bitpack.rs:47-133) straight from the lane: in the u32 form lane l
+// of the 32 packs rows [16 h, 16 h + 16) (h = half of the wavefront), in the
+// u64 form lane l of the 16 packs rows [16 q, 16 q + 16) (q = quarter); each
+// residual is ORed into its one or two LDS words. The body is instantiated
+// per lane storage kind so the loads are straight-line code.
 __global__ void __launch_bounds__(256) k_emit_tiles(EncArgs a) {
     __shared__ uint64_t outs[4][QA_TILE + 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.t1) return;
-    const DTile &ts = a.tiles[t];
-    if (!ts.active) return;
-    const int c = col_by_tile(a.cols, a.ncols, t);
-    const DCol &col = a.cols[c];
-    const DColState &st = a.cst[c];
-    if (st.err) return;
-    const void *lp = st.lanep;
-    const uint32_t lk = st.lkind;
-    const uint64_t lt = t - col.tile_base;
-    const uint32_t lr = (uint32_t)(lt / col.tpr);
-    const uint32_t o = a.owner[col.region_base + lr];
-    const DBlock &b = a.blocks[o];
-    const uint32_t bt = (col.region_base + lr - o) * col.tpr + (uint32_t)(lt % col.tpr);
-    const uint64_t k0 = b.start + (uint64_t)bt * QA_TILE;
-    const uint32_t cnt = ts.cnt;
-    const uint32_t mode = b.mode;
-    if (mode != QA_M_RAW && mode != QA_M_FOR && mode != QA_M_DELTA) return;
-    uint8_t *p = col.dst + b.frame_off + 9;
-    const uint32_t dt = col.dtype;
-    switch (lk) {
-    case QA_LK_U64: emit_tile_t<QA_LK_U64>(lp, lk, dt, mode, ts, bt, k0, cnt, p, outs[wv], lane); break;
-    case QA_LK_U32: emit_tile_t<QA_LK_U32>(lp, lk, dt, mode, ts, bt, k0, cnt, p, outs[wv], lane); break;
-    case QA_LK_I32: emit_tile_t<QA_LK_I32>(lp, lk, dt, mode, ts, bt, k0, cnt, p, outs[wv], lane); break;
-    case QA_LK_F32: emit_tile_t<QA_LK_F32>(lp, lk, dt, mode, ts, bt, k0, cnt, p, outs[wv], lane); break;
-    default: emit_tile_t<QA_LK_ANY>(lp, lk, dt, mode, ts, bt, k0, cnt, p, outs[wv], lane);
+    const TileEmit e = a.temit[t];
+    if (!e.mode) return;
+    switch (e.lk) {
+    case QA_LK_U64: emit_tile_t<QA_LK_U64>(e, outs[wv], lane); break;
+    case QA_LK_U32: emit_tile_t<QA_LK_U32>(e, outs[wv], lane); break;
+    case QA_LK_I32: emit_tile_t<QA_LK_I32>(e, outs[wv], lane); break;
+    case QA_LK_F32: emit_tile_t<QA_LK_F32>(e, outs[wv], lane); break;
+    default: emit_tile_t<QA_LK_ANY>(e, outs[wv], lane);
     }
 }
 
@@ -966,6 +1117,7 @@ void qa_encode_layout_launch(const EncArgs &a, hipStream_t s) {
 // per-block stages over [r0, r1) / [t0, t1): frame headers and payloads
 void qa_encode_emit_launch(const EncArgs &a, hipStream_t s) {
     QA_LAUNCH(k_emit_frames, dim3(nb(a.r1 - a.r0, 64)), dim3(64), 0, s, a);
+    QA_LAUNCH(k_tile_desc, dim3(nb(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
     QA_LAUNCH(k_emit_tiles, dim3(nb(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
     qa_fast_emit_launch(a, s);
 }

@@ -59,9 +59,9 @@ struct QuoinGpu {
     std::mutex mu;
     // encode arenas
     DevBuf cols, cst, lane, vpre, tpre, tpos, rsz, colbase, scan_tmp, rfeat, owner, blocks, tiles,
-        wins, tilesf, blocksx, dsorted, rstart, hkeys, hcnt;
+        wins, tilesf, blocksx, dsorted, rstart, hkeys, hcnt, rgrown, gsets, temit;
     // decode arenas
-    DevBuf dcols, dstate, frames, dtiles, dlane, dvpre, derr, dbitmaps, dvbpre, dvruns, dscan,
+    DevBuf dcols, dstate, frames, dtiles, dtdec, dvpre, derr, dbitmaps, dvbpre, dvruns, dscan,
         dfx, dscr, dscr2;
     // host-API staging
     DevBuf st_in, st_valid, st_out, st_vout;
@@ -258,10 +258,11 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
     hipStreamSynchronize(g->stream);
     DevBuf *all[] = {&g->cols,  &g->cst,      &g->lane,   &g->vpre,    &g->tpre,    &g->tpos,
                      &g->rsz,   &g->colbase,  &g->scan_tmp, &g->rfeat, &g->owner,   &g->blocks,
-                     &g->tiles, &g->wins,     &g->dcols,  &g->frames,  &g->dtiles,  &g->dlane,
+                     &g->tiles, &g->wins,     &g->dcols,  &g->frames,  &g->dtiles,  &g->dtdec,
                      &g->dvpre, &g->derr,     &g->dbitmaps, &g->dscan, &g->st_in,   &g->st_valid,
                      &g->dstate, &g->dvbpre, &g->dvruns, &g->tilesf, &g->blocksx, &g->dsorted,
-                     &g->rstart, &g->hkeys, &g->hcnt, &g->dfx, &g->dscr, &g->dscr2,
+                     &g->rstart, &g->hkeys, &g->hcnt, &g->rgrown, &g->gsets, &g->temit, &g->dfx,
+                     &g->dscr, &g->dscr2,
                      &g->st_out, &g->st_vout};
     for (DevBuf *b : all)
         if (b->p) hipFree(b->p);
@@ -334,7 +335,8 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     uint64_t nruns_max = ntrans + ncols;
     size_t scan_words = qa_scan_tmp_words(std::max<uint64_t>(vword_base, nruns_max) + 1);
     uint64_t need = lane_off * 8 + (vword_base + 1) * 16 + ntrans * 8 + (nruns_max + 1) * 8 +
-                    (uint64_t)region_base * (sizeof(DBlock) + 8) + tile_base * sizeof(DTile);
+                    (uint64_t)region_base * (sizeof(DBlock) + sizeof(DRegionG) + 2048 + 8) +
+                    tile_base * sizeof(DTile);
     if (need > g->budget) return QUOIN_ERR_BUFFER_TOO_SMALL;
     if (!ensure(g->cols, ncols * sizeof(DCol)) || !ensure(g->cst, ncols * sizeof(DColState)) ||
         !ensure(g->lane, lane_off * 8) || !ensure(g->vpre, (vword_base + 1) * 8) ||
@@ -342,7 +344,9 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         !ensure(g->rsz, (nruns_max + 1) * 8) || !ensure(g->colbase, (ncols + 2) * 8) ||
         !ensure(g->scan_tmp, scan_words * 8) || !ensure(g->rfeat, region_base * 4ull) ||
         !ensure(g->owner, region_base * 4ull) || !ensure(g->blocks, region_base * sizeof(DBlock)) ||
-        !ensure(g->tiles, tile_base * sizeof(DTile)) || !ensure(g->wins, 64 * 8))
+        !ensure(g->tiles, tile_base * sizeof(DTile)) || !ensure(g->wins, 64 * 8) ||
+        !ensure(g->rgrown, region_base * sizeof(DRegionG)) ||
+        !ensure(g->gsets, region_base * 2048ull) || !ensure(g->temit, tile_base * sizeof(TileEmit)))
         return QUOIN_ERR_DEVICE;
     EncArgs a{};
     a.cols = dp<DCol>(g->cols);
@@ -365,6 +369,9 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     a.tiles = dp<DTile>(g->tiles);
     a.ntrans_max = ntrans;
     a.wins = dp<unsigned long long>(g->wins);
+    a.rgrown = dp<DRegionG>(g->rgrown);
+    a.temit = dp<TileEmit>(g->temit);
+    a.gsets = dp<uint64_t>(g->gsets);
     a.r0 = 0;
     a.r1 = region_base;
     a.t0 = 0;
@@ -516,6 +523,7 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     if (!ensure(g->dcols, ns * sizeof(DDecCol)) || !ensure(g->dstate, ns * sizeof(DDecState)) ||
         !ensure(g->frames, frames.size() * sizeof(DFrame)) ||
         !ensure(g->dtiles, tile_base * sizeof(DTileD)) ||
+        !ensure(g->dtdec, tile_base * sizeof(TileDec)) ||
         !ensure(g->dvpre, (vword_base + 1) * 8) || !ensure(g->derr, 8) ||
         !ensure(g->dbitmaps, (vword_base + 1) * 8) || !ensure(g->dvbpre, (vbyte_base + 1) * 8) ||
         !ensure(g->dvruns, (vbyte_base + 1) * 8) ||
@@ -544,6 +552,7 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     a.vbpre = dp<uint64_t>(g->dvbpre);
     a.vruns = dp<uint64_t>(g->dvruns);
     a.tiles = dp<DTileD>(g->dtiles);
+    a.tdec = dp<TileDec>(g->dtdec);
     a.err = dp<uint32_t>(g->derr);
     a.scan_tmp = dp<uint64_t>(g->dscan);
     if (any_fast) {

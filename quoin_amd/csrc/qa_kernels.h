@@ -10,6 +10,22 @@ void qa_exclusive_scan(const uint64_t *in, uint64_t n, uint64_t *out, uint64_t *
 void qa_exclusive_scan_dn(const uint64_t *in, uint64_t nmax, const uint64_t *ndev, uint64_t *out,
                           uint64_t *tmp, hipStream_t s);
 
+// Everything a wavefront needs to write one tile's payload (RAW, FoR or
+// DeltaBitpack), resolved by k_tile_desc so the emitting wavefront starts
+// with one load instead of a chain of lookups.
+struct TileEmit {
+    const void *lp;   // lane storage of the column
+    uint8_t *dst;     // RAW: the tile's first byte; FoR/Delta: its sub-block header
+    uint64_t k0;      // lane index of the tile's first value
+    uint64_t mn;      // FoR minimum / Delta zig-zag minimum
+    uint32_t cnt;     // values in the tile
+    uint8_t w;        // FastLanes width
+    uint8_t mode;     // QA_M_RAW / FOR / DELTA, 0 = written elsewhere
+    uint8_t lk;       // lane storage kind
+    uint8_t first;    // the first tile of its block (Delta: own predecessor)
+    uint32_t dtype, pad;
+};
+
 // Device pointers of one encode batch (all arrays sized by the host from upper
 // bounds; counts that depend on the data live in DColState).
 struct EncArgs {
@@ -42,6 +58,9 @@ struct EncArgs {
     uint32_t *rstart;      // per lane value slot (+1 per block): RLE run starts
     uint64_t *hkeys;       // 4 per lane value slot: global hash fallback keys
     uint32_t *hcnt;        // 4 per lane value slot: their counts
+    TileEmit *temit;       // per tile slot
+    DRegionG *rgrown;      // per region slot
+    uint64_t *gsets;       // 256 words per region slot
     // launch range of the per-block stages: region slots [r0, r1), tile slots
     // [t0, t1) (the whole batch, or one block in the plain form)
     uint32_t r0, r1;
@@ -91,6 +110,22 @@ struct DTileD {
     uint32_t w;      // FastLanes width of the sub-block at off
 };
 
+// Everything a wavefront needs to decode one tile, resolved by k_dec_desc
+// (frame, stream, output, logical range) so the decoding wavefront starts with
+// one load instead of a chain of lookups.
+struct TileDec {
+    const uint8_t *src;       // the stream
+    void *out;                // typed output
+    const uint64_t *bitmap;   // decoded validity words (streams with nulls)
+    uint64_t off, tail;       // DTileD.off / .tail
+    uint64_t payload;         // stream offset of the frame's payload
+    uint64_t base;            // compact index of the tile's first value
+    uint64_t p0;              // index of the tile's first value within its frame
+    uint64_t L0, L1;          // streams with nulls: logical range of the tile
+    uint32_t frame, cnt, w, mode;  // mode 0: nothing to do
+    uint32_t dtype, oform, nulls, pad;
+};
+
 struct DDecCol {
     const uint8_t *src;     // device stream
     void *out;              // typed output
@@ -133,6 +168,7 @@ struct DecArgs {
     uint64_t *vbpre;       // varint-end prefix over blob bytes (nvbytes + 1)
     uint64_t *vruns;       // run lengths, then their prefix (nvbytes + 1)
     DTileD *tiles;         // per decode tile
+    TileDec *tdec;         // per decode tile
     uint32_t *err;         // error flags
     uint64_t *scan_tmp;
     DFrameX *fx;           // per frame
