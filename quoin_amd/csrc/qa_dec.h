@@ -210,14 +210,14 @@ __device__ __forceinline__ void emit_gather_f(void *out, const uint64_t *bm, uin
         const uint64_t inc = qa_wave_scan_add64(pc);
         const uint64_t ex = inc - pc;
         const uint32_t nw = (uint32_t)(je - c0 + 1 < (uint64_t)WAVE ? je - c0 + 1 : (uint64_t)WAVE);
-        for (uint32_t i = 0; i < nw; i++) {
-            const uint64_t mi = __shfl(m, (int)i, 64);
-            const uint64_t idx = r + __shfl(ex, (int)i, 64) + (uint64_t)__popcll(mi & below);
+        for (uint32_t i = 0; i < nw; i++) {  // word i of the chunk: uniform, scalar reads
+            const uint64_t mi = qa_readlane64(m, (int)i);
+            const uint64_t idx = r + qa_readlane64(ex, (int)i) + (uint64_t)__popcll(mi & below);
             const uint64_t pos = ((c0 + i) << 6) + lane;
             const uint64_t x = ((mi >> lane) & 1) && idx < cnt ? vv[idx] : 0;
             if (pos >= L0 && pos < L1) st_out<F>(out, pos, x);
         }
-        r += __shfl(inc, 63, 64);
+        r += qa_readlane64(inc, 63);
     }
 }
 __device__ __forceinline__ void emit_gather(void *out, uint32_t oform, const uint64_t *bm,

@@ -432,7 +432,7 @@ __device__ __forceinline__ void region_stats_body(const EncArgs &a, uint32_t r, 
         }                                                                           \
     } while (0)
 
-__global__ void __launch_bounds__(1024) k_region_stats(EncArgs a) {
+__global__ void __launch_bounds__(512) k_region_stats(EncArgs a) {
     __shared__ uint64_t seen[256], seen32[256];
     __shared__ RegionShared sh;
     const uint32_t r = blockIdx.x;
@@ -1084,7 +1084,7 @@ void qa_encode_plan_launch(const EncArgs &a, hipStream_t s) {
     }
     QA_LAUNCH(k_col_counts, dim3(nb(a.ncols, 64)), dim3(64), 0, s, a);
     if (a.nvwords) QA_LAUNCH(k_compact, dim3(nb(a.nvwords, 4 * WAVE)), dim3(256), 0, s, a);
-    QA_LAUNCH(k_region_stats, dim3(a.nregions), dim3(1024), 0, s, a);
+    QA_LAUNCH(k_region_stats, dim3(a.nregions), dim3(512), 0, s, a);
     QA_LAUNCH(k_plan, dim3(a.ncols), dim3(256), 0, s, a);
 }
 

@@ -690,6 +690,188 @@ __device__ void dict_values_size(const EncArgs &a, uint32_t slot, const uint64_t
     *vlen = delta < raw ? delta : raw;
 }
 
+// the loads of these loops are straight-line code per lane storage kind, so
+// the eight of a batch are in flight together
+template <int LK>
+__device__ __forceinline__ void bucket_loop(const void *lp, uint32_t lk, uint64_t s0, uint64_t n,
+                                            uint64_t *bits) {
+    const uint64_t bd = blockDim.x;
+    for (uint64_t i0 = threadIdx.x; i0 < n; i0 += 8 * bd) {
+        uint64_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint64_t i = i0 + u * bd;
+            v[u] = i < n ? lane_get_t<LK>(lp, lk, s0 + i) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (i0 + u * bd >= n) break;
+            const uint32_t bk = (uint32_t)(mix64(v[u]) >> 45);
+            atomicOr((unsigned long long *)&bits[bk >> 6], 1ull << (bk & 63));
+        }
+    }
+}
+
+// slot of value v in the 8192-slot LDS table (any function of v gives the
+// same set; the dictionary comes out sorted)
+__device__ __forceinline__ uint32_t dh_hash(uint64_t v) {
+    return ((uint32_t)(v ^ (v >> 32)) * 0x9E3779B1u) >> 19;
+}
+// insert v (not the all-ones value) into the LDS set; *overflow is raised when
+// the set reaches DH_LIMIT or exceeds max_card
+__device__ __forceinline__ void dh_insert(uint64_t *keys, uint64_t v, uint32_t *card_l,
+                                          uint32_t *overflow, uint64_t max_card) {
+    uint32_t h = dh_hash(v);
+    for (uint32_t probe = 0; probe < DH_CAP; probe++) {
+        uint64_t cur = keys[h];
+        if (cur == v) return;
+        if (cur == HEMPTY) {
+            unsigned long long old =
+                atomicCAS((unsigned long long *)&keys[h], HEMPTY, (unsigned long long)v);
+            if (old == HEMPTY) {
+                uint32_t c = atomicAdd(card_l, 1u) + 1;
+                if (c >= DH_LIMIT || c > max_card) *overflow = 1;
+                return;
+            }
+            if (old == v) return;
+        }
+        h = (h + 1) & (DH_CAP - 1);
+    }
+}
+
+// exact insertion of values [s0, s0 + n) into the LDS set (dict phase 1)
+template <int LK>
+__device__ __forceinline__ void insert_loop(const void *lp, uint32_t lk, uint64_t s0, uint64_t n,
+                                            uint64_t *keys, uint32_t *card_l, uint32_t *overflow,
+                                            uint32_t *has_max, uint64_t max_card) {
+    const uint64_t bd = blockDim.x;
+    for (uint64_t i0 = threadIdx.x; i0 < n; i0 += 8 * bd) {
+        if (*(volatile uint32_t *)overflow) break;
+        uint64_t vs[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint64_t i = i0 + u * bd;
+            vs[u] = i < n ? lane_get_t<LK>(lp, lk, s0 + i) : 0;
+        }
+        for (int u = 0; u < 8; u++) {
+            if (i0 + u * bd >= n) break;
+            const uint64_t v = vs[u];
+            if (v == HEMPTY) {
+                *has_max = 1;
+                continue;
+            }
+            dh_insert(keys, v, card_l, overflow, max_card);
+        }
+    }
+}
+
+// Lower bound on the distinct count of values [s0, s0 + n): the number of
+// occupied buckets among 2^19 hash buckets (distinct buckets need distinct
+// values). bits: 8192 LDS words. Every thread returns the bound.
+__device__ uint64_t dict_bucket_bound(const DColState &st, uint64_t s0, uint64_t n, uint64_t *bits) {
+    __shared__ unsigned long long lb;
+    for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x) bits[i] = 0;
+    if (threadIdx.x == 0) lb = 0;
+    __syncthreads();
+    const void *lp = st.lanep;
+    const uint32_t lk = st.lkind;
+    switch (lk) {
+    case QA_LK_U64: bucket_loop<QA_LK_U64>(lp, lk, s0, n, bits); break;
+    case QA_LK_U32: bucket_loop<QA_LK_U32>(lp, lk, s0, n, bits); break;
+    case QA_LK_I32: bucket_loop<QA_LK_I32>(lp, lk, s0, n, bits); break;
+    case QA_LK_F32: bucket_loop<QA_LK_F32>(lp, lk, s0, n, bits); break;
+    default: bucket_loop<QA_LK_ANY>(lp, lk, s0, n, bits);
+    }
+    __syncthreads();
+    uint64_t pc = 0;
+    for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x) pc += __popcll(bits[i]);
+    pc = wave_reduce(pc, OpAdd());
+    if ((threadIdx.x & 63) == 0) atomicAdd(&lb, pc);
+    __syncthreads();
+    const uint64_t r = lb;
+    __syncthreads();
+    return r;
+}
+
+#define QA_LANE_SWITCH(F, lk, ...)                                   \
+    do {                                                             \
+        switch (lk) {                                                \
+        case QA_LK_U64: F<QA_LK_U64>(__VA_ARGS__); break;            \
+        case QA_LK_U32: F<QA_LK_U32>(__VA_ARGS__); break;            \
+        case QA_LK_I32: F<QA_LK_I32>(__VA_ARGS__); break;            \
+        case QA_LK_F32: F<QA_LK_F32>(__VA_ARGS__); break;            \
+        default: F<QA_LK_ANY>(__VA_ARGS__);                          \
+        }                                                            \
+    } while (0)
+
+// A block grown past one region (131072 values) is too much for one
+// workgroup: each of its region slots builds the exact distinct set of its
+// part in LDS and leaves it in dparts (dpart[slot] = count | has-max << 31, or
+// all ones when the part alone reaches the LDS limit). k_dict_set merges them.
+__global__ void __launch_bounds__(1024) k_dict_part(EncArgs a) {
+    __shared__ uint64_t keys[DH_CAP];
+    __shared__ uint32_t card_l, overflow, has_max, wpos;
+    const uint32_t slot = a.r0 + blockIdx.x;
+    if (slot >= a.r1) return;
+    const uint32_t o = a.owner[slot];
+    if (o == NONE32) return;
+    const DBlock &b = a.blocks[o];
+    if (!dict_candidate(a, b)) return;
+    const DCol &col = a.cols[b.col];
+    if (b.len <= col.region_size) return;  // one-region blocks: k_dict_set alone
+    const DColState &st = a.cst[b.col];
+    const uint64_t ps = (uint64_t)(slot - col.region_base) * col.region_size;
+    const uint64_t pe = min(ps + (uint64_t)col.region_size, b.start + (uint64_t)b.len);
+    for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x) keys[i] = HEMPTY;
+    if (threadIdx.x == 0) {
+        card_l = 0;
+        overflow = 0;
+        has_max = 0;
+        wpos = 0;
+    }
+    __syncthreads();
+    QA_LANE_SWITCH(insert_loop, st.lkind, st.lanep, st.lkind, ps, pe - ps, keys, &card_l, &overflow,
+                   &has_max, ~0ull);
+    __syncthreads();
+    if (overflow) {
+        if (threadIdx.x == 0) a.dpart[slot] = NONE32;
+        return;
+    }
+    uint64_t *out = a.dparts + (uint64_t)slot * DH_LIMIT;
+    for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x)
+        if (keys[i] != HEMPTY) out[atomicAdd(&wpos, 1u)] = keys[i];
+    if (threadIdx.x == 0) a.dpart[slot] = card_l | (has_max << 31);
+}
+
+// Move the occupied slots of the LDS set (card_l of them) to the front, in
+// slot order, pad with the all-ones value to a power of two of at least
+// card_l + has_max (the all-ones value then lands at index card_l), and sort
+// that prefix. Each thread reads its 8 slots before anyone writes.
+__device__ void lds_compact_sort(uint64_t *keys, uint32_t card_l, uint32_t has_max) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t per = DH_CAP / blockDim.x;  // 8 at 1024 threads
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t v[8];
+    uint32_t c = 0;
+    for (uint32_t j = 0; j < per; j++) {
+        v[j] = keys[threadIdx.x * per + j];
+        c += v[j] != HEMPTY;
+    }
+    const uint32_t incl = (uint32_t)qa_wave_scan_add64(c);
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    uint32_t base = incl - c;
+    for (int q = 0; q < wv; q++) base += wsum[q];
+    for (uint32_t j = 0; j < per; j++)
+        if (v[j] != HEMPTY) keys[base++] = v[j];
+    uint32_t np2 = 2;
+    while (np2 < card_l + has_max) np2 <<= 1;
+    __syncthreads();
+    for (uint32_t i = card_l + threadIdx.x; i < np2; i += blockDim.x) keys[i] = HEMPTY;
+    __syncthreads();
+    lds_bitonic_sort(keys, np2);
+}
+
 __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
     __shared__ uint64_t keys[DH_CAP];
     __shared__ uint32_t card_l, overflow, has_max;
@@ -703,6 +885,27 @@ __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
     const uint64_t n = b.len, max_card = n / 2 + 1;
     DTileF *tf = a.tilesf + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
     uint64_t *sorted = a.dsorted + col.lane_off + b.start;
+    // a grown block whose parts all fit: merge their sets (at most 4 x 6143
+    // keys, below n / 2 + 1, so no bail is possible from the count alone)
+    const uint32_t nparts = (uint32_t)((n + col.region_size - 1) / col.region_size);
+    bool parts_ok = nparts > 1;
+    for (uint32_t j = 0; j < nparts && parts_ok; j++) parts_ok = a.dpart[slot + j] != NONE32;
+    // a block whose probe saw many distinct values first tries the cheap
+    // certificate that its cardinality exceeds n / 2 + 1 (dict.rs bails then)
+    bool lb_known = false;
+    if (!b.distinct_low && !parts_ok) {
+        const uint64_t lb = dict_bucket_bound(st, b.start, n, keys);
+        if (lb > max_card) {
+            if (threadIdx.x == 0) {
+                x.dict_state = 1;
+                x.card = 0;
+                x.dict_vlen = 0;
+                x.dict_vtag = 0;
+            }
+            return;
+        }
+        lb_known = true;
+    }
     for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x) keys[i] = HEMPTY;
     if (threadIdx.x == 0) {
         card_l = 0;
@@ -710,30 +913,22 @@ __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
         has_max = 0;
     }
     __syncthreads();
-    // phase 1: exact set in LDS (the all-ones value is tracked by a flag)
-    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        if (overflow) break;
-        uint64_t v = lane_at(st, b.start + i);
-        if (v == HEMPTY) {
-            has_max = 1;
-            continue;
-        }
-        uint32_t h = (uint32_t)(mix64(v) >> 51);
-        for (uint32_t probe = 0; probe < DH_CAP; probe++) {
-            uint64_t cur = keys[h];
-            if (cur == v) break;
-            if (cur == HEMPTY) {
-                unsigned long long old =
-                    atomicCAS((unsigned long long *)&keys[h], HEMPTY, (unsigned long long)v);
-                if (old == HEMPTY) {
-                    uint32_t c = atomicAdd(&card_l, 1u) + 1;
-                    if (c >= DH_LIMIT || c > max_card) overflow = 1;
-                    break;
-                }
-                if (old == v) break;
+    // phase 1: exact set in LDS (the all-ones value is tracked by a flag),
+    // from the parts' sets or from the values
+    if (parts_ok) {
+        for (uint32_t j = 0; j < nparts; j++) {
+            const uint32_t pc = a.dpart[slot + j];
+            if (threadIdx.x == 0 && (pc >> 31)) has_max = 1;
+            const uint64_t *pk = a.dparts + (uint64_t)(slot + j) * DH_LIMIT;
+            const uint32_t cnt = pc & 0x7FFFFFFFu;
+            for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+                if (*(volatile uint32_t *)&overflow) break;
+                dh_insert(keys, pk[i], &card_l, &overflow, max_card);
             }
-            h = (h + 1) & (DH_CAP - 1);
         }
+    } else {
+        QA_LANE_SWITCH(insert_loop, st.lkind, st.lanep, st.lkind, b.start, n, keys, &card_l,
+                       &overflow, &has_max, max_card);
     }
     __syncthreads();
     uint64_t card = 0;
@@ -746,21 +941,8 @@ __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
         bail = true;  // the LDS set already holds more distinct values than allowed
     } else {
         // phase 2: a lower bound on the cardinality from 2^19 hash buckets
-        for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x) keys[i] = 0;
-        __syncthreads();
-        for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
-            uint32_t bk = (uint32_t)(mix64(lane_at(st, b.start + i)) >> 45);
-            atomicOr((unsigned long long *)&keys[bk >> 6], 1ull << (bk & 63));
-        }
-        __syncthreads();
-        uint64_t pc = 0;
-        for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x) pc += __popcll(keys[i]);
-        pc = wave_reduce(pc, OpAdd());
-        __shared__ unsigned long long lb;
-        if (threadIdx.x == 0) lb = 0;
-        __syncthreads();
-        if ((threadIdx.x & 63) == 0) atomicAdd(&lb, pc);
-        __syncthreads();
+        // (already known to be within the limit when it ran first)
+        const uint64_t lb = lb_known ? 0 : dict_bucket_bound(st, b.start, n, keys);
         if (lb > max_card) {
             bail = true;
         } else {
@@ -835,7 +1017,7 @@ __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
         }
     }
     if (exact && !bail) {
-        lds_bitonic_sort(keys, DH_CAP);
+        lds_compact_sort(keys, card_l, has_max);
         for (uint32_t i = threadIdx.x; i < card; i += blockDim.x) sorted[i] = keys[i];
         __threadfence_block();
         __syncthreads();
@@ -1117,6 +1299,7 @@ void qa_fast_sizes_launch(const EncArgs &a, hipStream_t s) {
 // Dict / RLE sizing, then the final decision and layout
 void qa_fast_final_launch(const EncArgs &a, hipStream_t s) {
     if (a.fast) {
+        QA_LAUNCH(k_dict_part, dim3(a.r1 - a.r0), dim3(1024), 0, s, a);
         QA_LAUNCH(k_dict_set, dim3(a.r1 - a.r0), dim3(1024), 0, s, a);
         QA_LAUNCH(k_dict_tiles, dim3(nbf(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
         QA_LAUNCH(k_rle_starts, dim3(nbf(a.t1 - a.t0, 4)), dim3(256), 0, s, a);

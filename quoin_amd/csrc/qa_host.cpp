@@ -59,7 +59,7 @@ struct QuoinGpu {
     std::mutex mu;
     // encode arenas
     DevBuf cols, cst, lane, vpre, tpre, tpos, rsz, colbase, scan_tmp, rfeat, owner, blocks, tiles,
-        wins, tilesf, blocksx, dsorted, rstart, hkeys, hcnt, rgrown, gsets, temit;
+        wins, tilesf, blocksx, dsorted, rstart, hkeys, hcnt, rgrown, gsets, temit, dpart, dparts;
     // decode arenas
     DevBuf dcols, dstate, frames, dtiles, dtdec, dvpre, derr, dbitmaps, dvbpre, dvruns, dscan,
         dfx, dscr, dscr2;
@@ -261,7 +261,7 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
                      &g->tiles, &g->wins,     &g->dcols,  &g->frames,  &g->dtiles,  &g->dtdec,
                      &g->dvpre, &g->derr,     &g->dbitmaps, &g->dscan, &g->st_in,   &g->st_valid,
                      &g->dstate, &g->dvbpre, &g->dvruns, &g->tilesf, &g->blocksx, &g->dsorted,
-                     &g->rstart, &g->hkeys, &g->hcnt, &g->rgrown, &g->gsets, &g->temit, &g->dfx,
+                     &g->rstart, &g->hkeys, &g->hcnt, &g->rgrown, &g->gsets, &g->temit, &g->dpart, &g->dparts, &g->dfx,
                      &g->dscr, &g->dscr2,
                      &g->st_out, &g->st_vout};
     for (DevBuf *b : all)
@@ -382,7 +382,8 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         if (!ensure(g->tilesf, tile_base * sizeof(DTileF)) ||
             !ensure(g->blocksx, region_base * sizeof(DBlockX)) || !ensure(g->dsorted, lane_off * 8) ||
             !ensure(g->rstart, lane_off * 4) || !ensure(g->hkeys, lane_off * 32) ||
-            !ensure(g->hcnt, lane_off * 16))
+            !ensure(g->hcnt, lane_off * 16) || !ensure(g->dpart, region_base * 4ull) ||
+            !ensure(g->dparts, region_base * 6144ull * 8))
             return QUOIN_ERR_DEVICE;
         a.fast = 1;
         a.tilesf = dp<DTileF>(g->tilesf);
@@ -391,6 +392,8 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         a.rstart = dp<uint32_t>(g->rstart);
         a.hkeys = dp<uint64_t>(g->hkeys);
         a.hcnt = dp<uint32_t>(g->hcnt);
+        a.dpart = dp<uint32_t>(g->dpart);
+        a.dparts = dp<uint64_t>(g->dparts);
     }
     if (hipMemcpyAsync(g->cols.p, cols.data(), ncols * sizeof(DCol), hipMemcpyHostToDevice, s) !=
         hipSuccess)

@@ -59,6 +59,8 @@ struct EncArgs {
     uint64_t *hkeys;       // 4 per lane value slot: global hash fallback keys
     uint32_t *hcnt;        // 4 per lane value slot: their counts
     TileEmit *temit;       // per tile slot
+    uint32_t *dpart;       // per region slot: Dict part set size (Level::Fast)
+    uint64_t *dparts;      // per region slot: DH_LIMIT keys of its part set
     DRegionG *rgrown;      // per region slot
     uint64_t *gsets;       // 256 words per region slot
     // launch range of the per-block stages: region slots [r0, r1), tile slots
