@@ -39,6 +39,23 @@ def make_workload(n, seed=0):
     return cols
 
 
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    counter summary (profiles/traffic_<workload>.json: rocprofv3 FETCH_SIZE
+    and WRITE_SIZE passes, scaled by the same counters on a plain copy of
+    known size), or None when no summary matches this configuration."""
+    path = os.path.join(ROOT, "profiles", f"traffic_intlanes_{args.level}.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if t.get("values_per_column") != args.n or not k:
+        return None
+    return k.get("hbm_bytes_per_launch")
+
+
 def cpu_baseline(cols, level, sample_n):
     """Oracle (single-threaded C restatement) over a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -206,6 +223,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="steps of the plain form (one launch sequence per block); 0 skips it")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="bytes of a plain elementwise copy run after the measurement "
+                         "(for scaling HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU in this build; --gpus {args.gpus} is not supported "
@@ -267,15 +287,20 @@ def main():
     torch.cuda.synchronize()
     tc = td = 0.0
     dev_c = dev_d = 0.0
+    kern_c = kern_d = 0.0  # HIP events on the library's stream around its tile kernels
     t_start = time.perf_counter()
     for _ in range(args.steps):
         t0 = time.perf_counter()
         gpu.compress(dcols, cfg)
         t1 = time.perf_counter()
-        dev_c += gpu.last_timing()[0]
+        tot, kms, _ = gpu.last_timing()
+        dev_c += tot
+        kern_c += kms
         gpu.decompress(dsts)
         t2 = time.perf_counter()
-        dev_d += gpu.last_timing()[0]
+        tot, kms, _ = gpu.last_timing()
+        dev_d += tot
+        kern_d += kms
         tc += t1 - t0
         td += t2 - t1
     torch.cuda.synchronize()
@@ -303,12 +328,34 @@ def main():
     dev_c_ms = dev_c / args.steps
     dev_d_ms = dev_d / args.steps
 
-    # decode roofline: algorithmic bytes = compressed payload read + typed output written
-    dec_alg = comp_bytes + raw_bytes
-    dec_gbs = dec_alg / (dev_d_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "decode sequence (all stages)",
-                "achieved": round(dec_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dec_gbs / HBM_PEAK_GBS, 4), "traffic": None}
+    # Roofline of the dominant decode kernel (k_dec_tiles, one launch per
+    # decompress call): algorithmic bytes per launch = the stream bytes it
+    # reads (an upper bound of the frame payloads) + the typed values it writes
+    # + the decoded validity words it reads for the streams with nulls.
+    bitmap_bytes = sum(((len(v) + 63) // 64) * 8 for _, v, bm in cols if bm is not None)
+    dec_alg = comp_bytes + raw_bytes + bitmap_bytes
+    kd_ms = kern_d / args.steps
+    dec_gbs = dec_alg / (kd_ms * 1e-3) / 1e9 if kd_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": "k_dec_tiles", "achieved": round(dec_gbs, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dec_gbs / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic("k_dec_tiles", args), "algorithmic_bytes": dec_alg,
+                "kernel_ms": round(kd_ms, 4),
+                "decode_sequence_gbs": round((comp_bytes + raw_bytes) / (dev_d_ms * 1e-3) / 1e9, 1)}
+    # the encode side's tile kernel: reads the (compacted) lane, writes payloads
+    kc_ms = kern_c / args.steps
+    enc_alg = raw_bytes + comp_bytes
+    enc_roofline = {"bound": "hbm", "kernel": "k_emit_tiles", "kernel_ms": round(kc_ms, 4),
+                    "achieved": round(enc_alg / (kc_ms * 1e-3) / 1e9, 1) if kc_ms > 0 else 0.0,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_bytes": enc_alg}
+    enc_roofline["frac"] = round(enc_roofline["achieved"] / HBM_PEAK_GBS, 4)
+
+    if args.calibrate_copy:
+        # a plain copy of known size for scaling FETCH_SIZE / WRITE_SIZE counters
+        src = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
 
     serving = None
     if not args.no_serve:
@@ -337,6 +384,7 @@ def main():
                    "plain_form_gbs": round(plain, 3) if plain else None,
                    "batched_over_plain": round(value / plain, 2) if plain else None},
         "roofline": roofline,
+        "encode_roofline": enc_roofline,
         "serving": serving,
         "cpu_baseline": cpu,
     }

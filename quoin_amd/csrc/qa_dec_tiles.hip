@@ -348,5 +348,7 @@ static inline unsigned nbt(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
     if (a.t1 <= a.t0) return;
     hipLaunchKernelGGL(k_dec_desc, dim3(nbt(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
+    if (a.kev0) hipEventRecord((hipEvent_t)a.kev0, s);
     hipLaunchKernelGGL(k_dec_tiles, dim3(nbt(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
+    if (a.kev1) hipEventRecord((hipEvent_t)a.kev1, s);
 }

@@ -1009,10 +1009,11 @@ __global__ void k_tile_desc(EncArgs a) {
     const int c = col_by_tile(a.cols, a.ncols, t);
     const DCol &col = a.cols[c];
     const DColState &st = a.cst[c];
-    if (ts.active && !st.err) {
-        const uint64_t lt = t - col.tile_base;
-        const uint32_t lr = (uint32_t)(lt / col.tpr);
-        const uint32_t o = a.owner[col.region_base + lr];
+    const uint64_t lt = t - col.tile_base;
+    const uint32_t lr = (uint32_t)(lt / col.tpr);
+    const uint32_t o = lr < col.nregions ? a.owner[col.region_base + lr] : NONE32;
+    if (o != NONE32 && !st.err && a.blocks[o].len) e.bmode = a.blocks[o].mode;
+    if (ts.active && !st.err && o != NONE32) {
         const DBlock &b = a.blocks[o];
         const uint32_t bt = (col.region_base + lr - o) * col.tpr + (uint32_t)(lt % col.tpr);
         const uint32_t mode = b.mode;
@@ -1118,7 +1119,9 @@ void qa_encode_layout_launch(const EncArgs &a, hipStream_t s) {
 void qa_encode_emit_launch(const EncArgs &a, hipStream_t s) {
     QA_LAUNCH(k_emit_frames, dim3(nb(a.r1 - a.r0, 64)), dim3(64), 0, s, a);
     QA_LAUNCH(k_tile_desc, dim3(nb(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
+    if (a.kev0) hipEventRecord((hipEvent_t)a.kev0, s);
     QA_LAUNCH(k_emit_tiles, dim3(nb(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
+    if (a.kev1) hipEventRecord((hipEvent_t)a.kev1, s);
     qa_fast_emit_launch(a, s);
 }
 

@@ -872,6 +872,80 @@ __device__ void lds_compact_sort(uint64_t *keys, uint32_t card_l, uint32_t has_m
     lds_bitonic_sort(keys, np2);
 }
 
+// insertion of the values of [s0, s0 + n) that fall in [lo, hi) (the
+// all-ones value excluded) into the LDS set, overflowing at DH_LIMIT
+template <int LK>
+__device__ __forceinline__ void range_insert_loop(const void *lp, uint32_t lk, uint64_t s0,
+                                                  uint64_t n, uint64_t lo, uint64_t hi,
+                                                  uint64_t *keys, uint32_t *cnt, uint32_t *ovf) {
+    const uint64_t bd = blockDim.x;
+    for (uint64_t i0 = threadIdx.x; i0 < n; i0 += 8 * bd) {
+        if (*(volatile uint32_t *)ovf) break;
+        uint64_t vs[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint64_t i = i0 + u * bd;
+            vs[u] = i < n ? lane_get_t<LK>(lp, lk, s0 + i) : HEMPTY;
+        }
+        for (int u = 0; u < 8; u++) {
+            const uint64_t v = vs[u];
+            if (v != HEMPTY && v >= lo && v < hi) dh_insert(keys, v, cnt, ovf, ~0ull);
+        }
+    }
+}
+
+// Exact sorted dictionary of a block whose cardinality is above the LDS set's
+// limit: the values are split into P ranges at splitters taken from a sorted
+// sample of 4096 values (about 2048 distinct values per range on average);
+// each range's exact set is built and sorted in LDS and appended to sorted[]
+// in range order, which is value order. Returns 0 (done: *card_out), 1 (more
+// than max_card distinct values) or 2 (a range reached the LDS limit).
+__device__ int dict_ranges(const DColState &st, uint64_t s0, uint64_t n, uint64_t max_card,
+                           uint32_t has_max, uint64_t *keys, uint64_t *sorted,
+                           uint64_t *card_out) {
+    __shared__ uint64_t split[66];
+    __shared__ uint32_t cnt_s, ovf_s;
+    const void *lp = st.lanep;
+    const uint32_t lk = st.lkind;
+    const uint32_t ns = (uint32_t)(n < 4096 ? n : 4096);
+    const uint64_t stride = n / ns;
+    for (uint32_t i = threadIdx.x; i < 4096; i += blockDim.x)
+        keys[i] = i < ns ? qa_lane_get(lp, lk, s0 + (uint64_t)i * stride) : HEMPTY;
+    __syncthreads();
+    lds_bitonic_sort(keys, 4096);
+    uint32_t P = (uint32_t)((max_card + 2047) / 2048);
+    if (P < 2) P = 2;
+    if (P > 64) P = 64;
+    if (threadIdx.x <= P)
+        split[threadIdx.x] = threadIdx.x == 0 ? 0 : (threadIdx.x == P ? HEMPTY : keys[threadIdx.x * ns / P]);
+    __syncthreads();
+    uint64_t base = 0;
+    for (uint32_t pr = 0; pr < P; pr++) {
+        const uint64_t lo = split[pr], hi = split[pr + 1];
+        if (lo >= hi) continue;
+        for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x) keys[i] = HEMPTY;
+        if (threadIdx.x == 0) {
+            cnt_s = 0;
+            ovf_s = 0;
+        }
+        __syncthreads();
+        QA_LANE_SWITCH(range_insert_loop, lk, lp, lk, s0, n, lo, hi, keys, &cnt_s, &ovf_s);
+        __syncthreads();
+        const uint32_t c = cnt_s;
+        if (ovf_s) return 2;
+        if (base + c + has_max > max_card) return 1;
+        lds_compact_sort(keys, c, 0);
+        for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) sorted[base + i] = keys[i];
+        base += c;
+        __syncthreads();
+    }
+    if (has_max && threadIdx.x == 0) sorted[base] = HEMPTY;
+    *card_out = base + has_max;
+    __threadfence_block();
+    __syncthreads();
+    return 0;
+}
+
 __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
     __shared__ uint64_t keys[DH_CAP];
     __shared__ uint32_t card_l, overflow, has_max;
@@ -943,10 +1017,18 @@ __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
         // phase 2: a lower bound on the cardinality from 2^19 hash buckets
         // (already known to be within the limit when it ran first)
         const uint64_t lb = lb_known ? 0 : dict_bucket_bound(st, b.start, n, keys);
+        int ranged = 2;
         if (lb > max_card) {
             bail = true;
         } else {
-            // phase 3: exact set in the block's global table
+            // phase 3: exact sets of value ranges in LDS
+            uint64_t rc = 0;
+            ranged = dict_ranges(st, b.start, n, max_card, has_max, keys, sorted, &rc);
+            if (ranged == 0) card = rc;
+            if (ranged == 1) bail = true;
+        }
+        if (!bail && ranged == 2) {
+            // phase 3b (a range overflowed): exact set in the block's global table
             uint32_t cap = 1;
             while (cap < n + 4) cap <<= 1;
             uint64_t *gk = a.hkeys + 4 * (col.lane_off + b.start);

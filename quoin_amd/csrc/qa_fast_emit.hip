@@ -134,6 +134,12 @@ __global__ void __launch_bounds__(64) k_emit_fast_tiles(EncArgs a) {
     const int lane = threadIdx.x;
     const uint64_t t = a.t0 + blockIdx.x;
     if (t >= a.t1) return;
+    {  // the emit descriptor knows the block's mode: leave after one load
+        const uint32_t bm = a.temit[t].bmode;
+        if (bm != QA_M_XORZ && bm != QA_M_FLOAT_MULT && bm != QA_M_ALP && bm != QA_M_ALP_RD &&
+            bm != QA_M_DICT)
+            return;
+    }
     const DTile &ts = a.tiles[t];
     if (!ts.active) return;
     int c;
@@ -326,6 +332,10 @@ __global__ void __launch_bounds__(64) k_emit_fast_chunks(EncArgs a) {
     const int lane = threadIdx.x;
     const uint64_t t = a.t0 + blockIdx.x;
     if (t >= a.t1) return;
+    {
+        const uint32_t bm = a.temit[t].bmode;
+        if (bm != QA_M_DICT && bm != QA_M_RLE) return;
+    }
     int c;
     uint32_t o, bt;
     if (!owner_of(a, t, &c, &o, &bt)) return;

@@ -65,7 +65,7 @@ struct QuoinGpu {
         dfx, dscr, dscr2;
     // host-API staging
     DevBuf st_in, st_valid, st_out, st_vout;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, kev0 = nullptr, kev1 = nullptr;
     double last_total_ms = 0, last_kernel_ms = 0;
     const char *last_kernel = "";
 };
@@ -245,7 +245,8 @@ extern "C" QuoinGpu *quoin_gpu_create(int device, uint64_t hbm_budget_bytes) {
     g->device = device;
     g->budget = hbm_budget_bytes ? hbm_budget_bytes : kDefaultBudget;
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
+        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess ||
+        hipEventCreate(&g->kev0) != hipSuccess || hipEventCreate(&g->kev1) != hipSuccess) {
         delete g;
         return nullptr;
     }
@@ -268,6 +269,8 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
         if (b->p) hipFree(b->p);
     hipEventDestroy(g->ev0);
     hipEventDestroy(g->ev1);
+    hipEventDestroy(g->kev0);
+    hipEventDestroy(g->kev1);
     hipStreamDestroy(g->stream);
     delete g;
 }
@@ -402,6 +405,8 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     zero_async(g->cst.p, ncols * sizeof(DColState), s);
     zero_async(g->wins.p, 64 * 8, s);
     if (g->form == 0) {
+        a.kev0 = g->kev0;
+        a.kev1 = g->kev1;
         qa_encode_launch(a, s, level);
     } else {
         // plain form: column-level planning, then one launch sequence per
@@ -449,6 +454,13 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     g->last_total_ms = ms;
     g->last_kernel_ms = 0;
     g->last_kernel = "";
+    if (g->form == 0 && tile_base) {
+        float kms = 0;
+        if (hipEventElapsedTime(&kms, g->kev0, g->kev1) == hipSuccess) {
+            g->last_kernel_ms = kms;
+            g->last_kernel = "k_emit_tiles";
+        }
+    }
     for (int m = 0; m < 64; m++) g_wins[m] += wins[m];
     int rc = QUOIN_OK;
     for (size_t c = 0; c < ncols; c++) {
@@ -572,6 +584,8 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     zero_async(g->derr.p, 8, s);
     zero_async(g->dstate.p, ns * sizeof(DDecState), s);
     if (g->form == 0) {
+        a.kev0 = g->kev0;
+        a.kev1 = g->kev1;
         qa_decode_launch(a, s);
     } else {
         // plain form: one launch sequence per frame (the tile starts of
@@ -598,6 +612,15 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     float ms = 0;
     hipEventElapsedTime(&ms, g->ev0, g->ev1);
     g->last_total_ms = ms;
+    g->last_kernel_ms = 0;
+    g->last_kernel = "";
+    if (g->form == 0 && tile_base && !frames.empty()) {
+        float kms = 0;
+        if (hipEventElapsedTime(&kms, g->kev0, g->kev1) == hipSuccess) {
+            g->last_kernel_ms = kms;
+            g->last_kernel = "k_dec_tiles";
+        }
+    }
     if (err & QA_ERR_UNSUPPORTED) return QUOIN_ERR_UNSUPPORTED;
     if (err) return QUOIN_ERR_CORRUPT;
     return QUOIN_OK;

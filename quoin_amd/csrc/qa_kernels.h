@@ -23,7 +23,8 @@ struct TileEmit {
     uint8_t mode;     // QA_M_RAW / FOR / DELTA, 0 = written elsewhere
     uint8_t lk;       // lane storage kind
     uint8_t first;    // the first tile of its block (Delta: own predecessor)
-    uint32_t dtype, pad;
+    uint32_t dtype;
+    uint32_t bmode;   // mode of the block owning this tile slot (0: none, or failed column)
 };
 
 // Device pointers of one encode batch (all arrays sized by the host from upper
@@ -67,6 +68,8 @@ struct EncArgs {
     // [t0, t1) (the whole batch, or one block in the plain form)
     uint32_t r0, r1;
     uint64_t t0, t1;
+    // host side: HIP events recorded around the tile emit kernel, or null
+    void *kev0, *kev1;
 };
 
 void qa_fast_sizes_launch(const EncArgs &a, hipStream_t s);
@@ -181,6 +184,8 @@ struct DecArgs {
     // (the whole batch, or one frame in the plain form)
     uint32_t f0, f1;
     uint64_t t0, t1;
+    // host side: HIP events recorded around the tile decode kernel, or null
+    void *kev0, *kev1;
 };
 
 void qa_decode_launch(const DecArgs &a, hipStream_t s);

@@ -82,7 +82,14 @@ def test_codec_coverage_fast(qa):
         "specials": np.array([0.0, -0.0, np.nan, np.inf, -np.inf, 1e308, 5e-324] * 100),
         "dict_card_mid": r.integers(0, 3000, n).astype(np.int64) * 1_000_003,
         "dict_card_hi": r.integers(0, 20000, 131072).astype(np.int64) * 7,
-        "alprd_normal": r.normal(0, 1, n),
+        # distinct counts above the LDS set limit (6144) and within n / 2 + 1:
+        # the value-range path, a skewed split that falls back to the global
+        # table, and a grown (repeats) block with 32768 distinct values
+        "dict_card_ranged": r.integers(0, 9000, n).astype(np.int64) * 13,
+        "dict_card_skewed": r.permutation(np.concatenate([
+            np.repeat(r.integers(0, 100, 100).astype(np.int64), 260),
+            1_000_000 + np.arange(6700, dtype=np.int64) * 3]))[: 32768],
+        "dict_grown_repeats": np.repeat(r.permutation(40000).astype(np.int64) * 5, 4)[:131072],
         "alprd_normal_exc": np.concatenate([r.normal(0, 1, n - 50), r.random(50) * 1e300]),
     }
     for name, v in cases.items():
