@@ -201,6 +201,48 @@ struct ProbeAcc {
     uint64_t pmn, pmx, pdz, gmn, gmx, gdz;
 };
 
+// 16 consecutive lane values from index i0: vector loads when the address is
+// 16-byte aligned, else one load per value
+template <int LK>
+__device__ __forceinline__ void load16(const void *lp, uint32_t lk, uint64_t i0, uint64_t (&v)[16]) {
+    if (LK == QA_LK_U32 || LK == QA_LK_I32 || LK == QA_LK_F32) {
+        const uint32_t *p = (const uint32_t *)lp + i0;
+        if (((uintptr_t)p & 15) == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint4 w = ((const uint4 *)p)[j];
+                const uint32_t e[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (LK == QA_LK_U32) v[4 * j + u] = e[u];
+                    else if (LK == QA_LK_I32) v[4 * j + u] = (uint64_t)(int64_t)(int32_t)e[u];
+                    else v[4 * j + u] = qa_f32_to_lane(e[u]);
+                }
+            }
+            return;
+        }
+    } else if (LK == QA_LK_U64) {
+        const uint64_t *p = (const uint64_t *)lp + i0;
+        if (((uintptr_t)p & 15) == 0) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const ulonglong2 w = ((const ulonglong2 *)p)[j];
+                v[2 * j] = w.x;
+                v[2 * j + 1] = w.y;
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = lane_get_t<LK>(lp, lk, i0 + u);
+}
+
+// Each lane takes 16 consecutive values of the tile (lane L: values 16 L ..
+// 16 L + 15), so neighbours are registers of the same lane (the first one is
+// the lane below's last, or the value before the tile) and the probe's sample
+// positions are fixed register slots: every 8th value of the region is slot 0
+// or 8, every 16th slot 0, every 32nd slot 0 of even lanes, every 64th slot 0
+// of every fourth lane.
 template <int LK, bool SG, bool FASTL, bool REG>
 __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp, uint32_t lk,
                                                  uint64_t rs, uint64_t len, DTile *tl,
@@ -221,67 +263,61 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
         const uint64_t vf = lane_get_t<LK>(lp, lk, k0);
         const uint64_t vl = lane_get_t<LK>(lp, lk, k0 + cnt - 1);
         const uint64_t step = cnt > 1 ? lane_get_t<LK>(lp, lk, k0 + 1) - vf : 0;
-        uint64_t umin = ~0ull, umax = 0, dzmin = ~0ull, dzmax = 0, xb = 0;
-        int64_t smin = INT64_MAX, smax = INT64_MIN;
-        uint32_t eqin = 0, eqp = 0;
-        // every load of the tile in flight before the first use
-        uint64_t xs[QA_TILE / WAVE];
+        const uint32_t kb = 16u * (uint32_t)lane;  // this lane's first tile index
+        uint64_t v[16];
+        if (kb + 16 <= cnt) {
+            load16<LK>(lp, lk, k0 + kb, v);
+        } else {
 #pragma unroll
-        for (int q = 0; q < QA_TILE / WAVE; q++) {
-            const uint32_t k = q * WAVE + lane;
-            xs[q] = k < cnt ? lane_get_t<LK>(lp, lk, k0 + k) : vf;
+            for (int u = 0; u < 16; u++) v[u] = kb + u < cnt ? lane_get_t<LK>(lp, lk, k0 + kb + u) : vf;
         }
-        uint64_t carry = before;
+        // the value before this lane's first: the lane below's last
+        const uint64_t pv0 = qa_dpp64<QA_DPP_WAVE_SHR1>(before, v[15]);
+        uint64_t umin = ~0ull, umax = 0, dzmin = ~0ull, dzmax = 0, xb = 0;
+        uint64_t smin = 0x7FFFFFFFFFFFFFFFull, smax = 0x8000000000000000ull;
+        uint32_t eqin = 0, eqx = 0;
 #pragma unroll
-        for (int q = 0; q < QA_TILE / WAVE; q++) {
-            const uint32_t k = q * WAVE + lane;
+        for (int i = 0; i < 16; i++) {
+            const uint32_t k = kb + i;
             const bool in = k < cnt;
-            const uint64_t x = xs[q];  // out-of-range lanes hold vf
-            // previous element: the lane below, or the carry for lane 0
-            const uint64_t p = qa_dpp64<QA_DPP_WAVE_SHR1>(carry, x);
-            carry = qa_readlane64(x, 63);
+            const uint64_t x = v[i];  // past cnt: vf, neutral for min and max
+            const uint64_t p = i ? v[i - 1] : pv0;
             umin = x < umin ? x : umin;
             umax = x > umax ? x : umax;
             if (SG) {
-                smin = (int64_t)x < smin ? (int64_t)x : smin;
-                smax = (int64_t)x > smax ? (int64_t)x : smax;
+                smin = (int64_t)x < (int64_t)smin ? x : smin;
+                smax = (int64_t)x > (int64_t)smax ? x : smax;
             }
+            const uint64_t z = qa_zigzag(x - p);
             const bool pair = in && k > 0;  // neighbours inside the tile
             if (pair) {
-                const uint64_t z = qa_zigzag(x - p);
                 dzmin = z < dzmin ? z : dzmin;
                 dzmax = z > dzmax ? z : dzmax;
                 if (FASTL) xb += qa_varint_len(x ^ p);
+                eqin += x == p;
             }
+            if (i == 0 && k == 0) eqx = k0 > rs && x == p;  // across tiles of the region
             if (REG) {  // every tile of a full region is full
-                const uint32_t rpos = t * QA_TILE + q * WAVE + lane;  // region position
-                if ((lane & 7) == 0) {
+                const bool first = t == 0 && k == 0;  // region position 0
+                if (i == 0 || i == 8) {
                     const uint64_t h = x * 0x9E3779B97F4A7C15ull;
                     const uint32_t bucket = (uint32_t)((h >> 50) & 0x3FFF);
                     atomicOr((unsigned long long *)&seen8[bucket >> 6], 1ull << (bucket & 63));
-                    if ((lane & 31) == 0)
+                    if (i == 0 && (lane & 1) == 0)
                         atomicOr((unsigned long long *)&seen32[bucket >> 6], 1ull << (bucket & 63));
                 }
-                if ((lane & 15) == 0 && rpos != 0) {
-                    const uint64_t z = qa_zigzag(x - p);
+                if (i == 0 && !first) {
                     pa.pmn = x < pa.pmn ? x : pa.pmn;
                     pa.pmx = x > pa.pmx ? x : pa.pmx;
                     pa.pdz = z > pa.pdz ? z : pa.pdz;
-                    if (lane == 0) {
+                    if ((lane & 3) == 0) {
                         pa.gmn = x < pa.gmn ? x : pa.gmn;
                         pa.gmx = x > pa.gmx ? x : pa.gmx;
                         pa.gdz = z > pa.gdz ? z : pa.gdz;
                     }
                 }
             }
-            const uint64_t eqm = __ballot(x == p);
-            eqin += (uint32_t)__popcll(eqm & __ballot(pair));
-            // equal neighbours of the region: inside the tile, and across
-            // tile boundaries inside the region
-            if (q == 0 && k0 > rs) eqp += (uint32_t)(eqm & 1);
         }
-        eqp += eqin;
-        const uint32_t chg = cnt ? cnt - 1 - eqin : 0;  // changes between neighbours in the tile
         umin = wave_reduce(umin, OpMinU());
         umax = wave_reduce(umax, OpMaxU());
         if (SG) {
@@ -291,13 +327,16 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
         dzmin = wave_reduce(dzmin, OpMinU());
         dzmax = wave_reduce(dzmax, OpMaxU());
         if (FASTL) xb = wave_reduce(xb, OpAdd());
+        const uint32_t eqt = (uint32_t)wave_reduce((uint64_t)eqin, OpAdd());
+        const uint32_t eqp = eqt + (uint32_t)__builtin_amdgcn_readfirstlane((int)eqx);
+        const uint32_t chg = cnt ? cnt - 1 - eqt : 0;  // changes between neighbours in the tile
         // all equal <=> min == max; arithmetic <=> every zig-zag delta equal
         const bool same = umin == umax, arith = cnt <= 1 || dzmin == dzmax;
         if (lane == 0) {
             ts.umin = umin;
             ts.umax = umax;
-            ts.smin = smin;
-            ts.smax = smax;
+            ts.smin = (int64_t)smin;
+            ts.smax = (int64_t)smax;
             ts.dzmin = dzmin;
             ts.dzmax = dzmax;
             ts.xorz_rest = xb;
