@@ -300,6 +300,12 @@ __device__ __forceinline__ uint64_t qa_wave_scan_add64(uint64_t v) {
     return v;
 }
 
+// index of the calling wavefront in its workgroup, known to be uniform (so
+// what it indexes is read with scalar loads and scalar address arithmetic)
+__device__ __forceinline__ int qa_wave_id() {
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // LDS written by one wavefront and read back by its other lanes
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

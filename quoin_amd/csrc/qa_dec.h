@@ -210,12 +210,30 @@ __device__ __forceinline__ void emit_gather_f(void *out, const uint64_t *bm, uin
         const uint64_t inc = qa_wave_scan_add64(pc);
         const uint64_t ex = inc - pc;
         const uint32_t nw = (uint32_t)(je - c0 + 1 < (uint64_t)WAVE ? je - c0 + 1 : (uint64_t)WAVE);
-        for (uint32_t i = 0; i < nw; i++) {  // word i of the chunk: uniform, scalar reads
-            const uint64_t mi = qa_readlane64(m, (int)i);
-            const uint64_t idx = r + qa_readlane64(ex, (int)i) + (uint64_t)__popcll(mi & below);
-            const uint64_t pos = ((c0 + i) << 6) + lane;
-            const uint64_t x = ((mi >> lane) & 1) && idx < cnt ? vv[idx] : 0;
-            if (pos >= L0 && pos < L1) st_out<F>(out, pos, x);
+        // four words per step so their LDS reads overlap; word fields are
+        // uniform (scalar reads)
+        for (uint32_t i0 = 0; i0 < nw; i0 += 4) {
+            uint64_t xv[4], pv[4];
+            bool sv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t i = i0 + u;
+                sv[u] = false;
+                xv[u] = 0;
+                pv[u] = 0;
+                if (i < nw) {
+                    const uint64_t mi = qa_readlane64(m, (int)i);
+                    const uint32_t idx = (uint32_t)(r + qa_readlane64(ex, (int)i)) +
+                                         (uint32_t)__popcll(mi & below);
+                    const uint64_t pos = ((c0 + i) << 6) + lane;
+                    xv[u] = ((mi >> lane) & 1) && idx < cnt ? vv[idx] : 0;
+                    pv[u] = pos;
+                    sv[u] = pos >= L0 && pos < L1;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (sv[u]) st_out<F>(out, pv[u], xv[u]);
         }
         r += qa_readlane64(inc, 63);
     }

@@ -105,7 +105,7 @@ __global__ void k_dec_desc(DecArgs a) {
 __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
     __shared__ uint64_t lds[4][QA_TILE];
     if (*a.err) return;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
     const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.t1) return;
     const TileDec td = a.tdec[t];
@@ -171,14 +171,15 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
             x[q] = k < cnt ? mn + unpack_lds(wd, w, k) : 0;
         }
         if (td.mode == QA_M_DELTA) {
+            // wrapping prefix sum of the unzigzagged deltas, seeded with the
+            // tile carry (a wavefront scan per 64 values)
             uint64_t carry = ts.tail;
 #pragma unroll
             for (int q = 0; q < QV; q++) {
-                uint32_t k = lane + WAVE * q;
-                uint64_t d = k < cnt ? qa_unzigzag(x[q]) : 0;
-                uint64_t inc = wave_incl_scan(d, lane);
+                const uint32_t k = lane + WAVE * q;
+                const uint64_t inc = qa_wave_scan_add64(k < cnt ? qa_unzigzag(x[q]) : 0);
                 x[q] = carry + inc;
-                carry += __shfl(inc, 63, 64);
+                carry += qa_readlane64(inc, 63);
             }
         } else if (td.mode == QA_M_FLOAT_MULT) {
             const double sc = d_fm_scales[a.fx[f].scale];

@@ -177,7 +177,7 @@ __global__ void k_dec_colfin(DecArgs a) {
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_dec_tilemap(DecArgs a) {
     const int lane = threadIdx.x & 63;
-    uint32_t f = a.f0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint32_t f = a.f0 + blockIdx.x * 4 + qa_wave_id();
     if (f >= a.f1) return;
     const DFrame &fr = a.frames[f];
     for (uint32_t t = lane; t < fr.ntiles; t += WAVE) {
@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(256) k_dec_tilemap(DecArgs a) {
 
 __global__ void __launch_bounds__(256) k_dec_index(DecArgs a) {
     const int lane = threadIdx.x & 63;
-    uint32_t f = a.f0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint32_t f = a.f0 + blockIdx.x * 4 + qa_wave_id();
     if (f >= a.f1 || *a.err || !frame_used(a, f)) return;
     const DFrame &fr = a.frames[f];
     const DDecCol &col = a.cols[fr.col];
@@ -279,7 +279,7 @@ __device__ __forceinline__ T wsum(T v) {
 __global__ void __launch_bounds__(256) k_dec_delta_sums(DecArgs a) {
     __shared__ uint32_t wbuf[4][2 * QA_TILE + 4];
     if (*a.err) return;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
     uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.t1) return;
     DTileD &ts = a.tiles[t];
@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(256) k_dec_delta_sums(DecArgs a) {
 // one wavefront per frame: exclusive scan of the tile sums, seeded with the base
 __global__ void __launch_bounds__(256) k_dec_delta_scan(DecArgs a) {
     const int lane = threadIdx.x & 63;
-    uint32_t f = a.f0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint32_t f = a.f0 + blockIdx.x * 4 + qa_wave_id();
     if (f >= a.f1 || *a.err) return;
     const DFrame &fr = a.frames[f];
     if (fr.mode != QA_M_DELTA || !frame_used(a, f)) return;

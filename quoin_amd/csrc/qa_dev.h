@@ -113,12 +113,22 @@ __device__ __forceinline__ void wave_store_bytes(uint8_t *dst, const uint32_t *l
     if ((uint32_t)lane < head) dst[lane] = sb[lane];
     uint32_t rem = nbytes - head, nw = rem >> 2;
     uint32_t *dw = (uint32_t *)(dst + head);
-    if (head == 0) {
-        for (uint32_t i = lane; i < nw; i += WAVE) dw[i] = lds[i];
-    } else {
-        for (uint32_t i = lane; i < nw; i += WAVE) {
-            uint64_t pair = ((uint64_t)lds[i + 1] << 32) | lds[i];
-            dw[i] = (uint32_t)(pair >> (8 * head));
+    // four words per lane per step: the LDS reads of a step overlap
+    for (uint32_t i0 = 0; i0 < nw; i0 += 4 * WAVE) {
+        uint32_t o[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t i = i0 + u * WAVE + lane;
+            o[u] = 0;
+            if (i < nw) {
+                const uint64_t pair = ((uint64_t)(head ? lds[i + 1] : 0) << 32) | lds[i];
+                o[u] = (uint32_t)(pair >> (8 * head));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t i = i0 + u * WAVE + lane;
+            if (i < nw) dw[i] = o[u];
         }
     }
     uint32_t tb = head + 4 * nw;

@@ -128,7 +128,7 @@ __device__ __forceinline__ void compact_words(uint64_t w, uint64_t rank0, uint64
 
 __global__ void __launch_bounds__(256) k_compact(EncArgs a) {
     const int lane = threadIdx.x & 63;
-    const uint64_t j0 = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * WAVE;
+    const uint64_t j0 = ((uint64_t)blockIdx.x * 4 + qa_wave_id()) * WAVE;
     if (j0 >= a.nvwords) return;
     const uint64_t j = j0 + lane;
     uint64_t w = 0, rank0 = 0, pos0 = 0;
@@ -248,7 +248,7 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
                                                  uint64_t rs, uint64_t len, DTile *tl,
                                                  uint64_t *seen8, uint64_t *seen32,
                                                  ProbeAcc &pa) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wv = qa_wave_id(), nwv = blockDim.x >> 6;
     const uint32_t ntl = (uint32_t)((len + QA_TILE - 1) / QA_TILE);
     uint32_t eq_region = 0;
     for (uint32_t t = wv; t < col.tpr; t += nwv) {
@@ -1090,7 +1090,7 @@ __global__ void k_tile_desc(EncArgs a) {
 // per lane storage kind so the loads are straight-line code.
 __global__ void __launch_bounds__(256) k_emit_tiles(EncArgs a) {
     __shared__ uint64_t outs[4][QA_TILE + 1];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
     const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.t1) return;
     const TileEmit e = a.temit[t];

@@ -86,7 +86,7 @@ __device__ __forceinline__ bool tile_owner(const EncArgs &a, uint64_t t, int *cp
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_float_tiles(EncArgs a) {
     __shared__ uint64_t samp[4][32];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
     const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.t1) return;
     const DTile &ts = a.tiles[t];
@@ -448,7 +448,7 @@ __global__ void __launch_bounds__(1024) k_alprd_hist(EncArgs a) {
 // per tile: codes / rights ranges and the exceptions (alp_rd.rs:145-163)
 __global__ void __launch_bounds__(256) k_alprd_tiles(EncArgs a) {
     const int lane = threadIdx.x & 63;
-    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + qa_wave_id();
     if (t >= a.t1) return;
     const DTile &ts = a.tiles[t];
     if (!ts.active) return;
@@ -519,7 +519,7 @@ __device__ __forceinline__ bool consider(uint64_t sz, uint32_t m, uint64_t lambd
 
 __global__ void __launch_bounds__(256) k_block_decide(EncArgs a) {
     const int lane = threadIdx.x & 63;
-    const uint32_t slot = a.r0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t slot = a.r0 + blockIdx.x * 4 + qa_wave_id();
     if (slot >= a.r1) return;
     DBlock &b = a.blocks[slot];
     if (b.len == 0) return;
@@ -664,7 +664,7 @@ __device__ void dict_values_size(const EncArgs &a, uint32_t slot, const uint64_t
     __shared__ unsigned long long tot;
     if (threadIdx.x == 0) tot = 0;
     __syncthreads();
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wv = qa_wave_id(), nwv = blockDim.x >> 6;
     uint64_t nch = (card + QA_TILE - 1) / QA_TILE;
     for (uint64_t ch = wv; ch < nch; ch += nwv) {
         uint64_t mn = ~0ull, mx = 0;
@@ -850,7 +850,7 @@ __global__ void __launch_bounds__(1024) k_dict_part(EncArgs a) {
 __device__ void lds_compact_sort(uint64_t *keys, uint32_t card_l, uint32_t has_max) {
     __shared__ uint32_t wsum[16];
     const uint32_t per = DH_CAP / blockDim.x;  // 8 at 1024 threads
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
     uint64_t v[8];
     uint32_t c = 0;
     for (uint32_t j = 0; j < per; j++) {
@@ -1152,7 +1152,7 @@ __device__ __forceinline__ bool rle_candidate(const EncArgs &a, const DBlock &b,
 
 __global__ void __launch_bounds__(256) k_rle_starts(EncArgs a) {
     const int lane = threadIdx.x & 63;
-    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + qa_wave_id();
     if (t >= a.t1) return;
     const DTile &ts = a.tiles[t];
     if (!ts.active) return;
@@ -1189,7 +1189,7 @@ __global__ void __launch_bounds__(256) k_rle_chunks(EncArgs a) {
     const DCol &col = a.cols[b.col];
     const uint32_t *rs = a.rstart + col.lane_off + b.start;
     DTileF *tf = a.tilesf + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
     const uint64_t R = x.nruns;
     if (threadIdx.x == 0) tot = 0;
     __syncthreads();
@@ -1248,7 +1248,7 @@ __device__ __forceinline__ void wave_offsets(uint32_t count, uint64_t carry, int
 
 __global__ void __launch_bounds__(256) k_block_final(EncArgs a) {
     const int lane = threadIdx.x & 63;
-    const uint32_t slot = a.r0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t slot = a.r0 + blockIdx.x * 4 + qa_wave_id();
     if (slot >= a.r1) return;
     DBlock &b = a.blocks[slot];
     if (b.len == 0) return;

@@ -16,7 +16,7 @@
 // control flow); the sub-block walks use the whole wavefront.
 __global__ void __launch_bounds__(256) k_dec_index_fast(DecArgs a) {
     const int lane = threadIdx.x & 63;
-    uint32_t f = a.f0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint32_t f = a.f0 + blockIdx.x * 4 + qa_wave_id();
     if (f >= a.f1 || *a.err || !frame_used(a, f)) return;
     const DFrame &fr = a.frames[f];
     if (!fast_mode(fr.mode) || fr.mode == QA_M_XORZ) return;
@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(256) k_dec_index_fast(DecArgs a) {
 template <typename T, typename Op>
 __device__ __forceinline__ T wg_incl_scan(T v, Op op, T *tot) {
     __shared__ T wpart[16];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         T y = __shfl_up(v, o, 64);
