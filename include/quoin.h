@@ -236,6 +236,46 @@ void quoin_server_stats(const QuoinServer *s, uint64_t *jobs, uint64_t *batches)
 /* Finishes every queued job, then frees the server. */
 void quoin_server_destroy(QuoinServer *s);
 
+/* ---- additions: rANS entropy coder on device buffers -------------------- */
+
+/* One byte stream for quoin_gpu_rans_compress. */
+typedef struct {
+    const uint8_t *src;        /* device, n bytes                          */
+    uint64_t n;
+    uint8_t *dst;              /* device, dst_cap bytes                    */
+    uint64_t dst_cap;          /* >= quoin_rans_bound(n)                   */
+} QuoinRansInput;
+
+/* One coded stream for quoin_gpu_rans_decompress. */
+typedef struct {
+    const uint8_t *src;        /* device, src_len bytes                    */
+    uint64_t src_len;
+    uint64_t max_len;          /* decode bound: a stream declaring more is corrupt */
+    uint8_t *out;              /* device, max_len bytes                    */
+} QuoinRansStream;
+
+/* Capacity a coded stream of n bytes needs (0 for n = 0). */
+size_t quoin_rans_bound(size_t n);
+
+/* Replaces rans::compress_bytes (src/entropy/rans.rs:52-108) for a batch:
+ * out_lens[i] = coded length of stream i, byte-identical to the reference
+ * (0 for an empty input, where the reference returns None). Every stream is
+ * coded by one launch sequence on the device. QUOIN_ERR_BUFFER_TOO_SMALL when
+ * a dst_cap is below quoin_rans_bound(n); n must be below 2^32. */
+int quoin_gpu_rans_compress(QuoinGpu *g, const QuoinRansInput *in, size_t n_streams,
+                            uint64_t *out_lens, void *stream);
+
+/* Replaces rans::decompress_bytes (src/entropy/rans.rs:110-175) for a batch,
+ * and with tagged = 1 the TAG_RANS arm of decode_entropy_only
+ * (src/entropy/mod.rs:153-161): src starts with the entropy tag byte; tags 0,
+ * 1 and 3 (range coder, tANS, LZ cascade) are host coders and give
+ * QUOIN_ERR_UNSUPPORTED, other tags QUOIN_ERR_CORRUPT. out_lens[i] = decoded
+ * bytes; status[i] (optional) = QUOIN_OK or the stream's error, where the
+ * reference returns Err (the bytes of such a stream's out are unspecified).
+ * Returns QUOIN_OK or the first stream's error. */
+int quoin_gpu_rans_decompress(QuoinGpu *g, const QuoinRansStream *streams, size_t n_streams,
+                              int tagged, uint64_t *out_lens, int32_t *status, void *stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

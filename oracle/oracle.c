@@ -783,17 +783,21 @@ int orc_rans_decompress(const uint8_t *pl, size_t len, size_t max_len, uint8_t *
     for (uint64_t k = 0; k < nsym; k++) {
         if (pos >= len) return ORC_TRUNCATED;
         uint8_t s = pl[pos++];
-        uint64_t f;
-        rc = varint_read(pl, len, &pos, &f);
+        uint64_t f64v;
+        rc = varint_read(pl, len, &pos, &f64v);
         if (rc) return rc;
+        uint32_t f = (uint32_t)f64v; /* `as u32` before the range check (rans.rs:125-128) */
         if (f == 0 || f > RANS_M) return ORC_CORRUPT;
-        freq[s] = (uint32_t)f;
-        sum += (uint32_t)f;
+        freq[s] = f;
+        sum += f;
     }
     if (sum != RANS_M) return ORC_CORRUPT;
     cum[0] = 0;
     for (int s = 0; s < 256; s++) cum[s + 1] = cum[s] + freq[s];
+    /* zero-initialised like rans.rs:141: with a repeated symbol in the table the
+     * frequencies sum to less than M and the slots past cum[256] decode as 0 */
     static __thread uint8_t slot_to_sym[RANS_M];
+    memset(slot_to_sym, 0, sizeof slot_to_sym);
     for (int s = 0; s < 256; s++)
         for (uint32_t sl = cum[s]; sl < cum[s + 1]; sl++) slot_to_sym[sl] = (uint8_t)s;
     uint32_t x[4];

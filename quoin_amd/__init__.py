@@ -97,6 +97,18 @@ class Job(ctypes.Structure):
 JOB_COMPRESS, JOB_DECOMPRESS = 0, 1
 
 
+class RansInput(ctypes.Structure):
+    """QuoinRansInput (include/quoin.h): device bytes to code and their output."""
+    _fields_ = [("src", ctypes.c_void_p), ("n", ctypes.c_uint64), ("dst", ctypes.c_void_p),
+                ("dst_cap", ctypes.c_uint64)]
+
+
+class RansStream(ctypes.Structure):
+    """QuoinRansStream (include/quoin.h): a device rANS payload and its output."""
+    _fields_ = [("src", ctypes.c_void_p), ("src_len", ctypes.c_uint64), ("max_len", ctypes.c_uint64),
+                ("out", ctypes.c_void_p)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run python quoin_amd/build.py")
@@ -141,6 +153,13 @@ def _load():
     lib.quoin_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64),
                                        ctypes.POINTER(ctypes.c_uint64)]
     lib.quoin_server_destroy.argtypes = [vp]
+    lib.quoin_rans_bound.restype = sz
+    lib.quoin_rans_bound.argtypes = [sz]
+    lib.quoin_gpu_rans_compress.argtypes = [vp, ctypes.POINTER(RansInput), sz,
+                                            ctypes.POINTER(ctypes.c_uint64), vp]
+    lib.quoin_gpu_rans_decompress.argtypes = [vp, ctypes.POINTER(RansStream), sz, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_uint64),
+                                              ctypes.POINTER(ctypes.c_int32), vp]
     return lib
 
 
@@ -156,7 +175,8 @@ EXPORTS = [
     "quoin_reset_mode_win_counts", "quoin_last_error", "quoin_gpu_create", "quoin_gpu_destroy", "quoin_gpu_compress",
     "quoin_gpu_decompress", "quoin_gpu_set_launch_form", "quoin_gpu_last_timing",
     "quoin_server_create", "quoin_server_submit", "quoin_server_wait", "quoin_server_stats",
-    "quoin_server_destroy",
+    "quoin_server_destroy", "quoin_rans_bound", "quoin_gpu_rans_compress",
+    "quoin_gpu_rans_decompress",
 ]
 
 
@@ -278,6 +298,27 @@ class Gpu:
         rc = lib.quoin_gpu_decompress(self.h, arr, len(streams), stream)
         if rc != OK:
             raise QuoinError(rc)
+
+    def rans_compress(self, inputs, stream=None):
+        """inputs: list of RansInput; returns the coded lengths (src/entropy/rans.rs:52)."""
+        arr = (RansInput * len(inputs))(*inputs)
+        lens = (ctypes.c_uint64 * len(inputs))()
+        rc = lib.quoin_gpu_rans_compress(self.h, arr, len(inputs), lens, stream)
+        if rc != OK:
+            raise QuoinError(rc)
+        return [int(x) for x in lens]
+
+    def rans_decompress(self, streams, tagged=False, stream=None):
+        """streams: list of RansStream; returns (decoded lengths, per-stream status)
+        (src/entropy/rans.rs:110). Failed streams are reported, not raised."""
+        arr = (RansStream * len(streams))(*streams)
+        lens = (ctypes.c_uint64 * len(streams))()
+        st = (ctypes.c_int32 * len(streams))()
+        rc = lib.quoin_gpu_rans_decompress(self.h, arr, len(streams), 1 if tagged else 0, lens, st,
+                                           stream)
+        if rc not in (OK, ERR_CORRUPT, ERR_UNSUPPORTED):
+            raise QuoinError(rc)
+        return [int(x) for x in lens], [int(x) for x in st]
 
     def last_timing(self):
         t, k = ctypes.c_double(), ctypes.c_double()

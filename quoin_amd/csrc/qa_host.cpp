@@ -65,6 +65,8 @@ struct QuoinGpu {
         dfx, dscr, dscr2;
     // host-API staging
     DevBuf st_in, st_valid, st_out, st_vout;
+    // rANS arenas
+    DevBuf rdesc, rres, rtabs, rscr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, kev0 = nullptr, kev1 = nullptr;
     double last_total_ms = 0, last_kernel_ms = 0;
     const char *last_kernel = "";
@@ -633,6 +635,107 @@ extern "C" int quoin_gpu_decompress(QuoinGpu *g, const QuoinDeviceStream *stream
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
     return gpu_decompress_locked(g, streams, ns, s);
+}
+
+// ---------------------------------------------------------------------------
+// rANS entropy coder (src/entropy/rans.rs) on device buffers
+// ---------------------------------------------------------------------------
+extern "C" size_t quoin_rans_bound(size_t n) { return n ? 2 * n + 1024 : 0; }
+
+extern "C" int quoin_gpu_rans_compress(QuoinGpu *g, const QuoinRansInput *in, size_t ns,
+                                       uint64_t *out_lens, void *stream) {
+    if (!g || (ns && (!in || !out_lens))) return QUOIN_ERR_NULL;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    std::vector<RansEncD> d;
+    std::vector<size_t> which;
+    uint64_t scr = 0;
+    for (size_t i = 0; i < ns; i++) {
+        out_lens[i] = 0;
+        if (in[i].n == 0) continue;  // compress_bytes returns None (rans.rs:53-56)
+        if (!in[i].src || !in[i].dst) return QUOIN_ERR_NULL;
+        if (in[i].n > 0xFFFFFFFFull) return QUOIN_ERR_UNSUPPORTED;  // u32 symbol counts
+        if (in[i].dst_cap < quoin_rans_bound(in[i].n)) return QUOIN_ERR_BUFFER_TOO_SMALL;
+        RansEncD e;
+        e.src = in[i].src;
+        e.dst = in[i].dst;
+        e.n = in[i].n;
+        scr += (2 * in[i].n + 96 + 15) & ~15ull;
+        e.scr_end = scr;
+        d.push_back(e);
+        which.push_back(i);
+    }
+    if (d.empty()) return QUOIN_OK;
+    const size_t m = d.size();
+    if (m > 0xFFFFFFFFull) return QUOIN_ERR_UNSUPPORTED;
+    if (!ensure(g->rdesc, m * sizeof(RansEncD)) || !ensure(g->rres, m * sizeof(RansEncR)) ||
+        !ensure(g->rtabs, m * 256 * 8) || !ensure(g->rscr, scr + 64))
+        return QUOIN_ERR_DEVICE;
+    hipEventRecord(g->ev0, s);
+    hipMemcpyAsync(g->rdesc.p, d.data(), m * sizeof(RansEncD), hipMemcpyHostToDevice, s);
+    qa_rans_encode_launch(dp<RansEncD>(g->rdesc), dp<RansEncR>(g->rres), dp<uint64_t>(g->rtabs),
+                          dp<uint8_t>(g->rscr), (uint32_t)m, s);
+    hipEventRecord(g->ev1, s);
+    std::vector<RansEncR> r(m);
+    hipMemcpyAsync(r.data(), g->rres.p, m * sizeof(RansEncR), hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    float ms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    g->last_total_ms = ms;
+    g->last_kernel_ms = 0;
+    g->last_kernel = "";
+    for (size_t j = 0; j < m; j++) out_lens[which[j]] = r[j].out_len;
+    return QUOIN_OK;
+}
+
+extern "C" int quoin_gpu_rans_decompress(QuoinGpu *g, const QuoinRansStream *in, size_t ns, int tagged,
+                                         uint64_t *out_lens, int32_t *status, void *stream) {
+    if (!g || (ns && (!in || !out_lens))) return QUOIN_ERR_NULL;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    if (ns == 0) return QUOIN_OK;
+    if (ns > 0xFFFFFFFFull) return QUOIN_ERR_UNSUPPORTED;
+    std::vector<RansDecD> d(ns);
+    for (size_t i = 0; i < ns; i++) {
+        if ((in[i].src_len && !in[i].src) || (in[i].max_len && !in[i].out)) return QUOIN_ERR_NULL;
+        d[i].src = in[i].src;
+        d[i].out = in[i].out;
+        d[i].src_len = in[i].src_len;
+        d[i].max_len = in[i].max_len;
+        d[i].tagged = tagged ? 1u : 0u;
+        d[i].pad = 0;
+    }
+    if (!ensure(g->rdesc, ns * sizeof(RansDecD)) || !ensure(g->rres, ns * sizeof(RansDecR)))
+        return QUOIN_ERR_DEVICE;
+    hipEventRecord(g->ev0, s);
+    hipMemcpyAsync(g->rdesc.p, d.data(), ns * sizeof(RansDecD), hipMemcpyHostToDevice, s);
+    hipEventRecord(g->kev0, s);
+    qa_rans_decode_launch(dp<RansDecD>(g->rdesc), dp<RansDecR>(g->rres), (uint32_t)ns, s);
+    hipEventRecord(g->kev1, s);
+    hipEventRecord(g->ev1, s);
+    std::vector<RansDecR> r(ns);
+    hipMemcpyAsync(r.data(), g->rres.p, ns * sizeof(RansDecR), hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    float ms = 0, kms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    hipEventElapsedTime(&kms, g->kev0, g->kev1);
+    g->last_total_ms = ms;
+    g->last_kernel_ms = kms;
+    g->last_kernel = "k_rans_dec";
+    int rc = QUOIN_OK;
+    for (size_t i = 0; i < ns; i++) {
+        const int st = r[i].status == 0 ? QUOIN_OK
+                       : r[i].status == 2 ? QUOIN_ERR_UNSUPPORTED
+                                          : QUOIN_ERR_CORRUPT;
+        out_lens[i] = st == QUOIN_OK ? r[i].n : 0;
+        if (status) status[i] = st;
+        if (rc == QUOIN_OK) rc = st;
+    }
+    return rc;
 }
 
 // ---------------------------------------------------------------------------

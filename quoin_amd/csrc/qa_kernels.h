@@ -196,3 +196,32 @@ void qa_decode_values_launch(const DecArgs &a, hipStream_t s);
 void qa_decode_final_launch(const DecArgs &a, hipStream_t s);
 void qa_fast_dec_index_launch(const DecArgs &a, hipStream_t s);
 void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s);
+
+// rANS (src/entropy/rans.rs): per-stream descriptors of a batch
+struct RansDecD {
+    const uint8_t *src;   // payload, or [tag] + payload when tagged
+    uint8_t *out;         // max_len bytes
+    uint64_t src_len;
+    uint64_t max_len;
+    uint32_t tagged;      // 1: entropy tag byte first (src/entropy/mod.rs:153-161)
+    uint32_t pad;
+};
+struct RansDecR {
+    uint64_t n;           // decoded bytes
+    int32_t status;       // 0 ok, 1 corrupt, 2 other coder (tagged)
+    uint32_t pad;
+};
+struct RansEncD {
+    const uint8_t *src;   // bytes to code (n >= 1)
+    uint8_t *dst;         // header + states + bytes
+    uint64_t n;
+    uint64_t scr_end;     // end of the stream's backward scratch region
+};
+struct RansEncR {
+    uint64_t hdr_len;
+    uint64_t start;       // scratch offset of the states
+    uint64_t out_len;
+};
+void qa_rans_decode_launch(const RansDecD *d, RansDecR *r, uint32_t ns, hipStream_t s);
+void qa_rans_encode_launch(const RansEncD *d, RansEncR *r, uint64_t *tabs, uint8_t *scr, uint32_t ns,
+                           hipStream_t s);

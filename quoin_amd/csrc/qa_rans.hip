@@ -1,0 +1,646 @@
+// qa_rans.hip — 4-way interleaved order-0 rANS (src/entropy/rans.rs) on gfx950.
+//
+// Format (rans.rs:52-108): varint(n) varint(nsym) (sym:u8 varint(f))^nsym
+// x0 x1 x2 x3 (u32 LE) then the renormalization bytes in decode order. M = 4096,
+// L = 2^23, symbol i uses state i & 3, and the four states share the byte
+// stream: the bytes of symbol i come after those of every earlier symbol.
+//
+// Decode: one stream per 4 lanes (lane = state), four streams per wavefront.
+// A step decodes 4 symbols: each lane looks its slot up in the stream's LDS
+// table (sym | f | slot - cum, one 32-bit read) and reads the 12 payload bytes
+// at the stream position from an LDS window (issued together, so one LDS
+// latency per step); the byte counts follow from the new state alone (0, 1 or
+// 2 bytes for every state a valid stream reaches), so a 4-lane DPP scan places
+// each lane's bytes. The window is a 1 KiB ring per stream, refilled from
+// registers loaded a round (64 steps) ahead. States that need 3 or more bytes
+// (only a corrupt stream reaches them) take an exact sequential path.
+//
+// Encode: a workgroup per stream builds the histogram, normalizes it
+// (rans.rs:25-50), writes the header and the per-symbol encode table; then one
+// stream per 4 lanes (16 per wavefront) runs the backward state recurrence,
+// dividing by f with a per-symbol reciprocal, and writes the renormalization
+// bytes backward into scratch; a copy places them after the header.
+#include "qa_kernels.h"
+
+namespace {
+
+constexpr uint32_t RM = 4096;          // total frequency (rans.rs:17-18)
+constexpr uint32_t RL = 1u << 23;      // lower renormalization bound (rans.rs:20)
+constexpr int DG = 4;                  // decode: streams per wavefront
+constexpr int RING = 1024;             // decode: payload window bytes per stream
+constexpr int STAGE = 1024;            // decode: header staging bytes (in the table)
+constexpr int ROUND = 64;              // decode: steps between window refills
+constexpr int EG = 16;                 // encode: streams per wavefront
+
+struct DecLds {
+    uint32_t tab[DG][RM];              // slot -> sym | (f & 4095) << 8 | (slot - cum) << 20
+    uint32_t ring[DG][RING / 4];       // payload dwords by aligned position
+    uint32_t cum[DG][260];             // freq during the header parse, then cum
+    uint32_t outs[DG][64];             // 256 staged output bytes per stream
+};
+
+// 4-lane (quad) helpers on DPP; every lane of the wavefront must be active
+template <int SRC>
+__device__ __forceinline__ uint32_t quad_bcast(uint32_t v) {
+    constexpr int ctrl = SRC | (SRC << 2) | (SRC << 4) | (SRC << 6);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, ctrl, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint32_t quad_bcast_dyn(uint32_t v, int j) {
+    switch (j) {
+    case 0: return quad_bcast<0>(v);
+    case 1: return quad_bcast<1>(v);
+    case 2: return quad_bcast<2>(v);
+    default: return quad_bcast<3>(v);
+    }
+}
+__device__ __forceinline__ uint64_t quad_bcast64_dyn(uint64_t v, int j) {
+    return (uint64_t)quad_bcast_dyn((uint32_t)v, j) |
+           ((uint64_t)quad_bcast_dyn((uint32_t)(v >> 32), j) << 32);
+}
+// inclusive prefix sum inside each quad (idx = lane & 3)
+__device__ __forceinline__ uint32_t quad_scan(uint32_t v, int idx) {
+    // quad_perm [0,0,1,2]: the left neighbour
+    uint32_t t = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x90, 0xF, 0xF, false);
+    v += idx >= 1 ? t : 0u;
+    // quad_perm [0,0,0,1]: lanes 2, 3 add lanes 0, 1
+    t = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x40, 0xF, 0xF, false);
+    return v + (idx >= 2 ? t : 0u);
+}
+__device__ __forceinline__ uint32_t quad_or(uint32_t v) {
+    v |= (uint32_t)__builtin_amdgcn_mov_dpp((int)v, QA_DPP_QP_1032, 0xF, 0xF, false);
+    v |= (uint32_t)__builtin_amdgcn_mov_dpp((int)v, QA_DPP_QP_2301, 0xF, 0xF, false);
+    return v;
+}
+// value of lane `src` (any lane, per-lane source) — round-level bookkeeping only
+__device__ __forceinline__ uint32_t lane_pull(uint32_t v, int src) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
+}
+__device__ __forceinline__ uint64_t lane_pull64(uint64_t v, int src) {
+    return (uint64_t)lane_pull((uint32_t)v, src) | ((uint64_t)lane_pull((uint32_t)(v >> 32), src) << 32);
+}
+
+// header byte at aligned position q (q = byte offset + (src & 15)): staged in
+// LDS for q < STAGE, else read from HBM; false past the payload end
+__device__ __forceinline__ bool hdr_byte(const DecLds &L, int g, const uint8_t *base, uint64_t qe,
+                                         uint64_t q, uint32_t *b) {
+    if (q >= qe) return false;
+    if (q < STAGE) *b = ((const uint8_t *)L.tab[g])[q];
+    else *b = base[q];
+    return true;
+}
+// LEB128 (src/varint.rs:24-39): 0 ok, 1 truncated, 2 overflow
+__device__ __forceinline__ int hdr_varint(const DecLds &L, int g, const uint8_t *base, uint64_t qe,
+                                          uint64_t *q, uint64_t *out) {
+    uint64_t r = 0;
+    uint32_t shift = 0;
+    for (;;) {
+        uint32_t b;
+        if (!hdr_byte(L, g, base, qe, *q, &b)) return 1;
+        (*q)++;
+        r |= (uint64_t)(b & 0x7f) << shift;
+        if (!(b & 0x80)) {
+            *out = r;
+            return 0;
+        }
+        shift += 7;
+        if (shift >= 64) return 2;
+    }
+}
+
+// 16-byte aligned chunk at aligned position q of a payload whose bytes occupy
+// [la, qe); zero when the chunk holds no payload byte (a chunk that holds one
+// lies in the same page as that byte)
+__device__ __forceinline__ uint4 chunk16(const uint8_t *base, uint32_t la, uint64_t qe, uint64_t q) {
+    if (q >= qe || q + 16 <= la) return make_uint4(0, 0, 0, 0);
+    return *(const uint4 *)(base + q);
+}
+
+// flush lanes: bytes [c0, c0 + 16) of an output of ln bytes from staged words
+__device__ __forceinline__ void flush16(uint8_t *out, bool al16, uint64_t c0, uint64_t ln, uint4 v) {
+    if (c0 >= ln) return;
+    if (al16 && c0 + 16 <= ln) {
+        *(uint4 *)(out + c0) = v;
+        return;
+    }
+    const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int b = 0; b < 16; b++)
+        if (c0 + b < ln) out[c0 + b] = (uint8_t)(vv[b >> 2] >> (8 * (b & 3)));
+}
+
+// Decode kernel: wavefront w decodes streams 4w .. 4w+3.
+__global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ descs,
+                                                 RansDecR *__restrict__ res, uint32_t ns) {
+    __shared__ DecLds L;
+    const int lane = threadIdx.x;
+    const uint32_t s0 = blockIdx.x * DG;
+    // decode-side mapping: group g = lane >> 2 (lanes 0..15), state idx = lane & 3
+    const int g = (lane >> 2) & (DG - 1);
+    const int idx = lane & 3;
+    const bool dlane = lane < 4 * DG && s0 + (lane >> 2) < ns;
+    // load-side mapping: stream gl = lane >> 4, sub-lane t
+    const int gl = lane >> 4;
+    const int t = lane & 15;
+    const bool llane = s0 + gl < ns;
+
+    RansDecD dl = {};
+    if (llane) dl = descs[s0 + gl];
+    // payload base aligned down to 16 bytes, a = offset of the first byte
+    uint32_t tagged = dl.tagged;
+    const uint8_t *psrc = dl.src + (tagged ? 1 : 0);
+    uint64_t plen = dl.src_len > 0 && tagged ? dl.src_len - 1 : dl.src_len;
+    const uint32_t la = (uint32_t)((uintptr_t)psrc & 15);
+    const uint8_t *lbase = psrc - la;
+    const uint64_t lqe = la + plen;
+
+    // stage the first STAGE bytes (aligned) of each payload into its table area
+    for (int c = 0; c < STAGE / (16 * 16); c++) {
+        uint64_t q = (uint64_t)(c * 16 + t) * 16;
+        uint4 v = llane ? chunk16(lbase, la, lqe, q) : make_uint4(0, 0, 0, 0);
+        *(uint4 *)((uint8_t *)L.tab[gl] + q) = v;
+    }
+    for (int i = lane; i < DG * 260; i += 64) (&L.cum[0][0])[i] = 0;
+    wave_lds_sync();
+
+    // header parse: the group's lane 0 (rans.rs:110-154)
+    uint32_t st = 0;            // 0 ok, 1 corrupt, 2 unsupported (tagged, other coder)
+    uint64_t n = 0, q = 0;
+    uint32_t x[4] = {0, 0, 0, 0};
+    uint32_t fz = 0;
+    // the group's stream descriptor, from the load-side lanes
+    const uint8_t *gsrc = (const uint8_t *)lane_pull64((uint64_t)(uintptr_t)lbase, (lane >> 2) * 16);
+    const uint64_t gqe = lane_pull64(lqe, (lane >> 2) * 16);
+    const uint32_t ga = lane_pull(la, (lane >> 2) * 16);
+    const uint64_t gmax = lane_pull64(dl.max_len, (lane >> 2) * 16);
+    const uint32_t gtag = lane_pull(tagged, (lane >> 2) * 16);
+    const uint64_t gslen = lane_pull64(dl.src_len, (lane >> 2) * 16);
+    const uint8_t gtagb = (uint8_t)lane_pull(llane && tagged && dl.src_len ? dl.src[0] : 0,
+                                             (lane >> 2) * 16);
+    if (dlane && idx == 0) {
+        q = ga;
+        if (gtag) {
+            // decode_entropy_only (src/entropy/mod.rs:153-161): TAG_RANS = 2 here;
+            // range coder (0), tANS (1) and the LZ cascade (3) are host coders
+            if (gslen == 0) st = 1;
+            else if (gtagb == 2) st = 0;
+            else if (gtagb <= 3) st = 2;
+            else st = 1;
+        }
+        uint64_t nsym = 0;
+        if (st == 0 && hdr_varint(L, g, gsrc, gqe, &q, &n)) st = 1;
+        if (st == 0 && n > gmax) st = 1;
+        if (st == 0 && hdr_varint(L, g, gsrc, gqe, &q, &nsym)) st = 1;
+        if (st == 0 && (nsym == 0 || nsym > 256)) st = 1;
+        uint32_t sum = 0;
+        for (uint64_t k = 0; st == 0 && k < nsym; k++) {
+            uint32_t s;
+            uint64_t f64v;
+            if (!hdr_byte(L, g, gsrc, gqe, q, &s)) {
+                st = 1;
+                break;
+            }
+            q++;
+            if (hdr_varint(L, g, gsrc, gqe, &q, &f64v)) {
+                st = 1;
+                break;
+            }
+            uint32_t f = (uint32_t)f64v;  // `as u32` before the check (rans.rs:125-128)
+            if (f == 0 || f > RM) {
+                st = 1;
+                break;
+            }
+            L.cum[g][s] = f;
+            sum += f;
+        }
+        if (st == 0 && sum != RM) st = 1;
+        for (int i = 0; st == 0 && i < 4; i++) {
+            uint32_t v = 0;
+            for (int b = 0; b < 4; b++) {
+                uint32_t by;
+                if (!hdr_byte(L, g, gsrc, gqe, q + b, &by)) {
+                    st = 1;
+                    break;
+                }
+                v |= by << (8 * b);
+            }
+            x[i] = v;
+            q += 4;
+        }
+        // one symbol holding all of M: its f field (f & 4095) reads as 4096
+        fz = nsym == 1 ? RM : 0;
+    }
+    wave_lds_sync();
+    // broadcast the leader's results inside each quad
+    st = quad_bcast<0>(st);
+    n = (uint64_t)quad_bcast<0>((uint32_t)n) | ((uint64_t)quad_bcast<0>((uint32_t)(n >> 32)) << 32);
+    q = (uint64_t)quad_bcast<0>((uint32_t)q) | ((uint64_t)quad_bcast<0>((uint32_t)(q >> 32)) << 32);
+    fz = quad_bcast<0>(fz);
+    uint32_t x0 = quad_bcast<0>(x[0]), x1 = quad_bcast<0>(x[1]), x2 = quad_bcast<0>(x[2]),
+             x3 = quad_bcast<0>(x[3]);
+    uint32_t xs = idx == 0 ? x0 : idx == 1 ? x1 : idx == 2 ? x2 : x3;
+    if (!dlane) st = 1;
+
+    // slot tables (rans.rs:136-146): cum by a wavefront scan, then 64 slots per lane
+    for (int gg = 0; gg < DG; gg++) {
+        const uint32_t sg = (uint32_t)__builtin_amdgcn_readlane((int)st, gg * 4);
+        if (s0 + gg >= ns || sg != 0) continue;
+        uint32_t f0 = L.cum[gg][4 * lane], f1 = L.cum[gg][4 * lane + 1], f2 = L.cum[gg][4 * lane + 2],
+                 f3 = L.cum[gg][4 * lane + 3];
+        uint32_t loc = f0 + f1 + f2 + f3;
+        uint32_t incl = (uint32_t)qa_wave_scan_add64(loc);
+        uint32_t ex = incl - loc;
+        wave_lds_sync();
+        L.cum[gg][4 * lane] = ex;
+        L.cum[gg][4 * lane + 1] = ex + f0;
+        L.cum[gg][4 * lane + 2] = ex + f0 + f1;
+        L.cum[gg][4 * lane + 3] = ex + f0 + f1 + f2;
+        if (lane == 63) L.cum[gg][256] = incl;
+        wave_lds_sync();
+        const uint32_t tot = L.cum[gg][256];
+        const uint32_t freq0 = L.cum[gg][1] - L.cum[gg][0];
+        const uint32_t slot0 = (uint32_t)lane * 64;
+        // largest s in [0, 255] with cum[s] <= slot0
+        uint32_t lo = 0, hi = 255;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (L.cum[gg][mid] <= slot0) lo = mid;
+            else hi = mid - 1;
+        }
+        uint32_t s = lo;
+        for (int j = 0; j < 64; j++) {
+            const uint32_t slot = slot0 + j;
+            uint32_t e;
+            if (slot >= tot) {
+                // slots past cum[256] keep symbol 0 (rans.rs:141 zero-fills)
+                e = ((freq0 & 4095) << 8) | (slot << 20);
+            } else {
+                while (L.cum[gg][s + 1] <= slot) s++;
+                const uint32_t c = L.cum[gg][s];
+                e = s | (((L.cum[gg][s + 1] - c) & 4095) << 8) | ((slot - c) << 20);
+            }
+            L.tab[gg][slot] = e;
+        }
+        wave_lds_sync();
+    }
+
+    // ---- decode loop -------------------------------------------------------
+    uint32_t err = st != 0 ? 1u : 0u;
+    uint64_t pos = q;                        // aligned position of the next byte
+    uint64_t rb = 0;                         // ring covers [rb, rb + RING)
+    uint4 pf0 = make_uint4(0, 0, 0, 0), pf1 = pf0;  // prefetched [rb + RING, rb + RING + 512)
+    const uint64_t nk_g = st == 0 ? (n + 3) >> 2 : 0;
+    uint64_t kmax = 0;
+    for (int gg = 0; gg < DG; gg++) {
+        uint64_t v = qa_readlane64(nk_g, gg * 4);
+        kmax = v > kmax ? v : kmax;
+    }
+    // load-side view of each stream's state (no bytes for a failed header)
+    const uint64_t ln = lane_pull64(st == 0 ? n : 0, gl * 4);
+    uint8_t *const lout = dl.out;
+    const bool lout16 = ((uintptr_t)lout & 15) == 0;
+
+    bool first = true;
+    uint64_t k = 0;
+    while (k < kmax) {
+        // window maintenance for the coming round (<= ROUND steps of <= 8 bytes,
+        // each reading 12 bytes at pos & ~3): 0 keep, 1 shift a half, 2 reload
+        const bool act_g = err == 0 && (k << 2) < n;
+        // every stream done or failed: stop (a corrupt length must not keep
+        // the wavefront stepping)
+        if (!__builtin_amdgcn_ballot_w64(act_g)) break;
+        uint32_t mode = 0;
+        if (act_g) {
+            // a round reads below pos + 516: keep while pos <= rb + 508; the
+            // shift drops [rb, rb + 512), so it needs pos >= rb + 512
+            if (first) mode = 2;
+            else if (pos > rb + (RING - 516))
+                mode = pos >= rb + 512 && pos <= rb + 512 + (RING - 516) ? 1 : 2;
+        }
+        const uint32_t lmode = lane_pull(mode, gl * 4);
+        const uint64_t lpos = lane_pull64(pos, gl * 4);
+        uint64_t lrb = lane_pull64(rb, gl * 4);
+        if (llane && lmode != 0) {
+            if (lmode == 1) {
+                // the prefetched [rb + RING, rb + RING + 512) replaces [rb, rb + 512)
+                const uint64_t h = lrb + RING + (uint64_t)t * 32;
+                *(uint4 *)&L.ring[gl][(h >> 2) & (RING / 4 - 1)] = pf0;
+                *(uint4 *)&L.ring[gl][((h + 16) >> 2) & (RING / 4 - 1)] = pf1;
+                lrb += 512;
+            } else {
+                lrb = lpos & ~(uint64_t)31;
+                for (int c = 0; c < RING / 512; c++) {
+                    const uint64_t h = lrb + (uint64_t)c * 512 + (uint64_t)t * 32;
+                    const uint4 a0 = chunk16(lbase, la, lqe, h), a1 = chunk16(lbase, la, lqe, h + 16);
+                    *(uint4 *)&L.ring[gl][(h >> 2) & (RING / 4 - 1)] = a0;
+                    *(uint4 *)&L.ring[gl][((h + 16) >> 2) & (RING / 4 - 1)] = a1;
+                }
+            }
+            const uint64_t h = lrb + RING + (uint64_t)t * 32;
+            pf0 = chunk16(lbase, la, lqe, h);
+            pf1 = chunk16(lbase, la, lqe, h + 16);
+        }
+        rb = lane_pull64(lrb, ((lane >> 2) & (DG - 1)) * 16);
+        first = false;
+        wave_lds_sync();
+
+        for (int j = 0; j < ROUND && k < kmax; j++, k++) {
+            const uint64_t i = (k << 2) + idx;
+            const bool act = err == 0 && i < n;
+            const uint32_t slot = xs & (RM - 1);
+            const uint32_t e = L.tab[g][slot];
+            const uint32_t di = (uint32_t)(pos >> 2);
+            const uint32_t w0 = L.ring[g][di & (RING / 4 - 1)];
+            const uint32_t w1 = L.ring[g][(di + 1) & (RING / 4 - 1)];
+            const uint32_t w2 = L.ring[g][(di + 2) & (RING / 4 - 1)];
+            uint32_t f = (e >> 8) & 4095;
+            f = f ? f : fz;
+            const uint32_t xn = f * (xs >> 12) + (e >> 20);
+            if (act) ((uint8_t *)L.outs[g])[i & 255] = (uint8_t)e;
+            const bool slow = __builtin_amdgcn_ballot_w64(act && xn < 128) != 0;
+            if (slow) {
+                // exact renormalization, state by state in symbol order
+                // (rans.rs:164-172); only corrupt streams get here
+                uint64_t p = pos;
+                uint32_t serr = 0, xv = xn;
+                for (int jj = 0; jj < 4; jj++) {
+                    if (act && idx == jj) {
+                        while (xv < RL) {
+                            if (p >= gqe) {
+                                serr = 1;
+                                break;
+                            }
+                            xv = (xv << 8) | gsrc[p];
+                            p++;
+                        }
+                    }
+                    p = quad_bcast64_dyn(p, jj);
+                }
+                if (act) xs = xv;
+                pos = p;
+                err |= quad_or(serr);
+            } else {
+                const uint32_t cnt = act ? (uint32_t)(xn < RL) + (uint32_t)(xn < (1u << 15)) : 0u;
+                const uint32_t incl = quad_scan(cnt, idx);
+                const uint32_t ex = incl - cnt;
+                const uint32_t tot = quad_bcast<3>(incl);
+                const uint32_t off = (uint32_t)(pos & 3) + ex;
+                const uint32_t sel = off >> 2;
+                const uint32_t lo = sel == 0 ? w0 : sel == 1 ? w1 : w2;
+                const uint32_t hi = sel == 0 ? w1 : w2;
+                const uint32_t d = __builtin_amdgcn_alignbyte(hi, lo, off & 3);
+                const uint32_t b0 = d & 255, b1 = (d >> 8) & 255;
+                const uint32_t over = act && pos + ex + cnt > gqe ? 1u : 0u;
+                const uint32_t xr =
+                    cnt == 0 ? xn : cnt == 1 ? (xn << 8) | b0 : (xn << 16) | (b0 << 8) | b1;
+                if (act) xs = xr;
+                pos += tot;
+                err |= quad_or(over);
+            }
+            if ((k & 63) == 63) {
+                // the staged 256 bytes of chunk k >> 6
+                wave_lds_sync();
+                if (llane)
+                    flush16(lout, lout16, ((k >> 6) << 8) + (uint64_t)t * 16, ln,
+                            *(const uint4 *)&L.outs[gl][t * 4]);
+                wave_lds_sync();
+            }
+            if (slow) {
+                // the window may no longer cover pos: end the round
+                k++;
+                break;
+            }
+        }
+    }
+    // the last, partial chunk (k = kmax, or the step where every stream ended)
+    if (k & 63) {
+        wave_lds_sync();
+        if (llane)
+            flush16(lout, lout16, (((k - 1) >> 6) << 8) + (uint64_t)t * 16, ln,
+                    *(const uint4 *)&L.outs[gl][t * 4]);
+    }
+    if (dlane && idx == 0) {
+        RansDecR r;
+        r.n = n;
+        r.status = st == 2 ? 2 : (err ? 1 : 0);
+        r.pad = 0;
+        res[s0 + (lane >> 2)] = r;
+    }
+}
+
+
+// ---- encode ----------------------------------------------------------------
+
+// histogram, normalize (rans.rs:25-50), header and encode table of one stream
+// per workgroup. Table entry: f | cum << 13 | l << 25 | m << 32 with
+// l = ceil(log2 f) and m = ceil(2^(31+l) / f), so that for every x < 2^31
+// floor(x / f) = (x * m) >> (31 + l) (Granlund and Montgomery 1994, thm 4.2:
+// 2^(31+l) <= m f <= 2^(31+l) + 2^l).
+__global__ void __launch_bounds__(256) k_rans_hist(const RansEncD *__restrict__ descs,
+                                                   RansEncR *__restrict__ res,
+                                                   uint64_t *__restrict__ tabs) {
+    __shared__ uint32_t h[4][8][256];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const RansEncD D = descs[blockIdx.x];
+    for (int i = tid; i < 4 * 8 * 256; i += 256) (&h[0][0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t la = (uint32_t)((uintptr_t)D.src & 15);
+    const uint8_t *base = D.src - la;
+    const uint64_t qe = la + D.n;
+    for (uint64_t q = (uint64_t)tid * 16; q < qe; q += 256 * 16) {
+        const uint4 v = chunk16(base, la, qe, q);
+        const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int b = 0; b < 16; b++) {
+            const uint64_t p = q + b;
+            if (p >= la && p < qe) atomicAdd(&h[w][lane & 7][(vv[b >> 2] >> (8 * (b & 3))) & 255], 1u);
+        }
+    }
+    __syncthreads();
+    if (w != 0) return;
+    // lane owns symbols 4 lane + j
+    uint32_t f[4];
+    uint64_t loc = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int s = 4 * lane + j;
+        uint32_t c = 0;
+        for (int ww = 0; ww < 4; ww++)
+            for (int cc = 0; cc < 8; cc++) c += h[ww][cc][s];
+        uint32_t fj = 0;
+        if (c) {
+            fj = (uint32_t)(((uint64_t)c * RM) / D.n);
+            fj = fj > 1 ? fj : 1;
+        }
+        f[j] = fj;
+        loc += fj;
+    }
+    auto add = [](uint64_t a, uint64_t b) { return a + b; };
+    auto mx = [](uint64_t a, uint64_t b) { return a > b ? a : b; };
+    uint64_t sum = qa_wave_reduce64(loc, add, 0);
+    // while sum > M: the largest f > 1 loses one, the last such symbol on ties
+    while (sum > RM) {
+        uint64_t key = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (f[j] > 1) key = mx(key, ((uint64_t)f[j] << 8) | (uint64_t)(4 * lane + j));
+        key = qa_wave_reduce64(key, mx, 0);
+        const uint32_t sb = (uint32_t)(key & 255);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if ((uint32_t)(4 * lane + j) == sb) f[j]--;
+        sum--;
+    }
+    if (sum < RM) {
+        uint64_t key = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) key = mx(key, ((uint64_t)f[j] << 8) | (uint64_t)(4 * lane + j));
+        key = qa_wave_reduce64(key, mx, 0);
+        const uint32_t sb = (uint32_t)(key & 255);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if ((uint32_t)(4 * lane + j) == sb) f[j] += RM - (uint32_t)sum;
+    }
+    // cum, present symbols and header entry bytes (sym, varint f)
+    uint64_t fl = 0, el = 0, np = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        fl += f[j];
+        if (f[j]) {
+            el += 1 + qa_varint_len(f[j]);
+            np++;
+        }
+    }
+    const uint64_t cinc = qa_wave_scan_add64(fl), einc = qa_wave_scan_add64(el);
+    const uint64_t nsym = qa_wave_reduce64(np, add, 0);
+    uint64_t cx = cinc - fl, ex = einc - el;
+    const uint32_t h0 = qa_varint_len(D.n) + qa_varint_len(nsym);
+    const uint64_t hdr = h0 + qa_wave_reduce64(el, add, 0);
+    if (lane == 0) {
+        uint32_t k = qa_put_varint(D.dst, D.n);
+        qa_put_varint(D.dst + k, nsym);
+        res[blockIdx.x].hdr_len = hdr;
+    }
+    uint64_t *tab = tabs + (uint64_t)blockIdx.x * 256;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t s = 4 * lane + j, fj = f[j];
+        uint64_t e = 0;
+        if (fj) {
+            D.dst[h0 + ex] = (uint8_t)s;
+            ex += 1 + qa_put_varint(D.dst + h0 + ex + 1, fj);
+            const uint32_t l = fj > 1 ? 32u - (uint32_t)__clz((int)(fj - 1)) : 0u;
+            const uint64_t m = ((1ull << (31 + l)) + fj - 1) / fj;
+            e = (uint64_t)fj | (cx << 13) | ((uint64_t)l << 25) | (m << 32);
+        }
+        tab[s] = e;
+        cx += fj;
+    }
+}
+
+// backward encode (rans.rs:73-97): one stream per 4 lanes, state = lane & 3;
+// the bytes go backward from the stream's scratch end, then the 4 states
+__global__ void __launch_bounds__(64) k_rans_enc(const RansEncD *__restrict__ descs,
+                                                 RansEncR *__restrict__ res,
+                                                 const uint64_t *__restrict__ tabs,
+                                                 uint8_t *__restrict__ scr, uint32_t ns) {
+    __shared__ uint64_t T[EG][256];
+    const int lane = threadIdx.x;
+    const int g = lane >> 2, idx = lane & 3;
+    const uint32_t s0 = blockIdx.x * EG;
+    for (int i = lane; i < EG * 256; i += 64) {
+        const uint32_t sg = s0 + (i >> 8);
+        T[i >> 8][i & 255] = sg < ns ? tabs[(uint64_t)sg * 256 + (i & 255)] : 0;
+    }
+    const bool valid = s0 + g < ns;
+    RansEncD D = {};
+    if (valid) D = descs[s0 + g];
+    __syncthreads();
+    const uint64_t n = D.n;
+    const uint64_t nk = (n + 3) >> 2;
+    auto mx = [](uint64_t a, uint64_t b) { return a > b ? a : b; };
+    const uint64_t tmax = qa_wave_reduce64(nk, mx, 0);
+    uint32_t x = RL;
+    uint64_t P = D.scr_end;
+    constexpr int U = 16;
+    uint32_t cur[U], nxt[U];
+    auto fetch = [&](uint64_t tb, uint32_t *buf) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint64_t t = tb + u;
+            const uint64_t i = ((nk - 1 - t) << 2) + idx;
+            buf[u] = t < nk && i < n ? D.src[i] : 0u;
+        }
+    };
+    fetch(0, cur);
+    for (uint64_t tb = 0; tb < tmax; tb += U) {
+        fetch(tb + U, nxt);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            // steps past tmax are inactive in every group
+            const uint64_t t = tb + u;
+            const uint64_t i = ((nk - 1 - t) << 2) + idx;
+            const bool act = t < nk && i < n;
+            const uint64_t e = T[g][cur[u]];
+            const uint32_t f = (uint32_t)e & 0x1FFF, c = (uint32_t)(e >> 13) & 0xFFF;
+            const uint32_t l = (uint32_t)(e >> 25) & 0xF, m = (uint32_t)(e >> 32);
+            const uint32_t xmax = f << 19;  // ((L >> 12) << 8) * f
+            const uint32_t cnt = act ? (uint32_t)(x >= xmax) + (uint32_t)((x >> 8) >= xmax) : 0u;
+            const uint32_t xsh = x >> (8 * cnt);
+            const uint32_t qd = (uint32_t)(((uint64_t)xsh * m) >> (31 + l));
+            const uint32_t xn = (qd << 12) + (xsh - qd * f) + c;
+            const uint32_t incl = quad_scan(cnt, idx);
+            const uint32_t tot = quad_bcast<3>(incl);
+            const uint64_t at = P - (tot - incl);  // one past this lane's bytes
+            if (cnt >= 1) scr[at - 1] = (uint8_t)x;
+            if (cnt == 2) scr[at - 2] = (uint8_t)(x >> 8);
+            P -= tot;
+            if (act) x = xn;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) cur[u] = nxt[u];
+    }
+    // states x0..x3 in front of the bytes (rans.rs:99-103)
+    P -= 16;
+    if (valid && n) {
+#pragma unroll
+        for (int b = 0; b < 4; b++) scr[P + 4 * idx + b] = (uint8_t)(x >> (8 * b));
+    }
+    if (valid && idx == 0) {
+        res[s0 + g].start = P;
+        res[s0 + g].out_len = n ? res[s0 + g].hdr_len + (D.scr_end - P) : 0;
+    }
+}
+
+// place each coded stream after its header
+__global__ void __launch_bounds__(256) k_rans_copy(const RansEncD *__restrict__ descs,
+                                                   const RansEncR *__restrict__ res,
+                                                   const uint8_t *__restrict__ scr) {
+    const RansEncD D = descs[blockIdx.x];
+    const RansEncR R = res[blockIdx.x];
+    const uint64_t len = D.scr_end - R.start;
+    const uint8_t *src = scr + R.start;
+    uint8_t *dst = D.dst + R.hdr_len;
+    const uint32_t head = (uint32_t)((4 - ((uintptr_t)dst & 3)) & 3);
+    const uint64_t h = head < len ? head : len;
+    if (threadIdx.x < h) dst[threadIdx.x] = src[threadIdx.x];
+    const uint64_t body = (len - h) >> 2;
+    uint32_t *dw = (uint32_t *)(dst + h);
+    for (uint64_t j = threadIdx.x; j < body; j += 256) dw[j] = qa_ld32u(src + h + 4 * j);
+    const uint64_t tail0 = h + 4 * body;
+    if (tail0 + threadIdx.x < len) dst[tail0 + threadIdx.x] = src[tail0 + threadIdx.x];
+}
+
+}  // namespace
+
+void qa_rans_decode_launch(const RansDecD *d, RansDecR *r, uint32_t ns, hipStream_t s) {
+    if (!ns) return;
+    hipLaunchKernelGGL(k_rans_dec, dim3((ns + DG - 1) / DG), dim3(64), 0, s, d, r, ns);
+}
+
+void qa_rans_encode_launch(const RansEncD *d, RansEncR *r, uint64_t *tabs, uint8_t *scr, uint32_t ns,
+                           hipStream_t s) {
+    if (!ns) return;
+    hipLaunchKernelGGL(k_rans_hist, dim3(ns), dim3(256), 0, s, d, r, tabs);
+    hipLaunchKernelGGL(k_rans_enc, dim3((ns + EG - 1) / EG), dim3(64), 0, s, d, r, tabs, scr, ns);
+    hipLaunchKernelGGL(k_rans_copy, dim3(ns), dim3(256), 0, s, d, r, scr);
+}

@@ -549,3 +549,121 @@ def test_rans_inverse_small_alphabets():
                 tot += f
             assert tot == 4096
     assert orc.rans_compress(b"") is None
+
+
+def _rans_table(c):
+    """(n, {sym: f}, offset of the states) of a rANS payload (rans.rs:98-108)."""
+    def rv(pos):
+        v, sh = 0, 0
+        while True:
+            b = c[pos]
+            pos += 1
+            v |= (b & 0x7F) << sh
+            sh += 7
+            if not b & 0x80:
+                return v, pos
+    n, pos = rv(0)
+    nsym, pos = rv(pos)
+    tab = {}
+    for _ in range(nsym):
+        s = c[pos]
+        f, pos = rv(pos + 1)
+        tab[s] = f
+    return n, tab, pos
+
+
+def test_rans_normalize_known_answers():
+    # f = max(1, floor(c * 4096 / n)), then the total is corrected to 4096
+    # (rans.rs:25-50); each case is worked by hand
+    # exact: counts 1, 1, 2 of n = 4 -> 1024, 1024, 2048
+    assert _rans_table(orc.rans_compress(bytes([0, 1, 2, 2])))[1] == {0: 1024, 1: 1024, 2: 2048}
+    # short by one: 1365 * 3 = 4095; the remainder goes to the LAST largest
+    assert _rans_table(orc.rans_compress(bytes([0, 1, 2])))[1] == {0: 1365, 1: 1365, 2: 1366}
+    # over by one: 5000 zeros and ones give floor(2047.39) = 2047 each, three
+    # singletons are raised to 1: 4097; the LAST largest f > 1 loses one
+    data = bytes([0] * 5000 + [1] * 5000 + [2, 3, 4])
+    assert _rans_table(orc.rans_compress(data))[1] == {0: 2047, 1: 2046, 2: 1, 3: 1, 4: 1}
+    # over by two with one dominant symbol: 9997 zeros -> floor(4094.77) = 4094
+    data = bytes([0] * 9997 + [1, 2, 3])
+    assert _rans_table(orc.rans_compress(data))[1] == {0: 4093, 1: 1, 2: 1, 3: 1}
+    # single symbol: all of M
+    assert _rans_table(orc.rans_compress(bytes([42] * 1000)))[1] == {42: 4096}
+
+
+def _lcg_bytes(seed, n, f):
+    s, out = seed, bytearray()
+    for _ in range(n):
+        s = (s * 6364136223846793005 + 1) & (2**64 - 1)
+        out.append(f(s))
+    return bytes(out)
+
+
+def test_rans_reference_roundtrip_shapes():
+    # the inputs of rans.rs tests::roundtrips and compresses_skewed
+    cases = [b"\x00", bytes([42] * 1000), bytes([1, 2, 3, 4, 5]),
+             _lcg_bytes(1, 20000, lambda s: (s >> 60) & (0xFF if s & 0xF == 0 else 0x07)),
+             bytes(i % 256 for i in range(256 * 40)),
+             _lcg_bytes(0x9E3779B9, 30000, lambda s: s >> 56)]
+    for d in cases:
+        c = orc.rans_compress(d)
+        n, tab, pos = _rans_table(c)
+        assert n == len(d) and sum(tab.values()) == 4096
+        assert set(tab) == set(d)
+        assert orc.rans_decompress(c, len(d)) == d
+    skew = bytes([0] * 9000 + [i % 255 + 1 for i in range(1000)])
+    enc = orc.rans_compress(skew)
+    assert len(enc) < len(skew) // 2
+    assert orc.rans_decompress(enc, len(skew)) == skew
+
+
+def test_rans_single_symbol_stream_is_states_only():
+    # f = M: x_max = 2^31 > every state, so no renormalization byte is ever
+    # written and each state stays L = 2^23 (x' = (x / M) * M + x % M + 0)
+    c = orc.rans_compress(bytes([7] * 999))
+    n, tab, pos = _rans_table(c)
+    assert tab == {7: 4096} and len(c) == pos + 16
+    assert struct.unpack("<4I", c[pos:]) == (1 << 23,) * 4
+
+
+def test_rans_decode_corner_cases():
+    # a frequency varint above u32 is narrowed before the range check
+    # (rans.rs:125 `as u32`): 2^32 + 4096 reads as 4096
+    def v(x):
+        out = bytearray()
+        while True:
+            b = x & 0x7F
+            x >>= 7
+            if x:
+                out.append(b | 0x80)
+            else:
+                out.append(b)
+                return bytes(out)
+    states = struct.pack("<4I", 1 << 23, 1 << 23, 1 << 23, 1 << 23)
+    p = v(8) + v(1) + bytes([9]) + v((1 << 32) + 4096) + states
+    assert orc.rans_decompress(p, 8) == bytes([9] * 8)
+    # a repeated symbol: freq keeps the last value, the sum counts both, and
+    # slots past cum[256] decode as symbol 0 (zero-filled table, rans.rs:141).
+    # With (5, 2048) twice, symbol 5 holds slots [0, 2048); state 0 = 2^23 + 2048
+    # has slot 2048 -> symbol 0 with f = 0: x' = 0 + 2048 - 0 = 2048, two bytes
+    st0 = struct.pack("<4I", (1 << 23) + 2048, 1 << 23, 1 << 23, 1 << 23)
+    p = v(1) + v(2) + bytes([5]) + v(2048) + bytes([5]) + v(2048) + st0 + b"\x01\x02"
+    assert orc.rans_decompress(p, 1) == b"\x00"
+    with pytest.raises(ValueError):
+        orc.rans_decompress(p[:-1], 1)  # the second byte is missing
+    # state 0: slot 0 -> symbol 5, x' = 2048 * 0 + 0 - 0 = 0, and bytes are
+    # pulled until x >= 2^23: 0x01, 0x0102, 0x010203, 0x01020304 (four bytes)
+    z = struct.pack("<4I", 0, 1 << 23, 1 << 23, 1 << 23)
+    p = v(1) + v(2) + bytes([5]) + v(2048) + bytes([5]) + v(2048) + z + b"\x01\x02\x03\x04"
+    assert orc.rans_decompress(p, 1) == b"\x05"
+    with pytest.raises(ValueError):
+        orc.rans_decompress(p[:-1], 1)
+    # frequencies not summing to M, an empty table, too many symbols
+    for bad in (v(1) + v(1) + bytes([1]) + v(4095) + states,
+                v(1) + v(0) + states, v(1) + v(257) + states,
+                v(1) + v(1) + bytes([1]) + v(0) + states,
+                v(1) + v(1) + bytes([1]) + v(4097) + states):
+        with pytest.raises(ValueError):
+            orc.rans_decompress(bad, 1)
+    # n above the bound
+    with pytest.raises(ValueError):
+        orc.rans_decompress(v(9) + v(1) + bytes([9]) + v(4096) + states, 8)
