@@ -211,6 +211,145 @@ def serve_bench(qa, dev, budget_gb=64, slots=2, pool_size=48, per_point=800):
             "points": points}
 
 
+DELTA2_COEFFS = [[], [1.0], [2.0, -1.0], [3.0, -3.0, 1.0], [4.0, -6.0, 4.0, -1.0]]
+
+
+def delta2_residuals(torch, V, block):
+    """Delta2 residual streams (src/codecs/linear.rs:125-160) of each `block`-value
+    block of the f64 lane V, on the device: order = the smallest mean |finite
+    difference| over the central 1024 values (torch's mean, not the reference's
+    sequential sum: a workload generator, not a parity path), pred = 0.0 +
+    c_0 h_0 + c_1 h_1 + ... with the order dropped to the history available, and
+    the stream [order] ++ varint(bits(v) ^ bits(pred)). Returns the concatenated
+    bytes (uint8 tensor) and each block's (offset, length)."""
+    nb = V.numel() // block
+    W = V[: nb * block].view(nb, block)
+    start = (block - 1024) // 2
+    d = W[:, start:start + 1024]
+    mags = []
+    for _ in range(4):
+        d = d[:, 1:] - d[:, :-1]
+        m = d.abs().mean(1)
+        mags.append(torch.where(torch.isfinite(m), m, torch.full_like(m, float("inf"))))
+    order = torch.stack(mags, 1).argmin(1) + 1
+    P = torch.zeros_like(W)
+    for o in range(1, 5):
+        rows = (order == o).nonzero().flatten()
+        if rows.numel() == 0:
+            continue
+        Wo = W[rows]
+        Po = torch.zeros_like(Wo)
+        for i in range(1, 4):  # warm-up: order limited to the i values before
+            c = DELTA2_COEFFS[min(o, i)]
+            acc = torch.zeros_like(Wo[:, 0])
+            for k, ck in enumerate(c):
+                acc = acc + ck * Wo[:, i - 1 - k]
+            Po[:, i] = acc
+        c = DELTA2_COEFFS[o]
+        acc = torch.zeros_like(Wo[:, 4:])
+        for k, ck in enumerate(c):
+            acc = acc + ck * Wo[:, 3 - k: block - 1 - k]
+        Po[:, 4:] = acc
+        P[rows] = Po
+    R = (W.contiguous().view(torch.int64) ^ P.view(torch.int64)).flatten()
+    L = torch.ones_like(R)
+    for j in range(1, 10):
+        L += ((R >> (7 * j)) != 0).to(torch.int64)
+    blen = L.view(nb, block).sum(1) + 1
+    boff = torch.cumsum(blen, 0) - blen
+    total = int(blen.sum())
+    # value bytes follow every earlier value and one order byte per block so far
+    nbk = torch.arange(1, nb + 1, device=V.device)
+    pos = torch.cumsum(L, 0) - L + torch.repeat_interleave(nbk, block)
+    out = torch.empty(total, dtype=torch.uint8, device=V.device)
+    out[boff] = order.to(torch.uint8)
+    for j in range(10):
+        m = L > j
+        b = ((R >> (7 * j)) & (1 if j == 9 else 0x7F)) | ((L > j + 1).to(torch.int64) << 7)
+        out[pos[m] + j] = b[m].to(torch.uint8)
+    return out, list(zip(boff.tolist(), blen.tolist()))
+
+
+def entropy_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=3):
+    """rANS decode on the entropy workload (config: 32 Mi f64 sin(2 pi i / 1000)
+    * 1e3 + N(0, 1e-3) and 32 Mi f32 ((u32 >> 8) 2^-23) - 1 widened to f64
+    lanes): each 32768-value block's Delta2 residual bytes are rANS-coded by
+    the GPU coder, then every block stream is decoded in one launch; decoded
+    bytes are checked against the residuals."""
+    import torch
+    r = np.random.default_rng(seed)
+    i = np.arange(n, dtype=np.float64)
+    f64 = np.sin(2 * np.pi * i / 1000) * 1000 + r.normal(0, 1e-3, n)
+    u = r.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    f32 = (u >> 8).astype(np.float32) * np.float32(2.0**-23) - np.float32(1)
+    res, blocks = [], []
+    for lane in (f64, f32.astype(np.float64)):
+        b, bl = delta2_residuals(torch, torch.from_numpy(lane).to(dev), block)
+        res.append(b)
+        blocks.append(bl)
+    syms = torch.cat(res)
+    base = [0, res[0].numel()]
+    spans = [(base[c] + o, ln) for c in range(2) for o, ln in blocks[c]]
+    caps = [qa.lib.quoin_rans_bound(ln) for _, ln in spans]
+    coff = np.concatenate([[0], np.cumsum(caps)]).tolist()
+    coded = torch.empty(coff[-1], dtype=torch.uint8, device=dev)
+    dec = torch.empty(syms.numel(), dtype=torch.uint8, device=dev)
+    sp, cp, dp = syms.data_ptr(), coded.data_ptr(), dec.data_ptr()
+    ins = [qa.RansInput(sp + o, ln, cp + coff[k], caps[k]) for k, (o, ln) in enumerate(spans)]
+    torch.cuda.synchronize()
+    lens = gpu.rans_compress(ins)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        gpu.rans_compress(ins)
+    torch.cuda.synchronize()
+    enc_s = (time.perf_counter() - t0) / 2
+    sts = [qa.RansStream(cp + coff[k], lens[k], ln, dp + o) for k, (o, ln) in enumerate(spans)]
+    dl, st = gpu.rans_decompress(sts)
+    torch.cuda.synchronize()
+    assert all(x == qa.OK for x in st) and torch.equal(dec, syms), "rANS round trip mismatch"
+    for _ in range(warmup):
+        gpu.rans_decompress(sts)
+    torch.cuda.synchronize()
+    kern = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gpu.rans_decompress(sts)
+        kern += gpu.last_timing()[1]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    nsym = syms.numel()
+    cbytes = int(sum(lens))
+    k_ms = kern / steps
+    alg = cbytes + nsym
+    sample = [bytes(coded[coff[k]:coff[k] + lens[k]].cpu().numpy()) for k in range(0, len(spans), 64)]
+    sample_len = [spans[k][1] for k in range(0, len(spans), 64)]
+    return {
+        "workload": "entropy decode: 32Mi f64 sine+noise and 32Mi f32 uniform[-1,1), Delta2 "
+                    "residual bytes of each 32768-value block, rANS-coded (one stream per block)",
+        "streams": len(spans), "symbols": nsym, "coded_bytes": cbytes,
+        "decode_gbs": round(nsym / (k_ms * 1e-3) / 1e9, 2),
+        "decode_wall_gbs": round(nsym * steps / wall / 1e9, 2),
+        "encode_gbs": round(nsym / enc_s / 1e9, 2),
+        "roofline": {"bound": "hbm", "kernel": "k_rans_dec", "kernel_ms": round(k_ms, 4),
+                     "algorithmic_bytes": alg,
+                     "achieved": round(alg / (k_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
+    }, (sample, sample_len)
+
+
+def rans_cpu_baseline(sample, sample_len):
+    """Oracle rANS decode (single-threaded C) over every 64th block stream."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as orc
+    t0 = time.perf_counter()
+    for c, ln in zip(sample, sample_len):
+        orc.rans_decompress(c, ln)
+    dt = time.perf_counter() - t0
+    return {"value": round(sum(sample_len) / dt / 1e9, 4), "unit": "GB/s", "cores": 1,
+            "kind": "port", "sample": f"{len(sample)} of the block streams ({sum(sample_len)} symbols), "
+                                      "rANS decode, oracle C, 1 thread"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,6 +362,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="steps of the plain form (one launch sequence per block); 0 skips it")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--no-entropy", action="store_true", help="skip the rANS decode leg")
+    ap.add_argument("--entropy-n", type=int, default=32 << 20,
+                    help="values per column of the rANS decode leg")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="bytes of a plain elementwise copy run after the measurement "
                          "(for scaling HBM traffic counters)")
@@ -361,12 +503,18 @@ def main():
     if not args.no_serve:
         serving = serve_bench(qa, dev)
 
+    entropy = ent_sample = None
+    if not args.no_entropy:
+        entropy, ent_sample = entropy_bench(qa, gpu, dev, args.entropy_n, 10, 2)
+
     cpu = None
     if not args.no_cpu:
         cpu_gbs, sample_raw = cpu_baseline(cols, int(level), args.cpu_sample)
         cpu = {"value": round(cpu_gbs, 4), "unit": "GB/s", "cores": 1, "kind": "port",
                "sample": f"first {args.cpu_sample} values of each of the 4 columns "
                          f"({sample_raw / 1e6:.1f} MB), compress+decompress, oracle C, 1 thread"}
+        if entropy is not None:
+            entropy["cpu_baseline"] = rans_cpu_baseline(*ent_sample)
 
     out = {
         "metric": "compress+decompress GB/s (bit-exact), batched",
@@ -386,6 +534,7 @@ def main():
         "roofline": roofline,
         "encode_roofline": enc_roofline,
         "serving": serving,
+        "entropy_decode": entropy,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -34,9 +34,11 @@ constexpr int EG = 16;                 // encode: streams per wavefront
 
 struct DecLds {
     uint32_t tab[DG][RM];              // slot -> sym | (f & 4095) << 8 | (slot - cum) << 20
-    uint32_t ring[DG][RING / 4];       // payload dwords by aligned position
+    uint32_t ring[DG][RING / 4 + 4];   // payload dwords by aligned position; the
+                                       // first 4 repeated after the end
     uint32_t cum[DG][260];             // freq during the header parse, then cum
-    uint32_t outs[DG][64];             // 256 staged output bytes per stream
+    uint32_t outs[DG + 1][64];         // 256 staged output bytes per stream; the
+                                       // last row takes the idle lanes' writes
 };
 
 // 4-lane (quad) helpers on DPP; every lane of the wavefront must be active
@@ -166,7 +168,6 @@ __global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ de
     uint32_t st = 0;            // 0 ok, 1 corrupt, 2 unsupported (tagged, other coder)
     uint64_t n = 0, q = 0;
     uint32_t x[4] = {0, 0, 0, 0};
-    uint32_t fz = 0;
     // the group's stream descriptor, from the load-side lanes
     const uint8_t *gsrc = (const uint8_t *)lane_pull64((uint64_t)(uintptr_t)lbase, (lane >> 2) * 16);
     const uint64_t gqe = lane_pull64(lqe, (lane >> 2) * 16);
@@ -189,6 +190,7 @@ __global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ de
         uint64_t nsym = 0;
         if (st == 0 && hdr_varint(L, g, gsrc, gqe, &q, &n)) st = 1;
         if (st == 0 && n > gmax) st = 1;
+        if (st == 0 && n >= (1ull << 34)) st = 2;  // step counter is 32-bit
         if (st == 0 && hdr_varint(L, g, gsrc, gqe, &q, &nsym)) st = 1;
         if (st == 0 && (nsym == 0 || nsym > 256)) st = 1;
         uint32_t sum = 0;
@@ -226,21 +228,23 @@ __global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ de
             x[i] = v;
             q += 4;
         }
-        // one symbol holding all of M: its f field (f & 4095) reads as 4096
-        fz = nsym == 1 ? RM : 0;
     }
     wave_lds_sync();
     // broadcast the leader's results inside each quad
     st = quad_bcast<0>(st);
     n = (uint64_t)quad_bcast<0>((uint32_t)n) | ((uint64_t)quad_bcast<0>((uint32_t)(n >> 32)) << 32);
     q = (uint64_t)quad_bcast<0>((uint32_t)q) | ((uint64_t)quad_bcast<0>((uint32_t)(q >> 32)) << 32);
-    fz = quad_bcast<0>(fz);
     uint32_t x0 = quad_bcast<0>(x[0]), x1 = quad_bcast<0>(x[1]), x2 = quad_bcast<0>(x[2]),
              x3 = quad_bcast<0>(x[3]);
     uint32_t xs = idx == 0 ? x0 : idx == 1 ? x1 : idx == 2 ? x2 : x3;
     if (!dlane) st = 1;
 
-    // slot tables (rans.rs:136-146): cum by a wavefront scan, then 64 slots per lane
+    // slot tables (rans.rs:136-146): cum by a wavefront scan, then 64 slots per
+    // lane. Entry: sym | (f - 1) << 8 | (slot - cum) << 20. A table whose
+    // frequencies sum below M (a repeated symbol) leaves slots past cum[256] to
+    // symbol 0; when freq[0] = 0 those slots have f = 0, which the entry cannot
+    // hold: such a stream ("irregular") decodes on the exact path.
+    uint32_t irr = 0, ctot = RM;
     for (int gg = 0; gg < DG; gg++) {
         const uint32_t sg = (uint32_t)__builtin_amdgcn_readlane((int)st, gg * 4);
         if (s0 + gg >= ns || sg != 0) continue;
@@ -258,6 +262,10 @@ __global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ de
         wave_lds_sync();
         const uint32_t tot = L.cum[gg][256];
         const uint32_t freq0 = L.cum[gg][1] - L.cum[gg][0];
+        if (g == gg) {
+            irr = tot < RM && freq0 == 0 ? 1u : 0u;
+            ctot = tot;
+        }
         const uint32_t slot0 = (uint32_t)lane * 64;
         // largest s in [0, 255] with cum[s] <= slot0
         uint32_t lo = 0, hi = 255;
@@ -272,11 +280,11 @@ __global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ de
             uint32_t e;
             if (slot >= tot) {
                 // slots past cum[256] keep symbol 0 (rans.rs:141 zero-fills)
-                e = ((freq0 & 4095) << 8) | (slot << 20);
+                e = (((freq0 - 1) & 4095) << 8) | (slot << 20);
             } else {
                 while (L.cum[gg][s + 1] <= slot) s++;
                 const uint32_t c = L.cum[gg][s];
-                e = s | (((L.cum[gg][s + 1] - c) & 4095) << 8) | ((slot - c) << 20);
+                e = s | (((L.cum[gg][s + 1] - c - 1) & 4095) << 8) | ((slot - c) << 20);
             }
             L.tab[gg][slot] = e;
         }
@@ -284,27 +292,36 @@ __global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ de
     }
 
     // ---- decode loop -------------------------------------------------------
+    // lane idx decodes symbols 4k + idx < n: steps k < klim
+    uint32_t klim = st == 0 ? (uint32_t)((n + 3 - idx) >> 2) : 0u;
     uint32_t err = st != 0 ? 1u : 0u;
-    uint64_t pos = q;                        // aligned position of the next byte
-    uint64_t rb = 0;                         // ring covers [rb, rb + RING)
+    uint32_t pos = (uint32_t)q;              // aligned position of the next byte
+    const uint32_t qe32 = (uint32_t)gqe;
+    uint32_t rb = 0;                         // the ring holds [rb, rb + RING)
     uint4 pf0 = make_uint4(0, 0, 0, 0), pf1 = pf0;  // prefetched [rb + RING, rb + RING + 512)
-    const uint64_t nk_g = st == 0 ? (n + 3) >> 2 : 0;
-    uint64_t kmax = 0;
-    for (int gg = 0; gg < DG; gg++) {
-        uint64_t v = qa_readlane64(nk_g, gg * 4);
-        kmax = v > kmax ? v : kmax;
-    }
-    // load-side view of each stream's state (no bytes for a failed header)
+    auto mx = [](uint64_t a, uint64_t b) { return a > b ? a : b; };
+    const uint32_t kmax = (uint32_t)qa_wave_reduce64(klim, mx, 0);
+    const uint32_t sh8 = 8u * (uint32_t)idx, below = (1u << sh8) - 1u;
+    const int og = lane < 4 * DG ? g : DG;
+    // slow-path threshold: states below 128 need 3+ bytes; irregular tables
+    // always take the exact path
+    const uint32_t thr = irr ? 0xFFFFFFFFu : 128u;
+    // load-side view of each stream (no bytes for a failed header)
     const uint64_t ln = lane_pull64(st == 0 ? n : 0, gl * 4);
     uint8_t *const lout = dl.out;
     const bool lout16 = ((uintptr_t)lout & 15) == 0;
+    auto ring_put = [&](uint32_t h, uint4 v) {
+        const uint32_t di = (h >> 2) & (RING / 4 - 1);
+        *(uint4 *)&L.ring[gl][di] = v;
+        if (di == 0) *(uint4 *)&L.ring[gl][RING / 4] = v;
+    };
 
     bool first = true;
-    uint64_t k = 0;
+    uint32_t k = 0;
     while (k < kmax) {
         // window maintenance for the coming round (<= ROUND steps of <= 8 bytes,
         // each reading 12 bytes at pos & ~3): 0 keep, 1 shift a half, 2 reload
-        const bool act_g = err == 0 && (k << 2) < n;
+        const bool act_g = err == 0 && k < klim;
         // every stream done or failed: stop (a corrupt length must not keep
         // the wavefront stepping)
         if (!__builtin_amdgcn_ballot_w64(act_g)) break;
@@ -316,106 +333,127 @@ __global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ de
             else if (pos > rb + (RING - 516))
                 mode = pos >= rb + 512 && pos <= rb + 512 + (RING - 516) ? 1 : 2;
         }
+        mode = quad_or(mode);  // a group whose lanes 1..3 finished still moves
         const uint32_t lmode = lane_pull(mode, gl * 4);
-        const uint64_t lpos = lane_pull64(pos, gl * 4);
-        uint64_t lrb = lane_pull64(rb, gl * 4);
+        const uint32_t lpos = lane_pull(pos, gl * 4);
+        uint32_t lrb = lane_pull(rb, gl * 4);
         if (llane && lmode != 0) {
             if (lmode == 1) {
                 // the prefetched [rb + RING, rb + RING + 512) replaces [rb, rb + 512)
-                const uint64_t h = lrb + RING + (uint64_t)t * 32;
-                *(uint4 *)&L.ring[gl][(h >> 2) & (RING / 4 - 1)] = pf0;
-                *(uint4 *)&L.ring[gl][((h + 16) >> 2) & (RING / 4 - 1)] = pf1;
+                const uint32_t h = lrb + RING + (uint32_t)t * 32;
+                ring_put(h, pf0);
+                ring_put(h + 16, pf1);
                 lrb += 512;
             } else {
-                lrb = lpos & ~(uint64_t)31;
+                lrb = lpos & ~31u;
                 for (int c = 0; c < RING / 512; c++) {
-                    const uint64_t h = lrb + (uint64_t)c * 512 + (uint64_t)t * 32;
+                    const uint32_t h = lrb + (uint32_t)c * 512 + (uint32_t)t * 32;
                     const uint4 a0 = chunk16(lbase, la, lqe, h), a1 = chunk16(lbase, la, lqe, h + 16);
-                    *(uint4 *)&L.ring[gl][(h >> 2) & (RING / 4 - 1)] = a0;
-                    *(uint4 *)&L.ring[gl][((h + 16) >> 2) & (RING / 4 - 1)] = a1;
+                    ring_put(h, a0);
+                    ring_put(h + 16, a1);
                 }
             }
-            const uint64_t h = lrb + RING + (uint64_t)t * 32;
+            const uint32_t h = lrb + RING + (uint32_t)t * 32;
             pf0 = chunk16(lbase, la, lqe, h);
             pf1 = chunk16(lbase, la, lqe, h + 16);
         }
-        rb = lane_pull64(lrb, ((lane >> 2) & (DG - 1)) * 16);
+        rb = lane_pull(lrb, ((lane >> 2) & (DG - 1)) * 16);
         first = false;
         wave_lds_sync();
 
-        for (int j = 0; j < ROUND && k < kmax; j++, k++) {
-            const uint64_t i = (k << 2) + idx;
-            const bool act = err == 0 && i < n;
-            const uint32_t slot = xs & (RM - 1);
-            const uint32_t e = L.tab[g][slot];
-            const uint32_t di = (uint32_t)(pos >> 2);
-            const uint32_t w0 = L.ring[g][di & (RING / 4 - 1)];
-            const uint32_t w1 = L.ring[g][(di + 1) & (RING / 4 - 1)];
-            const uint32_t w2 = L.ring[g][(di + 2) & (RING / 4 - 1)];
-            uint32_t f = (e >> 8) & 4095;
-            f = f ? f : fz;
-            const uint32_t xn = f * (xs >> 12) + (e >> 20);
-            if (act) ((uint8_t *)L.outs[g])[i & 255] = (uint8_t)e;
-            const bool slow = __builtin_amdgcn_ballot_w64(act && xn < 128) != 0;
-            if (slow) {
-                // exact renormalization, state by state in symbol order
-                // (rans.rs:164-172); only corrupt streams get here
-                uint64_t p = pos;
-                uint32_t serr = 0, xv = xn;
-                for (int jj = 0; jj < 4; jj++) {
-                    if (act && idx == jj) {
-                        while (xv < RL) {
-                            if (p >= gqe) {
-                                serr = 1;
-                                break;
-                            }
-                            xv = (xv << 8) | gsrc[p];
-                            p++;
-                        }
-                    }
-                    p = quad_bcast64_dyn(p, jj);
-                }
-                if (act) xs = xv;
-                pos = p;
-                err |= quad_or(serr);
-            } else {
-                const uint32_t cnt = act ? (uint32_t)(xn < RL) + (uint32_t)(xn < (1u << 15)) : 0u;
-                const uint32_t incl = quad_scan(cnt, idx);
-                const uint32_t ex = incl - cnt;
-                const uint32_t tot = quad_bcast<3>(incl);
-                const uint32_t off = (uint32_t)(pos & 3) + ex;
-                const uint32_t sel = off >> 2;
-                const uint32_t lo = sel == 0 ? w0 : sel == 1 ? w1 : w2;
-                const uint32_t hi = sel == 0 ? w1 : w2;
-                const uint32_t d = __builtin_amdgcn_alignbyte(hi, lo, off & 3);
-                const uint32_t b0 = d & 255, b1 = (d >> 8) & 255;
-                const uint32_t over = act && pos + ex + cnt > gqe ? 1u : 0u;
-                const uint32_t xr =
-                    cnt == 0 ? xn : cnt == 1 ? (xn << 8) | b0 : (xn << 16) | (b0 << 8) | b1;
-                if (act) xs = xr;
-                pos += tot;
-                err |= quad_or(over);
-            }
-            if ((k & 63) == 63) {
-                // the staged 256 bytes of chunk k >> 6
-                wave_lds_sync();
-                if (llane)
-                    flush16(lout, lout16, ((k >> 6) << 8) + (uint64_t)t * 16, ln,
-                            *(const uint4 *)&L.outs[gl][t * 4]);
-                wave_lds_sync();
-            }
-            if (slow) {
-                // the window may no longer cover pos: end the round
-                k++;
+        // steps to the end of the current 64-step output chunk
+        const uint32_t rend = min(kmax, (k | 63u) + 1u);
+        uint32_t sp = (uint32_t)og * 256 + ((k & 63) << 2) + (uint32_t)idx;  // staging byte
+        bool slow = false;
+        for (; k < rend; k++, sp += 4) {
+            const bool act = k < klim;
+            const uint32_t e = L.tab[g][xs & (RM - 1)];
+            const uint32_t *wp = &L.ring[g][(pos >> 2) & (RING / 4 - 1)];
+            const uint32_t w0 = wp[0], w1 = wp[1], w2 = wp[2];
+            // keep the window reads next to the table read (one LDS wait per
+            // step) instead of sunk into the branch that uses them
+            asm volatile("" ::"v"(w0), "v"(w1), "v"(w2), "v"(e));
+            const uint32_t xhi = xs >> 12;
+            // x' = f (x >> 12) + slot - cum with f - 1 from the table
+            const uint32_t xn = __umul24(((e >> 8) & 4095) + 1, xhi) + (e >> 20);
+            // lanes past their end (or of a failed stream) need no bytes
+            const uint32_t xe = act ? xn : 0xFFFFFFFFu;
+            if (__builtin_amdgcn_ballot_w64(xe < thr)) {
+                // 3+ bytes or an irregular table: the exact step below
+                slow = true;
                 break;
             }
+            // staged symbol (lanes past their end write bytes the flush skips;
+            // lanes without a stream write the spare row)
+            ((uint8_t *)L.outs)[sp] = (uint8_t)e;
+            // bytes per state: 0, 1 or 2 (xn >= 128); the quad's counts packed in
+            // one word give each lane's offset and the total by byte sums
+            const uint32_t cnt = (xe < RL ? 1u : 0u) + (xe < (1u << 15) ? 1u : 0u);
+            uint32_t w = cnt << sh8;
+            w |= (uint32_t)__builtin_amdgcn_mov_dpp((int)w, QA_DPP_QP_1032, 0xF, 0xF, false);
+            w |= (uint32_t)__builtin_amdgcn_mov_dpp((int)w, QA_DPP_QP_2301, 0xF, 0xF, false);
+            const uint32_t ex = __builtin_amdgcn_sad_u8(w & below, 0u, 0u);
+            const uint32_t tot = __builtin_amdgcn_sad_u8(w, 0u, 0u);
+            const uint32_t off = (pos & 3) + ex;
+            const uint32_t lo = off < 4 ? w0 : (off < 8 ? w1 : w2);
+            const uint32_t hi = off < 4 ? w1 : w2;
+            const uint32_t d = __builtin_amdgcn_alignbyte(hi, lo, off);
+            const uint32_t bs = __builtin_amdgcn_perm(0u, d, 0x0c0c0001u);  // b0 << 8 | b1
+            const uint32_t sh = cnt << 3;
+            // (a lane past its end keeps stepping on a state nothing reads)
+            xs = (xn << sh) | (bs >> (16 - sh));
+            pos += tot;
+            // a byte at or past the payload end: truncated (rans.rs:167)
+            const bool over = pos > qe32;
+            err |= over ? 1u : 0u;
+            klim = over ? 0u : klim;
+        }
+        if (slow) {
+            // exact renormalization of step k, state by state in symbol order
+            // (rans.rs:157-173): states needing 3+ bytes, irregular tables. The
+            // window may not cover pos afterwards, so this step ends the round.
+            const bool act = k < klim;
+            const uint32_t slot = xs & (RM - 1);
+            const uint32_t e = L.tab[g][slot];
+            ((uint8_t *)L.outs)[sp] = (uint8_t)e;
+            const uint32_t fx = irr && slot >= ctot ? 0u : ((e >> 8) & 4095) + 1;
+            uint32_t xv = fx * (xs >> 12) + (e >> 20);
+            uint64_t p = pos;
+            uint32_t serr = 0;
+            for (int jj = 0; jj < 4; jj++) {
+                if (act && idx == jj) {
+                    while (xv < RL) {
+                        if (p >= gqe) {
+                            serr = 1;
+                            break;
+                        }
+                        xv = (xv << 8) | gsrc[p];
+                        p++;
+                    }
+                }
+                p = quad_bcast64_dyn(p, jj);
+            }
+            xs = xv;
+            pos = (uint32_t)p;
+            const uint32_t bad = quad_or(serr);
+            err |= bad;
+            klim = bad ? 0u : klim;
+            k++;
+        }
+        if ((k & 63) == 0) {
+            // the staged 256 bytes of chunk (k - 1) >> 6 are complete
+            wave_lds_sync();
+            if (llane)
+                flush16(lout, lout16, ((uint64_t)((k - 1) >> 6) << 8) + (uint64_t)t * 16, ln,
+                        *(const uint4 *)&L.outs[gl][t * 4]);
+            wave_lds_sync();
         }
     }
     // the last, partial chunk (k = kmax, or the step where every stream ended)
     if (k & 63) {
         wave_lds_sync();
         if (llane)
-            flush16(lout, lout16, (((k - 1) >> 6) << 8) + (uint64_t)t * 16, ln,
+            flush16(lout, lout16, ((uint64_t)((k - 1) >> 6) << 8) + (uint64_t)t * 16, ln,
                     *(const uint4 *)&L.outs[gl][t * 4]);
     }
     if (dlane && idx == 0) {
