@@ -59,6 +59,19 @@ struct DCol {
 // narrow Arrow inputs widened on load (capi/src/typed.rs:147-153)
 enum { QA_IN_NATIVE = 0, QA_IN_I8 = 1, QA_IN_I16 = 2, QA_IN_U8 = 3, QA_IN_U16 = 4 };
 
+// element form of a decoded column's output (src/lib.rs:492-500)
+enum { QA_OUT_U8 = 0, QA_OUT_U16, QA_OUT_U32, QA_OUT_F32, QA_OUT_U64 };
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline uint32_t qa_out_form(uint32_t dtype, uint32_t out_kind) {
+    if (out_kind == QA_IN_I8 || out_kind == QA_IN_U8) return QA_OUT_U8;
+    if (out_kind == QA_IN_I16 || out_kind == QA_IN_U16) return QA_OUT_U16;
+    if (dtype == QA_F32) return QA_OUT_F32;
+    if (dtype == QA_I32 || dtype == QA_U32) return QA_OUT_U32;
+    return QA_OUT_U64;
+}
+
 // How a column's values are read as u64 lane words (src/lib.rs:389-403): the
 // kernels widen on load, so a column without nulls is read in place and a
 // column with nulls is compacted in its own element width.

@@ -141,18 +141,8 @@ __device__ __forceinline__ uint32_t select64(uint64_t w, uint32_t r) {
 }
 
 // Output element of the column (its narrow Arrow type, src/lib.rs:492-506).
-enum { QA_OUT_U8 = 0, QA_OUT_U16, QA_OUT_U32, QA_OUT_F32, QA_OUT_U64 };
 __device__ __forceinline__ int out_form(const DDecCol &col) {
-    switch (col.out_kind) {
-    case QA_IN_I8:
-    case QA_IN_U8: return QA_OUT_U8;
-    case QA_IN_I16:
-    case QA_IN_U16: return QA_OUT_U16;
-    default: break;
-    }
-    if (col.dtype == QA_F32) return QA_OUT_F32;
-    if (col.dtype == QA_I32 || col.dtype == QA_U32) return QA_OUT_U32;
-    return QA_OUT_U64;
+    return (int)qa_out_form(col.dtype, col.out_kind);
 }
 template <int F>
 __device__ __forceinline__ void st_out(void *out, uint64_t i, uint64_t x) {
@@ -189,10 +179,9 @@ __device__ __forceinline__ void emit_dense(void *out, uint32_t oform, uint64_t b
 // vv[0..cnt) fill the valid slots of logical positions [L0, L1), null slots get
 // zero. Lanes load up to 64 bitmap words of the range at once and scan their
 // popcounts; the wavefront then stores one word's 64 positions per step.
-template <int F>
-__device__ __forceinline__ void emit_gather_f(void *out, const uint64_t *bm, uint64_t L0,
-                                              uint64_t L1, const uint64_t *vv, uint32_t cnt,
-                                              int lane) {
+template <int F, typename V>
+__device__ __forceinline__ void emit_gather_t(void *out, const uint64_t *bm, uint64_t L0,
+                                              uint64_t L1, const V *vv, uint32_t cnt, int lane) {
     if (L1 <= L0) return;
     const uint64_t jb = L0 >> 6, je = (L1 - 1) >> 6;  // inclusive word range
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -238,6 +227,12 @@ __device__ __forceinline__ void emit_gather_f(void *out, const uint64_t *bm, uin
         r += qa_readlane64(inc, 63);
     }
 }
+template <int F>
+__device__ __forceinline__ void emit_gather_f(void *out, const uint64_t *bm, uint64_t L0,
+                                              uint64_t L1, const uint64_t *vv, uint32_t cnt,
+                                              int lane) {
+    emit_gather_t<F, uint64_t>(out, bm, L0, L1, vv, cnt, lane);
+}
 __device__ __forceinline__ void emit_gather(void *out, uint32_t oform, const uint64_t *bm,
                                             uint64_t L0, uint64_t L1, const uint64_t *vv,
                                             uint32_t cnt, int lane) {
@@ -247,6 +242,97 @@ __device__ __forceinline__ void emit_gather(void *out, uint32_t oform, const uin
     case QA_OUT_U32: emit_gather_f<QA_OUT_U32>(out, bm, L0, L1, vv, cnt, lane); break;
     case QA_OUT_F32: emit_gather_f<QA_OUT_F32>(out, bm, L0, L1, vv, cnt, lane); break;
     default: emit_gather_f<QA_OUT_U64>(out, bm, L0, L1, vv, cnt, lane); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lane-contiguous FastLanes unpack of the 32-lane layout (w <= 32,
+// src/bitpack.rs:78-103): lane L gets the tile's values k = 16 L + i (i < 16).
+// They all lie in row r = L >> 1, in FastLanes lanes 16 (L & 1) + i, so each is
+// one funnel shift of the row-words j = (r w) >> 5 and j + 1 at the lane-wide
+// shift (r w) & 31: two runs of 16 consecutive staged words per lane.
+// Row-word j + 1 may lie past the packed words (read from the staging buffer,
+// which is larger, and masked off).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void unpack16_u32(const uint32_t *wd, uint32_t w, int lane,
+                                             uint32_t (&v)[16]) {
+    if (w == 0) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = 0;
+        return;
+    }
+    const uint32_t bit = (uint32_t)(lane >> 1) * w, j = bit >> 5, sh = bit & 31;
+    const uint32_t mask = w >= 32 ? 0xFFFFFFFFu : ((1u << w) - 1);
+    const uint4 *lo = (const uint4 *)(wd + j * 32 + (lane & 1) * 16);
+    const uint4 *hi = (const uint4 *)(wd + (j + 1) * 32 + (lane & 1) * 16);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const uint4 a = lo[c], b = hi[c];
+        v[4 * c + 0] = __builtin_amdgcn_alignbit(b.x, a.x, sh) & mask;
+        v[4 * c + 1] = __builtin_amdgcn_alignbit(b.y, a.y, sh) & mask;
+        v[4 * c + 2] = __builtin_amdgcn_alignbit(b.z, a.z, sh) & mask;
+        v[4 * c + 3] = __builtin_amdgcn_alignbit(b.w, a.w, sh) & mask;
+    }
+}
+
+// The same for the 16-lane layout (32 < w <= 64, src/bitpack.rs:136-161): lane
+// L gets row r = L, i.e. FastLanes lanes i of row r: 64-bit row-words
+// j = (r w) >> 6 and j + 1 (read only when the value straddles, so never past
+// the 16 w packed words) at the lane-wide shift (r w) & 63.
+__device__ __forceinline__ void unpack16_u64(const uint32_t *wd, uint32_t w, int lane,
+                                             uint64_t (&v)[16]) {
+    const uint64_t *w64 = (const uint64_t *)wd;
+    const uint32_t bit = (uint32_t)lane * w, j = bit >> 6, sh = bit & 63;
+    const uint64_t mask = w >= 64 ? ~0ull : ((1ull << w) - 1);
+    const bool straddle = sh + w > 64;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint64_t lo = w64[j * 16 + i];
+        const uint64_t hi = straddle ? w64[(j + 1) * 16 + i] : 0;
+        v[i] = (sh ? (lo >> sh) | (hi << (64 - sh)) : lo) & mask;
+    }
+}
+
+// Values y[i] (value k = 16 lane + i) of a stream without nulls to
+// out[base + k]: 16-byte stores when the lane's 16 values are all in the tile
+// and the destination is aligned, element stores otherwise.
+template <int F>
+__device__ __forceinline__ void emit_dense16_f(void *out, uint64_t base, uint32_t cnt,
+                                               const uint64_t (&y)[16], int lane) {
+    const uint32_t k0 = 16u * (uint32_t)lane;
+    if (F == QA_OUT_U32 || F == QA_OUT_F32) {
+        uint32_t *o = (uint32_t *)out + base + k0;
+        if (k0 + 16 <= cnt && ((uintptr_t)o & 15) == 0) {
+            uint32_t u[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) u[i] = F == QA_OUT_F32 ? qa_lane_to_f32(y[i]) : (uint32_t)y[i];
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                ((uint4 *)o)[c] = make_uint4(u[4 * c], u[4 * c + 1], u[4 * c + 2], u[4 * c + 3]);
+            return;
+        }
+    } else if (F == QA_OUT_U64) {
+        uint64_t *o = (uint64_t *)out + base + k0;
+        if (k0 + 16 <= cnt && ((uintptr_t)o & 15) == 0) {
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                ((uint4 *)o)[c] = make_uint4((uint32_t)y[2 * c], (uint32_t)(y[2 * c] >> 32),
+                                             (uint32_t)y[2 * c + 1], (uint32_t)(y[2 * c + 1] >> 32));
+            return;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        if (k0 + i < cnt) st_out<F>(out, base + k0 + i, y[i]);
+}
+__device__ __forceinline__ void emit_dense16(void *out, uint32_t oform, uint64_t base, uint32_t cnt,
+                                             const uint64_t (&y)[16], int lane) {
+    switch (oform) {
+    case QA_OUT_U8: emit_dense16_f<QA_OUT_U8>(out, base, cnt, y, lane); break;
+    case QA_OUT_U16: emit_dense16_f<QA_OUT_U16>(out, base, cnt, y, lane); break;
+    case QA_OUT_U32: emit_dense16_f<QA_OUT_U32>(out, base, cnt, y, lane); break;
+    case QA_OUT_F32: emit_dense16_f<QA_OUT_F32>(out, base, cnt, y, lane); break;
+    default: emit_dense16_f<QA_OUT_U64>(out, base, cnt, y, lane); break;
     }
 }
 

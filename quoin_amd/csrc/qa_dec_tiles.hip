@@ -78,7 +78,7 @@ __global__ void k_dec_desc(DecArgs a) {
         d.out = col.out;
         d.bitmap = col.bitmap;
         d.off = ts.off;
-        d.tail = ts.tail;
+        d.tail = ts.tail;  // DeltaBitpack: replaced by the carry (k_dec_delta_sums/scan)
         d.payload = fr.payload_off;
         d.p0 = (uint64_t)lt * QA_TILE;
         d.base = fr.out_off + d.p0;
@@ -89,6 +89,7 @@ __global__ void k_dec_desc(DecArgs a) {
         d.dtype = col.dtype;
         d.oform = (uint32_t)out_form(col);
         d.nulls = st.nv != col.n;
+        d.fl = tile_kind(d.mode, d.oform);
         if (d.nulls) {
             // logical positions [lpos, next tile's lpos in the same stream, or n)
             d.L0 = ts.lpos;
@@ -103,13 +104,14 @@ __global__ void k_dec_desc(DecArgs a) {
 }
 
 __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
-    __shared__ uint64_t lds[4][QA_TILE];
-    if (*a.err) return;
+    __shared__ __align__(16) uint64_t lds[4][QA_TILE];
+    // (an error flagged before k_dec_desc left every descriptor empty; no
+    // kernel between it and this one flags errors)
     const int lane = threadIdx.x & 63, wv = qa_wave_id();
     const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.t1) return;
     const TileDec td = a.tdec[t];
-    if (!td.mode) return;
+    if (!td.mode || td.fl) return;
     const uint32_t f = td.frame;
     const bool nulls = td.nulls;
     const uint32_t cnt = td.cnt;
@@ -346,10 +348,147 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
 
 static inline unsigned nbt(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
-void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
+
+// ---------------------------------------------------------------------------
+// FoR / DeltaBitpack tiles into 4- or 8-byte integers, the bulk of integer
+// columns (tile_kind in qa_kernels.h: k_dec_fl<4> takes FoR into 4-byte
+// integers, k_dec_fl<8> FoR and Delta into 8-byte integers and Delta into
+// 4-byte ones). Values are unpacked lane-contiguously (k = 16 lane + i: runs of
+// staged row-words and a funnel shift each), Delta is an in-lane prefix plus
+// one wavefront scan, and the values go back through LDS to k = lane + 64 q so
+// every store writes whole cache lines. Into 4-byte integers only the low 32
+// bits of a FoR value matter, so k_dec_fl<4> computes in 32 bits with 4.25 KiB
+// of LDS per wavefront: up to 32 wavefronts per CU keep more tiles' loads in
+// flight. (FoR of a 4-byte column has w <= 32; a stream with a wider sub-block
+// still decodes, by direct reads.)
+// ---------------------------------------------------------------------------
+template <int OW>
+__global__ void __launch_bounds__(256) k_dec_fl(DecArgs a) {
+    // staged words (32 w, plus a row-word read past the last one) and the
+    // transposed values
+    constexpr int LW = OW == 4 ? 1088 : 2 * QA_TILE;
+    __shared__ __align__(16) uint32_t lds[4][LW];
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
+    if (t >= a.t1) return;
+    const TileDec td = a.tdec[t];
+    if (td.fl != OW) return;
+    const uint32_t cnt = td.cnt, w = td.w;
+    uint32_t *wd = lds[wv];
+    const uint8_t *h = td.src + td.off;
+    const uint64_t mn = qa_ld64u(h + 1);
+    const uint32_t k0 = 16u * (uint32_t)lane;
+    if (OW == 4) {
+        uint32_t y[16];
+        if (w <= 32) {
+            stage_words(h + 9, 128 * w, wd, lane);
+            wave_lds_sync();
+            uint32_t v[16];
+            unpack16_u32(wd, w, lane, v);
+#pragma unroll
+            for (int i = 0; i < 16; i++) y[i] = (uint32_t)mn + v[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; i++) y[i] = k0 + i < cnt ? (uint32_t)for_value_direct(h, k0 + i) : 0u;
+        }
+        wave_lds_sync();  // every lane is done with the staged words
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            ((uint4 *)(wd + k0))[c] = make_uint4(y[4 * c], y[4 * c + 1], y[4 * c + 2], y[4 * c + 3]);
+    } else {
+        stage_words(h + 9, 128 * w, wd, lane);
+        wave_lds_sync();
+        uint64_t y[16];
+        if (w <= 32) {
+            uint32_t v[16];
+            unpack16_u32(wd, w, lane, v);
+#pragma unroll
+            for (int i = 0; i < 16; i++) y[i] = mn + v[i];
+        } else {
+            uint64_t v[16];
+            unpack16_u64(wd, w, lane, v);
+#pragma unroll
+            for (int i = 0; i < 16; i++) y[i] = mn + v[i];
+        }
+        if (td.mode == QA_M_DELTA) {
+            // wrapping prefix sum of the unzigzagged deltas: in-lane over the 16
+            // values, then one wavefront scan of the lane totals, seeded with
+            // the tile carry (delta_bitpack.rs:48-64)
+            uint64_t run = 0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                run += k0 + i < cnt ? qa_unzigzag(y[i]) : 0;
+                y[i] = run;
+            }
+            const uint64_t inc = qa_wave_scan_add64(run);
+            const uint64_t off = td.tail + inc - run;
+#pragma unroll
+            for (int i = 0; i < 16; i++) y[i] += off;
+        }
+        wave_lds_sync();  // every lane is done with the staged words
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            ((uint4 *)(wd + 2 * k0))[c] = make_uint4((uint32_t)y[2 * c], (uint32_t)(y[2 * c] >> 32),
+                                                     (uint32_t)y[2 * c + 1], (uint32_t)(y[2 * c + 1] >> 32));
+    }
+    wave_lds_sync();
+    const bool o32 = OW == 4 || td.oform == QA_OUT_U32;
+    if (!td.nulls) {
+        if (OW == 4) {
+            uint32_t *o = (uint32_t *)td.out + td.base + lane;
+#pragma unroll
+            for (int q = 0; q < QV; q++)
+                if (lane + WAVE * q < cnt) o[WAVE * q] = wd[lane + WAVE * q];
+        } else if (o32) {
+            uint32_t *o = (uint32_t *)td.out + td.base + lane;
+            const uint64_t *vv = (const uint64_t *)wd;
+#pragma unroll
+            for (int q = 0; q < QV; q++)
+                if (lane + WAVE * q < cnt) o[WAVE * q] = (uint32_t)vv[lane + WAVE * q];
+        } else {
+            uint64_t *o = (uint64_t *)td.out + td.base + lane;
+            const uint64_t *vv = (const uint64_t *)wd;
+#pragma unroll
+            for (int q = 0; q < QV; q++)
+                if (lane + WAVE * q < cnt) o[WAVE * q] = vv[lane + WAVE * q];
+        }
+        return;
+    }
+    if (OW == 4) emit_gather_t<QA_OUT_U32>(td.out, td.bitmap, td.L0, td.L1, wd, cnt, lane);
+    else if (o32)
+        emit_gather_t<QA_OUT_U32>(td.out, td.bitmap, td.L0, td.L1, (const uint64_t *)wd, cnt, lane);
+    else emit_gather_t<QA_OUT_U64>(td.out, td.bitmap, td.L0, td.L1, (const uint64_t *)wd, cnt, lane);
+}
+
+void qa_dec_desc_launch(const DecArgs &a, hipStream_t s) {
     if (a.t1 <= a.t0) return;
     hipLaunchKernelGGL(k_dec_desc, dim3(nbt(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
+}
+
+void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
+    if (a.t1 <= a.t0) return;
     if (a.kev0) hipEventRecord((hipEvent_t)a.kev0, s);
-    hipLaunchKernelGGL(k_dec_tiles, dim3(nbt(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
+    // each kernel over the runs of frames of its kind (host side), or every
+    // kernel over the whole range, each taking its own tiles
+    auto launch = [&](uint32_t kind, uint64_t t0, uint64_t t1) {
+        DecArgs b = a;
+        b.t0 = t0;
+        b.t1 = t1;
+        const dim3 g(nbt(t1 - t0, 4));
+        if (kind == 4) hipLaunchKernelGGL(k_dec_fl<4>, g, dim3(256), 0, s, b);
+        else if (kind == 8) hipLaunchKernelGGL(k_dec_fl<8>, g, dim3(256), 0, s, b);
+        else hipLaunchKernelGGL(k_dec_tiles, g, dim3(256), 0, s, b);
+    };
+    if (a.runs) {
+        for (uint32_t r = 0; r < a.nruns; r++) {
+            const uint64_t t0 = a.runs[r].t0 > a.t0 ? a.runs[r].t0 : a.t0;
+            const uint64_t t1 = a.runs[r].t1 < a.t1 ? a.runs[r].t1 : a.t1;
+            if (t1 > t0) launch(a.runs[r].kind, t0, t1);
+        }
+    } else {
+        launch(4, a.t0, a.t1);
+        launch(8, a.t0, a.t1);
+        launch(0, a.t0, a.t1);
+    }
     if (a.kev1) hipEventRecord((hipEvent_t)a.kev1, s);
 }

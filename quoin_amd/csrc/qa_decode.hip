@@ -276,29 +276,34 @@ __device__ __forceinline__ T wsum(T v) {
     return v;
 }
 
+// per DeltaBitpack tile: the wrapping sum of its unzigzagged deltas, into the
+// tile descriptor's tail (k_dec_delta_scan turns the sums into carries)
 __global__ void __launch_bounds__(256) k_dec_delta_sums(DecArgs a) {
-    __shared__ uint32_t wbuf[4][2 * QA_TILE + 4];
-    if (*a.err) return;
+    __shared__ __align__(16) uint32_t wbuf[4][2 * QA_TILE + 4];
     const int lane = threadIdx.x & 63, wv = qa_wave_id();
     uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.t1) return;
-    DTileD &ts = a.tiles[t];
-    const uint32_t f = ts.frame;
-    const DFrame &fr = a.frames[f];
-    if (fr.mode != QA_M_DELTA || !frame_used(a, f)) return;
-    const DDecCol &col = a.cols[fr.col];
-    uint32_t lt = (uint32_t)(t - fr.tile_base);
-    uint32_t cnt = min((uint32_t)QA_TILE, fr.n - lt * QA_TILE);
-    const uint8_t *h = col.src + ts.off;
-    const uint32_t w = ts.w;
+    TileDec &td = a.tdec[t];
+    if (td.mode != QA_M_DELTA) return;  // also every tile once an error is flagged
+    const uint32_t cnt = td.cnt;
+    const uint8_t *h = td.src + td.off;
+    const uint32_t w = td.w;
     uint64_t mn = qa_ld64u(h + 1);
     uint32_t *wd = wbuf[wv];
     stage_words(h + 9, 128 * w, wd, lane);
     wave_lds_sync();
     uint64_t s = 0;
-    for (uint32_t k = lane; k < cnt; k += WAVE) s += qa_unzigzag(mn + unpack_lds(wd, w, k));
+    if (w <= 32) {
+        uint32_t v[16];
+        unpack16_u32(wd, w, lane, v);
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            s += 16u * (uint32_t)lane + i < cnt ? qa_unzigzag(mn + v[i]) : 0;
+    } else {
+        for (uint32_t k = lane; k < cnt; k += WAVE) s += qa_unzigzag(mn + unpack_lds(wd, w, k));
+    }
     s = wsum(s);
-    if (lane == 0) ts.tail = s;
+    if (lane == 0) td.tail = s;
 }
 
 // one wavefront per frame: exclusive scan of the tile sums, seeded with the base
@@ -311,9 +316,9 @@ __global__ void __launch_bounds__(256) k_dec_delta_scan(DecArgs a) {
     uint64_t carry = qa_ld64u(a.cols[fr.col].src + fr.payload_off);  // base
     for (uint32_t t0 = 0; t0 < fr.ntiles; t0 += WAVE) {
         uint32_t t = t0 + lane;
-        uint64_t s = t < fr.ntiles ? a.tiles[fr.tile_base + t].tail : 0;
+        uint64_t s = t < fr.ntiles ? a.tdec[fr.tile_base + t].tail : 0;
         const uint64_t inc = qa_wave_scan_add64(s);
-        if (t < fr.ntiles) a.tiles[fr.tile_base + t].tail = carry + inc - s;
+        if (t < fr.ntiles) a.tdec[fr.tile_base + t].tail = carry + inc - s;
         carry += __shfl(inc, 63, 64);
     }
 }
@@ -382,6 +387,7 @@ void qa_decode_lpos_launch(const DecArgs &a, hipStream_t s) {
 // per frame [f0, f1) / tile [t0, t1): Delta carries, tile decode
 void qa_decode_values_launch(const DecArgs &a, hipStream_t s) {
     if (a.t1 <= a.t0) return;
+    qa_dec_desc_launch(a, s);
     QA_LAUNCH(k_dec_delta_sums, dim3(nbk(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
     QA_LAUNCH(k_dec_delta_scan, dim3(nbk(a.f1 - a.f0, 4)), dim3(256), 0, s, a);
     qa_dec_tiles_launch(a, s);

@@ -582,6 +582,23 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     a.f1 = (uint32_t)frames.size();
     a.t0 = 0;
     a.t1 = tile_base;
+    // runs of consecutive tiles taken by one tile decoder, so that each is
+    // launched over its own tiles only (beyond 32 runs every decoder is
+    // launched over all tiles and takes its own)
+    std::vector<TileRun> runs;
+    for (const DFrame &f : frames) {
+        if (!f.ntiles) continue;
+        const uint32_t kind =
+            tile_kind(f.mode, qa_out_form(cols[f.col].dtype, cols[f.col].out_kind));
+        if (!runs.empty() && runs.back().kind == kind && runs.back().t1 == f.tile_base)
+            runs.back().t1 = f.tile_base + f.ntiles;
+        else
+            runs.push_back(TileRun{f.tile_base, f.tile_base + f.ntiles, kind, 0});
+    }
+    if (runs.size() <= 32) {
+        a.runs = runs.data();
+        a.nruns = (uint32_t)runs.size();
+    }
     hipEventRecord(g->ev0, s);
     zero_async(g->derr.p, 8, s);
     zero_async(g->dstate.p, ns * sizeof(DDecState), s);

@@ -128,7 +128,8 @@ struct TileDec {
     uint64_t p0;              // index of the tile's first value within its frame
     uint64_t L0, L1;          // streams with nulls: logical range of the tile
     uint32_t frame, cnt, w, mode;  // mode 0: nothing to do
-    uint32_t dtype, oform, nulls, pad;
+    uint32_t dtype, oform, nulls;
+    uint32_t fl;  // FoR / Delta, w <= 32, into 4 or 8-byte integers: k_dec_fl<fl>
 };
 
 struct DDecCol {
@@ -186,6 +187,23 @@ struct DecArgs {
     uint64_t t0, t1;
     // host side: HIP events recorded around the tile decode kernel, or null
     void *kev0, *kev1;
+    // host side: runs of consecutive tiles of one tile_kind, or null
+    const struct TileRun *runs;
+    uint32_t nruns, pad2;
+};
+
+// Which tile decoder takes a tile (from its frame's mode and the output
+// form): 4 = k_dec_fl<4> (FoR into 4-byte integers), 8 = k_dec_fl<8> (FoR and
+// Delta into 8-byte integers, Delta into 4-byte ones), 0 = k_dec_tiles.
+__host__ __device__ inline uint32_t tile_kind(uint32_t mode, uint32_t oform) {
+    if (mode == QA_M_FOR && oform == QA_OUT_U32) return 4;
+    if ((mode == QA_M_FOR || mode == QA_M_DELTA) && oform == QA_OUT_U64) return 8;
+    if (mode == QA_M_DELTA && oform == QA_OUT_U32) return 8;
+    return 0;
+}
+struct TileRun {
+    uint64_t t0, t1;
+    uint32_t kind, pad;
 };
 
 void qa_decode_launch(const DecArgs &a, hipStream_t s);
@@ -195,6 +213,7 @@ void qa_decode_lpos_launch(const DecArgs &a, hipStream_t s);
 void qa_decode_values_launch(const DecArgs &a, hipStream_t s);
 void qa_decode_final_launch(const DecArgs &a, hipStream_t s);
 void qa_fast_dec_index_launch(const DecArgs &a, hipStream_t s);
+void qa_dec_desc_launch(const DecArgs &a, hipStream_t s);
 void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s);
 
 // rANS (src/entropy/rans.rs): per-stream descriptors of a batch
