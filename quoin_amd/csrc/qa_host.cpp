@@ -46,6 +46,16 @@ bool ensure(DevBuf &b, size_t bytes) {
 template <typename T>
 T *dp(DevBuf &b) { return static_cast<T *>(b.p); }
 
+void zero_async(void *p, size_t bytes, hipStream_t s);
+
+// a scan workspace: its first word (the scan's ticket) starts at zero
+bool ensure_scan(DevBuf &b, size_t bytes, hipStream_t s) {
+    const bool grow = b.cap < bytes || !b.p;
+    if (!ensure(b, bytes)) return false;
+    if (grow) zero_async(b.p, 8, s);
+    return true;
+}
+
 std::atomic<uint64_t> g_wins[64];
 thread_local int t_last_err = 0;
 
@@ -347,7 +357,7 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         !ensure(g->lane, lane_off * 8) || !ensure(g->vpre, (vword_base + 1) * 8) ||
         !ensure(g->tpre, (vword_base + 1) * 8) || !ensure(g->tpos, ntrans * 8) ||
         !ensure(g->rsz, (nruns_max + 1) * 8) || !ensure(g->colbase, (ncols + 2) * 8) ||
-        !ensure(g->scan_tmp, scan_words * 8) || !ensure(g->rfeat, region_base * 4ull) ||
+        !ensure_scan(g->scan_tmp, scan_words * 8, s) || !ensure(g->rfeat, region_base * 4ull) ||
         !ensure(g->owner, region_base * 4ull) || !ensure(g->blocks, region_base * sizeof(DBlock)) ||
         !ensure(g->tiles, tile_base * sizeof(DTile)) || !ensure(g->wins, 64 * 8) ||
         !ensure(g->rgrown, region_base * sizeof(DRegionG)) ||
@@ -544,7 +554,7 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         !ensure(g->dvpre, (vword_base + 1) * 8) || !ensure(g->derr, 8) ||
         !ensure(g->dbitmaps, (vword_base + 1) * 8) || !ensure(g->dvbpre, (vbyte_base + 1) * 8) ||
         !ensure(g->dvruns, (vbyte_base + 1) * 8) ||
-        !ensure(g->dscan, qa_scan_tmp_words(nscan) * 8) ||
+        !ensure_scan(g->dscan, qa_scan_tmp_words(nscan) * 8, s) ||
         (any_fast && (!ensure(g->dfx, frames.size() * sizeof(DFrameX)) ||
                       !ensure(g->dscr, scr_base * 8) || !ensure(g->dscr2, scr_base * 8))))
         return QUOIN_ERR_DEVICE;
