@@ -175,56 +175,38 @@ __device__ __forceinline__ void emit_dense(void *out, uint32_t oform, uint64_t b
     }
 }
 
-// Streams with nulls (src/validity.rs:130-146): the tile's cnt compact values
-// vv[0..cnt) fill the valid slots of logical positions [L0, L1), null slots get
-// zero. Lanes load up to 64 bitmap words of the range at once and scan their
-// popcounts; the wavefront then stores one word's 64 positions per step.
+// Values vv[0..cnt) of a stream with nulls go to the set bits of the bitmap
+// in [L0, L1) in order; the other positions get zero (src/validity.rs:130-146).
+// The wavefront walks the 64-position windows of bitmap words (L0 rounded
+// down), so every store is one aligned run of 64 outputs: lane i loads word i
+// of the range once (64 windows per load), a window is one readlane (uniform)
+// with the positions outside [L0, L1) masked off, each lane's rank inside it
+// is one mbcnt, and the count before the window carries in a scalar register.
+// A rank past cnt (a bitmap with more set bits than values) stores zero.
 template <int F, typename V>
 __device__ __forceinline__ void emit_gather_t(void *out, const uint64_t *bm, uint64_t L0,
                                               uint64_t L1, const V *vv, uint32_t cnt, int lane) {
     if (L1 <= L0) return;
-    const uint64_t jb = L0 >> 6, je = (L1 - 1) >> 6;  // inclusive word range
-    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    uint64_t r = 0;  // valid values before the chunk
-    for (uint64_t c0 = jb; c0 <= je; c0 += WAVE) {
-        const uint64_t j = c0 + lane;
-        uint64_t m = 0;
-        if (j <= je) {
-            m = bm[j];
-            const uint64_t p = j << 6;
-            if (L0 > p) m &= ~0ull << (L0 - p);
-            if (L1 < p + 64) m &= (1ull << (L1 - p)) - 1;
+    const uint64_t w0 = L0 >> 6, w1 = (L1 - 1) >> 6;  // inclusive word range
+    uint32_t R = 0;  // values before the window
+    for (uint64_t c0 = w0; c0 <= w1; c0 += WAVE) {
+        const uint64_t wi = c0 + (uint64_t)lane;
+        const uint64_t myw = bm[wi < w1 ? wi : w1];
+        const uint32_t nwin = (uint32_t)(w1 - c0 + 1 < (uint64_t)WAVE ? w1 - c0 + 1 : WAVE);
+        for (uint32_t q = 0; q < nwin; q++) {
+            const uint64_t p = (c0 + q) << 6;
+            uint64_t W = qa_readlane64(myw, (int)q);
+            if (p < L0) W &= ~0ull << (L0 - p);
+            if (L1 - p < 64) W &= (1ull << (L1 - p)) - 1;
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(W >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)W, 0u));
+            const uint32_t idx = R + below;
+            const bool set = (W >> lane) & 1;
+            const uint64_t x = set && idx < cnt ? (uint64_t)vv[idx] : 0;
+            const uint64_t pos = p + (uint64_t)lane;
+            if (pos >= L0 && pos < L1) st_out<F>(out, pos, x);
+            R += (uint32_t)__popcll(W);
         }
-        const uint64_t pc = (uint64_t)__popcll(m);
-        const uint64_t inc = qa_wave_scan_add64(pc);
-        const uint64_t ex = inc - pc;
-        const uint32_t nw = (uint32_t)(je - c0 + 1 < (uint64_t)WAVE ? je - c0 + 1 : (uint64_t)WAVE);
-        // four words per step so their LDS reads overlap; word fields are
-        // uniform (scalar reads)
-        for (uint32_t i0 = 0; i0 < nw; i0 += 4) {
-            uint64_t xv[4], pv[4];
-            bool sv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t i = i0 + u;
-                sv[u] = false;
-                xv[u] = 0;
-                pv[u] = 0;
-                if (i < nw) {
-                    const uint64_t mi = qa_readlane64(m, (int)i);
-                    const uint32_t idx = (uint32_t)(r + qa_readlane64(ex, (int)i)) +
-                                         (uint32_t)__popcll(mi & below);
-                    const uint64_t pos = ((c0 + i) << 6) + lane;
-                    xv[u] = ((mi >> lane) & 1) && idx < cnt ? vv[idx] : 0;
-                    pv[u] = pos;
-                    sv[u] = pos >= L0 && pos < L1;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (sv[u]) st_out<F>(out, pv[u], xv[u]);
-        }
-        r += qa_readlane64(inc, 63);
     }
 }
 template <int F>
