@@ -1069,6 +1069,7 @@ __global__ void k_tile_desc(EncArgs a) {
         } else if (mode == QA_M_FOR || mode == QA_M_DELTA) {
             e.mode = (uint8_t)mode;
             e.w = (uint8_t)ts.w;
+            e.fast = ts.w <= 32 ? 1u : 0u;
             e.dst = p + ts.off;
             if (mode == QA_M_DELTA) {
                 uint64_t dmx;
@@ -1079,29 +1080,113 @@ __global__ void k_tile_desc(EncArgs a) {
         }
     }
     a.temit[t] = e;
+    // the tiles k_emit_fl does not take go on k_emit_tiles' list (one atomic
+    // per wavefront)
+    const bool gen = e.mode && !e.fast;
+    const uint64_t m = __ballot(gen);
+    if (m) {
+        const int lane = threadIdx.x & 63, lead = __ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == lead) base = atomicAdd(a.tlist, (uint32_t)__popcll(m));
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, lead);
+        const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (gen) a.tlist[16 + base + r] = (uint32_t)t;
+    }
 }
 
-// One wavefront per tile, four independent tiles per workgroup (LDS is
-// per-wavefront, so only wavefront-level synchronization is used). FastLanes
-// pack (src/bitpack.rs:47-133) straight from the lane: in the u32 form lane l
-// of the 32 packs rows [16 h, 16 h + 16) (h = half of the wavefront), in the
-// u64 form lane l of the 16 packs rows [16 q, 16 q + 16) (q = quarter); each
-// residual is ORed into its one or two LDS words. The body is instantiated
-// per lane storage kind so the loads are straight-line code.
-__global__ void __launch_bounds__(256) k_emit_tiles(EncArgs a) {
-    __shared__ uint64_t outs[4][QA_TILE + 1];
+// FoR / DeltaBitpack tiles with w <= 32 (the 32-lane FastLanes form,
+// src/bitpack.rs:47-75): lane l of the 32 packs rows [16 h, 16 h + 16) (h = half
+// of the wavefront), each residual ORed into its one or two LDS words, then
+// the 128 w bytes go out. 4 KiB of LDS per wavefront, so more tiles are in
+// flight per CU than in the general kernel. Instantiated per lane storage kind
+// so the loads are straight-line code.
+template <int LK>
+__device__ __forceinline__ void emit_fl_t(const TileEmit &e, uint32_t *outw, int lane) {
+    const void *lp = e.lp;
+    const uint32_t lk = e.lk, cnt = e.cnt;
+    const uint64_t k0 = e.k0;
+    const bool delta = e.mode == QA_M_DELTA;
+    const uint64_t mn = e.mn;
+    const uint32_t w = e.w;
+    uint8_t *q = e.dst;
+    if (lane == 0) q[0] = (uint8_t)w;
+    if (lane >= 1 && lane < 9) q[lane] = (uint8_t)(mn >> (8 * (lane - 1)));
+    if (w == 0) return;
+    for (uint32_t i = lane; i < 32 * w; i += WAVE) outw[i] = 0;
+    wave_lds_sync();
+    // residual of tile value k: FoR x - min; Delta zigzag(x - prev) - min with
+    // the block's first value as its own predecessor (delta_bitpack.rs:23-46);
+    // zero past cnt (padding to 1024, for_bitpack.rs:60-72)
+    const uint32_t l = lane & 31, r0 = 16 * (lane >> 5);
+    uint32_t res[16];
+    {
+        uint64_t xs[16], ps[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t k = (r0 + i) * 32 + l;
+            xs[i] = k < cnt ? lane_get_t<LK>(lp, lk, k0 + k) : 0;
+            ps[i] = (delta && k < cnt && (!e.first || k != 0)) ? lane_get_t<LK>(lp, lk, k0 + k - 1)
+                                                               : xs[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t k = (r0 + i) * 32 + l;
+            res[i] = k >= cnt ? 0u : (uint32_t)(delta ? qa_zigzag(xs[i] - ps[i]) - mn : xs[i] - mn);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t bit = (r0 + i) * w, word = bit >> 5, sh = bit & 31;
+        atomicOr(&outw[word * 32 + l], res[i] << sh);
+        if (sh + w > 32) atomicOr(&outw[(word + 1) * 32 + l], res[i] >> (32 - sh));
+    }
+    wave_lds_sync();
+    wave_store_bytes(q + 9, outw, 128 * w, lane);
+}
+
+__global__ void __launch_bounds__(256) k_emit_fl(EncArgs a) {
+    __shared__ __align__(16) uint32_t outs[4][QA_TILE];
     const int lane = threadIdx.x & 63, wv = qa_wave_id();
     const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.t1) return;
     const TileEmit e = a.temit[t];
-    if (!e.mode) return;
+    if (!e.fast) return;
     switch (e.lk) {
-    case QA_LK_U64: emit_tile_t<QA_LK_U64>(e, outs[wv], lane); break;
-    case QA_LK_U32: emit_tile_t<QA_LK_U32>(e, outs[wv], lane); break;
-    case QA_LK_I32: emit_tile_t<QA_LK_I32>(e, outs[wv], lane); break;
-    case QA_LK_F32: emit_tile_t<QA_LK_F32>(e, outs[wv], lane); break;
-    default: emit_tile_t<QA_LK_ANY>(e, outs[wv], lane);
+    case QA_LK_U64: emit_fl_t<QA_LK_U64>(e, outs[wv], lane); break;
+    case QA_LK_U32: emit_fl_t<QA_LK_U32>(e, outs[wv], lane); break;
+    case QA_LK_I32: emit_fl_t<QA_LK_I32>(e, outs[wv], lane); break;
+    case QA_LK_F32: emit_fl_t<QA_LK_F32>(e, outs[wv], lane); break;
+    default: emit_fl_t<QA_LK_ANY>(e, outs[wv], lane);
     }
+}
+
+// The other tiles (RAW, and FoR / Delta with w > 32), from the list k_tile_desc
+// builds: persistent wavefronts, four per workgroup (LDS is per-wavefront, so
+// only wavefront-level synchronization is used). FastLanes pack
+// (src/bitpack.rs:47-133) straight from the lane: in the u32 form lane l of the
+// 32 packs rows [16 h, 16 h + 16) (h = half of the wavefront), in the u64 form
+// lane l of the 16 packs rows [16 q, 16 q + 16) (q = quarter); each residual is
+// ORed into its one or two LDS words.
+__global__ void __launch_bounds__(256) k_emit_tiles(EncArgs a) {
+    __shared__ uint64_t outs[4][QA_TILE + 1];
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
+    const uint32_t n = *a.tlist;
+    for (uint32_t i = blockIdx.x * 4 + wv; i < n; i += gridDim.x * 4) {
+        const TileEmit e = a.temit[a.tlist[16 + i]];
+        switch (e.lk) {
+        case QA_LK_U64: emit_tile_t<QA_LK_U64>(e, outs[wv], lane); break;
+        case QA_LK_U32: emit_tile_t<QA_LK_U32>(e, outs[wv], lane); break;
+        case QA_LK_I32: emit_tile_t<QA_LK_I32>(e, outs[wv], lane); break;
+        case QA_LK_F32: emit_tile_t<QA_LK_F32>(e, outs[wv], lane); break;
+        default: emit_tile_t<QA_LK_ANY>(e, outs[wv], lane);
+        }
+        wave_lds_sync();  // the next tile reuses the wavefront's LDS
+    }
+}
+
+__global__ void k_tlist_clear(EncArgs a) {
+    if (threadIdx.x == 0) a.tlist[0] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1157,9 +1242,15 @@ void qa_encode_layout_launch(const EncArgs &a, hipStream_t s) {
 // per-block stages over [r0, r1) / [t0, t1): frame headers and payloads
 void qa_encode_emit_launch(const EncArgs &a, hipStream_t s) {
     QA_LAUNCH(k_emit_frames, dim3(nb(a.r1 - a.r0, 64)), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_tlist_clear, dim3(1), dim3(64), 0, s, a);
     QA_LAUNCH(k_tile_desc, dim3(nb(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
     if (a.kev0) hipEventRecord((hipEvent_t)a.kev0, s);
-    QA_LAUNCH(k_emit_tiles, dim3(nb(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
+    QA_LAUNCH(k_emit_fl, dim3(nb(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
+    {
+        // persistent over the list: at most four groups per CU
+        const unsigned g = nb(a.t1 - a.t0, 4);
+        QA_LAUNCH(k_emit_tiles, dim3(g < 1024 ? g : 1024), dim3(256), 0, s, a);
+    }
     if (a.kev1) hipEventRecord((hipEvent_t)a.kev1, s);
     qa_fast_emit_launch(a, s);
 }

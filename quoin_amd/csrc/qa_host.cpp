@@ -69,7 +69,7 @@ struct QuoinGpu {
     std::mutex mu;
     // encode arenas
     DevBuf cols, cst, lane, vpre, tpre, tpos, rsz, colbase, scan_tmp, rfeat, owner, blocks, tiles,
-        wins, tilesf, blocksx, dsorted, rstart, hkeys, hcnt, rgrown, gsets, temit, dpart, dparts;
+        wins, tilesf, blocksx, dsorted, rstart, hkeys, hcnt, rgrown, gsets, temit, tlist, dpart, dparts;
     // decode arenas
     DevBuf dcols, dstate, frames, dtiles, dtdec, dvpre, derr, dbitmaps, dvbpre, dvruns, dscan,
         dfx, dscr, dscr2;
@@ -274,7 +274,7 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
                      &g->tiles, &g->wins,     &g->dcols,  &g->frames,  &g->dtiles,  &g->dtdec,
                      &g->dvpre, &g->derr,     &g->dbitmaps, &g->dscan, &g->st_in,   &g->st_valid,
                      &g->dstate, &g->dvbpre, &g->dvruns, &g->tilesf, &g->blocksx, &g->dsorted,
-                     &g->rstart, &g->hkeys, &g->hcnt, &g->rgrown, &g->gsets, &g->temit, &g->dpart, &g->dparts, &g->dfx,
+                     &g->rstart, &g->hkeys, &g->hcnt, &g->rgrown, &g->gsets, &g->temit, &g->tlist, &g->dpart, &g->dparts, &g->dfx,
                      &g->dscr, &g->dscr2,
                      &g->st_out, &g->st_vout};
     for (DevBuf *b : all)
@@ -361,7 +361,8 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         !ensure(g->owner, region_base * 4ull) || !ensure(g->blocks, region_base * sizeof(DBlock)) ||
         !ensure(g->tiles, tile_base * sizeof(DTile)) || !ensure(g->wins, 64 * 8) ||
         !ensure(g->rgrown, region_base * sizeof(DRegionG)) ||
-        !ensure(g->gsets, region_base * 2048ull) || !ensure(g->temit, tile_base * sizeof(TileEmit)))
+        !ensure(g->gsets, region_base * 2048ull) || !ensure(g->temit, tile_base * sizeof(TileEmit)) ||
+        !ensure(g->tlist, (tile_base + 16) * 4))
         return QUOIN_ERR_DEVICE;
     EncArgs a{};
     a.cols = dp<DCol>(g->cols);
@@ -386,6 +387,7 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     a.wins = dp<unsigned long long>(g->wins);
     a.rgrown = dp<DRegionG>(g->rgrown);
     a.temit = dp<TileEmit>(g->temit);
+    a.tlist = dp<uint32_t>(g->tlist);
     a.gsets = dp<uint64_t>(g->gsets);
     a.r0 = 0;
     a.r1 = region_base;

@@ -25,6 +25,8 @@ struct TileEmit {
     uint8_t first;    // the first tile of its block (Delta: own predecessor)
     uint32_t dtype;
     uint32_t bmode;   // mode of the block owning this tile slot (0: none, or failed column)
+    uint32_t fast;    // FoR / Delta with w <= 32: written by k_emit_fl
+    uint32_t pad;
 };
 
 // Device pointers of one encode batch (all arrays sized by the host from upper
@@ -60,6 +62,7 @@ struct EncArgs {
     uint64_t *hkeys;       // 4 per lane value slot: global hash fallback keys
     uint32_t *hcnt;        // 4 per lane value slot: their counts
     TileEmit *temit;       // per tile slot
+    uint32_t *tlist;       // [0]: count, [16 + i]: the tiles k_emit_tiles writes
     uint32_t *dpart;       // per region slot: Dict part set size (Level::Fast)
     uint64_t *dparts;      // per region slot: DH_LIMIT keys of its part set
     DRegionG *rgrown;      // per region slot
