@@ -84,6 +84,23 @@ __device__ __forceinline__ void stage_words(const uint8_t *src, uint32_t nbytes,
         }
     }
 }
+// FoR sub-block value k at stream sub-block header h (direct global reads;
+// sub-block offsets are validated by the index walk), src/for_bitpack.rs:108-157
+__device__ __forceinline__ uint64_t for_value_direct(const uint8_t *h, uint32_t k) {
+    uint32_t w = h[0];
+    uint64_t mn = qa_ld64u(h + 1);
+    if (w == 0) return mn;
+    if (w <= 32) {
+        uint32_t r = k >> 5, l = k & 31, bit = r * w, word = bit >> 5, sh = bit & 31;
+        uint32_t v = qa_ld32u(h + 9 + 4ull * (word * 32 + l)) >> sh;
+        if (sh + w > 32) v |= qa_ld32u(h + 9 + 4ull * ((word + 1) * 32 + l)) << (32 - sh);
+        return mn + (v & (w >= 32 ? 0xFFFFFFFFu : ((1u << w) - 1)));
+    }
+    uint32_t r = k >> 4, l = k & 15, bit = r * w, word = bit >> 6, sh = bit & 63;
+    uint64_t v = qa_ld64u(h + 9 + 8ull * (word * 16 + l)) >> sh;
+    if (sh + w > 64) v |= qa_ld64u(h + 9 + 8ull * ((word + 1) * 16 + l)) << (64 - sh);
+    return mn + (v & (w >= 64 ? ~0ull : ((1ull << w) - 1)));
+}
 __device__ __forceinline__ uint64_t unpack_lds(const uint32_t *wd, uint32_t w, uint32_t k) {
     if (w == 0) return 0;
     if (w <= 32) {

@@ -32,23 +32,6 @@ __constant__ double d_fm_scales[7] = {10.0, 100.0, 1000.0, 10000.0, 100000.0, 10
 
 #define QV (QA_TILE / WAVE)
 
-// FoR sub-block value k at stream sub-block header h (direct global reads;
-// sub-block offsets are validated by the index walk)
-__device__ __forceinline__ uint64_t for_value_direct(const uint8_t *h, uint32_t k) {
-    uint32_t w = h[0];
-    uint64_t mn = qa_ld64u(h + 1);
-    if (w == 0) return mn;
-    if (w <= 32) {
-        uint32_t r = k >> 5, l = k & 31, bit = r * w, word = bit >> 5, sh = bit & 31;
-        uint32_t v = qa_ld32u(h + 9 + 4ull * (word * 32 + l)) >> sh;
-        if (sh + w > 32) v |= qa_ld32u(h + 9 + 4ull * ((word + 1) * 32 + l)) << (32 - sh);
-        return mn + (v & (w >= 32 ? 0xFFFFFFFFu : ((1u << w) - 1)));
-    }
-    uint32_t r = k >> 4, l = k & 15, bit = r * w, word = bit >> 6, sh = bit & 63;
-    uint64_t v = qa_ld64u(h + 9 + 8ull * (word * 16 + l)) >> sh;
-    if (sh + w > 64) v |= qa_ld64u(h + 9 + 8ull * ((word + 1) * 16 + l)) << (64 - sh);
-    return mn + (v & (w >= 64 ? ~0ull : ((1ull << w) - 1)));
-}
 
 // Patch value v at tile index k (uniform across the wavefront) into the lane
 // that holds it. Static register indexing keeps x[] in registers.

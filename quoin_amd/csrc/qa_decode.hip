@@ -279,7 +279,9 @@ __device__ __forceinline__ T wsum(T v) {
 // per DeltaBitpack tile: the wrapping sum of its unzigzagged deltas, into the
 // tile descriptor's tail (k_dec_delta_scan turns the sums into carries)
 __global__ void __launch_bounds__(256) k_dec_delta_sums(DecArgs a) {
-    __shared__ __align__(16) uint32_t wbuf[4][2 * QA_TILE + 4];
+    // staged words of a w <= 32 sub-block (32 w, plus the row-word read past the
+    // last one); wider sub-blocks (rare) are read directly
+    __shared__ __align__(16) uint32_t wbuf[4][1088];
     const int lane = threadIdx.x & 63, wv = qa_wave_id();
     uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
     if (t >= a.t1) return;
@@ -289,18 +291,18 @@ __global__ void __launch_bounds__(256) k_dec_delta_sums(DecArgs a) {
     const uint8_t *h = td.src + td.off;
     const uint32_t w = td.w;
     uint64_t mn = qa_ld64u(h + 1);
-    uint32_t *wd = wbuf[wv];
-    stage_words(h + 9, 128 * w, wd, lane);
-    wave_lds_sync();
     uint64_t s = 0;
     if (w <= 32) {
+        uint32_t *wd = wbuf[wv];
+        stage_words(h + 9, 128 * w, wd, lane);
+        wave_lds_sync();
         uint32_t v[16];
         unpack16_u32(wd, w, lane, v);
 #pragma unroll
         for (int i = 0; i < 16; i++)
             s += 16u * (uint32_t)lane + i < cnt ? qa_unzigzag(mn + v[i]) : 0;
     } else {
-        for (uint32_t k = lane; k < cnt; k += WAVE) s += qa_unzigzag(mn + unpack_lds(wd, w, k));
+        for (uint32_t k = lane; k < cnt; k += WAVE) s += qa_unzigzag(for_value_direct(h, k));
     }
     s = wsum(s);
     if (lane == 0) td.tail = s;

@@ -165,22 +165,6 @@ struct OpXor64 { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { 
 struct OpAdd64 { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a + b; } };
 
 // FoR sub-block value k at stream sub-block header h
-__device__ __forceinline__ uint64_t for_value_at(const uint8_t *h, uint32_t k) {
-    uint32_t w = h[0];
-    uint64_t mn = qa_ld64u(h + 1);
-    if (w == 0) return mn;
-    // direct unaligned extraction (sub-block offsets are validated)
-    if (w <= 32) {
-        uint32_t r = k >> 5, l = k & 31, bit = r * w, word = bit >> 5, sh = bit & 31;
-        uint32_t v = qa_ld32u(h + 9 + 4ull * (word * 32 + l)) >> sh;
-        if (sh + w > 32) v |= qa_ld32u(h + 9 + 4ull * ((word + 1) * 32 + l)) << (32 - sh);
-        return mn + (v & (w >= 32 ? 0xFFFFFFFFu : ((1u << w) - 1)));
-    }
-    uint32_t r = k >> 4, l = k & 15, bit = r * w, word = bit >> 6, sh = bit & 63;
-    uint64_t v = qa_ld64u(h + 9 + 8ull * (word * 16 + l)) >> sh;
-    if (sh + w > 64) v |= qa_ld64u(h + 9 + 8ull * ((word + 1) * 16 + l)) << (64 - sh);
-    return mn + (v & (w >= 64 ? ~0ull : ((1ull << w) - 1)));
-}
 
 __global__ void __launch_bounds__(1024) k_dec_frame_wg(DecArgs a) {
     const uint32_t f = a.f0 + blockIdx.x;
@@ -256,7 +240,7 @@ __global__ void __launch_bounds__(1024) k_dec_frame_wg(DecArgs a) {
     for (uint64_t b0 = 0; b0 < m; b0 += blockDim.x) {
         uint64_t i = b0 + threadIdx.x, v = 0;
         if (i < m) {
-            uint64_t r = for_value_at(s + tl[i / QA_TILE].tail, (uint32_t)(i % QA_TILE));
+            uint64_t r = for_value_direct(s + tl[i / QA_TILE].tail, (uint32_t)(i % QA_TILE));
             if (dict) {
                 v = qa_unzigzag(r);
             } else {

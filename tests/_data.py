@@ -29,6 +29,13 @@ def int_columns(n, seed=0):
         "i64_ramp": np.arange(n, dtype=np.int64) * 3 - 7,
         "i64_const": np.full(n, -5, dtype=np.int64),
         "i64_runs": np.repeat(r.integers(0, 50, n // 8 + 1), 8)[:n].astype(np.int64),
+        # FoR wider than 32 bits (the 16-lane FastLanes form)
+        "i64_w40": r.integers(0, 2**40, n).astype(np.int64),
+        "i64_w57": r.integers(0, 2**57, n, dtype=np.int64),
+        # Delta with zig-zag deltas of about 33 bits
+        "i64_wide_walk": np.cumsum(r.integers(-2**31, 2**31, n)).astype(np.int64),
+        # Delta into a 4-byte column
+        "i32_walk": (7 + np.cumsum(r.integers(-3, 4, n))).astype(np.int32),
     }
     return cols
 
