@@ -39,8 +39,8 @@ def make_workload(n, seed=0):
     return cols
 
 
-def measured_traffic(kernel, args):
-    """HBM bytes per launch of `kernel` on this workload, from the committed
+def measured_traffic(stage, args):
+    """HBM bytes per call of a tile stage on this workload, from the committed
     counter summary (profiles/traffic_<workload>.json: rocprofv3 FETCH_SIZE
     and WRITE_SIZE passes, scaled by the same counters on a plain copy of
     known size), or None when no summary matches this configuration."""
@@ -50,10 +50,10 @@ def measured_traffic(kernel, args):
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    k = t.get("kernels", {}).get(kernel)
-    if t.get("values_per_column") != args.n or not k:
+    st = t.get("stages", {}).get(stage)
+    if t.get("values_per_column") != args.n or not st:
         return None
-    return k.get("hbm_bytes_per_launch")
+    return st.get("hbm_bytes_per_call")
 
 
 def cpu_baseline(cols, level, sample_n):
@@ -470,25 +470,32 @@ def main():
     dev_c_ms = dev_c / args.steps
     dev_d_ms = dev_d / args.steps
 
-    # Roofline of the dominant decode kernel (k_dec_tiles, one launch per
-    # decompress call): algorithmic bytes per launch = the stream bytes it
-    # reads (an upper bound of the frame payloads) + the typed values it writes
-    # + the decoded validity words it reads for the streams with nulls.
+    # Roofline of the dominant decode kernels: the tile decode stage
+    # (k_dec_fl<4> for the FoR u32 columns, k_dec_fl<8> for the Delta i64
+    # column, k_dec_tiles for other modes: none here), timed by the library's
+    # HIP events around the stage on its own stream. Algorithmic bytes per call
+    # = the stream bytes it reads (an upper bound of the frame payloads) + the
+    # typed values it writes + the decoded validity words it reads for the
+    # streams with nulls.
     bitmap_bytes = sum(((len(v) + 63) // 64) * 8 for _, v, bm in cols if bm is not None)
     dec_alg = comp_bytes + raw_bytes + bitmap_bytes
     kd_ms = kern_d / args.steps
     dec_gbs = dec_alg / (kd_ms * 1e-3) / 1e9 if kd_ms > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": "k_dec_tiles", "achieved": round(dec_gbs, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dec_gbs / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic("k_dec_tiles", args), "algorithmic_bytes": dec_alg,
+    roofline = {"bound": "hbm", "kernel": "tile decode: k_dec_fl<4> + k_dec_fl<8>",
+                "achieved": round(dec_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dec_gbs / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic("tile_decode", args), "algorithmic_bytes": dec_alg,
                 "kernel_ms": round(kd_ms, 4),
                 "decode_sequence_gbs": round((comp_bytes + raw_bytes) / (dev_d_ms * 1e-3) / 1e9, 1)}
-    # the encode side's tile kernel: reads the (compacted) lane, writes payloads
+    # the encode side's tile stage (k_emit_fl, and k_emit_tiles over its list:
+    # empty here): reads the (compacted) lane, writes payloads
     kc_ms = kern_c / args.steps
     enc_alg = raw_bytes + comp_bytes
-    enc_roofline = {"bound": "hbm", "kernel": "k_emit_tiles", "kernel_ms": round(kc_ms, 4),
+    enc_roofline = {"bound": "hbm", "kernel": "tile encode: k_emit_fl + k_emit_tiles",
+                    "kernel_ms": round(kc_ms, 4),
                     "achieved": round(enc_alg / (kc_ms * 1e-3) / 1e9, 1) if kc_ms > 0 else 0.0,
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_bytes": enc_alg}
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_bytes": enc_alg,
+                    "traffic": measured_traffic("tile_encode", args)}
     enc_roofline["frac"] = round(enc_roofline["achieved"] / HBM_PEAK_GBS, 4)
 
     if args.calibrate_copy:

@@ -472,7 +472,7 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         float kms = 0;
         if (hipEventElapsedTime(&kms, g->kev0, g->kev1) == hipSuccess) {
             g->last_kernel_ms = kms;
-            g->last_kernel = "k_emit_tiles";
+            g->last_kernel = "tile encode (k_emit_fl, k_emit_tiles)";
         }
     }
     for (int m = 0; m < 64; m++) g_wins[m] += wins[m];
@@ -649,7 +649,7 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         float kms = 0;
         if (hipEventElapsedTime(&kms, g->kev0, g->kev1) == hipSuccess) {
             g->last_kernel_ms = kms;
-            g->last_kernel = "k_dec_tiles";
+            g->last_kernel = "tile decode (k_dec_fl, k_dec_tiles)";
         }
     }
     if (err & QA_ERR_UNSUPPORTED) return QUOIN_ERR_UNSUPPORTED;
