@@ -211,85 +211,134 @@ def serve_bench(qa, dev, budget_gb=64, slots=2, pool_size=48, per_point=800):
             "points": points}
 
 
-DELTA2_COEFFS = [[], [1.0], [2.0, -1.0], [3.0, -3.0, 1.0], [4.0, -6.0, 4.0, -1.0]]
-
-
-def delta2_residuals(torch, V, block):
-    """Delta2 residual streams (src/codecs/linear.rs:125-160) of each `block`-value
-    block of the f64 lane V, on the device: order = the smallest mean |finite
-    difference| over the central 1024 values (torch's mean, not the reference's
-    sequential sum: a workload generator, not a parity path), pred = 0.0 +
-    c_0 h_0 + c_1 h_1 + ... with the order dropped to the history available, and
-    the stream [order] ++ varint(bits(v) ^ bits(pred)). Returns the concatenated
-    bytes (uint8 tensor) and each block's (offset, length)."""
-    nb = V.numel() // block
-    W = V[: nb * block].view(nb, block)
-    start = (block - 1024) // 2
-    d = W[:, start:start + 1024]
-    mags = []
-    for _ in range(4):
-        d = d[:, 1:] - d[:, :-1]
-        m = d.abs().mean(1)
-        mags.append(torch.where(torch.isfinite(m), m, torch.full_like(m, float("inf"))))
-    order = torch.stack(mags, 1).argmin(1) + 1
-    P = torch.zeros_like(W)
-    for o in range(1, 5):
-        rows = (order == o).nonzero().flatten()
-        if rows.numel() == 0:
-            continue
-        Wo = W[rows]
-        Po = torch.zeros_like(Wo)
-        for i in range(1, 4):  # warm-up: order limited to the i values before
-            c = DELTA2_COEFFS[min(o, i)]
-            acc = torch.zeros_like(Wo[:, 0])
-            for k, ck in enumerate(c):
-                acc = acc + ck * Wo[:, i - 1 - k]
-            Po[:, i] = acc
-        c = DELTA2_COEFFS[o]
-        acc = torch.zeros_like(Wo[:, 4:])
-        for k, ck in enumerate(c):
-            acc = acc + ck * Wo[:, 3 - k: block - 1 - k]
-        Po[:, 4:] = acc
-        P[rows] = Po
-    R = (W.contiguous().view(torch.int64) ^ P.view(torch.int64)).flatten()
-    L = torch.ones_like(R)
-    for j in range(1, 10):
-        L += ((R >> (7 * j)) != 0).to(torch.int64)
-    blen = L.view(nb, block).sum(1) + 1
-    boff = torch.cumsum(blen, 0) - blen
-    total = int(blen.sum())
-    # value bytes follow every earlier value and one order byte per block so far
-    nbk = torch.arange(1, nb + 1, device=V.device)
-    pos = torch.cumsum(L, 0) - L + torch.repeat_interleave(nbk, block)
-    out = torch.empty(total, dtype=torch.uint8, device=V.device)
-    out[boff] = order.to(torch.uint8)
-    for j in range(10):
-        m = L > j
-        b = ((R >> (7 * j)) & (1 if j == 9 else 0x7F)) | ((L > j + 1).to(torch.int64) << 7)
-        out[pos[m] + j] = b[m].to(torch.uint8)
-    return out, list(zip(boff.tolist(), blen.tolist()))
-
-
-def entropy_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=3):
-    """rANS decode on the entropy workload (config: 32 Mi f64 sin(2 pi i / 1000)
-    * 1e3 + N(0, 1e-3) and 32 Mi f32 ((u32 >> 8) 2^-23) - 1 widened to f64
-    lanes): each 32768-value block's Delta2 residual bytes are rANS-coded by
-    the GPU coder, then every block stream is decoded in one launch; decoded
-    bytes are checked against the residuals."""
-    import torch
+def entropy_lanes(n, seed=3):
+    """The entropy workload's two columns as f64 lanes: n f64 values
+    sin(2 pi i / 1000) * 1e3 + N(0, 1e-3) and n f32 values ((u32 >> 8) 2^-23) - 1
+    widened to f64."""
     r = np.random.default_rng(seed)
     i = np.arange(n, dtype=np.float64)
     f64 = np.sin(2 * np.pi * i / 1000) * 1000 + r.normal(0, 1e-3, n)
     u = r.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
     f32 = (u >> 8).astype(np.float32) * np.float32(2.0**-23) - np.float32(1)
-    res, blocks = [], []
-    for lane in (f64, f32.astype(np.float64)):
-        b, bl = delta2_residuals(torch, torch.from_numpy(lane).to(dev), block)
-        res.append(b)
-        blocks.append(bl)
+    return f64, f32.astype(np.float64)
+
+
+def linear_blocks(qa, gpu, torch, V, block, kind, order=0):
+    """Residual streams of every `block`-value block of the device f64 lane V in
+    one quoin_gpu_linear_encode call (src/codecs/linear.rs). Returns the slotted
+    output tensor, the LinearInput list, and each block's (offset, length)."""
+    nb = V.numel() // block
+    cap = int(qa.lib.quoin_linear_bound(block))
+    out = torch.empty(nb * cap, dtype=torch.uint8, device=V.device)
+    vp, op = V.data_ptr(), out.data_ptr()
+    ins = [qa.LinearInput(vp + 8 * b * block, block, op + b * cap, cap) for b in range(nb)]
+    lens, _, sts = gpu.linear_encode(kind, ins, order)
+    spans = [(b * cap, lens[b]) for b in range(nb) if not sts[b]]
+    return out, ins, spans
+
+
+def resid_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=3):
+    """The linear residual coders (src/codecs/linear.rs) on n f64 values
+    sin(2 pi i / 1000) * 1e3 + N(0, 1e-3) in 32768-value blocks, every block of
+    a kind in one call: encode = select_order + the residual streams, decode =
+    the varint parse + the recurrence, checked against the input. GB/s are of
+    raw f64 bytes; the roofline is Delta2 encode's (n * 8 + stream bytes per
+    call over its kernels' time)."""
+    import torch
+    f64, _ = entropy_lanes(n, seed)
+    V = torch.from_numpy(f64.view(np.int64)).to(dev)
+    nb = n // block
+    raw = nb * block * 8
+    res = {"workload": f"{nb} blocks of {block} f64 values, sine + noise", "kinds": {}}
+    samples = []
+    for name, kind in (("delta2", qa.LINEAR_DELTA2), ("delta_dp", qa.LINEAR_DELTA_DP),
+                       ("ordered_delta", qa.LINEAR_ORDERED_DELTA)):
+        out, ins, _ = linear_blocks(qa, gpu, torch, V, block, kind)
+        lens, ords, sts = gpu.linear_encode(kind, ins)
+        D = torch.zeros_like(V)
+        ss = [qa.LinearStream(ins[b].dst, lens[b], block, D.data_ptr() + 8 * b * block)
+              for b in range(nb) if not sts[b]]
+        dst = gpu.linear_decode(kind, ss)
+        torch.cuda.synchronize()
+        assert all(x == qa.OK for x in dst), f"{name}: decode failed"
+        ok = [b for b in range(nb) if not sts[b]]
+        if ok:
+            sel = torch.tensor(ok, device=dev)
+            assert torch.equal(D.view(nb, block)[sel], V[:nb * block].view(nb, block)[sel]), \
+                f"{name}: round trip mismatch"
+        for _ in range(warmup):
+            gpu.linear_encode(kind, ins)
+        ek = 0.0
+        for _ in range(steps):
+            gpu.linear_encode(kind, ins)
+            ek += gpu.last_timing()[1]
+        for _ in range(warmup):
+            gpu.linear_decode(kind, ss)
+        dt = dk = 0.0
+        for _ in range(steps):
+            gpu.linear_decode(kind, ss)
+            t, k, _ = gpu.last_timing()
+            dt += t
+            dk += k
+        ek, dt, dk = ek / steps, dt / steps, dk / steps
+        coded = int(sum(lens))
+        res["kinds"][name] = {
+            "stream_bytes": coded, "declined_blocks": int(sum(sts)),
+            "orders": {str(o): ords.count(o) for o in sorted(set(ords))} if kind != 7 else None,
+            "encode_ms": round(ek, 4), "encode_gbs": round(raw / (ek * 1e-3) / 1e9, 1),
+            "decode_ms": round(dt, 4), "decode_gbs": round(raw / (dt * 1e-3) / 1e9, 1),
+            "recurrence_ms": round(dk, 4)}
+        if name == "delta2":
+            alg = raw + coded
+            res["roofline"] = {"bound": "hbm", "kernel": "linear encode: k_lin_order + k_lin_enc",
+                               "kernel_ms": round(ek, 4), "algorithmic_bytes": alg,
+                               "achieved": round(alg / (ek * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s",
+                               "frac": round(alg / (ek * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+            samples = [f64[b * block:(b + 1) * block].view(np.uint64) for b in range(0, nb, 64)]
+        del out, D
+    del V
+    return res, samples
+
+
+def resid_cpu_baseline(samples):
+    """Oracle Delta2 encode (select_order included) and decode, single-threaded C,
+    over every 64th block."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as orc
+    t0 = time.perf_counter()
+    enc = [orc.linear_encode(20, b) for b in samples]
+    te = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for e, b in zip(enc, samples):
+        orc.linear_decode(20, e, len(b))
+    td = time.perf_counter() - t0
+    raw = sum(len(b) for b in samples) * 8
+    return {"value": round(raw / te / 1e9, 4), "decode_value": round(raw / td / 1e9, 4),
+            "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"{len(samples)} of the 32768-value blocks, Delta2 encode / decode, "
+                      "oracle C, 1 thread"}
+
+
+def entropy_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=3):
+    """rANS decode on the entropy workload (entropy_lanes): each 32768-value
+    block's Delta2 residual bytes, from the device residual coder, are rANS-coded
+    by the GPU coder, then every block stream is decoded in one launch; decoded
+    bytes are checked against the residuals."""
+    import torch
+    res, bspans = [], []
+    for lane in entropy_lanes(n, seed):
+        V = torch.from_numpy(lane.view(np.int64)).to(dev)
+        out, _, sp = linear_blocks(qa, gpu, torch, V, block, qa.LINEAR_DELTA2)
+        res.append(torch.cat([out[o:o + ln] for o, ln in sp]))
+        bspans.append([ln for _, ln in sp])
+        del V, out
     syms = torch.cat(res)
-    base = [0, res[0].numel()]
-    spans = [(base[c] + o, ln) for c in range(2) for o, ln in blocks[c]]
+    spans, off = [], 0
+    for lns in bspans:
+        for ln in lns:
+            spans.append((off, ln))
+            off += ln
     caps = [qa.lib.quoin_rans_bound(ln) for _, ln in spans]
     coff = np.concatenate([[0], np.cumsum(caps)]).tolist()
     coded = torch.empty(coff[-1], dtype=torch.uint8, device=dev)
@@ -365,6 +414,9 @@ def main():
     ap.add_argument("--no-entropy", action="store_true", help="skip the rANS decode leg")
     ap.add_argument("--entropy-n", type=int, default=32 << 20,
                     help="values per column of the rANS decode leg")
+    ap.add_argument("--no-resid", action="store_true", help="skip the residual coder leg")
+    ap.add_argument("--resid-n", type=int, default=32 << 20,
+                    help="f64 values of the residual coder leg")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="bytes of a plain elementwise copy run after the measurement "
                          "(for scaling HBM traffic counters)")
@@ -514,6 +566,10 @@ def main():
     if not args.no_entropy:
         entropy, ent_sample = entropy_bench(qa, gpu, dev, args.entropy_n, 10, 2)
 
+    resid = resid_sample = None
+    if not args.no_resid:
+        resid, resid_sample = resid_bench(qa, gpu, dev, args.resid_n, 10, 2)
+
     cpu = None
     if not args.no_cpu:
         cpu_gbs, sample_raw = cpu_baseline(cols, int(level), args.cpu_sample)
@@ -522,6 +578,8 @@ def main():
                          f"({sample_raw / 1e6:.1f} MB), compress+decompress, oracle C, 1 thread"}
         if entropy is not None:
             entropy["cpu_baseline"] = rans_cpu_baseline(*ent_sample)
+        if resid is not None:
+            resid["cpu_baseline"] = resid_cpu_baseline(resid_sample)
 
     out = {
         "metric": "compress+decompress GB/s (bit-exact), batched",
@@ -542,6 +600,7 @@ def main():
         "encode_roofline": enc_roofline,
         "serving": serving,
         "entropy_decode": entropy,
+        "residual_coders": resid,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -276,6 +276,59 @@ int quoin_gpu_rans_compress(QuoinGpu *g, const QuoinRansInput *in, size_t n_stre
 int quoin_gpu_rans_decompress(QuoinGpu *g, const QuoinRansStream *streams, size_t n_streams,
                               int tagged, uint64_t *out_lens, int32_t *status, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Linear residual coders on the device (src/codecs/linear.rs)               */
+/* ------------------------------------------------------------------------ */
+/* kind = the frame mode the residuals belong to (src/mode.rs:27-34). */
+#define QUOIN_LINEAR_ORDERED_DELTA 7  /* varint(zigzag(v - pred)), integer pred */
+#define QUOIN_LINEAR_DELTA2        20 /* [order] ++ varint(bits(v) ^ bits(pred)) */
+#define QUOIN_LINEAR_DELTA_DP      45 /* [order] ++ varint(bits(r) ^ bits(r_prev)) */
+
+/* One block of lane words for quoin_gpu_linear_encode. */
+typedef struct {
+    const uint64_t *vals;      /* device, n u64 lane words (f64 bit patterns) */
+    uint64_t n;
+    uint8_t *dst;              /* device, dst_cap bytes                    */
+    uint64_t dst_cap;          /* >= quoin_linear_bound(n)                 */
+} QuoinLinearInput;
+
+/* One residual stream for quoin_gpu_linear_decode. */
+typedef struct {
+    const uint8_t *src;        /* device, src_len bytes                    */
+    uint64_t src_len;
+    uint64_t n;                /* values the block holds                   */
+    uint64_t *out;             /* device, n u64 lane words                 */
+} QuoinLinearStream;
+
+/* Largest residual stream of n values: 10 n + 1. */
+uint64_t quoin_linear_bound(uint64_t n);
+
+/* Replaces, for a batch of blocks coded by one launch sequence:
+ *   kind 7:  linear::idelta2_encode (src/codecs/linear.rs:31-45);
+ *   kind 20: linear::encode (linear.rs:149-159) with order from
+ *            linear::select_order (linear.rs:125-143) when order = 0;
+ *   kind 45: linear::dp_encode (linear.rs:191-212), same order rule.
+ * order: 0 = select_order per block, 1..4 = that order for every block
+ * (ignored by kind 7). out_lens[i] = stream bytes; orders[i] (optional) = the
+ * order used; status[i] (optional) = 1 where dp_encode returns None (an empty
+ * block, or a value whose residual does not add back exactly), 0 otherwise.
+ * f64 arithmetic follows x86-64 SSE2 (NaN results keep the first NaN operand,
+ * quieted; invalid operations give 0xFFF8000000000000). QUOIN_ERR_UNSUPPORTED
+ * for another kind or order, QUOIN_ERR_BUFFER_TOO_SMALL when a dst_cap is
+ * below quoin_linear_bound(n). */
+int quoin_gpu_linear_encode(QuoinGpu *g, int32_t kind, int32_t order, const QuoinLinearInput *in,
+                            size_t n_blocks, uint64_t *out_lens, int32_t *orders, int32_t *status,
+                            void *stream);
+
+/* Replaces linear::idelta2_decode (linear.rs:47-66), linear::decode
+ * (linear.rs:161-180) and linear::dp_decode (linear.rs:214-236) for a batch.
+ * status[i] (optional) = QUOIN_OK, or QUOIN_ERR_CORRUPT where the reference
+ * returns Err (a bad order byte, too few varints, trailing bytes, a varint
+ * over 10 bytes; the block's out is then unspecified). Returns QUOIN_OK or
+ * the first block's error. */
+int quoin_gpu_linear_decode(QuoinGpu *g, int32_t kind, const QuoinLinearStream *streams,
+                            size_t n_blocks, int32_t *status, void *stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -1820,3 +1820,161 @@ uint8_t *orc_encode_mode(int mode, const uint64_t *v, size_t n, int dt, size_t *
 int orc_decode_mode(int mode, const uint8_t *payload, size_t len, size_t n, int dt, uint64_t *out) {
     return decode_frame(mode, payload, len, n, dt, out);
 }
+
+/* ------------------------------------------------------------------------ */
+/* linear residual coders (codecs/linear.rs)                                  */
+/* ------------------------------------------------------------------------ */
+/*
+ * Float arithmetic follows x86-64 SSE2 scalar semantics, spelled out so the
+ * result does not depend on how this file is compiled: a NaN result is the
+ * first NaN operand made quiet, else (an invalid operation such as inf - inf)
+ * the default NaN 0xFFF8000000000000. Operands are taken in source order
+ * (`pred + c*h`, `v - pred`, `pred + r`); only when both addends are NaN does
+ * the order decide the payload.
+ */
+#define LIN_DEFAULT_NAN 0xFFF8000000000000ull
+static int lin_isnan(double x) { return x != x; }
+static double lin_fix(double r, double a, double b) {
+    if (!lin_isnan(r)) return r;
+    if (lin_isnan(a)) return bits_f64(f64_bits(a) | (1ull << 51));
+    if (lin_isnan(b)) return bits_f64(f64_bits(b) | (1ull << 51));
+    return bits_f64(LIN_DEFAULT_NAN);
+}
+static double lin_add(double a, double b) { return lin_fix(a + b, a, b); }
+static double lin_sub(double a, double b) { return lin_fix(a - b, a, b); }
+static double lin_mul(double a, double b) { return lin_fix(a * b, a, b); }
+
+/* linear.rs:75 — Newton extrapolation coefficients, most recent first */
+static const int LIN_COEF[5][4] = {{0}, {1}, {2, -1}, {3, -3, 1}, {4, -6, 4, -1}};
+
+/* linear.rs:83-94: pred = 0.0; pred += c_k * h[k] for k < min(order, avail) */
+static double lin_predict(const double h[4], size_t avail, int order) {
+    size_t eff = (size_t)order < avail ? (size_t)order : avail;
+    if (eff == 0) return 0.0;
+    double pred = 0.0;
+    for (size_t k = 0; k < eff; k++) pred = lin_add(pred, lin_mul((double)LIN_COEF[eff][k], h[k]));
+    return pred;
+}
+/* linear.rs:96-102 */
+static void lin_push(double h[4], double v) {
+    h[3] = h[2];
+    h[2] = h[1];
+    h[1] = h[0];
+    h[0] = v;
+}
+
+/* linear.rs:125-143 (diff_in_place 105-111, mean_abs 113-119): the order with
+ * the smallest finite mean |d-th difference| over the central min(1024, n)
+ * values; the sum runs left to right. */
+int orc_linear_select_order(const uint64_t *vals, size_t n) {
+    if (n < 8) return 2;
+    size_t win = n < 1024 ? n : 1024, start = (n - win) / 2;
+    double d[1024];
+    for (size_t i = 0; i < win; i++) d[i] = bits_f64(vals[start + i]);
+    size_t len = win;
+    int best = 1;
+    double best_mag = INFINITY;
+    for (int order = 1; order <= 4; order++) {
+        for (size_t i = 0; i + 1 < len; i++) d[i] = d[i + 1] - d[i];
+        if (len) len--;
+        double mag = INFINITY;
+        if (len) {
+            double s = 0.0;
+            for (size_t i = 0; i < len; i++) s += fabs(d[i]);
+            mag = s / (double)len;
+        }
+        if (isfinite(mag) && mag < best_mag) {
+            best_mag = mag;
+            best = order;
+        }
+    }
+    return best;
+}
+
+/* linear.rs:31-45 (OrderedDelta), 149-159 (Delta2), 191-212 (DeltaDp).
+ * Returns NULL with *len = 0 when DeltaDp bails (empty input, or a value whose
+ * residual does not add back exactly). */
+uint8_t *orc_linear_encode(int kind, const uint64_t *vals, size_t n, int order, size_t *len) {
+    buf_t p = {0};
+    *len = 0;
+    if (kind == 7) {
+        uint64_t p1 = 0, p2 = 0;
+        for (size_t i = 0; i < n; i++) {
+            uint64_t pred = i == 0 ? 0 : i == 1 ? p1 : p1 * 2 - p2;
+            varint_write(&p, zigzag(vals[i] - pred));
+            p2 = p1;
+            p1 = vals[i];
+        }
+    } else if (kind == 20 || kind == 45) {
+        if (kind == 45 && n == 0) return NULL;
+        buf_push(&p, (uint8_t)order);
+        double h[4] = {0, 0, 0, 0};
+        uint64_t prev_r = 0;
+        for (size_t i = 0; i < n; i++) {
+            double v = bits_f64(vals[i]);
+            double pred = lin_predict(h, i < 4 ? i : 4, order);
+            if (kind == 20) {
+                varint_write(&p, vals[i] ^ f64_bits(pred));
+            } else {
+                double r = lin_sub(v, pred);
+                if (f64_bits(lin_add(pred, r)) != vals[i]) {
+                    buf_free(&p);
+                    return NULL;
+                }
+                uint64_t rb = f64_bits(r);
+                varint_write(&p, rb ^ prev_r);
+                prev_r = rb;
+            }
+            lin_push(h, v);
+        }
+    } else {
+        return NULL;
+    }
+    *len = p.len;
+    if (!p.p) p.p = (uint8_t *)malloc(1);
+    return p.p;
+}
+
+/* linear.rs:47-66 (OrderedDelta), 161-180 (Delta2), 214-236 (DeltaDp) */
+int orc_linear_decode(int kind, const uint8_t *pl, size_t len, size_t n, uint64_t *out) {
+    size_t pos = 0;
+    if (kind == 7) {
+        uint64_t p1 = 0, p2 = 0;
+        for (size_t i = 0; i < n; i++) {
+            uint64_t pred = i == 0 ? 0 : i == 1 ? p1 : p1 * 2 - p2, z;
+            int rc = varint_read(pl, len, &pos, &z);
+            if (rc) return rc;
+            uint64_t v = pred + unzigzag(z);
+            out[i] = v;
+            p2 = p1;
+            p1 = v;
+        }
+        return pos == len ? ORC_OK : ORC_CORRUPT;
+    }
+    if (kind != 20 && kind != 45) return ORC_UNSUPPORTED;
+    if (len == 0) return ORC_TRUNCATED;
+    int order = pl[0];
+    if (order < 1 || order > 4) return ORC_CORRUPT;
+    const uint8_t *rest = pl + 1;
+    size_t rl = len - 1;
+    double h[4] = {0, 0, 0, 0};
+    uint64_t prev_r = 0;
+    for (size_t i = 0; i < n; i++) {
+        double pred = lin_predict(h, i < 4 ? i : 4, order);
+        uint64_t x;
+        int rc = varint_read(rest, rl, &pos, &x);
+        if (rc) return rc;
+        double v;
+        if (kind == 20) {
+            out[i] = x ^ f64_bits(pred);
+            v = bits_f64(out[i]);
+        } else {
+            uint64_t rb = x ^ prev_r;
+            v = lin_add(pred, bits_f64(rb));
+            out[i] = f64_bits(v);
+            prev_r = rb;
+        }
+        lin_push(h, v);
+    }
+    return pos == rl ? ORC_OK : ORC_CORRUPT;
+}

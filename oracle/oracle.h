@@ -74,6 +74,14 @@ uint8_t *orc_rans_compress(const uint8_t *data, size_t n, size_t *len);
 int orc_rans_decompress(const uint8_t *payload, size_t len, size_t max_len, uint8_t **out,
                         size_t *out_len);
 
+/* Linear residual coders (codecs/linear.rs). kind = the frame mode: 7
+ * OrderedDelta, 20 Delta2, 45 DeltaDp. encode: order 1..4 (ignored by
+ * OrderedDelta); returns NULL with *len = 0 when DeltaDp bails. decode: n values
+ * into out. */
+int orc_linear_select_order(const uint64_t *vals, size_t n);
+uint8_t *orc_linear_encode(int kind, const uint64_t *vals, size_t n, int order, size_t *len);
+int orc_linear_decode(int kind, const uint8_t *payload, size_t len, size_t n, uint64_t *out);
+
 /* Feature probe + planning (encoder.rs:154-178, 705-750). ranges gets
  * (start,end) pairs; returns the number of blocks. ranges must hold n/1+1 pairs
  * in the worst case of block_size 1; callers size it as n+1 pairs. */

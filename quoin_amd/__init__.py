@@ -109,6 +109,22 @@ class RansStream(ctypes.Structure):
                 ("out", ctypes.c_void_p)]
 
 
+class LinearInput(ctypes.Structure):
+    """QuoinLinearInput (include/quoin.h): a device block of u64 lane words and
+    the buffer its residual stream goes to."""
+    _fields_ = [("vals", ctypes.c_void_p), ("n", ctypes.c_uint64), ("dst", ctypes.c_void_p),
+                ("dst_cap", ctypes.c_uint64)]
+
+
+class LinearStream(ctypes.Structure):
+    """QuoinLinearStream (include/quoin.h): a device residual stream and its output."""
+    _fields_ = [("src", ctypes.c_void_p), ("src_len", ctypes.c_uint64), ("n", ctypes.c_uint64),
+                ("out", ctypes.c_void_p)]
+
+
+LINEAR_ORDERED_DELTA, LINEAR_DELTA2, LINEAR_DELTA_DP = 7, 20, 45
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run python quoin_amd/build.py")
@@ -160,6 +176,15 @@ def _load():
     lib.quoin_gpu_rans_decompress.argtypes = [vp, ctypes.POINTER(RansStream), sz, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_uint64),
                                               ctypes.POINTER(ctypes.c_int32), vp]
+    lib.quoin_linear_bound.restype = ctypes.c_uint64
+    lib.quoin_linear_bound.argtypes = [ctypes.c_uint64]
+    lib.quoin_gpu_linear_encode.argtypes = [vp, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.POINTER(LinearInput), sz,
+                                            ctypes.POINTER(ctypes.c_uint64),
+                                            ctypes.POINTER(ctypes.c_int32),
+                                            ctypes.POINTER(ctypes.c_int32), vp]
+    lib.quoin_gpu_linear_decode.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(LinearStream), sz,
+                                            ctypes.POINTER(ctypes.c_int32), vp]
     return lib
 
 
@@ -176,7 +201,8 @@ EXPORTS = [
     "quoin_gpu_decompress", "quoin_gpu_set_launch_form", "quoin_gpu_last_timing",
     "quoin_server_create", "quoin_server_submit", "quoin_server_wait", "quoin_server_stats",
     "quoin_server_destroy", "quoin_rans_bound", "quoin_gpu_rans_compress",
-    "quoin_gpu_rans_decompress",
+    "quoin_gpu_rans_decompress", "quoin_linear_bound", "quoin_gpu_linear_encode",
+    "quoin_gpu_linear_decode",
 ]
 
 
@@ -319,6 +345,31 @@ class Gpu:
         if rc not in (OK, ERR_CORRUPT, ERR_UNSUPPORTED):
             raise QuoinError(rc)
         return [int(x) for x in lens], [int(x) for x in st]
+
+    def linear_encode(self, kind, inputs, order=0, stream=None):
+        """Residual streams of a batch of blocks (src/codecs/linear.rs:31-45,
+        125-159, 191-212). inputs: list of LinearInput. Returns (lengths, orders,
+        statuses); status 1 = DeltaDp declined the block (the reference's None)."""
+        k = len(inputs)
+        arr = (LinearInput * k)(*inputs)
+        lens = (ctypes.c_uint64 * k)()
+        ords = (ctypes.c_int32 * k)()
+        st = (ctypes.c_int32 * k)()
+        rc = lib.quoin_gpu_linear_encode(self.h, kind, order, arr, k, lens, ords, st, stream)
+        if rc != OK:
+            raise QuoinError(rc)
+        return [int(x) for x in lens], [int(x) for x in ords], [int(x) for x in st]
+
+    def linear_decode(self, kind, streams, stream=None):
+        """Values of a batch of residual streams (linear.rs:47-66, 161-180,
+        214-236). Returns the per-block status (OK or ERR_CORRUPT)."""
+        k = len(streams)
+        arr = (LinearStream * k)(*streams)
+        st = (ctypes.c_int32 * k)()
+        rc = lib.quoin_gpu_linear_decode(self.h, kind, arr, k, st, stream)
+        if rc not in (OK, ERR_CORRUPT):
+            raise QuoinError(rc)
+        return [int(x) for x in st]
 
     def last_timing(self):
         t, k = ctypes.c_double(), ctypes.c_double()

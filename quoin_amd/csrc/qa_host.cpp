@@ -77,6 +77,8 @@ struct QuoinGpu {
     DevBuf st_in, st_valid, st_out, st_vout;
     // rANS arenas
     DevBuf rdesc, rres, rtabs, rscr;
+    // linear residual arenas
+    DevBuf ldesc, lres, ltile, lscan;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, kev0 = nullptr, kev1 = nullptr;
     double last_total_ms = 0, last_kernel_ms = 0;
     const char *last_kernel = "";
@@ -761,6 +763,130 @@ extern "C" int quoin_gpu_rans_decompress(QuoinGpu *g, const QuoinRansStream *in,
                        : r[i].status == 2 ? QUOIN_ERR_UNSUPPORTED
                                           : QUOIN_ERR_CORRUPT;
         out_lens[i] = st == QUOIN_OK ? r[i].n : 0;
+        if (status) status[i] = st;
+        if (rc == QUOIN_OK) rc = st;
+    }
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// linear residual coders (src/codecs/linear.rs)
+// ---------------------------------------------------------------------------
+namespace {
+bool lin_kind(int32_t kind) {
+    return kind == QUOIN_LINEAR_ORDERED_DELTA || kind == QUOIN_LINEAR_DELTA2 ||
+           kind == QUOIN_LINEAR_DELTA_DP;
+}
+}  // namespace
+
+extern "C" uint64_t quoin_linear_bound(uint64_t n) {
+    // [order] + one varint of at most 10 bytes per value (varint.rs:9-21)
+    return n > (UINT64_MAX - 1) / 10 ? UINT64_MAX : 10 * n + 1;
+}
+
+extern "C" int quoin_gpu_linear_encode(QuoinGpu *g, int32_t kind, int32_t order,
+                                       const QuoinLinearInput *in, size_t nb, uint64_t *out_lens,
+                                       int32_t *orders, int32_t *status, void *stream) {
+    if (!g || (nb && (!in || !out_lens))) return QUOIN_ERR_NULL;
+    if (!lin_kind(kind) || order < 0 || order > 4) return QUOIN_ERR_UNSUPPORTED;
+    if (nb > 0xFFFFFFFFull) return QUOIN_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    if (nb == 0) return QUOIN_OK;
+    std::vector<LinD> d(nb);
+    uint64_t tiles = 0;
+    for (size_t i = 0; i < nb; i++) {
+        const uint64_t n = in[i].n;
+        const bool writes = n > 0 || kind == QUOIN_LINEAR_DELTA2;  // Delta2 of nothing is [order]
+        if ((n && !in[i].vals) || (writes && !in[i].dst)) return QUOIN_ERR_NULL;
+        if (writes && in[i].dst_cap < quoin_linear_bound(n)) return QUOIN_ERR_BUFFER_TOO_SMALL;
+        d[i] = LinD{};
+        d[i].vals = in[i].vals;
+        d[i].dst = in[i].dst;
+        d[i].n = n;
+        d[i].tile0 = tiles;
+        d[i].order = (uint32_t)order;
+        tiles += qa_lin_enc_tiles(n);
+    }
+    if (tiles > 0x7FFFFFFFull) return QUOIN_ERR_UNSUPPORTED;
+    if (!ensure(g->ldesc, nb * sizeof(LinD)) || !ensure(g->lres, nb * sizeof(LinR)) ||
+        !ensure(g->ltile, (2 * tiles + 2) * 8) ||
+        !ensure_scan(g->lscan, qa_scan_tmp_words(tiles) * 8, s))
+        return QUOIN_ERR_DEVICE;
+    uint64_t *tsz = dp<uint64_t>(g->ltile), *tpre = tsz + tiles + 1;
+    hipEventRecord(g->ev0, s);
+    hipMemcpyAsync(g->ldesc.p, d.data(), nb * sizeof(LinD), hipMemcpyHostToDevice, s);
+    hipEventRecord(g->kev0, s);
+    qa_lin_encode_launch((uint32_t)kind, dp<LinD>(g->ldesc), dp<LinR>(g->lres), (uint32_t)nb, tiles,
+                         tsz, tpre, dp<uint64_t>(g->lscan), s);
+    hipEventRecord(g->kev1, s);
+    hipEventRecord(g->ev1, s);
+    std::vector<LinR> r(nb);
+    hipMemcpyAsync(r.data(), g->lres.p, nb * sizeof(LinR), hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    float ms = 0, kms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    hipEventElapsedTime(&kms, g->kev0, g->kev1);
+    g->last_total_ms = ms;
+    g->last_kernel_ms = kms;
+    g->last_kernel = "linear encode (k_lin_order, k_lin_enc)";
+    for (size_t i = 0; i < nb; i++) {
+        out_lens[i] = r[i].status ? 0 : r[i].len;
+        if (orders) orders[i] = (int32_t)r[i].order;
+        if (status) status[i] = r[i].status ? 1 : 0;
+    }
+    return QUOIN_OK;
+}
+
+extern "C" int quoin_gpu_linear_decode(QuoinGpu *g, int32_t kind, const QuoinLinearStream *in,
+                                       size_t nb, int32_t *status, void *stream) {
+    if (!g || (nb && !in)) return QUOIN_ERR_NULL;
+    if (!lin_kind(kind)) return QUOIN_ERR_UNSUPPORTED;
+    if (nb > 0xFFFFFFFFull) return QUOIN_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    if (nb == 0) return QUOIN_OK;
+    const uint64_t hdr = kind == QUOIN_LINEAR_ORDERED_DELTA ? 0 : 1;
+    std::vector<LinD> d(nb);
+    uint64_t tiles = 0;
+    for (size_t i = 0; i < nb; i++) {
+        if ((in[i].src_len && !in[i].src) || (in[i].n && !in[i].out)) return QUOIN_ERR_NULL;
+        d[i] = LinD{};
+        d[i].src = in[i].src;
+        d[i].out = in[i].out;
+        d[i].n = in[i].n;
+        d[i].len = in[i].src_len;
+        d[i].tile0 = tiles;
+        tiles += qa_lin_dec_tiles(in[i].src_len > hdr ? in[i].src_len - hdr : 0);
+    }
+    if (tiles > 0x7FFFFFFFull) return QUOIN_ERR_UNSUPPORTED;
+    if (!ensure(g->ldesc, nb * sizeof(LinD)) || !ensure(g->lres, nb * sizeof(LinR)) ||
+        !ensure(g->ltile, (2 * tiles + 2) * 8) ||
+        !ensure_scan(g->lscan, qa_scan_tmp_words(tiles) * 8, s))
+        return QUOIN_ERR_DEVICE;
+    uint64_t *cnt = dp<uint64_t>(g->ltile), *pre = cnt + tiles + 1;
+    std::vector<LinR> r(nb, LinR{0, 0, 0});
+    hipEventRecord(g->ev0, s);
+    hipMemcpyAsync(g->ldesc.p, d.data(), nb * sizeof(LinD), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(g->lres.p, r.data(), nb * sizeof(LinR), hipMemcpyHostToDevice, s);
+    qa_lin_decode_launch((uint32_t)kind, dp<LinD>(g->ldesc), dp<LinR>(g->lres), (uint32_t)nb, tiles,
+                         cnt, pre, dp<uint64_t>(g->lscan), s, g->kev0, g->kev1);
+    hipEventRecord(g->ev1, s);
+    hipMemcpyAsync(r.data(), g->lres.p, nb * sizeof(LinR), hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    float ms = 0, kms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    hipEventElapsedTime(&kms, g->kev0, g->kev1);
+    g->last_total_ms = ms;
+    g->last_kernel_ms = kms;
+    g->last_kernel = kind == QUOIN_LINEAR_ORDERED_DELTA ? "k_lin_od" : "k_lin_seq";
+    int rc = QUOIN_OK;
+    for (size_t i = 0; i < nb; i++) {
+        const int st = r[i].status ? QUOIN_ERR_CORRUPT : QUOIN_OK;
         if (status) status[i] = st;
         if (rc == QUOIN_OK) rc = st;
     }

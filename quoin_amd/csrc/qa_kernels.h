@@ -247,3 +247,28 @@ struct RansEncR {
 void qa_rans_decode_launch(const RansDecD *d, RansDecR *r, uint32_t ns, hipStream_t s);
 void qa_rans_encode_launch(const RansEncD *d, RansEncR *r, uint64_t *tabs, uint8_t *scr, uint32_t ns,
                            hipStream_t s);
+
+// linear residual coders (src/codecs/linear.rs): one block of a batch
+struct LinD {
+    const uint64_t *vals;  // encode: the block's lane words
+    uint64_t *out;         // decode: n lane words
+    const uint8_t *src;    // decode: the residual stream
+    uint8_t *dst;          // encode: the residual stream (10 n + 1 bytes)
+    uint64_t n;            // values
+    uint64_t len;          // decode: stream bytes
+    uint64_t tile0;        // first tile (encode: 2048 values, decode: 4096 stream bytes)
+    uint32_t order;        // encode: 0 = select_order, else 1..4
+    uint32_t pad;
+};
+struct LinR {
+    uint64_t len;          // encode: stream bytes
+    int32_t status;        // encode: 1 = DeltaDp declined; decode: 1 = corrupt
+    uint32_t order;
+};
+uint64_t qa_lin_enc_tiles(uint64_t n);
+uint64_t qa_lin_dec_tiles(uint64_t bytes);
+void qa_lin_encode_launch(uint32_t kind, const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles,
+                          uint64_t *tsz, uint64_t *tpre, uint64_t *tmp, hipStream_t s);
+void qa_lin_decode_launch(uint32_t kind, const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles,
+                          uint64_t *cnt, uint64_t *pre, uint64_t *tmp, hipStream_t s, void *kev0,
+                          void *kev1);

@@ -1,0 +1,557 @@
+// qa_resid.hip — the linear residual coders of src/codecs/linear.rs on the
+// device, batched over independent blocks:
+//   OrderedDelta (mode 7)  varint(zigzag(v - pred)), pred in {0, p1, 2 p1 - p2}
+//                          wrapping (linear.rs:31-66);
+//   Delta2 (mode 20)       [order] ++ varint(bits(v) ^ bits(pred)) with an f64
+//                          polynomial prediction (linear.rs:125-180);
+//   DeltaDp (mode 45)      [order] ++ varint(bits(r) ^ bits(r_prev)), r = v - pred,
+//                          verified to add back exactly (linear.rs:191-236).
+//
+// Encode is a map from each value and the four before it, so it runs on
+// 2048-value tiles: k_lin_order picks each block's order (select_order: four
+// finite-difference rounds in LDS, then four lanes each run one order's
+// left-to-right sum), k_lin_enc<false> sizes every tile's varints, a prefix sum
+// places the tiles, and k_lin_enc<true> recomputes the residuals, stages the
+// tile's bytes in LDS at the alignment of their destination and writes them
+// with 16-byte stores.
+//
+// Decode splits the varint parse from the recurrence. The parse runs on
+// 4096-byte tiles: terminator counts, a prefix sum, then every terminator byte
+// assembles its value from the bytes before it. OrderedDelta is then two
+// wrapping prefix sums (one workgroup per block); Delta2 and DeltaDp are f64
+// recurrences, one lane per block.
+//
+// f64 results follow x86-64 SSE2 scalar semantics: a NaN result is the first
+// NaN operand made quiet, else the default NaN 0xFFF8000000000000. The
+// prediction runs on the hardware first; only a NaN prediction (never the
+// case without NaN or infinite history) is recomputed step by step with that
+// rule. The file is compiled without FP contraction.
+#include "qa_kernels.h"
+
+#define LIN_OD 7u
+#define LIN_D2 20u
+#define LIN_DP 45u
+#define LE_TILE 2048  // values per encode tile
+#define LE_WG 256
+#define LE_PER 8
+#define LD_TILE 4096  // stream bytes per decode tile
+#define LD_WG 256
+#define LD_PER 16
+#define LIN_DNAN 0xFFF8000000000000ull
+
+namespace {
+
+__device__ __forceinline__ double b2d(uint64_t b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ uint64_t d2b(double x) { return (uint64_t)__double_as_longlong(x); }
+__device__ __forceinline__ bool is_nan(double x) { return x != x; }
+
+// x86-64 SSE2 result of op(a, b) whose hardware result is r
+__device__ __forceinline__ double sse_fix(double r, double a, double b) {
+    if (!is_nan(r)) return r;
+    if (is_nan(a)) return b2d(d2b(a) | (1ull << 51));
+    if (is_nan(b)) return b2d(d2b(b) | (1ull << 51));
+    return b2d(LIN_DNAN);
+}
+__device__ __forceinline__ double sse_add(double a, double b) {
+    const double r = a + b;
+    return is_nan(r) ? sse_fix(r, a, b) : r;
+}
+__device__ __forceinline__ double sse_sub(double a, double b) {
+    const double r = a - b;
+    return is_nan(r) ? sse_fix(r, a, b) : r;
+}
+
+// Newton coefficients COEFFS[eff][k] (linear.rs:75)
+__device__ __forceinline__ double lin_coef(uint32_t eff, uint32_t k) {
+    if (k == 0) return (double)eff;
+    if (k == 1) return eff == 2 ? -1.0 : eff == 3 ? -3.0 : -6.0;
+    if (k == 2) return eff == 3 ? 1.0 : 4.0;
+    return -1.0;
+}
+
+// predict_f64 (linear.rs:83-94) step by step with the SSE2 NaN rule
+__device__ __noinline__ double pred_sse(uint32_t eff, double h0, double h1, double h2, double h3) {
+    const double h[4] = {h0, h1, h2, h3};
+    double p = 0.0;
+    for (uint32_t k = 0; k < eff; k++) {
+        const double c = lin_coef(eff, k);
+        const double m = sse_fix(c * h[k], c, h[k]);
+        p = sse_fix(p + m, p, m);
+    }
+    return p;
+}
+
+// predict_f64: pred = 0.0; pred += c_k h_k for k < eff, h0 the latest value.
+// Without a NaN anywhere the hardware steps are the SSE2 steps; NaN is
+// absorbing, so a non-NaN result proves no step needed the rule.
+__device__ __forceinline__ double lin_pred(uint32_t eff, double h0, double h1, double h2,
+                                           double h3) {
+    if (eff == 0) return 0.0;
+    double p = 0.0 + (double)eff * h0;
+    if (eff >= 2) p = p + lin_coef(eff, 1) * h1;
+    if (eff >= 3) p = p + lin_coef(eff, 2) * h2;
+    if (eff >= 4) p = p - h3;
+    if (__builtin_expect(is_nan(p), 0)) p = pred_sse(eff, h0, h1, h2, h3);
+    return p;
+}
+
+// the block of a batch owning global tile t: the last b with tile0 <= t
+__device__ __forceinline__ uint32_t owner_block(const LinD *d, uint32_t nblk, uint64_t t) {
+    uint32_t lo = 0, hi = nblk;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (d[mid].tile0 <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// workgroup exclusive scan of one value per thread (256 threads); *total gets
+// the sum. Ends with a barrier, so LDS may be reused after it.
+__device__ __forceinline__ uint64_t wg_excl_scan(uint64_t v, uint64_t *total) {
+    __shared__ uint64_t ws[LE_WG / 64 + 1];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t inc = qa_wave_scan_add64(v);
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t s = 0;
+        for (uint32_t k = 0; k < LE_WG / 64; k++) {
+            const uint64_t x = ws[k];
+            ws[k] = s;
+            s += x;
+        }
+        ws[LE_WG / 64] = s;
+    }
+    __syncthreads();
+    const uint64_t r = inc - v + ws[w];
+    *total = ws[LE_WG / 64];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// encode
+// ---------------------------------------------------------------------------
+
+// select_order (linear.rs:125-143) and the per-block result slot. One
+// workgroup per block.
+template <uint32_t KIND>
+__global__ void __launch_bounds__(256) k_lin_order(const LinD *d, LinR *r) {
+    __shared__ double df[4][1024 + 2];  // padded: the four summing lanes hit distinct banks
+    __shared__ double mag[4];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    const uint64_t n = d[b].n;
+    uint32_t order = d[b].order;
+    if (KIND != LIN_OD && order == 0) {
+        if (n < 8) {
+            order = 2;
+        } else {
+            const uint32_t win = n < 1024 ? (uint32_t)n : 1024u;
+            const uint64_t *v = d[b].vals + (n - win) / 2;
+            // diff_in_place (linear.rs:105-111): d[i] <- d[i+1] - d[i]
+            for (uint32_t i = tid; i + 1 < win; i += 256) df[0][i] = b2d(v[i + 1]) - b2d(v[i]);
+            __syncthreads();
+            for (uint32_t o = 1; o < 4; o++) {
+                for (uint32_t i = tid; i + 1 + o < win; i += 256)
+                    df[o][i] = df[o - 1][i + 1] - df[o - 1][i];
+                __syncthreads();
+            }
+            if (tid < 4) {
+                // mean_abs (linear.rs:113-119): a left-to-right sum, then / len
+                const uint32_t len = win - 1 - tid;
+                double s = 0.0;
+#pragma unroll 8
+                for (uint32_t i = 0; i < win - 1; i++) {
+                    const double x = fabs(df[tid][i]);
+                    s = i < len ? s + x : s;
+                }
+                mag[tid] = s / (double)len;
+            }
+            __syncthreads();
+            order = 1;
+            double best = __builtin_inf();
+            for (uint32_t o = 0; o < 4; o++) {
+                const double m = mag[o];
+                if (isfinite(m) && m < best) {
+                    best = m;
+                    order = o + 1;
+                }
+            }
+        }
+    }
+    if (tid == 0) {
+        LinR x;
+        x.len = 0;
+        x.status = 0;
+        x.order = KIND == LIN_OD ? 0u : order;
+        if (n == 0) {
+            x.len = KIND == LIN_D2 ? 1 : 0;  // Delta2 of nothing is [order]
+            x.status = KIND == LIN_DP ? 1 : 0;  // dp_encode returns None (linear.rs:192-194)
+        }
+        r[b] = x;
+        if (KIND == LIN_D2 || (KIND == LIN_DP && n > 0)) d[b].dst[0] = (uint8_t)order;
+    }
+}
+
+// Residuals of one 2048-value tile. WRITE = false: tile byte counts into
+// tsz[t] (and DeltaDp's verification); WRITE = true: tsz holds their exclusive
+// prefix over the batch and the bytes are written.
+template <uint32_t KIND, bool WRITE>
+__global__ void __launch_bounds__(LE_WG) k_lin_enc(const LinD *d, LinR *r, uint32_t nblk,
+                                                   uint64_t *tsz) {
+    __shared__ uint64_t sv[LE_TILE + 8];  // sv[k]: value s0 - 5 + k
+    const uint64_t t = blockIdx.x;
+    const uint32_t b = owner_block(d, nblk, t);
+    if (KIND == LIN_DP && WRITE && r[b].status) return;  // the block bailed
+    const uint64_t *vals = d[b].vals;
+    const uint64_t n = d[b].n, tile0 = d[b].tile0;
+    const uint64_t s0 = (t - tile0) * LE_TILE;
+    const uint32_t cnt = (uint32_t)min((uint64_t)LE_TILE, n - s0);
+    const uint32_t order = r[b].order;
+    for (uint32_t k = threadIdx.x; k < cnt + 5; k += LE_WG) {
+        const int64_t i = (int64_t)s0 - 5 + (int64_t)k;
+        sv[k] = i >= 0 ? vals[i] : 0;
+    }
+    __syncthreads();
+
+    const uint32_t j0 = threadIdx.x * LE_PER;
+    uint64_t res[LE_PER];
+    uint64_t len = 0;
+    bool bad = false;
+    if (j0 < cnt) {
+        uint64_t w[LE_PER + 5];
+#pragma unroll
+        for (int m = 0; m < LE_PER + 5; m++) w[m] = sv[j0 + m];
+        const uint64_t ib = s0 + j0;  // block index of w[5]
+        if (KIND == LIN_OD) {
+#pragma unroll
+            for (int m = 0; m < LE_PER; m++) {
+                const uint64_t i = ib + m;
+                const uint64_t pred = i == 0 ? 0 : i == 1 ? w[m + 4] : 2 * w[m + 4] - w[m + 3];
+                res[m] = qa_zigzag(w[m + 5] - pred);
+            }
+        } else {
+            uint64_t prev = 0;
+            if (KIND == LIN_DP && ib > 0) {  // residual of the value before, for the chain
+                const uint64_t i = ib - 1;
+                const uint32_t eff = i < 4 ? min(order, (uint32_t)i) : order;
+                const double p = lin_pred(eff, b2d(w[3]), b2d(w[2]), b2d(w[1]), b2d(w[0]));
+                prev = d2b(sse_sub(b2d(w[4]), p));
+            }
+#pragma unroll
+            for (int m = 0; m < LE_PER; m++) {
+                const uint64_t i = ib + m;
+                const uint32_t eff = i < 4 ? min(order, (uint32_t)i) : order;
+                const double p =
+                    lin_pred(eff, b2d(w[m + 4]), b2d(w[m + 3]), b2d(w[m + 2]), b2d(w[m + 1]));
+                if (KIND == LIN_D2) {
+                    res[m] = w[m + 5] ^ d2b(p);
+                } else {
+                    const double rr = sse_sub(b2d(w[m + 5]), p);
+                    const bool ok = d2b(sse_add(p, rr)) == w[m + 5];
+                    bad |= (j0 + m < cnt) && !ok;
+                    res[m] = d2b(rr) ^ prev;
+                    prev = d2b(rr);
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < LE_PER; m++)
+            if (j0 + m < cnt) len += qa_varint_len(res[m]);
+    }
+
+    if (!WRITE) {
+        if (KIND == LIN_DP) {
+            const uint64_t any = __ballot(bad);
+            if (any && (threadIdx.x & 63) == 0) atomicOr(&r[b].status, 1);
+        }
+        uint64_t tot;
+        wg_excl_scan(len, &tot);
+        if (threadIdx.x == 0) tsz[t] = tot;
+        return;
+    } else {
+        __shared__ __align__(16) uint8_t stg[LE_TILE * 10 + 32];
+        uint64_t tot;
+        const uint64_t ex = wg_excl_scan(len, &tot);
+        const uint64_t hdr = KIND == LIN_OD ? 0 : 1;
+        uint8_t *g0 = d[b].dst + hdr + (tsz[t] - tsz[tile0]);
+        const uint32_t a0 = (uint32_t)((uintptr_t)g0 & 15);
+        if (j0 < cnt) {
+            uint32_t p = a0 + (uint32_t)ex;
+#pragma unroll
+            for (int m = 0; m < LE_PER; m++) {
+                if (j0 + m >= cnt) break;
+                uint64_t v = res[m];
+                while (v >= 0x80) {
+                    stg[p++] = (uint8_t)(v | 0x80);
+                    v >>= 7;
+                }
+                stg[p++] = (uint8_t)v;
+            }
+        }
+        __syncthreads();
+        uint8_t *gb = g0 - a0;  // 16-byte aligned
+        const uint32_t end = a0 + (uint32_t)tot;
+        const uint32_t nch = (end + 15) / 16;
+        for (uint32_t c = threadIdx.x; c < nch; c += LE_WG) {
+            const uint32_t lo = c * 16, hi = lo + 16;
+            if (lo >= a0 && hi <= end) {
+                *(uint4 *)(gb + lo) = *(const uint4 *)(stg + lo);
+            } else {
+                for (uint32_t q = max(lo, a0); q < min(hi, end); q++) gb[q] = stg[q];
+            }
+        }
+        if (t == tile0 && threadIdx.x == 0) {
+            const uint64_t nt = (n + LE_TILE - 1) / LE_TILE;
+            r[b].len = tsz[tile0 + nt] - tsz[tile0] + hdr;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// decode
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t content_len(const LinD &x, uint64_t hdr) {
+    return x.len > hdr ? x.len - hdr : 0;
+}
+
+// terminator bytes (high bit clear) of each 4096-byte tile
+template <uint32_t KIND>
+__global__ void __launch_bounds__(LD_WG) k_lin_vcount(const LinD *d, uint32_t nblk, uint64_t *cnt) {
+    const uint64_t t = blockIdx.x;
+    const uint32_t b = owner_block(d, nblk, t);
+    const uint64_t hdr = KIND == LIN_OD ? 0 : 1;
+    const uint8_t *c = d[b].src + hdr;
+    const uint64_t b0 = (t - d[b].tile0) * LD_TILE;
+    const uint32_t m = (uint32_t)min((uint64_t)LD_TILE, content_len(d[b], hdr) - b0);
+    uint64_t k = 0;
+    for (uint32_t q = threadIdx.x; q < m; q += LD_WG) k += c[b0 + q] < 0x80;
+    uint64_t tot;
+    wg_excl_scan(k, &tot);
+    if (threadIdx.x == 0) cnt[t] = tot;
+}
+
+// every terminator assembles its varint (read_u64, varint.rs:24-39) from the
+// bytes before it and stores it at its rank: out[j] = residual word j
+template <uint32_t KIND>
+__global__ void __launch_bounds__(LD_WG) k_lin_vparse(const LinD *d, LinR *r, uint32_t nblk,
+                                                      const uint64_t *pre) {
+    __shared__ __align__(16) uint8_t sb[16 + LD_TILE];  // sb[16 + q]: content byte b0 + q
+    const uint64_t t = blockIdx.x;
+    const uint32_t b = owner_block(d, nblk, t);
+    const uint64_t hdr = KIND == LIN_OD ? 0 : 1;
+    const uint8_t *c = d[b].src + hdr;
+    const uint64_t n = d[b].n, tile0 = d[b].tile0;
+    uint64_t *out = d[b].out;
+    const uint64_t b0 = (t - tile0) * LD_TILE;
+    const uint32_t m = (uint32_t)min((uint64_t)LD_TILE, content_len(d[b], hdr) - b0);
+    for (uint32_t q = threadIdx.x; q < m; q += LD_WG) sb[16 + q] = c[b0 + q];
+    if (threadIdx.x < 16) {
+        const int64_t g = (int64_t)b0 - 16 + (int64_t)threadIdx.x;
+        sb[threadIdx.x] = g >= 0 ? c[g] : 0;
+    }
+    __syncthreads();
+    const uint32_t q0 = threadIdx.x * LD_PER;
+    uint32_t term = 0;  // bit k: byte q0 + k ends a varint
+    if (q0 < m) {
+        const uint4 u = *(const uint4 *)(sb + 16 + q0);
+        const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            if (q0 + k < m && !((wv[k >> 2] >> (8 * (k & 3) + 7)) & 1)) term |= 1u << k;
+    }
+    uint64_t tot;
+    uint64_t j = wg_excl_scan((uint64_t)__popc(term), &tot) + (pre[t] - pre[tile0]);
+    bool bad = false;
+    while (term) {
+        const uint32_t k = __ffs(term) - 1;
+        term &= term - 1;
+        const int32_t q = (int32_t)(q0 + k);
+        // continuation bytes before the terminator, back to the content start
+        uint32_t cn = 0;
+        while (cn < 10 && (int64_t)b0 + q - (int64_t)cn - 1 >= 0 && sb[16 + q - cn - 1] >= 0x80) cn++;
+        if (cn == 10) {
+            bad = true;  // leb128 overflow (varint.rs:35-37)
+        } else {
+            uint64_t x = 0;
+            for (uint32_t i = 0; i <= cn; i++)
+                x |= (uint64_t)(sb[16 + q - cn + i] & 0x7f) << (7 * i);
+            if (j < n) out[j] = x;
+        }
+        j++;
+    }
+    if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(&r[b].status, 1);
+}
+
+// per block: the order byte, the varint count and the last byte
+template <uint32_t KIND>
+__global__ void __launch_bounds__(64) k_lin_dcheck(const LinD *d, LinR *r, uint32_t nblk,
+                                                   const uint64_t *pre) {
+    const uint32_t b = blockIdx.x * 64 + threadIdx.x;
+    if (b >= nblk) return;
+    const uint64_t hdr = KIND == LIN_OD ? 0 : 1;
+    const uint8_t *src = d[b].src;
+    const uint64_t len = d[b].len, n = d[b].n, tile0 = d[b].tile0;
+    int32_t bad = 0;
+    uint32_t order = 0;
+    if (KIND != LIN_OD) {
+        if (len == 0) {
+            bad = 1;  // Truncated (linear.rs:162)
+        } else {
+            order = src[0];
+            if (order < 1 || order > 4) bad = 1;  // linear.rs:164-166
+        }
+    }
+    const uint64_t cl = content_len(d[b], hdr);
+    const uint64_t nt = (cl + LD_TILE - 1) / LD_TILE;
+    if (pre[tile0 + nt] - pre[tile0] != n) bad = 1;  // too few varints, or trailing bytes
+    if (cl && src[hdr + cl - 1] >= 0x80) bad = 1;     // an unfinished varint
+    if (bad) atomicOr(&r[b].status, 1);
+    r[b].order = order;
+}
+
+// OrderedDelta (linear.rs:47-66): with e_i = unzigzag(x_i), a_0 = 0 and
+// a_k = e_1 + ... + e_k, v_i = e_0 + a_0 + ... + a_i (all wrapping). One
+// workgroup per block, 4096 values a round.
+__global__ void __launch_bounds__(LE_WG) k_lin_od(const LinD *d, const LinR *r) {
+    const uint32_t b = blockIdx.x;
+    if (r[b].status) return;
+    const uint64_t n = d[b].n;
+    uint64_t *out = d[b].out;
+    if (n == 0) return;
+    const uint64_t e0 = qa_unzigzag(out[0]);
+    __syncthreads();  // every thread has read out[0] before it is rewritten
+    uint64_t ca = 0, cv = e0;  // a and v at the end of the previous round
+    for (uint64_t c0 = 0; c0 < n; c0 += LE_WG * 16) {
+        const uint64_t i0 = c0 + threadIdx.x * 16;
+        uint64_t x[16];
+        uint64_t s = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint64_t i = i0 + k;
+            const uint64_t e = (i < n && i > 0) ? qa_unzigzag(out[i]) : 0;
+            s += e;
+            x[k] = s;  // local inclusive prefix of e
+        }
+        uint64_t tot1;
+        const uint64_t ea = wg_excl_scan(s, &tot1) + ca;
+        uint64_t s2 = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint64_t a = i0 + k < n ? x[k] + ea : 0;
+            s2 += a;
+            x[k] = s2;  // local inclusive prefix of a
+        }
+        uint64_t tot2;
+        const uint64_t ev = wg_excl_scan(s2, &tot2) + cv;
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            if (i0 + k < n) out[i0 + k] = x[k] + ev;
+        ca += tot1;
+        cv += tot2;
+    }
+}
+
+// Delta2 (linear.rs:161-180) and DeltaDp (linear.rs:214-236): the
+// recurrence over a block's residual words, one lane per block.
+template <uint32_t KIND>
+__global__ void __launch_bounds__(64) k_lin_seq(const LinD *d, const LinR *r, uint32_t nblk) {
+    const uint32_t b = blockIdx.x * 64 + threadIdx.x;
+    if (b >= nblk || r[b].status) return;
+    const uint64_t n = d[b].n;
+    const uint32_t order = r[b].order;
+    uint64_t *out = d[b].out;
+    double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
+    uint64_t prev = 0;
+    auto step = [&](uint64_t x, uint32_t eff) -> uint64_t {
+        const double p = lin_pred(eff, h0, h1, h2, h3);
+        uint64_t bits;
+        double v;
+        if (KIND == LIN_D2) {
+            bits = x ^ d2b(p);
+            v = b2d(bits);
+        } else {
+            const uint64_t rb = x ^ prev;
+            prev = rb;
+            v = sse_add(p, b2d(rb));
+            bits = d2b(v);
+        }
+        h3 = h2;
+        h2 = h1;
+        h1 = h0;
+        h0 = v;
+        return bits;
+    };
+    const uint64_t nw = n < 4 ? n : 4;
+    for (uint64_t i = 0; i < nw; i++) out[i] = step(out[i], min(order, (uint32_t)i));
+    // the residual words are loaded SEQ_AHEAD values before they are used, so
+    // memory latency overlaps the f64 dependency chain
+    constexpr int SEQ_AHEAD = 32;
+    uint64_t i = 4;
+    uint64_t x[SEQ_AHEAD];
+#pragma unroll
+    for (int k = 0; k < SEQ_AHEAD; k++) x[k] = i + k < n ? out[i + k] : 0;
+    for (; i + SEQ_AHEAD <= n; i += SEQ_AHEAD) {
+#pragma unroll
+        for (int k = 0; k < SEQ_AHEAD; k++) {
+            out[i + k] = step(x[k], order);
+            x[k] = i + SEQ_AHEAD + k < n ? out[i + SEQ_AHEAD + k] : 0;
+        }
+    }
+    for (int k = 0; k < SEQ_AHEAD; k++)
+        if (i + k < n) out[i + k] = step(x[k], order);
+}
+
+template <uint32_t KIND>
+void enc_launch(const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles, uint64_t *tsz,
+                uint64_t *tpre, uint64_t *tmp, hipStream_t s) {
+    hipLaunchKernelGGL(k_lin_order<KIND>, dim3(nblk), dim3(256), 0, s, d, r);
+    if (!ntiles) return;
+    hipLaunchKernelGGL((k_lin_enc<KIND, false>), dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, r,
+                       nblk, tsz);
+    qa_exclusive_scan(tsz, ntiles, tpre, tmp, s);
+    hipLaunchKernelGGL((k_lin_enc<KIND, true>), dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, r,
+                       nblk, tpre);
+}
+
+template <uint32_t KIND>
+void dec_launch(const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles, uint64_t *cnt,
+                uint64_t *pre, uint64_t *tmp, hipStream_t s, void *kev0, void *kev1) {
+    if (ntiles)
+        hipLaunchKernelGGL(k_lin_vcount<KIND>, dim3((unsigned)ntiles), dim3(LD_WG), 0, s, d, nblk,
+                           cnt);
+    qa_exclusive_scan(cnt, ntiles, pre, tmp, s);
+    if (ntiles)
+        hipLaunchKernelGGL(k_lin_vparse<KIND>, dim3((unsigned)ntiles), dim3(LD_WG), 0, s, d, r,
+                           nblk, pre);
+    const unsigned g64 = (nblk + 63) / 64;
+    hipLaunchKernelGGL(k_lin_dcheck<KIND>, dim3(g64), dim3(64), 0, s, d, r, nblk, pre);
+    if (kev0) (void)hipEventRecord((hipEvent_t)kev0, s);
+    if constexpr (KIND == LIN_OD)
+        hipLaunchKernelGGL(k_lin_od, dim3(nblk), dim3(LE_WG), 0, s, d, r);
+    else
+        hipLaunchKernelGGL(k_lin_seq<KIND>, dim3(g64), dim3(64), 0, s, d, r, nblk);
+    if (kev1) (void)hipEventRecord((hipEvent_t)kev1, s);
+}
+
+}  // namespace
+
+uint64_t qa_lin_enc_tiles(uint64_t n) { return (n + LE_TILE - 1) / LE_TILE; }
+uint64_t qa_lin_dec_tiles(uint64_t bytes) { return (bytes + LD_TILE - 1) / LD_TILE; }
+
+void qa_lin_encode_launch(uint32_t kind, const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles,
+                          uint64_t *tsz, uint64_t *tpre, uint64_t *tmp, hipStream_t s) {
+    if (kind == LIN_OD) enc_launch<LIN_OD>(d, r, nblk, ntiles, tsz, tpre, tmp, s);
+    else if (kind == LIN_D2) enc_launch<LIN_D2>(d, r, nblk, ntiles, tsz, tpre, tmp, s);
+    else enc_launch<LIN_DP>(d, r, nblk, ntiles, tsz, tpre, tmp, s);
+}
+
+void qa_lin_decode_launch(uint32_t kind, const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles,
+                          uint64_t *cnt, uint64_t *pre, uint64_t *tmp, hipStream_t s, void *kev0,
+                          void *kev1) {
+    if (kind == LIN_OD) dec_launch<LIN_OD>(d, r, nblk, ntiles, cnt, pre, tmp, s, kev0, kev1);
+    else if (kind == LIN_D2) dec_launch<LIN_D2>(d, r, nblk, ntiles, cnt, pre, tmp, s, kev0, kev1);
+    else dec_launch<LIN_DP>(d, r, nblk, ntiles, cnt, pre, tmp, s, kev0, kev1);
+}

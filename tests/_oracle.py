@@ -45,6 +45,12 @@ def _load():
                                       ctypes.POINTER(ctypes.c_size_t)]
     lib.orc_rans_decompress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
                                         ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_linear_select_order.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+    lib.orc_linear_encode.restype = u8p
+    lib.orc_linear_encode.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_linear_decode.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                      ctypes.c_size_t, ctypes.c_void_p]
     lib.orc_plan_blocks.restype = ctypes.c_size_t
     lib.orc_plan_blocks.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
                                     ctypes.c_void_p]
@@ -141,6 +147,35 @@ def rans_decompress(payload, max_len):
     b = ctypes.string_at(out, n.value)
     lib.orc_free(out)
     return b
+
+
+def linear_select_order(lane):
+    lane = np.ascontiguousarray(lane, dtype=np.uint64)
+    return lib.orc_linear_select_order(_ptr(lane), len(lane))
+
+
+def linear_encode(kind, lane, order=None):
+    """Residual stream of `lane` (u64 words) for kind 7 / 20 / 45; order None =
+    select_order. None when DeltaDp bails."""
+    lane = np.ascontiguousarray(lane, dtype=np.uint64)
+    if order is None:
+        order = linear_select_order(lane)
+    n = ctypes.c_size_t()
+    p = lib.orc_linear_encode(kind, _ptr(lane) if len(lane) else None, len(lane), order,
+                              ctypes.byref(n))
+    if not p:
+        return None
+    b = ctypes.string_at(p, n.value)
+    lib.orc_free(p)
+    return b
+
+
+def linear_decode(kind, payload, n):
+    """(rc, values): rc 0 on success."""
+    buf = np.frombuffer(bytes(payload) + b"\0", dtype=np.uint8)
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    rc = lib.orc_linear_decode(kind, _ptr(buf), len(payload), n, _ptr(out))
+    return rc, out[:n]
 
 
 def plan_blocks(lane, block_size=0):

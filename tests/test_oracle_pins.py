@@ -667,3 +667,174 @@ def test_rans_decode_corner_cases():
     # n above the bound
     with pytest.raises(ValueError):
         orc.rans_decompress(v(9) + v(1) + bytes([9]) + v(4096) + states, 8)
+
+
+# --- linear residual coders (codecs/linear.rs) ------------------------------
+
+OD, D2, DP = 7, 20, 45
+LIN_COEF = {1: [1], 2: [2, -1], 3: [3, -3, 1], 4: [4, -6, 4, -1]}
+
+
+def _varint(v):
+    out = bytearray()
+    while True:
+        b = v & 0x7F
+        v >>= 7
+        if v:
+            out.append(b | 0x80)
+        else:
+            out.append(b)
+            return bytes(out)
+
+
+def _f64_bits(x):
+    return struct.unpack("<Q", struct.pack("<d", float(x)))[0]
+
+
+def _bits_lane(xs):
+    return np.array([_f64_bits(x) for x in xs], dtype=np.uint64)
+
+
+def test_linear_zigzag_values():
+    # linear.rs:246-250: zigzag round trips, including negatives
+    lane = np.array([0, 1, 2, M64, M64 - 1, 1 << 63, 12345], dtype=np.uint64)
+    enc = orc.linear_encode(OD, lane)
+    rc, back = orc.linear_decode(OD, enc, len(lane))
+    assert rc == 0 and back.tolist() == lane.tolist()
+
+
+def test_linear_reference_test_shapes():
+    # linear.rs:253-261: OrderedDelta of i*3-7 (wrapping) round trips
+    lane = np.array([(i * 3 - 7) & M64 for i in range(1000)], dtype=np.uint64)
+    enc = orc.linear_encode(OD, lane)
+    assert orc.linear_decode(OD, enc, 1000)[1].tolist() == lane.tolist()
+    # hand-derived: zigzag(-7) = 13, then the first difference 3 -> 6, then
+    # every second difference is 0
+    assert enc == bytes([13, 6]) + bytes(998)
+    # linear.rs:264-273: Delta2 and DeltaDp of i*0.5 round trip at every order
+    lane = _bits_lane([i * 0.5 for i in range(1000)])
+    for order in range(1, 5):
+        e = orc.linear_encode(D2, lane, order)
+        assert orc.linear_decode(D2, e, 1000)[1].tolist() == lane.tolist()
+        d = orc.linear_encode(DP, lane, order)
+        if d is not None:
+            assert orc.linear_decode(DP, d, 1000)[1].tolist() == lane.tolist()
+
+
+def test_linear_select_order_reference_test():
+    # linear.rs:276-291: a fine sine picks order >= 3, LCG noise <= 2
+    smooth = _bits_lane([math.sin(i * 0.002) * 100.0 + i * 0.01 for i in range(2000)])
+    assert orc.linear_select_order(smooth) >= 3
+    s, noise = 1, []
+    for _ in range(2000):
+        s = lcg1(s)
+        noise.append(s >> 2)
+    assert orc.linear_select_order(np.array(noise, dtype=np.uint64)) <= 2
+
+
+def test_linear_select_order_closed_forms():
+    # exact polynomials: the d-th difference of a degree-(d-1) polynomial is 0,
+    # and only a strictly smaller mean wins, so the order is degree + 1
+    for deg, f in ((0, lambda i: 5.0), (1, lambda i: 3.0 * i - 1), (2, lambda i: 0.5 * i * i),
+                   (3, lambda i: float(i) ** 3)):
+        assert orc.linear_select_order(_bits_lane([f(i) for i in range(3000)])) == deg + 1
+    assert orc.linear_select_order(_bits_lane([1.0, 2.0, 4.0, 8.0, 3.0, 1.0, 0.5])) == 2  # n < 8
+    # a window with an infinity has no finite mean at any order: order 1
+    xs = [float(i) for i in range(64)]
+    xs[30] = math.inf
+    assert orc.linear_select_order(_bits_lane(xs)) == 1
+
+
+def _exact_pred(hist, i, order):
+    """Prediction with Python integers (every step exact for small integers)."""
+    eff = min(order, i, 4)
+    return sum(c * hist[i - 1 - k] for k, c in enumerate(LIN_COEF.get(eff, [])))
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_linear_integer_valued_known_answers(order):
+    # integer values below 2^30: every product and partial sum of the f64
+    # prediction is exact, so it equals the integer extrapolation
+    r = np.random.default_rng(order)
+    ints = r.integers(-2**30, 2**30, 700).tolist()
+    lane = _bits_lane(ints)
+    want = bytearray([order])
+    want_dp = bytearray([order])
+    prev = 0
+    for i, v in enumerate(ints):
+        p = _exact_pred(ints, i, order)
+        want += _varint(_f64_bits(v) ^ _f64_bits(p))
+        rb = _f64_bits(v - p)
+        want_dp += _varint(rb ^ prev)
+        prev = rb
+    assert orc.linear_encode(D2, lane, order) == bytes(want)
+    assert orc.linear_encode(DP, lane, order) == bytes(want_dp)
+    # OrderedDelta with Python's unbounded integers
+    raw = [_f64_bits(v) for v in ints]
+    want_od = bytearray()
+    for i, v in enumerate(raw):
+        p = 0 if i == 0 else raw[0] if i == 1 else 2 * raw[i - 1] - raw[i - 2]
+        d = (v - p) & M64
+        want_od += _varint(((d << 1) ^ (M64 if d >> 63 else 0)) & M64)
+    assert orc.linear_encode(OD, lane) == bytes(want_od)
+
+
+def test_linear_dp_parabola_known_answer():
+    # 0.5 i^2 at order 2: the residual is the second difference, exactly 1.0
+    # from i = 2 on, so the XOR chain is 0 after it
+    lane = _bits_lane([0.5 * i * i for i in range(500)])
+    want = bytes([2, 0]) + _varint(_f64_bits(0.5)) + _varint(_f64_bits(1.0) ^ _f64_bits(0.5))
+    assert orc.linear_encode(DP, lane, 2) == want + bytes(497)
+    # Delta2 of i*0.5 at order 2: v1 is predicted by v0 alone, then exactly
+    lane = _bits_lane([i * 0.5 for i in range(1000)])
+    assert orc.linear_encode(D2, lane, 2) == bytes([2, 0]) + _varint(_f64_bits(0.5)) + bytes(998)
+
+
+def test_linear_dp_bails():
+    assert orc.linear_encode(DP, np.zeros(0, dtype=np.uint64), 2) is None  # empty
+    # 1e16 + 1: v - pred rounds, pred + r does not give v back
+    lane = _bits_lane([0.0, 1e16, 1.0, 3.0])
+    assert orc.linear_encode(DP, lane, 2) is None
+    assert orc.linear_encode(D2, lane, 2) is not None
+
+
+def test_linear_nan_rules():
+    # NaN results follow x86-64 SSE2: the first NaN operand made quiet, or the
+    # default NaN 0xFFF8000000000000 for inf - inf
+    inf = _f64_bits(math.inf)
+    snan = 0x7FF0000000000001
+    lane = np.array([inf, inf, _f64_bits(1.0)], dtype=np.uint64)
+    e = orc.linear_encode(D2, lane, 2)
+    # i=2: pred = (0 + 2*inf) + (-1*inf) = inf - inf -> default NaN
+    assert e == bytes([2]) + _varint(inf) + _varint(0) + _varint(_f64_bits(1.0) ^ 0xFFF8000000000000)
+    lane = np.array([snan, _f64_bits(2.0)], dtype=np.uint64)
+    e = orc.linear_encode(D2, lane, 1)
+    # i=1: pred = 0 + 1*sNaN -> the quiet form of that NaN
+    assert e == bytes([1]) + _varint(snan) + _varint(_f64_bits(2.0) ^ (snan | 1 << 51))
+    for kind, order in ((D2, 1), (D2, 3), (DP, 1)):
+        lane = np.array([snan, _f64_bits(2.0), 0x7FF8000000000123, inf, 0, 1 << 63], dtype=np.uint64)
+        enc = orc.linear_encode(kind, lane, order)
+        if enc is not None:
+            rc, back = orc.linear_decode(kind, enc, len(lane))
+            if kind == D2:
+                assert rc == 0 and back.tolist() == lane.tolist()
+
+
+def test_linear_decode_errors():
+    lane = _bits_lane([i * 0.25 for i in range(50)])
+    for kind in (D2, DP, OD):
+        enc = orc.linear_encode(kind, lane, 3)
+        assert orc.linear_decode(kind, enc, 50)[0] == 0
+        assert orc.linear_decode(kind, enc + b"\x00", 50)[0] != 0      # trailing bytes
+        assert orc.linear_decode(kind, enc[:-1], 50)[0] != 0           # truncated
+        assert orc.linear_decode(kind, enc, 51)[0] != 0                # one value short
+    for bad in (0, 5, 255):
+        assert orc.linear_decode(D2, bytes([bad, 0]), 1)[0] != 0          # order byte
+        assert orc.linear_decode(DP, bytes([bad, 0]), 1)[0] != 0
+    assert orc.linear_decode(D2, b"", 0)[0] != 0                          # no order byte
+    assert orc.linear_decode(D2, bytes([1]), 0)[0] == 0
+    assert orc.linear_decode(OD, b"", 0)[0] == 0
+    # an over-long but terminated varint is accepted; ten continuation bytes are not
+    assert orc.linear_decode(OD, b"\x82\x80\x00", 1) [1].tolist() == [1]
+    assert orc.linear_decode(OD, b"\xff" * 9 + b"\x01", 1)[1].tolist() == [(M64 >> 1) ^ M64]
+    assert orc.linear_decode(OD, b"\x80" * 10 + b"\x00", 1)[0] != 0
