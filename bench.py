@@ -399,6 +399,152 @@ def rans_cpu_baseline(sample, sample_len):
                                       "rANS decode, oracle C, 1 thread"}
 
 
+MODE_NAMES = {1: "const", 2: "stride", 3: "xorz", 5: "raw", 9: "alp", 17: "float_mult",
+              19: "dict", 50: "for_bitpack", 51: "delta_bitpack", 52: "alp_rd", 53: "rle"}
+
+
+def float_columns(total=64 << 20, ncols=40, seed=4):
+    """The mixed-float workload: `ncols` f64 columns whose lengths are
+    exp(U[ln 65536, ln 8388608]) rescaled to sum to `total` and kept in that
+    range. Column i is, by i % 4: 0, 1 decimals k / 1000 with k uniform in
+    [0, 10^9) (scale 10^3 in [0, 1e6)); 2 full-precision uniform [0, 1) doubles
+    (u64 >> 11) 2^-53; 3 a 256-value dictionary of such decimals, written in
+    runs of Geometric(1/8) length (mean 8), each run's value uniform over the
+    256."""
+    r = np.random.default_rng(seed)
+    lo, hi = 65536, 8388608
+    L = np.exp(r.uniform(np.log(lo), np.log(hi), ncols))
+    for _ in range(50):
+        L = np.clip(L * total / L.sum(), lo, hi)
+    L = np.floor(L).astype(np.int64)
+    for i in np.argsort(L):  # the rounding remainder goes to columns with room
+        d = int(total - L.sum())
+        if d == 0:
+            break
+        L[i] = min(max(L[i] + d, lo), hi)
+    cols = []
+    for i, n in enumerate(L.tolist()):
+        kind = ("decimal", "decimal", "full", "dict")[i % 4]
+        if kind == "decimal":
+            v = r.integers(0, 10**9, n) / 1000.0
+        elif kind == "full":
+            v = (r.integers(0, 2**64, n, dtype=np.uint64) >> np.uint64(11)) * 2.0**-53
+        else:
+            vals = r.integers(0, 10**9, 256) / 1000.0
+            runs = r.geometric(1 / 8, n // 4 + 64)
+            while runs.sum() < n:
+                runs = np.concatenate([runs, r.geometric(1 / 8, n // 8 + 64)])
+            v = np.repeat(vals[r.integers(0, 256, len(runs))], runs)[:n]
+        cols.append((kind, np.ascontiguousarray(v, dtype=np.float64)))
+    return cols
+
+
+def float_bench(qa, gpu, dev, steps, warmup, check_oracle=True):
+    """The mixed-float workload (float_columns) at Level::Fast: all blocks of
+    all 40 columns in one batched compress call and one batched decompress
+    call, device-resident. The first call is checked: every column's round
+    trip in full, and the stream bytes of the shortest column of each kind
+    against the oracle. Reports GB/s of uncompressed f64, the per-mode block
+    wins, and the tile decode stage (k_dec_tiles / k_dec_fl) against 8 TB/s."""
+    import torch
+    cols = float_columns()
+    cfg = qa.Config(level=qa.Level.FAST)
+    dv, ds, do, dcols = [], [], [], []
+    raw = 0
+    for kind, v in cols:
+        t = torch.from_numpy(v.view(np.uint8)).to(dev)
+        cap = qa.lib.quoin_typed_compress_bound(int(qa.DType.F64), len(v))
+        s = torch.empty(cap, dtype=torch.uint8, device=dev)
+        dv.append(t)
+        ds.append(s)
+        do.append(torch.empty(v.nbytes, dtype=torch.uint8, device=dev))
+        dcols.append(qa.DeviceColumn(int(qa.DType.F64), 0, len(v), t.data_ptr(), None,
+                                     s.data_ptr(), cap))
+        raw += v.nbytes
+    torch.cuda.synchronize()
+    qa.reset_mode_win_counts()
+    lens = gpu.compress(dcols, cfg)
+    wins = qa.mode_win_counts()
+    host = [ds[i][:lens[i]].cpu().numpy() for i in range(len(cols))]
+    dsts = [qa.DeviceStream(int(qa.DType.F64), 0, len(v), ds[i].data_ptr(), host[i].ctypes.data,
+                            lens[i], do[i].data_ptr(), None) for i, (_, v) in enumerate(cols)]
+    gpu.decompress(dsts)
+    torch.cuda.synchronize()
+    for i in range(len(cols)):
+        assert torch.equal(do[i], dv[i]), f"mixed-float round trip mismatch on column {i}"
+    checked = []
+    if check_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as orc
+        for kind in ("decimal", "full", "dict"):
+            i = min((j for j in range(len(cols)) if cols[j][0] == kind), key=lambda j: len(cols[j][1]))
+            exp = orc.compress(cols[i][1], orc.F64, None, orc.FAST)
+            assert host[i].tobytes() == exp, f"mixed-float stream differs from the oracle on column {i}"
+            checked.append(i)
+    for _ in range(warmup):
+        gpu.compress(dcols, cfg)
+        gpu.decompress(dsts)
+    torch.cuda.synchronize()
+    dc = dd = kd = kc = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gpu.compress(dcols, cfg)
+        t, k, _ = gpu.last_timing()
+        dc += t
+        kc += k
+        gpu.decompress(dsts)
+        t, k, _ = gpu.last_timing()
+        dd += t
+        kd += k
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dc, dd, kd, kc = dc / steps, dd / steps, kd / steps, kc / steps
+    comp = int(sum(lens))
+    alg = comp + raw
+    return {
+        "workload": "mixed float: 64Mi f64 in 40 columns, lengths log-uniform 64Ki-8Mi; 20 decimal "
+                    "(scale 10^3), 10 full-precision uniform [0,1), 10 256-value dictionary runs "
+                    "(geometric, mean 8); Level::Fast, one batched launch sequence per direction",
+        "column_lengths": [len(v) for _, v in cols], "uncompressed_bytes": raw,
+        "compressed_bytes": comp, "ratio": round(raw / comp, 3),
+        "round_trip_gbs": round(raw * steps / wall / 1e9, 2),
+        "compress_gbs": round(raw / (dc * 1e-3) / 1e9, 2),
+        "decompress_gbs": round(raw / (dd * 1e-3) / 1e9, 2),
+        "device_ms_compress": round(dc, 3), "device_ms_decompress": round(dd, 3),
+        "wins": {MODE_NAMES.get(m, str(m)): int(wins[m]) for m in range(64) if wins[m]},
+        "oracle_checked_columns": checked,
+        "roofline": {"bound": "hbm", "kernel": "tile decode: k_dec_tiles + k_dec_fl",
+                     "kernel_ms": round(kd, 4), "algorithmic_bytes": alg,
+                     "achieved": round(alg / (kd * 1e-3) / 1e9, 1) if kd > 0 else 0.0,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg / (kd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if kd > 0 else 0.0,
+                     "decode_sequence_gbs": round(alg / (dd * 1e-3) / 1e9, 1)},
+        "encode_roofline": {"bound": "hbm", "kernel": "tile encode: k_emit_fl + k_emit_tiles",
+                            "kernel_ms": round(kc, 4),
+                            "achieved": round(alg / (kc * 1e-3) / 1e9, 1) if kc > 0 else 0.0,
+                            "encode_sequence_gbs": round(alg / (dc * 1e-3) / 1e9, 1)},
+    }, cols
+
+
+def float_cpu_baseline(cols, budget_values=1 << 20):
+    """Oracle compress + decompress at Level::Fast (single-threaded C) over the
+    first `budget_values` values of each kind's shortest column."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as orc
+    sample = []
+    for kind in ("decimal", "full", "dict"):
+        i = min((j for j in range(len(cols)) if cols[j][0] == kind), key=lambda j: len(cols[j][1]))
+        sample.append(cols[i][1][:budget_values])
+    t0 = time.perf_counter()
+    for v in sample:
+        orc.decompress(orc.compress(v, orc.F64, None, orc.FAST))
+    dt = time.perf_counter() - t0
+    rawb = sum(v.nbytes for v in sample)
+    return {"value": round(rawb / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"the shortest column of each kind ({rawb / 1e6:.1f} MB), compress+decompress "
+                      "at Level::Fast, oracle C, 1 thread"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -415,6 +561,7 @@ def main():
     ap.add_argument("--entropy-n", type=int, default=32 << 20,
                     help="values per column of the rANS decode leg")
     ap.add_argument("--no-resid", action="store_true", help="skip the residual coder leg")
+    ap.add_argument("--no-float", action="store_true", help="skip the mixed-float leg")
     ap.add_argument("--resid-n", type=int, default=32 << 20,
                     help="f64 values of the residual coder leg")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -570,6 +717,10 @@ def main():
     if not args.no_resid:
         resid, resid_sample = resid_bench(qa, gpu, dev, args.resid_n, 10, 2)
 
+    floatmix = float_cols = None
+    if not args.no_float:
+        floatmix, float_cols = float_bench(qa, gpu, dev, 5, 1)
+
     cpu = None
     if not args.no_cpu:
         cpu_gbs, sample_raw = cpu_baseline(cols, int(level), args.cpu_sample)
@@ -580,6 +731,8 @@ def main():
             entropy["cpu_baseline"] = rans_cpu_baseline(*ent_sample)
         if resid is not None:
             resid["cpu_baseline"] = resid_cpu_baseline(resid_sample)
+        if floatmix is not None:
+            floatmix["cpu_baseline"] = float_cpu_baseline(float_cols)
 
     out = {
         "metric": "compress+decompress GB/s (bit-exact), batched",
@@ -601,6 +754,7 @@ def main():
         "serving": serving,
         "entropy_decode": entropy,
         "residual_coders": resid,
+        "mixed_float": floatmix,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -313,6 +313,17 @@ __device__ __forceinline__ uint64_t qa_wave_scan_add64(uint64_t v) {
     return v;
 }
 
+// inclusive prefix maximum over the wavefront (lane order), unsigned
+__device__ __forceinline__ uint32_t qa_wave_scan_max32(uint32_t v) {
+    v = max(v, qa_dpp32<QA_DPP_ROW_SHR(1)>(0, v));
+    v = max(v, qa_dpp32<QA_DPP_ROW_SHR(2)>(0, v));
+    v = max(v, qa_dpp32<QA_DPP_ROW_SHR(4)>(0, v));
+    v = max(v, qa_dpp32<QA_DPP_ROW_SHR(8)>(0, v));
+    v = max(v, qa_dpp32<QA_DPP_ROW_BCAST15, 0xA>(0, v));
+    v = max(v, qa_dpp32<QA_DPP_ROW_BCAST31, 0xC>(0, v));
+    return v;
+}
+
 // index of the calling wavefront in its workgroup, known to be uniform (so
 // what it indexes is read with scalar loads and scalar address arithmetic)
 __device__ __forceinline__ int qa_wave_id() {

@@ -45,6 +45,24 @@ __device__ __forceinline__ void patch_reg(uint64_t (&x)[QV], uint32_t k, uint64_
     }
 }
 
+// the tile that wavefront index g of a tile decoder's launch takes: the
+// launch range [t0, t1), or the g-th tile of the launch's run table (g is
+// uniform, so the table is read with scalar loads)
+__device__ __forceinline__ bool dec_tile_at(const DecArgs &a, uint64_t g, uint64_t *t) {
+    if (!a.krun) {
+        *t = a.t0 + g;
+        return *t < a.t1;
+    }
+    if (g >= a.ktotal) return false;
+    uint32_t lo = 0, hi = a.nkrun;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.krun[mid].pre <= g) lo = mid; else hi = mid;
+    }
+    *t = a.krun[lo].t0 + (g - a.krun[lo].pre);
+    return true;
+}
+
 // one thread per tile: its decode descriptor
 __global__ void k_dec_desc(DecArgs a) {
     const uint64_t t = a.t0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,8 +109,8 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
     // (an error flagged before k_dec_desc left every descriptor empty; no
     // kernel between it and this one flags errors)
     const int lane = threadIdx.x & 63, wv = qa_wave_id();
-    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
-    if (t >= a.t1) return;
+    uint64_t t;
+    if (!dec_tile_at(a, (uint64_t)blockIdx.x * 4 + wv, &t)) return;
     const TileDec td = a.tdec[t];
     if (!td.mode || td.fl) return;
     const uint32_t f = td.frame;
@@ -281,22 +299,57 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
         break;
     }
     case QA_M_RLE: {
+        // run starts (prefix of the run lengths, non-decreasing, covering the
+        // frame exactly: checked by k_dec_frame_wg) -> the value at position p
+        // is the last run with start <= p (rle.rs:45-75).
         const uint64_t R = a.fx[f].card, aux = a.fx[f].aux_off;
         const uint64_t *rs = a.scr + a.frames[f].scr;
-#pragma unroll
-        for (int q = 0; q < QV; q++) {
-            uint32_t k = lane + WAVE * q;
-            uint64_t v = 0;
-            if (k < cnt && R) {
-                uint64_t p = p0 + k, lo = 0, hi = R;  // last run with start <= p
-                while (hi - lo > 1) {
-                    uint64_t mid = (lo + hi) >> 1;
-                    if (rs[mid] <= p) lo = mid; else hi = mid;
-                }
-                v = qa_ld64u(s + aux + 8 * lo);
+        // r0 = the last run with start <= p0, by a 64-ary search (rs[0] = 0)
+        uint64_t lo = 0, hi = R;
+        while (hi - lo > 1) {
+            const uint64_t span = hi - lo;
+            const uint64_t pi = lo + 1 + (span - 1) * (uint64_t)lane / WAVE;
+            const uint64_t le = __ballot(pi < hi && rs[pi] <= p0);
+            const uint32_t c = (uint32_t)__builtin_popcountll(le);
+            if (!c) {
+                hi = lo + 1;
+            } else {
+                const uint64_t plo = lo + 1 + (span - 1) * (uint64_t)(c - 1) / WAVE;
+                hi = c < WAVE ? lo + 1 + (span - 1) * (uint64_t)c / WAVE : hi;
+                lo = plo;
             }
-            x[q] = v;
         }
+        const uint64_t r0 = lo;
+        // each later run starting inside the tile marks its first position
+        // with its index - r0; a prefix maximum then gives every position its run
+        uint32_t *mark = wd;
+#pragma unroll
+        for (int q = 0; q < QV; q++) mark[lane + WAVE * q] = 0;
+        wave_lds_sync();
+        for (uint64_t c0 = r0 + 1; c0 < R; c0 += WAVE) {
+            const uint64_t j = c0 + lane;
+            const uint64_t st = j < R ? rs[j] : ~0ull;
+            const bool in = st < p0 + cnt;  // and st > p0: r0 is the last start <= p0
+            if (in) atomicMax(&mark[st - p0], (uint32_t)(j - r0));
+            if (!(__ballot(in) >> 63)) break;
+        }
+        wave_lds_sync();
+        uint32_t run[16], m = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            m = max(m, mark[16 * lane + i]);
+            run[i] = m;
+        }
+        const uint32_t before = qa_dpp32<QA_DPP_WAVE_SHR1>(0, qa_wave_scan_max32(m));
+        wave_lds_sync();  // every lane has read its marks
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t k = 16 * lane + i;
+            vv[k] = k < cnt && R ? qa_ld64u(s + aux + 8 * (r0 + max(run[i], before))) : 0;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int q = 0; q < QV; q++) x[q] = vv[lane + WAVE * q];
         break;
     }
     case QA_M_XORZ: {
@@ -352,8 +405,8 @@ __global__ void __launch_bounds__(256) k_dec_fl(DecArgs a) {
     constexpr int LW = OW == 4 ? 1088 : 2 * QA_TILE;
     __shared__ __align__(16) uint32_t lds[4][LW];
     const int lane = threadIdx.x & 63, wv = qa_wave_id();
-    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
-    if (t >= a.t1) return;
+    uint64_t t;
+    if (!dec_tile_at(a, (uint64_t)blockIdx.x * 4 + wv, &t)) return;
     const TileDec td = a.tdec[t];
     if (td.fl != OW) return;
     const uint32_t cnt = td.cnt, w = td.w;
@@ -379,19 +432,22 @@ __global__ void __launch_bounds__(256) k_dec_fl(DecArgs a) {
         for (int c = 0; c < 4; c++)
             ((uint4 *)(wd + k0))[c] = make_uint4(y[4 * c], y[4 * c + 1], y[4 * c + 2], y[4 * c + 3]);
     } else {
-        stage_words(h + 9, 128 * w, wd, lane);
+        // FoR-family sub-blocks: [w][min u64][128 w B]; ALP: [e][f][exc u16][min][w][128 w B]
+        const bool alp = td.mode == QA_M_ALP;
+        const uint64_t base = alp ? qa_ld64u(h + 4) : mn;
+        stage_words(h + (alp ? 13 : 9), 128 * w, wd, lane);
         wave_lds_sync();
         uint64_t y[16];
         if (w <= 32) {
             uint32_t v[16];
             unpack16_u32(wd, w, lane, v);
 #pragma unroll
-            for (int i = 0; i < 16; i++) y[i] = mn + v[i];
+            for (int i = 0; i < 16; i++) y[i] = base + v[i];
         } else {
             uint64_t v[16];
             unpack16_u64(wd, w, lane, v);
 #pragma unroll
-            for (int i = 0; i < 16; i++) y[i] = mn + v[i];
+            for (int i = 0; i < 16; i++) y[i] = base + v[i];
         }
         if (td.mode == QA_M_DELTA) {
             // wrapping prefix sum of the unzigzagged deltas: in-lane over the 16
@@ -407,12 +463,52 @@ __global__ void __launch_bounds__(256) k_dec_fl(DecArgs a) {
             const uint64_t off = td.tail + inc - run;
 #pragma unroll
             for (int i = 0; i < 16; i++) y[i] += off;
+        } else if (td.mode == QA_M_FLOAT_MULT) {
+            // k / scale in IEEE f64 (float_mult.rs:100-133)
+            const double sc = d_fm_scales[a.fx[td.frame].scale];
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                y[i] = (uint64_t)__double_as_longlong(__ddiv_rn((double)(int64_t)y[i], sc));
+        } else if (alp) {
+            // (d * 10^f) * 10^-e in IEEE f64, two roundings (alp.rs:168-248)
+            const double m1 = d_exp10[h[1]], m2 = d_frac10[h[0]];
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                y[i] = (uint64_t)__double_as_longlong(__dmul_rn(__dmul_rn((double)(int64_t)y[i], m1), m2));
         }
         wave_lds_sync();  // every lane is done with the staged words
 #pragma unroll
         for (int c = 0; c < 8; c++)
             ((uint4 *)(wd + 2 * k0))[c] = make_uint4((uint32_t)y[2 * c], (uint32_t)(y[2 * c] >> 32),
                                                      (uint32_t)y[2 * c + 1], (uint32_t)(y[2 * c + 1] >> 32));
+        if (alp) {
+            // exceptions in list order (alp.rs:224-244), validated sizes: chunks
+            // of 64 read in parallel, written one by one so a repeated position
+            // keeps the last
+            const uint32_t exc = h[2] | ((uint32_t)h[3] << 8);
+            const uint8_t *ep = h + 13 + 128ull * w;
+            uint64_t *vv = (uint64_t *)wd;
+            bool bad = false;
+            if (exc) wave_lds_sync();
+            for (uint32_t j0 = 0; j0 < exc; j0 += WAVE) {
+                const uint32_t j = j0 + lane;
+                uint32_t pp = 0;
+                uint64_t v = 0;
+                if (j < exc) {
+                    pp = ep[10 * j] | ((uint32_t)ep[10 * j + 1] << 8);
+                    v = qa_ld64u(ep + 10 * j + 2);
+                    bad |= pp >= cnt;
+                }
+                if (__ballot(bad)) break;
+                const uint32_t m = min((uint32_t)WAVE, exc - j0);
+                for (uint32_t i = 0; i < m; i++)
+                    if ((uint32_t)lane == i) vv[pp] = v;
+            }
+            if (__ballot(bad)) {
+                if (lane == 0) atomicOr(a.err, QA_ERR_CORRUPT);
+                return;
+            }
+        }
     }
     wave_lds_sync();
     const bool o32 = OW == 4 || td.oform == QA_OUT_U32;
@@ -462,7 +558,20 @@ void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
         else if (kind == 8) hipLaunchKernelGGL(k_dec_fl<8>, g, dim3(256), 0, s, b);
         else hipLaunchKernelGGL(k_dec_tiles, g, dim3(256), 0, s, b);
     };
-    if (a.runs) {
+    if (a.krun_all) {
+        const uint32_t kinds[3] = {4, 8, 0};
+        for (int i = 0; i < 3; i++) {
+            if (!a.kr_tot[i]) continue;
+            DecArgs b = a;
+            b.krun = a.krun_all + a.kr_off[i];
+            b.nkrun = a.kr_n[i];
+            b.ktotal = a.kr_tot[i];
+            const dim3 g(nbt(b.ktotal, 4));
+            if (kinds[i] == 4) hipLaunchKernelGGL(k_dec_fl<4>, g, dim3(256), 0, s, b);
+            else if (kinds[i] == 8) hipLaunchKernelGGL(k_dec_fl<8>, g, dim3(256), 0, s, b);
+            else hipLaunchKernelGGL(k_dec_tiles, g, dim3(256), 0, s, b);
+        }
+    } else if (a.runs) {
         for (uint32_t r = 0; r < a.nruns; r++) {
             const uint64_t t0 = a.runs[r].t0 > a.t0 ? a.runs[r].t0 : a.t0;
             const uint64_t t1 = a.runs[r].t1 < a.t1 ? a.runs[r].t1 : a.t1;

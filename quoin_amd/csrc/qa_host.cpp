@@ -72,7 +72,7 @@ struct QuoinGpu {
         wins, tilesf, blocksx, dsorted, rstart, hkeys, hcnt, rgrown, gsets, temit, tlist, dpart, dparts;
     // decode arenas
     DevBuf dcols, dstate, frames, dtiles, dtdec, dvpre, derr, dbitmaps, dvbpre, dvruns, dscan,
-        dfx, dscr, dscr2;
+        dfx, dscr, dscr2, dkrun;
     // host-API staging
     DevBuf st_in, st_valid, st_out, st_vout;
     // rANS arenas
@@ -277,8 +277,9 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
                      &g->dvpre, &g->derr,     &g->dbitmaps, &g->dscan, &g->st_in,   &g->st_valid,
                      &g->dstate, &g->dvbpre, &g->dvruns, &g->tilesf, &g->blocksx, &g->dsorted,
                      &g->rstart, &g->hkeys, &g->hcnt, &g->rgrown, &g->gsets, &g->temit, &g->tlist, &g->dpart, &g->dparts, &g->dfx,
-                     &g->dscr, &g->dscr2,
-                     &g->st_out, &g->st_vout};
+                     &g->dscr, &g->dscr2, &g->dkrun,
+                     &g->st_out, &g->st_vout, &g->rdesc, &g->rres, &g->rtabs, &g->rscr,
+                     &g->ldesc, &g->lres, &g->ltile, &g->lscan};
     for (DevBuf *b : all)
         if (b->p) hipFree(b->p);
     hipEventDestroy(g->ev0);
@@ -609,9 +610,29 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         else
             runs.push_back(TileRun{f.tile_base, f.tile_base + f.ntiles, kind, 0});
     }
-    if (runs.size() <= 32) {
+    if (runs.size() <= 3) {
         a.runs = runs.data();
         a.nruns = (uint32_t)runs.size();
+    }
+    // beyond that, each tile decoder is launched once over a device table of
+    // its runs (prefix tile count, first tile)
+    std::vector<KindRun> kr;
+    if (!a.runs && !runs.empty()) {
+        const uint32_t kinds[3] = {4, 8, 0};
+        for (int i = 0; i < 3; i++) {
+            a.kr_off[i] = (uint32_t)kr.size();
+            uint64_t tot = 0;
+            for (const TileRun &r : runs)
+                if (r.kind == kinds[i]) {
+                    kr.push_back(KindRun{tot, r.t0});
+                    tot += r.t1 - r.t0;
+                }
+            a.kr_n[i] = (uint32_t)kr.size() - a.kr_off[i];
+            a.kr_tot[i] = tot;
+        }
+        if (!ensure(g->dkrun, kr.size() * sizeof(KindRun))) return QUOIN_ERR_DEVICE;
+        hipMemcpyAsync(g->dkrun.p, kr.data(), kr.size() * sizeof(KindRun), hipMemcpyHostToDevice, s);
+        a.krun_all = dp<KindRun>(g->dkrun);
     }
     hipEventRecord(g->ev0, s);
     zero_async(g->derr.p, 8, s);
@@ -626,6 +647,7 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         qa_decode_prep_launch(a, s);
         auto one = [&](size_t f) {
             DecArgs b = a;
+            b.krun_all = nullptr;  // the frame's own tiles, by range
             b.f0 = (uint32_t)f;
             b.f1 = (uint32_t)f + 1;
             b.t0 = frames[f].tile_base;

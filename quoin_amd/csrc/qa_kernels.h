@@ -193,20 +193,34 @@ struct DecArgs {
     // host side: runs of consecutive tiles of one tile_kind, or null
     const struct TileRun *runs;
     uint32_t nruns, pad2;
+    // device table of every tile kind's runs (kinds 4, 8, 0 at kr_off[0..2]),
+    // or null; a tile decoder's launch takes one kind's slice as krun
+    const struct KindRun *krun_all;
+    uint32_t kr_off[3], kr_n[3];
+    uint64_t kr_tot[3];
+    const struct KindRun *krun;
+    uint32_t nkrun, pad3;
+    uint64_t ktotal;
 };
 
 // Which tile decoder takes a tile (from its frame's mode and the output
 // form): 4 = k_dec_fl<4> (FoR into 4-byte integers), 8 = k_dec_fl<8> (FoR and
-// Delta into 8-byte integers, Delta into 4-byte ones), 0 = k_dec_tiles.
+// Delta into 8-byte integers, Delta into 4-byte ones, FloatMult and ALP into
+// 8-byte lanes), 0 = k_dec_tiles.
 __host__ __device__ inline uint32_t tile_kind(uint32_t mode, uint32_t oform) {
     if (mode == QA_M_FOR && oform == QA_OUT_U32) return 4;
     if ((mode == QA_M_FOR || mode == QA_M_DELTA) && oform == QA_OUT_U64) return 8;
     if (mode == QA_M_DELTA && oform == QA_OUT_U32) return 8;
+    if ((mode == QA_M_FLOAT_MULT || mode == QA_M_ALP) && oform == QA_OUT_U64) return 8;
     return 0;
 }
 struct TileRun {
     uint64_t t0, t1;
     uint32_t kind, pad;
+};
+// a run of one kind's tiles in its launch: launch index `pre` is tile t0
+struct KindRun {
+    uint64_t pre, t0;
 };
 
 void qa_decode_launch(const DecArgs &a, hipStream_t s);

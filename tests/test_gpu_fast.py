@@ -91,12 +91,15 @@ def test_codec_coverage_fast(qa):
             1_000_000 + np.arange(6700, dtype=np.int64) * 3]))[: 32768],
         "dict_grown_repeats": np.repeat(r.permutation(40000).astype(np.int64) * 5, 4)[:131072],
         "alprd_normal_exc": np.concatenate([r.normal(0, 1, n - 50), r.random(50) * 1e300]),
+        # ALP with exceptions in most sub-blocks: every 97th value full precision
+        "alp_exc": np.where(np.arange(n) % 97 == 5, r.random(n) * 1e3, np.round(r.normal(1000, 50, n), 2)),
+        "alp_exc_specials": np.where(np.arange(n) % 331 == 7, np.nan, np.round(r.normal(0, 9, n), 1)),
     }
     for name, v in cases.items():
         s = enc(qa, name, v, decode=DECODE)
         seen |= modes_of(s)
     print("modes seen:", sorted(x for x in seen if x))
-    for m in ("DICT", "RLE", "ALP", "ALP_RD", "FOR_BITPACK", "DELTA_BITPACK"):
+    for m in ("DICT", "RLE", "ALP", "ALP_RD", "FOR_BITPACK", "DELTA_BITPACK", "FLOAT_MULT"):
         assert m in seen, m
 
 
@@ -107,6 +110,52 @@ def test_nulls_fast(qa):
                 list(_data.int_columns(30000, seed=4).items()):
             vv, bm = _data.with_nulls(v, density, seed=9)
             enc(qa, f"{name}+nulls{density}", vv, bm, decode=DECODE)
+
+
+def test_alp_exceptions_with_nulls_fast(qa):
+    """ALP tiles with exceptions gathered through a null bitmap."""
+    r = np.random.default_rng(11)
+    n = 50000
+    v = np.where(np.arange(n) % 89 == 3, r.random(n) * 7, np.round(r.normal(50, 5, n), 2))
+    for density in (0.01, 0.3):
+        vv, bm = _data.with_nulls(v, density, seed=12)
+        s = enc(qa, f"alp_exc+nulls{density}", vv, bm, decode=DECODE)
+        assert "ALP" in modes_of(s)
+
+
+def _varint(v):
+    out = bytearray()
+    while True:
+        b = v & 0x7F
+        v >>= 7
+        out.append(b | (0x80 if v else 0))
+        if not v:
+            return bytes(out)
+
+
+def test_rle_frame_zero_and_long_runs(qa):
+    """A hand-built RLE frame (rle.rs:45-75) with zero-length runs, runs that
+    span several 1024-value tiles and runs of 1: the device decode equals the
+    oracle's and the expansion written out directly."""
+    import struct
+    r = np.random.default_rng(13)
+    n = 100_000
+    lens = []
+    while sum(lens) < n:
+        k = r.integers(0, 4)
+        lens.append(int(0 if k == 0 else r.integers(1, 4) if k == 1 else r.integers(1, 40)
+                        if k == 2 else r.integers(1000, 5000)))
+    lens[-1] -= sum(lens) - n
+    vals = r.integers(0, 1 << 63, len(lens), dtype=np.uint64)
+    payload = (_varint(len(lens)) + vals.astype("<u8").tobytes()
+               + orc.encode_mode("FOR_BITPACK", np.array(lens, dtype=np.uint64)))
+    head = orc.compress(np.zeros(n, np.uint64), orc.U64, None, orc.FAST)[:16]
+    stream = head + struct.pack("<BII", 53, n, len(payload)) + payload
+    want = np.repeat(vals, lens)
+    _, ov, _ = orc.decompress(stream)
+    assert ov.tobytes() == want.tobytes()
+    out, _ = qa.decompress_column(stream, qa.DType.U64, n)
+    assert out.tobytes() == want.tobytes()
 
 
 def test_fixed_block_sizes_fast(qa):

@@ -70,6 +70,45 @@ def test_plain_equals_batched(qa, level, n):
         gpu.close()
 
 
+def test_many_columns_alternating_kinds(qa):
+    """A batch whose columns cycle through the tile decoders' kinds (FoR into
+    4-byte integers, Delta / FloatMult / ALP into 8-byte lanes, RLE and RAW
+    f32 through the general decoder), so each decoder takes many separate
+    runs of tiles in one launch; streams and decodes equal the oracle's."""
+    r = np.random.default_rng(21)
+    cols = []
+    for i in range(24):
+        n = int(r.integers(1, 90000)) if i % 5 else int(r.integers(1, 3000))
+        k = i % 6
+        if k == 0:
+            v = (r.integers(0, 2**32, n, dtype=np.uint64) >> np.uint64(21)).astype(np.uint32)
+            cols.append(_data.with_nulls(v, 0.05, i))
+        elif k == 1:
+            cols.append((np.cumsum(r.integers(1, 1000, n)).astype(np.int64), None))
+        elif k == 2:
+            cols.append((r.integers(0, 10**9, n) / 1000.0, None))
+        elif k == 3:
+            cols.append((np.repeat(r.integers(0, 256, n) / 8.0, r.geometric(1 / 8, n))[:n], None))
+        elif k == 4:
+            cols.append((np.round(r.normal(100, 9, n), 2), None))
+        else:
+            cols.append((((r.integers(0, 2**24, n) * 2.0**-23) - 1).astype(np.float32), None))
+    cfg = qa.Config(level=qa.Level.FAST)
+    gpu = qa.Gpu(0)
+    try:
+        print(f"case alternating kinds, {len(cols)} columns", flush=True)
+        streams = _gpu.device_compress(qa, gpu, cols, cfg)
+        for i, ((v, bm), st) in enumerate(zip(cols, streams)):
+            assert st == orc.compress(v, ODT[v.dtype], bm, orc.FAST), f"column {i}: oracle"
+        outs = _gpu.device_decompress(qa, gpu, streams, [qa.FROM_NP[v.dtype] for v, _ in cols],
+                                      [len(v) for v, _ in cols], False)
+        for i, (st, (o, _)) in enumerate(zip(streams, outs)):
+            _, want, _ = orc.decompress(st)
+            assert o.view(np.uint8).tobytes() == want.view(np.uint8).tobytes(), f"column {i}"
+    finally:
+        gpu.close()
+
+
 def test_launch_form_rejects_unknown(qa):
     gpu = qa.Gpu(0)
     try:
