@@ -142,7 +142,9 @@ struct DTileF {
     uint32_t alp_exc;
     uint8_t fm_pass;           // bit s: every value round-trips at scale s
     uint8_t alp_e, alp_f, alp_w;
-    uint8_t alp_bail, pad0, pad1, pad2;
+    uint8_t alp_bail;
+    uint8_t fm_fail;           // bit s: some value does not round-trip at scale s
+    uint8_t pad1, pad2;        // (scales in neither mask were not evaluated)
     uint64_t alp_off, rd_exc_off, rd_coff, rd_roff;  // emit offsets
     uint64_t x_off;            // XORZ byte offset of the tile
     uint64_t rle_base;         // runs starting before this tile inside its block

@@ -1077,6 +1077,19 @@ __global__ void k_tile_desc(EncArgs a) {
             } else {
                 e.mn = is_signed(col.dtype) ? (uint64_t)ts.smin : ts.umin;
             }
+        } else if (mode == QA_M_FLOAT_MULT && a.fast) {
+            // a FloatMult sub-block is a FoR sub-block of k = round(x * scale)
+            // (float_mult.rs:66-98); wider ones stay with k_emit_fast_tiles
+            const DTileF &tf = a.tilesf[t];
+            const uint32_t sc = a.blocksx[o].fm_scale;
+            if (tf.fm_w <= 32) {
+                e.mode = QA_M_FLOAT_MULT;
+                e.w = (uint8_t)tf.fm_w;
+                e.fast = 1;
+                e.dst = p + ts.off;
+                e.mn = (uint64_t)tf.kmin[sc];
+                e.scale = sc;
+            }
         }
     }
     a.temit[t] = e;
@@ -1095,7 +1108,10 @@ __global__ void k_tile_desc(EncArgs a) {
     }
 }
 
-// FoR / DeltaBitpack tiles with w <= 32 (the 32-lane FastLanes form,
+__constant__ double c_emit_fm_scales[7] = {10.0, 100.0, 1000.0, 10000.0, 100000.0, 1000000.0,
+                                           10000000.0};
+
+// FoR / DeltaBitpack / FloatMult tiles with w <= 32 (the 32-lane FastLanes form,
 // src/bitpack.rs:47-75): lane l of the 32 packs rows [16 h, 16 h + 16) (h = half
 // of the wavefront), each residual ORed into its one or two LDS words, then
 // the 128 w bytes go out. 4 KiB of LDS per wavefront, so more tiles are in
@@ -1106,7 +1122,8 @@ __device__ __forceinline__ void emit_fl_t(const TileEmit &e, uint32_t *outw, int
     const void *lp = e.lp;
     const uint32_t lk = e.lk, cnt = e.cnt;
     const uint64_t k0 = e.k0;
-    const bool delta = e.mode == QA_M_DELTA;
+    const bool delta = e.mode == QA_M_DELTA, fm = e.mode == QA_M_FLOAT_MULT;
+    const double sc = fm ? c_emit_fm_scales[e.scale] : 1.0;
     const uint64_t mn = e.mn;
     const uint32_t w = e.w;
     uint8_t *q = e.dst;
@@ -1132,7 +1149,10 @@ __device__ __forceinline__ void emit_fl_t(const TileEmit &e, uint32_t *outw, int
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const uint32_t k = (r0 + i) * 32 + l;
-            res[i] = k >= cnt ? 0u : (uint32_t)(delta ? qa_zigzag(xs[i] - ps[i]) - mn : xs[i] - mn);
+            uint64_t v = xs[i];
+            if (fm)  // k = round(x * s), half away from zero, no contraction
+                v = (uint64_t)(int64_t)round(__dmul_rn(__longlong_as_double((long long)v), sc));
+            res[i] = k >= cnt ? 0u : (uint32_t)(delta ? qa_zigzag(xs[i] - ps[i]) - mn : v - mn);
         }
     }
 #pragma unroll

@@ -1,10 +1,13 @@
 // qa_fast.hip — candidate sizing for the Level::Fast pool and the block
 // decisions (src/encoder.rs:275-469 with entropy off).
 //
-//   k_float_tiles   FloatMult per-scale checks (src/codecs/float_mult.rs:45-64)
-//                   and ALP's (e, f) search + digit range per sub-block
+//   k_float_tiles   FloatMult per-scale checks up to a tile's first passing
+//                   scale (src/codecs/float_mult.rs:45-64) and ALP's (e, f)
+//                   search + digit range per sub-block
 //                   (src/codecs/alp.rs:49-166); one wavefront per tile, FP64
 //                   with no contraction
+//   k_fm_resolve    FloatMult: the block's first scale every tile passes,
+//                   evaluating tiles only at the scales the block needs
 //   k_alprd_cut     ALP-RD cut search on the block sample (alp_rd.rs:80-107):
 //                   sorted once in LDS, every cut's histogram is a run scan
 //   k_alprd_hist    full-block left-part histogram and its top 8
@@ -65,6 +68,37 @@ __device__ __forceinline__ bool fm_k(uint64_t bits, int s, int64_t *k) {
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
 
+// FloatMult at scale s over a tile held in registers (k = lane + 64 q): every
+// value round-trips (float_mult.rs:45-64), with a wave-wide early exit on the
+// first failure; on success the k range goes to the tile record
+__device__ __forceinline__ bool fm_tile(const uint64_t (&xv)[QA_TILE / WAVE], uint32_t cnt, int s,
+                                        int lane, DTileF *tf) {
+    int64_t kmn = INT64_MAX, kmx = INT64_MIN;
+#pragma unroll
+    for (int q = 0; q < QA_TILE / WAVE; q++) {
+        if ((uint32_t)(WAVE * q) >= cnt) break;
+        const uint32_t k = lane + WAVE * q;
+        bool bad = false;
+        if (k < cnt) {
+            int64_t kk;
+            if (fm_k(xv[q], s, &kk)) {
+                kmn = kk < kmn ? kk : kmn;
+                kmx = kk > kmx ? kk : kmx;
+            } else {
+                bad = true;
+            }
+        }
+        if (wave_any(bad)) return false;
+    }
+    kmn = wave_reduce(kmn, OpMinS());
+    kmx = wave_reduce(kmx, OpMaxS());
+    if (lane == 0) {
+        tf->kmin[s] = kmn;
+        tf->kmax[s] = kmx;
+    }
+    return true;
+}
+
 // locate a tile slot: column, owner block slot, tile index inside the block
 __device__ __forceinline__ bool tile_owner(const EncArgs &a, uint64_t t, int *cp, uint32_t *op,
                                            uint32_t *btp) {
@@ -101,34 +135,23 @@ __global__ void __launch_bounds__(256) k_float_tiles(EncArgs a) {
     const uint64_t k0 = b.start + (uint64_t)bt * QA_TILE;
     const uint32_t cnt = ts.cnt;
     DTileF &tf = a.tilesf[t];
-    // FloatMult: every scale, with a wave-wide early exit on the first failure
-    uint8_t pass = 0;
+    // the tile's values in registers (k = lane + 64 q), all loads in flight at once
+    uint64_t xv[QA_TILE / WAVE];
+#pragma unroll
+    for (int q = 0; q < QA_TILE / WAVE; q++) {
+        const uint32_t k = lane + WAVE * q;
+        xv[q] = k < cnt ? lane_at(st, k0 + k) : 0;
+    }
+    // FloatMult: scales in order up to the tile's first passing one; the block
+    // needs the first scale every tile passes, and k_fm_resolve evaluates the
+    // scales a tile did not reach only where the block needs them
+    uint8_t pass = 0, fail = 0;
     for (int s = 0; s < 7; s++) {
-        int64_t kmn = INT64_MAX, kmx = INT64_MIN;
-        bool ok = true;
-        for (uint32_t kb = 0; kb < cnt && ok; kb += WAVE) {
-            uint32_t k = kb + lane;
-            bool bad = false;
-            if (k < cnt) {
-                int64_t kk;
-                if (fm_k(lane_at(st, k0 + k), s, &kk)) {
-                    kmn = kk < kmn ? kk : kmn;
-                    kmx = kk > kmx ? kk : kmx;
-                } else {
-                    bad = true;
-                }
-            }
-            if (wave_any(bad)) ok = false;
+        if (fm_tile(xv, cnt, s, lane, &tf)) {
+            pass = (uint8_t)(1u << s);
+            break;
         }
-        if (ok) {
-            pass |= (uint8_t)(1u << s);
-            kmn = wave_reduce(kmn, OpMinS());
-            kmx = wave_reduce(kmx, OpMaxS());
-            if (lane == 0) {
-                tf.kmin[s] = kmn;
-                tf.kmax[s] = kmx;
-            }
-        }
+        fail |= (uint8_t)(1u << s);
     }
     // ALP: (e, f) on a 32-value sample, 190 pairs spread over the lanes
     const uint32_t stride = cnt / 32 > 1 ? cnt / 32 : 1;
@@ -175,11 +198,14 @@ __global__ void __launch_bounds__(256) k_float_tiles(EncArgs a) {
         int64_t lo = INT64_MAX, hi = INT64_MIN;
         uint32_t first_valid = 0xFFFFFFFFu;
         int64_t fv_digit = 0;
-        for (uint32_t kb = 0; kb < cnt; kb += WAVE) {
-            uint32_t k = kb + lane;
+#pragma unroll
+        for (int q = 0; q < QA_TILE / WAVE; q++) {
+            const uint32_t kb = WAVE * q;
+            if (kb >= cnt) break;
+            const uint32_t k = kb + lane;
             bool valid = false;
             int64_t d = 0;
-            if (k < cnt) valid = alp_digit(lane_at(st, k0 + k), be, bf, &d);
+            if (k < cnt) valid = alp_digit(xv[q], be, bf, &d);
             if (k < cnt && !valid) nexc++;
             if (valid) {
                 lo = d < lo ? d : lo;
@@ -206,12 +232,74 @@ __global__ void __launch_bounds__(256) k_float_tiles(EncArgs a) {
     }
     if (lane == 0) {
         tf.fm_pass = pass;
+        tf.fm_fail = fail;
         tf.alp_e = (uint8_t)be;
         tf.alp_f = (uint8_t)bf;
         tf.alp_w = (uint8_t)w;
         tf.alp_exc = nexc;
         tf.dmin = dmin;
         tf.alp_bail = bail;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FloatMult: the block's scale (float_mult.rs:24-43, the first scale at which
+// every value round-trips). k_float_tiles left each tile's first passing scale
+// and the scales below it as failed; a scale no tile failed is the candidate,
+// and only the tiles that did not reach it are evaluated there. When every
+// tile passes the candidate, the AND of the tiles' pass masks has it as its
+// lowest bit (each lower scale failed on some tile); when every scale failed
+// on some tile, the AND is empty.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool fast_float_block(const EncArgs &a, const DBlock &b);
+
+__global__ void __launch_bounds__(256) k_fm_resolve(EncArgs a) {
+    __shared__ uint32_t sh_fail, sh_open;
+    const uint32_t slot = a.r0 + blockIdx.x;
+    if (slot >= a.r1) return;
+    const DBlock &b = a.blocks[slot];
+    if (!fast_float_block(a, b)) return;
+    const DCol &col = a.cols[b.col];
+    const DColState &st = a.cst[b.col];
+    const uint64_t tb = col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
+    const DTile *tl = a.tiles + tb;
+    DTileF *tf = a.tilesf + tb;
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
+    for (int round = 0; round < 8; round++) {
+        if (threadIdx.x == 0) {
+            sh_fail = 0;
+            sh_open = 0;
+        }
+        __syncthreads();
+        uint32_t f = 0;
+        for (uint32_t t = threadIdx.x; t < b.ntiles; t += blockDim.x) f |= tf[t].fm_fail;
+        if (f) atomicOr(&sh_fail, f);
+        __syncthreads();
+        const uint32_t fail = sh_fail & 0x7F;
+        if (fail == 0x7F) return;
+        const int s = __ffs((int)(~fail & 0x7F)) - 1;
+        for (uint32_t t = (uint32_t)wv; t < b.ntiles; t += blockDim.x / WAVE) {
+            DTileF *x = tf + t;
+            if (((x->fm_pass | x->fm_fail) >> s) & 1) continue;
+            const uint32_t cnt = tl[t].cnt;
+            const uint64_t k0 = b.start + (uint64_t)t * QA_TILE;
+            uint64_t xv[QA_TILE / WAVE];
+#pragma unroll
+            for (int q = 0; q < QA_TILE / WAVE; q++) {
+                const uint32_t k = lane + WAVE * q;
+                xv[q] = k < cnt ? lane_at(st, k0 + k) : 0;
+            }
+            const bool ok = fm_tile(xv, cnt, s, lane, x);
+            if (lane == 0) {
+                if (ok) x->fm_pass |= (uint8_t)(1u << s);
+                else x->fm_fail |= (uint8_t)(1u << s);
+                sh_open = 1;
+            }
+        }
+        __syncthreads();
+        const uint32_t open = sh_open;
+        __syncthreads();  // every thread has read the flags before they are reset
+        if (!open) return;  // every tile passes s
     }
 }
 
@@ -266,39 +354,61 @@ __global__ void __launch_bounds__(1024) k_alprd_cut(EncArgs a) {
     uint32_t np2 = 1;
     while (np2 < sn) np2 <<= 1;
     lds_bitonic_sort(s, np2 < 2 ? 2 : np2);
-    if (threadIdx.x < 24) {
-        const uint32_t left = threadIdx.x + 1, rbw = 64 - left;
+    // Every cut's left parts are sorted too, so its histogram is the runs of
+    // equal keys: wavefront w takes cuts w + 1 and w + 17. Run starts come
+    // from a ballot per 64 samples; the cost needs only the distinct count and
+    // the sum of the 8 largest counts (ties do not change the sum), kept as a
+    // wavefront-uniform sorted top 8.
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
+    for (uint32_t left = (uint32_t)wv + 1; left <= 24; left += 16) {
+        const uint32_t rbw = 64 - left;
         uint32_t top[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        uint32_t distinct = 0, run = 0;
-        uint64_t prev = 0;
-        for (uint32_t i = 0; i <= sn; i++) {
-            bool end = i == sn;
-            uint64_t key = end ? 0 : s[i] >> rbw;
-            if (i > 0 && (end || key != prev)) {
-                uint32_t v = run;  // insert the finished run into the top-8
+        uint32_t distinct = 0, open = 0;  // start of the run still open
+        for (uint32_t c = 0; c < sn; c += WAVE) {
+            const uint32_t i = c + (uint32_t)lane;
+            const bool st = i < sn && (i == 0 || (s[i] >> rbw) != (s[i - 1] >> rbw));
+            const uint64_t m = __ballot(st);
+            distinct += (uint32_t)__builtin_popcountll(m);
+            // runs completed in this chunk: from each start to the next one
+            uint32_t len = 0;
+            if (st) {
+                const uint64_t later = lane == 63 ? 0 : m & (~0ull << (lane + 1));
+                if (later) len = (uint32_t)__builtin_ctzll(later) - (uint32_t)lane;
+            }
+            uint32_t closed = 0;  // the open run, ended by this chunk's first start
+            if (m && c > 0) closed = c + (uint32_t)__builtin_ctzll(m) - open;
+            if (m) open = c + 63 - (uint32_t)__builtin_clzll(m);
+            for (int pass = 0; pass < 2; pass++) {
+                uint64_t cand = pass == 0 ? (closed > top[7] ? 1ull : 0ull) : __ballot(len > top[7]);
+                while (cand) {
+                    const int src = __builtin_ctzll(cand);
+                    cand &= cand - 1;
+                    uint32_t v = pass == 0 ? closed : (uint32_t)__builtin_amdgcn_readlane((int)len, src);
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    if (v > top[q]) {
-                        uint32_t tmp = top[q];
-                        top[q] = v;
-                        v = tmp;
+                    for (int q = 0; q < 8; q++) {
+                        const uint32_t hi = v > top[q] ? v : top[q];
+                        v = v > top[q] ? top[q] : v;
+                        top[q] = hi;
                     }
                 }
-                distinct++;
-                run = 0;
             }
-            if (!end) {
-                run++;
-                prev = key;
+        }
+        {  // the last run ends at the sample's end
+            uint32_t v = sn - open;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const uint32_t hi = v > top[q] ? v : top[q];
+                v = v > top[q] ? top[q] : v;
+                top[q] = hi;
             }
         }
         uint64_t covered = 0;
 #pragma unroll
         for (int q = 0; q < 8; q++) covered += top[q];
-        uint32_t dl = distinct < 8 ? distinct : 8;
-        uint64_t exc = sn - covered;
-        uint64_t num = (uint64_t)sn * (rbw + rd_code_bits(dl)) + exc * 96;
-        bits_of[threadIdx.x] = __ddiv_rn((double)num, (double)sn);
+        const uint32_t dl = distinct < 8 ? distinct : 8;
+        const uint64_t exc = sn - covered;
+        const uint64_t num = (uint64_t)sn * (rbw + rd_code_bits(dl)) + exc * 96;
+        if (lane == 0) bits_of[left - 1] = __ddiv_rn((double)num, (double)sn);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1371,6 +1481,7 @@ static inline unsigned nbf(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 void qa_fast_sizes_launch(const EncArgs &a, hipStream_t s) {
     if (a.fast) {
         QA_LAUNCH(k_float_tiles, dim3(nbf(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_fm_resolve, dim3(a.r1 - a.r0), dim3(256), 0, s, a);
         QA_LAUNCH(k_alprd_cut, dim3(a.r1 - a.r0), dim3(1024), 0, s, a);
         QA_LAUNCH(k_alprd_hist, dim3(a.r1 - a.r0), dim3(1024), 0, s, a);
         QA_LAUNCH(k_alprd_tiles, dim3(nbf(a.t1 - a.t0, 4)), dim3(256), 0, s, a);

@@ -135,10 +135,12 @@ __global__ void __launch_bounds__(64) k_emit_fast_tiles(EncArgs a) {
     const uint64_t t = a.t0 + blockIdx.x;
     if (t >= a.t1) return;
     {  // the emit descriptor knows the block's mode: leave after one load
+        // (k_emit_fl wrote the FloatMult tiles with w <= 32)
         const uint32_t bm = a.temit[t].bmode;
         if (bm != QA_M_XORZ && bm != QA_M_FLOAT_MULT && bm != QA_M_ALP && bm != QA_M_ALP_RD &&
             bm != QA_M_DICT)
             return;
+        if (a.temit[t].fast) return;
     }
     const DTile &ts = a.tiles[t];
     if (!ts.active) return;

@@ -20,13 +20,13 @@ struct TileEmit {
     uint64_t mn;      // FoR minimum / Delta zig-zag minimum
     uint32_t cnt;     // values in the tile
     uint8_t w;        // FastLanes width
-    uint8_t mode;     // QA_M_RAW / FOR / DELTA, 0 = written elsewhere
+    uint8_t mode;     // QA_M_RAW / FOR / DELTA / FLOAT_MULT, 0 = written elsewhere
     uint8_t lk;       // lane storage kind
     uint8_t first;    // the first tile of its block (Delta: own predecessor)
     uint32_t dtype;
     uint32_t bmode;   // mode of the block owning this tile slot (0: none, or failed column)
-    uint32_t fast;    // FoR / Delta with w <= 32: written by k_emit_fl
-    uint32_t pad;
+    uint32_t fast;    // FoR / Delta / FloatMult with w <= 32: written by k_emit_fl
+    uint32_t scale;   // FloatMult scale index
 };
 
 // Device pointers of one encode batch (all arrays sized by the host from upper

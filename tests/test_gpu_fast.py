@@ -112,6 +112,34 @@ def test_nulls_fast(qa):
             enc(qa, f"{name}+nulls{density}", vv, bm, decode=DECODE)
 
 
+def _tiles(n, makers):
+    """Tile k (1024 values) from makers[k % len(makers)]."""
+    return np.concatenate([makers[(i // 1024) % len(makers)](min(1024, n - i))
+                           for i in range(0, n, 1024)])
+
+
+def test_float_mult_tiles_with_different_first_scales(qa):
+    """FloatMult's block scale is the first scale every value passes
+    (float_mult.rs:24-43); here tiles pass first at different scales, so the
+    block's scale is above some tiles' first, and in the last case one kind of
+    tile fails every scale the other kind passes."""
+    r = np.random.default_rng(23)
+    n = 40000
+    one = lambda m: r.integers(0, 2**30, m) / 10          # first passes at 10
+    three = lambda m: r.integers(0, 2**40, m) / 1000      # 1000; ALP bails (w > 32)
+    four = lambda m: r.integers(0, 2**40, m) / 10000
+    big = lambda m: (9.2e15 + r.integers(0, 10**6, m)).astype(np.float64)  # fails at 1000+
+    small3 = lambda m: r.integers(0, 10**6, m) / 1000
+    for name, makers, scale in (("fm_1_3", [one, three], 2), ("fm_3_1_1_1", [three, one, one, one], 2),
+                                ("fm_1_1_1_4", [one, one, one, four], 3),
+                                ("fm_big_small3", [big, small3], None)):
+        v = _tiles(n, makers)
+        st = enc(qa, name, v, decode=DECODE)
+        if scale is not None:
+            fr = orc.frames(st)
+            assert all(MODES.get(m) == "FLOAT_MULT" and p[1] == scale for m, _, p in fr), name
+
+
 def test_alp_exceptions_with_nulls_fast(qa):
     """ALP tiles with exceptions gathered through a null bitmap."""
     r = np.random.default_rng(11)
