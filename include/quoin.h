@@ -276,6 +276,34 @@ int quoin_gpu_rans_compress(QuoinGpu *g, const QuoinRansInput *in, size_t n_stre
 int quoin_gpu_rans_decompress(QuoinGpu *g, const QuoinRansStream *streams, size_t n_streams,
                               int tagged, uint64_t *out_lens, int32_t *status, void *stream);
 
+/* ---- additions: the residual entropy layer (src/entropy/mod.rs) ---------- */
+
+/* Capacity the coded form of n residual bytes needs: 1 + quoin_rans_bound(n),
+ * and 7 for n = 0 (the tag and an empty range-coded stream). */
+size_t quoin_residual_bound(size_t n);
+
+/* Replaces entropy::code_residuals (src/entropy/mod.rs:121-149, with
+ * entropy_pick at mod.rs:45-78) for a batch of device byte streams: dst gets
+ * [tag] ++ coded, byte-identical to the reference at the same lambda and
+ * allow_lz. rANS is coded on the GPU; the range coder (src/entropy/rc.rs)
+ * competes on host cores when lambda < 2 (and codes the empty stream), and
+ * with allow_lz the LZ cascade (src/codecs/lz.rs) runs on host cores for
+ * streams over 64 bytes. lambda is Level::lambda (src/lib.rs:124-139: Max and
+ * High 0, Balanced 2, Fast 4, Fastest 16); allow_lz is Level::Max's
+ * (lib.rs:145-147). out_lens[i] = coded bytes. QUOIN_ERR_BUFFER_TOO_SMALL when a
+ * dst_cap is below quoin_residual_bound(n) (out_lens[i] = the bound). */
+int quoin_gpu_code_residuals(QuoinGpu *g, uint64_t lambda, int allow_lz, const QuoinRansInput *in,
+                             size_t n_streams, uint64_t *out_lens, void *stream);
+
+/* Replaces entropy::decode_residuals (src/entropy/mod.rs:151-178) for a batch:
+ * TAG_RANS streams decode on the GPU; TAG_RC and TAG_LZ streams on host cores.
+ * TAG_TANS (a legacy tag the reference still decodes but no encoder writes)
+ * gives QUOIN_ERR_UNSUPPORTED. out_lens[i] = decoded bytes; status[i]
+ * (optional) = QUOIN_OK or the stream's error (QUOIN_ERR_CORRUPT where the
+ * reference returns Err). Returns QUOIN_OK or the first stream's error. */
+int quoin_gpu_decode_residuals(QuoinGpu *g, const QuoinRansStream *streams, size_t n_streams,
+                               uint64_t *out_lens, int32_t *status, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Linear residual coders on the device (src/codecs/linear.rs)               */
 /* ------------------------------------------------------------------------ */

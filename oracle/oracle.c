@@ -827,14 +827,362 @@ int orc_rans_decompress(const uint8_t *pl, size_t len, size_t max_len, uint8_t *
     return ORC_OK;
 }
 
-/* Tag layer decode (entropy/mod.rs:153-178) restricted to TAG_RANS; range
- * coder, tANS and the LZ cascade streams are out of this oracle's scope. */
+/* ------------------------------------------------------------------------ */
+/* Range coder with the adaptive order-1 byte model (entropy/rc.rs)          */
+/* ------------------------------------------------------------------------ */
+#define RC_PROB_BITS 11
+#define RC_MOVE_BITS 5
+#define RC_TOP (1u << 24)
+
+typedef struct {
+    uint64_t low;
+    uint32_t range;
+    uint8_t cache;
+    uint64_t cache_size;
+    buf_t *out;
+} rc_enc_t;
+
+/* rc.rs:42-58 */
+static void rc_shift_low(rc_enc_t *e) {
+    if ((e->low >> 32) != 0 || e->low < 0xFF000000ull) {
+        uint8_t carry = (uint8_t)(e->low >> 32);
+        uint8_t temp = e->cache;
+        do {
+            buf_push(e->out, (uint8_t)(temp + carry));
+            temp = 0xFF;
+        } while (--e->cache_size != 0);
+        e->cache = (uint8_t)((e->low >> 24) & 0xFF);
+    }
+    e->cache_size++;
+    e->low = (e->low & 0x00FFFFFFull) << 8;
+}
+/* rc.rs:60-78 (the branch-free form there computes the same values) */
+static void rc_enc_bit(rc_enc_t *e, uint16_t *prob, uint32_t bit) {
+    uint32_t bound = (e->range >> RC_PROB_BITS) * (uint32_t)*prob;
+    if (bit) {
+        e->low += bound;
+        e->range -= bound;
+        *prob = (uint16_t)(*prob - (*prob >> RC_MOVE_BITS));
+    } else {
+        e->range = bound;
+        *prob = (uint16_t)(*prob + (((1u << RC_PROB_BITS) - *prob) >> RC_MOVE_BITS));
+    }
+    while (e->range < RC_TOP) {
+        e->range <<= 8;
+        rc_shift_low(e);
+    }
+}
+/* rc.rs:206-219: varint(len) ++ coded bytes; each byte is a bit-tree walk of
+ * 8 decisions (MSB first) under the previous byte's 256-node tree (rc.rs:162-179) */
+uint8_t *orc_rc_compress(const uint8_t *src, size_t n, size_t *len) {
+    buf_t out = {0};
+    varint_write(&out, n);
+    uint16_t *probs = (uint16_t *)malloc(256 * 256 * sizeof(uint16_t));
+    for (size_t i = 0; i < 256 * 256; i++) probs[i] = 1u << (RC_PROB_BITS - 1);
+    rc_enc_t e = {0, 0xFFFFFFFFu, 0, 1, &out};
+    uint32_t ctx = 0;
+    for (size_t i = 0; i < n; i++) {
+        uint32_t node = 1;
+        for (int b = 7; b >= 0; b--) {
+            uint32_t bit = (src[i] >> b) & 1;
+            rc_enc_bit(&e, &probs[ctx * 256 + node], bit);
+            node = (node << 1) | bit;
+        }
+        ctx = src[i];
+    }
+    for (int k = 0; k < 5; k++) rc_shift_low(&e); /* rc.rs:80-85 */
+    free(probs);
+    *len = out.len;
+    if (out.oom) {
+        buf_free(&out);
+        return NULL;
+    }
+    return out.p;
+}
+/* rc.rs:221-243 with the decoder of rc.rs:94-150: one dummy byte, four code
+ * bytes; reads past the end give 0 */
+int orc_rc_decompress(const uint8_t *src, size_t len, size_t max_len, uint8_t **outp,
+                      size_t *out_len) {
+    size_t pos = 0;
+    uint64_t n;
+    int rc = varint_read(src, len, &pos, &n);
+    if (rc) return rc;
+    if (n > max_len) return ORC_CORRUPT;
+#define RC_NEXT() (pos < len ? (uint32_t)src[pos++] : (pos++, 0u))
+    uint32_t code = 0, range = 0xFFFFFFFFu;
+    (void)RC_NEXT();
+    for (int k = 0; k < 4; k++) code = (code << 8) | RC_NEXT();
+    uint16_t *probs = (uint16_t *)malloc(256 * 256 * sizeof(uint16_t));
+    uint8_t *out = (uint8_t *)malloc(n ? n : 1);
+    if (!probs || !out) {
+        free(probs);
+        free(out);
+        return ORC_NOMEM;
+    }
+    for (size_t i = 0; i < 256 * 256; i++) probs[i] = 1u << (RC_PROB_BITS - 1);
+    uint32_t ctx = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t node = 1;
+        for (int b = 0; b < 8; b++) {
+            uint16_t *p = &probs[ctx * 256 + node];
+            uint32_t bound = (range >> RC_PROB_BITS) * (uint32_t)*p, bit;
+            if (code >= bound) {
+                code -= bound;
+                range -= bound;
+                *p = (uint16_t)(*p - (*p >> RC_MOVE_BITS));
+                bit = 1;
+            } else {
+                range = bound;
+                *p = (uint16_t)(*p + (((1u << RC_PROB_BITS) - *p) >> RC_MOVE_BITS));
+                bit = 0;
+            }
+            while (range < RC_TOP) {
+                range <<= 8;
+                code = (code << 8) | RC_NEXT();
+            }
+            node = (node << 1) | bit;
+        }
+        out[i] = (uint8_t)node;
+        ctx = node & 0xFF;
+    }
+#undef RC_NEXT
+    free(probs);
+    *outp = out;
+    *out_len = n;
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LZSS over bytes (codecs/lz.rs)                                            */
+/* ------------------------------------------------------------------------ */
+#define LZ_MIN_MATCH 4
+#define LZ_HASH_BITS 17
+#define LZ_MAX_CHAIN 64
+
+static uint32_t lz_hash4(const uint8_t *w) { /* lz.rs:19-23 */
+    uint32_t v = (uint32_t)w[0] | ((uint32_t)w[1] << 8) | ((uint32_t)w[2] << 16) |
+                 ((uint32_t)w[3] << 24);
+    return (v * 2654435761u) >> (32 - LZ_HASH_BITS);
+}
+/* lz.rs:30-113: greedy parse, hash chains of at most 64 candidates, the
+ * longest match (the first found on ties), then LZSS framing: a control byte
+ * of 8 flags (MSB first), literal bytes and (varint offset, varint len - 4) */
+uint8_t *orc_lz_compress(const uint8_t *in, size_t n, size_t *len) {
+    int64_t *head = (int64_t *)malloc(sizeof(int64_t) << LZ_HASH_BITS);
+    int64_t *prev = (int64_t *)malloc(sizeof(int64_t) * (n ? n : 1));
+    /* tokens: match length (0 = literal) and offset */
+    uint32_t *tl = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t *to = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    for (size_t h = 0; h < ((size_t)1 << LZ_HASH_BITS); h++) head[h] = -1;
+    size_t nt = 0, i = 0;
+    while (i < n) {
+        size_t best_len = 0, best_off = 0;
+        if (i + LZ_MIN_MATCH <= n) {
+            uint32_t h = lz_hash4(in + i);
+            size_t max_len = n - i;
+            int64_t cand = head[h];
+            int chain = LZ_MAX_CHAIN;
+            while (cand >= 0 && chain > 0) {
+                size_t c = (size_t)cand;
+                if (best_len == 0 || (c + best_len < n && in[c + best_len] == in[i + best_len])) {
+                    size_t l = 0;
+                    while (l < max_len && in[c + l] == in[i + l]) l++;
+                    if (l > best_len) {
+                        best_len = l;
+                        best_off = i - c;
+                        if (l == max_len) break;
+                    }
+                }
+                cand = prev[c];
+                chain--;
+            }
+        }
+        if (best_len >= LZ_MIN_MATCH) {
+            tl[nt] = (uint32_t)best_len;
+            to[nt++] = (uint32_t)best_off;
+            size_t end = i + best_len;
+            for (; i < end; i++)
+                if (i + LZ_MIN_MATCH <= n) {
+                    uint32_t h = lz_hash4(in + i);
+                    prev[i] = head[h];
+                    head[h] = (int64_t)i;
+                }
+        } else {
+            tl[nt] = 0;
+            to[nt++] = in[i];
+            if (i + LZ_MIN_MATCH <= n) {
+                uint32_t h = lz_hash4(in + i);
+                prev[i] = head[h];
+                head[h] = (int64_t)i;
+            }
+            i++;
+        }
+    }
+    buf_t out = {0};
+    varint_write(&out, n);
+    for (size_t g = 0; g < nt; g += 8) {
+        uint8_t ctrl = 0;
+        for (size_t k = 0; k < 8 && g + k < nt; k++)
+            if (tl[g + k]) ctrl |= (uint8_t)(1u << (7 - k));
+        buf_push(&out, ctrl);
+        for (size_t k = 0; k < 8 && g + k < nt; k++) {
+            if (tl[g + k]) {
+                varint_write(&out, to[g + k]);
+                varint_write(&out, tl[g + k] - LZ_MIN_MATCH);
+            } else {
+                buf_push(&out, (uint8_t)to[g + k]);
+            }
+        }
+    }
+    free(head);
+    free(prev);
+    free(tl);
+    free(to);
+    *len = out.len;
+    return out.p;
+}
+/* lz.rs:115-148. A match length that would overrun the output is corrupt
+ * (compared without wrapping). */
+int orc_lz_decompress(const uint8_t *s, size_t len, size_t expected, uint8_t **outp) {
+    size_t pos = 0;
+    uint64_t n;
+    int rc = varint_read(s, len, &pos, &n);
+    if (rc) return rc;
+    if (n != expected) return ORC_CORRUPT;
+    uint8_t *out = (uint8_t *)malloc(n ? n : 1);
+    size_t o = 0;
+    while (o < n) {
+        if (pos >= len) {
+            free(out);
+            return ORC_TRUNCATED;
+        }
+        uint8_t ctrl = s[pos++];
+        for (int k = 0; k < 8 && o < n; k++) {
+            if ((ctrl >> (7 - k)) & 1) {
+                uint64_t off, l;
+                if ((rc = varint_read(s, len, &pos, &off)) || (rc = varint_read(s, len, &pos, &l))) {
+                    free(out);
+                    return rc;
+                }
+                if (off == 0 || off > o || n - o < LZ_MIN_MATCH || l > n - o - LZ_MIN_MATCH) {
+                    free(out);
+                    return ORC_CORRUPT;
+                }
+                l += LZ_MIN_MATCH;
+                for (uint64_t j = 0; j < l; j++, o++) out[o] = out[o - off];
+            } else {
+                if (pos >= len) {
+                    free(out);
+                    return ORC_TRUNCATED;
+                }
+                out[o++] = s[pos++];
+            }
+        }
+    }
+    *outp = out;
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Entropy tag layer (entropy/mod.rs)                                        */
+/* ------------------------------------------------------------------------ */
+#define TAG_RC 0
+#define TAG_TANS 1
+#define TAG_RANS 2
+#define TAG_LZ 3
+#define W_RC 30u
+#define W_RANS 3u
+#define RC_LAMBDA_CUTOFF 2u
+
+static size_t ent_penalty(uint64_t lambda, uint64_t w, size_t n) { /* mod.rs:64 */
+    return (size_t)(sat_mul(sat_mul(lambda, w), (uint64_t)n) >> 8);
+}
+/* mod.rs:45-78: [tag] ++ coded */
+static void entropy_pick(const uint8_t *b, size_t n, uint64_t lambda, buf_t *out) {
+    size_t rl = 0;
+    uint8_t *r = orc_rans_compress(b, n, &rl);
+    if (lambda >= RC_LAMBDA_CUTOFF && r) {
+        buf_push(out, TAG_RANS);
+        buf_append(out, r, rl);
+        free(r);
+        return;
+    }
+    size_t cl = 0;
+    uint8_t *c = orc_rc_compress(b, n, &cl);
+    if (lambda < RC_LAMBDA_CUTOFF && r && rl + ent_penalty(lambda, W_RANS, n) <
+                                              cl + ent_penalty(lambda, W_RC, n)) {
+        buf_push(out, TAG_RANS);
+        buf_append(out, r, rl);
+    } else {
+        buf_push(out, TAG_RC);
+        buf_append(out, c, cl);
+    }
+    free(r);
+    free(c);
+}
+/* mod.rs:121-149 */
+uint8_t *orc_code_residuals(const uint8_t *r, size_t n, uint64_t lambda, int allow_lz,
+                            size_t *len) {
+    buf_t basic = {0};
+    entropy_pick(r, n, lambda, &basic);
+    if (allow_lz && n > 64) {
+        size_t zl = 0;
+        uint8_t *z = orc_lz_compress(r, n, &zl);
+        if (zl < n) {
+            buf_t cand = {0};
+            buf_push(&cand, TAG_LZ);
+            varint_write(&cand, n);
+            entropy_pick(z, zl, lambda, &cand);
+            if (cand.len < basic.len) {
+                free(z);
+                buf_free(&basic);
+                *len = cand.len;
+                return cand.p;
+            }
+            buf_free(&cand);
+        }
+        free(z);
+    }
+    *len = basic.len;
+    return basic.p;
+}
+/* mod.rs:151-161; tANS streams (TAG_TANS, no longer produced) are out of this
+ * oracle's scope */
+static int decode_entropy_only(const uint8_t *p, size_t len, size_t max_len, uint8_t **out,
+                               size_t *olen) {
+    if (len == 0) return ORC_TRUNCATED;
+    switch (p[0]) {
+    case TAG_RC: return orc_rc_decompress(p + 1, len - 1, max_len, out, olen);
+    case TAG_RANS: return orc_rans_decompress(p + 1, len - 1, max_len, out, olen);
+    case TAG_TANS: return ORC_UNSUPPORTED;
+    default: return ORC_CORRUPT;
+    }
+}
+/* mod.rs:163-178 */
+int orc_decode_residuals(const uint8_t *p, size_t len, size_t max_len, uint8_t **out,
+                         size_t *olen) {
+    if (len && p[0] == TAG_LZ) {
+        size_t pos = 1;
+        uint64_t orig;
+        int rc = varint_read(p, len, &pos, &orig);
+        if (rc) return rc;
+        if (orig > max_len) return ORC_CORRUPT;
+        uint64_t bound = orig > (UINT64_MAX - 64) / 2 ? UINT64_MAX : orig * 2 + 64;
+        uint8_t *z = NULL;
+        size_t zl = 0;
+        rc = decode_entropy_only(p + pos, len - pos, (size_t)bound, &z, &zl);
+        if (rc) return rc;
+        rc = orc_lz_decompress(z, zl, (size_t)orig, out);
+        free(z);
+        if (rc) return rc;
+        *olen = (size_t)orig;
+        return ORC_OK;
+    }
+    return decode_entropy_only(p, len, max_len, out, olen);
+}
 static int decode_residuals(const uint8_t *p, size_t len, size_t max_len, uint8_t **out,
                             size_t *olen) {
-    if (len == 0) return ORC_TRUNCATED;
-    if (p[0] == 2) return orc_rans_decompress(p + 1, len - 1, max_len, out, olen);
-    if (p[0] <= 3) return ORC_UNSUPPORTED;
-    return ORC_CORRUPT;
+    return orc_decode_residuals(p, len, max_len, out, olen);
 }
 
 static int fm_decode(const uint8_t *p, size_t len, size_t n, uint64_t *out) {

@@ -8,8 +8,9 @@
  * Scope: the whole Level::Fastest / Level::Fast competition (encoder.rs:275-469
  * with entropy disabled), every decoder those levels can emit, the validity
  * container (validity.rs), the frame format (format.rs), and the 4-way rANS
- * coder (entropy/rans.rs) used by the entropy levels. Levels Balanced/High/Max
- * return ORC_UNSUPPORTED (they need pco, the range coder and LZ).
+ * coder (entropy/rans.rs), the range coder (entropy/rc.rs), the LZ cascade
+ * (codecs/lz.rs) and the entropy tag layer (entropy/mod.rs). Whole streams at
+ * Levels Balanced/High/Max return ORC_UNSUPPORTED (their competition needs pco).
  */
 #ifndef QUOIN_ORACLE_H
 #define QUOIN_ORACLE_H
@@ -73,6 +74,22 @@ int orc_decode_mode(int mode, const uint8_t *payload, size_t len, size_t n, int 
 uint8_t *orc_rans_compress(const uint8_t *data, size_t n, size_t *len);
 int orc_rans_decompress(const uint8_t *payload, size_t len, size_t max_len, uint8_t **out,
                         size_t *out_len);
+
+/* Range coder with the order-1 byte model (entropy/rc.rs): varint(len) ++
+ * coded bytes. */
+uint8_t *orc_rc_compress(const uint8_t *src, size_t n, size_t *len);
+int orc_rc_decompress(const uint8_t *src, size_t len, size_t max_len, uint8_t **out,
+                      size_t *out_len);
+/* LZSS over bytes (codecs/lz.rs). decompress writes `expected` bytes. */
+uint8_t *orc_lz_compress(const uint8_t *in, size_t n, size_t *len);
+int orc_lz_decompress(const uint8_t *s, size_t len, size_t expected, uint8_t **out);
+/* Entropy tag layer (entropy/mod.rs): code_residuals picks rANS / range coder
+ * by lambda and, with allow_lz, tries the LZ cascade; decode_residuals takes
+ * TAG_RC / TAG_RANS / TAG_LZ (TAG_TANS gives ORC_UNSUPPORTED). */
+uint8_t *orc_code_residuals(const uint8_t *r, size_t n, uint64_t lambda, int allow_lz,
+                            size_t *len);
+int orc_decode_residuals(const uint8_t *p, size_t len, size_t max_len, uint8_t **out,
+                         size_t *out_len);
 
 /* Linear residual coders (codecs/linear.rs). kind = the frame mode: 7
  * OrderedDelta, 20 Delta2, 45 DeltaDp. encode: order 1..4 (ignored by

@@ -176,6 +176,14 @@ def _load():
     lib.quoin_gpu_rans_decompress.argtypes = [vp, ctypes.POINTER(RansStream), sz, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_uint64),
                                               ctypes.POINTER(ctypes.c_int32), vp]
+    lib.quoin_residual_bound.restype = sz
+    lib.quoin_residual_bound.argtypes = [sz]
+    lib.quoin_gpu_code_residuals.argtypes = [vp, ctypes.c_uint64, ctypes.c_int,
+                                             ctypes.POINTER(RansInput), sz,
+                                             ctypes.POINTER(ctypes.c_uint64), vp]
+    lib.quoin_gpu_decode_residuals.argtypes = [vp, ctypes.POINTER(RansStream), sz,
+                                               ctypes.POINTER(ctypes.c_uint64),
+                                               ctypes.POINTER(ctypes.c_int32), vp]
     lib.quoin_linear_bound.restype = ctypes.c_uint64
     lib.quoin_linear_bound.argtypes = [ctypes.c_uint64]
     lib.quoin_gpu_linear_encode.argtypes = [vp, ctypes.c_int32, ctypes.c_int32,
@@ -202,7 +210,8 @@ EXPORTS = [
     "quoin_server_create", "quoin_server_submit", "quoin_server_wait", "quoin_server_stats",
     "quoin_server_destroy", "quoin_rans_bound", "quoin_gpu_rans_compress",
     "quoin_gpu_rans_decompress", "quoin_linear_bound", "quoin_gpu_linear_encode",
-    "quoin_gpu_linear_decode",
+    "quoin_gpu_linear_decode", "quoin_residual_bound", "quoin_gpu_code_residuals",
+    "quoin_gpu_decode_residuals",
 ]
 
 
@@ -342,6 +351,31 @@ class Gpu:
         st = (ctypes.c_int32 * len(streams))()
         rc = lib.quoin_gpu_rans_decompress(self.h, arr, len(streams), 1 if tagged else 0, lens, st,
                                            stream)
+        if rc not in (OK, ERR_CORRUPT, ERR_UNSUPPORTED):
+            raise QuoinError(rc)
+        return [int(x) for x in lens], [int(x) for x in st]
+
+    def code_residuals(self, inputs, lam, allow_lz=False, stream=None):
+        """[tag] ++ coded for a batch of device byte streams
+        (src/entropy/mod.rs:121-149). inputs: list of RansInput with dst_cap >=
+        residual_bound(n). lam: Level::lambda (Max/High 0, Balanced 2, Fast 4,
+        Fastest 16); allow_lz: Level::Max. Returns the coded lengths."""
+        arr = (RansInput * len(inputs))(*inputs)
+        lens = (ctypes.c_uint64 * len(inputs))()
+        rc = lib.quoin_gpu_code_residuals(self.h, lam, 1 if allow_lz else 0, arr, len(inputs),
+                                          lens, stream)
+        if rc != OK:
+            raise QuoinError(rc)
+        return [int(x) for x in lens]
+
+    def decode_residuals(self, streams, stream=None):
+        """Inverse of code_residuals (mod.rs:151-178) for a batch of RansStream.
+        Returns (decoded lengths, per-stream status); failed streams are
+        reported, not raised."""
+        arr = (RansStream * len(streams))(*streams)
+        lens = (ctypes.c_uint64 * len(streams))()
+        st = (ctypes.c_int32 * len(streams))()
+        rc = lib.quoin_gpu_decode_residuals(self.h, arr, len(streams), lens, st, stream)
         if rc not in (OK, ERR_CORRUPT, ERR_UNSUPPORTED):
             raise QuoinError(rc)
         return [int(x) for x in lens], [int(x) for x in st]

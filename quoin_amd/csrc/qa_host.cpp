@@ -84,6 +84,11 @@ struct QuoinGpu {
     const char *last_kernel = "";
 };
 
+int qa_gpu_device(const QuoinGpu *g) { return g->device; }
+hipStream_t qa_gpu_stream(const QuoinGpu *g, void *stream) {
+    return stream ? (hipStream_t)stream : g->stream;
+}
+
 namespace {
 
 __global__ void k_zero_u64(uint64_t *p, uint64_t n) {

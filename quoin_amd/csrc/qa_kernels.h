@@ -258,6 +258,22 @@ struct RansEncR {
     uint64_t start;       // scratch offset of the states
     uint64_t out_len;
 };
+// one byte written at each of up to QA_PUT_MAX device addresses (kernel
+// arguments only: no device descriptor array)
+#define QA_PUT_MAX 120
+struct BytePuts {
+    uint8_t *p[QA_PUT_MAX];
+    uint8_t v[QA_PUT_MAX];
+    uint32_t n;
+};
+void qa_put_bytes_launch(const BytePuts &b, hipStream_t s);
+
+// the handle's device, and the stream a call runs on (the caller's or the
+// handle's own); the handle itself is defined in qa_host.cpp
+struct QuoinGpu;
+int qa_gpu_device(const QuoinGpu *g);
+hipStream_t qa_gpu_stream(const QuoinGpu *g, void *stream);
+
 void qa_rans_decode_launch(const RansDecD *d, RansDecR *r, uint32_t ns, hipStream_t s);
 void qa_rans_encode_launch(const RansEncD *d, RansEncR *r, uint64_t *tabs, uint8_t *scr, uint32_t ns,
                            hipStream_t s);

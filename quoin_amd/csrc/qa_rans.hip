@@ -670,6 +670,14 @@ __global__ void __launch_bounds__(256) k_rans_copy(const RansEncD *__restrict__ 
 
 }  // namespace
 
+__global__ void k_put_bytes(BytePuts b) {
+    const uint32_t i = threadIdx.x;
+    if (i < b.n) b.p[i][0] = b.v[i];
+}
+void qa_put_bytes_launch(const BytePuts &b, hipStream_t s) {
+    if (b.n) hipLaunchKernelGGL(k_put_bytes, dim3(1), dim3(128), 0, s, b);
+}
+
 void qa_rans_decode_launch(const RansDecD *d, RansDecR *r, uint32_t ns, hipStream_t s) {
     if (!ns) return;
     hipLaunchKernelGGL(k_rans_dec, dim3((ns + DG - 1) / DG), dim3(64), 0, s, d, r, ns);

@@ -45,6 +45,19 @@ def _load():
                                       ctypes.POINTER(ctypes.c_size_t)]
     lib.orc_rans_decompress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
                                         ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
+    for f in ("orc_rc_compress", "orc_lz_compress"):
+        getattr(lib, f).restype = u8p
+        getattr(lib, f).argtypes = [ctypes.c_void_p, ctypes.c_size_t,
+                                    ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_rc_decompress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                                      ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_lz_decompress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                                      ctypes.POINTER(u8p)]
+    lib.orc_code_residuals.restype = u8p
+    lib.orc_code_residuals.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64,
+                                       ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_decode_residuals.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                                         ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
     lib.orc_linear_select_order.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
     lib.orc_linear_encode.restype = u8p
     lib.orc_linear_encode.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
@@ -147,6 +160,71 @@ def rans_decompress(payload, max_len):
     b = ctypes.string_at(out, n.value)
     lib.orc_free(out)
     return b
+
+
+def _bytes_in(data):
+    data = np.ascontiguousarray(np.frombuffer(bytes(data), dtype=np.uint8))
+    return data, (_ptr(data) if len(data) else None)
+
+
+def _take(p, n):
+    b = ctypes.string_at(p, n)
+    lib.orc_free(p)
+    return b
+
+
+def rc_compress(data):
+    """entropy/rc.rs compress_bytes: varint(len) ++ range-coded bytes."""
+    data, dp = _bytes_in(data)
+    n = ctypes.c_size_t()
+    p = lib.orc_rc_compress(dp, len(data), ctypes.byref(n))
+    return _take(p, n.value)
+
+
+def rc_decompress(payload, max_len):
+    buf, bp = _bytes_in(payload)
+    out = ctypes.POINTER(ctypes.c_uint8)()
+    n = ctypes.c_size_t()
+    rc = lib.orc_rc_decompress(bp, len(buf), max_len, ctypes.byref(out), ctypes.byref(n))
+    if rc != 0:
+        raise ValueError(f"rc rc={rc}")
+    return _take(out, n.value)
+
+
+def lz_compress(data):
+    """codecs/lz.rs lz_compress."""
+    data, dp = _bytes_in(data)
+    n = ctypes.c_size_t()
+    p = lib.orc_lz_compress(dp, len(data), ctypes.byref(n))
+    return _take(p, n.value)
+
+
+def lz_decompress(stream, expected):
+    buf, bp = _bytes_in(stream)
+    out = ctypes.POINTER(ctypes.c_uint8)()
+    rc = lib.orc_lz_decompress(bp, len(buf), expected, ctypes.byref(out))
+    if rc != 0:
+        raise ValueError(f"lz rc={rc}")
+    return _take(out, expected)
+
+
+def code_residuals(data, lam, allow_lz):
+    """entropy/mod.rs code_residuals: [tag] ++ coded."""
+    data, dp = _bytes_in(data)
+    n = ctypes.c_size_t()
+    p = lib.orc_code_residuals(dp, len(data), lam, 1 if allow_lz else 0, ctypes.byref(n))
+    return _take(p, n.value)
+
+
+def decode_residuals(payload, max_len):
+    """entropy/mod.rs decode_residuals; raises ValueError(rc) on error."""
+    buf, bp = _bytes_in(payload)
+    out = ctypes.POINTER(ctypes.c_uint8)()
+    n = ctypes.c_size_t()
+    rc = lib.orc_decode_residuals(bp, len(buf), max_len, ctypes.byref(out), ctypes.byref(n))
+    if rc != 0:
+        raise ValueError(f"decode_residuals rc={rc}", rc)
+    return _take(out, n.value)
 
 
 def linear_select_order(lane):

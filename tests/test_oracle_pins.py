@@ -838,3 +838,173 @@ def test_linear_decode_errors():
     assert orc.linear_decode(OD, b"\x82\x80\x00", 1) [1].tolist() == [1]
     assert orc.linear_decode(OD, b"\xff" * 9 + b"\x01", 1)[1].tolist() == [(M64 >> 1) ^ M64]
     assert orc.linear_decode(OD, b"\x80" * 10 + b"\x00", 1)[0] != 0
+
+
+# ---------------------------------------------------------------------------
+# range coder (entropy/rc.rs), LZ (codecs/lz.rs) and the tag layer
+# (entropy/mod.rs)
+# ---------------------------------------------------------------------------
+def _lcg_stream(n, seed, shift=33):
+    s, out = seed, bytearray()
+    for _ in range(n):
+        s = (s * 6364136223846793005 + 1442695040888963407) & M64
+        out.append((s >> shift) & 0xFF)
+    return bytes(out)
+
+
+def test_rc_empty_stream_is_header_and_flush():
+    # varint(0), then the encoder's five flushing shift_low calls with low = 0
+    # and cache = 0 (rc.rs:42-58, 80-85): six zero bytes
+    assert orc.rc_compress(b"") == b"\x00" * 6
+    assert orc.rc_decompress(b"\x00" * 6, 0) == b""
+
+
+def test_rc_reference_unit_cases():
+    """rc.rs tests: random bytes round-trip, skewed bytes shrink and round-trip."""
+    data = _lcg_stream(50_000, 1)
+    assert orc.rc_decompress(orc.rc_compress(data), len(data)) == data
+    s, skew = 7, bytearray()
+    for _ in range(50_000):
+        s = (s * 6364136223846793005 + 1442695040888963407) & M64
+        if s % 16 == 0:
+            s = (s * 6364136223846793005 + 1442695040888963407) & M64
+            skew.append((s >> 40) & 0xFF)
+        else:
+            skew.append(0)
+    c = orc.rc_compress(bytes(skew))
+    assert len(c) < len(skew)
+    assert orc.rc_decompress(c, len(skew)) == bytes(skew)
+
+
+def _order1_ideal_bits(data):
+    """Ideal code length (bits) of `data` under the adaptive order-1 bit-tree
+    model alone (11-bit probabilities of a 0 bit starting at 1024, moved by a
+    1/32 step, context = previous byte, MSB first): an arithmetic coder's output
+    is within a few bytes of it, whatever its interval arithmetic."""
+    import math
+    probs = {}
+    ctx, bits = 0, 0.0
+    for b in data:
+        node = 1
+        for i in range(7, -1, -1):
+            bit = (b >> i) & 1
+            p = probs.get((ctx, node), 1024)
+            bits -= math.log2((p if bit == 0 else 2048 - p) / 2048)
+            probs[(ctx, node)] = p + ((2048 - p) >> 5) if bit == 0 else p - (p >> 5)
+            node = (node << 1) | bit
+        ctx = b
+    return bits
+
+
+def test_rc_size_follows_the_order1_model():
+    r = np.random.default_rng(3)
+    # order-1 structure: each byte a function of the previous one plus noise,
+    # so a wrong context or update rule moves the size well past the slack
+    prev, data = 0, bytearray()
+    for _ in range(20_000):
+        prev = (prev * 5 + 3 + int(r.integers(0, 3))) & 0xFF
+        data.append(prev)
+    for d in (bytes(data), _lcg_stream(5000, 9), b"\x07" * 3000, bytes(range(256)) * 8):
+        got = len(orc.rc_compress(d)) - len(_varint(len(d)))
+        ideal = _order1_ideal_bits(d) / 8
+        assert ideal - 1 <= got <= ideal + 6, (len(d), got, ideal)
+        assert orc.rc_decompress(orc.rc_compress(d), len(d)) == d
+
+
+def test_lz_known_streams():
+    # LZSS framing (lz.rs:93-112): varint(n), a control byte per 8 tokens
+    # (MSB first, 1 = match), literals raw, matches as varint(off), varint(len - 4)
+    assert orc.lz_compress(b"") == b"\x00"
+    assert orc.lz_compress(b"abc") == b"\x03\x00abc"
+    # greedy parse: 'a' literal, then the longest match at offset 1 runs to the end
+    assert orc.lz_compress(b"aaaaaaaa") == b"\x08\x40a\x01\x03"
+    assert orc.lz_compress(b"abcdabcdX") == b"\x09\x08abcd\x04\x00X"
+    for s in (b"", b"abc", b"aaaaaaaa", b"abcdabcdX"):
+        assert orc.lz_decompress(orc.lz_compress(s), len(s)) == s
+
+
+def test_lz_reference_unit_cases():
+    """lz.rs tests: a cycled 16-value dictionary shrinks below a quarter;
+    random words and the edge sizes round-trip."""
+    vals = np.array([float(i & 15) for i in range(50_000)]).view(np.uint64)
+    enc = orc.lz_compress(vals.tobytes())
+    assert len(enc) < vals.nbytes // 4
+    assert orc.lz_decompress(enc, vals.nbytes) == vals.tobytes()
+    s, words = 1, []
+    for _ in range(5000):
+        s = (s * 6364136223846793005 + 1) & M64
+        words.append(s)
+    raw = np.array(words, dtype=np.uint64).tobytes()
+    assert orc.lz_decompress(orc.lz_compress(raw), len(raw)) == raw
+    for w in ([], [42]):
+        raw = np.array(w, dtype=np.uint64).tobytes()
+        assert orc.lz_decompress(orc.lz_compress(raw), len(raw)) == raw
+
+
+def test_lz_decode_errors():
+    with pytest.raises(ValueError):
+        orc.lz_decompress(b"\x08\x40a\x01\x03", 7)        # length mismatch
+    with pytest.raises(ValueError):
+        orc.lz_decompress(b"\x08\x40a\x02\x03", 8)        # offset past the output
+    with pytest.raises(ValueError):
+        orc.lz_decompress(b"\x08\x40a\x00\x03", 8)        # offset 0
+    with pytest.raises(ValueError):
+        orc.lz_decompress(b"\x08\x40a\x01\x04", 8)        # match overruns n
+    with pytest.raises(ValueError):
+        orc.lz_decompress(b"\x08\x00abc", 8)              # truncated literals
+
+
+def test_code_residuals_reference_cascade_case():
+    """entropy/mod.rs cascade test: long-range repeats of a 1 KiB pseudo-random
+    block take the LZ cascade at Max and shrink below a quarter; LCG noise does
+    not; lambda 1 (no allow_lz) never does; all round-trip."""
+    block = bytes(((i * 2654435761) >> 16) & 0xFF for i in range(1024))
+    data = block * 50
+    c = orc.code_residuals(data, 0, True)
+    assert c[0] == 3 and len(c) < len(data) // 4
+    assert orc.decode_residuals(c, len(data)) == data
+    s, noise = 0x12345678, bytearray()
+    for _ in range(4000):
+        s = (s * 6364136223846793005 + 1442695040888963407) & M64
+        noise.append((s >> 33) & 0xFF)
+    c2 = orc.code_residuals(bytes(noise), 0, True)
+    assert c2[0] != 3 and orc.decode_residuals(c2, len(noise)) == bytes(noise)
+    c3 = orc.code_residuals(data, 1, False)
+    assert c3[0] != 3 and orc.decode_residuals(c3, len(data)) == data
+
+
+def test_entropy_pick_policy():
+    """mod.rs:45-78: lambda >= 2 takes rANS whenever it codes (every non-empty
+    input), the range coder only for the empty input; below 2 the range coder
+    competes with the decode-cost penalties (W_RC 30, W_RANS 3)."""
+    assert orc.code_residuals(b"", 4, False) == b"\x00" + b"\x00" * 6
+    assert orc.code_residuals(b"", 0, True) == b"\x00" + b"\x00" * 6
+    for d in (_lcg_stream(3000, 5), bytes(range(256)) * 4, b"\x01\x02" * 700, b"x"):
+        r, c = orc.rans_compress(d), orc.rc_compress(d)
+        for lam in (2, 4, 16):
+            assert orc.code_residuals(d, lam, False) == b"\x02" + r
+        for lam in (0, 1):
+            pen = lambda w: (lam * w * len(d)) >> 8
+            want = b"\x02" + r if len(r) + pen(3) < len(c) + pen(30) else b"\x00" + c
+            assert orc.code_residuals(d, lam, False) == want
+            assert orc.decode_residuals(want, len(d)) == d
+
+
+def test_decode_residuals_errors():
+    with pytest.raises(ValueError):
+        orc.decode_residuals(b"", 10)                       # no tag
+    with pytest.raises(ValueError) as e:
+        orc.decode_residuals(b"\x01\x00", 10)               # tANS: not produced, not built
+    assert e.value.args[1] == -17
+    with pytest.raises(ValueError) as e:
+        orc.decode_residuals(b"\x07\x00", 10)               # unknown tag
+    assert e.value.args[1] == -14
+    data = b"abcdefgh" * 40
+    c = orc.code_residuals(data, 0, True)
+    assert c[0] == 3
+    with pytest.raises(ValueError):
+        orc.decode_residuals(c, len(data) - 1)              # cascade length over the bound
+    with pytest.raises(ValueError):
+        orc.decode_residuals(b"\x02" + orc.rans_compress(data), len(data) - 1)
+    with pytest.raises(ValueError):
+        orc.decode_residuals(b"\x00" + orc.rc_compress(data), len(data) - 1)
