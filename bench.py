@@ -372,7 +372,10 @@ def entropy_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=3):
     alg = cbytes + nsym
     sample = [bytes(coded[coff[k]:coff[k] + lens[k]].cpu().numpy()) for k in range(0, len(spans), 64)]
     sample_len = [spans[k][1] for k in range(0, len(spans), 64)]
+    del coded, dec
+    layer = entropy_layer_bench(qa, gpu, syms.device, syms, spans)
     return {
+        "entropy_layer": layer,
         "workload": "entropy decode: 32Mi f64 sine+noise and 32Mi f32 uniform[-1,1), Delta2 "
                     "residual bytes of each 32768-value block, rANS-coded (one stream per block)",
         "streams": len(spans), "symbols": nsym, "coded_bytes": cbytes,
@@ -384,6 +387,51 @@ def entropy_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=3):
                      "achieved": round(alg / (k_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
     }, (sample, sample_len)
+
+
+def entropy_layer_bench(qa, gpu, dev, syms, spans, every=16):
+    """The residual entropy layer (src/entropy/mod.rs code_residuals /
+    decode_residuals) on every `every`-th block's Delta2 residual bytes of the
+    entropy workload, at Level::Max (lambda 0 + LZ cascade: the range coder and
+    LZ on host cores, rANS on the GPU) and Level::Balanced (lambda 2:
+    rANS-only). Decode is checked against the residuals."""
+    import torch
+    pick = list(range(0, len(spans), every))
+    caps = [int(qa.lib.quoin_residual_bound(spans[k][1])) for k in pick]
+    coff = np.concatenate([[0], np.cumsum(caps)]).tolist()
+    coded = torch.empty(coff[-1], dtype=torch.uint8, device=dev)
+    dec = torch.empty(sum(spans[k][1] for k in pick), dtype=torch.uint8, device=dev)
+    doff = np.concatenate([[0], np.cumsum([spans[k][1] for k in pick])]).tolist()
+    sp, cp, dp = syms.data_ptr(), coded.data_ptr(), dec.data_ptr()
+    ins = [qa.RansInput(sp + spans[k][0], spans[k][1], cp + coff[j], caps[j])
+           for j, k in enumerate(pick)]
+    raw = int(sum(spans[k][1] for k in pick))
+    out = {"streams": len(pick), "residual_bytes": raw, "hbm_workers": "rANS on the GPU; range "
+           "coder and LZ cascade on host cores", "levels": {}}
+    for name, lam, lz in (("max", 0, True), ("balanced", 2, False)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lens = gpu.code_residuals(ins, lam, lz)
+        torch.cuda.synchronize()
+        te = time.perf_counter() - t0
+        tags = [int(coded[coff[j]].item()) for j in range(len(pick))]
+        sts = [qa.RansStream(cp + coff[j], lens[j], spans[k][1], dp + doff[j])
+               for j, k in enumerate(pick)]
+        t0 = time.perf_counter()
+        dl, st = gpu.decode_residuals(sts)
+        torch.cuda.synchronize()
+        td = time.perf_counter() - t0
+        ok = all(x == qa.OK for x in st)
+        for j, k in enumerate(pick):
+            o, ln = spans[k]
+            ok = ok and torch.equal(dec[doff[j]:doff[j] + ln], syms[o:o + ln])
+        assert ok, f"entropy layer round trip mismatch at {name}"
+        out["levels"][name] = {
+            "coded_bytes": int(sum(lens)),
+            "tags": {t: tags.count(c) for t, c in (("rc", 0), ("rans", 2), ("lz", 3)) if c in tags},
+            "host_routed_streams": sum(1 for t in tags if t != 2),
+            "encode_gbs": round(raw / te / 1e9, 3), "decode_gbs": round(raw / td / 1e9, 3)}
+    return out
 
 
 def rans_cpu_baseline(sample, sample_len):

@@ -112,6 +112,19 @@ def test_nulls_fast(qa):
             enc(qa, f"{name}+nulls{density}", vv, bm, decode=DECODE)
 
 
+def test_plain_workload_walk(qa):
+    """The plain-method workload: 1 Mi f64 values round(100 (1000 + cumsum of
+    N(0, 1) steps)) / 100, seed 0, half away from zero; ALP takes the blocks at
+    Level::Fast. Streams equal the oracle's at both GPU levels."""
+    r = np.random.default_rng(0)
+    x = 100 * (1000 + np.cumsum(r.normal(size=1 << 20)))
+    v = np.sign(x) * np.floor(np.abs(x) + 0.5) / 100
+    s = enc(qa, "walk_1Mi_fast", v, decode=DECODE)
+    assert modes_of(s) == {"ALP"}, modes_of(s)
+    cfg = qa.Config(level=qa.Level.FASTEST)
+    assert qa.compress_column(v, None, cfg) == orc.compress(v, orc.F64, None, orc.FASTEST)
+
+
 def _tiles(n, makers):
     """Tile k (1024 values) from makers[k % len(makers)]."""
     return np.concatenate([makers[(i // 1024) % len(makers)](min(1024, n - i))
