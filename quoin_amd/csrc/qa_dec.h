@@ -161,13 +161,15 @@ __device__ __forceinline__ uint32_t select64(uint64_t w, uint32_t r) {
 __device__ __forceinline__ int out_form(const DDecCol &col) {
     return (int)qa_out_form(col.dtype, col.out_kind);
 }
+// Decoded values are written once and not read back by the decoder: stores
+// are non-temporal (streaming) so they do not displace the payloads in L2.
 template <int F>
 __device__ __forceinline__ void st_out(void *out, uint64_t i, uint64_t x) {
-    if (F == QA_OUT_U8) ((uint8_t *)out)[i] = (uint8_t)x;
-    else if (F == QA_OUT_U16) ((uint16_t *)out)[i] = (uint16_t)x;
-    else if (F == QA_OUT_U32) ((uint32_t *)out)[i] = (uint32_t)x;
-    else if (F == QA_OUT_F32) ((uint32_t *)out)[i] = qa_lane_to_f32(x);
-    else ((uint64_t *)out)[i] = x;
+    if (F == QA_OUT_U8) __builtin_nontemporal_store((uint8_t)x, (uint8_t *)out + i);
+    else if (F == QA_OUT_U16) __builtin_nontemporal_store((uint16_t)x, (uint16_t *)out + i);
+    else if (F == QA_OUT_U32) __builtin_nontemporal_store((uint32_t)x, (uint32_t *)out + i);
+    else if (F == QA_OUT_F32) __builtin_nontemporal_store(qa_lane_to_f32(x), (uint32_t *)out + i);
+    else __builtin_nontemporal_store(x, (uint64_t *)out + i);
 }
 
 // Tile values x[q] (value k = lane + 64 q) of a stream without nulls go to

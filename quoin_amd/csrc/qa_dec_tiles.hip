@@ -517,19 +517,20 @@ __global__ void __launch_bounds__(256) k_dec_fl(DecArgs a) {
             uint32_t *o = (uint32_t *)td.out + td.base + lane;
 #pragma unroll
             for (int q = 0; q < QV; q++)
-                if (lane + WAVE * q < cnt) o[WAVE * q] = wd[lane + WAVE * q];
+                if (lane + WAVE * q < cnt) __builtin_nontemporal_store(wd[lane + WAVE * q], o + WAVE * q);
         } else if (o32) {
             uint32_t *o = (uint32_t *)td.out + td.base + lane;
             const uint64_t *vv = (const uint64_t *)wd;
 #pragma unroll
             for (int q = 0; q < QV; q++)
-                if (lane + WAVE * q < cnt) o[WAVE * q] = (uint32_t)vv[lane + WAVE * q];
+                if (lane + WAVE * q < cnt)
+                    __builtin_nontemporal_store((uint32_t)vv[lane + WAVE * q], o + WAVE * q);
         } else {
             uint64_t *o = (uint64_t *)td.out + td.base + lane;
             const uint64_t *vv = (const uint64_t *)wd;
 #pragma unroll
             for (int q = 0; q < QV; q++)
-                if (lane + WAVE * q < cnt) o[WAVE * q] = vv[lane + WAVE * q];
+                if (lane + WAVE * q < cnt) __builtin_nontemporal_store(vv[lane + WAVE * q], o + WAVE * q);
         }
         return;
     }
