@@ -202,28 +202,45 @@ __device__ __forceinline__ void emit_dense(void *out, uint32_t oform, uint64_t b
 // with the positions outside [L0, L1) masked off, each lane's rank inside it
 // is one mbcnt, and the count before the window carries in a scalar register.
 // A rank past cnt (a bitmap with more set bits than values) stores zero.
-template <int F, typename V>
+// LDS positions of the tile decoders' transposed values (k_dec_fl): lane L
+// writes its values 16 L .. 16 L + 15 as 16-byte chunks, chunk c at chunk slot
+// c ^ (a few bits of L), so the lanes of one write spread over all banks. SW =
+// 4 for 4-byte values (4 chunks per lane), 8 for 8-byte values (8 chunks), 0
+// for an unswizzled array.
+template <int SW>
+__device__ __forceinline__ uint32_t lds_pos(uint32_t k) {
+    if (SW == 4) return (k & ~15u) | ((((k >> 2) & 3u) ^ ((k >> 5) & 3u)) << 2) | (k & 3u);
+    if (SW == 8) return (k & ~15u) | ((((k >> 1) & 7u) ^ ((k >> 4) & 7u)) << 1) | (k & 1u);
+    return k;
+}
+
+template <int F, typename V, int SW = 0>
 __device__ __forceinline__ void emit_gather_t(void *out, const uint64_t *bm, uint64_t L0,
                                               uint64_t L1, const V *vv, uint32_t cnt, int lane) {
     if (L1 <= L0) return;
     const uint64_t w0 = L0 >> 6, w1 = (L1 - 1) >> 6;  // inclusive word range
+    // positions relative to the first window (a tile's range spans a few
+    // thousand positions at most), so the per-window work is 32-bit
+    const uint64_t base = w0 << 6;
+    const uint32_t l0 = (uint32_t)(L0 - base), l1 = (uint32_t)(L1 - base);
+    const uint32_t nw = (uint32_t)(w1 - w0 + 1);
+    const uint64_t lbit = 1ull << lane;
     uint32_t R = 0;  // values before the window
-    for (uint64_t c0 = w0; c0 <= w1; c0 += WAVE) {
-        const uint64_t wi = c0 + (uint64_t)lane;
-        const uint64_t myw = bm[wi < w1 ? wi : w1];
-        const uint32_t nwin = (uint32_t)(w1 - c0 + 1 < (uint64_t)WAVE ? w1 - c0 + 1 : WAVE);
+    for (uint32_t c0 = 0; c0 < nw; c0 += WAVE) {
+        const uint32_t wi = c0 + (uint32_t)lane;
+        const uint64_t myw = bm[w0 + (wi < nw ? wi : nw - 1)];
+        const uint32_t nwin = nw - c0 < (uint32_t)WAVE ? nw - c0 : (uint32_t)WAVE;
         for (uint32_t q = 0; q < nwin; q++) {
-            const uint64_t p = (c0 + q) << 6;
+            const uint32_t p = (c0 + q) << 6;
             uint64_t W = qa_readlane64(myw, (int)q);
-            if (p < L0) W &= ~0ull << (L0 - p);
-            if (L1 - p < 64) W &= (1ull << (L1 - p)) - 1;
+            if (p < l0) W &= ~0ull << (l0 - p);
+            if (l1 - p < 64) W &= (1ull << (l1 - p)) - 1;
             const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(W >> 32),
                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)W, 0u));
             const uint32_t idx = R + below;
-            const bool set = (W >> lane) & 1;
-            const uint64_t x = set && idx < cnt ? (uint64_t)vv[idx] : 0;
-            const uint64_t pos = p + (uint64_t)lane;
-            if (pos >= L0 && pos < L1) st_out<F>(out, pos, x);
+            const uint64_t x = (W & lbit) && idx < cnt ? (uint64_t)vv[lds_pos<SW>(idx)] : 0;
+            const uint32_t pos = p + (uint32_t)lane;
+            if (pos >= l0 && pos < l1) st_out<F>(out, base + pos, x);
             R += (uint32_t)__popcll(W);
         }
     }

@@ -428,9 +428,10 @@ __global__ void __launch_bounds__(256) k_dec_fl(DecArgs a) {
             for (int i = 0; i < 16; i++) y[i] = k0 + i < cnt ? (uint32_t)for_value_direct(h, k0 + i) : 0u;
         }
         wave_lds_sync();  // every lane is done with the staged words
+        const uint32_t swz = ((uint32_t)lane >> 1) & 3;  // chunk slot swizzle (lds_pos<4>)
 #pragma unroll
         for (int c = 0; c < 4; c++)
-            ((uint4 *)(wd + k0))[c] = make_uint4(y[4 * c], y[4 * c + 1], y[4 * c + 2], y[4 * c + 3]);
+            ((uint4 *)(wd + k0))[c ^ swz] = make_uint4(y[4 * c], y[4 * c + 1], y[4 * c + 2], y[4 * c + 3]);
     } else {
         // FoR-family sub-blocks: [w][min u64][128 w B]; ALP: [e][f][exc u16][min][w][128 w B]
         const bool alp = td.mode == QA_M_ALP;
@@ -477,10 +478,11 @@ __global__ void __launch_bounds__(256) k_dec_fl(DecArgs a) {
                 y[i] = (uint64_t)__double_as_longlong(__dmul_rn(__dmul_rn((double)(int64_t)y[i], m1), m2));
         }
         wave_lds_sync();  // every lane is done with the staged words
+        const uint32_t swz = (uint32_t)lane & 7;  // chunk slot swizzle (lds_pos<8>)
 #pragma unroll
         for (int c = 0; c < 8; c++)
-            ((uint4 *)(wd + 2 * k0))[c] = make_uint4((uint32_t)y[2 * c], (uint32_t)(y[2 * c] >> 32),
-                                                     (uint32_t)y[2 * c + 1], (uint32_t)(y[2 * c + 1] >> 32));
+            ((uint4 *)(wd + 2 * k0))[c ^ swz] = make_uint4((uint32_t)y[2 * c], (uint32_t)(y[2 * c] >> 32),
+                                                           (uint32_t)y[2 * c + 1], (uint32_t)(y[2 * c + 1] >> 32));
         if (alp) {
             // exceptions in list order (alp.rs:224-244), validated sizes: chunks
             // of 64 read in parallel, written one by one so a repeated position
@@ -502,7 +504,7 @@ __global__ void __launch_bounds__(256) k_dec_fl(DecArgs a) {
                 if (__ballot(bad)) break;
                 const uint32_t m = min((uint32_t)WAVE, exc - j0);
                 for (uint32_t i = 0; i < m; i++)
-                    if ((uint32_t)lane == i) vv[pp] = v;
+                    if ((uint32_t)lane == i) vv[lds_pos<8>(pp)] = v;
             }
             if (__ballot(bad)) {
                 if (lane == 0) atomicOr(a.err, QA_ERR_CORRUPT);
@@ -517,27 +519,32 @@ __global__ void __launch_bounds__(256) k_dec_fl(DecArgs a) {
             uint32_t *o = (uint32_t *)td.out + td.base + lane;
 #pragma unroll
             for (int q = 0; q < QV; q++)
-                if (lane + WAVE * q < cnt) __builtin_nontemporal_store(wd[lane + WAVE * q], o + WAVE * q);
+                if (lane + WAVE * q < cnt)
+                    __builtin_nontemporal_store(wd[lds_pos<4>(lane + WAVE * q)], o + WAVE * q);
         } else if (o32) {
             uint32_t *o = (uint32_t *)td.out + td.base + lane;
             const uint64_t *vv = (const uint64_t *)wd;
 #pragma unroll
             for (int q = 0; q < QV; q++)
                 if (lane + WAVE * q < cnt)
-                    __builtin_nontemporal_store((uint32_t)vv[lane + WAVE * q], o + WAVE * q);
+                    __builtin_nontemporal_store((uint32_t)vv[lds_pos<8>(lane + WAVE * q)], o + WAVE * q);
         } else {
             uint64_t *o = (uint64_t *)td.out + td.base + lane;
             const uint64_t *vv = (const uint64_t *)wd;
 #pragma unroll
             for (int q = 0; q < QV; q++)
-                if (lane + WAVE * q < cnt) __builtin_nontemporal_store(vv[lane + WAVE * q], o + WAVE * q);
+                if (lane + WAVE * q < cnt)
+                    __builtin_nontemporal_store(vv[lds_pos<8>(lane + WAVE * q)], o + WAVE * q);
         }
         return;
     }
-    if (OW == 4) emit_gather_t<QA_OUT_U32>(td.out, td.bitmap, td.L0, td.L1, wd, cnt, lane);
+    if (OW == 4) emit_gather_t<QA_OUT_U32, uint32_t, 4>(td.out, td.bitmap, td.L0, td.L1, wd, cnt, lane);
     else if (o32)
-        emit_gather_t<QA_OUT_U32>(td.out, td.bitmap, td.L0, td.L1, (const uint64_t *)wd, cnt, lane);
-    else emit_gather_t<QA_OUT_U64>(td.out, td.bitmap, td.L0, td.L1, (const uint64_t *)wd, cnt, lane);
+        emit_gather_t<QA_OUT_U32, uint64_t, 8>(td.out, td.bitmap, td.L0, td.L1, (const uint64_t *)wd, cnt,
+                                               lane);
+    else
+        emit_gather_t<QA_OUT_U64, uint64_t, 8>(td.out, td.bitmap, td.L0, td.L1, (const uint64_t *)wd, cnt,
+                                               lane);
 }
 
 void qa_dec_desc_launch(const DecArgs &a, hipStream_t s) {
