@@ -128,7 +128,7 @@ __device__ __forceinline__ void wave_store_bytes(uint8_t *dst, const uint32_t *l
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const uint32_t i = i0 + u * WAVE + lane;
-            if (i < nw) __builtin_nontemporal_store(o[u], dw + i);  // stream bytes: not re-read here
+            if (i < nw) dw[i] = o[u];
         }
     }
     uint32_t tb = head + 4 * nw;
