@@ -296,9 +296,9 @@ int quoin_gpu_code_residuals(QuoinGpu *g, uint64_t lambda, int allow_lz, const Q
                              size_t n_streams, uint64_t *out_lens, void *stream);
 
 /* Replaces entropy::decode_residuals (src/entropy/mod.rs:151-178) for a batch:
- * TAG_RANS streams decode on the GPU; TAG_RC and TAG_LZ streams on host cores.
- * TAG_TANS (a legacy tag the reference still decodes but no encoder writes)
- * gives QUOIN_ERR_UNSUPPORTED. out_lens[i] = decoded bytes; status[i]
+ * TAG_RANS streams decode on the GPU; TAG_RC, TAG_LZ and the legacy TAG_TANS
+ * (src/entropy/tans.rs, still decoded, no longer written) on host cores.
+ * out_lens[i] = decoded bytes; status[i]
  * (optional) = QUOIN_OK or the stream's error (QUOIN_ERR_CORRUPT where the
  * reference returns Err). Returns QUOIN_OK or the first stream's error. */
 int quoin_gpu_decode_residuals(QuoinGpu *g, const QuoinRansStream *streams, size_t n_streams,

@@ -1084,6 +1084,204 @@ int orc_lz_decompress(const uint8_t *s, size_t len, size_t expected, uint8_t **o
 }
 
 /* ------------------------------------------------------------------------ */
+/* tANS (entropy/tans.rs, bit I/O of bitio.rs): the legacy TAG_TANS coder     */
+/* ------------------------------------------------------------------------ */
+#define TANS_LOG 10
+#define TANS_SIZE (1u << TANS_LOG)
+
+/* tans.rs:29-81, in f32 like the reference */
+static int tans_quantize(const uint32_t *counts, const uint8_t *which, int num, uint64_t total,
+                         uint16_t *weights) {
+    if (num == 0 || num > 256) return 0;
+    float mult = total > 0 ? (float)TANS_SIZE / (float)total : 0.0f;
+    float desired[256], total_desired = 0.0f;
+    for (int i = 0; i < num; i++) {
+        float v = (float)counts[which[i]] * mult - 1.0f;
+        desired[i] = v > 0.0f ? v : 0.0f;
+        total_desired += desired[i];
+    }
+    float surplus = total_desired > 0.0f ? ((float)TANS_SIZE - (float)num) / total_desired : 0.0f;
+    int64_t sum = 0;
+    for (int i = 0; i < num; i++) {
+        float w = 1.0f + desired[i] * surplus;
+        float r = w + 0.5f;
+        uint16_t q = r >= 65535.0f ? 65535 : (uint16_t)r;
+        weights[i] = q > 1 ? q : 1;
+        sum += weights[i];
+    }
+    int guard = num * 8 + 16, i = 0;
+    while (sum > TANS_SIZE && guard > 0) {
+        if (weights[i] > 1) {
+            weights[i]--;
+            sum--;
+        }
+        i = (i + 1) % num;
+        guard--;
+    }
+    guard = num * 8 + 16;
+    i = 0;
+    while (sum < TANS_SIZE && guard > 0) {
+        weights[i]++;
+        sum++;
+        i = (i + 1) % num;
+        guard--;
+    }
+    return sum == TANS_SIZE;
+}
+/* tans.rs:83-97 */
+static void tans_spread(const uint16_t *weights, const uint8_t *which, int num, uint8_t *table) {
+    memset(table, 0, TANS_SIZE);
+    uint32_t stride = (3 * TANS_SIZE) / 5, step = 0;
+    if ((stride & 1) == 0) stride++;
+    for (int s = 0; s < num; s++)
+        for (uint32_t k = 0; k < weights[s]; k++, step++) table[(stride * step) & (TANS_SIZE - 1)] = which[s];
+}
+/* bitio.rs BitWriter: MSB-first, a final partial byte left-aligned */
+typedef struct {
+    buf_t *out;
+    uint64_t acc;
+    uint32_t n;
+} bitw_t;
+static void bitw_put(bitw_t *w, uint64_t v, uint32_t bits) {
+    if (!bits) return;
+    w->acc = (w->acc << bits) | (v & ((1ull << bits) - 1));
+    w->n += bits;
+    while (w->n >= 8) {
+        w->n -= 8;
+        buf_push(w->out, (uint8_t)(w->acc >> w->n));
+    }
+    w->acc &= (1ull << w->n) - 1;
+}
+/* tans.rs:107-195; NULL for empty input or a model that does not quantize */
+uint8_t *orc_tans_compress(const uint8_t *src, size_t n, size_t *len) {
+    *len = 0;
+    if (n == 0) return NULL;
+    uint32_t counts[256] = {0};
+    for (size_t i = 0; i < n; i++) counts[src[i]]++;
+    uint8_t which[256];
+    int num = 0;
+    for (int b = 0; b < 256; b++)
+        if (counts[b]) which[num++] = (uint8_t)b;
+    uint16_t weights[256];
+    if (!tans_quantize(counts, which, num, n, weights)) return NULL;
+    uint8_t table[TANS_SIZE];
+    tans_spread(weights, which, num, table);
+    uint32_t cutoff[256] = {0}, mrb[256] = {0}, wt[256] = {0}, cum[256] = {0}, slot_of[256];
+    uint16_t fill[256];
+    uint16_t c = 0;
+    for (int s = 0; s < num; s++) {
+        uint32_t w = weights[s], max_xs = 2 * w - 1, lg = 0;
+        while ((max_xs >> lg) > 1) lg++;
+        mrb[which[s]] = TANS_LOG - lg;
+        cutoff[which[s]] = (w * 2) << mrb[which[s]];
+        wt[which[s]] = w;
+        cum[which[s]] = c;
+        slot_of[which[s]] = (uint32_t)s;
+        fill[s] = c;
+        c = (uint16_t)(c + w);
+    }
+    uint32_t next[TANS_SIZE];
+    for (uint32_t i = 0; i < TANS_SIZE; i++) next[fill[slot_of[table[i]]]++] = TANS_SIZE + i;
+    uint16_t *low = (uint16_t *)malloc(n * sizeof(uint16_t));
+    uint8_t *nb = (uint8_t *)malloc(n);
+    uint32_t state = TANS_SIZE;
+    for (size_t k = 0; k < n; k++) { /* reverse symbol order */
+        uint8_t sym = src[n - 1 - k];
+        uint32_t bits = mrb[sym] + (state >= cutoff[sym]);
+        low[k] = bits ? (uint16_t)(state & ((1u << bits) - 1)) : 0;
+        nb[k] = (uint8_t)bits;
+        state = next[cum[sym] + ((state >> bits) - wt[sym])];
+    }
+    buf_t out = {0};
+    varint_write(&out, n);
+    buf_push(&out, (uint8_t)(num - 1));
+    buf_append(&out, which, (size_t)num);
+    for (int s = 0; s < num; s++) buf_u16le(&out, weights[s]);
+    buf_u32le(&out, state);
+    bitw_t bw = {&out, 0, 0};
+    for (size_t k = n; k-- > 0;) bitw_put(&bw, low[k], nb[k]);
+    if (bw.n) buf_push(&out, (uint8_t)(bw.acc << (8 - bw.n)));
+    free(low);
+    free(nb);
+    *len = out.len;
+    return out.p;
+}
+/* tans.rs:197-303 with bitio.rs BitReader (reads past the end give 0) */
+int orc_tans_decompress(const uint8_t *src, size_t len, size_t max_len, uint8_t **outp,
+                        size_t *out_len) {
+    size_t pos = 0;
+    uint64_t n;
+    int rc = varint_read(src, len, &pos, &n);
+    if (rc) return rc;
+    if (n > max_len) return ORC_CORRUPT;
+    if (pos >= len) return ORC_TRUNCATED;
+    int num = src[pos++] + 1;
+    if (len - pos < (size_t)num) return ORC_TRUNCATED;
+    const uint8_t *which = src + pos;
+    pos += (size_t)num;
+    uint16_t weights[256];
+    for (int s = 0; s < num; s++) {
+        if (len - pos < 2) return ORC_TRUNCATED;
+        weights[s] = (uint16_t)(src[pos] | (src[pos + 1] << 8));
+        pos += 2;
+    }
+    if (len - pos < 4) return ORC_TRUNCATED;
+    uint32_t state = rd_u32le(src + pos);
+    pos += 4;
+    if (state < TANS_SIZE || state >= 2 * TANS_SIZE) return ORC_CORRUPT;
+    int seen[256] = {0};
+    uint32_t wsum = 0;
+    for (int s = 0; s < num; s++) {
+        if (weights[s] == 0 || seen[which[s]]) return ORC_CORRUPT;
+        seen[which[s]] = 1;
+        wsum += weights[s];
+    }
+    if (wsum != TANS_SIZE) return ORC_CORRUPT;
+    uint8_t table[TANS_SIZE];
+    tans_spread(weights, which, num, table);
+    uint32_t xs[256] = {0};
+    for (int s = 0; s < num; s++) xs[which[s]] = weights[s];
+    uint16_t nbase[TANS_SIZE];
+    uint8_t nbits[TANS_SIZE], nsym[TANS_SIZE];
+    for (uint32_t i = 0; i < TANS_SIZE; i++) {
+        uint32_t base = xs[table[i]];
+        if (base == 0) return ORC_CORRUPT;
+        uint32_t bits = (uint32_t)__builtin_clz(base) - (uint32_t)__builtin_clz(TANS_SIZE);
+        nbase[i] = (uint16_t)((base << bits) - TANS_SIZE);
+        nbits[i] = (uint8_t)bits;
+        nsym[i] = table[i];
+        xs[table[i]]++;
+    }
+    uint8_t *out = (uint8_t *)malloc(n ? n : 1);
+    uint64_t acc = 0;
+    uint32_t have = 0;
+    for (uint64_t k = 0; k < n; k++) {
+        uint32_t idx = state - TANS_SIZE;
+        if (idx >= TANS_SIZE) {
+            free(out);
+            return ORC_CORRUPT;
+        }
+        uint32_t bits = nbits[idx];
+        uint32_t extra = 0;
+        if (bits) {
+            while (have < bits) {
+                acc = (acc << 8) | (pos < len ? src[pos] : 0);
+                pos++;
+                have += 8;
+            }
+            have -= bits;
+            extra = (uint32_t)((acc >> have) & ((1ull << bits) - 1));
+            acc &= (1ull << have) - 1;
+        }
+        out[k] = nsym[idx];
+        state = TANS_SIZE + nbase[idx] + extra;
+    }
+    *outp = out;
+    *out_len = n;
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Entropy tag layer (entropy/mod.rs)                                        */
 /* ------------------------------------------------------------------------ */
 #define TAG_RC 0
@@ -1146,15 +1344,14 @@ uint8_t *orc_code_residuals(const uint8_t *r, size_t n, uint64_t lambda, int all
     *len = basic.len;
     return basic.p;
 }
-/* mod.rs:151-161; tANS streams (TAG_TANS, no longer produced) are out of this
- * oracle's scope */
+/* mod.rs:151-161 (TAG_TANS streams are no longer produced, still decoded) */
 static int decode_entropy_only(const uint8_t *p, size_t len, size_t max_len, uint8_t **out,
                                size_t *olen) {
     if (len == 0) return ORC_TRUNCATED;
     switch (p[0]) {
     case TAG_RC: return orc_rc_decompress(p + 1, len - 1, max_len, out, olen);
     case TAG_RANS: return orc_rans_decompress(p + 1, len - 1, max_len, out, olen);
-    case TAG_TANS: return ORC_UNSUPPORTED;
+    case TAG_TANS: return orc_tans_decompress(p + 1, len - 1, max_len, out, olen);
     default: return ORC_CORRUPT;
     }
 }

@@ -83,9 +83,14 @@ int orc_rc_decompress(const uint8_t *src, size_t len, size_t max_len, uint8_t **
 /* LZSS over bytes (codecs/lz.rs). decompress writes `expected` bytes. */
 uint8_t *orc_lz_compress(const uint8_t *in, size_t n, size_t *len);
 int orc_lz_decompress(const uint8_t *s, size_t len, size_t expected, uint8_t **out);
+/* tANS (entropy/tans.rs): the legacy coder behind TAG_TANS. compress returns
+ * NULL for empty input or a model that does not quantize. */
+uint8_t *orc_tans_compress(const uint8_t *src, size_t n, size_t *len);
+int orc_tans_decompress(const uint8_t *src, size_t len, size_t max_len, uint8_t **out,
+                        size_t *out_len);
 /* Entropy tag layer (entropy/mod.rs): code_residuals picks rANS / range coder
  * by lambda and, with allow_lz, tries the LZ cascade; decode_residuals takes
- * TAG_RC / TAG_RANS / TAG_LZ (TAG_TANS gives ORC_UNSUPPORTED). */
+ * every tag (TAG_RC, TAG_TANS, TAG_RANS, TAG_LZ). */
 uint8_t *orc_code_residuals(const uint8_t *r, size_t n, uint64_t lambda, int allow_lz,
                             size_t *len);
 int orc_decode_residuals(const uint8_t *p, size_t len, size_t max_len, uint8_t **out,

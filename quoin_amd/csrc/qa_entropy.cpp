@@ -4,8 +4,8 @@
 // byte coders, so they run here on host cores, one stream per worker thread,
 // and only the streams whose level asks for them leave the device.
 //
-// Tags (mod.rs:24-30): 0 range coder, 1 tANS (decode only in the reference; not
-// built here), 2 rANS, 3 LZ cascade.
+// Tags (mod.rs:24-30): 0 range coder, 1 tANS (decode only: no encoder writes it
+// any more), 2 rANS, 3 LZ cascade.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -348,6 +348,66 @@ bool rans_decode(const uint8_t *s, size_t len, uint64_t max_len, Bytes &out) {
     return true;
 }
 
+// ---- tANS decode (src/entropy/tans.rs:197-303) ------------------------------
+// The legacy TAG_TANS coder: no encoder writes it any more, the reference still
+// decodes it. 1024-state table; symbols are spread with the odd stride
+// 3 * 1024 / 5 (+1) over the weights in header order, and each table entry's
+// node gives the symbol, the renorm bit count and the next state's base.
+// Renorm bits are read MSB first (src/bitio.rs), 0 past the end.
+constexpr uint32_t kTansLog = 10, kTansSize = 1u << kTansLog;
+
+bool tans_decode(const uint8_t *s, size_t len, uint64_t max_len, Bytes &out) {
+    size_t pos = 0;
+    uint64_t n;
+    if (!get_varint(s, len, pos, n) || n > max_len || pos >= len) return false;
+    const uint32_t num = s[pos++] + 1u;
+    if (len - pos < num) return false;
+    const uint8_t *which = s + pos;
+    pos += num;
+    if (len - pos < 2ull * num + 4) return false;
+    uint32_t weight[256], xs[256] = {0}, wsum = 0;
+    bool seen[256] = {false};
+    for (uint32_t k = 0; k < num; k++, pos += 2) {
+        weight[k] = (uint32_t)s[pos] | (uint32_t)s[pos + 1] << 8;
+        if (!weight[k] || seen[which[k]]) return false;
+        seen[which[k]] = true;
+        wsum += weight[k];
+        xs[which[k]] = weight[k];
+    }
+    uint32_t state = (uint32_t)s[pos] | (uint32_t)s[pos + 1] << 8 | (uint32_t)s[pos + 2] << 16 |
+                     (uint32_t)s[pos + 3] << 24;
+    pos += 4;
+    // the reference checks the state before the model
+    if (state < kTansSize || state >= 2 * kTansSize || wsum != kTansSize) return false;
+    uint8_t sym[kTansSize], nbits[kTansSize];
+    uint16_t nbase[kTansSize];
+    uint32_t stride = 3 * kTansSize / 5 | 1u, step = 0;
+    for (uint32_t k = 0; k < num; k++)
+        for (uint32_t r = 0; r < weight[k]; r++, step++) sym[(stride * step) & (kTansSize - 1)] = which[k];
+    for (uint32_t i = 0; i < kTansSize; i++) {
+        const uint32_t base = xs[sym[i]]++;
+        const uint32_t bits = (uint32_t)__builtin_clz(base) - (uint32_t)__builtin_clz(kTansSize);
+        nbits[i] = (uint8_t)bits;
+        nbase[i] = (uint16_t)((base << bits) - kTansSize);
+    }
+    out.resize(n);
+    uint64_t acc = 0;
+    uint32_t have = 0;
+    for (uint64_t k = 0; k < n; k++) {
+        const uint32_t idx = state - kTansSize, bits = nbits[idx];
+        uint32_t extra = 0;
+        if (bits) {
+            for (; have < bits; have += 8) acc = acc << 8 | (pos < len ? s[pos++] : (pos++, 0u));
+            have -= bits;
+            extra = (uint32_t)(acc >> have) & ((1u << bits) - 1);
+            acc &= (1ull << have) - 1;
+        }
+        out[k] = sym[idx];
+        state = kTansSize + nbase[idx] + extra;
+    }
+    return true;
+}
+
 // ---- the tag layer ---------------------------------------------------------
 // entropy_pick (mod.rs:45-78) with every coder on the host
 void pick_host(const uint8_t *d, size_t n, uint64_t lambda, Bytes &out) {
@@ -372,7 +432,7 @@ int entropy_only_host(const uint8_t *p, size_t len, uint64_t max_len, Bytes &out
     switch (p[0]) {
     case kTagRc: return rc_decode(p + 1, len - 1, max_len, out) ? QUOIN_OK : QUOIN_ERR_CORRUPT;
     case kTagRans: return rans_decode(p + 1, len - 1, max_len, out) ? QUOIN_OK : QUOIN_ERR_CORRUPT;
-    case kTagTans: return QUOIN_ERR_UNSUPPORTED;
+    case kTagTans: return tans_decode(p + 1, len - 1, max_len, out) ? QUOIN_OK : QUOIN_ERR_CORRUPT;
     default: return QUOIN_ERR_CORRUPT;
     }
 }

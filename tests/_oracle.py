@@ -45,12 +45,14 @@ def _load():
                                       ctypes.POINTER(ctypes.c_size_t)]
     lib.orc_rans_decompress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
                                         ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
-    for f in ("orc_rc_compress", "orc_lz_compress"):
+    for f in ("orc_rc_compress", "orc_lz_compress", "orc_tans_compress"):
         getattr(lib, f).restype = u8p
         getattr(lib, f).argtypes = [ctypes.c_void_p, ctypes.c_size_t,
                                     ctypes.POINTER(ctypes.c_size_t)]
     lib.orc_rc_decompress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
                                       ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_tans_decompress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                                        ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
     lib.orc_lz_decompress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
                                       ctypes.POINTER(u8p)]
     lib.orc_code_residuals.restype = u8p
@@ -188,6 +190,24 @@ def rc_decompress(payload, max_len):
     rc = lib.orc_rc_decompress(bp, len(buf), max_len, ctypes.byref(out), ctypes.byref(n))
     if rc != 0:
         raise ValueError(f"rc rc={rc}")
+    return _take(out, n.value)
+
+
+def tans_compress(data):
+    """entropy/tans.rs compress_bytes; None where the reference returns None."""
+    data, dp = _bytes_in(data)
+    n = ctypes.c_size_t()
+    p = lib.orc_tans_compress(dp, len(data), ctypes.byref(n))
+    return _take(p, n.value) if p else None
+
+
+def tans_decompress(payload, max_len):
+    buf, bp = _bytes_in(payload)
+    out = ctypes.POINTER(ctypes.c_uint8)()
+    n = ctypes.c_size_t()
+    rc = lib.orc_tans_decompress(bp, len(buf), max_len, ctypes.byref(out), ctypes.byref(n))
+    if rc != 0:
+        raise ValueError(f"tans rc={rc}", rc)
     return _take(out, n.value)
 
 

@@ -119,12 +119,20 @@ def test_decode_residuals_oracle_streams_and_errors(gpu):
             payloads.append(orc.code_residuals(d, lam, lz))
             maxes.append(len(d))
             expect.append(d)
+    # legacy TAG_TANS streams (tans.rs; decoded, no longer produced)
+    for name in ("skewed", "alphabet", "residuals", "one"):
+        d = cs[name]
+        t = orc.tans_compress(d)
+        if t is not None:
+            payloads.append(b"\x01" + t)
+            maxes.append(len(d))
+            expect.append(d)
     rep = cs["repeats"]
     lzs = orc.code_residuals(rep, 0, True)
     assert lzs[0] == 3
     bad = [
         (b"", 10),                               # no tag
-        (b"\x01\x00", 10),                       # tANS: not produced, not built
+        (b"\x01\x00", 10),                       # tANS, header cut short
         (b"\x07\x00", 10),                       # unknown tag
         (lzs, len(rep) - 1),                     # cascade length over the bound
         (b"\x00" + orc.rc_compress(rep), len(rep) - 1),

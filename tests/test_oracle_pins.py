@@ -994,8 +994,8 @@ def test_decode_residuals_errors():
     with pytest.raises(ValueError):
         orc.decode_residuals(b"", 10)                       # no tag
     with pytest.raises(ValueError) as e:
-        orc.decode_residuals(b"\x01\x00", 10)               # tANS: not produced, not built
-    assert e.value.args[1] == -17
+        orc.decode_residuals(b"\x01\x00", 10)               # tANS, header cut short
+    assert e.value.args[1] == -10
     with pytest.raises(ValueError) as e:
         orc.decode_residuals(b"\x07\x00", 10)               # unknown tag
     assert e.value.args[1] == -14
@@ -1008,3 +1008,56 @@ def test_decode_residuals_errors():
         orc.decode_residuals(b"\x02" + orc.rans_compress(data), len(data) - 1)
     with pytest.raises(ValueError):
         orc.decode_residuals(b"\x00" + orc.rc_compress(data), len(data) - 1)
+
+
+def test_tans_known_stream_single_symbol():
+    """tans.rs wire format for five 7s: varint(5), count - 1 = 0, the symbol,
+    weight 1024 (quantize gives the only symbol the whole table), final state
+    1024 (every step maps state 1024 to next_states[0] = 1024 with 0 renorm
+    bits), and no bit bytes."""
+    assert orc.tans_compress(bytes([7] * 5)) == bytes.fromhex("05" "00" "07" "0004" "00040000")
+    assert orc.tans_decompress(bytes.fromhex("050007000400040000"), 5) == bytes([7] * 5)
+    assert orc.tans_compress(b"") is None
+
+
+def test_tans_reference_unit_cases():
+    """tans.rs tests: a skewed stream compresses below its length and
+    round-trips; uniform bytes, a single symbol and a single byte round-trip."""
+    s, skew = 1, bytearray()
+    for _ in range(40_000):
+        s = (s * 6364136223846793005 + 1442695040888963407) & M64
+        if s % 8 == 0:
+            s = (s * 6364136223846793005 + 1442695040888963407) & M64
+            skew.append((s >> 50) & 0xFF)
+        else:
+            skew.append(0)
+    c = orc.tans_compress(bytes(skew))
+    assert c is not None and len(c) < len(skew)
+    assert orc.tans_decompress(c, len(skew)) == bytes(skew)
+    s, uni = 99, bytearray()
+    for _ in range(40_000):
+        s = (s * 6364136223846793005 + 1442695040888963407) & M64
+        uni.append((s >> 40) & 0xFF)
+    for d in (bytes(uni), bytes([7] * 5000), bytes([42])):
+        c = orc.tans_compress(d)
+        if c is not None:
+            assert orc.tans_decompress(c, len(d)) == d
+
+
+def test_tans_decode_errors():
+    good = bytes.fromhex("050007000400040000")
+    bad = [
+        (good, 4),                                               # length over the bound
+        (bytes.fromhex("050007000400ff070000"), 5),              # state past 2048
+        (bytes.fromhex("050007000400ff030000"), 5),              # state below 1024
+        (bytes.fromhex("05010708000400000000040000"), 5),        # weights sum past 1024
+        (bytes.fromhex("0501070700020002" "00040000"), 5),       # duplicate symbol
+        (bytes.fromhex("050107080000000400040000"), 5),          # zero weight
+        (bytes.fromhex("05000700"), 5),                          # truncated header
+    ]
+    for pl, ml in bad:
+        with pytest.raises(ValueError):
+            orc.tans_decompress(pl, ml)
+    # the tag layer takes TAG_TANS streams
+    d = bytes(range(50)) * 30
+    assert orc.decode_residuals(b"\x01" + orc.tans_compress(d), len(d)) == d
