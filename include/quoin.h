@@ -357,6 +357,31 @@ int quoin_gpu_linear_encode(QuoinGpu *g, int32_t kind, int32_t order, const Quoi
 int quoin_gpu_linear_decode(QuoinGpu *g, int32_t kind, const QuoinLinearStream *streams,
                             size_t n_blocks, int32_t *status, void *stream);
 
+/* ---- additions: FCM / DFCM predictors (src/codecs/pred.rs) --------------- */
+#define QUOIN_PRED_FCM  0   /* Mode::Pred residuals: v ^ table[ctx] (pred.rs:23-37)          */
+#define QUOIN_PRED_DFCM 25  /* Mode::Pred2 residuals: v ^ (last + table[dctx]) (pred.rs:59-75) */
+
+/* Replaces pred::encode (kind 0) and pred::dfcm_encode (kind 25) for a batch
+ * of blocks, byte-identical: out_lens[i] = residual stream bytes (varints,
+ * written to in[i].dst, dst_cap >= quoin_linear_bound(n)). The stream is
+ * Mode::Pred's payload as it is; Mode::PredRc and Mode::Pred2 code it with
+ * quoin_gpu_code_residuals. predictor_log2 1..16 (Config::predictor_log2
+ * clamps to 10..16, src/lib.rs:236-238); fewer than 65535 blocks per call. */
+int quoin_gpu_pred_encode(QuoinGpu *g, int32_t kind, uint32_t predictor_log2,
+                          const QuoinLinearInput *in, size_t n_blocks, uint64_t *out_lens,
+                          void *stream);
+
+/* Replaces pred::decode / pred::dfcm_decode (pred.rs:39-57, 77-97) for a
+ * batch: the recurrence runs on the device, one block per lane, each with its
+ * 8 << predictor_log2 byte table in HBM (refused as
+ * QUOIN_ERR_BUFFER_TOO_SMALL past the handle's budget). status[i] (optional) =
+ * QUOIN_OK or QUOIN_ERR_CORRUPT where the reference returns Err (too few
+ * varints, trailing bytes, a varint over 10 bytes). */
+int quoin_gpu_pred_decode(QuoinGpu *g, int32_t kind, uint32_t predictor_log2,
+                          const QuoinLinearStream *streams, size_t n_blocks, int32_t *status,
+                          void *stream);
+
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -828,6 +828,75 @@ int orc_rans_decompress(const uint8_t *pl, size_t len, size_t max_len, uint8_t *
 }
 
 /* ------------------------------------------------------------------------ */
+/* FCM / DFCM predictors (codecs/pred.rs) over CRC32C (hash.rs)              */
+/* ------------------------------------------------------------------------ */
+#define PRED_SEED 0x9e3779b9u /* hash.rs:10 */
+
+/* hash.rs:15-30: bitwise CRC32C over the 8 little-endian bytes, no inversion */
+uint32_t orc_crc32c_u64(uint32_t crc, uint64_t v) {
+    for (int b = 0; b < 8; b++) {
+        crc ^= (uint32_t)(v & 0xff);
+        v >>= 8;
+        for (int k = 0; k < 8; k++) crc = (crc >> 1) ^ (0x82F63B78u & (0u - (crc & 1u)));
+    }
+    return crc;
+}
+/* pred.rs:23-37 (kind 0) and 59-75 (kind 25) */
+uint8_t *orc_pred_encode(int kind, const uint64_t *v, size_t n, int log2, size_t *len) {
+    const size_t mask = ((size_t)1 << log2) - 1;
+    uint64_t *table = (uint64_t *)calloc(mask + 1, sizeof(uint64_t));
+    buf_t out = {0};
+    size_t ctx = 0;
+    uint64_t last = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (kind == 0) {
+            varint_write(&out, v[i] ^ table[ctx]);
+            table[ctx] = v[i];
+            ctx = orc_crc32c_u64(PRED_SEED, v[i]) & mask;
+        } else {
+            varint_write(&out, v[i] ^ (last + table[ctx]));
+            uint64_t delta = v[i] - last;
+            table[ctx] = delta;
+            ctx = orc_crc32c_u64(PRED_SEED, delta) & mask;
+            last = v[i];
+        }
+    }
+    free(table);
+    *len = out.len;
+    if (!out.p) out.p = (uint8_t *)malloc(1);
+    return out.p;
+}
+/* pred.rs:39-57 and 77-97: n varints, then no trailing bytes */
+int orc_pred_decode(int kind, const uint8_t *p, size_t len, size_t n, int log2, uint64_t *outv) {
+    const size_t mask = ((size_t)1 << log2) - 1;
+    uint64_t *table = (uint64_t *)calloc(mask + 1, sizeof(uint64_t));
+    size_t pos = 0, ctx = 0;
+    uint64_t last = 0;
+    int rc = ORC_OK;
+    for (size_t i = 0; i < n && rc == ORC_OK; i++) {
+        uint64_t r;
+        rc = varint_read(p, len, &pos, &r);
+        if (rc) break;
+        if (kind == 0) {
+            uint64_t x = r ^ table[ctx];
+            outv[i] = x;
+            table[ctx] = x;
+            ctx = orc_crc32c_u64(PRED_SEED, x) & mask;
+        } else {
+            uint64_t x = r ^ (last + table[ctx]);
+            outv[i] = x;
+            uint64_t delta = x - last;
+            table[ctx] = delta;
+            ctx = orc_crc32c_u64(PRED_SEED, delta) & mask;
+            last = x;
+        }
+    }
+    free(table);
+    if (rc == ORC_OK && pos != len) rc = ORC_CORRUPT;
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Range coder with the adaptive order-1 byte model (entropy/rc.rs)          */
 /* ------------------------------------------------------------------------ */
 #define RC_PROB_BITS 11

@@ -75,6 +75,12 @@ uint8_t *orc_rans_compress(const uint8_t *data, size_t n, size_t *len);
 int orc_rans_decompress(const uint8_t *payload, size_t len, size_t max_len, uint8_t **out,
                         size_t *out_len);
 
+/* CRC32C step of hash.rs (8 little-endian bytes, no inversion) and the FCM
+ * (kind 0) / DFCM (kind 25) predictors of codecs/pred.rs. */
+uint32_t orc_crc32c_u64(uint32_t crc, uint64_t v);
+uint8_t *orc_pred_encode(int kind, const uint64_t *vals, size_t n, int log2, size_t *len);
+int orc_pred_decode(int kind, const uint8_t *payload, size_t len, size_t n, int log2,
+                    uint64_t *out);
 /* Range coder with the order-1 byte model (entropy/rc.rs): varint(len) ++
  * coded bytes. */
 uint8_t *orc_rc_compress(const uint8_t *src, size_t n, size_t *len);

@@ -176,6 +176,12 @@ def _load():
     lib.quoin_gpu_rans_decompress.argtypes = [vp, ctypes.POINTER(RansStream), sz, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_uint64),
                                               ctypes.POINTER(ctypes.c_int32), vp]
+    lib.quoin_gpu_pred_encode.argtypes = [vp, ctypes.c_int32, ctypes.c_uint32,
+                                          ctypes.POINTER(LinearInput), sz,
+                                          ctypes.POINTER(ctypes.c_uint64), vp]
+    lib.quoin_gpu_pred_decode.argtypes = [vp, ctypes.c_int32, ctypes.c_uint32,
+                                          ctypes.POINTER(LinearStream), sz,
+                                          ctypes.POINTER(ctypes.c_int32), vp]
     lib.quoin_residual_bound.restype = sz
     lib.quoin_residual_bound.argtypes = [sz]
     lib.quoin_gpu_code_residuals.argtypes = [vp, ctypes.c_uint64, ctypes.c_int,
@@ -211,7 +217,7 @@ EXPORTS = [
     "quoin_server_destroy", "quoin_rans_bound", "quoin_gpu_rans_compress",
     "quoin_gpu_rans_decompress", "quoin_linear_bound", "quoin_gpu_linear_encode",
     "quoin_gpu_linear_decode", "quoin_residual_bound", "quoin_gpu_code_residuals",
-    "quoin_gpu_decode_residuals",
+    "quoin_gpu_decode_residuals", "quoin_gpu_pred_encode", "quoin_gpu_pred_decode",
 ]
 
 
@@ -292,6 +298,9 @@ def mode_win_counts():
 
 def reset_mode_win_counts():
     lib.quoin_reset_mode_win_counts()
+
+
+PRED_FCM, PRED_DFCM = 0, 25  # frame modes Pred / Pred2 (src/mode.rs:11, 31)
 
 
 class Gpu:
@@ -379,6 +388,29 @@ class Gpu:
         if rc not in (OK, ERR_CORRUPT, ERR_UNSUPPORTED):
             raise QuoinError(rc)
         return [int(x) for x in lens], [int(x) for x in st]
+
+    def pred_encode(self, kind, inputs, predictor_log2=16, stream=None):
+        """FCM (kind PRED_FCM) / DFCM (kind PRED_DFCM) residual streams of a batch
+        of blocks (src/codecs/pred.rs:23-37, 59-75). inputs: list of LinearInput.
+        Returns the stream lengths."""
+        k = len(inputs)
+        arr = (LinearInput * k)(*inputs)
+        lens = (ctypes.c_uint64 * k)()
+        rc = lib.quoin_gpu_pred_encode(self.h, kind, predictor_log2, arr, k, lens, stream)
+        if rc != OK:
+            raise QuoinError(rc)
+        return [int(x) for x in lens]
+
+    def pred_decode(self, kind, streams, predictor_log2=16, stream=None):
+        """Values of a batch of FCM / DFCM residual streams (pred.rs:39-57,
+        77-97). Returns the per-block status (OK or ERR_CORRUPT)."""
+        k = len(streams)
+        arr = (LinearStream * k)(*streams)
+        st = (ctypes.c_int32 * k)()
+        rc = lib.quoin_gpu_pred_decode(self.h, kind, predictor_log2, arr, k, st, stream)
+        if rc not in (OK, ERR_CORRUPT):
+            raise QuoinError(rc)
+        return [int(x) for x in st]
 
     def linear_encode(self, kind, inputs, order=0, stream=None):
         """Residual streams of a batch of blocks (src/codecs/linear.rs:31-45,

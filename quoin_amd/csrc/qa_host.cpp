@@ -79,6 +79,9 @@ struct QuoinGpu {
     DevBuf rdesc, rres, rtabs, rscr;
     // linear residual arenas
     DevBuf ldesc, lres, ltile, lscan;
+    // FCM / DFCM predictor arenas: sort keys and indices, residual words, sort
+    // scratch, decode tables
+    DevBuf pkeys, pres, psort, ptab;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, kev0 = nullptr, kev1 = nullptr;
     double last_total_ms = 0, last_kernel_ms = 0;
     const char *last_kernel = "";
@@ -284,7 +287,8 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
                      &g->rstart, &g->hkeys, &g->hcnt, &g->rgrown, &g->gsets, &g->temit, &g->tlist, &g->dpart, &g->dparts, &g->dfx,
                      &g->dscr, &g->dscr2, &g->dkrun,
                      &g->st_out, &g->st_vout, &g->rdesc, &g->rres, &g->rtabs, &g->rscr,
-                     &g->ldesc, &g->lres, &g->ltile, &g->lscan};
+                     &g->ldesc, &g->lres, &g->ltile, &g->lscan, &g->pkeys, &g->pres,
+                     &g->psort, &g->ptab};
     for (DevBuf *b : all)
         if (b->p) hipFree(b->p);
     hipEventDestroy(g->ev0);
@@ -911,6 +915,128 @@ extern "C" int quoin_gpu_linear_decode(QuoinGpu *g, int32_t kind, const QuoinLin
     g->last_total_ms = ms;
     g->last_kernel_ms = kms;
     g->last_kernel = kind == QUOIN_LINEAR_ORDERED_DELTA ? "k_lin_od" : "k_lin_seq";
+    int rc = QUOIN_OK;
+    for (size_t i = 0; i < nb; i++) {
+        const int st = r[i].status ? QUOIN_ERR_CORRUPT : QUOIN_OK;
+        if (status) status[i] = st;
+        if (rc == QUOIN_OK) rc = st;
+    }
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// FCM / DFCM predictors (src/codecs/pred.rs)
+// ---------------------------------------------------------------------------
+extern "C" int quoin_gpu_pred_encode(QuoinGpu *g, int32_t kind, uint32_t predictor_log2,
+                                     const QuoinLinearInput *in, size_t nb, uint64_t *out_lens,
+                                     void *stream) {
+    if (!g || (nb && (!in || !out_lens))) return QUOIN_ERR_NULL;
+    if ((kind != QUOIN_PRED_FCM && kind != QUOIN_PRED_DFCM) || predictor_log2 < 1 ||
+        predictor_log2 > 16)
+        return QUOIN_ERR_UNSUPPORTED;
+    if (nb >= 0xFFFFull) return QUOIN_ERR_UNSUPPORTED;  // block index in the sort key's 16 bits
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    if (nb == 0) return QUOIN_OK;
+    std::vector<LinD> d(nb);
+    uint64_t tiles = 0;
+    for (size_t i = 0; i < nb; i++) {
+        const uint64_t n = in[i].n;
+        out_lens[i] = 0;
+        if (n && (!in[i].vals || !in[i].dst)) return QUOIN_ERR_NULL;
+        if (n && in[i].dst_cap < quoin_linear_bound(n)) return QUOIN_ERR_BUFFER_TOO_SMALL;
+        d[i] = LinD{};
+        d[i].vals = in[i].vals;
+        d[i].dst = in[i].dst;
+        d[i].n = n;
+        d[i].tile0 = tiles;
+        tiles += qa_lin_enc_tiles(n);
+    }
+    const uint64_t total = tiles * 2048;
+    if (total >= 0xFFFFFFFFull) return QUOIN_ERR_UNSUPPORTED;  // u32 flat indices
+    const size_t sb = qa_pred_sort_bytes(total);
+    if (!ensure(g->ldesc, nb * sizeof(LinD)) || !ensure(g->lres, nb * sizeof(LinR)) ||
+        !ensure(g->ltile, (2 * tiles + 2) * 8) ||
+        !ensure_scan(g->lscan, qa_scan_tmp_words(tiles) * 8, s) ||
+        !ensure(g->pkeys, total * 16 + 16) || !ensure(g->pres, total * 8 + 8) ||
+        !ensure(g->psort, sb + 256))
+        return QUOIN_ERR_DEVICE;
+    std::vector<LinR> r(nb, LinR{0, 0, 0});
+    uint64_t *tsz = dp<uint64_t>(g->ltile), *tpre = tsz + tiles + 1;
+    uint32_t *keys = dp<uint32_t>(g->pkeys), *idx = keys + total, *keys2 = idx + total,
+             *idx2 = keys2 + total;
+    hipEventRecord(g->ev0, s);
+    hipMemcpyAsync(g->ldesc.p, d.data(), nb * sizeof(LinD), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(g->lres.p, r.data(), nb * sizeof(LinR), hipMemcpyHostToDevice, s);
+    hipEventRecord(g->kev0, s);
+    qa_pred_encode_launch((uint32_t)kind, dp<LinD>(g->ldesc), dp<LinR>(g->lres), (uint32_t)nb, tiles,
+                          predictor_log2, keys, idx, keys2, idx2, dp<uint64_t>(g->pres), g->psort.p,
+                          sb, tsz, tpre, dp<uint64_t>(g->lscan), s);
+    hipEventRecord(g->kev1, s);
+    hipEventRecord(g->ev1, s);
+    hipMemcpyAsync(r.data(), g->lres.p, nb * sizeof(LinR), hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    float ms = 0, kms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    hipEventElapsedTime(&kms, g->kev0, g->kev1);
+    g->last_total_ms = ms;
+    g->last_kernel_ms = kms;
+    g->last_kernel = "predictor encode (k_pred_keys, radix sort, k_pred_res, k_pred_enc)";
+    for (size_t i = 0; i < nb; i++) out_lens[i] = r[i].len;
+    return QUOIN_OK;
+}
+
+extern "C" int quoin_gpu_pred_decode(QuoinGpu *g, int32_t kind, uint32_t predictor_log2,
+                                     const QuoinLinearStream *in, size_t nb, int32_t *status,
+                                     void *stream) {
+    if (!g || (nb && !in)) return QUOIN_ERR_NULL;
+    if ((kind != QUOIN_PRED_FCM && kind != QUOIN_PRED_DFCM) || predictor_log2 < 1 ||
+        predictor_log2 > 16)
+        return QUOIN_ERR_UNSUPPORTED;
+    if (nb > 0xFFFFFFFFull) return QUOIN_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    hipStream_t s = stream ? (hipStream_t)stream : g->stream;
+    if (nb == 0) return QUOIN_OK;
+    std::vector<LinD> d(nb);
+    uint64_t tiles = 0;
+    for (size_t i = 0; i < nb; i++) {
+        if ((in[i].src_len && !in[i].src) || (in[i].n && !in[i].out)) return QUOIN_ERR_NULL;
+        d[i] = LinD{};
+        d[i].src = in[i].src;
+        d[i].out = in[i].out;
+        d[i].n = in[i].n;
+        d[i].len = in[i].src_len;
+        d[i].tile0 = tiles;
+        tiles += qa_lin_dec_tiles(in[i].src_len);
+    }
+    if (tiles > 0x7FFFFFFFull) return QUOIN_ERR_UNSUPPORTED;
+    const uint64_t twords = (uint64_t)nb << predictor_log2;
+    if (twords * 8 > g->budget) return QUOIN_ERR_BUFFER_TOO_SMALL;  // the decode tables
+    if (!ensure(g->ldesc, nb * sizeof(LinD)) || !ensure(g->lres, nb * sizeof(LinR)) ||
+        !ensure(g->ltile, (2 * tiles + 2) * 8) ||
+        !ensure_scan(g->lscan, qa_scan_tmp_words(tiles) * 8, s) || !ensure(g->ptab, twords * 8))
+        return QUOIN_ERR_DEVICE;
+    uint64_t *cnt = dp<uint64_t>(g->ltile), *pre = cnt + tiles + 1;
+    std::vector<LinR> r(nb, LinR{0, 0, 0});
+    hipEventRecord(g->ev0, s);
+    hipMemcpyAsync(g->ldesc.p, d.data(), nb * sizeof(LinD), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(g->lres.p, r.data(), nb * sizeof(LinR), hipMemcpyHostToDevice, s);
+    qa_pred_decode_launch((uint32_t)kind, dp<LinD>(g->ldesc), dp<LinR>(g->lres), (uint32_t)nb, tiles,
+                          predictor_log2, dp<uint64_t>(g->ptab), cnt, pre, dp<uint64_t>(g->lscan), s,
+                          g->kev0, g->kev1);
+    hipEventRecord(g->ev1, s);
+    hipMemcpyAsync(r.data(), g->lres.p, nb * sizeof(LinR), hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    float ms = 0, kms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    hipEventElapsedTime(&kms, g->kev0, g->kev1);
+    g->last_total_ms = ms;
+    g->last_kernel_ms = kms;
+    g->last_kernel = "k_pred_seq";
     int rc = QUOIN_OK;
     for (size_t i = 0; i < nb; i++) {
         const int st = r[i].status ? QUOIN_ERR_CORRUPT : QUOIN_OK;

@@ -295,6 +295,15 @@ struct LinR {
     int32_t status;        // encode: 1 = DeltaDp declined; decode: 1 = corrupt
     uint32_t order;
 };
+// FCM / DFCM predictors (kind 0 / 25), on the linear coders' descriptors
+size_t qa_pred_sort_bytes(uint64_t total);
+void qa_pred_encode_launch(uint32_t kind, const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles,
+                           uint32_t log2, uint32_t *keys, uint32_t *idx, uint32_t *keys2,
+                           uint32_t *idx2, uint64_t *res, void *sort_tmp, size_t sort_bytes,
+                           uint64_t *tsz, uint64_t *tpre, uint64_t *tmp, hipStream_t s);
+void qa_pred_decode_launch(uint32_t kind, const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles,
+                           uint32_t log2, uint64_t *tables, uint64_t *cnt, uint64_t *pre,
+                           uint64_t *tmp, hipStream_t s, void *kev0, void *kev1);
 uint64_t qa_lin_enc_tiles(uint64_t n);
 uint64_t qa_lin_dec_tiles(uint64_t bytes);
 void qa_lin_encode_launch(uint32_t kind, const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles,

@@ -26,6 +26,8 @@
 // prediction runs on the hardware first; only a NaN prediction (never the
 // case without NaN or infinite history) is recomputed step by step with that
 // rule. The file is compiled without FP contraction.
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "qa_kernels.h"
 
 #define LIN_OD 7u
@@ -197,6 +199,60 @@ __global__ void __launch_bounds__(256) k_lin_order(const LinD *d, LinR *r) {
 // Residuals of one 2048-value tile. WRITE = false: tile byte counts into
 // tsz[t] (and DeltaDp's verification); WRITE = true: tsz holds their exclusive
 // prefix over the batch and the bytes are written.
+// A tile's residual words (thread j holds words j0 .. j0 + LE_PER) as LEB128
+// varints (varint.rs:9-21). Sizing pass (!WRITE): the tile's byte count goes
+// to tsz[t]. Writing pass: tsz is the exclusive prefix over the tiles; the
+// bytes are staged in LDS at the alignment of their destination and written
+// with 16-byte stores, and the block's first tile records its stream length
+// (hdr header bytes before the varints).
+template <bool WRITE>
+__device__ __forceinline__ void tile_varints(const LinD &db, LinR &rb, uint64_t t, uint32_t cnt,
+                                             uint32_t j0, const uint64_t (&res)[LE_PER],
+                                             uint64_t len, uint64_t *tsz, uint64_t hdr) {
+    const uint64_t tile0 = db.tile0, n = db.n;
+    if (!WRITE) {
+        uint64_t tot;
+        wg_excl_scan(len, &tot);
+        if (threadIdx.x == 0) tsz[t] = tot;
+        return;
+    } else {
+        __shared__ __align__(16) uint8_t stg[LE_TILE * 10 + 32];
+        uint64_t tot;
+        const uint64_t ex = wg_excl_scan(len, &tot);
+        uint8_t *g0 = db.dst + hdr + (tsz[t] - tsz[tile0]);
+        const uint32_t a0 = (uint32_t)((uintptr_t)g0 & 15);
+        if (j0 < cnt) {
+            uint32_t p = a0 + (uint32_t)ex;
+#pragma unroll
+            for (int m = 0; m < LE_PER; m++) {
+                if (j0 + m >= cnt) break;
+                uint64_t v = res[m];
+                while (v >= 0x80) {
+                    stg[p++] = (uint8_t)(v | 0x80);
+                    v >>= 7;
+                }
+                stg[p++] = (uint8_t)v;
+            }
+        }
+        __syncthreads();
+        uint8_t *gb = g0 - a0;  // 16-byte aligned
+        const uint32_t end = a0 + (uint32_t)tot;
+        const uint32_t nch = (end + 15) / 16;
+        for (uint32_t c = threadIdx.x; c < nch; c += LE_WG) {
+            const uint32_t lo = c * 16, hi = lo + 16;
+            if (lo >= a0 && hi <= end) {
+                *(uint4 *)(gb + lo) = *(const uint4 *)(stg + lo);
+            } else {
+                for (uint32_t q = max(lo, a0); q < min(hi, end); q++) gb[q] = stg[q];
+            }
+        }
+        if (t == tile0 && threadIdx.x == 0) {
+            const uint64_t nt = (n + LE_TILE - 1) / LE_TILE;
+            rb.len = tsz[tile0 + nt] - tsz[tile0] + hdr;
+        }
+    }
+}
+
 template <uint32_t KIND, bool WRITE>
 __global__ void __launch_bounds__(LE_WG) k_lin_enc(const LinD *d, LinR *r, uint32_t nblk,
                                                    uint64_t *tsz) {
@@ -261,52 +317,11 @@ __global__ void __launch_bounds__(LE_WG) k_lin_enc(const LinD *d, LinR *r, uint3
             if (j0 + m < cnt) len += qa_varint_len(res[m]);
     }
 
-    if (!WRITE) {
-        if (KIND == LIN_DP) {
-            const uint64_t any = __ballot(bad);
-            if (any && (threadIdx.x & 63) == 0) atomicOr(&r[b].status, 1);
-        }
-        uint64_t tot;
-        wg_excl_scan(len, &tot);
-        if (threadIdx.x == 0) tsz[t] = tot;
-        return;
-    } else {
-        __shared__ __align__(16) uint8_t stg[LE_TILE * 10 + 32];
-        uint64_t tot;
-        const uint64_t ex = wg_excl_scan(len, &tot);
-        const uint64_t hdr = KIND == LIN_OD ? 0 : 1;
-        uint8_t *g0 = d[b].dst + hdr + (tsz[t] - tsz[tile0]);
-        const uint32_t a0 = (uint32_t)((uintptr_t)g0 & 15);
-        if (j0 < cnt) {
-            uint32_t p = a0 + (uint32_t)ex;
-#pragma unroll
-            for (int m = 0; m < LE_PER; m++) {
-                if (j0 + m >= cnt) break;
-                uint64_t v = res[m];
-                while (v >= 0x80) {
-                    stg[p++] = (uint8_t)(v | 0x80);
-                    v >>= 7;
-                }
-                stg[p++] = (uint8_t)v;
-            }
-        }
-        __syncthreads();
-        uint8_t *gb = g0 - a0;  // 16-byte aligned
-        const uint32_t end = a0 + (uint32_t)tot;
-        const uint32_t nch = (end + 15) / 16;
-        for (uint32_t c = threadIdx.x; c < nch; c += LE_WG) {
-            const uint32_t lo = c * 16, hi = lo + 16;
-            if (lo >= a0 && hi <= end) {
-                *(uint4 *)(gb + lo) = *(const uint4 *)(stg + lo);
-            } else {
-                for (uint32_t q = max(lo, a0); q < min(hi, end); q++) gb[q] = stg[q];
-            }
-        }
-        if (t == tile0 && threadIdx.x == 0) {
-            const uint64_t nt = (n + LE_TILE - 1) / LE_TILE;
-            r[b].len = tsz[tile0 + nt] - tsz[tile0] + hdr;
-        }
+    if (!WRITE && KIND == LIN_DP) {
+        const uint64_t any = __ballot(bad);
+        if (any && (threadIdx.x & 63) == 0) atomicOr(&r[b].status, 1);
     }
+    tile_varints<WRITE>(d[b], r[b], t, cnt, j0, res, len, tsz, KIND == LIN_OD ? 0 : 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -536,7 +551,230 @@ void dec_launch(const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles, uint64_t
     if (kev1) (void)hipEventRecord((hipEvent_t)kev1, s);
 }
 
+// ---------------------------------------------------------------------------
+// FCM / DFCM predictors (src/codecs/pred.rs)
+// ---------------------------------------------------------------------------
+// FCM (mode 0): r_i = v_i ^ T[ctx_i], ctx_0 = 0, ctx_i = crc(v_{i-1}) & mask,
+// where T[ctx] holds the value of the last earlier position with the same
+// context (0 if none). DFCM (mode 25): r_i = v_i ^ (v_{i-1} + T[ctx_i]) with
+// ctx_i = crc(d_{i-1}) & mask, d_j = v_j - v_{j-1} (v_{-1} = 0) and T holding
+// the last earlier d at that context. Encode is therefore a "last earlier
+// occurrence of the same key" query: each value's key (block << 16 | ctx) and
+// flat index are radix-sorted (stable), and a value's predecessor in sorted
+// order, if it has the same key, is that occurrence. Decode is the table
+// recurrence of pred.rs, one lane per block with its table in HBM.
+#define PRED_FCM 0u
+#define PRED_DFCM 25u
+#define PRED_SEED 0x9e3779b9u  // hash.rs:10
+#define PRED_NONE 0xFFFFFFFFu
+
+// CRC32C (reflected polynomial 0x82F63B78) over the 8 little-endian bytes of
+// v, no inversion (hash.rs:15-30), one table lookup per byte
+__device__ __forceinline__ void crc_table(uint32_t *t) {
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
+        uint32_t c = i;
+        for (int k = 0; k < 8; k++) c = (c >> 1) ^ (0x82F63B78u & (0u - (c & 1u)));
+        t[i] = c;
+    }
+}
+__device__ __forceinline__ uint32_t crc32c_u64(const uint32_t *t, uint32_t crc, uint64_t v) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) crc = (crc >> 8) ^ t[(crc ^ (uint32_t)(v >> (8 * k))) & 0xFFu];
+    return crc;
+}
+
+// block index of a flat position (tile t, slot k) of the encode tiles
+__device__ __forceinline__ uint64_t pred_local(const LinD &db, uint64_t f) {
+    return (f / LE_TILE - db.tile0) * LE_TILE + f % LE_TILE;
+}
+
+template <uint32_t PK>
+__global__ void __launch_bounds__(LE_WG) k_pred_keys(const LinD *d, uint32_t nblk, uint32_t mask,
+                                                     uint32_t *keys, uint32_t *idx) {
+    __shared__ uint32_t ct[256];
+    crc_table(ct);
+    __syncthreads();
+    const uint64_t t = blockIdx.x;
+    const uint32_t b = owner_block(d, nblk, t);
+    const uint64_t *vals = d[b].vals;
+    const uint64_t n = d[b].n, s0 = (t - d[b].tile0) * LE_TILE;
+    const uint32_t cnt = (uint32_t)min((uint64_t)LE_TILE, n - s0);
+    for (uint32_t k = threadIdx.x; k < LE_TILE; k += LE_WG) {
+        const uint64_t f = t * LE_TILE + k;
+        uint32_t key = PRED_NONE;
+        if (k < cnt) {
+            const uint64_t i = s0 + k;
+            uint32_t ctx = 0;
+            if (i > 0) {
+                uint64_t h = vals[i - 1];
+                if (PK == PRED_DFCM) h -= i > 1 ? vals[i - 2] : 0;
+                ctx = crc32c_u64(ct, PRED_SEED, h) & mask;
+            }
+            key = (b << 16) | ctx;
+        }
+        keys[f] = key;
+        idx[f] = (uint32_t)f;
+    }
+}
+
+// each sorted position: the residual of its value from the previous one with
+// the same key
+template <uint32_t PK>
+__global__ void __launch_bounds__(256) k_pred_res(const LinD *d, const uint32_t *keys,
+                                                  const uint32_t *idx, uint64_t total,
+                                                  uint64_t *res) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= total) return;
+    const uint32_t key = keys[p];
+    if (key == PRED_NONE) return;
+    const LinD &db = d[key >> 16];
+    const uint64_t f = idx[p], i = pred_local(db, f);
+    const uint64_t *v = db.vals;
+    const bool has = p > 0 && keys[p - 1] == key;
+    const uint64_t j = has ? pred_local(db, idx[p - 1]) : 0;
+    uint64_t pred;
+    if (PK == PRED_FCM) pred = has ? v[j] : 0;
+    else pred = (i > 0 ? v[i - 1] : 0) + (has ? v[j] - (j > 0 ? v[j - 1] : 0) : 0);
+    res[f] = v[i] ^ pred;
+}
+
+template <bool WRITE>
+__global__ void __launch_bounds__(LE_WG) k_pred_enc(const LinD *d, LinR *r, uint32_t nblk,
+                                                    uint64_t *tsz, const uint64_t *resg) {
+    const uint64_t t = blockIdx.x;
+    const uint32_t b = owner_block(d, nblk, t);
+    const uint64_t n = d[b].n, s0 = (t - d[b].tile0) * LE_TILE;
+    const uint32_t cnt = (uint32_t)min((uint64_t)LE_TILE, n - s0);
+    const uint32_t j0 = threadIdx.x * LE_PER;
+    uint64_t res[LE_PER];
+    uint64_t len = 0;
+#pragma unroll
+    for (int m = 0; m < LE_PER; m++) {
+        res[m] = j0 + m < cnt ? resg[t * LE_TILE + j0 + m] : 0;
+        if (j0 + m < cnt) len += qa_varint_len(res[m]);
+    }
+    tile_varints<WRITE>(d[b], r[b], t, cnt, j0, res, len, tsz, 0);
+}
+
+__global__ void k_pred_zero(uint64_t *p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// pred.rs decode / dfcm_decode over a block's residual words (already parsed
+// into out), one lane per block, table at tables + (b << log2)
+template <uint32_t PK>
+__global__ void __launch_bounds__(64) k_pred_seq(const LinD *d, const LinR *r, uint32_t nblk,
+                                                 uint32_t log2, uint64_t *tables) {
+    __shared__ uint32_t ct[256];
+    crc_table(ct);
+    __syncthreads();
+    const uint32_t b = blockIdx.x * 64 + threadIdx.x;
+    if (b >= nblk || r[b].status) return;
+    const uint32_t mask = (1u << log2) - 1;
+    uint64_t *tab = tables + ((uint64_t)b << log2);
+    uint64_t *out = d[b].out;
+    const uint64_t n = d[b].n;
+    uint32_t ctx = 0;
+    uint64_t last = 0;
+    auto step = [&](uint64_t x) -> uint64_t {
+        if (PK == PRED_FCM) {
+            const uint64_t v = x ^ tab[ctx];
+            tab[ctx] = v;
+            ctx = crc32c_u64(ct, PRED_SEED, v) & mask;
+            return v;
+        } else {
+            const uint64_t v = x ^ (last + tab[ctx]);
+            const uint64_t delta = v - last;
+            tab[ctx] = delta;
+            ctx = crc32c_u64(ct, PRED_SEED, delta) & mask;
+            last = v;
+            return v;
+        }
+    };
+    // residual words are loaded AHEAD values before they are used
+    constexpr int AHEAD = 32;
+    uint64_t x[AHEAD];
+#pragma unroll
+    for (int k = 0; k < AHEAD; k++) x[k] = (uint64_t)k < n ? out[k] : 0;
+    uint64_t i = 0;
+    for (; i + AHEAD <= n; i += AHEAD) {
+#pragma unroll
+        for (int k = 0; k < AHEAD; k++) {
+            out[i + k] = step(x[k]);
+            x[k] = i + AHEAD + k < n ? out[i + AHEAD + k] : 0;
+        }
+    }
+    for (int k = 0; k < AHEAD; k++)
+        if (i + k < n) out[i + k] = step(x[k]);
+}
+
+template <uint32_t PK>
+void pred_enc_launch(const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles, uint32_t log2,
+                     uint32_t *keys, uint32_t *idx, uint32_t *keys2, uint32_t *idx2, uint64_t *res,
+                     void *sort_tmp, size_t sort_bytes, uint64_t *tsz, uint64_t *tpre,
+                     uint64_t *tmp, hipStream_t s) {
+    if (!ntiles) return;
+    const uint64_t total = ntiles * LE_TILE;
+    hipLaunchKernelGGL(k_pred_keys<PK>, dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, nblk,
+                       (1u << log2) - 1, keys, idx);
+    size_t bytes = sort_bytes;
+    (void)rocprim::radix_sort_pairs(sort_tmp, bytes, keys, keys2, idx, idx2, (size_t)total, 0, 32, s);
+    hipLaunchKernelGGL(k_pred_res<PK>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, d,
+                       keys2, idx2, total, res);
+    hipLaunchKernelGGL((k_pred_enc<false>), dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, r, nblk, tsz,
+                       res);
+    qa_exclusive_scan(tsz, ntiles, tpre, tmp, s);
+    hipLaunchKernelGGL((k_pred_enc<true>), dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, r, nblk, tpre,
+                       res);
+}
+
 }  // namespace
+
+size_t qa_pred_sort_bytes(uint64_t total) {
+    size_t bytes = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                    (uint32_t *)nullptr, (uint32_t *)nullptr, (size_t)total, 0, 32);
+    return bytes;
+}
+
+void qa_pred_encode_launch(uint32_t kind, const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles,
+                           uint32_t log2, uint32_t *keys, uint32_t *idx, uint32_t *keys2,
+                           uint32_t *idx2, uint64_t *res, void *sort_tmp, size_t sort_bytes,
+                           uint64_t *tsz, uint64_t *tpre, uint64_t *tmp, hipStream_t s) {
+    if (kind == PRED_FCM)
+        pred_enc_launch<PRED_FCM>(d, r, nblk, ntiles, log2, keys, idx, keys2, idx2, res, sort_tmp,
+                                  sort_bytes, tsz, tpre, tmp, s);
+    else
+        pred_enc_launch<PRED_DFCM>(d, r, nblk, ntiles, log2, keys, idx, keys2, idx2, res, sort_tmp,
+                                   sort_bytes, tsz, tpre, tmp, s);
+}
+
+void qa_pred_decode_launch(uint32_t kind, const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles,
+                           uint32_t log2, uint64_t *tables, uint64_t *cnt, uint64_t *pre,
+                           uint64_t *tmp, hipStream_t s, void *kev0, void *kev1) {
+    // the varint parse and its checks are OrderedDelta's (no header byte)
+    if (ntiles)
+        hipLaunchKernelGGL(k_lin_vcount<LIN_OD>, dim3((unsigned)ntiles), dim3(LD_WG), 0, s, d, nblk,
+                           cnt);
+    qa_exclusive_scan(cnt, ntiles, pre, tmp, s);
+    if (ntiles)
+        hipLaunchKernelGGL(k_lin_vparse<LIN_OD>, dim3((unsigned)ntiles), dim3(LD_WG), 0, s, d, r,
+                           nblk, pre);
+    const unsigned g64 = (nblk + 63) / 64;
+    hipLaunchKernelGGL(k_lin_dcheck<LIN_OD>, dim3(g64), dim3(64), 0, s, d, r, nblk, pre);
+    const uint64_t words = (uint64_t)nblk << log2;
+    const uint64_t zg = (words + 255) / 256;
+    hipLaunchKernelGGL(k_pred_zero, dim3((unsigned)(zg < 65536 ? zg : 65536)), dim3(256), 0, s,
+                       tables, words);
+    if (kev0) (void)hipEventRecord((hipEvent_t)kev0, s);
+    if (kind == PRED_FCM)
+        hipLaunchKernelGGL(k_pred_seq<PRED_FCM>, dim3(g64), dim3(64), 0, s, d, r, nblk, log2, tables);
+    else
+        hipLaunchKernelGGL(k_pred_seq<PRED_DFCM>, dim3(g64), dim3(64), 0, s, d, r, nblk, log2, tables);
+    if (kev1) (void)hipEventRecord((hipEvent_t)kev1, s);
+}
 
 uint64_t qa_lin_enc_tiles(uint64_t n) { return (n + LE_TILE - 1) / LE_TILE; }
 uint64_t qa_lin_dec_tiles(uint64_t bytes) { return (bytes + LD_TILE - 1) / LD_TILE; }

@@ -55,6 +55,13 @@ def _load():
                                         ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
     lib.orc_lz_decompress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
                                       ctypes.POINTER(u8p)]
+    lib.orc_crc32c_u64.restype = ctypes.c_uint32
+    lib.orc_crc32c_u64.argtypes = [ctypes.c_uint32, ctypes.c_uint64]
+    lib.orc_pred_encode.restype = u8p
+    lib.orc_pred_encode.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_pred_decode.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                                    ctypes.c_int, ctypes.c_void_p]
     lib.orc_code_residuals.restype = u8p
     lib.orc_code_residuals.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64,
                                        ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
@@ -226,6 +233,27 @@ def lz_decompress(stream, expected):
     if rc != 0:
         raise ValueError(f"lz rc={rc}")
     return _take(out, expected)
+
+
+def crc32c_u64(crc, v):
+    """hash.rs crc32c_u64_sw."""
+    return int(lib.orc_crc32c_u64(crc, v))
+
+
+def pred_encode(kind, lane, log2):
+    """codecs/pred.rs encode (kind 0) / dfcm_encode (kind 25)."""
+    lane = np.ascontiguousarray(lane, dtype=np.uint64)
+    n = ctypes.c_size_t()
+    p = lib.orc_pred_encode(kind, _ptr(lane) if len(lane) else None, len(lane), log2, ctypes.byref(n))
+    return _take(p, n.value)
+
+
+def pred_decode(kind, payload, n, log2):
+    """(rc, values) of pred.rs decode / dfcm_decode."""
+    buf, bp = _bytes_in(payload)
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    rc = lib.orc_pred_decode(kind, bp, len(buf), n, log2, _ptr(out))
+    return rc, out[:n]
 
 
 def code_residuals(data, lam, allow_lz):

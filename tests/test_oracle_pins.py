@@ -1061,3 +1061,75 @@ def test_tans_decode_errors():
     # the tag layer takes TAG_TANS streams
     d = bytes(range(50)) * 30
     assert orc.decode_residuals(b"\x01" + orc.tans_compress(d), len(d)) == d
+
+
+# ---------------------------------------------------------------------------
+# FCM / DFCM predictors (codecs/pred.rs) over CRC32C (hash.rs)
+# ---------------------------------------------------------------------------
+def test_crc32c_check_value():
+    """The u64 step is CRC32C (Castagnoli, reflected 0x82F63B78): "12345678"
+    as one little-endian word from the all-ones state, then the byte "9" by the
+    textbook bitwise step, gives the published check value 0xE3069283."""
+    c = orc.crc32c_u64(0xFFFFFFFF, int.from_bytes(b"12345678", "little")) ^ ord("9")
+    for _ in range(8):
+        c = (c >> 1) ^ (0x82F63B78 if c & 1 else 0)
+    assert c ^ 0xFFFFFFFF == 0xE3069283
+    assert orc.crc32c_u64(0, 0) == 0
+
+
+def test_pred_known_streams():
+    # FCM of 5, 5, 5: context 0 predicts 0, the context of 5 is new the first
+    # time (predicts 0) and then predicts 5
+    assert orc.pred_encode(0, np.array([5, 5, 5], dtype=np.uint64), 16) == bytes([5, 5, 0])
+    # DFCM of 10, 20, 30, 40: 10 ^ 0, then 20 ^ (10 + 0) = 30, then the delta
+    # 10 repeats at the delta context of 10: residuals 0
+    assert orc.pred_encode(25, np.array([10, 20, 30, 40], dtype=np.uint64), 16) == bytes([10, 30, 0, 0])
+    for kind in (0, 25):
+        assert orc.pred_encode(kind, np.array([], dtype=np.uint64), 16) == b""
+
+
+def _last_occurrence_residuals(kind, v, log2):
+    """The same residuals by another route: sort positions by (context,
+    index); each position's predictor comes from its predecessor in that
+    order when the context matches."""
+    mask = (1 << log2) - 1
+    n = len(v)
+    ctx = [0] * n
+    for i in range(1, n):
+        h = int(v[i - 1]) if kind == 0 else (int(v[i - 1]) - (int(v[i - 2]) if i > 1 else 0)) & M64
+        ctx[i] = orc.crc32c_u64(0x9E3779B9, h) & mask
+    order = sorted(range(n), key=lambda i: (ctx[i], i))
+    res = [0] * n
+    for p, i in enumerate(order):
+        j = order[p - 1] if p > 0 and ctx[order[p - 1]] == ctx[i] else None
+        if kind == 0:
+            pred = int(v[j]) if j is not None else 0
+        else:
+            dj = (int(v[j]) - (int(v[j - 1]) if j > 0 else 0)) & M64 if j is not None else 0
+            pred = ((int(v[i - 1]) if i > 0 else 0) + dj) & M64
+        res[i] = int(v[i]) ^ pred
+    return b"".join(_varint(x) for x in res)
+
+
+def test_pred_matches_last_occurrence_formulation():
+    r = np.random.default_rng(17)
+    cases = [r.integers(0, 8, 3000, dtype=np.uint64),                       # many repeats
+             np.cumsum(r.integers(0, 3, 3000)).astype(np.uint64),          # ramps
+             r.integers(0, 1 << 63, 500, dtype=np.uint64),
+             (np.arange(2000) % 37 * 1000).astype(np.uint64)]
+    for v in cases:
+        for kind in (0, 25):
+            for log2 in (4, 10, 16):
+                assert orc.pred_encode(kind, v, log2) == _last_occurrence_residuals(kind, v, log2)
+
+
+def test_pred_round_trip_and_errors():
+    r = np.random.default_rng(18)
+    v = np.concatenate([r.integers(0, 50, 4000), np.arange(4000) * 7]).astype(np.uint64)
+    for kind in (0, 25):
+        e = orc.pred_encode(kind, v, 12)
+        rc, back = orc.pred_decode(kind, e, len(v), 12)
+        assert rc == 0 and back.tolist() == v.tolist()
+        assert orc.pred_decode(kind, e + b"\x00", len(v), 12)[0] != 0      # trailing bytes
+        assert orc.pred_decode(kind, e[:-1], len(v), 12)[0] != 0           # truncated
+        assert orc.pred_decode(kind, e, len(v) + 1, 12)[0] != 0            # one value short
