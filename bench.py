@@ -238,7 +238,8 @@ def linear_blocks(qa, gpu, torch, V, block, kind, order=0):
 
 
 def resid_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=3):
-    """The linear residual coders (src/codecs/linear.rs) on n f64 values
+    """The linear residual coders (src/codecs/linear.rs) and the FCM / DFCM
+    predictors (src/codecs/pred.rs) on n f64 values
     sin(2 pi i / 1000) * 1e3 + N(0, 1e-3) in 32768-value blocks, every block of
     a kind in one call: encode = select_order + the residual streams, decode =
     the varint parse + the recurrence, checked against the input. GB/s are of
@@ -296,6 +297,38 @@ def resid_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=3):
                                "unit": "GB/s",
                                "frac": round(alg / (ek * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
             samples = [f64[b * block:(b + 1) * block].view(np.uint64) for b in range(0, nb, 64)]
+        del out, D
+    # FCM / DFCM (src/codecs/pred.rs), predictor_log2 16, on the same blocks
+    cap = int(qa.lib.quoin_linear_bound(block))
+    for name, kind in (("fcm", qa.PRED_FCM), ("dfcm", qa.PRED_DFCM)):
+        out = torch.empty(nb * cap, dtype=torch.uint8, device=dev)
+        ins = [qa.LinearInput(V.data_ptr() + 8 * b * block, block, out.data_ptr() + b * cap, cap)
+               for b in range(nb)]
+        lens = gpu.pred_encode(kind, ins, 16)
+        D = torch.zeros_like(V)
+        ss = [qa.LinearStream(ins[b].dst, lens[b], block, D.data_ptr() + 8 * b * block) for b in range(nb)]
+        st = gpu.pred_decode(kind, ss, 16)
+        torch.cuda.synchronize()
+        assert all(x == qa.OK for x in st) and torch.equal(D[:nb * block], V[:nb * block]), \
+            f"{name}: round trip mismatch"
+        for _ in range(warmup):
+            gpu.pred_encode(kind, ins, 16)
+        ek = 0.0
+        for _ in range(steps):
+            gpu.pred_encode(kind, ins, 16)
+            ek += gpu.last_timing()[1]
+        dt = dk = 0.0
+        for _ in range(max(steps // 5, 1)):
+            gpu.pred_decode(kind, ss, 16)
+            t, k, _ = gpu.last_timing()
+            dt += t
+            dk += k
+        ek, dt, dk = ek / steps, dt / max(steps // 5, 1), dk / max(steps // 5, 1)
+        res["kinds"][name] = {
+            "stream_bytes": int(sum(lens)), "predictor_log2": 16,
+            "encode_ms": round(ek, 4), "encode_gbs": round(raw / (ek * 1e-3) / 1e9, 1),
+            "decode_ms": round(dt, 4), "decode_gbs": round(raw / (dt * 1e-3) / 1e9, 2),
+            "recurrence_ms": round(dk, 4)}
         del out, D
     del V
     return res, samples
