@@ -95,8 +95,12 @@ size_t quoin_typed_compress_bound(int32_t dtype, size_t n);
 /* capi/include/quoin.h:128-129 — returns bytes written, 0 on any failure. */
 size_t quoin_typed_compress(int32_t dtype, size_t n, const void *values, const uint8_t *validity,
                             uint8_t *dst, size_t dst_cap);
-/* capi/include/quoin.h:136-138 — decimals are outside this build's path:
- * always returns 0 (the documented failure value). */
+/* capi/include/quoin.h:136-138 (capi/src/typed.rs:176-222, src/decimal.rs):
+ * the decimal container, byte-identical to the reference at the same level.
+ * DECIMAL32 / 64 widen to the 128-bit container. vmin and the 64-bit limbs of
+ * v - vmin are computed on the device, each limb is compressed by the engine
+ * as a U64 column, and the host lays out the container. Returns bytes written,
+ * 0 on any failure. quoin_typed_decompress decodes the container. */
 size_t quoin_typed_compress_decimal(int32_t dtype, size_t n, const void *values,
                                     const uint8_t *validity, int8_t scale, uint8_t precision,
                                     uint8_t *dst, size_t dst_cap);

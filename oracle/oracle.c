@@ -2592,3 +2592,171 @@ int orc_linear_decode(int kind, const uint8_t *pl, size_t len, size_t n, uint64_
     }
     return pos == rl ? ORC_OK : ORC_CORRUPT;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Decimal container (decimal.rs): Decimal128 / Decimal256 significands as  */
+/* 64-bit limbs of v - vmin, each compressed as a U64 column                 */
+/* ------------------------------------------------------------------------ */
+#define FLAG_VALIDITY 1
+#define FLAG_DECIMAL 2
+
+/* signed a < b over nl little-endian limbs (decimal.rs:286-302) */
+static int limbs_lt(const uint64_t *a, const uint64_t *b, int nl) {
+    uint64_t sa = a[nl - 1] >> 63, sb = b[nl - 1] >> 63;
+    if (sa != sb) return sa == 1;
+    for (int i = nl - 1; i >= 0; i--)
+        if (a[i] != b[i]) return a[i] < b[i];
+    return 0;
+}
+/* wire dtype 6 (Decimal128, 2 limbs) or 7 (Decimal256, 4 limbs); values are
+ * nl * 8-byte little-endian two's-complement significands (decimal.rs:174-212,
+ * 339-377; the container of decimal.rs:43-84) */
+int orc_compress_decimal(int wire, size_t n, const uint8_t *values, const uint8_t *validity,
+                         int8_t scale, uint8_t precision, int level, uint8_t **outp,
+                         size_t *out_len) {
+    if (wire != 6 && wire != 7) return ORC_BAD_DTYPE;
+    const int nl = wire == 6 ? 2 : 4;
+    int has_nulls = validity && count_valid(validity, n) < n;
+    size_t nv = has_nulls ? count_valid(validity, n) : n;
+    uint64_t *val = (uint64_t *)malloc(8 * nl * (nv ? nv : 1));
+    size_t j = 0;
+    for (size_t i = 0; i < n; i++)
+        if (!has_nulls || ((validity[i >> 3] >> (i & 7)) & 1)) {
+            for (int l = 0; l < nl; l++) val[j * nl + l] = rd_u64le(values + (i * nl + l) * 8);
+            j++;
+        }
+    uint64_t vmin[4] = {0, 0, 0, 0};
+    for (size_t k = 0; k < nv; k++)
+        if (k == 0 || limbs_lt(val + k * nl, vmin, nl)) memcpy(vmin, val + k * nl, 8 * nl);
+    buf_t out = {0};
+    buf_append(&out, "FCR1", 4);
+    buf_push(&out, 2);
+    buf_push(&out, FLAG_DECIMAL | (has_nulls ? FLAG_VALIDITY : 0));
+    buf_push(&out, 0);
+    buf_push(&out, (uint8_t)wire);
+    buf_u64le(&out, n);
+    buf_push(&out, (uint8_t)scale);
+    buf_push(&out, precision);
+    buf_push(&out, (uint8_t)nl);
+    for (int l = 0; l < nl; l++) buf_u64le(&out, vmin[l]);
+    if (has_nulls) {
+        buf_t vb = {0};
+        validity_encode(validity, n, &vb);
+        varint_write(&out, vb.len);
+        buf_append(&out, vb.p, vb.len);
+        buf_free(&vb);
+    }
+    uint64_t *limb = (uint64_t *)malloc(8 * (nv ? nv : 1));
+    int rc = ORC_OK;
+    for (int l = 0; l < nl && rc == ORC_OK; l++) {
+        uint64_t borrow = 0;
+        (void)borrow;
+        for (size_t k = 0; k < nv; k++) { /* limb l of v - vmin (decimal.rs:304-318) */
+            uint64_t br = 0, d = 0;
+            for (int q = 0; q <= l; q++) {
+                uint64_t a = val[k * nl + q], b = vmin[q];
+                uint64_t d1 = a - b, b1 = a < b;
+                uint64_t d2 = d1 - br, b2 = d1 < br;
+                d = d2;
+                br = b1 | b2;
+            }
+            limb[k] = d;
+        }
+        uint8_t *sub = NULL;
+        size_t sl = 0;
+        rc = orc_compress_column(ORC_U64, nv, limb, NULL, level, 0, &sub, &sl);
+        if (rc == ORC_OK) {
+            varint_write(&out, sl);
+            buf_append(&out, sub, sl);
+            free(sub);
+        }
+    }
+    free(limb);
+    free(val);
+    if (rc != ORC_OK || out.oom) {
+        buf_free(&out);
+        return rc ? rc : ORC_NOMEM;
+    }
+    *outp = out.p;
+    *out_len = out.len;
+    return ORC_OK;
+}
+
+/* decimal.rs:100-170, 214-240, 379-435: values (malloc'ed, n * nl * 8 bytes,
+ * null slots 0), validity (malloc'ed bitmap or NULL) */
+int orc_decompress_decimal(const uint8_t *src, size_t len, int *wire, size_t *n_out,
+                           int8_t *scale, uint8_t *precision, uint8_t **values, uint8_t **validity) {
+    if (len < 19) return ORC_TRUNCATED;
+    if (memcmp(src, "FCR1", 4) != 0) return ORC_BAD_MAGIC;
+    if (src[4] != 2) return ORC_BAD_VERSION;
+    if (!(src[5] & FLAG_DECIMAL)) return ORC_CORRUPT;
+    int has_v = src[5] & FLAG_VALIDITY;
+    int w = src[7];
+    if (w > 7) return ORC_BAD_DTYPE;
+    if (w != 6 && w != 7) return ORC_CORRUPT; /* not a decimal dtype */
+    uint64_t n = rd_u64le(src + 8);
+    int8_t sc = (int8_t)src[16];
+    uint8_t pr = src[17];
+    int nl = src[18];
+    if (nl < 1 || nl > 4) return ORC_CORRUPT;
+    size_t pos = 19 + 8 * (size_t)nl;
+    if (pos > len) return ORC_TRUNCATED;
+    const uint8_t *vmin_p = src + 19;
+    uint8_t *bm = NULL;
+    int rc;
+    if (has_v) {
+        uint64_t vlen;
+        if ((rc = varint_read(src, len, &pos, &vlen))) return rc;
+        if (vlen > len - pos) return ORC_TRUNCATED;
+        if ((rc = validity_decode(src + pos, (size_t)vlen, (size_t)n, &bm))) return rc;
+        pos += vlen;
+    }
+    size_t nv = bm ? count_valid(bm, (size_t)n) : (size_t)n;
+    uint64_t *limbs[4] = {NULL, NULL, NULL, NULL};
+    rc = ORC_OK;
+    for (int l = 0; l < nl && rc == ORC_OK; l++) {
+        uint64_t sl;
+        if ((rc = varint_read(src, len, &pos, &sl))) break;
+        if (sl > len - pos) {
+            rc = ORC_TRUNCATED;
+            break;
+        }
+        int dt;
+        size_t m;
+        void *lv = NULL;
+        uint8_t *lb = NULL;
+        rc = orc_decompress_column(src + pos, (size_t)sl, &dt, &m, &lv, &lb);
+        free(lb);
+        if (rc == ORC_OK && m != nv) rc = ORC_CORRUPT;
+        limbs[l] = (uint64_t *)lv;
+        pos += sl;
+    }
+    if (rc == ORC_OK && nl != (w == 6 ? 2 : 4)) rc = ORC_CORRUPT;
+    if (rc != ORC_OK) {
+        for (int l = 0; l < 4; l++) free(limbs[l]);
+        free(bm);
+        return rc;
+    }
+    uint8_t *out = (uint8_t *)calloc(n ? n * nl * 8 : 1, 1);
+    size_t k = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (bm && !((bm[i >> 3] >> (i & 7)) & 1)) continue;
+        uint64_t carry = 0;
+        for (int l = 0; l < nl; l++) { /* vmin + offset, wrapping (decimal.rs:320-335) */
+            uint64_t a = rd_u64le(vmin_p + 8 * l), b = limbs[l][k];
+            uint64_t s1 = a + b, c1 = s1 < a;
+            uint64_t s2 = s1 + carry, c2 = s2 < s1;
+            carry = c1 | c2;
+            for (int q = 0; q < 8; q++) out[(i * nl + l) * 8 + q] = (uint8_t)(s2 >> (8 * q));
+        }
+        k++;
+    }
+    for (int l = 0; l < 4; l++) free(limbs[l]);
+    *wire = w;
+    *n_out = (size_t)n;
+    *scale = sc;
+    *precision = pr;
+    *values = out;
+    *validity = bm;
+    return ORC_OK;
+}

@@ -110,6 +110,15 @@ int orc_linear_select_order(const uint64_t *vals, size_t n);
 uint8_t *orc_linear_encode(int kind, const uint64_t *vals, size_t n, int order, size_t *len);
 int orc_linear_decode(int kind, const uint8_t *payload, size_t len, size_t n, uint64_t *out);
 
+/* Decimal container (decimal.rs). wire 6 = Decimal128 (2 limbs), 7 =
+ * Decimal256 (4 limbs); values are n significands of limbs * 8 little-endian
+ * two's-complement bytes. decompress returns malloc'ed values (null slots 0)
+ * and the bitmap (or NULL). */
+int orc_compress_decimal(int wire, size_t n, const uint8_t *values, const uint8_t *validity,
+                         int8_t scale, uint8_t precision, int level, uint8_t **out, size_t *out_len);
+int orc_decompress_decimal(const uint8_t *src, size_t len, int *wire, size_t *n, int8_t *scale,
+                           uint8_t *precision, uint8_t **values, uint8_t **validity);
+
 /* Feature probe + planning (encoder.rs:154-178, 705-750). ranges gets
  * (start,end) pairs; returns the number of blocks. ranges must hold n/1+1 pairs
  * in the worst case of block_size 1; callers size it as n+1 pairs. */

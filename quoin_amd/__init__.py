@@ -51,6 +51,13 @@ class DType(enum.IntEnum):
     U64 = 8
     F32 = 9
     F64 = 10
+    DECIMAL32 = 11
+    DECIMAL64 = 12
+    DECIMAL128 = 13
+    DECIMAL256 = 14
+
+
+DECIMAL_WIDTH = {DType.DECIMAL32: 4, DType.DECIMAL64: 8, DType.DECIMAL128: 16, DType.DECIMAL256: 32}
 
 
 NP = {DType.BOOL: np.uint8, DType.I8: np.int8, DType.I16: np.int16, DType.I32: np.int32,
@@ -143,6 +150,9 @@ def _load():
     lib.quoin_typed_compress.restype = sz
     lib.quoin_typed_compress.argtypes = [ctypes.c_int32, sz, vp, u8p, u8p, sz]
     lib.quoin_typed_decompress.argtypes = [ctypes.c_int32, sz, u8p, sz, vp, u8p]
+    lib.quoin_typed_compress_decimal.restype = sz
+    lib.quoin_typed_compress_decimal.argtypes = [ctypes.c_int32, sz, vp, u8p, ctypes.c_int8,
+                                                 ctypes.c_uint8, u8p, sz]
     lib.quoin_compress.argtypes = [vp, sz, u8p, sz, ctypes.POINTER(sz)]
     lib.quoin_compress_cfg.argtypes = [ctypes.POINTER(_CConfig), vp, sz, u8p, sz,
                                        ctypes.POINTER(sz)]
@@ -254,6 +264,38 @@ def compress_column(values, validity=None, config=None, dtype=None):
     if got == 0:
         raise QuoinError(lib.quoin_last_error(), "compress failed")
     return dst[:got].tobytes()
+
+
+def compress_decimal(values, dtype, validity=None, scale=0, precision=38):
+    """Decimal column -> stream bytes (src/decimal.rs container; capi
+    quoin_typed_compress_decimal). values: bytes of n little-endian
+    significands of DECIMAL_WIDTH[dtype] bytes each."""
+    dtype = DType(dtype)
+    raw = np.frombuffer(bytes(values), dtype=np.uint8)
+    n = len(raw) // DECIMAL_WIDTH[dtype]
+    if validity is not None:
+        validity = np.ascontiguousarray(validity, dtype=np.uint8)
+    cap = 40 * n + 8192  # four U64 limb streams at their bound, and the container
+    dst = np.empty(cap, dtype=np.uint8)
+    got = lib.quoin_typed_compress_decimal(int(dtype), n, _ptr(raw) or ctypes.c_void_p(dst.ctypes.data),
+                                           _ptr(validity), scale, precision,
+                                           ctypes.c_void_p(dst.ctypes.data), cap)
+    if got == 0:
+        raise QuoinError(lib.quoin_last_error(), "decimal compress failed")
+    return dst[:got].tobytes()
+
+
+def decompress_decimal(stream, dtype, n, with_validity=False):
+    """Decimal stream -> (significand bytes, validity bytes or None)."""
+    dtype = DType(dtype)
+    src = np.frombuffer(stream, dtype=np.uint8)
+    out = np.empty(max(n * DECIMAL_WIDTH[dtype], 1), dtype=np.uint8)
+    vo = np.empty(max((n + 7) // 8, 1), dtype=np.uint8) if with_validity else None
+    rc = lib.quoin_typed_decompress(int(dtype), n, ctypes.c_void_p(src.ctypes.data), len(src),
+                                    ctypes.c_void_p(out.ctypes.data), _ptr(vo))
+    if rc != OK:
+        raise QuoinError(rc)
+    return out[:n * DECIMAL_WIDTH[dtype]].tobytes(), (vo[:(n + 7) // 8].tobytes() if vo is not None else None)
 
 
 def decompress_column(stream, dtype, n, with_validity=False):

@@ -82,6 +82,8 @@ struct QuoinGpu {
     // FCM / DFCM predictor arenas: sort keys and indices, residual words, sort
     // scratch, decode tables
     DevBuf pkeys, pres, psort, ptab;
+    // decimal container: partial minima, vmin and limbs
+    DevBuf dm;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, kev0 = nullptr, kev1 = nullptr;
     double last_total_ms = 0, last_kernel_ms = 0;
     const char *last_kernel = "";
@@ -288,7 +290,7 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
                      &g->dscr, &g->dscr2, &g->dkrun,
                      &g->st_out, &g->st_vout, &g->rdesc, &g->rres, &g->rtabs, &g->rscr,
                      &g->ldesc, &g->lres, &g->ltile, &g->lscan, &g->pkeys, &g->pres,
-                     &g->psort, &g->ptab};
+                     &g->psort, &g->ptab, &g->dm};
     for (DevBuf *b : all)
         if (b->p) hipFree(b->p);
     hipEventDestroy(g->ev0);
@@ -1168,15 +1170,251 @@ extern "C" size_t quoin_typed_compress(int32_t dtype, size_t n, const void *valu
     return quoin_typed_compress_cfg(nullptr, dtype, n, values, validity, dst, dst_cap);
 }
 
-extern "C" size_t quoin_typed_compress_decimal(int32_t, size_t, const void *, const uint8_t *,
-                                               int8_t, uint8_t, uint8_t *, size_t) {
-    return 0;  // the decimal container is outside this build's path
+// ---------------------------------------------------------------------------
+// decimal container (src/decimal.rs; capi/src/typed.rs:176-222 for the dtypes)
+// ---------------------------------------------------------------------------
+namespace {
+// bytes per significand of a decimal dtype and its container's wire dtype
+// (6 = Decimal128, 7 = Decimal256; DECIMAL32 / 64 widen to Decimal128)
+bool decimal_dtype(int32_t dtype, uint32_t *in_w, uint32_t *wire) {
+    switch (dtype) {
+    case QUOIN_DTYPE_DECIMAL32: *in_w = 4; *wire = 6; return true;
+    case QUOIN_DTYPE_DECIMAL64: *in_w = 8; *wire = 6; return true;
+    case QUOIN_DTYPE_DECIMAL128: *in_w = 16; *wire = 6; return true;
+    case QUOIN_DTYPE_DECIMAL256: *in_w = 32; *wire = 7; return true;
+    default: return false;
+    }
+}
+void put_varint(std::vector<uint8_t> &o, uint64_t v) {
+    while (v >= 0x80) {
+        o.push_back((uint8_t)(v | 0x80));
+        v >>= 7;
+    }
+    o.push_back((uint8_t)v);
+}
+void put_header(std::vector<uint8_t> &o, uint8_t flags, uint8_t plog2, uint8_t wire, uint64_t n) {
+    const uint8_t h[8] = {'F', 'C', 'R', '1', 2, flags, plog2, wire};
+    o.insert(o.end(), h, h + 8);
+    for (int i = 0; i < 8; i++) o.push_back((uint8_t)(n >> (8 * i)));
+}
+}  // namespace
+
+// The significands' limbs of v - vmin (device), each compressed by the engine
+// as a U64 column under the column's bitmap in one batched call; the host
+// then lays out the container (decimal.rs:43-84): the header, scale /
+// precision / limb count / vmin, the validity blob once (the limb streams all
+// carry it), and each limb's stream re-headed as the reference's sub-stream
+// (no validity, n = valid values) behind its varint length.
+static long long host_compress_decimal(int32_t dtype, size_t n, const void *values,
+                                       const uint8_t *validity, int8_t scale, uint8_t precision,
+                                       uint8_t *dst, size_t dst_cap) {
+    uint32_t in_w, wire;
+    if (!decimal_dtype(dtype, &in_w, &wire)) return QUOIN_ERR_UNSUPPORTED;
+    const uint32_t nl = wire == 6 ? 2 : 4;
+    QuoinGpu *g = default_gpu();
+    if (!g) return QUOIN_ERR_DEVICE;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    hipStream_t s = g->stream;
+    const size_t vbytes = (n + 7) / 8;
+    const size_t lbound = quoin_typed_compress_bound(QUOIN_DTYPE_U64, n);
+    const size_t dm_words = QA_DECM_PARTS * 5 + 4 + (size_t)nl * n;
+    if (!ensure(g->st_in, n * in_w + 8) || !ensure(g->st_out, nl * lbound) ||
+        (validity && !ensure(g->st_valid, vbytes + 8)) || !ensure(g->dm, dm_words * 8))
+        return QUOIN_ERR_DEVICE;
+    if (n) hipMemcpyAsync(g->st_in.p, values, n * in_w, hipMemcpyHostToDevice, s);
+    if (validity && vbytes) hipMemcpyAsync(g->st_valid.p, validity, vbytes, hipMemcpyHostToDevice, s);
+    DecmArgs a{};
+    a.values = (const uint8_t *)g->st_in.p;
+    a.bitmap = validity ? (const uint8_t *)g->st_valid.p : nullptr;
+    a.n = n;
+    a.in_w = in_w;
+    a.nl = nl;
+    a.part = dp<uint64_t>(g->dm);
+    a.vmin = a.part + QA_DECM_PARTS * 5;
+    a.limbs = a.vmin + 4;
+    qa_decm_limbs_launch(a, s);
+    std::vector<QuoinDeviceColumn> cols(nl);
+    for (uint32_t l = 0; l < nl; l++) {
+        cols[l] = QuoinDeviceColumn{};
+        cols[l].dtype = QUOIN_DTYPE_U64;
+        cols[l].n = n;
+        cols[l].values = a.limbs + (size_t)l * n;
+        cols[l].validity = a.bitmap;
+        cols[l].dst = (uint8_t *)g->st_out.p + l * lbound;
+        cols[l].dst_cap = lbound;
+    }
+    uint64_t lens[4] = {0, 0, 0, 0};
+    int rc = gpu_compress_locked(g, nullptr, cols.data(), nl, lens, s);
+    if (rc) return rc;
+    uint64_t vmin[4] = {0, 0, 0, 0};
+    std::vector<std::vector<uint8_t>> st(nl);
+    hipMemcpyAsync(vmin, a.vmin, 8 * nl, hipMemcpyDeviceToHost, s);
+    for (uint32_t l = 0; l < nl; l++) {
+        st[l].resize(lens[l]);
+        hipMemcpyAsync(st[l].data(), cols[l].dst, lens[l], hipMemcpyDeviceToHost, s);
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return QUOIN_ERR_DEVICE;
+    uint64_t nv = n;
+    if (validity) {
+        nv = 0;
+        for (size_t i = 0; i < n; i++) nv += (validity[i >> 3] >> (i & 7)) & 1;
+    }
+    const bool has_v = nv < n;
+    std::vector<uint8_t> out;
+    put_header(out, 0x02 | (has_v ? 0x01 : 0x00), 0, (uint8_t)wire, n);
+    out.push_back((uint8_t)scale);
+    out.push_back(precision);
+    out.push_back((uint8_t)nl);
+    for (uint32_t l = 0; l < nl; l++)
+        for (int b = 0; b < 8; b++) out.push_back((uint8_t)(vmin[l] >> (8 * b)));
+    for (uint32_t l = 0; l < nl; l++) {
+        const std::vector<uint8_t> &x = st[l];
+        if (x.size() < 16 || ((x[5] & 1) != 0) != has_v) return QUOIN_ERR_PANIC;
+        size_t pos = 16;
+        if (has_v) {
+            uint64_t vlen;
+            if (!rd_varint(x.data(), x.size(), &pos, &vlen) || vlen > x.size() - pos)
+                return QUOIN_ERR_PANIC;
+            if (l == 0) {
+                put_varint(out, vlen);
+                out.insert(out.end(), x.begin() + (ptrdiff_t)pos, x.begin() + (ptrdiff_t)(pos + vlen));
+            }
+            pos += vlen;
+        }
+        std::vector<uint8_t> sub;
+        put_header(sub, 0, x[6], 2 /* U64 */, nv);
+        sub.insert(sub.end(), x.begin() + (ptrdiff_t)pos, x.end());
+        put_varint(out, sub.size());
+        out.insert(out.end(), sub.begin(), sub.end());
+    }
+    if (out.size() > dst_cap) return -(long long)1000 - (long long)out.size();
+    memcpy(dst, out.data(), out.size());
+    return (long long)out.size();
+}
+
+// The container parsed on the host (decimal.rs:100-170); each limb's sub-stream
+// gets the container's validity blob back and is decoded by the engine to
+// logical positions in one batched call (the bitmap comes out of limb 0), then
+// vmin + limbs are joined on the device into the caller's width, nulls 0.
+// Sub-streams that the reference encoder never writes (their own validity, or
+// a dtype whose lanes are not 8-byte words) are refused as corrupt.
+static int host_decompress_decimal(int32_t dtype, size_t n, const uint8_t *src, size_t len,
+                                   void *values_out, uint8_t *validity_out) {
+    uint32_t out_w, want_wire;
+    if (!decimal_dtype(dtype, &out_w, &want_wire)) return QUOIN_ERR_CORRUPT;  // column type mismatch
+    if (len < 19 || memcmp(src, "FCR1", 4) != 0 || src[4] != 2) return QUOIN_ERR_CORRUPT;
+    const bool has_v = src[5] & 1;
+    const uint32_t wire = src[7];
+    const uint64_t ln = rd64(src + 8);
+    const uint32_t nl = src[18];
+    if (nl < 1 || nl > 4) return QUOIN_ERR_CORRUPT;
+    size_t pos = 19 + 8ull * nl;
+    if (pos > len) return QUOIN_ERR_CORRUPT;
+    const uint8_t *vminb = src + 19;
+    size_t boff = 0;
+    uint64_t blen = 0;
+    if (has_v) {
+        if (!rd_varint(src, len, &pos, &blen) || blen > len - pos) return QUOIN_ERR_CORRUPT;
+        boff = pos;
+        pos += blen;
+    }
+    std::vector<std::vector<uint8_t>> syn(nl);
+    for (uint32_t l = 0; l < nl; l++) {
+        uint64_t sl;
+        if (!rd_varint(src, len, &pos, &sl) || sl > len - pos) return QUOIN_ERR_CORRUPT;
+        const uint8_t *sub = src + pos;
+        ParsedStream ps;
+        if (parse_stream(sub, (size_t)sl, ps) != QUOIN_OK || ps.has_validity || sub[7] > 2)
+            return QUOIN_ERR_CORRUPT;
+        std::vector<uint8_t> &y = syn[l];
+        put_header(y, has_v ? 0x01 : 0x00, sub[6], 2, ln);
+        if (has_v) {
+            put_varint(y, blen);
+            y.insert(y.end(), src + boff, src + boff + blen);
+        } else if (ps.n != ln) {
+            return QUOIN_ERR_CORRUPT;  // the limb length is the valid count (decimal.rs:153-155)
+        }
+        y.insert(y.end(), sub + 16, sub + sl);
+        pos += sl;
+    }
+    if (wire != want_wire || nl != (wire == 6 ? 2u : 4u) || ln != n) return QUOIN_ERR_CORRUPT;
+    QuoinGpu *g = default_gpu();
+    if (!g) return QUOIN_ERR_DEVICE;
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    hipStream_t s = g->stream;
+    const size_t vbytes = (n + 7) / 8;
+    size_t total = 0;
+    for (auto &y : syn) total += y.size();
+    if (!ensure(g->st_in, total + 8) || !ensure(g->st_out, n * out_w + 8) ||
+        !ensure(g->st_vout, vbytes + 8) || !ensure(g->dm, (4 + (size_t)nl * n) * 8 + 8))
+        return QUOIN_ERR_DEVICE;
+    DecmArgs a{};
+    a.n = n;
+    a.nl = nl;
+    a.vmin = dp<uint64_t>(g->dm);
+    a.limbs = a.vmin + 4;
+    a.out = (uint8_t *)g->st_out.p;
+    a.out_w = out_w;
+    a.bitmap = has_v ? (const uint8_t *)g->st_vout.p : nullptr;
+    std::vector<QuoinDeviceStream> qs(nl);
+    size_t off = 0;
+    for (uint32_t l = 0; l < nl; l++) {
+        hipMemcpyAsync((uint8_t *)g->st_in.p + off, syn[l].data(), syn[l].size(), hipMemcpyHostToDevice, s);
+        qs[l] = QuoinDeviceStream{};
+        qs[l].dtype = QUOIN_DTYPE_U64;
+        qs[l].n = n;
+        qs[l].src = (const uint8_t *)g->st_in.p + off;
+        qs[l].host_src = syn[l].data();
+        qs[l].src_len = syn[l].size();
+        qs[l].values_out = a.limbs + (size_t)l * n;
+        qs[l].validity_out = (l == 0 && has_v) ? (uint8_t *)g->st_vout.p : nullptr;
+        off += syn[l].size();
+    }
+    hipMemcpyAsync(a.vmin, vminb, 8ull * nl, hipMemcpyHostToDevice, s);
+    int rc = gpu_decompress_locked(g, qs.data(), nl, s);
+    if (rc) return rc == QUOIN_ERR_DEVICE || rc == QUOIN_ERR_UNSUPPORTED ? rc : QUOIN_ERR_CORRUPT;
+    qa_decm_join_launch(a, s);
+    if (n && hipMemcpyAsync(values_out, a.out, n * out_w, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    if (validity_out && vbytes) {
+        if (has_v) {
+            hipMemcpyAsync(validity_out, g->st_vout.p, vbytes, hipMemcpyDeviceToHost, s);
+        } else {  // all valid, trailing bits clear (capi/src/typed.rs:280-300)
+            memset(validity_out, 0xFF, vbytes);
+            if (n % 8) validity_out[vbytes - 1] = (uint8_t)((1u << (n % 8)) - 1);
+        }
+    }
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    return QUOIN_OK;
+}
+
+extern "C" size_t quoin_typed_compress_decimal(int32_t dtype, size_t n, const void *values,
+                                               const uint8_t *validity, int8_t scale,
+                                               uint8_t precision, uint8_t *dst, size_t dst_cap) {
+    if (!values || !dst) return 0;
+    try {
+        long long r = host_compress_decimal(dtype, n, values, validity, scale, precision, dst, dst_cap);
+        t_last_err = r > 0 ? QUOIN_OK : r <= -1000 ? QUOIN_ERR_BUFFER_TOO_SMALL : (int)r;
+        return r > 0 ? (size_t)r : 0;
+    } catch (...) {
+        t_last_err = QUOIN_ERR_PANIC;
+        return 0;
+    }
 }
 
 extern "C" int quoin_typed_decompress(int32_t dtype, size_t n, const uint8_t *src, size_t src_len,
                                       void *values_out, uint8_t *validity_out) {
     if (!src || !values_out) return QUOIN_ERR_NULL;
     try {
+        uint32_t w, wire;
+        // a decimal container (src/lib.rs:465-490), or a decimal dtype asked of
+        // a lane stream: the column type does not match
+        if (src_len > 5 && (src[5] & 0x02))
+            return host_decompress_decimal(dtype, n, src, src_len, values_out, validity_out);
+        if (decimal_dtype(dtype, &w, &wire)) return QUOIN_ERR_CORRUPT;
         return host_decompress(dtype, n, src, src_len, values_out, validity_out);
     } catch (...) {
         return QUOIN_ERR_PANIC;

@@ -295,6 +295,22 @@ struct LinR {
     int32_t status;        // encode: 1 = DeltaDp declined; decode: 1 = corrupt
     uint32_t order;
 };
+// decimal container (src/decimal.rs): significands -> limbs of v - vmin, and back
+#define QA_DECM_PARTS 1024
+struct DecmArgs {
+    const uint8_t *values;  // encode: in_w-byte significands
+    const uint8_t *bitmap;  // Arrow validity bitmap (bytes) or null
+    uint64_t n;             // logical values
+    uint32_t in_w, nl;      // bytes per input value; limbs (2: Decimal128, 4: Decimal256)
+    uint64_t *part;         // encode: QA_DECM_PARTS x 5 words of partial minima
+    uint64_t *vmin;         // nl words
+    uint64_t *limbs;        // nl x n words at logical positions
+    uint8_t *out;           // decode: out_w bytes per value
+    uint32_t out_w, pad;
+};
+void qa_decm_limbs_launch(const DecmArgs &a, hipStream_t s);
+void qa_decm_join_launch(const DecmArgs &a, hipStream_t s);
+
 // FCM / DFCM predictors (kind 0 / 25), on the linear coders' descriptors
 size_t qa_pred_sort_bytes(uint64_t total);
 void qa_pred_encode_launch(uint32_t kind, const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles,

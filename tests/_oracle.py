@@ -62,6 +62,16 @@ def _load():
                                     ctypes.POINTER(ctypes.c_size_t)]
     lib.orc_pred_decode.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
                                     ctypes.c_int, ctypes.c_void_p]
+    lib.orc_compress_decimal.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_int8, ctypes.c_uint8,
+                                         ctypes.c_int, ctypes.POINTER(u8p),
+                                         ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_decompress_decimal.argtypes = [ctypes.c_void_p, ctypes.c_size_t,
+                                           ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_size_t),
+                                           ctypes.POINTER(ctypes.c_int8),
+                                           ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(u8p),
+                                           ctypes.POINTER(u8p)]
     lib.orc_code_residuals.restype = u8p
     lib.orc_code_residuals.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64,
                                        ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
@@ -233,6 +243,40 @@ def lz_decompress(stream, expected):
     if rc != 0:
         raise ValueError(f"lz rc={rc}")
     return _take(out, expected)
+
+
+def compress_decimal(values, wire, validity=None, scale=0, precision=38, level=FAST):
+    """decimal.rs container: values = bytes of n significands, 16 (wire 6,
+    Decimal128) or 32 (wire 7, Decimal256) little-endian bytes each."""
+    w = 16 if wire == 6 else 32
+    buf = np.frombuffer(bytes(values), dtype=np.uint8)
+    n = len(buf) // w
+    vb = np.ascontiguousarray(validity, dtype=np.uint8) if validity is not None else None
+    out = ctypes.POINTER(ctypes.c_uint8)()
+    ln = ctypes.c_size_t()
+    rc = lib.orc_compress_decimal(wire, n, _ptr(buf) if n else None, _ptr(vb), scale, precision,
+                                  level, ctypes.byref(out), ctypes.byref(ln))
+    if rc != 0:
+        raise RuntimeError(f"oracle decimal compress rc={rc}")
+    return _take(out, ln.value)
+
+
+def decompress_decimal(stream):
+    """(wire, values bytes, validity bitmap or None, scale, precision); raises
+    ValueError(rc) on error."""
+    buf, bp = _bytes_in(stream)
+    wire, n = ctypes.c_int(), ctypes.c_size_t()
+    sc, pr = ctypes.c_int8(), ctypes.c_uint8()
+    vals, vb = ctypes.POINTER(ctypes.c_uint8)(), ctypes.POINTER(ctypes.c_uint8)()
+    rc = lib.orc_decompress_decimal(bp, len(buf), ctypes.byref(wire), ctypes.byref(n),
+                                    ctypes.byref(sc), ctypes.byref(pr), ctypes.byref(vals),
+                                    ctypes.byref(vb))
+    if rc != 0:
+        raise ValueError(f"decimal rc={rc}", rc)
+    w = 16 if wire.value == 6 else 32
+    v = _take(vals, n.value * w)
+    bm = _take(vb, (n.value + 7) // 8) if vb else None
+    return wire.value, v, bm, sc.value, pr.value
 
 
 def crc32c_u64(crc, v):
