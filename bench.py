@@ -829,7 +829,8 @@ def main():
                    "device_ms_compress": round(dev_c_ms, 3),
                    "device_ms_decompress": round(dev_d_ms, 3),
                    "plain_form_gbs": round(plain, 3) if plain else None,
-                   "batched_over_plain": round(value / plain, 2) if plain else None},
+                   "batched_over_plain": round(value / plain, 2) if plain else None,
+                   "waves_per_workgroup": gpu.tuned_shapes()},
         "roofline": roofline,
         "encode_roofline": enc_roofline,
         "serving": serving,

@@ -195,6 +195,17 @@ int quoin_gpu_decompress(QuoinGpu *g, const QuoinDeviceStream *streams, size_t n
  * block, the per-block-launch baseline). */
 int quoin_gpu_set_launch_form(QuoinGpu *g, int form);
 
+/* No reference counterpart: the autotuner of the tile kernels' launch shape
+ * (wavefronts per workgroup of the FoR / Delta tile decoders and the FoR / Delta
+ * / FloatMult tile emitter). With on = 1 (the default) a handle times the
+ * shapes 1, 2, 4 and 8 on its first batch with at least 4096 tiles of a
+ * stage, re-running that idempotent stage, and keeps the fastest; 0 uses the
+ * fixed shape 4. Results do not depend on the shape. */
+int quoin_gpu_set_autotune(QuoinGpu *g, int on);
+/* The tuned shapes: out[0] k_dec_fl<4>, out[1] k_dec_fl<8>, out[2] k_emit_fl
+ * (0 = not tuned yet). */
+int quoin_gpu_tuned_shapes(const QuoinGpu *g, uint32_t out[3]);
+
 /* Device time (ms, HIP events on the handle's stream) of the last compress /
  * decompress call, and of its dominant kernel with its name. */
 int quoin_gpu_last_timing(const QuoinGpu *g, double *total_ms, double *kernel_ms,

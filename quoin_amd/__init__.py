@@ -192,6 +192,8 @@ def _load():
     lib.quoin_gpu_pred_decode.argtypes = [vp, ctypes.c_int32, ctypes.c_uint32,
                                           ctypes.POINTER(LinearStream), sz,
                                           ctypes.POINTER(ctypes.c_int32), vp]
+    lib.quoin_gpu_set_autotune.argtypes = [vp, ctypes.c_int]
+    lib.quoin_gpu_tuned_shapes.argtypes = [vp, ctypes.c_void_p]
     lib.quoin_residual_bound.restype = sz
     lib.quoin_residual_bound.argtypes = [sz]
     lib.quoin_gpu_code_residuals.argtypes = [vp, ctypes.c_uint64, ctypes.c_int,
@@ -228,6 +230,7 @@ EXPORTS = [
     "quoin_gpu_rans_decompress", "quoin_linear_bound", "quoin_gpu_linear_encode",
     "quoin_gpu_linear_decode", "quoin_residual_bound", "quoin_gpu_code_residuals",
     "quoin_gpu_decode_residuals", "quoin_gpu_pred_encode", "quoin_gpu_pred_decode",
+    "quoin_gpu_set_autotune", "quoin_gpu_tuned_shapes",
 ]
 
 
@@ -478,6 +481,18 @@ class Gpu:
         if rc not in (OK, ERR_CORRUPT):
             raise QuoinError(rc)
         return [int(x) for x in st]
+
+    def set_autotune(self, on):
+        rc = lib.quoin_gpu_set_autotune(self.h, 1 if on else 0)
+        if rc != OK:
+            raise QuoinError(rc)
+
+    def tuned_shapes(self):
+        """Tuned wavefronts per workgroup of k_dec_fl<4>, k_dec_fl<8>, k_emit_fl
+        (0 = not tuned yet)."""
+        out = (ctypes.c_uint32 * 3)()
+        lib.quoin_gpu_tuned_shapes(self.h, out)
+        return {"k_dec_fl<4>": out[0], "k_dec_fl<8>": out[1], "k_emit_fl": out[2]}
 
     def last_timing(self):
         t, k = ctypes.c_double(), ctypes.c_double()

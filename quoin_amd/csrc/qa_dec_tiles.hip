@@ -398,15 +398,15 @@ static inline unsigned nbt(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 // flight. (FoR of a 4-byte column has w <= 32; a stream with a wider sub-block
 // still decodes, by direct reads.)
 // ---------------------------------------------------------------------------
-template <int OW>
-__global__ void __launch_bounds__(256) k_dec_fl(DecArgs a) {
+template <int OW, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_dec_fl(DecArgs a) {
     // staged words (32 w, plus a row-word read past the last one) and the
-    // transposed values
+    // transposed values; WPB wavefronts per workgroup (the autotuned launch shape)
     constexpr int LW = OW == 4 ? 1088 : 2 * QA_TILE;
-    __shared__ __align__(16) uint32_t lds[4][LW];
+    __shared__ __align__(16) uint32_t lds[WPB][LW];
     const int lane = threadIdx.x & 63, wv = qa_wave_id();
     uint64_t t;
-    if (!dec_tile_at(a, (uint64_t)blockIdx.x * 4 + wv, &t)) return;
+    if (!dec_tile_at(a, (uint64_t)blockIdx.x * WPB + wv, &t)) return;
     const TileDec td = a.tdec[t];
     if (td.fl != OW) return;
     const uint32_t cnt = td.cnt, w = td.w;
@@ -552,19 +552,55 @@ void qa_dec_desc_launch(const DecArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_dec_desc, dim3(nbt(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
 }
 
+// k_dec_fl<OW> over n tiles at WPB wavefronts per workgroup
+template <int OW>
+static void launch_fl(const DecArgs &b, uint64_t n, uint32_t wpb, hipStream_t s) {
+    switch (wpb) {
+    case 1: hipLaunchKernelGGL((k_dec_fl<OW, 1>), dim3(nbt(n, 1)), dim3(64), 0, s, b); break;
+    case 2: hipLaunchKernelGGL((k_dec_fl<OW, 2>), dim3(nbt(n, 2)), dim3(128), 0, s, b); break;
+    case 8: hipLaunchKernelGGL((k_dec_fl<OW, 8>), dim3(nbt(n, 8)), dim3(512), 0, s, b); break;
+    default: hipLaunchKernelGGL((k_dec_fl<OW, 4>), dim3(nbt(n, 4)), dim3(256), 0, s, b);
+    }
+}
+
 void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
     if (a.t1 <= a.t0) return;
+    // launches of one tile decoder over [t0, t1) or its run table, at WPB
+    // wavefronts per workgroup
+    auto fl = [&](uint32_t kind, DecArgs b, uint64_t n, uint32_t wpb) {
+        if (kind == 4) launch_fl<4>(b, n, wpb, s);
+        else launch_fl<8>(b, n, wpb, s);
+    };
+    // the handle's tuned shape per decoder; untuned (0) decoders with enough
+    // tiles are tuned here first, on this same idempotent stage
+    uint32_t *tune = a.tune;
+    for (uint32_t kind : {4u, 8u}) {
+        const int ti = kind == 4 ? QA_TUNE_DEC4 : QA_TUNE_DEC8;
+        if (!tune || tune[ti]) continue;
+        uint64_t tiles = 0;
+        DecArgs b = a;
+        if (a.krun_all) {
+            const int i = kind == 4 ? 0 : 1;
+            tiles = a.kr_tot[i];
+            b.krun = a.krun_all + a.kr_off[i];
+            b.nkrun = a.kr_n[i];
+            b.ktotal = tiles;
+        } else {
+            tiles = a.t1 - a.t0;  // the whole range, the other kinds' tiles skipped
+        }
+        if (tiles < QA_TUNE_MIN_TILES) continue;
+        tune[ti] = qa_tune_pick([&](uint32_t w) { fl(kind, b, tiles, w); }, s);
+    }
     if (a.kev0) hipEventRecord((hipEvent_t)a.kev0, s);
-    // each kernel over the runs of frames of its kind (host side), or every
-    // kernel over the whole range, each taking its own tiles
+    const uint32_t w4 = tune && tune[QA_TUNE_DEC4] ? tune[QA_TUNE_DEC4] : 4;
+    const uint32_t w8 = tune && tune[QA_TUNE_DEC8] ? tune[QA_TUNE_DEC8] : 4;
     auto launch = [&](uint32_t kind, uint64_t t0, uint64_t t1) {
         DecArgs b = a;
         b.t0 = t0;
         b.t1 = t1;
-        const dim3 g(nbt(t1 - t0, 4));
-        if (kind == 4) hipLaunchKernelGGL(k_dec_fl<4>, g, dim3(256), 0, s, b);
-        else if (kind == 8) hipLaunchKernelGGL(k_dec_fl<8>, g, dim3(256), 0, s, b);
-        else hipLaunchKernelGGL(k_dec_tiles, g, dim3(256), 0, s, b);
+        if (kind == 4) fl(4, b, t1 - t0, w4);
+        else if (kind == 8) fl(8, b, t1 - t0, w8);
+        else hipLaunchKernelGGL(k_dec_tiles, dim3(nbt(t1 - t0, 4)), dim3(256), 0, s, b);
     };
     if (a.krun_all) {
         const uint32_t kinds[3] = {4, 8, 0};
@@ -574,10 +610,8 @@ void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
             b.krun = a.krun_all + a.kr_off[i];
             b.nkrun = a.kr_n[i];
             b.ktotal = a.kr_tot[i];
-            const dim3 g(nbt(b.ktotal, 4));
-            if (kinds[i] == 4) hipLaunchKernelGGL(k_dec_fl<4>, g, dim3(256), 0, s, b);
-            else if (kinds[i] == 8) hipLaunchKernelGGL(k_dec_fl<8>, g, dim3(256), 0, s, b);
-            else hipLaunchKernelGGL(k_dec_tiles, g, dim3(256), 0, s, b);
+            if (kinds[i] == 0) hipLaunchKernelGGL(k_dec_tiles, dim3(nbt(b.ktotal, 4)), dim3(256), 0, s, b);
+            else fl(kinds[i], b, b.ktotal, kinds[i] == 4 ? w4 : w8);
         }
     } else if (a.runs) {
         for (uint32_t r = 0; r < a.nruns; r++) {

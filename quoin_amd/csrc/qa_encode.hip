@@ -1165,10 +1165,11 @@ __device__ __forceinline__ void emit_fl_t(const TileEmit &e, uint32_t *outw, int
     wave_store_bytes(q + 9, outw, 128 * w, lane);
 }
 
-__global__ void __launch_bounds__(256) k_emit_fl(EncArgs a) {
-    __shared__ __align__(16) uint32_t outs[4][QA_TILE];
+template <int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_emit_fl(EncArgs a) {
+    __shared__ __align__(16) uint32_t outs[WPB][QA_TILE];
     const int lane = threadIdx.x & 63, wv = qa_wave_id();
-    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
+    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * WPB + wv;
     if (t >= a.t1) return;
     const TileEmit e = a.temit[t];
     if (!e.fast) return;
@@ -1264,8 +1265,21 @@ void qa_encode_emit_launch(const EncArgs &a, hipStream_t s) {
     QA_LAUNCH(k_emit_frames, dim3(nb(a.r1 - a.r0, 64)), dim3(64), 0, s, a);
     hipLaunchKernelGGL(k_tlist_clear, dim3(1), dim3(64), 0, s, a);
     QA_LAUNCH(k_tile_desc, dim3(nb(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
+    // k_emit_fl at the handle's tuned wavefronts per workgroup (tuned here on
+    // the first batch with enough tiles: the stage is idempotent)
+    auto emit_fl = [&](uint32_t wpb) {
+        const uint64_t n = a.t1 - a.t0;
+        switch (wpb) {
+        case 1: QA_LAUNCH(k_emit_fl<1>, dim3(nb(n, 1)), dim3(64), 0, s, a); break;
+        case 2: QA_LAUNCH(k_emit_fl<2>, dim3(nb(n, 2)), dim3(128), 0, s, a); break;
+        case 8: QA_LAUNCH(k_emit_fl<8>, dim3(nb(n, 8)), dim3(512), 0, s, a); break;
+        default: QA_LAUNCH(k_emit_fl<4>, dim3(nb(n, 4)), dim3(256), 0, s, a);
+        }
+    };
+    if (a.tune && !a.tune[QA_TUNE_EMIT] && a.t1 - a.t0 >= QA_TUNE_MIN_TILES)
+        a.tune[QA_TUNE_EMIT] = qa_tune_pick(emit_fl, s);
     if (a.kev0) hipEventRecord((hipEvent_t)a.kev0, s);
-    QA_LAUNCH(k_emit_fl, dim3(nb(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
+    emit_fl(a.tune && a.tune[QA_TUNE_EMIT] ? a.tune[QA_TUNE_EMIT] : 4);
     {
         // persistent over the list: at most four groups per CU
         const unsigned g = nb(a.t1 - a.t0, 4);

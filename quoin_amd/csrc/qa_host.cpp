@@ -84,10 +84,59 @@ struct QuoinGpu {
     DevBuf pkeys, pres, psort, ptab;
     // decimal container: partial minima, vmin and limbs
     DevBuf dm;
+    // autotuned launch shapes of the tile kernels (QA_TUNE_*, 0 = untuned)
+    uint32_t tune[QA_TUNE_N] = {0, 0, 0};
+    bool autotune = true;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, kev0 = nullptr, kev1 = nullptr;
     double last_total_ms = 0, last_kernel_ms = 0;
     const char *last_kernel = "";
 };
+
+// The autotuner's measurement: one warm launch, then each candidate shape
+// twice, timed with HIP events on the stream; the fastest shape's best time wins
+// (ties to the smaller shape).
+uint32_t qa_tune_pick(const std::function<void(uint32_t)> &launch, hipStream_t s) {
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 4;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return 4;
+    }
+    launch(4);
+    float best = 1e30f;
+    uint32_t pick = 4;
+    for (uint32_t w : {1u, 2u, 4u, 8u}) {
+        float t = 1e30f;
+        for (int rep = 0; rep < 2; rep++) {
+            float ms = 1e30f;
+            hipEventRecord(e0, s);
+            launch(w);
+            hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+                t = ms < t ? ms : t;
+        }
+        if (t < best) {
+            best = t;
+            pick = w;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return pick;
+}
+
+extern "C" int quoin_gpu_set_autotune(QuoinGpu *g, int on) {
+    if (!g) return QUOIN_ERR_NULL;
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->autotune = on != 0;
+    return QUOIN_OK;
+}
+
+extern "C" int quoin_gpu_tuned_shapes(const QuoinGpu *g, uint32_t out[3]) {
+    if (!g || !out) return QUOIN_ERR_NULL;
+    for (int i = 0; i < QA_TUNE_N; i++) out[i] = g->tune[i];
+    return QUOIN_OK;
+}
 
 int qa_gpu_device(const QuoinGpu *g) { return g->device; }
 hipStream_t qa_gpu_stream(const QuoinGpu *g, void *stream) {
@@ -435,6 +484,7 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     if (g->form == 0) {
         a.kev0 = g->kev0;
         a.kev1 = g->kev1;
+        a.tune = g->autotune ? g->tune : nullptr;
         qa_encode_launch(a, s, level);
     } else {
         // plain form: column-level planning, then one launch sequence per
@@ -651,6 +701,7 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     if (g->form == 0) {
         a.kev0 = g->kev0;
         a.kev1 = g->kev1;
+        a.tune = g->autotune ? g->tune : nullptr;
         qa_decode_launch(a, s);
     } else {
         // plain form: one launch sequence per frame (the tile starts of

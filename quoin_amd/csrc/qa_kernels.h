@@ -29,6 +29,18 @@ struct TileEmit {
     uint32_t scale;   // FloatMult scale index
 };
 
+// Launch shapes picked by the autotuner: wavefronts per workgroup of the tile
+// kernels, per handle, 0 until tuned. A stage is tuned on the first batch with
+// at least QA_TUNE_MIN_TILES tiles of it, by timing each candidate (1, 2, 4, 8)
+// on that batch (the tile stages are idempotent) and keeping the fastest.
+#define QA_TUNE_DEC4 0  // k_dec_fl<4>
+#define QA_TUNE_DEC8 1  // k_dec_fl<8>
+#define QA_TUNE_EMIT 2  // k_emit_fl
+#define QA_TUNE_N 3
+#define QA_TUNE_MIN_TILES 4096
+#include <functional>
+uint32_t qa_tune_pick(const std::function<void(uint32_t)> &launch, hipStream_t s);
+
 // Device pointers of one encode batch (all arrays sized by the host from upper
 // bounds; counts that depend on the data live in DColState).
 struct EncArgs {
@@ -73,6 +85,8 @@ struct EncArgs {
     uint64_t t0, t1;
     // host side: HIP events recorded around the tile emit kernel, or null
     void *kev0, *kev1;
+    // host side: the handle's tuned launch shapes (QA_TUNE_*), or null
+    uint32_t *tune;
 };
 
 void qa_fast_sizes_launch(const EncArgs &a, hipStream_t s);
@@ -190,6 +204,8 @@ struct DecArgs {
     uint64_t t0, t1;
     // host side: HIP events recorded around the tile decode kernel, or null
     void *kev0, *kev1;
+    // host side: the handle's tuned launch shapes (QA_TUNE_*), or null
+    uint32_t *tune;
     // host side: runs of consecutive tiles of one tile_kind, or null
     const struct TileRun *runs;
     uint32_t nruns, pad2;

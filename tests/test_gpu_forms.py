@@ -116,3 +116,39 @@ def test_launch_form_rejects_unknown(qa):
             gpu.set_launch_form(2)
     finally:
         gpu.close()
+
+
+def test_autotuned_shapes_give_the_same_bytes(qa):
+    """A batch large enough for the autotuner (over 4096 tiles of each tile
+    stage) tunes k_dec_fl<4>, k_dec_fl<8> and k_emit_fl on its first call; the
+    streams and decodes equal those at the fixed shape and the oracle's."""
+    r = np.random.default_rng(61)
+    n = 5 << 20
+    ts = (1_700_000_000_000_000_000 + np.cumsum(r.integers(1, 1001, n))).astype(np.int64)
+    u = (r.integers(0, 2**32, n, dtype=np.uint64) >> np.uint64(21)).astype(np.uint32)
+    u, bm = _data.with_nulls(u, 0.01, 62)
+    cols = [(ts, None), (u, bm)]
+    cfg = qa.Config(level=qa.Level.FASTEST)
+    tuned = qa.Gpu(0)
+    fixed = qa.Gpu(0)
+    try:
+        fixed.set_autotune(False)
+        print("case autotune: tuned handle", flush=True)
+        a = _gpu.device_compress(qa, tuned, cols, cfg)
+        print("case autotune: fixed handle", flush=True)
+        b = _gpu.device_compress(qa, fixed, cols, cfg)
+        assert a == b
+        for (v, m), st in zip(cols, a):
+            assert st == orc.compress(v, ODT[v.dtype], m, orc.FASTEST)
+        dts = [qa.FROM_NP[v.dtype] for v, _ in cols]
+        oa = _gpu.device_decompress(qa, tuned, a, dts, [n, n], True)
+        ob = _gpu.device_decompress(qa, fixed, a, dts, [n, n], True)
+        for (v, _), (x, vx), (y, vy) in zip(cols, oa, ob):
+            assert x.tobytes() == y.tobytes() == v.tobytes() and vx == vy
+        shapes = tuned.tuned_shapes()
+        print("tuned shapes:", shapes, flush=True)
+        assert all(s in (1, 2, 4, 8) for s in shapes.values()), shapes
+        assert all(s == 0 for s in fixed.tuned_shapes().values())
+    finally:
+        tuned.close()
+        fixed.close()
