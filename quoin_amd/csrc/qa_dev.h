@@ -50,6 +50,20 @@ __device__ __forceinline__ uint64_t lane_get_t(const void *p, uint32_t lk, uint6
     return qa_lane_get(p, lk, i);
 }
 
+// lane_get_t through global-address-space pointers: the loads are global_load,
+// not flat_load (no LDS aperture check, vmcnt only)
+typedef const __attribute__((address_space(1))) uint32_t qa_gu32;
+typedef const __attribute__((address_space(1))) int32_t qa_gi32;
+typedef const __attribute__((address_space(1))) uint64_t qa_gu64;
+template <int LK>
+__device__ __forceinline__ uint64_t lane_get_g(const void *p, uint32_t lk, uint64_t i) {
+    if (LK == QA_LK_U64) return ((const qa_gu64 *)p)[i];
+    if (LK == QA_LK_U32) return ((const qa_gu32 *)p)[i];
+    if (LK == QA_LK_I32) return (uint64_t)(int64_t)((const qa_gi32 *)p)[i];
+    if (LK == QA_LK_F32) return qa_f32_to_lane(((const qa_gu32 *)p)[i]);
+    return qa_lane_get(p, lk, i);
+}
+
 // bitmap word j of a column (bits >= n cleared); byte loads: the caller's
 // bitmap is ceil(n/8) bytes with no padding or alignment promise
 __device__ __forceinline__ uint64_t bm_word(const uint8_t *bm, uint64_t n, uint64_t j) {
@@ -112,7 +126,8 @@ __device__ __forceinline__ void wave_store_bytes(uint8_t *dst, const uint32_t *l
     if (head > nbytes) head = nbytes;
     if ((uint32_t)lane < head) dst[lane] = sb[lane];
     uint32_t rem = nbytes - head, nw = rem >> 2;
-    uint32_t *dw = (uint32_t *)(dst + head);
+    __attribute__((address_space(1))) uint32_t *dw =
+        (__attribute__((address_space(1))) uint32_t *)(dst + head);
     // four words per lane per step: the LDS reads of a step overlap
     for (uint32_t i0 = 0; i0 < nw; i0 += 4 * WAVE) {
         uint32_t o[4];

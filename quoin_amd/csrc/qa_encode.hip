@@ -1112,59 +1112,123 @@ __constant__ double c_emit_fm_scales[7] = {10.0, 100.0, 1000.0, 10000.0, 100000.
                                            10000000.0};
 
 // FoR / DeltaBitpack / FloatMult tiles with w <= 32 (the 32-lane FastLanes form,
-// src/bitpack.rs:47-75): lane l of the 32 packs rows [16 h, 16 h + 16) (h = half
-// of the wavefront), each residual ORed into its one or two LDS words, then
-// the 128 w bytes go out. 4 KiB of LDS per wavefront, so more tiles are in
-// flight per CU than in the general kernel. Instantiated per lane storage kind
-// so the loads are straight-line code.
+// src/bitpack.rs:47-75): lane l of half h of the wavefront takes rows
+// [16 h, 16 h + 16) of FastLanes column l, i.e. tile values (16 h + i) 32 + l.
+//
+// Residual of tile value k: FoR x - min; Delta zigzag(x - prev) - min with the
+// block's first value as its own predecessor (delta_bitpack.rs:23-46);
+// FloatMult round(x * scale) - min (float_mult.rs:66-98); zero past cnt
+// (padding to 1024, for_bitpack.rs:60-72). Loads past cnt read the tile's last
+// value instead (clamped index, no branch) and are masked afterwards.
 template <int LK>
-__device__ __forceinline__ void emit_fl_t(const TileEmit &e, uint32_t *outw, int lane) {
+__device__ __forceinline__ void fl_residuals(const TileEmit &e, int lane, uint32_t (&res)[16]) {
     const void *lp = e.lp;
-    const uint32_t lk = e.lk, cnt = e.cnt;
-    const uint64_t k0 = e.k0;
+    const uint32_t lk = e.lk, cnt = e.cnt, l = lane & 31, r0 = 16 * (lane >> 5);
+    const uint64_t k0 = e.k0, mn = e.mn;
     const bool delta = e.mode == QA_M_DELTA, fm = e.mode == QA_M_FLOAT_MULT;
-    const double sc = fm ? c_emit_fm_scales[e.scale] : 1.0;
-    const uint64_t mn = e.mn;
-    const uint32_t w = e.w;
-    uint8_t *q = e.dst;
-    if (lane == 0) q[0] = (uint8_t)w;
-    if (lane >= 1 && lane < 9) q[lane] = (uint8_t)(mn >> (8 * (lane - 1)));
-    if (w == 0) return;
-    for (uint32_t i = lane; i < 32 * w; i += WAVE) outw[i] = 0;
-    wave_lds_sync();
-    // residual of tile value k: FoR x - min; Delta zigzag(x - prev) - min with
-    // the block's first value as its own predecessor (delta_bitpack.rs:23-46);
-    // zero past cnt (padding to 1024, for_bitpack.rs:60-72)
-    const uint32_t l = lane & 31, r0 = 16 * (lane >> 5);
-    uint32_t res[16];
-    {
-        uint64_t xs[16], ps[16];
+    uint64_t xs[16], ps[16];
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint32_t k = (r0 + i) * 32 + l;
-            xs[i] = k < cnt ? lane_get_t<LK>(lp, lk, k0 + k) : 0;
-            ps[i] = (delta && k < cnt && (!e.first || k != 0)) ? lane_get_t<LK>(lp, lk, k0 + k - 1)
-                                                               : xs[i];
-        }
+    for (int i = 0; i < 16; i++) {
+        const uint32_t k = (r0 + i) * 32 + l, kk = k < cnt ? k : cnt - 1;
+        xs[i] = lane_get_g<LK>(lp, lk, k0 + kk);
+        if (delta) ps[i] = lane_get_g<LK>(lp, lk, k0 + kk - ((e.first && kk == 0) ? 0 : 1));
+    }
+    if (fm) {
+        const double sc = c_emit_fm_scales[e.scale];
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint32_t k = (r0 + i) * 32 + l;
-            uint64_t v = xs[i];
-            if (fm)  // k = round(x * s), half away from zero, no contraction
-                v = (uint64_t)(int64_t)round(__dmul_rn(__longlong_as_double((long long)v), sc));
-            res[i] = k >= cnt ? 0u : (uint32_t)(delta ? qa_zigzag(xs[i] - ps[i]) - mn : v - mn);
-        }
+        for (int i = 0; i < 16; i++)  // k = round(x * s), half away from zero, no contraction
+            xs[i] = (uint64_t)(int64_t)round(__dmul_rn(__longlong_as_double((long long)xs[i]), sc));
     }
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const uint32_t bit = (r0 + i) * w, word = bit >> 5, sh = bit & 31;
-        atomicOr(&outw[word * 32 + l], res[i] << sh);
-        if (sh + w > 32) atomicOr(&outw[(word + 1) * 32 + l], res[i] >> (32 - sh));
+        const uint32_t k = (r0 + i) * 32 + l;
+        const uint32_t r = (uint32_t)(delta ? qa_zigzag(xs[i] - ps[i]) - mn : xs[i] - mn);
+        res[i] = k < cnt ? r : 0u;
     }
-    wave_lds_sync();
-    wave_store_bytes(q + 9, outw, 128 * w, lane);
 }
 
+// Pack at width W (a compile-time constant, so every shift and word index is
+// fixed): word (k, l) of the form holds bits [32 k, 32 k + 32) of column l's
+// rows laid end to end at W bits each. Half h holds bits [16 h W, 16 (h + 1) W)
+// of each column, its words L[j] are computed in registers and written to LDS
+// once, with no atomics. For odd W half 1's bits start at bit 16 of word
+// (W - 1) / 2, the one word both halves share: half 1's part of it crosses to
+// half 0 through a lane swap.
+template <int W, int I, int NJ>
+__device__ __forceinline__ void fl_pack_rows(const uint32_t (&res)[16], uint32_t (&L)[NJ]) {
+    if constexpr (I < 16) {
+        constexpr int bit = I * W, j = bit >> 5, sh = bit & 31;
+        if constexpr (sh == 0) L[j] = res[I];  // first row in this word
+        else L[j] |= res[I] << sh;
+        if constexpr (sh + W > 32) L[j + 1] = res[I] >> (32 - sh);
+        fl_pack_rows<W, I + 1, NJ>(res, L);
+    }
+}
+
+template <int W, int J, int NJ>
+__device__ __forceinline__ void fl_store_even(const uint32_t (&L)[NJ], uint32_t *outw, uint32_t base) {
+    if constexpr (J < NJ) {
+        outw[base + J * 32] = L[J];
+        fl_store_even<W, J + 1, NJ>(L, outw, base);
+    }
+}
+
+template <int W, int J, int NJ>
+__device__ __forceinline__ void fl_shift_odd(const uint32_t (&L)[NJ], uint32_t (&M)[NJ], bool h) {
+    if constexpr (J < NJ) {
+        uint32_t prev = 0;
+        if constexpr (J > 0) prev = L[J - 1];
+        M[J] = h ? __builtin_amdgcn_alignbit(L[J], prev, 16) : L[J];
+        fl_shift_odd<W, J + 1, NJ>(L, M, h);
+    }
+}
+
+template <int W, int T, int NJ>
+__device__ __forceinline__ void fl_store_odd(const uint32_t (&M)[NJ], uint32_t *outw, uint32_t base,
+                                             bool h) {
+    if constexpr (T + 1 < NJ) {
+        outw[base + T * 32] = h ? M[T + 1] : M[T];
+        fl_store_odd<W, T + 1, NJ>(M, outw, base, h);
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void fl_pack_w(const uint32_t (&res)[16], uint32_t *outw, int lane) {
+    constexpr int NJ = (16 * W + 31) / 32;
+    const uint32_t l = lane & 31;
+    const bool h = lane >= 32;
+    uint32_t L[NJ];
+    fl_pack_rows<W, 0, NJ>(res, L);
+    if constexpr (W % 2 == 0) {
+        fl_store_even<W, 0, NJ>(L, outw, (h ? W / 2 : 0) * 32 + l);
+    } else {
+        // half 1, shifted up 16 bits: M[j] = bits of words (W - 1) / 2 + j
+        uint32_t M[NJ];
+        fl_shift_odd<W, 0, NJ>(L, M, h);
+        const uint32_t x = (uint32_t)__shfl_xor((int)(h ? M[0] : M[NJ - 1]), 32);
+        // half 0 writes words 0 .. NJ - 2, half 1 words (W + 1) / 2 .. W - 1
+        fl_store_odd<W, 0, NJ>(M, outw, (h ? (W + 1) / 2 : 0) * 32 + l, h);
+        if (!h) outw[(NJ - 1) * 32 + l] = M[NJ - 1] | x;  // the shared word
+    }
+}
+
+__device__ __forceinline__ void fl_pack(uint32_t w, const uint32_t (&res)[16], uint32_t *outw, int lane) {
+    switch (w) {
+#define QA_FL_PACK(n) \
+    case n: fl_pack_w<n>(res, outw, lane); break;
+    QA_FL_PACK(1) QA_FL_PACK(2) QA_FL_PACK(3) QA_FL_PACK(4) QA_FL_PACK(5) QA_FL_PACK(6)
+    QA_FL_PACK(7) QA_FL_PACK(8) QA_FL_PACK(9) QA_FL_PACK(10) QA_FL_PACK(11) QA_FL_PACK(12)
+    QA_FL_PACK(13) QA_FL_PACK(14) QA_FL_PACK(15) QA_FL_PACK(16) QA_FL_PACK(17) QA_FL_PACK(18)
+    QA_FL_PACK(19) QA_FL_PACK(20) QA_FL_PACK(21) QA_FL_PACK(22) QA_FL_PACK(23) QA_FL_PACK(24)
+    QA_FL_PACK(25) QA_FL_PACK(26) QA_FL_PACK(27) QA_FL_PACK(28) QA_FL_PACK(29) QA_FL_PACK(30)
+    QA_FL_PACK(31) QA_FL_PACK(32)
+#undef QA_FL_PACK
+    default: break;
+    }
+}
+
+// One wavefront per tile, WPB wavefronts per workgroup, 4 KiB of LDS per
+// wavefront for the packed words; then the sub-block [w][min][128 w bytes].
 template <int WPB>
 __global__ void __launch_bounds__(64 * WPB) k_emit_fl(EncArgs a) {
     __shared__ __align__(16) uint32_t outs[WPB][QA_TILE];
@@ -1173,13 +1237,23 @@ __global__ void __launch_bounds__(64 * WPB) k_emit_fl(EncArgs a) {
     if (t >= a.t1) return;
     const TileEmit e = a.temit[t];
     if (!e.fast) return;
+    const uint32_t w = e.w;
+    uint8_t *q = e.dst;
+    if (lane == 0) q[0] = (uint8_t)w;
+    if (lane >= 1 && lane < 9) q[lane] = (uint8_t)(e.mn >> (8 * (lane - 1)));
+    if (w == 0 || e.cnt == 0) return;  // cnt >= 1 on every active tile; the clamp below needs it
+    uint32_t res[16];
     switch (e.lk) {
-    case QA_LK_U64: emit_fl_t<QA_LK_U64>(e, outs[wv], lane); break;
-    case QA_LK_U32: emit_fl_t<QA_LK_U32>(e, outs[wv], lane); break;
-    case QA_LK_I32: emit_fl_t<QA_LK_I32>(e, outs[wv], lane); break;
-    case QA_LK_F32: emit_fl_t<QA_LK_F32>(e, outs[wv], lane); break;
-    default: emit_fl_t<QA_LK_ANY>(e, outs[wv], lane);
+    case QA_LK_U64: fl_residuals<QA_LK_U64>(e, lane, res); break;
+    case QA_LK_U32: fl_residuals<QA_LK_U32>(e, lane, res); break;
+    case QA_LK_I32: fl_residuals<QA_LK_I32>(e, lane, res); break;
+    case QA_LK_F32: fl_residuals<QA_LK_F32>(e, lane, res); break;
+    default: fl_residuals<QA_LK_ANY>(e, lane, res);
     }
+    uint32_t *outw = outs[wv];
+    fl_pack(w, res, outw, lane);
+    wave_lds_sync();
+    wave_store_bytes(q + 9, outw, 128 * w, lane);
 }
 
 // The other tiles (RAW, and FoR / Delta with w > 32), from the list k_tile_desc
