@@ -55,6 +55,8 @@ __device__ __forceinline__ uint64_t lane_get_t(const void *p, uint32_t lk, uint6
 typedef const __attribute__((address_space(1))) uint32_t qa_gu32;
 typedef const __attribute__((address_space(1))) int32_t qa_gi32;
 typedef const __attribute__((address_space(1))) uint64_t qa_gu64;
+typedef uint32_t qa_u32x4 __attribute__((ext_vector_type(4)));
+typedef uint64_t qa_u64x2 __attribute__((ext_vector_type(2)));
 template <int LK>
 __device__ __forceinline__ uint64_t lane_get_g(const void *p, uint32_t lk, uint64_t i) {
     if (LK == QA_LK_U64) return ((const qa_gu64 *)p)[i];

@@ -201,16 +201,16 @@ struct ProbeAcc {
     uint64_t pmn, pmx, pdz, gmn, gmx, gdz;
 };
 
-// 16 consecutive lane values from index i0: vector loads when the address is
-// 16-byte aligned, else one load per value
+// 16 consecutive lane values from index i0 (global loads): vector loads when
+// the address is 16-byte aligned, else one load per value
 template <int LK>
 __device__ __forceinline__ void load16(const void *lp, uint32_t lk, uint64_t i0, uint64_t (&v)[16]) {
     if (LK == QA_LK_U32 || LK == QA_LK_I32 || LK == QA_LK_F32) {
-        const uint32_t *p = (const uint32_t *)lp + i0;
+        const qa_gu32 *p = (const qa_gu32 *)lp + i0;
         if (((uintptr_t)p & 15) == 0) {
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const uint4 w = ((const uint4 *)p)[j];
+                const qa_u32x4 w = ((const __attribute__((address_space(1))) qa_u32x4 *)p)[j];
                 const uint32_t e[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
@@ -222,11 +222,11 @@ __device__ __forceinline__ void load16(const void *lp, uint32_t lk, uint64_t i0,
             return;
         }
     } else if (LK == QA_LK_U64) {
-        const uint64_t *p = (const uint64_t *)lp + i0;
+        const qa_gu64 *p = (const qa_gu64 *)lp + i0;
         if (((uintptr_t)p & 15) == 0) {
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                const ulonglong2 w = ((const ulonglong2 *)p)[j];
+                const qa_u64x2 w = ((const __attribute__((address_space(1))) qa_u64x2 *)p)[j];
                 v[2 * j] = w.x;
                 v[2 * j + 1] = w.y;
             }
@@ -234,125 +234,160 @@ __device__ __forceinline__ void load16(const void *lp, uint32_t lk, uint64_t i0,
         }
     }
 #pragma unroll
-    for (int u = 0; u < 16; u++) v[u] = lane_get_t<LK>(lp, lk, i0 + u);
+    for (int u = 0; u < 16; u++) v[u] = lane_get_g<LK>(lp, lk, i0 + u);
 }
 
-// Each lane takes 16 consecutive values of the tile (lane L: values 16 L ..
-// 16 L + 15), so neighbours are registers of the same lane (the first one is
-// the lane below's last, or the value before the tile) and the probe's sample
-// positions are fixed register slots: every 8th value of the region is slot 0
-// or 8, every 16th slot 0, every 32nd slot 0 of even lanes, every 64th slot 0
-// of every fourth lane.
+#define QA_DPP_ROW_ROR1 0x121  // lane i of a row takes lane (i - 1) mod 16
+
+// reduction over a row of 16 lanes, the result in each of them
+template <typename Op>
+__device__ __forceinline__ uint64_t row_reduce64(uint64_t v, Op op) {
+    v = op(v, qa_dpp64<QA_DPP_QP_1032>(Op::id, v));
+    v = op(v, qa_dpp64<QA_DPP_QP_2301>(Op::id, v));
+    v = op(v, qa_dpp64<QA_DPP_ROW_HALF_MIRROR>(Op::id, v));
+    v = op(v, qa_dpp64<QA_DPP_ROW_MIRROR>(Op::id, v));
+    return v;
+}
+__device__ __forceinline__ uint32_t row_sum32(uint32_t v) {
+    v += qa_dpp32<QA_DPP_QP_1032>(0, v);
+    v += qa_dpp32<QA_DPP_QP_2301>(0, v);
+    v += qa_dpp32<QA_DPP_ROW_HALF_MIRROR>(0, v);
+    v += qa_dpp32<QA_DPP_ROW_MIRROR>(0, v);
+    return v;
+}
+
+// A wavefront takes four tiles at a time, one per row of 16 lanes. Lane L of
+// a row walks four chunks of its tile: chunk c holds tile values 256 c + 16 L
+// .. 256 c + 16 L + 15, so neighbours are registers of the same lane (the first
+// one is lane L - 1's last, for lane 0 lane 15's last of the previous chunk, or
+// the value before the tile) and the probe's sample positions are fixed
+// register slots: every 8th value of the region is slot 0 or 8, every 16th
+// slot 0, every 32nd slot 0 of even lanes, every 64th slot 0 of every fourth
+// lane. Per-tile totals are reductions over the row (four DPP steps), one set
+// per four tiles.
 template <int LK, bool SG, bool FASTL, bool REG>
 __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp, uint32_t lk,
                                                  uint64_t rs, uint64_t len, DTile *tl,
                                                  uint64_t *seen8, uint64_t *seen32,
                                                  ProbeAcc &pa) {
     const int lane = threadIdx.x & 63, wv = qa_wave_id(), nwv = blockDim.x >> 6;
+    const uint32_t L = lane & 15, row = lane >> 4;
     const uint32_t ntl = (uint32_t)((len + QA_TILE - 1) / QA_TILE);
     uint32_t eq_region = 0;
-    for (uint32_t t = wv; t < col.tpr; t += nwv) {
-        DTile &ts = tl[t];
-        if (t >= ntl) {
-            if (lane == 0) ts.active = 0;
-            continue;
-        }
-        const uint64_t k0 = rs + (uint64_t)t * QA_TILE;  // column index of the tile start
-        const uint32_t cnt = (uint32_t)min((uint64_t)QA_TILE, rs + len - k0);
-        const uint64_t before = k0 ? lane_get_t<LK>(lp, lk, k0 - 1) : 0;
-        const uint64_t vf = lane_get_t<LK>(lp, lk, k0);
-        const uint64_t vl = lane_get_t<LK>(lp, lk, k0 + cnt - 1);
-        const uint64_t step = cnt > 1 ? lane_get_t<LK>(lp, lk, k0 + 1) - vf : 0;
-        const uint32_t kb = 16u * (uint32_t)lane;  // this lane's first tile index
-        uint64_t v[16];
-        if (kb + 16 <= cnt) {
-            load16<LK>(lp, lk, k0 + kb, v);
-        } else {
-#pragma unroll
-            for (int u = 0; u < 16; u++) v[u] = kb + u < cnt ? lane_get_t<LK>(lp, lk, k0 + kb + u) : vf;
-        }
-        // the value before this lane's first: the lane below's last
-        const uint64_t pv0 = qa_dpp64<QA_DPP_WAVE_SHR1>(before, v[15]);
+    for (uint32_t t0 = 4 * wv; t0 < col.tpr; t0 += 4 * nwv) {
+        const uint32_t t = t0 + row;
+        const bool act = t < ntl;  // a row past the region's values reads its first value
+        const uint64_t k0 = act ? rs + (uint64_t)t * QA_TILE : rs;  // column index of the tile start
+        const uint32_t cnt = act ? (uint32_t)min((uint64_t)QA_TILE, rs + len - k0) : 1u;
+        const uint64_t before = k0 ? lane_get_g<LK>(lp, lk, k0 - 1) : 0;
+        const uint64_t vf = lane_get_g<LK>(lp, lk, k0);
         uint64_t umin = ~0ull, umax = 0, dzmin = ~0ull, dzmax = 0, xb = 0;
         uint64_t smin = 0x7FFFFFFFFFFFFFFFull, smax = 0x8000000000000000ull;
-        uint32_t eqin = 0, eqx = 0;
+        uint32_t eqs = 0;  // equal neighbours inside the tile, + (1 << 16) across tiles
+        uint64_t carry = before;  // lane 15: the last value of the previous chunk
+#pragma unroll 1
+        for (uint32_t c = 0; c < 4; c++) {
+            const uint32_t kb = 256 * c + 16 * L;  // this lane's first tile index
+            uint64_t v[16];
+            if (kb + 16 <= cnt) {
+                load16<LK>(lp, lk, k0 + kb, v);
+            } else {
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint32_t k = kb + i;
-            const bool in = k < cnt;
-            const uint64_t x = v[i];  // past cnt: vf, neutral for min and max
-            const uint64_t p = i ? v[i - 1] : pv0;
-            umin = x < umin ? x : umin;
-            umax = x > umax ? x : umax;
-            if (SG) {
-                smin = (int64_t)x < (int64_t)smin ? x : smin;
-                smax = (int64_t)x > (int64_t)smax ? x : smax;
+                for (int u = 0; u < 16; u++)  // past cnt: vf, neutral for min and max
+                    v[u] = kb + u < cnt ? lane_get_g<LK>(lp, lk, k0 + kb + u) : vf;
             }
-            const uint64_t z = qa_zigzag(x - p);
-            const bool pair = in && k > 0;  // neighbours inside the tile
-            if (pair) {
-                dzmin = z < dzmin ? z : dzmin;
-                dzmax = z > dzmax ? z : dzmax;
-                if (FASTL) xb += qa_varint_len(x ^ p);
-                eqin += x == p;
-            }
-            if (i == 0 && k == 0) eqx = k0 > rs && x == p;  // across tiles of the region
-            if (REG) {  // every tile of a full region is full
-                const bool first = t == 0 && k == 0;  // region position 0
-                if (i == 0 || i == 8) {
-                    const uint64_t h = x * 0x9E3779B97F4A7C15ull;
-                    const uint32_t bucket = (uint32_t)((h >> 50) & 0x3FFF);
-                    atomicOr((unsigned long long *)&seen8[bucket >> 6], 1ull << (bucket & 63));
-                    if (i == 0 && (lane & 1) == 0)
-                        atomicOr((unsigned long long *)&seen32[bucket >> 6], 1ull << (bucket & 63));
+            const uint64_t pv0 = qa_dpp64<QA_DPP_ROW_ROR1>(0, L == 15 ? carry : v[15]);
+            carry = v[15];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t k = kb + i;
+                const bool in = k < cnt;
+                const uint64_t x = v[i];
+                const uint64_t p = i ? v[i - 1] : pv0;
+                umin = x < umin ? x : umin;
+                umax = x > umax ? x : umax;
+                if (SG) {
+                    smin = (int64_t)x < (int64_t)smin ? x : smin;
+                    smax = (int64_t)x > (int64_t)smax ? x : smax;
                 }
-                if (i == 0 && !first) {
-                    pa.pmn = x < pa.pmn ? x : pa.pmn;
-                    pa.pmx = x > pa.pmx ? x : pa.pmx;
-                    pa.pdz = z > pa.pdz ? z : pa.pdz;
-                    if ((lane & 3) == 0) {
-                        pa.gmn = x < pa.gmn ? x : pa.gmn;
-                        pa.gmx = x > pa.gmx ? x : pa.gmx;
-                        pa.gdz = z > pa.gdz ? z : pa.gdz;
+                const uint64_t z = qa_zigzag(x - p);
+                const bool pair = in && k > 0;  // neighbours inside the tile
+                if (pair) {
+                    dzmin = z < dzmin ? z : dzmin;
+                    dzmax = z > dzmax ? z : dzmax;
+                    if (FASTL) xb += qa_varint_len(x ^ p);
+                    eqs += x == p;
+                }
+                if (i == 0 && k == 0 && k0 > rs && x == p) eqs += 1u << 16;  // across tiles of the region
+                if (REG && act) {  // every tile of a full region is full
+                    const bool first = t == 0 && k == 0;  // region position 0
+                    if (i == 0 || i == 8) {
+                        const uint64_t h = x * 0x9E3779B97F4A7C15ull;
+                        const uint32_t bucket = (uint32_t)((h >> 50) & 0x3FFF);
+                        atomicOr((unsigned long long *)&seen8[bucket >> 6], 1ull << (bucket & 63));
+                        if (i == 0 && (L & 1) == 0)
+                            atomicOr((unsigned long long *)&seen32[bucket >> 6], 1ull << (bucket & 63));
+                    }
+                    if (i == 0 && !first) {
+                        pa.pmn = x < pa.pmn ? x : pa.pmn;
+                        pa.pmx = x > pa.pmx ? x : pa.pmx;
+                        pa.pdz = z > pa.pdz ? z : pa.pdz;
+                        if ((L & 3) == 0) {
+                            pa.gmn = x < pa.gmn ? x : pa.gmn;
+                            pa.gmx = x > pa.gmx ? x : pa.gmx;
+                            pa.gdz = z > pa.gdz ? z : pa.gdz;
+                        }
                     }
                 }
             }
         }
-        umin = wave_reduce(umin, OpMinU());
-        umax = wave_reduce(umax, OpMaxU());
+        umin = row_reduce64(umin, OpMinU());
+        umax = row_reduce64(umax, OpMaxU());
         if (SG) {
-            smin = wave_reduce(smin, OpMinS());
-            smax = wave_reduce(smax, OpMaxS());
+            smin = row_reduce64(smin, OpMinS());
+            smax = row_reduce64(smax, OpMaxS());
         }
-        dzmin = wave_reduce(dzmin, OpMinU());
-        dzmax = wave_reduce(dzmax, OpMaxU());
-        if (FASTL) xb = wave_reduce(xb, OpAdd());
-        const uint32_t eqt = (uint32_t)wave_reduce((uint64_t)eqin, OpAdd());
-        const uint32_t eqp = eqt + (uint32_t)__builtin_amdgcn_readfirstlane((int)eqx);
+        dzmin = row_reduce64(dzmin, OpMinU());
+        dzmax = row_reduce64(dzmax, OpMaxU());
+        if (FASTL) xb = row_reduce64(xb, OpAdd());
+        eqs = row_sum32(eqs);
+        const uint32_t eqt = eqs & 0xFFFF;  // at most 1023
         const uint32_t chg = cnt ? cnt - 1 - eqt : 0;  // changes between neighbours in the tile
         // all equal <=> min == max; arithmetic <=> every zig-zag delta equal
         const bool same = umin == umax, arith = cnt <= 1 || dzmin == dzmax;
-        if (lane == 0) {
-            ts.umin = umin;
-            ts.umax = umax;
-            ts.smin = (int64_t)smin;
-            ts.smax = (int64_t)smax;
-            ts.dzmin = dzmin;
-            ts.dzmax = dzmax;
-            ts.xorz_rest = xb;
-            ts.chg_rest = chg;
-            ts.exp_min = 0;
-            ts.exp_max = 0;
-            ts.v_first = vf;
-            ts.v_last = vl;
-            ts.step = step;
-            ts.dz_first = k0 ? qa_zigzag(vf - before) : 0;
-            ts.cnt = cnt;
-            ts.arith = (uint8_t)arith;
-            ts.allsame = (uint8_t)same;
-            ts.active = 1;
+        if (L == 0 && t < col.tpr) {
+            DTile &ts = tl[t];
+            const uint64_t vl = lane_get_g<LK>(lp, lk, k0 + cnt - 1);
+            const uint64_t step = cnt > 1 ? lane_get_g<LK>(lp, lk, k0 + 1) - vf : 0;
+            if (!act) {
+                ts.active = 0;
+            } else {
+                ts.umin = umin;
+                ts.umax = umax;
+                ts.smin = (int64_t)smin;
+                ts.smax = (int64_t)smax;
+                ts.dzmin = dzmin;
+                ts.dzmax = dzmax;
+                ts.xorz_rest = xb;
+                ts.chg_rest = chg;
+                ts.exp_min = 0;
+                ts.exp_max = 0;
+                ts.v_first = vf;
+                ts.v_last = vl;
+                ts.step = step;
+                ts.dz_first = k0 ? qa_zigzag(vf - before) : 0;
+                ts.cnt = cnt;
+                ts.arith = (uint8_t)arith;
+                ts.allsame = (uint8_t)same;
+                ts.active = 1;
+            }
         }
-        eq_region += eqp;
+        // the four rows' equal-neighbour counts (in-tile + across), active rows only
+        const uint32_t eqp = act ? eqt + (eqs >> 16) : 0u;
+        eq_region += (uint32_t)__builtin_amdgcn_readlane((int)eqp, 0) +
+                     (uint32_t)__builtin_amdgcn_readlane((int)eqp, 16) +
+                     (uint32_t)__builtin_amdgcn_readlane((int)eqp, 32) +
+                     (uint32_t)__builtin_amdgcn_readlane((int)eqp, 48);
     }
     return eq_region;
 }
@@ -1299,8 +1334,7 @@ static inline unsigned nb(uint64_t n, unsigned b) { return (unsigned)((n + b - 1
 void qa_encode_plan_launch(const EncArgs &a, hipStream_t s) {
     if (a.nvwords) {
         QA_LAUNCH(k_vcount, dim3(nb(a.nvwords, 256)), dim3(256), 0, s, a);
-        qa_exclusive_scan(a.vpre, a.nvwords, a.vpre, a.scan_tmp, s);
-        qa_exclusive_scan(a.tpre, a.nvwords, a.tpre, a.scan_tmp, s);
+        qa_exclusive_scan2(a.vpre, a.vpre, a.tpre, a.tpre, a.nvwords, a.scan_tmp, s);
     }
     QA_LAUNCH(k_col_counts, dim3(nb(a.ncols, 64)), dim3(64), 0, s, a);
     if (a.nvwords) QA_LAUNCH(k_compact, dim3(nb(a.nvwords, 4 * WAVE)), dim3(256), 0, s, a);

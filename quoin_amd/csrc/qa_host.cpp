@@ -52,7 +52,7 @@ void zero_async(void *p, size_t bytes, hipStream_t s);
 bool ensure_scan(DevBuf &b, size_t bytes, hipStream_t s) {
     const bool grow = b.cap < bytes || !b.p;
     if (!ensure(b, bytes)) return false;
-    if (grow) zero_async(b.p, 8, s);
+    if (grow) zero_async(b.p, 16, s);  // the two scan tickets
     return true;
 }
 
@@ -411,7 +411,7 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         if (d.validity) ntrans += q.n;
     }
     uint64_t nruns_max = ntrans + ncols;
-    size_t scan_words = qa_scan_tmp_words(std::max<uint64_t>(vword_base, nruns_max) + 1);
+    size_t scan_words = 2 * qa_scan_tmp_words(std::max<uint64_t>(vword_base, nruns_max) + 1);
     uint64_t need = lane_off * 8 + (vword_base + 1) * 16 + ntrans * 8 + (nruns_max + 1) * 8 +
                     (uint64_t)region_base * (sizeof(DBlock) + sizeof(DRegionG) + 2048 + 8) +
                     tile_base * sizeof(DTile);

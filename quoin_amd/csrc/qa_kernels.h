@@ -9,6 +9,8 @@ void qa_exclusive_scan(const uint64_t *in, uint64_t n, uint64_t *out, uint64_t *
                        hipStream_t s);
 void qa_exclusive_scan_dn(const uint64_t *in, uint64_t nmax, const uint64_t *ndev, uint64_t *out,
                           uint64_t *tmp, hipStream_t s);
+void qa_exclusive_scan2(const uint64_t *in0, uint64_t *out0, const uint64_t *in1, uint64_t *out1,
+                        uint64_t n, uint64_t *tmp, hipStream_t s);
 
 // Everything a wavefront needs to write one tile's payload (RAW, FoR or
 // DeltaBitpack), resolved by k_tile_desc so the emitting wavefront starts

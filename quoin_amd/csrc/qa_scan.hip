@@ -4,10 +4,11 @@
 // no waiting) scanning the chunk sums; then per-chunk scans seeded with the
 // chunk's carry. Each thread holds 16 consecutive elements (four 32-byte
 // loads), so both passes stream at full width. out has n + 1 entries: out[n]
-// is the total. tmp[0] is the ticket, tmp[1 + b] the chunk sums (then their
-// exclusive prefix) and tmp[1 + nb] the total; every scan leaves the ticket at
-// zero (the apply kernel resets it) and the host zeroes it when the workspace
-// is allocated.
+// is the total. tmp[y] is scan y's ticket (y = 0, or 0 and 1 for a pair),
+// tmp[2 + y (nb + 1) + b] its chunk sums (then their exclusive prefix) and
+// tmp[2 + y (nb + 1) + nb] its total; every scan leaves its ticket at zero
+// (the apply kernel resets it) and the host zeroes both tickets when the
+// workspace is allocated.
 #include "qa_kernels.h"
 
 #define SCAN_WG 256
@@ -56,9 +57,17 @@ __device__ __forceinline__ void scan_load16(const uint64_t *in, uint64_t n, uint
     }
 }
 
-__global__ void __launch_bounds__(SCAN_WG) k_scan_reduce(const uint64_t *in, uint64_t nmax,
+// up to two independent scans of the same length per launch pair: blockIdx.y
+// picks the array (its own input, output and workspace)
+struct ScanJobs {
+    const uint64_t *in[2];
+    uint64_t *out[2];
+};
+
+__global__ void __launch_bounds__(SCAN_WG) k_scan_reduce(ScanJobs j, uint64_t nmax,
                                                          const uint64_t *ndev, uint64_t *tmp) {
     __shared__ bool last;
+    const uint64_t *in = j.in[blockIdx.y];
     const uint64_t n = scan_n(nmax, ndev);
     // chunks holding data (at least one): the others take no ticket, so a grid
     // sized for nmax costs nothing past n
@@ -73,8 +82,8 @@ __global__ void __launch_bounds__(SCAN_WG) k_scan_reduce(const uint64_t *in, uin
     uint64_t tot;
     block_excl_scan(s, &tot);
     const uint32_t nb = gridDim.x;
-    unsigned long long *ticket = (unsigned long long *)tmp;
-    uint64_t *cs = tmp + 1;
+    unsigned long long *ticket = (unsigned long long *)tmp + blockIdx.y;
+    uint64_t *cs = tmp + 2 + (uint64_t)blockIdx.y * (nb + 1);
     if (threadIdx.x == 0) {
         cs[blockIdx.x] = tot;
         __threadfence();
@@ -97,15 +106,16 @@ __global__ void __launch_bounds__(SCAN_WG) k_scan_reduce(const uint64_t *in, uin
     if (threadIdx.x == 0) cs[nb] = carry;
 }
 
-__global__ void __launch_bounds__(SCAN_WG) k_scan_apply(const uint64_t *in, uint64_t nmax,
-                                                        const uint64_t *ndev, uint64_t *tmp,
-                                                        uint64_t *out) {
+__global__ void __launch_bounds__(SCAN_WG) k_scan_apply(ScanJobs j, uint64_t nmax,
+                                                        const uint64_t *ndev, uint64_t *tmp) {
+    const uint64_t *in = j.in[blockIdx.y];
+    uint64_t *out = j.out[blockIdx.y];
     const uint64_t n = scan_n(nmax, ndev);
     const uint32_t nb = gridDim.x;
-    const uint64_t *cs = tmp + 1;
+    const uint64_t *cs = tmp + 2 + (uint64_t)blockIdx.y * (nb + 1);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         out[n] = cs[nb];
-        tmp[0] = 0;  // the ticket, for the next scan
+        tmp[blockIdx.y] = 0;  // the ticket, for the next scan
     }
     if (blockIdx.x * (uint64_t)SCAN_CHUNK >= n) return;
     const uint64_t i0 = (uint64_t)blockIdx.x * SCAN_CHUNK + (uint64_t)threadIdx.x * SCAN_PER;
@@ -138,13 +148,25 @@ __global__ void __launch_bounds__(SCAN_WG) k_scan_apply(const uint64_t *in, uint
 // given, n is read from device memory (n <= nmax) so a data-dependent length
 // needs no host round trip. tmp needs qa_scan_tmp_words(nmax) words, the
 // first of them zero before the first scan; in and out may alias.
-size_t qa_scan_tmp_words(uint64_t n) { return n / SCAN_CHUNK + 1 + 2; }
+size_t qa_scan_tmp_words(uint64_t n) { return n / SCAN_CHUNK + 1 + 3; }
 
 void qa_exclusive_scan_dn(const uint64_t *in, uint64_t nmax, const uint64_t *ndev, uint64_t *out,
                           uint64_t *tmp, hipStream_t s) {
     const unsigned grid = (unsigned)(nmax / SCAN_CHUNK + 1);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(grid), dim3(SCAN_WG), 0, s, in, nmax, ndev, tmp);
-    hipLaunchKernelGGL(k_scan_apply, dim3(grid), dim3(SCAN_WG), 0, s, in, nmax, ndev, tmp, out);
+    const ScanJobs j{{in, nullptr}, {out, nullptr}};
+    hipLaunchKernelGGL(k_scan_reduce, dim3(grid), dim3(SCAN_WG), 0, s, j, nmax, ndev, tmp);
+    hipLaunchKernelGGL(k_scan_apply, dim3(grid), dim3(SCAN_WG), 0, s, j, nmax, ndev, tmp);
+}
+
+// Two exclusive scans of n elements each in one launch pair (twice the
+// workgroups of one scan, half the launches). tmp needs twice
+// qa_scan_tmp_words(n) words, its first two (the tickets) zero.
+void qa_exclusive_scan2(const uint64_t *in0, uint64_t *out0, const uint64_t *in1, uint64_t *out1,
+                        uint64_t n, uint64_t *tmp, hipStream_t s) {
+    const unsigned grid = (unsigned)(n / SCAN_CHUNK + 1);
+    const ScanJobs j{{in0, in1}, {out0, out1}};
+    hipLaunchKernelGGL(k_scan_reduce, dim3(grid, 2), dim3(SCAN_WG), 0, s, j, n, nullptr, tmp);
+    hipLaunchKernelGGL(k_scan_apply, dim3(grid, 2), dim3(SCAN_WG), 0, s, j, n, nullptr, tmp);
 }
 
 void qa_exclusive_scan(const uint64_t *in, uint64_t n, uint64_t *out, uint64_t *tmp,
