@@ -93,45 +93,117 @@ __global__ void k_vb_colfin(DecArgs a) {
     s.nruns = lo;
 }
 
-__global__ void k_vb_words(DecArgs a) {
-    if (*a.err) return;
-    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= a.nvwords) return;
-    int ci = dcol_by_vword(a, j);
-    const DDecCol &c = a.cols[ci];
-    uint64_t w = j - c.vword_base, bit0 = w * 64, bit1 = bit0 + 64;
+// word w of an RLE validity blob's bitmap: the valid runs meeting bits
+// [64 w, 64 w + 64), starting from run k0, the last one starting at or before
+// bit 64 w (runs alternate starting from a null run, src/validity.rs:40-60);
+// rs[k] - base is the start of run k
+__device__ __forceinline__ uint64_t vb_word_from(const uint64_t *rs, uint64_t base, uint64_t nr,
+                                                 uint64_t k0, uint64_t w) {
+    const uint64_t bit0 = w * 64, bit1 = bit0 + 64;
     uint64_t out = 0;
-    if (vtag(c) == 0) {
-        const uint8_t *p = c.src + c.vblob_off + 1;
-        uint64_t nbytes = (c.n + 7) / 8;
-        for (int k = 0; k < 8; k++)
-            if (w * 8 + k < nbytes) out |= (uint64_t)p[w * 8 + k] << (8 * k);
-    } else {
-        uint64_t r0 = a.vbpre[c.vbyte_base], nr = a.dst[ci].nruns;
-        uint64_t base = a.vruns[r0];
-        // last run k with start P[k] <= bit0
-        uint64_t lo = 0, hi = nr ? nr - 1 : 0;
-        while (lo < hi) {
-            uint64_t mid = (lo + hi + 1) >> 1;
-            if (a.vruns[r0 + mid] - base <= bit0) lo = mid; else hi = mid - 1;
-        }
-        for (uint64_t k = lo; k < nr; k++) {
-            uint64_t st = a.vruns[r0 + k] - base;
-            if (st >= bit1) break;
-            if (k & 1) {  // runs alternate starting from a null run
-                uint64_t en = a.vruns[r0 + k + 1] - base;
-                uint64_t s0 = st > bit0 ? st : bit0, e0 = en < bit1 ? en : bit1;
-                if (e0 > s0) {
-                    uint64_t cntb = e0 - s0, sh = s0 - bit0;
-                    out |= (cntb == 64 ? ~0ull : ((1ull << cntb) - 1)) << sh;
-                }
+    for (uint64_t k = k0; k < nr; k++) {
+        const uint64_t st = rs[k] - base;
+        if (st >= bit1) break;
+        if (k & 1) {
+            const uint64_t en = rs[k + 1] - base;
+            const uint64_t s0 = st > bit0 ? st : bit0, e0 = en < bit1 ? en : bit1;
+            if (e0 > s0) {
+                const uint64_t cntb = e0 - s0, sh = s0 - bit0;
+                out |= (cntb == 64 ? ~0ull : ((1ull << cntb) - 1)) << sh;
             }
         }
     }
-    uint64_t bits = c.n - bit0;
+    return out;
+}
+
+// The bitmap words, the decoded validity_out bytes (trailing bits cleared,
+// src/lib.rs:503-506) and each word's popcount (into vpre, scanned next).
+// One wavefront per 64 consecutive words. A wavefront inside one RLE stream
+// finds the last run starting at or before its first bit by a 64-ary search;
+// each later run starting inside its 4096 bits marks the first word whose
+// bit 0 it reaches, and a prefix maximum over the 64 words gives every lane
+// the last run starting at or before its word, so long runs cost nothing
+// extra. Other wavefronts (raw bitmap blobs, a wavefront spanning two
+// streams) search per word.
+__global__ void __launch_bounds__(256) k_vb_words(DecArgs a) {
+    __shared__ uint32_t marks[4][WAVE];
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
+    const uint64_t j0 = ((uint64_t)blockIdx.x * 4 + wv) * WAVE;
+    if (j0 >= a.nvwords) return;
+    const uint64_t j = j0 + lane;
+    if (*a.err) {
+        if (j < a.nvwords) a.vpre[j] = 0;
+        return;
+    }
+    const int cw = dcol_by_vword(a, j0);
+    const DDecCol &cwc = a.cols[cw];
+    uint64_t out = 0;
+    int ci = cw;
+    if (j0 + WAVE <= cwc.vword_base + cwc.nvwords && vtag(cwc) == 1) {
+        const uint64_t r0 = a.vbpre[cwc.vbyte_base], nr = a.dst[cw].nruns;
+        const uint64_t *rs = a.vruns + r0;
+        const uint64_t base = rs[0];
+        const uint64_t B0 = (j0 - cwc.vword_base) * 64;
+        // lo = the last run with start <= B0 (run 0 starts at 0)
+        uint64_t lo = 0, hi = nr;
+        while (hi - lo > 1) {
+            const uint64_t span = hi - lo;
+            const uint64_t pi = lo + 1 + (span - 1) * (uint64_t)lane / WAVE;
+            const uint64_t le = __ballot(pi < hi && rs[pi] - base <= B0);
+            const uint32_t c = (uint32_t)__builtin_popcountll(le);
+            if (!c) {
+                hi = lo + 1;
+            } else {
+                const uint64_t plo = lo + 1 + (span - 1) * (uint64_t)(c - 1) / WAVE;
+                hi = c < WAVE ? lo + 1 + (span - 1) * (uint64_t)c / WAVE : hi;
+                lo = plo;
+            }
+        }
+        uint32_t *mk = marks[wv];
+        mk[lane] = 0;
+        wave_lds_sync();
+        const uint64_t Bend = B0 + 64 * WAVE;
+        for (uint64_t k0 = lo + 1; k0 < nr; k0 += WAVE) {
+            const uint64_t k = k0 + lane;
+            const uint64_t st = k < nr ? rs[k] - base : ~0ull;
+            const bool in = st < Bend;  // and st > B0: lo is the last start <= B0
+            if (in) {
+                const uint64_t wl = (st - B0 + 63) >> 6;  // first word with bit 0 >= st
+                if (wl < WAVE) atomicMax(&mk[wl], (uint32_t)(k - lo));
+            }
+            if (!(__ballot(in) >> 63)) break;
+        }
+        wave_lds_sync();
+        const uint32_t m = qa_wave_scan_max32(mk[lane]);
+        out = vb_word_from(rs, base, nr, lo + m, j - cwc.vword_base);
+    } else {
+        if (j >= a.nvwords) return;
+        ci = dcol_by_vword(a, j);
+        const DDecCol &c = a.cols[ci];
+        const uint64_t w = j - c.vword_base;
+        if (vtag(c) == 0) {
+            const uint8_t *p = c.src + c.vblob_off + 1;
+            uint64_t nbytes = (c.n + 7) / 8;
+            for (int k = 0; k < 8; k++)
+                if (w * 8 + k < nbytes) out |= (uint64_t)p[w * 8 + k] << (8 * k);
+        } else {
+            const uint64_t r0 = a.vbpre[c.vbyte_base], nr = a.dst[ci].nruns;
+            const uint64_t *rs = a.vruns + r0;
+            const uint64_t base = rs[0], bit0 = w * 64;
+            // last run k with start P[k] <= bit0
+            uint64_t lo = 0, hi = nr ? nr - 1 : 0;
+            while (lo < hi) {
+                uint64_t mid = (lo + hi + 1) >> 1;
+                if (rs[mid] - base <= bit0) lo = mid; else hi = mid - 1;
+            }
+            out = vb_word_from(rs, base, nr, lo, w);
+        }
+    }
+    const DDecCol &c = a.cols[ci];
+    const uint64_t w = j - c.vword_base, bits = c.n - w * 64;
     if (bits < 64) out &= (1ull << bits) - 1;
     c.bitmap[w] = out;
-    // validity_out: the decoded bitmap, trailing bits cleared (src/lib.rs:503-506)
+    a.vpre[j] = (uint64_t)__popcll(out);
     if (c.validity_out) {
         uint64_t nbytes = (c.n + 7) / 8, b0 = w * 8;
         uint8_t *vo = c.validity_out + b0;
@@ -141,13 +213,6 @@ __global__ void k_vb_words(DecArgs a) {
             for (int k = 0; k < 8 && b0 + k < nbytes; k++) vo[k] = (uint8_t)(out >> (8 * k));
         }
     }
-}
-
-__global__ void k_dec_vcount(DecArgs a) {
-    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= a.nvwords) return;
-    const DDecCol &c = a.cols[dcol_by_vword(a, j)];
-    a.vpre[j] = *a.err ? 0 : (uint64_t)__popcll(c.bitmap[j - c.vword_base]);
 }
 
 // valid count and the frames the reference decoder would read for it
@@ -173,9 +238,11 @@ __global__ void k_dec_colfin(DecArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// stage 3: tile map (one wavefront per frame), sub-block index
+// stage 3: tile map and sub-block index (one wavefront per frame)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_dec_tilemap(DecArgs a) {
+// one wavefront per frame: every tile's map entry (its frame, nothing indexed
+// yet), then the sub-block index of the frame's FoR / Delta payload
+__global__ void __launch_bounds__(256) k_dec_index(DecArgs a) {
     const int lane = threadIdx.x & 63;
     uint32_t f = a.f0 + blockIdx.x * 4 + qa_wave_id();
     if (f >= a.f1) return;
@@ -189,13 +256,8 @@ __global__ void __launch_bounds__(256) k_dec_tilemap(DecArgs a) {
         d.w = 0;
         a.tiles[fr.tile_base + t] = d;
     }
-}
-
-__global__ void __launch_bounds__(256) k_dec_index(DecArgs a) {
-    const int lane = threadIdx.x & 63;
-    uint32_t f = a.f0 + blockIdx.x * 4 + qa_wave_id();
-    if (f >= a.f1 || *a.err || !frame_used(a, f)) return;
-    const DFrame &fr = a.frames[f];
+    __threadfence_block();  // the walk below rewrites entries other lanes wrote
+    if (*a.err || !frame_used(a, f)) return;
     const DDecCol &col = a.cols[fr.col];
     const uint8_t *s = col.src;
     uint64_t pos = fr.payload_off, end = fr.payload_off + fr.plen;
@@ -365,8 +427,7 @@ void qa_decode_prep_launch(const DecArgs &a, hipStream_t s) {
     }
     QA_LAUNCH(k_vb_colfin, dim3(nbk(a.ncols, 64)), dim3(64), 0, s, a);
     if (a.nvwords) {
-        QA_LAUNCH(k_vb_words, dim3(nbk(a.nvwords, 256)), dim3(256), 0, s, a);
-        QA_LAUNCH(k_dec_vcount, dim3(nbk(a.nvwords, 256)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_vb_words, dim3(nbk(a.nvwords, 4 * WAVE)), dim3(256), 0, s, a);
         qa_exclusive_scan(a.vpre, a.nvwords, a.vpre, a.scan_tmp, s);
     }
     QA_LAUNCH(k_dec_colfin, dim3(nbk(a.ncols, 64)), dim3(64), 0, s, a);
@@ -375,7 +436,6 @@ void qa_decode_prep_launch(const DecArgs &a, hipStream_t s) {
 // per frame [f0, f1): tile map, sub-block index, Fast-pool frame stages
 void qa_decode_index_launch(const DecArgs &a, hipStream_t s) {
     if (a.f1 <= a.f0) return;
-    QA_LAUNCH(k_dec_tilemap, dim3(nbk(a.f1 - a.f0, 4)), dim3(256), 0, s, a);
     QA_LAUNCH(k_dec_index, dim3(nbk(a.f1 - a.f0, 4)), dim3(256), 0, s, a);
     qa_fast_dec_index_launch(a, s);
 }

@@ -27,6 +27,15 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+// Pinned host staging for the per-call descriptor uploads and result
+// readbacks: copies from pinned memory are DMA'd asynchronously, where copies
+// from pageable memory are staged by the runtime on the calling thread. Every
+// call ends with a stream synchronization, so the next call may overwrite it.
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
 // grow-only device buffer
 bool ensure(DevBuf &b, size_t bytes) {
     if (bytes == 0) bytes = 8;
@@ -56,6 +65,23 @@ bool ensure_scan(DevBuf &b, size_t bytes, hipStream_t s) {
     return true;
 }
 
+bool ensure_host(HostBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    if (b.cap >= bytes) return true;
+    if (b.p) hipHostFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 4 + 4096;
+    if (hipHostMalloc(&b.p, want, hipHostMallocDefault) != hipSuccess) {
+        b.p = nullptr;
+        return false;
+    }
+    b.cap = want;
+    return true;
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
 std::atomic<uint64_t> g_wins[64];
 thread_local int t_last_err = 0;
 
@@ -67,12 +93,13 @@ struct QuoinGpu {
     uint64_t budget = kDefaultBudget;
     int form = 0;
     std::mutex mu;
+    HostBuf hstage;  // pinned: descriptor uploads, then result readbacks
+    DevBuf dmeta;    // the uploaded descriptors of a decompress call
     // encode arenas
     DevBuf cols, cst, lane, vpre, tpre, tpos, rsz, colbase, scan_tmp, rfeat, owner, blocks, tiles,
         wins, tilesf, blocksx, dsorted, rstart, hkeys, hcnt, rgrown, gsets, temit, tlist, dpart, dparts;
     // decode arenas
-    DevBuf dcols, dstate, frames, dtiles, dtdec, dvpre, derr, dbitmaps, dvbpre, dvruns, dscan,
-        dfx, dscr, dscr2, dkrun;
+    DevBuf dstate, dtiles, dtdec, dvpre, derr, dbitmaps, dvbpre, dvruns, dscan, dfx, dscr, dscr2;
     // host-API staging
     DevBuf st_in, st_valid, st_out, st_vout;
     // rANS arenas
@@ -330,13 +357,15 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
     if (!g) return;
     hipSetDevice(g->device);
     hipStreamSynchronize(g->stream);
+    if (g->hstage.p) hipHostFree(g->hstage.p);
+    g->hstage.p = nullptr;
     DevBuf *all[] = {&g->cols,  &g->cst,      &g->lane,   &g->vpre,    &g->tpre,    &g->tpos,
                      &g->rsz,   &g->colbase,  &g->scan_tmp, &g->rfeat, &g->owner,   &g->blocks,
-                     &g->tiles, &g->wins,     &g->dcols,  &g->frames,  &g->dtiles,  &g->dtdec,
+                     &g->tiles, &g->wins,     &g->dtiles,  &g->dtdec,
                      &g->dvpre, &g->derr,     &g->dbitmaps, &g->dscan, &g->st_in,   &g->st_valid,
                      &g->dstate, &g->dvbpre, &g->dvruns, &g->tilesf, &g->blocksx, &g->dsorted,
                      &g->rstart, &g->hkeys, &g->hcnt, &g->rgrown, &g->gsets, &g->temit, &g->tlist, &g->dpart, &g->dparts, &g->dfx,
-                     &g->dscr, &g->dscr2, &g->dkrun,
+                     &g->dscr, &g->dscr2, &g->dmeta,
                      &g->st_out, &g->st_vout, &g->rdesc, &g->rres, &g->rtabs, &g->rscr,
                      &g->ldesc, &g->lres, &g->ltile, &g->lscan, &g->pkeys, &g->pres,
                      &g->psort, &g->ptab, &g->dm};
@@ -475,8 +504,11 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         a.dpart = dp<uint32_t>(g->dpart);
         a.dparts = dp<uint64_t>(g->dparts);
     }
-    if (hipMemcpyAsync(g->cols.p, cols.data(), ncols * sizeof(DCol), hipMemcpyHostToDevice, s) !=
-        hipSuccess)
+    const size_t o_wins = align256(ncols * sizeof(DCol)), o_st = o_wins + 64 * 8;
+    if (!ensure_host(g->hstage, o_st + ncols * sizeof(DColState))) return QUOIN_ERR_DEVICE;
+    uint8_t *hs = (uint8_t *)g->hstage.p;
+    memcpy(hs, cols.data(), ncols * sizeof(DCol));
+    if (hipMemcpyAsync(g->cols.p, hs, ncols * sizeof(DCol), hipMemcpyHostToDevice, s) != hipSuccess)
         return QUOIN_ERR_DEVICE;
     hipEventRecord(g->ev0, s);
     zero_async(g->cst.p, ncols * sizeof(DColState), s);
@@ -521,10 +553,10 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         for (const EncArgs &b : blocks) qa_encode_emit_launch(b, s);
     }
     hipEventRecord(g->ev1, s);
-    std::vector<DColState> st(ncols);
-    uint64_t wins[64];
-    hipMemcpyAsync(st.data(), g->cst.p, ncols * sizeof(DColState), hipMemcpyDeviceToHost, s);
-    hipMemcpyAsync(wins, g->wins.p, sizeof wins, hipMemcpyDeviceToHost, s);
+    const DColState *st = (const DColState *)(hs + o_st);
+    const uint64_t *wins = (const uint64_t *)(hs + o_wins);
+    hipMemcpyAsync(hs + o_st, g->cst.p, ncols * sizeof(DColState), hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(hs + o_wins, g->wins.p, 64 * 8, hipMemcpyDeviceToHost, s);
     if (hipStreamSynchronize(s) != hipSuccess) return QUOIN_ERR_DEVICE;
     if (hipGetLastError() != hipSuccess) return QUOIN_ERR_DEVICE;
     float ms = 0;
@@ -613,8 +645,7 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         d.fend = (uint32_t)frames.size();
     }
     uint64_t nscan = std::max<uint64_t>(vword_base, vbyte_base) + 1;
-    if (!ensure(g->dcols, ns * sizeof(DDecCol)) || !ensure(g->dstate, ns * sizeof(DDecState)) ||
-        !ensure(g->frames, frames.size() * sizeof(DFrame)) ||
+    if (!ensure(g->dstate, ns * sizeof(DDecState)) ||
         !ensure(g->dtiles, tile_base * sizeof(DTileD)) ||
         !ensure(g->dtdec, tile_base * sizeof(TileDec)) ||
         !ensure(g->dvpre, (vword_base + 1) * 8) || !ensure(g->derr, 8) ||
@@ -626,15 +657,9 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         return QUOIN_ERR_DEVICE;
     for (size_t c = 0; c < ns; c++)
         if (cols[c].nvwords) cols[c].bitmap = dp<uint64_t>(g->dbitmaps) + cols[c].vword_base;
-    hipMemcpyAsync(g->dcols.p, cols.data(), ns * sizeof(DDecCol), hipMemcpyHostToDevice, s);
-    if (!frames.empty())
-        hipMemcpyAsync(g->frames.p, frames.data(), frames.size() * sizeof(DFrame),
-                       hipMemcpyHostToDevice, s);
     DecArgs a{};
-    a.cols = dp<DDecCol>(g->dcols);
     a.dst = dp<DDecState>(g->dstate);
     a.ncols = (uint32_t)ns;
-    a.frames = dp<DFrame>(g->frames);
     a.nframes = (uint32_t)frames.size();
     a.ntiles = tile_base;
     a.nvals = val_base;
@@ -691,10 +716,20 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
             a.kr_n[i] = (uint32_t)kr.size() - a.kr_off[i];
             a.kr_tot[i] = tot;
         }
-        if (!ensure(g->dkrun, kr.size() * sizeof(KindRun))) return QUOIN_ERR_DEVICE;
-        hipMemcpyAsync(g->dkrun.p, kr.data(), kr.size() * sizeof(KindRun), hipMemcpyHostToDevice, s);
-        a.krun_all = dp<KindRun>(g->dkrun);
     }
+    // the column, frame and kind-run tables in one upload from pinned memory
+    const size_t o_frames = align256(ns * sizeof(DDecCol));
+    const size_t o_kr = o_frames + align256(frames.size() * sizeof(DFrame));
+    const size_t o_err = o_kr + align256(kr.size() * sizeof(KindRun));
+    if (!ensure(g->dmeta, o_err) || !ensure_host(g->hstage, o_err + 8)) return QUOIN_ERR_DEVICE;
+    uint8_t *hs = (uint8_t *)g->hstage.p, *dm = (uint8_t *)g->dmeta.p;
+    memcpy(hs, cols.data(), ns * sizeof(DDecCol));
+    if (!frames.empty()) memcpy(hs + o_frames, frames.data(), frames.size() * sizeof(DFrame));
+    if (!kr.empty()) memcpy(hs + o_kr, kr.data(), kr.size() * sizeof(KindRun));
+    if (hipMemcpyAsync(dm, hs, o_err, hipMemcpyHostToDevice, s) != hipSuccess) return QUOIN_ERR_DEVICE;
+    a.cols = (DDecCol *)dm;
+    a.frames = (DFrame *)(dm + o_frames);
+    if (!kr.empty()) a.krun_all = (KindRun *)(dm + o_kr);
     hipEventRecord(g->ev0, s);
     zero_async(g->derr.p, 8, s);
     zero_async(g->dstate.p, ns * sizeof(DDecState), s);
@@ -722,9 +757,9 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         qa_decode_final_launch(a, s);
     }
     hipEventRecord(g->ev1, s);
-    uint32_t err = 0;
-    hipMemcpyAsync(&err, g->derr.p, 4, hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(hs + o_err, g->derr.p, 4, hipMemcpyDeviceToHost, s);
     if (hipStreamSynchronize(s) != hipSuccess) return QUOIN_ERR_DEVICE;
+    const uint32_t err = *(const uint32_t *)(hs + o_err);
     if (hipGetLastError() != hipSuccess) return QUOIN_ERR_DEVICE;
     float ms = 0;
     hipEventElapsedTime(&ms, g->ev0, g->ev1);
