@@ -4,8 +4,8 @@
 // (src/format.rs:65-94, src/decoder.rs:65-94); every data-sized step runs here:
 //   1 validity blob -> bitmap + validity_out (src/validity.rs:73-119): RLE
 //     varints are split in parallel (a byte ends a varint iff its top bit is
-//     clear), run lengths are scanned, each bitmap word ORs the valid runs that
-//     meet it
+//     clear); one paired scan over the bytes gives each run its index and its
+//     start; each bitmap word ORs the valid runs that meet it
 //   2 valid count and the frames that cover it (the reference scans frames
 //     until it has counted the valid values)
 //   3 tile map and sub-block index: each frame's sub-block headers (widths,
@@ -26,71 +26,51 @@ __device__ __forceinline__ uint8_t vtag(const DDecCol &c) {
     return c.vblob_len ? c.src[c.vblob_off] : 0xFF;
 }
 
+// per blob byte: 1 if it ends a varint (its top bit is clear), and that
+// varint's value (src/varint.rs:24-39), read back over the bytes before it;
+// both are scanned together next
 __global__ void k_vb_ends(DecArgs a) {
     uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.nvbytes) return;
     const DDecCol &c = a.cols[dcol_by_vbyte(a, g)];
-    uint64_t k = g - c.vbyte_base;
-    uint8_t b = c.src[c.vblob_off + 1 + k];
-    a.vbpre[g] = (vtag(c) == 1 && !(b & 0x80)) ? 1 : 0;
+    const uint8_t *p = c.src + c.vblob_off + 1;
+    const uint64_t k = g - c.vbyte_base;
+    const bool end = vtag(c) == 1 && !(p[k] & 0x80);
+    uint64_t v = 0;
+    if (end) {
+        uint64_t j = k;
+        while (j > 0 && (p[j - 1] & 0x80)) j--;
+        const uint64_t len = k - j + 1;
+        if (len > 10) {
+            v = c.n + 1;  // overflowing varint: an error if the decoder reaches it
+        } else {
+            for (uint64_t q = 0; q < len; q++) v |= (uint64_t)(p[j + q] & 0x7f) << (7 * q);
+            if (v > c.n) v = c.n + 1;  // keeps the prefix sums from wrapping
+        }
+    }
+    a.vbpre[g] = end;
+    a.vbval[g] = v;
 }
 
-// each varint-ending byte assembles its run length (src/varint.rs:24-39)
+// each varint-ending byte: its run's start (the prefix of the values before
+// it) at the run's index; the runs' total after the last one. The run that
+// reaches the n-th value (start < n <= end) sets how many runs the reference
+// loop reads (the smallest m with P[m] >= n) and must end exactly at n.
 __global__ void k_vb_runs(DecArgs a) {
     uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.nvbytes) return;
-    const DDecCol &c = a.cols[dcol_by_vbyte(a, g)];
-    if (vtag(c) != 1) return;
-    const uint8_t *p = c.src + c.vblob_off + 1;
-    uint64_t k = g - c.vbyte_base;
-    if (p[k] & 0x80) return;
-    uint64_t j = k;
-    while (j > 0 && (p[j - 1] & 0x80)) j--;
-    uint64_t len = k - j + 1, v = 0;
-    if (len > 10) {
-        v = c.n + 1;  // overflowing varint: an error if the decoder reaches it
-    } else {
-        for (uint64_t q = 0; q < len; q++) v |= (uint64_t)(p[j + q] & 0x7f) << (7 * q);
-        if (v > c.n) v = c.n + 1;  // keeps the prefix sums from wrapping
-    }
-    a.vruns[a.vbpre[g]] = v;
-}
-
-// per column: how many runs the reference loop reads, and blob validation
-__global__ void k_vb_colfin(DecArgs a) {
-    int ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci >= (int)a.ncols) return;
+    if (g == 0) a.vruns[a.vbpre[a.nvbytes]] = a.vbval[a.nvbytes];
+    const int ci = dcol_by_vbyte(a, g);
     const DDecCol &c = a.cols[ci];
-    DDecState &s = a.dst[ci];
-    if (!c.nvwords) return;
-    uint8_t tag = vtag(c);
-    if (c.vblob_len == 0 || (tag != 0 && tag != 1)) {
-        atomicOr(a.err, QA_ERR_CORRUPT);
-        return;
+    if (vtag(c) != 1 || (c.src[c.vblob_off + 1 + (g - c.vbyte_base)] & 0x80)) return;
+    const uint64_t k = a.vbpre[g];
+    a.vruns[k] = a.vbval[g];
+    const uint64_t base = a.vbval[c.vbyte_base];
+    const uint64_t st = a.vbval[g] - base, en = a.vbval[g + 1] - base;
+    if (st < c.n && en >= c.n) {
+        a.dst[ci].nruns = k - a.vbpre[c.vbyte_base] + 1;
+        if (en != c.n) atomicOr(a.err, QA_ERR_CORRUPT);  // a run overruns
     }
-    if (tag == 0) {
-        if (c.vblob_len - 1 < (c.n + 7) / 8) atomicOr(a.err, QA_ERR_CORRUPT);
-        return;
-    }
-    uint64_t r0 = a.vbpre[c.vbyte_base], r1 = a.vbpre[c.vbyte_base + c.vblob_len - 1];
-    uint64_t base = a.vruns[r0];  // prefix at the column's first run
-    if (c.n == 0) {
-        s.nruns = 0;
-        return;
-    }
-    // smallest m = K+1 in [1, r1-r0] with P[m] >= n, P[m] = vruns[r0 + m] - base
-    uint64_t hi = r1 - r0;
-    if (hi == 0 || a.vruns[r0 + hi] - base < c.n) {
-        atomicOr(a.err, QA_ERR_CORRUPT);  // the runs end before n values
-        return;
-    }
-    uint64_t lo = 1;
-    while (lo < hi) {
-        uint64_t mid = (lo + hi) >> 1;
-        if (a.vruns[r0 + mid] - base >= c.n) hi = mid; else lo = mid + 1;
-    }
-    if (a.vruns[r0 + lo] - base != c.n) atomicOr(a.err, QA_ERR_CORRUPT);  // a run overruns
-    s.nruns = lo;
 }
 
 // word w of an RLE validity blob's bitmap: the valid runs meeting bits
@@ -182,10 +162,12 @@ __global__ void __launch_bounds__(256) k_vb_words(DecArgs a) {
         const DDecCol &c = a.cols[ci];
         const uint64_t w = j - c.vword_base;
         if (vtag(c) == 0) {
+            // a blob shorter than ceil(n / 8) is flagged by k_dec_colfin; no
+            // read goes past it
             const uint8_t *p = c.src + c.vblob_off + 1;
-            uint64_t nbytes = (c.n + 7) / 8;
+            const uint64_t nbytes = (c.n + 7) / 8, have = c.vblob_len - 1;
             for (int k = 0; k < 8; k++)
-                if (w * 8 + k < nbytes) out |= (uint64_t)p[w * 8 + k] << (8 * k);
+                if (w * 8 + k < nbytes && w * 8 + k < have) out |= (uint64_t)p[w * 8 + k] << (8 * k);
         } else {
             const uint64_t r0 = a.vbpre[c.vbyte_base], nr = a.dst[ci].nruns;
             const uint64_t *rs = a.vruns + r0;
@@ -222,7 +204,16 @@ __global__ void k_dec_colfin(DecArgs a) {
     const DDecCol &c = a.cols[ci];
     DDecState &s = a.dst[ci];
     uint64_t nv = c.n;
-    if (c.nvwords) nv = a.vpre[c.vword_base + c.nvwords] - a.vpre[c.vword_base];
+    if (c.nvwords) {
+        // the blob: a tag of 0 (raw bitmap, at least ceil(n / 8) bytes) or 1
+        // (runs, which must reach n: k_vb_runs set nruns if they do)
+        const uint8_t tag = vtag(c);
+        bool bad = c.vblob_len == 0 || (tag != 0 && tag != 1);
+        if (!bad && tag == 0) bad = c.vblob_len - 1 < (c.n + 7) / 8;
+        if (!bad && tag == 1) bad = s.nruns == 0;  // n > 0 here: nvwords > 0
+        if (bad) atomicOr(a.err, QA_ERR_CORRUPT);
+        nv = a.vpre[c.vword_base + c.nvwords] - a.vpre[c.vword_base];
+    }
     s.nv = nv;
     // frames are read while fewer than nv values are counted: the first frame
     // starting at or past nv ends the read (out_off is non-decreasing)
@@ -421,11 +412,9 @@ static inline unsigned nbk(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 void qa_decode_prep_launch(const DecArgs &a, hipStream_t s) {
     if (a.nvbytes) {
         QA_LAUNCH(k_vb_ends, dim3(nbk(a.nvbytes, 256)), dim3(256), 0, s, a);
-        qa_exclusive_scan(a.vbpre, a.nvbytes, a.vbpre, a.scan_tmp, s);
+        qa_exclusive_scan2(a.vbpre, a.vbpre, a.vbval, a.vbval, a.nvbytes, a.scan_tmp, s);
         QA_LAUNCH(k_vb_runs, dim3(nbk(a.nvbytes, 256)), dim3(256), 0, s, a);
-        qa_exclusive_scan_dn(a.vruns, a.nvbytes, a.vbpre + a.nvbytes, a.vruns, a.scan_tmp, s);
     }
-    QA_LAUNCH(k_vb_colfin, dim3(nbk(a.ncols, 64)), dim3(64), 0, s, a);
     if (a.nvwords) {
         QA_LAUNCH(k_vb_words, dim3(nbk(a.nvwords, 4 * WAVE)), dim3(256), 0, s, a);
         qa_exclusive_scan(a.vpre, a.nvwords, a.vpre, a.scan_tmp, s);

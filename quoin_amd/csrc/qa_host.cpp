@@ -99,7 +99,8 @@ struct QuoinGpu {
     DevBuf cols, cst, lane, vpre, tpre, tpos, rsz, colbase, scan_tmp, rfeat, owner, blocks, tiles,
         wins, tilesf, blocksx, dsorted, rstart, hkeys, hcnt, rgrown, gsets, temit, tlist, dpart, dparts;
     // decode arenas
-    DevBuf dstate, dtiles, dtdec, dvpre, derr, dbitmaps, dvbpre, dvruns, dscan, dfx, dscr, dscr2;
+    DevBuf dstate, dtiles, dtdec, dvpre, derr, dbitmaps, dvbpre, dvbval, dvruns, dscan, dfx, dscr,
+        dscr2;
     // host-API staging
     DevBuf st_in, st_valid, st_out, st_vout;
     // rANS arenas
@@ -363,7 +364,7 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
                      &g->rsz,   &g->colbase,  &g->scan_tmp, &g->rfeat, &g->owner,   &g->blocks,
                      &g->tiles, &g->wins,     &g->dtiles,  &g->dtdec,
                      &g->dvpre, &g->derr,     &g->dbitmaps, &g->dscan, &g->st_in,   &g->st_valid,
-                     &g->dstate, &g->dvbpre, &g->dvruns, &g->tilesf, &g->blocksx, &g->dsorted,
+                     &g->dstate, &g->dvbpre, &g->dvbval, &g->dvruns, &g->tilesf, &g->blocksx, &g->dsorted,
                      &g->rstart, &g->hkeys, &g->hcnt, &g->rgrown, &g->gsets, &g->temit, &g->tlist, &g->dpart, &g->dparts, &g->dfx,
                      &g->dscr, &g->dscr2, &g->dmeta,
                      &g->st_out, &g->st_vout, &g->rdesc, &g->rres, &g->rtabs, &g->rscr,
@@ -650,8 +651,8 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
         !ensure(g->dtdec, tile_base * sizeof(TileDec)) ||
         !ensure(g->dvpre, (vword_base + 1) * 8) || !ensure(g->derr, 8) ||
         !ensure(g->dbitmaps, (vword_base + 1) * 8) || !ensure(g->dvbpre, (vbyte_base + 1) * 8) ||
-        !ensure(g->dvruns, (vbyte_base + 1) * 8) ||
-        !ensure_scan(g->dscan, qa_scan_tmp_words(nscan) * 8, s) ||
+        !ensure(g->dvbval, (vbyte_base + 1) * 8) || !ensure(g->dvruns, (vbyte_base + 1) * 8) ||
+        !ensure_scan(g->dscan, 2 * qa_scan_tmp_words(nscan) * 8, s) ||
         (any_fast && (!ensure(g->dfx, frames.size() * sizeof(DFrameX)) ||
                       !ensure(g->dscr, scr_base * 8) || !ensure(g->dscr2, scr_base * 8))))
         return QUOIN_ERR_DEVICE;
@@ -668,6 +669,7 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
     a.navbytes = avbyte_base;
     a.vpre = dp<uint64_t>(g->dvpre);
     a.vbpre = dp<uint64_t>(g->dvbpre);
+    a.vbval = dp<uint64_t>(g->dvbval);
     a.vruns = dp<uint64_t>(g->dvruns);
     a.tiles = dp<DTileD>(g->dtiles);
     a.tdec = dp<TileDec>(g->dtdec);

@@ -191,7 +191,8 @@ struct DecArgs {
     uint64_t navbytes;     // validity_out bytes of streams without a bitmap
     uint64_t *vpre;        // rank prefix (nvwords + 1)
     uint64_t *vbpre;       // varint-end prefix over blob bytes (nvbytes + 1)
-    uint64_t *vruns;       // run lengths, then their prefix (nvbytes + 1)
+    uint64_t *vbval;       // per blob byte: its varint's value if it ends one, then the prefix (nvbytes + 1)
+    uint64_t *vruns;       // run starts, by run index (nvbytes + 1)
     DTileD *tiles;         // per decode tile
     TileDec *tdec;         // per decode tile
     uint32_t *err;         // error flags

@@ -119,3 +119,66 @@ def test_corrupt_streams_report_errors(qa):
             qa.decompress_column(b, qa.DType.F64, 10)
         except qa.QuoinError as e:
             assert e.code == qa.ERR_CORRUPT
+
+
+def _blob_range(stream):
+    """(start, end) of the validity blob: after the 16-byte header, varint
+    length then the blob (src/format.rs:24-94)."""
+    pos, shift, vlen = 16, 0, 0
+    while True:
+        b = stream[pos]
+        pos += 1
+        vlen |= (b & 0x7F) << shift
+        shift += 7
+        if not b & 0x80:
+            break
+    return pos, pos + vlen
+
+
+def test_validity_blob_damage_matches_oracle(qa):
+    """Damage inside the validity blob (runs or raw bitmap): the decoder gives
+    the oracle's values and bitmap when the oracle decodes, and CORRUPT when it
+    does not (src/validity.rs:73-119)."""
+    r = np.random.default_rng(17)
+    v = _data.int_columns(6000, seed=7)["u32_k11"]
+    n = len(v)
+    tags = set()
+    for frac, seed in ((0.03, 4), (0.5, 5)):  # runs blob, raw bitmap blob
+        vv, bm = _data.with_nulls(v, frac, seed=seed)
+        s = orc.compress(vv, orc.U32, bm, orc.FASTEST)
+        b0, b1 = _blob_range(s)
+        tags.add(s[b0])
+        muts = []
+        for tag in (0, 1, 2, 0x81):
+            muts.append([(b0, tag)])
+        for _ in range(120):
+            k = int(r.integers(b0 + 1, b1))
+            kind = int(r.integers(0, 3))
+            val = (s[k] ^ (1 << int(r.integers(0, 8)))) if kind == 0 else (s[k] | 0x80 if kind == 1 else 0)
+            muts.append([(k, val)])
+        print(f"case validity damage tag={s[b0]} blob={b1 - b0} B mutations={len(muts)}", flush=True)
+        ok = bad = 0
+        for i, m in enumerate(muts):
+            b = bytearray(s)
+            for k, val in m:
+                b[k] = val
+            b = bytes(b)
+            try:
+                _, want_vals, want_vb = orc.decompress(b)
+            except ValueError:
+                want_vals = None
+            try:
+                out, vout = qa.decompress_column(b, qa.DType.U32, n, True)
+            except qa.QuoinError as e:
+                assert want_vals is None and e.code == qa.ERR_CORRUPT, (i, m, e.code)
+                bad += 1
+                continue
+            assert want_vals is not None, (i, m)
+            assert out.tobytes() == want_vals.tobytes(), (i, m)
+            exp = np.frombuffer(want_vb, dtype=np.uint8).copy()
+            if n % 8:
+                exp[-1] &= (1 << (n % 8)) - 1
+            assert vout.tobytes() == exp.tobytes(), (i, m)
+            ok += 1
+        assert ok and bad, (ok, bad)
+    assert tags == {0, 1}, tags
