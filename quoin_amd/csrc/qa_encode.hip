@@ -256,6 +256,47 @@ __device__ __forceinline__ uint32_t row_sum32(uint32_t v) {
     return v;
 }
 
+// u32 lanes keep their values, min, max and equality tests in 32 bits
+template <int LK> struct LaneVal { using T = uint64_t; };
+template <> struct LaneVal<QA_LK_U32> { using T = uint32_t; };
+
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_t(T v) {
+    if constexpr (sizeof(T) == 4) return qa_dpp32<CTRL>(0, v);
+    else return qa_dpp64<CTRL>(0, v);
+}
+template <bool MAX, typename T>
+__device__ __forceinline__ T row_minmax(T v) {
+    T o = dpp_t<QA_DPP_QP_1032>(v);
+    v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+    o = dpp_t<QA_DPP_QP_2301>(v);
+    v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+    o = dpp_t<QA_DPP_ROW_HALF_MIRROR>(v);
+    v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+    o = dpp_t<QA_DPP_ROW_MIRROR>(v);
+    v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+    return v;
+}
+
+// 16 consecutive values of a u32 lane from index i0
+template <int LK>
+__device__ __forceinline__ void load16(const void *lp, uint32_t lk, uint64_t i0, uint32_t (&v)[16]) {
+    const qa_gu32 *p = (const qa_gu32 *)lp + i0;
+    if (((uintptr_t)p & 15) == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const qa_u32x4 w = ((const __attribute__((address_space(1))) qa_u32x4 *)p)[j];
+            v[4 * j] = w.x;
+            v[4 * j + 1] = w.y;
+            v[4 * j + 2] = w.z;
+            v[4 * j + 3] = w.w;
+        }
+        return;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = p[u];
+}
+
 // A wavefront takes four tiles at a time, one per row of 16 lanes. Lane L of
 // a row walks four chunks of its tile: chunk c holds tile values 256 c + 16 L
 // .. 256 c + 16 L + 15, so neighbours are registers of the same lane (the first
@@ -279,33 +320,36 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
         const bool act = t < ntl;  // a row past the region's values reads its first value
         const uint64_t k0 = act ? rs + (uint64_t)t * QA_TILE : rs;  // column index of the tile start
         const uint32_t cnt = act ? (uint32_t)min((uint64_t)QA_TILE, rs + len - k0) : 1u;
-        const uint64_t before = k0 ? lane_get_g<LK>(lp, lk, k0 - 1) : 0;
-        const uint64_t vf = lane_get_g<LK>(lp, lk, k0);
-        uint64_t umin = ~0ull, umax = 0, dzmin = ~0ull, dzmax = 0, xb = 0;
+        using VT = typename LaneVal<LK>::T;
+        const VT before = k0 ? (VT)lane_get_g<LK>(lp, lk, k0 - 1) : 0;
+        const VT vf = (VT)lane_get_g<LK>(lp, lk, k0);
+        VT umin = (VT)~0ull, umax = 0;
+        uint64_t dzmin = ~0ull, dzmax = 0, xb = 0;
         uint64_t smin = 0x7FFFFFFFFFFFFFFFull, smax = 0x8000000000000000ull;
         uint32_t eqs = 0;  // equal neighbours inside the tile, + (1 << 16) across tiles
-        uint64_t carry = before;  // lane 15: the last value of the previous chunk
+        VT carry = before;  // lane 15: the last value of the previous chunk
 #pragma unroll 1
         for (uint32_t c = 0; c < 4; c++) {
             const uint32_t kb = 256 * c + 16 * L;  // this lane's first tile index
-            uint64_t v[16];
+            VT v[16];
             if (kb + 16 <= cnt) {
                 load16<LK>(lp, lk, k0 + kb, v);
             } else {
 #pragma unroll
                 for (int u = 0; u < 16; u++)  // past cnt: vf, neutral for min and max
-                    v[u] = kb + u < cnt ? lane_get_g<LK>(lp, lk, k0 + kb + u) : vf;
+                    v[u] = kb + u < cnt ? (VT)lane_get_g<LK>(lp, lk, k0 + kb + u) : vf;
             }
-            const uint64_t pv0 = qa_dpp64<QA_DPP_ROW_ROR1>(0, L == 15 ? carry : v[15]);
+            const VT pv0 = dpp_t<QA_DPP_ROW_ROR1>(L == 15 ? carry : v[15]);
             carry = v[15];
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const uint32_t k = kb + i;
                 const bool in = k < cnt;
-                const uint64_t x = v[i];
-                const uint64_t p = i ? v[i - 1] : pv0;
-                umin = x < umin ? x : umin;
-                umax = x > umax ? x : umax;
+                const VT xv = v[i];
+                const VT pvv = i ? v[i - 1] : pv0;
+                const uint64_t x = xv, p = pvv;
+                umin = xv < umin ? xv : umin;
+                umax = xv > umax ? xv : umax;
                 if (SG) {
                     smin = (int64_t)x < (int64_t)smin ? x : smin;
                     smax = (int64_t)x > (int64_t)smax ? x : smax;
@@ -316,7 +360,7 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
                     dzmin = z < dzmin ? z : dzmin;
                     dzmax = z > dzmax ? z : dzmax;
                     if (FASTL) xb += qa_varint_len(x ^ p);
-                    eqs += x == p;
+                    eqs += xv == pvv;
                 }
                 if (i == 0 && k == 0 && k0 > rs && x == p) eqs += 1u << 16;  // across tiles of the region
                 if (REG && act) {  // every tile of a full region is full
@@ -341,8 +385,8 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
                 }
             }
         }
-        umin = row_reduce64(umin, OpMinU());
-        umax = row_reduce64(umax, OpMaxU());
+        umin = row_minmax<false>(umin);
+        umax = row_minmax<true>(umax);
         if (SG) {
             smin = row_reduce64(smin, OpMinS());
             smax = row_reduce64(smax, OpMaxS());
@@ -358,7 +402,7 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
         if (L == 0 && t < col.tpr) {
             DTile &ts = tl[t];
             const uint64_t vl = lane_get_g<LK>(lp, lk, k0 + cnt - 1);
-            const uint64_t step = cnt > 1 ? lane_get_g<LK>(lp, lk, k0 + 1) - vf : 0;
+            const uint64_t step = cnt > 1 ? lane_get_g<LK>(lp, lk, k0 + 1) - (uint64_t)vf : 0;
             if (!act) {
                 ts.active = 0;
             } else {
@@ -375,7 +419,7 @@ __device__ __forceinline__ uint32_t region_tiles(const DCol &col, const void *lp
                 ts.v_first = vf;
                 ts.v_last = vl;
                 ts.step = step;
-                ts.dz_first = k0 ? qa_zigzag(vf - before) : 0;
+                ts.dz_first = k0 ? qa_zigzag((uint64_t)vf - (uint64_t)before) : 0;
                 ts.cnt = cnt;
                 ts.arith = (uint8_t)arith;
                 ts.allsame = (uint8_t)same;
