@@ -627,15 +627,33 @@ __global__ void __launch_bounds__(512) k_region_stats(EncArgs a) {
 // ---------------------------------------------------------------------------
 #define PLAN_CHUNK 4096
 
+// The walk over an adaptive column's regions as a scan: s = regions left to
+// skip before the next block start. At a region, s = 0 makes it a start and
+// sets s = 3 if its probe grows the block (four regions), else 0; s > 0
+// decrements. A region's transition is a map of the four states, packed as
+// 4 x 2 bits; maps compose associatively, so the states come from a prefix
+// scan of the maps.
+#define PLAN_ID 0xE4u  // s -> s
+#define PLAN_T0 0x90u  // 0 -> 0 (a one-region block starts), 1 -> 0, 2 -> 1, 3 -> 2
+#define PLAN_T1 0x93u  // 0 -> 3 (a grown block starts), 1 -> 0, 2 -> 1, 3 -> 2
+__device__ __forceinline__ uint32_t plan_compose(uint32_t first, uint32_t then) {
+    uint32_t out = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) out |= ((then >> (2 * ((first >> (2 * q)) & 3))) & 3) << (2 * q);
+    return out;
+}
+
 // A block that starts at region lr ends at min(start + 131072, nv) when its
 // probe says distinct_low || repeats and the column is adaptive (regions are
 // 32768 values there), else at min(start + R, nv); the next block starts at
-// region min(lr + 4 or 1, nreg). The walk runs on one thread over bitmask
-// words of the region features held in registers; owners are found by a
-// backward scan of at most three regions (a block spans at most four).
+// region min(lr + 4 or 1, nreg). Each thread takes 16 regions of a chunk of
+// 4096: it composes their maps, a scan over the 256 threads gives each its
+// entry state, and it marks its starts; owners are found by a backward scan
+// of at most three regions (a block spans at most four).
 __global__ void __launch_bounds__(256) k_plan(EncArgs a) {
     __shared__ uint64_t featw[PLAN_CHUNK / 64], startw[PLAN_CHUNK / 64];
     __shared__ uint32_t next_s, nb_s, last_s;
+    __shared__ uint32_t fsc[256], cl_nb, cl_last, cl_next;
     const int c = blockIdx.x;
     const DCol &col = a.cols[c];
     const uint64_t nv = a.cst[c].nv;
@@ -668,36 +686,62 @@ __global__ void __launch_bounds__(256) k_plan(EncArgs a) {
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint32_t lim = min(base + cn, nreg);
-            uint32_t ns = next_s, nb = nb_s, last = last_s;
-            if (!adaptive) {
-                if (ns < lim) {
-                    nb += lim - ns;
-                    last = lim - 1;
-                    ns = lim;
-                }
-            } else {
-                uint32_t cur = NONE32;
-                uint64_t fw = 0, sw = 0;
-                while (ns < lim) {
-                    const uint32_t k = ns - base, wi = k >> 6;
-                    if (wi != cur) {
-                        if (cur != NONE32) startw[cur] = sw;
-                        cur = wi;
-                        fw = featw[wi];
-                        sw = 0;
-                    }
-                    sw |= 1ull << (k & 63);
-                    last = ns;
-                    nb++;
-                    ns = min(ns + (((fw >> (k & 63)) & 1) ? grow : 1u), nreg);
-                }
-                if (cur != NONE32) startw[cur] = sw;
+        const uint32_t lim = min(base + cn, nreg);
+        if (!adaptive) {
+            if (threadIdx.x == 0 && next_s < lim) {
+                nb_s += lim - next_s;
+                last_s = lim - 1;
+                next_s = lim;
             }
-            next_s = ns;
-            nb_s = nb;
-            last_s = last;
+        } else {
+            const uint32_t t = threadIdx.x, k0 = 16 * t;
+            uint32_t F = PLAN_ID;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t k = k0 + i;
+                if (base + k < lim)
+                    F = plan_compose(F, ((featw[k >> 6] >> (k & 63)) & 1) ? PLAN_T1 : PLAN_T0);
+            }
+            if (t == 0) {
+                cl_nb = 0;
+                cl_last = 0;
+                cl_next = next_s;
+            }
+            // inclusive scan of the maps in region order (Hillis-Steele over the threads)
+            fsc[t] = F;
+            __syncthreads();
+            for (uint32_t d = 1; d < 256; d <<= 1) {
+                const uint32_t other = t >= d ? fsc[t - d] : PLAN_ID;
+                __syncthreads();
+                if (t >= d) fsc[t] = plan_compose(other, fsc[t]);
+                __syncthreads();
+            }
+            // the state at the chunk's first region: next_s is the first start at or after it
+            const uint32_t s0 = next_s > base ? next_s - base : 0;
+            uint32_t st = ((t ? fsc[t - 1] : PLAN_ID) >> (2 * s0)) & 3;
+            uint32_t bits = 0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t k = k0 + i;
+                if (base + k < lim) {
+                    const bool f = (featw[k >> 6] >> (k & 63)) & 1;
+                    if (st == 0) bits |= 1u << i;
+                    st = st == 0 ? (f ? grow - 1 : 0u) : st - 1;
+                }
+            }
+            if (bits) {
+                atomicOr((unsigned long long *)&startw[k0 >> 6], (unsigned long long)bits << (k0 & 63));
+                atomicAdd(&cl_nb, (uint32_t)__popc(bits));
+                atomicMax(&cl_last, k0 + 31 - (uint32_t)__clz(bits));
+            }
+            if (lim > base && k0 <= lim - 1 - base && lim - 1 - base < k0 + 16)
+                cl_next = lim + st;  // the first start after the chunk's last region
+            __syncthreads();
+            if (t == 0) {
+                nb_s += cl_nb;
+                if (cl_nb) last_s = base + cl_last;
+                next_s = cl_next;
+            }
         }
         __syncthreads();
         for (uint32_t k = threadIdx.x; k < cn; k += blockDim.x) {
