@@ -54,6 +54,22 @@ __device__ __forceinline__ bool alp_digit(uint64_t bits, uint32_t e, uint32_t f,
     *d = di;
     return true;
 }
+// alp_digit without the integer round trip, for the searches: the rounded
+// digit r is an integer-valued double with |r| <= 2^63, so (double)(i64)r == r
+// (at 2^63 the saturated i64 converts back to 2^63) and the verify product is
+// the same; the digit is returned as that double (order-preserving, converted
+// by sat_i64 where an integer is needed). m1 = 10^e, m2 = 10^-f, m3 = 10^f,
+// m4 = 10^-e.
+__device__ __forceinline__ bool alp_digit_r(uint64_t bits, double m1, double m2, double m3, double m4,
+                                            double *r) {
+    const double v = b2f(bits);
+    const double tmp = __dmul_rn(__dmul_rn(v, m1), m2);
+    if (!isfinite(tmp) || tmp >= ALP_UPPER || tmp <= -ALP_UPPER) return false;
+    const double rr = __dsub_rn(__dadd_rn(tmp, ALP_MAGIC), ALP_MAGIC);
+    if (f2b(__dmul_rn(__dmul_rn(rr, m3), m4)) != bits) return false;
+    *r = rr;
+    return true;
+}
 // float_mult.rs:50-58
 __device__ __forceinline__ bool fm_k(uint64_t bits, int s, int64_t *k) {
     double x = b2f(bits);
@@ -68,28 +84,40 @@ __device__ __forceinline__ bool fm_k(uint64_t bits, int s, int64_t *k) {
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
 
+// fm_k without the integer round trip: round(x * s) is an integer-valued
+// double of magnitude < 9e18 < 2^63, so converting it to i64 and back is exact
+__device__ __forceinline__ bool fm_k_r(uint64_t bits, double sc, double *k) {
+    const double kk = round(__dmul_rn(b2f(bits), sc));
+    if (!isfinite(kk) || fabs(kk) >= 9.0e18) return false;
+    if (f2b(__ddiv_rn(kk, sc)) != bits) return false;
+    *k = kk;
+    return true;
+}
+
 // FloatMult at scale s over a tile held in registers (k = lane + 64 q): every
 // value round-trips (float_mult.rs:45-64), with a wave-wide early exit on the
 // first failure; on success the k range goes to the tile record
 __device__ __forceinline__ bool fm_tile(const uint64_t (&xv)[QA_TILE / WAVE], uint32_t cnt, int s,
                                         int lane, DTileF *tf) {
-    int64_t kmn = INT64_MAX, kmx = INT64_MIN;
+    const double sc = c_fm_scales[s];
+    double fmn = 1e300, fmx = -1e300;  // k as exact doubles: min / max commute with the conversion
 #pragma unroll
     for (int q = 0; q < QA_TILE / WAVE; q++) {
         if ((uint32_t)(WAVE * q) >= cnt) break;
         const uint32_t k = lane + WAVE * q;
         bool bad = false;
         if (k < cnt) {
-            int64_t kk;
-            if (fm_k(xv[q], s, &kk)) {
-                kmn = kk < kmn ? kk : kmn;
-                kmx = kk > kmx ? kk : kmx;
+            double kk;
+            if (fm_k_r(xv[q], sc, &kk)) {
+                fmn = kk < fmn ? kk : fmn;
+                fmx = kk > fmx ? kk : fmx;
             } else {
                 bad = true;
             }
         }
         if (wave_any(bad)) return false;
     }
+    int64_t kmn = fmn == 1e300 ? INT64_MAX : (int64_t)fmn, kmx = fmx == -1e300 ? INT64_MIN : (int64_t)fmx;
     kmn = wave_reduce(kmn, OpMinS());
     kmx = wave_reduce(kmx, OpMaxS());
     if (lane == 0) {
@@ -166,17 +194,19 @@ __global__ void __launch_bounds__(256) k_float_tiles(EncArgs a) {
         uint32_t e = 0;
         while ((e + 1) * (e + 2) / 2 <= combo) e++;
         uint32_t f = combo - e * (e + 1) / 2;
+        const double m1 = c_exp10[e], m2 = c_frac10[f], m3 = c_exp10[f], m4 = c_frac10[e];
         uint32_t exc = 0;
-        int64_t lo = INT64_MAX, hi = INT64_MIN;
+        double flo = 1e300, fhi = -1e300;
         for (uint32_t j = 0; j < sn; j++) {
-            int64_t d;
-            if (alp_digit(samp[wv][j], e, f, &d)) {
-                lo = d < lo ? d : lo;
-                hi = d > hi ? d : hi;
+            double r;
+            if (alp_digit_r(samp[wv][j], m1, m2, m3, m4, &r)) {
+                flo = r < flo ? r : flo;
+                fhi = r > fhi ? r : fhi;
             } else {
                 exc++;
             }
         }
+        const int64_t lo = exc == sn ? INT64_MAX : sat_i64(flo), hi = exc == sn ? INT64_MIN : sat_i64(fhi);
         uint32_t width;
         if (exc == sn || hi <= lo)
             width = exc == sn ? 64 : 0;
