@@ -41,25 +41,12 @@ __device__ __forceinline__ int64_t sat_i64(double x) {
     if (x < -9223372036854775808.0) return INT64_MIN;
     return (int64_t)x;
 }
-// alp.rs:33-45
-__device__ __forceinline__ bool alp_digit(uint64_t bits, uint32_t e, uint32_t f, int64_t *d) {
-    double v = b2f(bits);
-    double t1 = __dmul_rn(v, c_exp10[e]);
-    double tmp = __dmul_rn(t1, c_frac10[f]);
-    if (!isfinite(tmp) || tmp >= ALP_UPPER || tmp <= -ALP_UPPER) return false;
-    double r = __dsub_rn(__dadd_rn(tmp, ALP_MAGIC), ALP_MAGIC);
-    int64_t di = sat_i64(r);
-    double back = __dmul_rn(__dmul_rn((double)di, c_exp10[f]), c_frac10[e]);
-    if (f2b(back) != bits) return false;
-    *d = di;
-    return true;
-}
-// alp_digit without the integer round trip, for the searches: the rounded
-// digit r is an integer-valued double with |r| <= 2^63, so (double)(i64)r == r
-// (at 2^63 the saturated i64 converts back to 2^63) and the verify product is
-// the same; the digit is returned as that double (order-preserving, converted
-// by sat_i64 where an integer is needed). m1 = 10^e, m2 = 10^-f, m3 = 10^f,
-// m4 = 10^-e.
+// ALP's digit of a value (alp.rs:33-45) and its round-trip check, without an
+// integer round trip: the rounded digit r is an integer-valued double with
+// |r| <= 2^63, so (double)(i64)r == r (at 2^63 the saturated i64 converts back
+// to 2^63) and the verify product is the same; the digit is returned as that
+// double (order-preserving, converted by sat_i64 where an integer is needed).
+// m1 = 10^e, m2 = 10^-f, m3 = 10^f, m4 = 10^-e.
 __device__ __forceinline__ bool alp_digit_r(uint64_t bits, double m1, double m2, double m3, double m4,
                                             double *r) {
     const double v = b2f(bits);
@@ -70,21 +57,10 @@ __device__ __forceinline__ bool alp_digit_r(uint64_t bits, double m1, double m2,
     *r = rr;
     return true;
 }
-// float_mult.rs:50-58
-__device__ __forceinline__ bool fm_k(uint64_t bits, int s, int64_t *k) {
-    double x = b2f(bits);
-    double sc = c_fm_scales[s];
-    double kk = round(__dmul_rn(x, sc));
-    if (!isfinite(kk) || fabs(kk) >= 9.0e18) return false;
-    int64_t ki = (int64_t)kk;
-    if (f2b(__ddiv_rn((double)ki, sc)) != bits) return false;
-    *k = ki;
-    return true;
-}
-
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
 
-// fm_k without the integer round trip: round(x * s) is an integer-valued
+// FloatMult's k = round(x * s) and its round-trip check (float_mult.rs:50-58),
+// without an integer round trip: round(x * s) is an integer-valued
 // double of magnitude < 9e18 < 2^63, so converting it to i64 and back is exact
 __device__ __forceinline__ bool fm_k_r(uint64_t bits, double sc, double *k) {
     const double kk = round(__dmul_rn(b2f(bits), sc));
@@ -225,7 +201,8 @@ __global__ void __launch_bounds__(256) k_float_tiles(EncArgs a) {
     uint32_t nexc = 0, w = 0;
     int64_t dmin = 0;
     if (!bail) {
-        int64_t lo = INT64_MAX, hi = INT64_MIN;
+        const double m1 = c_exp10[be], m2 = c_frac10[bf], m3 = c_exp10[bf], m4 = c_frac10[be];
+        double flo = 1e300, fhi = -1e300;
         uint32_t first_valid = 0xFFFFFFFFu;
         int64_t fv_digit = 0;
 #pragma unroll
@@ -234,20 +211,21 @@ __global__ void __launch_bounds__(256) k_float_tiles(EncArgs a) {
             if (kb >= cnt) break;
             const uint32_t k = kb + lane;
             bool valid = false;
-            int64_t d = 0;
-            if (k < cnt) valid = alp_digit(xv[q], be, bf, &d);
+            double r = 0;
+            if (k < cnt) valid = alp_digit_r(xv[q], m1, m2, m3, m4, &r);
             if (k < cnt && !valid) nexc++;
             if (valid) {
-                lo = d < lo ? d : lo;
-                hi = d > hi ? d : hi;
+                flo = r < flo ? r : flo;
+                fhi = r > fhi ? r : fhi;
             }
             uint64_t bal = __ballot(valid);
             if (first_valid == 0xFFFFFFFFu && bal) {
                 int src = __ffsll((long long)bal) - 1;
                 first_valid = kb + src;
-                fv_digit = __shfl(d, src, 64);
+                fv_digit = sat_i64(b2f(__shfl(f2b(r), src, 64)));
             }
         }
+        int64_t lo = flo == 1e300 ? INT64_MAX : sat_i64(flo), hi = fhi == -1e300 ? INT64_MIN : sat_i64(fhi);
         nexc = (uint32_t)wave_reduce((uint64_t)nexc, OpAdd());
         lo = wave_reduce(lo, OpMinS());
         hi = wave_reduce(hi, OpMaxS());
