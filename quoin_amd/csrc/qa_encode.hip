@@ -1419,15 +1419,46 @@ static inline unsigned nb(uint64_t n, unsigned b) { return (unsigned)((n + b - 1
     } while (0)
 
 // column-level stages: validity counts, compaction, region statistics, plan
-void qa_encode_plan_launch(const EncArgs &a, hipStream_t s) {
+static void encode_counts_launch(const EncArgs &a, hipStream_t s) {
     if (a.nvwords) {
         QA_LAUNCH(k_vcount, dim3(nb(a.nvwords, 256)), dim3(256), 0, s, a);
         qa_exclusive_scan2(a.vpre, a.vpre, a.tpre, a.tpre, a.nvwords, a.scan_tmp, s);
     }
     QA_LAUNCH(k_col_counts, dim3(nb(a.ncols, 64)), dim3(64), 0, s, a);
+}
+
+static void encode_stats_launch(const EncArgs &a, hipStream_t s) {
     if (a.nvwords) QA_LAUNCH(k_compact, dim3(nb(a.nvwords, 4 * WAVE)), dim3(256), 0, s, a);
     QA_LAUNCH(k_region_stats, dim3(a.nregions), dim3(512), 0, s, a);
     QA_LAUNCH(k_plan, dim3(a.ncols), dim3(256), 0, s, a);
+}
+
+// the validity blob's runs: transition positions, varint sizes, their prefix
+// (reads only the counts and transition prefix; writes tpos and rsz, which
+// nothing reads before k_col_layout)
+static void encode_vruns_launch(const EncArgs &a, hipStream_t s) {
+    if (!a.nvwords) return;
+    const uint64_t nruns_max = a.ntrans_max + a.ncols;
+    QA_LAUNCH(k_vtrans_pos, dim3(nb(a.nvwords, 256)), dim3(256), 0, s, a);
+    unsigned g = nb(nruns_max, 256) < 4096 ? nb(nruns_max, 256) : 4096;
+    QA_LAUNCH(k_vrun_sizes, dim3(g), dim3(256), 0, s, a);
+    qa_exclusive_scan_dn(a.rsz, nruns_max, a.colbase + a.ncols + 1, a.rsz, a.scan_tmp, s);
+}
+
+static void encode_layout_rest_launch(const EncArgs &a, hipStream_t s) {
+    const uint64_t nruns_max = a.ntrans_max + a.ncols;
+    QA_LAUNCH(k_col_layout, dim3(a.ncols), dim3(256), 0, s, a);
+    QA_LAUNCH(k_emit_header, dim3(nb(a.ncols, 64)), dim3(64), 0, s, a);
+    if (a.nvwords) {
+        unsigned g = nb(nruns_max, 256) < 4096 ? nb(nruns_max, 256) : 4096;
+        QA_LAUNCH(k_emit_vruns, dim3(g), dim3(256), 0, s, a);
+        QA_LAUNCH(k_emit_vraw, dim3(nb(a.nvwords * 8, 256)), dim3(256), 0, s, a);
+    }
+}
+
+void qa_encode_plan_launch(const EncArgs &a, hipStream_t s) {
+    encode_counts_launch(a, s);
+    encode_stats_launch(a, s);
 }
 
 // per-block stages over region slots [r0, r1) and tiles [t0, t1): features of
@@ -1440,20 +1471,8 @@ void qa_encode_blocks_launch(const EncArgs &a, hipStream_t s) {
 
 // column-level stages: validity blob, frame layout, header
 void qa_encode_layout_launch(const EncArgs &a, hipStream_t s) {
-    uint64_t nruns_max = a.ntrans_max + a.ncols;
-    if (a.nvwords) {
-        QA_LAUNCH(k_vtrans_pos, dim3(nb(a.nvwords, 256)), dim3(256), 0, s, a);
-        unsigned g = nb(nruns_max, 256) < 4096 ? nb(nruns_max, 256) : 4096;
-        QA_LAUNCH(k_vrun_sizes, dim3(g), dim3(256), 0, s, a);
-        qa_exclusive_scan_dn(a.rsz, nruns_max, a.colbase + a.ncols + 1, a.rsz, a.scan_tmp, s);
-    }
-    QA_LAUNCH(k_col_layout, dim3(a.ncols), dim3(256), 0, s, a);
-    QA_LAUNCH(k_emit_header, dim3(nb(a.ncols, 64)), dim3(64), 0, s, a);
-    if (a.nvwords) {
-        unsigned g = nb(nruns_max, 256) < 4096 ? nb(nruns_max, 256) : 4096;
-        QA_LAUNCH(k_emit_vruns, dim3(g), dim3(256), 0, s, a);
-        QA_LAUNCH(k_emit_vraw, dim3(nb(a.nvwords * 8, 256)), dim3(256), 0, s, a);
-    }
+    encode_vruns_launch(a, s);
+    encode_layout_rest_launch(a, s);
 }
 
 // per-block stages over [r0, r1) / [t0, t1): frame headers and payloads
@@ -1486,9 +1505,24 @@ void qa_encode_emit_launch(const EncArgs &a, hipStream_t s) {
 }
 
 // the batched form: every block of every column in one launch per stage
-void qa_encode_launch(const EncArgs &a, hipStream_t s, int) {
-    qa_encode_plan_launch(a, s);
+// The batched sequence. With a side stream, the validity blob's run chain
+// (latency-bound small kernels) runs there, forked after the transition
+// counts and joined before the frame layout, beside compaction, statistics,
+// planning and the competition; both streams are at the default priority.
+void qa_encode_launch(const EncArgs &a, hipStream_t s, int, hipStream_t side, hipEvent_t fork,
+                      hipEvent_t join) {
+    encode_counts_launch(a, s);
+    const bool forked = side && a.nvwords;
+    if (forked) {
+        hipEventRecord(fork, s);
+        hipStreamWaitEvent(side, fork, 0);
+        encode_vruns_launch(a, side);
+        hipEventRecord(join, side);
+    }
+    encode_stats_launch(a, s);
     qa_encode_blocks_launch(a, s);
-    qa_encode_layout_launch(a, s);
+    if (forked) hipStreamWaitEvent(s, join, 0);
+    else encode_vruns_launch(a, s);
+    encode_layout_rest_launch(a, s);
     qa_encode_emit_launch(a, s);
 }

@@ -116,6 +116,9 @@ struct QuoinGpu {
     uint32_t tune[QA_TUNE_N] = {0, 0, 0};
     bool autotune = true;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, kev0 = nullptr, kev1 = nullptr;
+    // a second stream (default priority) for the encoder's validity-run chain
+    hipStream_t side = nullptr;
+    hipEvent_t efork = nullptr, ejoin = nullptr;
     double last_total_ms = 0, last_kernel_ms = 0;
     const char *last_kernel = "";
 };
@@ -347,7 +350,10 @@ extern "C" QuoinGpu *quoin_gpu_create(int device, uint64_t hbm_budget_bytes) {
     g->budget = hbm_budget_bytes ? hbm_budget_bytes : kDefaultBudget;
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess ||
-        hipEventCreate(&g->kev0) != hipSuccess || hipEventCreate(&g->kev1) != hipSuccess) {
+        hipEventCreate(&g->kev0) != hipSuccess || hipEventCreate(&g->kev1) != hipSuccess ||
+        hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&g->efork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ejoin, hipEventDisableTiming) != hipSuccess) {
         delete g;
         return nullptr;
     }
@@ -376,6 +382,10 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
     hipEventDestroy(g->ev1);
     hipEventDestroy(g->kev0);
     hipEventDestroy(g->kev1);
+    hipStreamSynchronize(g->side);
+    hipEventDestroy(g->efork);
+    hipEventDestroy(g->ejoin);
+    hipStreamDestroy(g->side);
     hipStreamDestroy(g->stream);
     delete g;
 }
@@ -518,7 +528,7 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         a.kev0 = g->kev0;
         a.kev1 = g->kev1;
         a.tune = g->autotune ? g->tune : nullptr;
-        qa_encode_launch(a, s, level);
+        qa_encode_launch(a, s, level, g->side, g->efork, g->ejoin);
     } else {
         // plain form: column-level planning, then one launch sequence per
         // block for its competition and again for its payload

@@ -95,7 +95,8 @@ void qa_fast_sizes_launch(const EncArgs &a, hipStream_t s);
 void qa_fast_final_launch(const EncArgs &a, hipStream_t s);
 void qa_fast_emit_launch(const EncArgs &a, hipStream_t s);
 
-void qa_encode_launch(const EncArgs &a, hipStream_t s, int level_max);
+void qa_encode_launch(const EncArgs &a, hipStream_t s, int level_max, hipStream_t side = nullptr,
+                      hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
 void qa_encode_plan_launch(const EncArgs &a, hipStream_t s);
 void qa_encode_blocks_launch(const EncArgs &a, hipStream_t s);
 void qa_encode_layout_launch(const EncArgs &a, hipStream_t s);
