@@ -152,3 +152,66 @@ def test_autotuned_shapes_give_the_same_bytes(qa):
     finally:
         tuned.close()
         fixed.close()
+
+
+def _validity_shapes():
+    """u32 columns whose validity blobs take every path of the bitmap decoder:
+    runs spanning many 4096-bit windows, hundreds of runs per window, a raw
+    bitmap, an all-null column, columns shorter than one 64-word window (their
+    windows straddle columns), runs ending on word edges and at n."""
+    r = np.random.default_rng(91)
+    out = []
+
+    def col(n, valid):
+        v = r.integers(0, 1 << 20, n, dtype=np.uint64).astype(np.uint32)
+        v[~valid] = 0
+        out.append((v, np.packbits(valid, bitorder="little")))
+
+    n = 600000
+    valid = np.ones(n, bool)
+    valid[r.choice(n, 5, replace=False)] = False
+    col(n, valid)                                   # sparse: runs of ~10^5
+    n = 70001
+    valid = np.ones(n, bool)
+    valid[1::2] = False
+    col(n, valid)                                   # alternating: raw bitmap blob
+    n = 200000
+    valid = np.ones(n, bool)
+    for s in range(0, n, 23):
+        valid[s:s + 1 + (s % 3)] = False
+    col(n, valid)                                   # ~350 runs per window
+    col(5000, np.zeros(5000, bool))                 # all null
+    valid = np.ones(100, bool)
+    valid[[3, 4, 50, 99]] = False
+    col(100, valid)                                 # shorter than a window
+    n = 64 * 300
+    valid = np.ones(n, bool)
+    valid[64 * 10:64 * 20] = False
+    valid[-64:] = False
+    col(n, valid)                                   # word edges, a null run ending at n
+    return out
+
+
+def test_validity_shapes_in_one_batch(qa):
+    """Every blob shape in one batch (so bitmap windows straddle columns):
+    streams equal the oracle's, and the decode returns the oracle's values and
+    bitmaps (src/validity.rs:40-119)."""
+    cols = _validity_shapes()
+    gpu = qa.Gpu(0)
+    try:
+        print(f"case validity shapes {len(cols)} columns", flush=True)
+        got = _gpu.device_compress(qa, gpu, cols, qa.Config(level=qa.Level.FASTEST))
+        for i, ((v, bm), g) in enumerate(zip(cols, got)):
+            assert g == orc.compress(v, orc.U32, bm, orc.FASTEST), f"column {i}: stream"
+        outs = _gpu.device_decompress(qa, gpu, got, [qa.DType.U32] * len(cols),
+                                      [len(v) for v, _ in cols], True)
+        for i, ((v, bm), (o, vo)) in enumerate(zip(cols, outs)):
+            _, want, want_vb = orc.decompress(got[i])
+            assert o.tobytes() == want.tobytes(), f"column {i}: values"
+            exp = np.frombuffer(want_vb, dtype=np.uint8).copy()
+            if len(v) % 8:
+                exp[-1] &= (1 << (len(v) % 8)) - 1
+            assert vo == exp.tobytes(), f"column {i}: bitmap"
+            assert vo == bm.tobytes(), f"column {i}: bitmap vs input"
+    finally:
+        gpu.close()
