@@ -1028,19 +1028,20 @@ __global__ void k_emit_vruns(EncArgs a) {
 }
 
 // raw validity blob: the bitmap bytes verbatim (src/validity.rs:63-68)
+// the raw bitmap blob [0][ceil(n / 8) bytes] of columns whose runs blob is not
+// smaller; one thread per bitmap word (8 bytes)
 __global__ void k_emit_vraw(EncArgs a) {
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.nvwords * 8) return;
-    int c = col_by_vword(a.cols, a.ncols, g / 8);
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.nvwords) return;
+    int c = col_by_vword(a.cols, a.ncols, j);
     const DCol &col = a.cols[c];
     const DColState &st = a.cst[c];
     if (!col.nvwords || !st.has_nulls || st.err) return;
-    uint64_t raw = (col.n + 7) / 8;
+    const uint64_t raw = (col.n + 7) / 8;
     if (st.vblob_rle < raw) return;
-    uint64_t bi = g - col.vword_base * 8;
-    if (bi >= raw) return;
-    uint64_t off = 16 + qa_varint_len(st.vblob_len) + 1;
-    col.dst[off + bi] = col.validity[bi];
+    const uint64_t b0 = (j - col.vword_base) * 8;
+    uint8_t *d = col.dst + 16 + qa_varint_len(st.vblob_len) + 1;
+    for (uint64_t bi = b0; bi < b0 + 8 && bi < raw; bi++) d[bi] = col.validity[bi];
 }
 
 __global__ void k_emit_frames(EncArgs a) {
@@ -1452,7 +1453,7 @@ static void encode_layout_rest_launch(const EncArgs &a, hipStream_t s) {
     if (a.nvwords) {
         unsigned g = nb(nruns_max, 256) < 4096 ? nb(nruns_max, 256) : 4096;
         QA_LAUNCH(k_emit_vruns, dim3(g), dim3(256), 0, s, a);
-        QA_LAUNCH(k_emit_vraw, dim3(nb(a.nvwords * 8, 256)), dim3(256), 0, s, a);
+        QA_LAUNCH(k_emit_vraw, dim3(nb(a.nvwords, 256)), dim3(256), 0, s, a);
     }
 }
 
