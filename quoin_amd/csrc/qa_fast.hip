@@ -581,13 +581,30 @@ __global__ void __launch_bounds__(256) k_alprd_tiles(EncArgs a) {
     const uint32_t rbw = 64 - x.rd_left;
     const uint64_t mask = (1ull << rbw) - 1;
     const uint64_t p0 = (uint64_t)bt * QA_TILE;  // block-relative position of the tile
+    const uint32_t cnt = ts.cnt;
+    // the dictionary in registers (entries are distinct left parts < 2^24, so
+    // ~0 in an unused slot never matches), the tile's values loaded together
+    uint64_t dict[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) dict[q] = (uint32_t)q < x.rd_dl ? x.rd_dict[q] : ~0ull;
+    uint64_t vv[QA_TILE / WAVE];
+#pragma unroll
+    for (int j = 0; j < QA_TILE / WAVE; j++) {
+        const uint32_t k = lane + WAVE * j;
+        vv[j] = k < cnt ? lane_at(st, b.start + p0 + k) : 0;
+    }
     uint64_t cmn = ~0ull, cmx = 0, rmn = ~0ull, rmx = 0, eb = 0, ne = 0;
-    for (uint32_t k = lane; k < ts.cnt; k += WAVE) {
-        uint64_t v = lane_at(st, b.start + p0 + k);
-        uint64_t left = v >> rbw, code = 0;
+#pragma unroll
+    for (int j = 0; j < QA_TILE / WAVE; j++) {
+        const uint32_t k = lane + WAVE * j;
+        if (k >= cnt) break;
+        const uint64_t v = vv[j];
+        const uint64_t left = v >> rbw;
+        uint64_t code = 0;
         bool found = false;
-        for (uint32_t q = 0; q < x.rd_dl; q++)
-            if (!found && x.rd_dict[q] == left) {
+#pragma unroll
+        for (int q = 7; q >= 0; q--)  // the one match, if any
+            if (dict[q] == left) {
                 code = q;
                 found = true;
             }
@@ -597,7 +614,7 @@ __global__ void __launch_bounds__(256) k_alprd_tiles(EncArgs a) {
         }
         cmn = code < cmn ? code : cmn;
         cmx = code > cmx ? code : cmx;
-        uint64_t rr = v & mask;
+        const uint64_t rr = v & mask;
         rmn = rr < rmn ? rr : rmn;
         rmx = rr > rmx ? rr : rmx;
     }
