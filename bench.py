@@ -562,6 +562,8 @@ def float_bench(qa, gpu, dev, steps, warmup, check_oracle=True):
             exp = orc.compress(cols[i][1], orc.F64, None, orc.FAST)
             assert host[i].tobytes() == exp, f"mixed-float stream differs from the oracle on column {i}"
             checked.append(i)
+    dcols = (qa.DeviceColumn * len(dcols))(*dcols)
+    dsts = (qa.DeviceStream * len(dsts))(*dsts)
     for _ in range(warmup):
         gpu.compress(dcols, cfg)
         gpu.decompress(dsts)
@@ -703,6 +705,9 @@ def main():
             assert vb.tobytes() == exp.tobytes(), f"validity mismatch on {name}"
     comp_bytes = sum(lens)
 
+    # the descriptor arrays are built once, as a C caller reusing them would
+    dcols = (qa.DeviceColumn * len(dcols))(*dcols)
+    dsts = (qa.DeviceStream * len(dsts))(*dsts)
     for _ in range(args.warmup):
         gpu.compress(dcols, cfg)
         gpu.decompress(dsts)

@@ -373,8 +373,9 @@ class Gpu:
             raise QuoinError(rc)
 
     def compress(self, cols, config=None, stream=None):
-        """cols: list of DeviceColumn; returns stream lengths."""
-        arr = (DeviceColumn * len(cols))(*cols)
+        """cols: list (or prebuilt ctypes array) of DeviceColumn; returns stream
+        lengths."""
+        arr = cols if isinstance(cols, ctypes.Array) else (DeviceColumn * len(cols))(*cols)
         lens = (ctypes.c_uint64 * len(cols))()
         cfg = _cfg(config)
         rc = lib.quoin_gpu_compress(self.h, ctypes.byref(cfg), arr, len(cols), lens, stream)
@@ -383,7 +384,8 @@ class Gpu:
         return [int(x) for x in lens]
 
     def decompress(self, streams, stream=None):
-        arr = (DeviceStream * len(streams))(*streams)
+        """streams: list (or prebuilt ctypes array) of DeviceStream."""
+        arr = streams if isinstance(streams, ctypes.Array) else (DeviceStream * len(streams))(*streams)
         rc = lib.quoin_gpu_decompress(self.h, arr, len(streams), stream)
         if rc != OK:
             raise QuoinError(rc)
