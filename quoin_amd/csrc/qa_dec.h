@@ -216,7 +216,10 @@ __device__ __forceinline__ uint32_t lds_pos(uint32_t k) {
 
 template <int F, typename V, int SW = 0>
 __device__ __forceinline__ void emit_gather_t(void *out, const uint64_t *bm, uint64_t L0,
-                                              uint64_t L1, const V *vv, uint32_t cnt, int lane) {
+                                              uint64_t L1, const V *vv, uint32_t cnt, int lane,
+                                              bool have_first = false, uint64_t first = 0) {
+    // have_first: the caller already loaded this lane's word of the first
+    // window (gather_first_word), early, so its latency overlapped the unpack
     if (L1 <= L0) return;
     const uint64_t w0 = L0 >> 6, w1 = (L1 - 1) >> 6;  // inclusive word range
     // positions relative to the first window (a tile's range spans a few
@@ -228,7 +231,7 @@ __device__ __forceinline__ void emit_gather_t(void *out, const uint64_t *bm, uin
     uint32_t R = 0;  // values before the window
     for (uint32_t c0 = 0; c0 < nw; c0 += WAVE) {
         const uint32_t wi = c0 + (uint32_t)lane;
-        const uint64_t myw = bm[w0 + (wi < nw ? wi : nw - 1)];
+        const uint64_t myw = c0 == 0 && have_first ? first : bm[w0 + (wi < nw ? wi : nw - 1)];
         const uint32_t nwin = nw - c0 < (uint32_t)WAVE ? nw - c0 : (uint32_t)WAVE;
         for (uint32_t q = 0; q < nwin; q++) {
             const uint32_t p = (c0 + q) << 6;
@@ -245,6 +248,16 @@ __device__ __forceinline__ void emit_gather_t(void *out, const uint64_t *bm, uin
         }
     }
 }
+// this lane's bitmap word of the first 64-word window of [L0, L1), the load
+// emit_gather_t makes first
+__device__ __forceinline__ uint64_t gather_first_word(const uint64_t *bm, uint64_t L0, uint64_t L1,
+                                                      int lane) {
+    if (L1 <= L0) return 0;
+    const uint64_t w0 = L0 >> 6, w1 = (L1 - 1) >> 6;
+    const uint32_t nw = (uint32_t)(w1 - w0 + 1), wi = (uint32_t)lane;
+    return bm[w0 + (wi < nw ? wi : nw - 1)];
+}
+
 template <int F>
 __device__ __forceinline__ void emit_gather_f(void *out, const uint64_t *bm, uint64_t L0,
                                               uint64_t L1, const uint64_t *vv, uint32_t cnt,

@@ -411,6 +411,8 @@ __global__ void __launch_bounds__(64 * WPB) k_dec_fl(DecArgs a) {
     if (td.fl != OW) return;
     const uint32_t cnt = td.cnt, w = td.w;
     uint32_t *wd = lds[wv];
+    // streams with nulls: the first bitmap window's words, loaded with the stream
+    const uint64_t bw0 = td.nulls ? gather_first_word(td.bitmap, td.L0, td.L1, lane) : 0;
     const uint8_t *h = td.src + td.off;
     const uint64_t mn = qa_ld64u(h + 1);
     const uint32_t k0 = 16u * (uint32_t)lane;
@@ -538,13 +540,14 @@ __global__ void __launch_bounds__(64 * WPB) k_dec_fl(DecArgs a) {
         }
         return;
     }
-    if (OW == 4) emit_gather_t<QA_OUT_U32, uint32_t, 4>(td.out, td.bitmap, td.L0, td.L1, wd, cnt, lane);
+    if (OW == 4)
+        emit_gather_t<QA_OUT_U32, uint32_t, 4>(td.out, td.bitmap, td.L0, td.L1, wd, cnt, lane, true, bw0);
     else if (o32)
         emit_gather_t<QA_OUT_U32, uint64_t, 8>(td.out, td.bitmap, td.L0, td.L1, (const uint64_t *)wd, cnt,
-                                               lane);
+                                               lane, true, bw0);
     else
         emit_gather_t<QA_OUT_U64, uint64_t, 8>(td.out, td.bitmap, td.L0, td.L1, (const uint64_t *)wd, cnt,
-                                               lane);
+                                               lane, true, bw0);
 }
 
 void qa_dec_desc_launch(const DecArgs &a, hipStream_t s) {
