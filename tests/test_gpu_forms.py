@@ -215,3 +215,45 @@ def test_validity_shapes_in_one_batch(qa):
             assert vo == bm.tobytes(), f"column {i}: bitmap vs input"
     finally:
         gpu.close()
+
+
+def test_handles_in_parallel_threads(qa):
+    """Three handles used from three host threads at once, each with its own
+    streams, events and pinned staging: every stream equals the oracle's and
+    every decode round-trips, on repeated calls."""
+    import threading
+    cfg = qa.Config(level=qa.Level.FASTEST)
+    jobs = []
+    for k in range(3):
+        n = 150000 + 1000 * k
+        ic = _data.int_columns(n, seed=40 + k)
+        cols = [(ic["ts_i64"], None), _data.with_nulls(ic["u32_k11"], 0.02, seed=50 + k)]
+        want = [orc.compress(v, ODT[v.dtype], bm, orc.FASTEST) for v, bm in cols]
+        jobs.append((n, cols, want))
+    errors = []
+
+    def work(k):
+        n, cols, want = jobs[k]
+        try:
+            g = qa.Gpu(0)
+            try:
+                for rep in range(3):
+                    got = _gpu.device_compress(qa, g, cols, cfg)
+                    assert got == want, f"thread {k} call {rep}: streams"
+                    outs = _gpu.device_decompress(qa, g, got, [qa.FROM_NP[v.dtype] for v, _ in cols],
+                                                  [n, n], True)
+                    for (v, _), (o, _) in zip(cols, outs):
+                        assert o.tobytes() == v.tobytes(), f"thread {k} call {rep}: decode"
+            finally:
+                g.close()
+        except Exception as e:  # reported by the main thread
+            errors.append((k, repr(e)))
+
+    print("case three handles in three threads", flush=True)
+    ts = [threading.Thread(target=work, args=(k,)) for k in range(3)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=100)
+    assert not any(t.is_alive() for t in ts), "a thread did not finish"
+    assert not errors, errors
