@@ -1513,17 +1513,22 @@ void qa_encode_emit_launch(const EncArgs &a, hipStream_t s) {
 void qa_encode_launch(const EncArgs &a, hipStream_t s, int, hipStream_t side, hipEvent_t fork,
                       hipEvent_t join) {
     encode_counts_launch(a, s);
-    const bool forked = side && a.nvwords;
+    // a fork that cannot be set up runs the chain on the main stream; a join
+    // that cannot be enqueued waits for the side stream on the host
+    bool forked = side && a.nvwords && hipEventRecord(fork, s) == hipSuccess &&
+                  hipStreamWaitEvent(side, fork, 0) == hipSuccess;
+    bool joined = false;
     if (forked) {
-        hipEventRecord(fork, s);
-        hipStreamWaitEvent(side, fork, 0);
         encode_vruns_launch(a, side);
-        hipEventRecord(join, side);
+        joined = hipEventRecord(join, side) == hipSuccess;
     }
     encode_stats_launch(a, s);
     qa_encode_blocks_launch(a, s);
-    if (forked) hipStreamWaitEvent(s, join, 0);
-    else encode_vruns_launch(a, s);
+    if (forked) {
+        if (!joined || hipStreamWaitEvent(s, join, 0) != hipSuccess) hipStreamSynchronize(side);
+    } else {
+        encode_vruns_launch(a, s);
+    }
     encode_layout_rest_launch(a, s);
     qa_encode_emit_launch(a, s);
 }
