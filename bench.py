@@ -653,7 +653,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU in this build; --gpus {args.gpus} is not supported "
-              f"(blocks shard across GPUs by index range, see DESIGN.md)", file=sys.stderr)
+              f"(blocks are independent: they would shard across GPUs by block index range)", file=sys.stderr)
         sys.exit(2)
 
     import torch
