@@ -8,8 +8,9 @@ four streams, device-resident (inputs already in HBM). The first step is checked
 bit for bit (stream bytes against the CPU oracle on the first step's sample,
 round trip in full).
 
-Prints one JSON line (the driver's contract). --gpus > 1 exits: this build runs
-on one GPU.
+Prints one JSON line: the metric, its value and the configuration, the
+roofline of the dominant kernels and the CPU baseline. --gpus > 1 exits: the
+benchmark runs on one GPU.
 """
 import argparse
 import ctypes
