@@ -23,6 +23,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -680,7 +681,15 @@ static int fm_find_scale(const uint64_t *v, size_t n, int *idx, int64_t *ks) {
     }
     return 0;
 }
-static int fm_encode(const uint64_t *v, size_t n, buf_t *out) {
+/* the entropy tag layer is defined further down */
+uint8_t *orc_code_residuals(const uint8_t *r, size_t n, uint64_t lambda, int allow_lz,
+                            size_t *len);
+static uint64_t zigzag64(uint64_t x) { return (x << 1) ^ (uint64_t)((int64_t)x >> 63); }
+
+/* float_mult.rs:66-98: bit-pack form, and at the entropy levels the zig-zag
+ * k deltas through code_residuals when strictly smaller */
+static int fm_encode_ex(const uint64_t *v, size_t n, int entropy, uint64_t lambda, int allow_lz,
+                        buf_t *out) {
     if (n == 0) return 0;
     int64_t *ks = (int64_t *)malloc(sizeof(int64_t) * n);
     int idx;
@@ -688,12 +697,37 @@ static int fm_encode(const uint64_t *v, size_t n, buf_t *out) {
         free(ks);
         return 0;
     }
-    buf_push(out, 0); /* FM_BITPACK */
+    buf_t bp = {0};
+    for_encode((const uint64_t *)ks, n, 1, &bp);
+    uint8_t tag = 0; /* FM_BITPACK */
+    if (entropy) {
+        buf_t d = {0};
+        int64_t prev = 0;
+        for (size_t i = 0; i < n; i++) {
+            varint_write(&d, zigzag64((uint64_t)ks[i] - (uint64_t)prev));
+            prev = ks[i];
+        }
+        size_t cl = 0;
+        uint8_t *c = orc_code_residuals(d.p, d.len, lambda, allow_lz, &cl);
+        buf_free(&d);
+        if (cl < bp.len) {
+            tag = 1; /* FM_ENTROPY */
+            buf_free(&bp);
+            bp.p = c;
+            bp.len = cl;
+            bp.cap = cl;
+        } else {
+            free(c);
+        }
+    }
+    buf_push(out, tag);
     buf_push(out, (uint8_t)idx);
-    for_encode((const uint64_t *)ks, n, 1, out);
+    buf_append(out, bp.p, bp.len);
+    buf_free(&bp);
     free(ks);
     return 1;
 }
+static int fm_encode(const uint64_t *v, size_t n, buf_t *out) { return fm_encode_ex(v, n, 0, 0, 0, out); }
 
 /* ------------------------------------------------------------------------ */
 /* rANS (entropy/rans.rs:17-175)                                             */
@@ -1565,8 +1599,11 @@ static int rd_choose_cut(const uint64_t *v, size_t n, uint32_t *cut) {
     *cut = best_cut;
     return 1;
 }
-/* alp_rd.rs:132-206 with entropy = false */
-static int alprd_encode(const uint64_t *v, size_t n, buf_t *out) {
+/* alp_rd.rs:132-206: codes and rights bit-packed, or at the entropy levels
+ * through code_residuals (codes as bytes, rights as their bit-packed blob)
+ * when strictly smaller */
+static int alprd_encode_ex(const uint64_t *v, size_t n, int entropy, uint64_t lambda, int allow_lz,
+                           buf_t *out) {
     if (n == 0) return 0;
     uint32_t left_bw;
     if (!rd_choose_cut(v, n, &left_bw)) return 0;
@@ -1607,18 +1644,53 @@ static int alprd_encode(const uint64_t *v, size_t n, buf_t *out) {
         }
         buf_t cb = {0};
         for_encode(codes, n, 0, &cb);
-        buf_push(out, 0); /* T_BITPACK */
+        uint8_t ctag = 0; /* T_BITPACK */
+        if (entropy) {
+            uint8_t *cbytes = (uint8_t *)malloc(n ? n : 1);
+            for (size_t i = 0; i < n; i++) cbytes[i] = (uint8_t)codes[i];
+            size_t el = 0;
+            uint8_t *e = orc_code_residuals(cbytes, n, lambda, allow_lz, &el);
+            free(cbytes);
+            if (el < cb.len) {
+                ctag = 1; /* T_ENT */
+                buf_free(&cb);
+                cb.p = e;
+                cb.len = el;
+                cb.cap = el;
+            } else {
+                free(e);
+            }
+        }
+        buf_push(out, ctag);
         varint_write(out, cb.len);
         buf_append(out, cb.p, cb.len);
         buf_free(&cb);
-        buf_push(out, 0); /* T_BITPACK */
-        for_encode(rights, n, 0, out);
+        buf_t rb = {0};
+        for_encode(rights, n, 0, &rb);
+        uint8_t rtag = 0;
+        if (entropy) {
+            size_t el = 0;
+            uint8_t *e = orc_code_residuals(rb.p, rb.len, lambda, allow_lz, &el);
+            if (el < rb.len) {
+                rtag = 1;
+                buf_free(&rb);
+                rb.p = e;
+                rb.len = el;
+                rb.cap = el;
+            } else {
+                free(e);
+            }
+        }
+        buf_push(out, rtag);
+        buf_append(out, rb.p, rb.len);
+        buf_free(&rb);
     }
     free(codes);
     free(rights);
     free(exc_pos);
     return ok;
 }
+static int alprd_encode(const uint64_t *v, size_t n, buf_t *out) { return alprd_encode_ex(v, n, 0, 0, 0, out); }
 /* alp_rd.rs:208-285 */
 static int alprd_decode(const uint8_t *p, size_t len, size_t n, uint64_t *out) {
     size_t pos = 0;
@@ -1741,9 +1813,30 @@ static uint64_t hmix(uint64_t x) {
     x ^= x >> 33;
     return x;
 }
-/* dict.rs:87-152 with entropy = false. First-occurrence codes are remapped to
- * sorted order, so the final codes are the value ranks. */
-static int dict_encode(const uint64_t *v, size_t n, buf_t *out) {
+/* transpose.rs:10-19 with transform.rs:23-35: 8 byte planes, plane-major */
+static uint8_t *transpose_encode(const uint64_t *v, size_t n) {
+    uint8_t *soa = (uint8_t *)malloc(n ? 8 * n : 1);
+    for (int pl = 0; pl < 8; pl++)
+        for (size_t i = 0; i < n; i++) soa[(size_t)pl * n + i] = (uint8_t)(v[i] >> (8 * pl));
+    return soa;
+}
+/* transpose.rs:21-32 */
+static int transpose_decode(const uint8_t *soa, size_t len, size_t n, uint64_t *out) {
+    if (len != 8 * n) return ORC_CORRUPT;
+    for (size_t i = 0; i < n; i++) {
+        uint64_t x = 0;
+        for (int pl = 0; pl < 8; pl++) x |= (uint64_t)soa[(size_t)pl * n + i] << (8 * pl);
+        out[i] = x;
+    }
+    return ORC_OK;
+}
+
+/* dict.rs:87-152. First-occurrence codes are remapped to sorted order, so the
+ * final codes are the value ranks. Values: raw / delta-bitpacked / (entropy
+ * levels) transposed through code_residuals; codes: bit-packed or (entropy
+ * levels, cardinality <= 65536) 1-2 byte planes through code_residuals. */
+static int dict_encode_ex(const uint64_t *v, size_t n, int entropy, uint64_t lambda, int allow_lz,
+                          buf_t *out) {
     if (n == 0) return 0;
     size_t max_card = n / 2 + 1;
     hmap_t h;
@@ -1786,22 +1879,67 @@ static int dict_encode(const uint64_t *v, size_t n, buf_t *out) {
             remap[c] = (uint32_t)lo;
         }
         for (size_t i = 0; i < n; i++) codes[i] = remap[codes[i]];
-        /* encode_values (dict.rs:40-61), entropy = false */
+        /* encode_values (dict.rs:40-61) */
+        buf_t vb = {0};
+        uint8_t vtag = 0; /* VAL_RAW */
+        for (size_t c = 0; c < card; c++) buf_u64le(&vb, sorted[c]);
         buf_t delta = {0};
         delta_encode(sorted, card, &delta);
-        varint_write(out, card);
-        if (delta.len < card * 8) {
-            buf_push(out, 1); /* VAL_DELTA */
-            varint_write(out, delta.len);
-            buf_append(out, delta.p, delta.len);
+        if (delta.len < vb.len) {
+            vtag = 1; /* VAL_DELTA */
+            buf_free(&vb);
+            vb = delta;
         } else {
-            buf_push(out, 0); /* VAL_RAW */
-            varint_write(out, card * 8);
-            for (size_t c = 0; c < card; c++) buf_u64le(out, sorted[c]);
+            buf_free(&delta);
         }
-        buf_free(&delta);
-        buf_push(out, 0); /* CODES_BITPACK */
-        for_encode(codes, n, 0, out);
+        if (entropy) {
+            uint8_t *tr = transpose_encode(sorted, card);
+            size_t tl = 0;
+            uint8_t *t = orc_code_residuals(tr, 8 * card, lambda, allow_lz, &tl);
+            free(tr);
+            if (tl < vb.len) {
+                vtag = 2; /* VAL_TRANSPOSE */
+                buf_free(&vb);
+                vb.p = t;
+                vb.len = tl;
+                vb.cap = tl;
+            } else {
+                free(t);
+            }
+        }
+        varint_write(out, card);
+        buf_push(out, vtag);
+        varint_write(out, vb.len);
+        buf_append(out, vb.p, vb.len);
+        buf_free(&vb);
+        buf_t cbp = {0};
+        for_encode(codes, n, 0, &cbp);
+        uint8_t ctag = 0; /* CODES_BITPACK */
+        if (entropy && card <= 65536) {
+            size_t nbytes = card <= 256 ? 1 : 2;
+            buf_t blob = {0};
+            buf_push(&blob, (uint8_t)nbytes);
+            uint8_t *plane = (uint8_t *)malloc(n ? n : 1);
+            for (size_t pl = 0; pl < nbytes; pl++) {
+                for (size_t i = 0; i < n; i++) plane[i] = (uint8_t)(codes[i] >> (8 * pl));
+                size_t cl = 0;
+                uint8_t *cc = orc_code_residuals(plane, n, lambda, allow_lz, &cl);
+                varint_write(&blob, cl);
+                buf_append(&blob, cc, cl);
+                free(cc);
+            }
+            free(plane);
+            if (blob.len < cbp.len) {
+                ctag = 1; /* CODES_ENTROPY */
+                buf_free(&cbp);
+                cbp = blob;
+            } else {
+                buf_free(&blob);
+            }
+        }
+        buf_push(out, ctag);
+        buf_append(out, cbp.p, cbp.len);
+        buf_free(&cbp);
         free(sorted);
         free(remap);
     }
@@ -1812,6 +1950,7 @@ static int dict_encode(const uint64_t *v, size_t n, buf_t *out) {
     free(codes);
     return ok;
 }
+static int dict_encode(const uint64_t *v, size_t n, buf_t *out) { return dict_encode_ex(v, n, 0, 0, 0, out); }
 /* dict.rs:154-207 */
 static int dict_decode(const uint8_t *p, size_t len, size_t n, uint64_t *out) {
     size_t pos = 0;
@@ -1838,9 +1977,14 @@ static int dict_decode(const uint8_t *p, size_t len, size_t n, uint64_t *out) {
     } else if (vtag == 1) {
         rc = delta_decode(vb, vlen, card, dict);
         if (rc) goto done;
-    } else if (vtag == 2) {
-        rc = ORC_UNSUPPORTED; /* transpose->entropy needs the range coder path */
-        goto done;
+    } else if (vtag == 2) { /* dict.rs:74-81 */
+        size_t tl = 0;
+        rc = decode_residuals(vb, vlen, card * 8, &tmp, &tl);
+        if (rc) goto done;
+        rc = transpose_decode(tmp, tl, card, dict);
+        free(tmp);
+        tmp = NULL;
+        if (rc) goto done;
     } else {
         rc = ORC_CORRUPT;
         goto done;
@@ -2059,44 +2203,233 @@ static int looks_compressible(size_t res, size_t raw) {
     return sat_mul(res, 20) < sat_mul(raw, 19);
 }
 
+/* encoder.rs:24-73: which modes may compete for a family at a level */
+static int is_entropy_mode(int m) {
+    return m == M_PRED_RC || m == M_PRED2 || m == M_DELTA2 || m == M_DELTA_DP ||
+           m == M_ORDERED_DELTA || m == M_FLOAT_MULT || m == M_LZ || m == M_BYTE_TRANSPOSE;
+}
+static int is_predictor_mode(int m) {
+    return m == M_PRED || m == M_PRED_RC || m == M_PRED2 || m == M_DELTA2 || m == M_DELTA_DP ||
+           m == M_ORDERED_DELTA;
+}
+static int mode_runs(int m, int isf, int level) {
+    int fam_float = m == M_DELTA2 || m == M_DELTA_DP || m == M_FLOAT_MULT || m == M_ALP || m == M_ALP_RD;
+    if (fam_float && !isf) return 0;
+    if (is_predictor_mode(m)) return level >= ORC_HIGH;
+    return level >= ORC_BALANCED || !is_entropy_mode(m);
+}
+/* encoder.rs:564-578 */
+static int mode_cascades_lz(int m) {
+    return m == M_PRED_RC || m == M_PRED2 || m == M_DELTA2 || m == M_DELTA_DP ||
+           m == M_ORDERED_DELTA || m == M_FLOAT_MULT || m == M_LZ || m == M_BYTE_TRANSPOSE ||
+           m == M_DICT;
+}
+
+/* encoder.rs:99-140: smallest score wins on strict <; the runner-up mode is
+ * kept for the LZ cascade */
 typedef struct {
     int mode;
     buf_t payload;
     size_t score;
     uint64_t lambda;
     size_t decoded_bytes;
+    int runner;
+    size_t runner_score;
 } best_t;
 
 static void best_consider(best_t *b, int mode, buf_t *p) {
     size_t score = p->len + penalty(mode, b->lambda, b->decoded_bytes);
     if (score < b->score) {
+        b->runner = b->mode;
+        b->runner_score = b->score;
         buf_free(&b->payload);
         b->payload = *p;
         b->mode = mode;
         b->score = score;
     } else {
+        if (score < b->runner_score && mode != b->mode) {
+            b->runner = mode;
+            b->runner_score = score;
+        }
         buf_free(p);
     }
     memset(p, 0, sizeof *p);
 }
+static void buf_take(buf_t *b, uint8_t *p, size_t len) {
+    b->p = p;
+    b->len = len;
+    b->cap = len;
+    b->oom = 0;
+}
 
-static void encode_block(const uint64_t *b, size_t n, int dt, int level, buf_t *out) {
+/* entropy/mod.rs:90-125: order-1 size estimate, glibc log2 summed in the
+ * reference's order */
+size_t orc_estimate_order1_bytes(const uint8_t *b, size_t n) {
+    if (n < 64) return n;
+    uint32_t *joint = (uint32_t *)calloc(65536, 4);
+    uint32_t marg[256];
+    memset(marg, 0, sizeof marg);
+    size_t prev = 0;
+    for (size_t i = 0; i < n; i++) {
+        joint[prev * 256 + b[i]]++;
+        marg[prev]++;
+        prev = b[i];
+    }
+    double bits = 0.0;
+    for (int pv = 0; pv < 256; pv++) {
+        uint32_t m = marg[pv];
+        if (m == 0) continue;
+        double mf = (double)m;
+        for (int c = 0; c < 256; c++) {
+            uint32_t cnt = joint[pv * 256 + c];
+            if (cnt > 0) bits += (double)cnt * log2(mf / (double)cnt);
+        }
+    }
+    free(joint);
+    return (size_t)ceil(bits / 8.0);
+}
+/* encoder.rs:263-273 */
+static int coded_if_competitive(const uint8_t *res, size_t n, uint64_t lambda, int allow_lz,
+                                size_t best_score, buf_t *out) {
+    if (lambda == 0 && sat_mul(orc_estimate_order1_bytes(res, n), 9) / 10 >= best_score) return 0;
+    size_t cl = 0;
+    uint8_t *c = orc_code_residuals(res, n, lambda, allow_lz, &cl);
+    buf_take(out, c, cl);
+    return 1;
+}
+static void code_res_buf(const uint8_t *res, size_t n, uint64_t lambda, int allow_lz, buf_t *out) {
+    size_t cl = 0;
+    uint8_t *c = orc_code_residuals(res, n, lambda, allow_lz, &cl);
+    buf_take(out, c, cl);
+}
+/* lz.rs:156-162 over the block's little-endian bytes */
+static uint8_t *lz_encode_block(const uint64_t *b, size_t n, size_t *len) {
+    uint8_t *bytes = (uint8_t *)malloc(n ? 8 * n : 1);
+    for (size_t i = 0; i < n; i++)
+        for (int k = 0; k < 8; k++) bytes[8 * i + k] = (uint8_t)(b[i] >> (8 * k));
+    uint8_t *z = orc_lz_compress(bytes, 8 * n, len);
+    free(bytes);
+    return z;
+}
+/* pcodec.rs:33-49: the pco number type of each dtype; the codec itself is the
+ * host restatement shared with the engine (qa_pco.cpp) */
+int qa_pco_compress(int ntype, unsigned level, const uint64_t *lane, size_t n, uint8_t *dst,
+                    size_t cap, size_t *out_len);
+int qa_pco_decompress(int ntype, const uint8_t *src, size_t len, size_t n, uint64_t *lane);
+static int pco_ntype(int dt) {
+    switch (dt) {
+    case ORC_F64: return 6;
+    case ORC_F32: return 5;
+    case ORC_I64: return 4;
+    case ORC_U64: return 2;
+    case ORC_I32: return 3;
+    default: return 1;
+    }
+}
+static int pco_encode(const uint64_t *b, size_t n, int dt, unsigned clevel, buf_t *out) {
+    if (n == 0) return 0;
+    size_t cap = 16 * n + 4096, ol = 0;
+    uint8_t *d = (uint8_t *)malloc(cap);
+    int rc = qa_pco_compress(pco_ntype(dt), clevel, b, n, d, cap, &ol);
+    if (rc != 0) {
+        free(d);
+        return 0;
+    }
+    buf_take(out, d, ol);
+    return 1;
+}
+static int lin_res(int kind, const uint64_t *b, size_t n, int order, buf_t *out) {
+    size_t l = 0;
+    uint8_t *r = orc_linear_encode(kind, b, n, order, &l);
+    if (!r) return 0;
+    buf_take(out, r, l);
+    return 1;
+}
+static void pred_res(int kind, const uint64_t *b, size_t n, int plog2, buf_t *out) {
+    size_t l = 0;
+    uint8_t *r = orc_pred_encode(kind, b, n, plog2, &l);
+    buf_take(out, r, l);
+}
+
+/* encoder.rs:543-581: one mode's payload at a level, with the level's LZ
+ * cascade inside code_residuals */
+static int encode_mode_lv(int mode, const uint64_t *b, size_t n, int plog2, int dt, int level,
+                          buf_t *out) {
+    uint64_t lambda = level_lambda(level);
+    int allow_lz = level == ORC_MAX, entropy = level >= ORC_BALANCED;
+    buf_t r = {0};
+    int ok = 1;
+    switch (mode) {
+    case M_RAW: raw_encode(b, n, dt, out); break;
+    case M_CONST: ok = const_encode(b, n, out); break;
+    case M_STRIDE: ok = stride_encode(b, n, out); break;
+    case M_XORZ: xorz_encode(b, n, out); break;
+    case M_PRED: pred_res(0, b, n, plog2, out); break;
+    case M_PRED_RC: pred_res(0, b, n, plog2, &r); code_res_buf(r.p, r.len, lambda, allow_lz, out); break;
+    case M_PRED2: pred_res(25, b, n, plog2, &r); code_res_buf(r.p, r.len, lambda, allow_lz, out); break;
+    case M_DELTA2:
+        lin_res(20, b, n, orc_linear_select_order(b, n), &r);
+        code_res_buf(r.p, r.len, lambda, allow_lz, out);
+        break;
+    case M_DELTA_DP:
+        ok = lin_res(45, b, n, orc_linear_select_order(b, n), &r);
+        if (ok) code_res_buf(r.p, r.len, lambda, allow_lz, out);
+        break;
+    case M_ORDERED_DELTA: lin_res(7, b, n, 0, &r); code_res_buf(r.p, r.len, lambda, allow_lz, out); break;
+    case M_FLOAT_MULT: ok = fm_encode_ex(b, n, entropy, lambda, allow_lz, out); break;
+    case M_LZ: {
+        size_t zl = 0;
+        uint8_t *z = lz_encode_block(b, n, &zl);
+        code_res_buf(z, zl, lambda, allow_lz, out);
+        free(z);
+        break;
+    }
+    case M_BYTE_TRANSPOSE: {
+        uint8_t *t = transpose_encode(b, n);
+        code_res_buf(t, 8 * n, lambda, allow_lz, out);
+        free(t);
+        break;
+    }
+    case M_FOR_BITPACK: for_encode(b, n, dtype_signed(dt), out); break;
+    case M_ALP: ok = alp_encode(b, n, out); break;
+    case M_ALP_RD: ok = alprd_encode_ex(b, n, entropy, lambda, allow_lz, out); break;
+    case M_DICT: ok = dict_encode_ex(b, n, entropy, lambda, allow_lz, out); break;
+    case M_RLE: ok = rle_encode(b, n, out); break;
+    case M_DELTA_BITPACK: delta_encode(b, n, out); break;
+    case M_PCO: ok = level >= ORC_BALANCED ? pco_encode(b, n, dt, level == ORC_MAX ? 12 : 8, out)
+                                           : pco_encode(b, n, dt, 8, out);
+        break;
+    default: ok = 0;
+    }
+    buf_free(&r);
+    if (!ok) buf_free(out);
+    return ok;
+}
+
+static void emit_frame(buf_t *out, int mode, size_t n, const buf_t *p) {
+    __atomic_fetch_add(&g_wins[mode & 63], 1, __ATOMIC_RELAXED);
+    buf_push(out, (uint8_t)mode);
+    buf_u32le(out, (uint32_t)n);
+    buf_u32le(out, (uint32_t)p->len);
+    buf_append(out, p->p, p->len);
+}
+
+/* encoder.rs:275-469 */
+static void encode_block_full(const uint64_t *b, size_t n, int plog2, int dt, int level, buf_t *out) {
     int isf = dtype_is_float(dt);
-    int reduced = level == ORC_FASTEST;
+    int entropy = level >= ORC_BALANCED, predictors = level >= ORC_HIGH;
+    int reduced = level == ORC_FASTEST, cascade = level == ORC_MAX, allow_lz = 0;
     uint64_t lambda = level_lambda(level);
     size_t decoded = n * dtype_lane_bytes(dt), raw_bytes = n * 8;
     feats_t f = probe(b, n);
     best_t best;
     memset(&best, 0, sizeof best);
+    best.runner = -1;
+    best.runner_score = SIZE_MAX;
     buf_t p = {0};
-    int mode = M_RAW;
-    if (!isf && looks_incompressible(b, n, f, dtype_lane_bytes(dt))) {
-        raw_encode(b, n, dt, &best.payload);
-        best.mode = M_RAW;
-        goto emit;
-    }
     raw_encode(b, n, dt, &best.payload);
     best.mode = M_RAW;
+    if (!isf && looks_incompressible(b, n, f, dtype_lane_bytes(dt))) goto emit;
     best.lambda = lambda;
     best.decoded_bytes = decoded;
     best.score = best.payload.len + penalty(M_RAW, lambda, decoded);
@@ -2111,23 +2444,142 @@ static void encode_block(const uint64_t *b, size_t n, int dt, int level, buf_t *
     delta_encode(b, n, &p);
     best_consider(&best, M_DELTA_BITPACK, &p);
     if (isf && !reduced) {
-        if (fm_encode(b, n, &p)) best_consider(&best, M_FLOAT_MULT, &p); else buf_free(&p);
+        if (fm_encode_ex(b, n, entropy, lambda, allow_lz, &p)) best_consider(&best, M_FLOAT_MULT, &p); else buf_free(&p);
         if (alp_encode(b, n, &p)) best_consider(&best, M_ALP, &p); else buf_free(&p);
-        if (alprd_encode(b, n, &p)) best_consider(&best, M_ALP_RD, &p); else buf_free(&p);
+        if (alprd_encode_ex(b, n, entropy, lambda, allow_lz, &p)) best_consider(&best, M_ALP_RD, &p); else buf_free(&p);
     }
-    if (looks_compressible(best.payload.len, raw_bytes) && !reduced) {
-        if (dict_encode(b, n, &p)) best_consider(&best, M_DICT, &p); else buf_free(&p);
-        if (rle_encode(b, n, &p)) best_consider(&best, M_RLE, &p); else buf_free(&p);
+    if (predictors) {
+        buf_t res = {0};
+        pred_res(0, b, n, plog2, &res);
+        if (looks_compressible(res.len, raw_bytes) &&
+            coded_if_competitive(res.p, res.len, lambda, allow_lz, best.score, &p))
+            best_consider(&best, M_PRED_RC, &p);
+        best_consider(&best, M_PRED, &res);
+        pred_res(25, b, n, plog2, &res);
+        if (looks_compressible(res.len, raw_bytes) &&
+            coded_if_competitive(res.p, res.len, lambda, allow_lz, best.score, &p))
+            best_consider(&best, M_PRED2, &p);
+        buf_free(&res);
+    }
+    if (isf && predictors) {
+        int order = orc_linear_select_order(b, n);
+        buf_t res = {0};
+        lin_res(20, b, n, order, &res);
+        if (looks_compressible(res.len, raw_bytes)) {
+            if (coded_if_competitive(res.p, res.len, lambda, allow_lz, best.score, &p))
+                best_consider(&best, M_DELTA2, &p);
+            buf_t dp = {0};
+            if (lin_res(45, b, n, order, &dp) &&
+                coded_if_competitive(dp.p, dp.len, lambda, allow_lz, best.score, &p))
+                best_consider(&best, M_DELTA_DP, &p);
+            buf_free(&dp);
+        }
+        buf_free(&res);
+    }
+    if (predictors) {
+        buf_t res = {0};
+        lin_res(7, b, n, 0, &res);
+        if (looks_compressible(res.len, raw_bytes) &&
+            coded_if_competitive(res.p, res.len, lambda, allow_lz, best.score, &p))
+            best_consider(&best, M_ORDERED_DELTA, &p);
+        buf_free(&res);
+    }
+    {
+        int block_compressible = looks_compressible(best.payload.len, raw_bytes);
+        int lz_worth = lambda == 0 || f.distinct_low || f.repeats;
+        if (entropy && block_compressible && lz_worth) {
+            size_t zl = 0;
+            uint8_t *z = lz_encode_block(b, n, &zl);
+            code_res_buf(z, zl, lambda, allow_lz, &p);
+            free(z);
+            best_consider(&best, M_LZ, &p);
+        }
+        if (entropy && block_compressible && (f.exp_range <= 64 || f.repeats)) {
+            uint8_t *t = transpose_encode(b, n);
+            if (coded_if_competitive(t, 8 * n, lambda, allow_lz, best.score, &p))
+                best_consider(&best, M_BYTE_TRANSPOSE, &p);
+            free(t);
+        }
+        if (block_compressible && !reduced) {
+            if (dict_encode_ex(b, n, entropy, lambda, allow_lz, &p)) best_consider(&best, M_DICT, &p); else buf_free(&p);
+            if (rle_encode(b, n, &p)) best_consider(&best, M_RLE, &p); else buf_free(&p);
+        }
+    }
+    if (level >= ORC_BALANCED) {
+        if (pco_encode(b, n, dt, level == ORC_MAX ? 12 : 8, &p)) best_consider(&best, M_PCO, &p);
+    }
+    if (cascade) {
+        int ms[2] = {best.mode, best.runner};
+        for (int k = 0; k < 2; k++) {
+            int m = ms[k];
+            if (m < 0 || !mode_cascades_lz(m)) continue;
+            if (encode_mode_lv(m, b, n, plog2, dt, level, &p)) best_consider(&best, m, &p);
+        }
     }
 emit:
-    mode = best.mode;
-    g_wins[mode & 63]++;
-    buf_push(out, (uint8_t)mode);
-    buf_u32le(out, (uint32_t)n);
-    buf_u32le(out, (uint32_t)best.payload.len);
-    buf_append(out, best.payload.p, best.payload.len);
+    emit_frame(out, best.mode, n, &best.payload);
     buf_free(&best.payload);
 }
+
+/* encoder.rs:483-641: Selection::Sample */
+static const int SAMPLE_MODES[15] = {M_XORZ, M_PRED, M_PRED_RC, M_PRED2, M_DELTA2, M_DELTA_DP,
+                                     M_ORDERED_DELTA, M_FLOAT_MULT, M_BYTE_TRANSPOSE, M_FOR_BITPACK,
+                                     M_DELTA_BITPACK, M_ALP, M_ALP_RD, M_DICT, M_RLE};
+static void encode_block_sampled(const uint64_t *b, size_t n, int plog2, int dt, int level, buf_t *out) {
+    int isf = dtype_is_float(dt);
+    size_t decoded = n * dtype_lane_bytes(dt);
+    feats_t f = probe(b, n);
+    best_t best;
+    memset(&best, 0, sizeof best);
+    best.runner = -1;
+    best.runner_score = SIZE_MAX;
+    best.lambda = level_lambda(level);
+    best.decoded_bytes = decoded;
+    raw_encode(b, n, dt, &best.payload);
+    best.mode = M_RAW;
+    best.score = best.payload.len + penalty(M_RAW, best.lambda, decoded);
+    buf_t p = {0};
+    if (encode_mode_lv(M_CONST, b, n, plog2, dt, level, &p)) best_consider(&best, M_CONST, &p);
+    if (encode_mode_lv(M_STRIDE, b, n, plog2, dt, level, &p)) best_consider(&best, M_STRIDE, &p);
+    if (mode_runs(M_LZ, isf, level) && (f.distinct_low || f.repeats) &&
+        encode_mode_lv(M_LZ, b, n, plog2, dt, level, &p))
+        best_consider(&best, M_LZ, &p);
+    if (level >= ORC_BALANCED && encode_mode_lv(M_PCO, b, n, plog2, dt, level, &p))
+        best_consider(&best, M_PCO, &p);
+    /* build_sample: 8 runs of 64 spread over the block */
+    size_t total = 8 * 64, sn;
+    uint64_t *smp;
+    if (n <= total) {
+        sn = n;
+        smp = (uint64_t *)malloc(8 * (n ? n : 1));
+        memcpy(smp, b, 8 * n);
+    } else {
+        sn = total;
+        smp = (uint64_t *)malloc(8 * total);
+        for (size_t r = 0; r < 8; r++) {
+            size_t start = r * (n - 64) / 7;
+            memcpy(smp + 64 * r, b + start, 8 * 64);
+        }
+    }
+    int win = -1;
+    size_t win_est = SIZE_MAX;
+    for (int k = 0; k < 15; k++) {
+        int m = SAMPLE_MODES[k];
+        if (!mode_runs(m, isf, level)) continue;
+        if (encode_mode_lv(m, smp, sn, 10, dt, level, &p)) {
+            if (p.len < win_est) {
+                win_est = p.len;
+                win = m;
+            }
+            buf_free(&p);
+        }
+    }
+    free(smp);
+    if (win >= 0 && encode_mode_lv(win, b, n, plog2, dt, level, &p)) best_consider(&best, win, &p);
+    emit_frame(out, best.mode, n, &best.payload);
+    buf_free(&best.payload);
+}
+
 
 /* validity.rs:40-70 */
 static void validity_encode(const uint8_t *bm, size_t n, buf_t *out) {
@@ -2227,11 +2679,40 @@ static uint64_t to_lane(int dt, const uint8_t *p, size_t i) {
     }
 }
 
-/* lib.rs:429-455 + encoder.rs:184-214 */
-int orc_compress_column(int dtype, size_t n, const void *values, const uint8_t *validity,
-                        int level, size_t block_size, uint8_t **outp, size_t *out_len) {
+/* the blocks of a column encoded by a pool of threads (the rayon fan-out of
+ * encoder.rs:216-234); each block's frame goes to its own buffer */
+typedef struct {
+    const uint64_t *lane;
+    const uint64_t *ranges;
+    size_t nb;
+    int plog2, dt, level, sample;
+    buf_t *frames;
+    size_t next;
+    pthread_mutex_t mu;
+} blockjob_t;
+static void *block_worker(void *arg) {
+    blockjob_t *j = (blockjob_t *)arg;
+    for (;;) {
+        pthread_mutex_lock(&j->mu);
+        size_t b = j->next++;
+        pthread_mutex_unlock(&j->mu);
+        if (b >= j->nb) break;
+        const uint64_t *v = j->lane + j->ranges[2 * b];
+        size_t n = j->ranges[2 * b + 1] - j->ranges[2 * b];
+        if (j->sample) encode_block_sampled(v, n, j->plog2, j->dt, j->level, &j->frames[b]);
+        else encode_block_full(v, n, j->plog2, j->dt, j->level, &j->frames[b]);
+    }
+    return NULL;
+}
+
+/* lib.rs:429-455 + encoder.rs:184-214. selection: 0 Full, 1 Sample;
+ * predictor_log2 is clamped to [10, 16] (lib.rs:237-239); threads 0 = 1. */
+int orc_compress_column_cfg(int dtype, size_t n, const void *values, const uint8_t *validity,
+                            int level, size_t block_size, int selection, int predictor_log2,
+                            int threads, uint8_t **outp, size_t *out_len) {
     if (dtype < 0 || dtype > 5) return ORC_BAD_DTYPE;
-    if (level > ORC_FAST) return ORC_UNSUPPORTED;
+    if (level < ORC_FASTEST || level > ORC_MAX) return ORC_UNSUPPORTED;
+    int plog2 = predictor_log2 < 10 ? 10 : predictor_log2 > 16 ? 16 : predictor_log2;
     const uint8_t *src = (const uint8_t *)values;
     int has_nulls = validity && count_valid(validity, n) < n;
     size_t nv = has_nulls ? count_valid(validity, n) : n;
@@ -2243,7 +2724,7 @@ int orc_compress_column(int dtype, size_t n, const void *values, const uint8_t *
     buf_append(&out, "FCR1", 4);
     buf_push(&out, 2);
     buf_push(&out, has_nulls ? 1 : 0);
-    buf_push(&out, 16);
+    buf_push(&out, (uint8_t)plog2);
     buf_push(&out, (uint8_t)dtype);
     buf_u64le(&out, n);
     if (has_nulls) {
@@ -2255,8 +2736,34 @@ int orc_compress_column(int dtype, size_t n, const void *values, const uint8_t *
     }
     uint64_t *ranges = (uint64_t *)malloc(16 * (nv + 1));
     size_t nb = orc_plan_blocks(lane, nv, block_size, ranges);
-    for (size_t b = 0; b < nb; b++)
-        encode_block(lane + ranges[2 * b], ranges[2 * b + 1] - ranges[2 * b], dtype, level, &out);
+    blockjob_t job;
+    job.lane = lane;
+    job.ranges = ranges;
+    job.nb = nb;
+    job.plog2 = plog2;
+    job.dt = dtype;
+    job.level = level;
+    job.sample = selection == 1;
+    job.frames = (buf_t *)calloc(nb ? nb : 1, sizeof(buf_t));
+    job.next = 0;
+    pthread_mutex_init(&job.mu, NULL);
+    int nt = threads < 1 ? 1 : threads;
+    if ((size_t)nt > nb) nt = nb ? (int)nb : 1;
+    if (nt <= 1) {
+        block_worker(&job);
+    } else {
+        pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nt);
+        for (int t = 0; t < nt; t++) pthread_create(&th[t], NULL, block_worker, &job);
+        for (int t = 0; t < nt; t++) pthread_join(th[t], NULL);
+        free(th);
+    }
+    pthread_mutex_destroy(&job.mu);
+    for (size_t b = 0; b < nb; b++) {
+        buf_append(&out, job.frames[b].p, job.frames[b].len);
+        if (job.frames[b].oom) out.oom = 1;
+        buf_free(&job.frames[b]);
+    }
+    free(job.frames);
     free(ranges);
     free(lane);
     if (out.oom) {
@@ -2267,8 +2774,33 @@ int orc_compress_column(int dtype, size_t n, const void *values, const uint8_t *
     *out_len = out.len;
     return ORC_OK;
 }
+int orc_compress_column(int dtype, size_t n, const void *values, const uint8_t *validity,
+                        int level, size_t block_size, uint8_t **outp, size_t *out_len) {
+    return orc_compress_column_cfg(dtype, n, values, validity, level, block_size, 0, 16, 1, outp,
+                                   out_len);
+}
 
-static int decode_frame(int mode, const uint8_t *p, size_t len, size_t n, int dt, uint64_t *out) {
+/* decoder.rs:18-20 */
+static size_t resid_bound(size_t n) { return n > (SIZE_MAX - 16) / 10 ? SIZE_MAX : n * 10 + 16; }
+/* a residual stream through decode_residuals, then a recurrence into out */
+static int resid_then(int mode, const uint8_t *p, size_t len, size_t n, int plog2, uint64_t *out) {
+    uint8_t *r = NULL;
+    size_t rl = 0;
+    int rc = decode_residuals(p, len, resid_bound(n), &r, &rl);
+    if (rc) return rc;
+    switch (mode) {
+    case M_PRED_RC: rc = orc_pred_decode(0, r, rl, n, plog2, out); break;
+    case M_PRED2: rc = orc_pred_decode(25, r, rl, n, plog2, out); break;
+    case M_DELTA2: rc = orc_linear_decode(20, r, rl, n, out); break;
+    case M_DELTA_DP: rc = orc_linear_decode(45, r, rl, n, out); break;
+    default: rc = orc_linear_decode(7, r, rl, n, out); break;
+    }
+    free(r);
+    return rc;
+}
+
+static int decode_frame(int mode, const uint8_t *p, size_t len, size_t n, int dt, int plog2,
+                        uint64_t *out) {
     switch (mode) {
     case M_RAW: return raw_decode(p, len, n, dt, out);
     case M_CONST:
@@ -2289,9 +2821,33 @@ static int decode_frame(int mode, const uint8_t *p, size_t len, size_t n, int dt
     case M_ALP_RD: return alprd_decode(p, len, n, out);
     case M_DICT: return dict_decode(p, len, n, out);
     case M_RLE: return rle_decode(p, len, n, out);
-    case M_PRED: case M_LZ: case M_ORDERED_DELTA: case M_BYTE_TRANSPOSE: case M_DELTA2:
-    case M_PRED2: case M_PRED_RC: case M_DELTA_DP: case M_PCO:
-        return ORC_UNSUPPORTED;
+    case M_PRED: return orc_pred_decode(0, p, len, n, plog2, out);
+    case M_PRED_RC: case M_PRED2: case M_DELTA2: case M_DELTA_DP: case M_ORDERED_DELTA:
+        return resid_then(mode, p, len, n, plog2, out);
+    case M_LZ: { /* decoder.rs:170-173, lz.rs:164-171 */
+        uint8_t *z = NULL, *bytes = NULL;
+        size_t zl = 0;
+        size_t bound = n > (SIZE_MAX - 1024) / 16 ? SIZE_MAX : n * 16 + 1024;
+        int rc = decode_residuals(p, len, bound, &z, &zl);
+        if (rc) return rc;
+        rc = orc_lz_decompress(z, zl, 8 * n, &bytes);
+        free(z);
+        if (rc) return rc;
+        for (size_t i = 0; i < n; i++) out[i] = rd_u64le(bytes + 8 * i);
+        free(bytes);
+        return ORC_OK;
+    }
+    case M_BYTE_TRANSPOSE: { /* decoder.rs:174-177 */
+        uint8_t *t = NULL;
+        size_t tl = 0;
+        int rc = decode_residuals(p, len, 8 * n + 16, &t, &tl);
+        if (rc) return rc;
+        rc = transpose_decode(t, tl, n, out);
+        free(t);
+        return rc;
+    }
+    case M_PCO: /* pcodec.rs:66-76 */
+        return qa_pco_decompress(pco_ntype(dt), p, len, n, out) == 0 ? ORC_OK : ORC_CORRUPT;
     default: return ORC_UNKNOWN_MODE;
     }
 }
@@ -2370,7 +2926,7 @@ int orc_decompress_column(const uint8_t *src, size_t len, int *dtype, size_t *no
         int mode = src[pos];
         size_t fn = rd_u32le(src + pos + 1), pl = rd_u32le(src + pos + 5);
         pos += 9;
-        rc = decode_frame(mode, src + pos, pl, fn, dt, lane + counted);
+        rc = decode_frame(mode, src + pos, pl, fn, dt, src[6], lane + counted);
         pos += pl;
         counted += fn;
     }
@@ -2431,8 +2987,24 @@ uint8_t *orc_encode_mode(int mode, const uint64_t *v, size_t n, int dt, size_t *
     if (!p.p) p.p = (uint8_t *)malloc(1);
     return p.p;
 }
+/* one mode's payload at a level (encoder.rs:543-581); NULL when it bails */
+uint8_t *orc_encode_mode_lv(int mode, const uint64_t *v, size_t n, int plog2, int dt, int level,
+                            size_t *len) {
+    buf_t p = {0};
+    if (!encode_mode_lv(mode, v, n, plog2, dt, level, &p)) {
+        *len = 0;
+        return NULL;
+    }
+    *len = p.len;
+    if (!p.p) p.p = (uint8_t *)malloc(1);
+    return p.p;
+}
+int orc_decode_mode_lv(int mode, const uint8_t *payload, size_t len, size_t n, int dt, int plog2,
+                       uint64_t *out) {
+    return decode_frame(mode, payload, len, n, dt, plog2, out);
+}
 int orc_decode_mode(int mode, const uint8_t *payload, size_t len, size_t n, int dt, uint64_t *out) {
-    return decode_frame(mode, payload, len, n, dt, out);
+    return decode_frame(mode, payload, len, n, dt, 16, out);
 }
 
 /* ------------------------------------------------------------------------ */

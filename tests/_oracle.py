@@ -35,6 +35,17 @@ def _load():
                                           ctypes.POINTER(ctypes.c_size_t),
                                           ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(u8p)]
     lib.orc_free.argtypes = [ctypes.c_void_p]
+    lib.orc_compress_column_cfg.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_encode_mode_lv.restype = u8p
+    lib.orc_encode_mode_lv.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
+    lib.orc_decode_mode_lv.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    lib.orc_estimate_order1_bytes.restype = ctypes.c_size_t
+    lib.orc_estimate_order1_bytes.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
     lib.orc_encode_mode.restype = u8p
     lib.orc_encode_mode.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_size_t)]
@@ -99,14 +110,17 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
 
-def compress(values, dtype, validity=None, level=FAST, block_size=0):
+def compress(values, dtype, validity=None, level=FAST, block_size=0, selection=0,
+             predictor_log2=16, threads=1):
+    """selection 0 = Full, 1 = Sample; threads = host threads over blocks."""
     values = np.ascontiguousarray(values, dtype=NP_DTYPE[dtype])
     if validity is not None:
         validity = np.ascontiguousarray(validity, dtype=np.uint8)
     out = ctypes.POINTER(ctypes.c_uint8)()
     n = ctypes.c_size_t()
-    rc = lib.orc_compress_column(dtype, len(values), _ptr(values), _ptr(validity), level,
-                                 block_size, ctypes.byref(out), ctypes.byref(n))
+    rc = lib.orc_compress_column_cfg(dtype, len(values), _ptr(values), _ptr(validity), level,
+                                     block_size, selection, predictor_log2, threads,
+                                     ctypes.byref(out), ctypes.byref(n))
     if rc != 0:
         raise RuntimeError(f"oracle compress rc={rc}")
     b = ctypes.string_at(out, n.value)
