@@ -31,13 +31,13 @@ def varint_len(v):
 # ---------------------------------------------------------------------------
 # known-answer streams (format.rs, encoder.rs; hand-derived)
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("level", [orc.FASTEST, orc.FAST])
+@pytest.mark.parametrize("level", [orc.FASTEST, orc.FAST, orc.BALANCED, orc.HIGH, orc.MAX])
 def test_empty_column_is_header_only(level):
     s = orc.compress(np.zeros(0), orc.F64, level=level)
     assert s == bytes.fromhex("46435231020010000000000000000000")
 
 
-@pytest.mark.parametrize("level", [orc.FASTEST, orc.FAST])
+@pytest.mark.parametrize("level", [orc.FASTEST, orc.FAST, orc.BALANCED, orc.HIGH, orc.MAX])
 def test_constant_column_frames(level):
     s = orc.compress(np.full(200000, 42.0), orc.F64, level=level)
     want = bytes.fromhex("4643523102001000") + struct.pack("<Q", 200000)
@@ -61,7 +61,7 @@ def test_ramp_below_16_prefers_xorz_at_fast():
     assert orc.frames(s)[0][0] in (3, 50, 51)
 
 
-@pytest.mark.parametrize("level", [orc.FASTEST, orc.FAST])
+@pytest.mark.parametrize("level", [orc.FASTEST, orc.FAST, orc.BALANCED, orc.HIGH, orc.MAX])
 def test_single_value_is_raw(level):
     s = orc.compress(np.array([2.5]), orc.F64, level=level)
     assert s == bytes.fromhex("46435231020010000100000000000000") + bytes([5]) + \
@@ -1205,3 +1205,255 @@ def test_decimal_decode_errors():
     for b in bad:
         with pytest.raises(ValueError):
             orc.decompress_decimal(b)
+
+
+# ---------------------------------------------------------------------------
+# Balanced / High / Max: the full competition (encoder.rs:275-469)
+# ---------------------------------------------------------------------------
+ALL_LEVELS = [orc.FASTEST, orc.FAST, orc.BALANCED, orc.HIGH, orc.MAX]
+
+
+def frame_modes(s):
+    return [m for m, _, _ in orc.frames(s)]
+
+
+@pytest.mark.parametrize("level", ALL_LEVELS)
+def test_integer_ramp_is_stride_every_level(level):
+    # pco can never reach 16 B (magic, versions, chunk metadata, termination),
+    # and STRIDE is considered before every mode that ties it
+    for n in (16, 1000, 40000):
+        s = orc.compress(np.arange(n, dtype=np.int64) * 7 + 3, orc.I64, level=level)
+        fr = orc.frames(s)
+        assert (fr[0][0], fr[0][2]) == (2, struct.pack("<qq", 3, 7))
+
+
+def test_ties_under_the_penalty_go_to_the_earlier_mode():
+    # encoder.rs:99-140 with the decode penalty (lambda * weight * bytes) >> 8.
+    # A ramp of 15 i64: STRIDE = 16 B (weight 0); XORZ = 15 B, 1 B per value.
+    # Fast (lambda 4): XORZ scores 15 + (4*1*120 >> 8) = 16, a tie, and STRIDE
+    # (considered first) keeps it under strict <. At 14 values XORZ scores 15
+    # and wins. Balanced (lambda 2): 15 + (2*120 >> 8) = 15 < 16, XORZ. At
+    # Max (lambda 0) a ramp of 16 ties 16 = 16 and STRIDE keeps it.
+    mode = lambda n, lv: frame_modes(orc.compress(np.arange(n, dtype=np.int64), orc.I64, level=lv))[0]
+    assert mode(15, orc.FAST) == orc.MODE["STRIDE"]
+    assert mode(14, orc.FAST) == orc.MODE["XORZ"]
+    assert mode(15, orc.BALANCED) == orc.MODE["XORZ"]
+    assert mode(15, orc.MAX) == orc.MODE["XORZ"]
+    assert mode(16, orc.MAX) == orc.MODE["STRIDE"]
+
+
+def test_ratio_floors_balanced_and_max():
+    # tests/ratio_regression.rs:40-56, every column at Balanced and Max
+    sine, ts, lc = ratio_columns()
+    s, cents = 0xfeed, []
+    for _ in range(50000):
+        s = lcg(s)
+        cents.append(10000 + (s >> 56))
+    raw = b"".join(int(v).to_bytes(16, "little", signed=True) for v in cents)
+    floors = {orc.BALANCED: (1.40, 4.70, 7.50, 15.30), orc.MAX: (7.90, 5.10, 7.70, 15.40)}
+    for lv, (fs, ft, fl, fc) in floors.items():
+        assert 400000 / len(orc.compress(sine, orc.F64, level=lv)) >= fs
+        assert 400000 / len(orc.compress(ts, orc.I64, level=lv)) >= ft
+        assert 200000 / len(orc.compress(lc, orc.I32, level=lv)) >= fl
+        assert 800000 / len(orc.compress_decimal(raw, 6, level=lv)) >= fc
+
+
+def reference_sine():
+    return np.array([math.sin(i * 0.01) * 1000.0 + i * 0.5 for i in range(200000)])
+
+
+def test_level_sizes_and_round_trips_every_level():
+    # tests/typed.rs:227-270: every level round-trips, none beats Max, and
+    # Fastest is larger than Fast which is larger than Max here
+    data = reference_sine()
+    sizes = []
+    for lv in ALL_LEVELS:
+        st = orc.compress(data, orc.F64, level=lv)
+        _, back, _ = orc.decompress(st)
+        assert back.view(np.uint64).tolist() == data.view(np.uint64).tolist()
+        sizes.append(len(st))
+    assert all(sz >= sizes[-1] for sz in sizes)
+    assert sizes[0] > sizes[-1] and sizes[0] > sizes[1]
+
+
+def test_levels_apply_to_integer_columns():
+    # tests/typed.rs:272-289
+    ids = np.array([1000 + (i % 500) for i in range(100000)], dtype=np.int64)
+    for lv in (orc.FASTEST, orc.FAST, orc.BALANCED, orc.MAX):
+        st = orc.compress(ids, orc.I64, level=lv)
+        assert len(st) < len(ids) * 2
+        assert (orc.decompress(st)[1] == ids).all()
+
+
+def test_reference_default_config_datasets():
+    # tests/roundtrip.rs:66-122 at Config::default (Level::Max): ratio lower
+    # bounds, the random floor, multi-block and special values
+    N = 1 << 16
+    i = np.arange(N)
+    ratio = lambda d: len(d) * 8 / len(orc.compress(d, orc.F64, level=orc.MAX))
+    for d in (np.zeros(0), np.array([2.5]), np.array([1.0, 2.0])):
+        assert orc.decompress(orc.compress(d, orc.F64, level=orc.MAX))[1].tobytes() == d.tobytes()
+    assert ratio(np.full(N, 42.0)) > 100.0
+    assert ratio(i.astype(np.float64)) > 2.0
+    assert ratio(i * 0.5) > 2.0
+    assert ratio((i % 1000) * 1000.0) > 1.5
+    assert ratio((i // 256) * 7.0) > 4.0
+    seed, rnd = 0x123456789abcdef0, []
+    for _ in range(N):
+        seed = lcg(seed)
+        rnd.append(seed)
+    rnd = np.array(rnd, dtype=np.uint64).view(np.float64)
+    assert ratio(rnd) > 0.95
+    for d in (np.sqrt(np.arange(200000, dtype=np.float64)),
+              np.array([0.0, -0.0, np.nan, np.inf, -np.inf, np.finfo(np.float64).min,
+                        np.finfo(np.float64).max, np.finfo(np.float64).tiny]), rnd):
+        st = orc.compress(d, orc.F64, level=orc.MAX)
+        assert orc.decompress(st)[1].view(np.uint64).tolist() == d.view(np.uint64).tolist()
+
+
+def test_sample_selection_round_trips():
+    # tests/roundtrip.rs:6-34 (Selection::Sample at Config::default) and the
+    # typed shapes of tests/typed.rs:53-105 under both selections
+    s, rnd = 0x9e3779b9, []
+    for _ in range(70000):
+        s = lcg1(s)
+        rnd.append(s)
+    i = np.arange(70000)
+    sets = [i * 0.5, np.sin(i * 1e-4), (i & 15).astype(np.float64), (i % 1000) / 100.0,
+            np.array(rnd, dtype=np.uint64).view(np.float64), np.full(70000, 42.0)]
+    for d in sets:
+        st = orc.compress(d, orc.F64, level=orc.MAX, selection=1)
+        assert orc.decompress(st)[1].view(np.uint64).tolist() == d.view(np.uint64).tolist()
+    for sel in (0, 1):
+        for vals in ([], [42], [-1, 0, 1, -(2 ** 63), 2 ** 63 - 1], [7] * 5000):
+            v = np.array(vals, dtype=np.int64)
+            assert (orc.decompress(orc.compress(v, orc.I64, level=orc.MAX, selection=sel))[1] == v).all()
+
+
+def test_sample_selection_encodes_the_sampled_winner():
+    # encoder.rs:583-638: a block of 8 x 64-value runs spread over it; every
+    # block's frame is either a full-block structural mode (CONST / STRIDE /
+    # LZ / pco / RAW) or the mode the sample ranked first. On a decimal walk at
+    # Fast (no pco, no LZ) that is ALP or FloatMult-class, never RAW.
+    walk = np.round(100 * (1000 + np.cumsum(np.random.default_rng(3).standard_normal(100000)))) / 100
+    st = orc.compress(walk, orc.F64, level=orc.FAST, selection=1)
+    assert set(frame_modes(st)) <= {orc.MODE["ALP"], orc.MODE["FLOAT_MULT"], orc.MODE["FOR_BITPACK"],
+                                    orc.MODE["DELTA_BITPACK"], orc.MODE["XORZ"], orc.MODE["ALP_RD"]}
+    # a 512-value block is its own sample: Sample picks what Full picks among
+    # the sampled modes when no structural mode applies
+    small = walk[:512]
+    assert frame_modes(orc.compress(small, orc.F64, level=orc.FAST, selection=1)) == \
+        frame_modes(orc.compress(small, orc.F64, level=orc.FAST))
+
+
+def estimate(b):
+    a = np.frombuffer(bytes(b), dtype=np.uint8)
+    return orc.lib.orc_estimate_order1_bytes(a.ctypes.data if len(a) else None, len(a))
+
+
+def test_order1_estimate():
+    # entropy/mod.rs:90-125. Under 64 bytes: the length. A constant stream has
+    # zero conditional entropy; an alternating stream 0,1,0,1,... is
+    # determined by its previous byte except the first (a few bits, where an
+    # order-0 estimate would give n/8 bytes); uniform random bytes cost 8 bits
+    # each minus the small-sample bias of the 65,536-cell histogram.
+    assert estimate(b"abc" * 10) == 30
+    assert estimate(bytes(100000)) == 0
+    alt = bytes([k & 1 for k in range(100000)])
+    assert estimate(alt) <= 4
+    rng = np.random.default_rng(5)
+    rnd = rng.integers(0, 256, 1 << 20, dtype=np.uint8).tobytes()
+    e = estimate(rnd)
+    assert 0.97 * len(rnd) < e <= len(rnd)
+    # two equiprobable symbols after every byte: exactly 1 bit per value
+    two = bytes(rng.integers(0, 2, 200000, dtype=np.uint8))
+    assert abs(estimate(two) - 200000 / 8) < 200000 / 8 * 0.01
+
+
+def test_range_coder_hand_derived_stream():
+    # entropy/rc.rs on one byte 0xFF, derived by hand from the LZMA-style
+    # coder: eight 1-bits at probability 1024/2048 leave low = 0xFEFFFC00 and
+    # range = 0x010003FF; the five flush shifts emit 00 FE FF FC 00 after the
+    # length varint. A zero byte never moves low: six zero bytes.
+    assert rc_bytes(b"\xff") == bytes.fromhex("0100fefffc00")
+    assert rc_bytes(b"\x00") == bytes.fromhex("01000000000000")
+
+
+def rc_bytes(b):
+    n = orc.ctypes.c_size_t()
+    buf = (orc.ctypes.c_uint8 * len(b)).from_buffer_copy(b)
+    p = orc.lib.orc_rc_compress(buf, len(b), orc.ctypes.byref(n))
+    out = orc.ctypes.string_at(p, n.value)
+    orc.lib.orc_free(p)
+    return out
+
+
+def test_lz_chain_stops_at_64_candidates():
+    # codecs/lz.rs:37-93: the hash chain walks at most 64 candidates. "ABCD"
+    # + a 20-byte tail, then 64 copies of "ABCD" + one distinct byte, then
+    # "ABCD" + the tail again. The full 24-byte match is the 65th candidate,
+    # so the last segment is a 4-byte match to the newest copy followed by a
+    # 20-byte match of the tail; with a deeper chain it would be one match.
+    seg0 = b"ABCD" + bytes(range(100, 120))
+    data = seg0 + b"".join(b"ABCD" + bytes([k]) for k in range(64)) + seg0
+    toks = [("L", b) for b in seg0] + [("M", 24, 4), ("L", 0)]
+    for k in range(1, 64):
+        toks += [("M", 5, 4), ("L", k)]
+    toks += [("M", 5, 4), ("M", 344, 20)]
+
+    def vw(v):
+        o = b""
+        while v >= 0x80:
+            o += bytes([v & 0x7F | 0x80])
+            v >>= 7
+        return o + bytes([v])
+
+    want = vw(len(data))
+    for g in range(0, len(toks), 8):
+        grp = toks[g:g + 8]
+        want += bytes([sum(1 << (7 - k) for k, t in enumerate(grp) if t[0] == "M")])
+        for t in grp:
+            want += bytes([t[1]]) if t[0] == "L" else vw(t[1]) + vw(t[2] - 4)
+    n = orc.ctypes.c_size_t()
+    buf = (orc.ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+    p = orc.lib.orc_lz_compress(buf, len(data), orc.ctypes.byref(n))
+    assert orc.ctypes.string_at(p, n.value) == want
+    orc.lib.orc_free(p)
+
+
+def test_validity_blob_known_answer():
+    # validity.rs:40-70: runs alternate starting with the null run; n = 64
+    # with only position 10 null: [1] + varints 0 (nulls), 10 (valid),
+    # 1 (null), 53 (valid) = 01 00 0A 01 35, shorter than the raw form (9 B)
+    valid = np.ones(64, bool)
+    valid[10] = False
+    bm = np.packbits(valid.astype(np.uint8), bitorder="little")
+    st = orc.compress(np.arange(64, dtype=np.int64), orc.I64, bm, level=orc.FAST)
+    assert st[5] == 1 and st[16:22] == bytes.fromhex("050100" "0a0135")
+    # all nulls but one at the end: one null run of 63, one valid run of 1
+    valid = np.zeros(64, bool)
+    valid[63] = True
+    bm = np.packbits(valid.astype(np.uint8), bitorder="little")
+    st = orc.compress(np.arange(64, dtype=np.int64), orc.I64, bm, level=orc.FAST)
+    assert st[16:20] == bytes.fromhex("03013f01")
+
+
+def test_lz_cascade_applies_to_the_top_two_at_max_only():
+    # encoder.rs:457-465: at Max the winner and the runner-up are re-encoded
+    # with the LZ cascade inside code_residuals; High runs the same
+    # competition without it. A repeating multi-value pattern (period 37, so
+    # no dictionary / run structure) is where the cascade pays.
+    pat = np.random.default_rng(9).integers(0, 1 << 40, 37)
+    data = np.tile(pat, 3000).astype(np.int64)
+    hi = orc.compress(data, orc.I64, level=orc.HIGH)
+    mx = orc.compress(data, orc.I64, level=orc.MAX)
+    assert len(mx) <= len(hi)
+    for st in (hi, mx):
+        assert (orc.decompress(st)[1] == data).all()
+    # every frame of the Max stream decodes through decode_residuals where it
+    # is an entropy mode, and a TAG_LZ payload appears only at Max
+    def lz_tagged(st):
+        ent = {orc.MODE[k] for k in ("PRED_RC", "PRED2", "DELTA2", "DELTA_DP", "ORDERED_DELTA",
+                                      "LZ", "BYTE_TRANSPOSE")}
+        return any(m in ent and p[:1] == b"\x03" for m, _, p in orc.frames(st))
+    assert not lz_tagged(hi)
