@@ -118,11 +118,18 @@ int quoin_typed_decompress(int32_t dtype, size_t n, const uint8_t *src, size_t s
 #define QUOIN_LEVEL_MAX      4
 #define QUOIN_DEFAULT_LEVEL  QUOIN_LEVEL_FAST
 
-/* Config (src/lib.rs:194-220): level and fixed block size in values
- * (0 = adaptive; clamped to [1, 131072]). */
+/* Selection (src/lib.rs:150-161). */
+#define QUOIN_SELECTION_FULL   0
+#define QUOIN_SELECTION_SAMPLE 1
+
+/* Config (src/lib.rs:194-245): level, fixed block size in values (0 =
+ * adaptive; clamped to [1, 131072]), mode selection, and the FCM / DFCM table
+ * size predictor_log2 (0 = 16; clamped to [10, 16]). */
 typedef struct {
     int32_t level;
     uint32_t block_size;
+    int32_t selection;
+    uint32_t predictor_log2;
 } QuoinConfig;
 
 /* As quoin_typed_compress with an explicit config; NULL cfg = defaults. */

@@ -2169,6 +2169,16 @@ static int looks_incompressible(const uint64_t *b, size_t n, feats_t f, size_t l
     uint32_t dw = dmax == 0 ? 0 : 64 - lz64(dmax);
     return vw >= thr && dw >= thr;
 }
+/* the block features of probe_block_features and looks_incompressible
+ * (encoder.rs:655-750): out = {exp_range, distinct_low, repeats,
+ * incompressible} */
+void orc_block_feats(const uint64_t *b, size_t n, int dt, uint32_t out[4]) {
+    feats_t f = probe(b, n);
+    out[0] = f.exp_range;
+    out[1] = (uint32_t)f.distinct_low;
+    out[2] = (uint32_t)f.repeats;
+    out[3] = n ? (uint32_t)looks_incompressible(b, n, f, dtype_lane_bytes(dt)) : 0;
+}
 #define BASE_Q 32768
 #define MAX_Q 131072
 size_t orc_plan_blocks(const uint64_t *v, size_t n, size_t bs, uint64_t *ranges) {

@@ -70,16 +70,25 @@ FROM_NP = {np.dtype(np.int8): DType.I8, np.dtype(np.int16): DType.I16,
            np.dtype(np.float32): DType.F32, np.dtype(np.float64): DType.F64}
 
 
+class Selection(enum.IntEnum):
+    """src/lib.rs:150-161"""
+    FULL = 0
+    SAMPLE = 1
+
+
 @dataclass
 class Config:
-    """src/lib.rs:194-245 (threads/selection/predictor_log2 have no GPU meaning:
-    the GPU path is Selection::Full with predictor_log2 = 16)."""
-    level: Level = Level.FAST
+    """src/lib.rs:194-245. Config::default is Level::Max, adaptive blocks,
+    Selection::Full, predictor_log2 16 (clamped to [10, 16])."""
+    level: Level = Level.MAX
     block_size: int = 0  # 0 = adaptive
+    selection: Selection = Selection.FULL
+    predictor_log2: int = 16
 
 
 class _CConfig(ctypes.Structure):
-    _fields_ = [("level", ctypes.c_int32), ("block_size", ctypes.c_uint32)]
+    _fields_ = [("level", ctypes.c_int32), ("block_size", ctypes.c_uint32),
+                ("selection", ctypes.c_int32), ("predictor_log2", ctypes.c_uint32)]
 
 
 class DeviceColumn(ctypes.Structure):
@@ -240,7 +249,8 @@ def version():
 
 def _cfg(config):
     config = config or Config()
-    return _CConfig(int(config.level), int(config.block_size))
+    return _CConfig(int(config.level), int(config.block_size), int(config.selection),
+                    int(config.predictor_log2))
 
 
 def _ptr(a):

@@ -28,7 +28,10 @@ enum { QA_FASTEST = 0, QA_FAST = 1 };
 enum {
     QA_M_CONST = 1, QA_M_STRIDE = 2, QA_M_XORZ = 3, QA_M_RAW = 5, QA_M_ALP = 9,
     QA_M_FLOAT_MULT = 17, QA_M_DICT = 19, QA_M_FOR = 50, QA_M_DELTA = 51, QA_M_ALP_RD = 52,
-    QA_M_RLE = 53
+    QA_M_RLE = 53,
+    // a block whose payload the host competition wrote: the device lays out
+    // its frame and leaves the mode byte and payload to k_host_frames
+    QA_M_HOST = 0xFF
 };
 
 // error flags raised by kernels (host maps them to QUOIN_ERR_*)
@@ -54,6 +57,7 @@ struct DCol {
     uint32_t adaptive;
     uint32_t level;
     uint32_t in_kind;         // QA_IN_*: input element type when narrower than the lane
+    uint32_t plog2;           // the header's predictor_log2 (Config::predictor_log2)
 };
 
 // narrow Arrow inputs widened on load (capi/src/typed.rs:147-153)

@@ -998,7 +998,7 @@ __global__ void k_emit_header(EncArgs a) {
     d[0] = 'F'; d[1] = 'C'; d[2] = 'R'; d[3] = '1';
     d[4] = 2;
     d[5] = st.has_nulls ? 1 : 0;
-    d[6] = 16;
+    d[6] = (uint8_t)(col.plog2 ? col.plog2 : 16);
     d[7] = (uint8_t)col.dtype;
     qa_st64u(d + 8, col.n);
     if (st.has_nulls) {

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <thread>
 #include <vector>
@@ -630,3 +631,44 @@ extern "C" int quoin_gpu_decode_residuals(QuoinGpu *g, const QuoinRansStream *in
     }
     return rc;
 }
+
+// ---------------------------------------------------------------------------
+// the host coders for the engine's entropy levels (qa_hybrid.cpp)
+// ---------------------------------------------------------------------------
+#include "qa_hostcoders.h"
+
+namespace qahost {
+
+// code_residuals (mod.rs:121-149) with every coder on host cores
+void code_residuals(const uint8_t *d, size_t n, uint64_t lambda, bool allow_lz, std::vector<uint8_t> &out) {
+    out.clear();
+    pick_host(d, n, lambda, out);
+    if (allow_lz && n > 64) {
+        Bytes z;
+        lz_encode(d, n, z);
+        if (z.size() < n) {
+            Bytes cand;
+            cand.push_back(kTagLz);
+            put_varint(cand, n);
+            pick_host(z.data(), z.size(), lambda, cand);
+            if (cand.size() < out.size()) out.swap(cand);
+        }
+    }
+}
+int decode_residuals(const uint8_t *p, size_t len, uint64_t max_len, std::vector<uint8_t> &out) {
+    out.clear();
+    return residuals_host(p, len, max_len, out);
+}
+void lz_compress(const uint8_t *d, size_t n, std::vector<uint8_t> &out) {
+    out.clear();
+    lz_encode(d, n, out);
+}
+bool lz_decompress(const uint8_t *s, size_t len, uint64_t expected, std::vector<uint8_t> &out) {
+    out.clear();
+    return lz_decode(s, len, expected, out);
+}
+void parallel_for(size_t n, std::vector<int> &err, const std::function<void(size_t)> &f) {
+    host_for(n, err, f);
+}
+
+}  // namespace qahost

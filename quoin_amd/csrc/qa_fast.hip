@@ -756,7 +756,9 @@ __global__ void __launch_bounds__(256) k_block_decide(EncArgs a) {
         b.v0 = v0;
         b.v1 = v1;
         for (int k = 0; k < C_N; k++) b.sz[k] = sz[k];
-        b.compressible = best_len * 20 < (n * 8) * 19 && !(!fl && b.incompressible);
+        // entropy levels: the host decides the gate after its own candidates
+        b.compressible = a.hybrid ? !(!fl && b.incompressible)
+                                  : best_len * 20 < (n * 8) * 19 && !(!fl && b.incompressible);
         b.mode = mode;
         b.payload_len = best_len;
         b.score = best_score;
@@ -789,7 +791,7 @@ __global__ void __launch_bounds__(256) k_block_decide(EncArgs a) {
 #define DH_LIMIT 6144
 
 __device__ __forceinline__ bool dict_candidate(const EncArgs &a, const DBlock &b) {
-    return b.len && b.compressible && a.cols[b.col].level != QA_FASTEST;
+    return b.len && b.compressible && a.cols[b.col].level != QA_FASTEST && !a.hybrid;
 }
 
 // delta_bitpack size of the sorted dictionary (dict.rs:40-61); chunk c's
@@ -1414,10 +1416,15 @@ __global__ void __launch_bounds__(256) k_block_final(EncArgs a) {
             b.sz[C_RLE] = rsz;
         }
     }
+    if (a.force) {
+        const HForce &f = a.force[slot];
+        mode = f.host ? QA_M_HOST : f.mode;
+        best_len = f.len;
+    }
     if (lane == 0) {
         b.mode = mode;
         b.payload_len = best_len;
-        atomicAdd(&a.wins[mode & 63], 1ull);
+        if (!a.hybrid) atomicAdd(&a.wins[mode & 63], 1ull);
     }
     uint64_t tot;
     const uint64_t vn = qa_varint_len(n);
@@ -1523,5 +1530,11 @@ void qa_fast_final_launch(const EncArgs &a, hipStream_t s) {
         QA_LAUNCH(k_rle_starts, dim3(nbf(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
         QA_LAUNCH(k_rle_chunks, dim3(a.r1 - a.r0), dim3(256), 0, s, a);
     }
+    QA_LAUNCH(k_block_final, dim3(nbf(a.r1 - a.r0, 4)), dim3(256), 0, s, a);
+}
+
+// Levels Balanced / High / Max, once the host competition has filled
+// a.force: the chosen modes' layout (the sizing stages ran before)
+void qa_fast_force_launch(const EncArgs &a, hipStream_t s) {
     QA_LAUNCH(k_block_final, dim3(nbf(a.r1 - a.r0, 4)), dim3(256), 0, s, a);
 }

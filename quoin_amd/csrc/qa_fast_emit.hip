@@ -406,3 +406,48 @@ void qa_fast_emit_launch(const EncArgs &a, hipStream_t s) {
         hipLaunchKernelGGL(k_emit_fast_chunks, dim3((unsigned)(a.t1 - a.t0)), dim3(64), 0, s, a);
     }
 }
+
+// ---------------------------------------------------------------------------
+// Levels Balanced / High / Max: the lane words the host competition reads,
+// and the frames whose payloads it wrote
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int col_by_lane(const DCol *c, int nc, uint64_t j) {
+    QA_COL_SEARCH(lane_off, j)
+}
+
+// out[lane_off + i] = lane word i of its column for i < nv (grid-stride over
+// the padded lane range; the padding is left as it is)
+__global__ void __launch_bounds__(256) k_lane_words(EncArgs a, uint64_t total, uint64_t *out) {
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total;
+         j += (uint64_t)gridDim.x * blockDim.x) {
+        const int c = col_by_lane(a.cols, (int)a.ncols, j);
+        const uint64_t i = j - a.cols[c].lane_off;
+        const DColState &st = a.cst[c];
+        if (i < st.nv) out[j] = lane_at(st, i);
+    }
+}
+
+void qa_lane_words_launch(const EncArgs &a, uint64_t total, uint64_t *out, hipStream_t s) {
+    if (!total || !a.ncols) return;
+    const uint64_t g = nbe(total, 256);
+    hipLaunchKernelGGL(k_lane_words, dim3((unsigned)(g < 8192 ? g : 8192)), dim3(256), 0, s, a,
+                       total, out);
+}
+
+// one workgroup per host-written frame: its mode byte over the placeholder
+// k_emit_frames wrote, then the payload (frame_off comes from the layout)
+__global__ void __launch_bounds__(256) k_host_frames(EncArgs a, const HScatter *sc,
+                                                     const uint8_t *src) {
+    const HScatter h = sc[blockIdx.x];
+    const DBlock &b = a.blocks[h.slot];
+    if (a.cst[b.col].err) return;
+    uint8_t *d = a.cols[b.col].dst + b.frame_off;
+    if (threadIdx.x == 0) d[0] = (uint8_t)h.mode;
+    const uint8_t *p = src + h.src_off;
+    for (uint64_t i = threadIdx.x; i < h.len; i += blockDim.x) d[9 + i] = p[i];
+}
+
+void qa_host_frames_launch(const EncArgs &a, const HScatter *sc, uint32_t ns, const uint8_t *src,
+                           hipStream_t s) {
+    if (ns) hipLaunchKernelGGL(k_host_frames, dim3(ns), dim3(256), 0, s, a, sc, src);
+}

@@ -15,6 +15,9 @@
 #include <new>
 #include <vector>
 
+#include "qa_hostcoders.h"
+#include "qa_hybrid.h"
+
 #include "../../include/quoin.h"
 #include "qa_kernels.h"
 
@@ -92,7 +95,7 @@ struct QuoinGpu {
     hipStream_t stream = nullptr;
     uint64_t budget = kDefaultBudget;
     int form = 0;
-    std::mutex mu;
+    std::recursive_mutex mu;  // the host stage of a call re-enters the handle's entry points
     HostBuf hstage;  // pinned: descriptor uploads, then result readbacks
     DevBuf dmeta;    // the uploaded descriptors of a decompress call
     // encode arenas
@@ -110,6 +113,9 @@ struct QuoinGpu {
     // FCM / DFCM predictor arenas: sort keys and indices, residual words, sort
     // scratch, decode tables
     DevBuf pkeys, pres, psort, ptab;
+    // Levels Balanced / High / Max: lane words, residual streams, the forced
+    // modes, the host-written frames and their payload bytes
+    DevBuf hlane, hres, hforce, hscat, hsrc;
     // decimal container: partial minima, vmin and limbs
     DevBuf dm;
     // autotuned launch shapes of the tile kernels (QA_TUNE_*, 0 = untuned)
@@ -158,7 +164,7 @@ uint32_t qa_tune_pick(const std::function<void(uint32_t)> &launch, hipStream_t s
 
 extern "C" int quoin_gpu_set_autotune(QuoinGpu *g, int on) {
     if (!g) return QUOIN_ERR_NULL;
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     g->autotune = on != 0;
     return QUOIN_OK;
 }
@@ -248,7 +254,7 @@ bool gpu_mode(int m, int) {
     }
 }
 
-int level_supported(int level) { return level == QUOIN_LEVEL_FASTEST || level == QUOIN_LEVEL_FAST; }
+int level_supported(int level) { return level >= QUOIN_LEVEL_FASTEST && level <= QUOIN_LEVEL_MAX; }
 
 std::mutex g_default_mu;
 QuoinGpu *g_default = nullptr;
@@ -303,6 +309,10 @@ int parse_stream(const uint8_t *src, size_t len, ParsedStream &ps) {
         uint64_t vlen;
         if (!rd_varint(src, len, &pos, &vlen)) return QUOIN_ERR_CORRUPT;
         if (vlen > len - pos) return QUOIN_ERR_CORRUPT;
+        // validity::decode reads the tag first, also for n == 0: an empty blob
+        // is truncated, a tag other than raw (0) / runs (1) is corrupt
+        // (src/validity.rs:73-81)
+        if (vlen == 0 || src[pos] > 1) return QUOIN_ERR_CORRUPT;
         ps.vblob_off = pos;
         ps.vblob_len = vlen;
         pos += vlen;
@@ -375,7 +385,8 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
                      &g->dscr, &g->dscr2, &g->dmeta,
                      &g->st_out, &g->st_vout, &g->rdesc, &g->rres, &g->rtabs, &g->rscr,
                      &g->ldesc, &g->lres, &g->ltile, &g->lscan, &g->pkeys, &g->pres,
-                     &g->psort, &g->ptab, &g->dm};
+                     &g->psort, &g->ptab, &g->dm, &g->hlane, &g->hres, &g->hforce,
+                     &g->hscat, &g->hsrc};
     for (DevBuf *b : all)
         if (b->p) hipFree(b->p);
     hipEventDestroy(g->ev0);
@@ -406,11 +417,189 @@ extern "C" int quoin_gpu_last_timing(const QuoinGpu *g, double *total_ms, double
     return QUOIN_OK;
 }
 
+// Levels Balanced / High / Max (src/encoder.rs:275-469). The device plans the
+// blocks and sizes the candidates it emits itself; at High / Max it also
+// writes the FCM / DFCM / Delta2 / DeltaDp / OrderedDelta residual streams.
+// Host cores run the rest of the competition per block (entropy forms, range
+// coder and rANS, LZ, ByteTranspose, pco, the LZ cascade). The device then
+// lays out every frame, emits the winners of its own modes, and
+// k_host_frames copies the host-coded payloads into their frames.
+static int hybrid_encode(QuoinGpu *g, EncArgs &a, const std::vector<DCol> &cols, int level,
+                         uint64_t lane_total, uint64_t *out_lens, hipStream_t s) {
+    const size_t ncols = cols.size();
+    const uint32_t nr = a.nregions;
+    a.hybrid = 1;
+    a.force = nullptr;
+    qa_encode_plan_launch(a, s);
+    qa_encode_blocks_launch(a, s);
+    if (!ensure(g->hlane, lane_total * 8)) return QUOIN_ERR_DEVICE;
+    uint64_t *dlane = dp<uint64_t>(g->hlane);
+    qa_lane_words_launch(a, lane_total, dlane, s);
+    std::vector<DBlock> blk(nr);
+    std::vector<DBlockX> bx(nr);
+    std::vector<DColState> st(ncols);
+    std::vector<uint64_t> lanes(lane_total);
+    bool ok = true;
+    if (nr) {
+        ok &= hipMemcpyAsync(blk.data(), g->blocks.p, nr * sizeof(DBlock), hipMemcpyDeviceToHost, s) ==
+              hipSuccess;
+        ok &= hipMemcpyAsync(bx.data(), g->blocksx.p, nr * sizeof(DBlockX), hipMemcpyDeviceToHost, s) ==
+              hipSuccess;
+    }
+    ok &= hipMemcpyAsync(st.data(), g->cst.p, ncols * sizeof(DColState), hipMemcpyDeviceToHost, s) ==
+          hipSuccess;
+    if (lane_total)
+        ok &= hipMemcpyAsync(lanes.data(), dlane, lane_total * 8, hipMemcpyDeviceToHost, s) == hipSuccess;
+    if (!ok || hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    std::vector<uint32_t> slots;
+    for (uint32_t r = 0; r < nr; r++)
+        if (blk[r].len && !st[blk[r].col].err) slots.push_back(r);
+    const size_t nb = slots.size();
+    std::vector<qahyb::EncBlock> eb(nb);
+    for (size_t k = 0; k < nb; k++) {
+        const DBlock &b = blk[slots[k]];
+        const DBlockX &x = bx[slots[k]];
+        const DCol &c = cols[b.col];
+        qahyb::EncBlock &e = eb[k];
+        e.v = lanes.data() + c.lane_off + b.start;
+        e.n = b.len;
+        e.dt = c.dtype;
+        for (int j = 0; j < C_N; j++) e.sz[j] = b.sz[j];
+        e.incompressible = b.incompressible;
+        e.distinct_low = b.distinct_low;
+        e.repeats = b.repeats;
+        if (c.dtype == QA_F64 || c.dtype == QA_F32) {
+            e.fm_scale = x.fm_scale;
+            e.rd_left = x.rd_left;
+            e.rd_dl = x.rd_dl;
+            e.rd_bail = x.rd_bail;
+            for (int q = 0; q < 8; q++) e.rd_dict[q] = x.rd_dict[q];
+        }
+    }
+    // High / Max: the residual streams of the five predictor-family modes
+    std::vector<uint8_t> hres[5];
+    if (level >= QUOIN_LEVEL_HIGH && nb) {
+        std::vector<uint64_t> roff(nb + 1, 0);
+        for (size_t k = 0; k < nb; k++) roff[k + 1] = roff[k] + quoin_linear_bound(eb[k].n);
+        if (!ensure(g->hres, roff[nb])) return QUOIN_ERR_DEVICE;
+        uint8_t *dres = dp<uint8_t>(g->hres);
+        const uint32_t plog2 = cols[0].plog2;
+        for (int j = 0; j < 5; j++) {
+            const bool float_only = j == 2 || j == 3;
+            std::vector<size_t> idx;
+            for (size_t k = 0; k < nb; k++)
+                if (!float_only || eb[k].dt == QA_F64 || eb[k].dt == QA_F32) idx.push_back(k);
+            if (idx.empty()) continue;
+            std::vector<QuoinLinearInput> in(idx.size());
+            for (size_t i = 0; i < idx.size(); i++) {
+                const size_t k = idx[i];
+                const DBlock &b = blk[slots[k]];
+                in[i] = QuoinLinearInput{dlane + cols[b.col].lane_off + b.start, eb[k].n, dres + roff[k],
+                                         quoin_linear_bound(eb[k].n)};
+            }
+            std::vector<uint64_t> lens(idx.size(), 0);
+            std::vector<int32_t> status(idx.size(), 0);
+            // at most 4096 blocks and 2^30 values per call (the predictor
+            // sort's key and index widths)
+            for (size_t i0 = 0; i0 < idx.size();) {
+                size_t i1 = i0;
+                uint64_t vals = 0;
+                while (i1 < idx.size() && i1 - i0 < 4096 && vals + in[i1].n <= (1ull << 30)) vals += in[i1++].n;
+                if (i1 == i0) i1++;
+                int rc;
+                if (j < 2)
+                    rc = quoin_gpu_pred_encode(g, j == 0 ? QUOIN_PRED_FCM : QUOIN_PRED_DFCM, plog2, &in[i0],
+                                               i1 - i0, &lens[i0], s);
+                else
+                    rc = quoin_gpu_linear_encode(g, j == 2 ? QUOIN_LINEAR_DELTA2
+                                                    : j == 3 ? QUOIN_LINEAR_DELTA_DP
+                                                             : QUOIN_LINEAR_ORDERED_DELTA,
+                                                 0, &in[i0], i1 - i0, &lens[i0], nullptr, &status[i0], s);
+                if (rc != QUOIN_OK) return rc;
+                i0 = i1;
+            }
+            hres[j].resize(roff[nb]);
+            if (hipMemcpyAsync(hres[j].data(), dres, roff[nb], hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                return QUOIN_ERR_DEVICE;
+            for (size_t i = 0; i < idx.size(); i++) {
+                qahyb::EncBlock &e = eb[idx[i]];
+                if (status[i]) continue;  // dp_encode's None
+                e.res[j] = hres[j].data() + roff[idx[i]];
+                e.rlen[j] = lens[i];
+                e.has_res[j] = true;
+            }
+        }
+    }
+    std::vector<int> err(nb, QUOIN_OK);
+    qahost::parallel_for(nb, err, [&](size_t k) { qahyb::compete(eb[k], level); });
+    for (int e : err)
+        if (e != QUOIN_OK) return e;
+    // the winners: forced modes for the layout, host payloads for the scatter
+    std::vector<HForce> force(nr ? nr : 1, HForce{QA_M_RAW, 0, 0});
+    std::vector<HScatter> sc;
+    uint64_t hbytes = 0;
+    for (size_t k = 0; k < nb; k++) {
+        const qahyb::EncBlock &e = eb[k];
+        if (e.len > 0xFFFFFFFFull) return QUOIN_ERR_PANIC;  // the frame's u32 payload length
+        force[slots[k]] = HForce{(uint32_t)e.mode, e.host ? 1u : 0u, e.len};
+        if (e.host) {
+            sc.push_back(HScatter{slots[k], (uint32_t)e.mode, hbytes, e.len});
+            hbytes += e.len;
+        }
+        g_wins[e.mode & 63]++;
+    }
+    std::vector<uint8_t> pay(hbytes ? hbytes : 1);
+    for (size_t k = 0, o = 0; k < nb; k++)
+        if (eb[k].host) {
+            memcpy(pay.data() + o, eb[k].payload.data(), eb[k].len);
+            o += eb[k].len;
+        }
+    if (!ensure(g->hforce, force.size() * sizeof(HForce)) ||
+        !ensure(g->hscat, (sc.size() + 1) * sizeof(HScatter)) || !ensure(g->hsrc, pay.size()))
+        return QUOIN_ERR_DEVICE;
+    ok = hipMemcpyAsync(g->hforce.p, force.data(), force.size() * sizeof(HForce), hipMemcpyHostToDevice, s) ==
+         hipSuccess;
+    if (!sc.empty())
+        ok &= hipMemcpyAsync(g->hscat.p, sc.data(), sc.size() * sizeof(HScatter), hipMemcpyHostToDevice, s) ==
+              hipSuccess;
+    ok &= hipMemcpyAsync(g->hsrc.p, pay.data(), pay.size(), hipMemcpyHostToDevice, s) == hipSuccess;
+    if (!ok) return QUOIN_ERR_DEVICE;
+    a.force = dp<HForce>(g->hforce);
+    qa_fast_force_launch(a, s);
+    qa_encode_layout_launch(a, s);
+    qa_encode_emit_launch(a, s);
+    qa_host_frames_launch(a, dp<HScatter>(g->hscat), (uint32_t)sc.size(), dp<uint8_t>(g->hsrc), s);
+    hipEventRecord(g->ev1, s);
+    if (hipMemcpyAsync(st.data(), g->cst.p, ncols * sizeof(DColState), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    float ms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    g->last_total_ms = ms;
+    g->last_kernel_ms = 0;
+    g->last_kernel = "";
+    int rc = QUOIN_OK;
+    for (size_t c = 0; c < ncols; c++) {
+        out_lens[c] = st[c].stream_len;
+        if (st[c].err & QA_ERR_OVERFLOW) rc = QUOIN_ERR_BUFFER_TOO_SMALL;
+        else if (st[c].err) rc = QUOIN_ERR_PANIC;
+    }
+    return rc;
+}
+
 static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinDeviceColumn *in,
                                size_t ncols, uint64_t *out_lens, hipStream_t s) {
     int level = cfg ? cfg->level : QUOIN_DEFAULT_LEVEL;
     if (level < 0 || level > QUOIN_LEVEL_MAX) return QUOIN_ERR_UNSUPPORTED;
     if (!level_supported(level)) return QUOIN_ERR_UNSUPPORTED;
+    if (cfg && cfg->selection != QUOIN_SELECTION_FULL) return QUOIN_ERR_UNSUPPORTED;
+    const bool hybrid = level >= QUOIN_LEVEL_BALANCED;
+    // Config::predictor_log2 clamps to [10, 16] (src/lib.rs:236-238)
+    uint32_t plog2 = cfg && cfg->predictor_log2 ? cfg->predictor_log2 : 16;
+    plog2 = plog2 < 10 ? 10 : plog2 > 16 ? 16 : plog2;
     uint32_t bs = cfg ? cfg->block_size : 0;
     if (bs > QA_MAX_Q) bs = QA_MAX_Q;
     if (ncols == 0) return QUOIN_OK;
@@ -432,7 +621,8 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         d.dtype = dt;
         d.in_kind = kind;
         d.in_width = width;
-        d.level = (uint32_t)level;
+        d.level = hybrid ? (uint32_t)QA_FAST : (uint32_t)level;
+        d.plog2 = plog2;
         d.adaptive = bs == 0;
         d.region_size = bs ? bs : QA_BASE_Q;
         d.tpr = (d.region_size + QA_TILE - 1) / QA_TILE;
@@ -496,7 +686,7 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     a.r1 = region_base;
     a.t0 = 0;
     a.t1 = tile_base;
-    if (level == QUOIN_LEVEL_FAST) {
+    if (level == QUOIN_LEVEL_FAST || hybrid) {
         // Level::Fast scratch: per-tile / per-block codec state and the
         // value-slot arrays of Dict / RLE / the hash fallbacks
         if (!ensure(g->tilesf, tile_base * sizeof(DTileF)) ||
@@ -524,6 +714,7 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     hipEventRecord(g->ev0, s);
     zero_async(g->cst.p, ncols * sizeof(DColState), s);
     zero_async(g->wins.p, 64 * 8, s);
+    if (hybrid) return hybrid_encode(g, a, cols, level, lane_off, out_lens, s);
     if (g->form == 0) {
         a.kev0 = g->kev0;
         a.kev1 = g->kev1;
@@ -595,14 +786,57 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
 extern "C" int quoin_gpu_compress(QuoinGpu *g, const QuoinConfig *cfg, const QuoinDeviceColumn *cols,
                                   size_t ncols, uint64_t *out_lens, void *stream) {
     if (!g || (ncols && (!cols || !out_lens))) return QUOIN_ERR_NULL;
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
     return gpu_compress_locked(g, cfg, cols, ncols, out_lens, s);
 }
 
+static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t ns, hipStream_t s);
+
+// Streams with frames of the host-coded modes are first rewritten into
+// device-decodable frames of the same values (qa_hybrid.cpp), then every
+// stream goes through the device tile path.
 static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_t ns,
                                  hipStream_t s) {
+    if (ns == 0) return QUOIN_OK;
+    std::vector<const uint8_t *> hs(ns), ds(ns);
+    std::vector<uint64_t> ls(ns);
+    for (size_t c = 0; c < ns; c++) {
+        if (!in[c].src || !in[c].host_src) return QUOIN_ERR_NULL;
+        hs[c] = in[c].host_src;
+        ds[c] = in[c].src;
+        ls[c] = in[c].src_len;
+    }
+    std::vector<qahyb::Bytes> rw;
+    int rc = qahyb::rewrite_streams(g, ns, hs.data(), ds.data(), ls.data(), rw, s);
+    if (rc != QUOIN_OK) return rc;
+    size_t tot = 0;
+    for (const auto &b : rw) tot += align256(b.size());
+    if (!tot) return gpu_decompress_tiles(g, in, ns, s);
+    uint8_t *dev = nullptr;
+    if (hipMalloc(&dev, tot) != hipSuccess) return QUOIN_ERR_DEVICE;
+    std::vector<QuoinDeviceStream> in2(in, in + ns);
+    size_t off = 0;
+    for (size_t c = 0; c < ns; c++) {
+        if (rw[c].empty()) continue;
+        if (hipMemcpyAsync(dev + off, rw[c].data(), rw[c].size(), hipMemcpyHostToDevice, s) != hipSuccess) {
+            hipFree(dev);
+            return QUOIN_ERR_DEVICE;
+        }
+        in2[c].src = dev + off;
+        in2[c].host_src = rw[c].data();
+        in2[c].src_len = rw[c].size();
+        off += align256(rw[c].size());
+    }
+    rc = gpu_decompress_tiles(g, in2.data(), ns, s);
+    hipStreamSynchronize(s);
+    hipFree(dev);
+    return rc;
+}
+
+static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t ns,
+                                hipStream_t s) {
     if (ns == 0) return QUOIN_OK;
     std::vector<ParsedStream> ps(ns);
     std::vector<DDecCol> cols(ns);
@@ -793,7 +1027,7 @@ static int gpu_decompress_locked(QuoinGpu *g, const QuoinDeviceStream *in, size_
 extern "C" int quoin_gpu_decompress(QuoinGpu *g, const QuoinDeviceStream *streams, size_t ns,
                                     void *stream) {
     if (!g || (ns && !streams)) return QUOIN_ERR_NULL;
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
     return gpu_decompress_locked(g, streams, ns, s);
@@ -807,7 +1041,7 @@ extern "C" size_t quoin_rans_bound(size_t n) { return n ? 2 * n + 1024 : 0; }
 extern "C" int quoin_gpu_rans_compress(QuoinGpu *g, const QuoinRansInput *in, size_t ns,
                                        uint64_t *out_lens, void *stream) {
     if (!g || (ns && (!in || !out_lens))) return QUOIN_ERR_NULL;
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
     std::vector<RansEncD> d;
@@ -855,7 +1089,7 @@ extern "C" int quoin_gpu_rans_compress(QuoinGpu *g, const QuoinRansInput *in, si
 extern "C" int quoin_gpu_rans_decompress(QuoinGpu *g, const QuoinRansStream *in, size_t ns, int tagged,
                                          uint64_t *out_lens, int32_t *status, void *stream) {
     if (!g || (ns && (!in || !out_lens))) return QUOIN_ERR_NULL;
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
     if (ns == 0) return QUOIN_OK;
@@ -921,7 +1155,7 @@ extern "C" int quoin_gpu_linear_encode(QuoinGpu *g, int32_t kind, int32_t order,
     if (!g || (nb && (!in || !out_lens))) return QUOIN_ERR_NULL;
     if (!lin_kind(kind) || order < 0 || order > 4) return QUOIN_ERR_UNSUPPORTED;
     if (nb > 0xFFFFFFFFull) return QUOIN_ERR_UNSUPPORTED;
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
     if (nb == 0) return QUOIN_OK;
@@ -976,7 +1210,7 @@ extern "C" int quoin_gpu_linear_decode(QuoinGpu *g, int32_t kind, const QuoinLin
     if (!g || (nb && !in)) return QUOIN_ERR_NULL;
     if (!lin_kind(kind)) return QUOIN_ERR_UNSUPPORTED;
     if (nb > 0xFFFFFFFFull) return QUOIN_ERR_UNSUPPORTED;
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
     if (nb == 0) return QUOIN_OK;
@@ -1035,7 +1269,7 @@ extern "C" int quoin_gpu_pred_encode(QuoinGpu *g, int32_t kind, uint32_t predict
         predictor_log2 > 16)
         return QUOIN_ERR_UNSUPPORTED;
     if (nb >= 0xFFFFull) return QUOIN_ERR_UNSUPPORTED;  // block index in the sort key's 16 bits
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
     if (nb == 0) return QUOIN_OK;
@@ -1096,7 +1330,7 @@ extern "C" int quoin_gpu_pred_decode(QuoinGpu *g, int32_t kind, uint32_t predict
         predictor_log2 > 16)
         return QUOIN_ERR_UNSUPPORTED;
     if (nb > 0xFFFFFFFFull) return QUOIN_ERR_UNSUPPORTED;
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = stream ? (hipStream_t)stream : g->stream;
     if (nb == 0) return QUOIN_OK;
@@ -1184,9 +1418,13 @@ static long long host_compress(const QuoinConfig *cfg, int32_t dtype, size_t n,
                                size_t dst_cap) {
     uint32_t dt, kind, width;
     if (!map_dtype(dtype, &dt, &kind, &width)) return QUOIN_ERR_UNSUPPORTED;
+    // a config the engine does not run is refused before any device work
+    if (cfg && (cfg->level < QUOIN_LEVEL_FASTEST || cfg->level > QUOIN_LEVEL_MAX ||
+                cfg->selection != QUOIN_SELECTION_FULL))
+        return QUOIN_ERR_UNSUPPORTED;
     QuoinGpu *g = default_gpu();
     if (!g) return QUOIN_ERR_DEVICE;
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = g->stream;
     // staging holds the stream even when it exceeds the caller's capacity, so
@@ -1223,7 +1461,7 @@ static int host_decompress(int32_t dtype, size_t n, const uint8_t *src, size_t s
     if (!map_dtype(dtype, &dt, &kind, &width)) return QUOIN_ERR_UNSUPPORTED;
     QuoinGpu *g = default_gpu();
     if (!g) return QUOIN_ERR_DEVICE;
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = g->stream;
     size_t vbytes = (n + 7) / 8;
@@ -1252,7 +1490,10 @@ static int host_decompress(int32_t dtype, size_t n, const uint8_t *src, size_t s
 extern "C" size_t quoin_typed_compress_cfg(const QuoinConfig *cfg, int32_t dtype, size_t n,
                                            const void *values, const uint8_t *validity,
                                            uint8_t *dst, size_t dst_cap) {
-    if (!values || !dst) return 0;
+    if (!values || !dst) {
+        t_last_err = QUOIN_ERR_NULL;
+        return 0;
+    }
     try {
         long long r = host_compress(cfg, dtype, n, values, validity, dst, dst_cap);
         t_last_err = r > 0 ? QUOIN_OK : r <= -1000 ? QUOIN_ERR_BUFFER_TOO_SMALL : (int)r;
@@ -1311,7 +1552,7 @@ static long long host_compress_decimal(int32_t dtype, size_t n, const void *valu
     const uint32_t nl = wire == 6 ? 2 : 4;
     QuoinGpu *g = default_gpu();
     if (!g) return QUOIN_ERR_DEVICE;
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = g->stream;
     const size_t vbytes = (n + 7) / 8;
@@ -1439,7 +1680,7 @@ static int host_decompress_decimal(int32_t dtype, size_t n, const uint8_t *src, 
     if (wire != want_wire || nl != (wire == 6 ? 2u : 4u) || ln != n) return QUOIN_ERR_CORRUPT;
     QuoinGpu *g = default_gpu();
     if (!g) return QUOIN_ERR_DEVICE;
-    std::lock_guard<std::mutex> lk(g->mu);
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
     hipStream_t s = g->stream;
     const size_t vbytes = (n + 7) / 8;
@@ -1556,20 +1797,23 @@ extern "C" int quoin_decompress(const uint8_t *src, size_t src_len, double *dst,
                                 size_t dst_cap_values, size_t *out_values) {
     if (!src || !dst || !out_values) return QUOIN_ERR_NULL;
     try {
-        ParsedStream ps;
         if (src_len < 16) return QUOIN_ERR_CORRUPT;
         ptrdiff_t n = quoin_decompressed_value_count(src, src_len);
         if (n < 0) return QUOIN_ERR_CORRUPT;
         if (src[7] != QA_F64) return QUOIN_ERR_CORRUPT;  // DTypeMismatch -> corrupt at the ABI
-        *out_values = (size_t)n;
+        // the reference decodes first and sets *out_values only after a
+        // successful decode, then reports a too-small dst (capi/src/lib.rs:80-110)
+        std::vector<double> tmp;
+        double *out = dst;
         if ((size_t)n > dst_cap_values) {
-            // the reference decodes first, then reports the size; validate likewise
-            int rc = parse_stream(src, src_len, ps);
-            if (rc) return rc == QUOIN_ERR_UNSUPPORTED ? rc : QUOIN_ERR_CORRUPT;
-            return QUOIN_ERR_BUFFER_TOO_SMALL;
+            tmp.resize((size_t)n);
+            out = tmp.data();
         }
-        int rc = host_decompress(QUOIN_DTYPE_F64, (size_t)n, src, src_len, dst, nullptr);
-        return rc;
+        int rc = host_decompress(QUOIN_DTYPE_F64, (size_t)n, src, src_len, out, nullptr);
+        if (rc == QUOIN_ERR_UNSUPPORTED) rc = QUOIN_ERR_CORRUPT;  // every decode Err is corrupt
+        if (rc) return rc;
+        *out_values = (size_t)n;
+        return (size_t)n > dst_cap_values ? QUOIN_ERR_BUFFER_TOO_SMALL : QUOIN_OK;
     } catch (...) {
         return QUOIN_ERR_PANIC;
     }

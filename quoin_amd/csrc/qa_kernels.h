@@ -68,7 +68,10 @@ struct EncArgs {
     unsigned long long *wins;  // 64 per-mode win counters
     // Level::Fast codecs
     uint32_t fast;         // some column runs at Level::Fast
-    uint32_t pad;
+    // Levels Balanced / High / Max: the device sizes its own candidates for
+    // every block (Dict is the host's), counts no wins, and with `force` set
+    // lays out the modes the host competition chose
+    uint32_t hybrid;
     DTileF *tilesf;        // per tile slot
     DBlockX *blocksx;      // per region slot
     uint64_t *dsorted;     // per lane value slot: sorted dictionaries
@@ -81,6 +84,7 @@ struct EncArgs {
     uint64_t *dparts;      // per region slot: DH_LIMIT keys of its part set
     DRegionG *rgrown;      // per region slot
     uint64_t *gsets;       // 256 words per region slot
+    const struct HForce *force;  // per region slot, or null
     // launch range of the per-block stages: region slots [r0, r1), tile slots
     // [t0, t1) (the whole batch, or one block in the plain form)
     uint32_t r0, r1;
@@ -91,7 +95,27 @@ struct EncArgs {
     uint32_t *tune;
 };
 
+// The winner of a region slot's block at Levels Balanced / High / Max: a mode
+// the device emits itself, or (host = 1) a payload of `len` bytes written by
+// k_host_frames.
+struct HForce {
+    uint32_t mode, host;
+    uint64_t len;
+};
+// One host-written frame: the mode byte and the payload bytes at src_off of
+// the uploaded payload buffer go to the frame of region slot `slot`.
+struct HScatter {
+    uint32_t slot, mode;
+    uint64_t src_off, len;
+};
+// the compacted lane words of every column, at lane_off + i (i < nv)
+void qa_lane_words_launch(const EncArgs &a, uint64_t total, uint64_t *out, hipStream_t s);
+// after the emission: the host-written frames
+void qa_host_frames_launch(const EncArgs &a, const HScatter *sc, uint32_t ns, const uint8_t *src,
+                           hipStream_t s);
+
 void qa_fast_sizes_launch(const EncArgs &a, hipStream_t s);
+void qa_fast_force_launch(const EncArgs &a, hipStream_t s);
 void qa_fast_final_launch(const EncArgs &a, hipStream_t s);
 void qa_fast_emit_launch(const EncArgs &a, hipStream_t s);
 

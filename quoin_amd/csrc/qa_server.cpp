@@ -50,7 +50,8 @@ uint64_t charge(const QuoinJob &j) {
 bool same_group(const QuoinJob &a, const QuoinJob &b) {
     if (a.kind != b.kind) return false;
     if (a.kind == QUOIN_JOB_DECOMPRESS) return true;
-    return a.cfg.level == b.cfg.level && a.cfg.block_size == b.cfg.block_size;
+    return a.cfg.level == b.cfg.level && a.cfg.block_size == b.cfg.block_size &&
+           a.cfg.selection == b.cfg.selection && a.cfg.predictor_log2 == b.cfg.predictor_log2;
 }
 }  // namespace
 
@@ -205,8 +206,11 @@ extern "C" int quoin_server_wait(QuoinServer *S, uint64_t ticket, uint64_t *out_
         auto it = S->reqs.find(ticket);
         if (it == S->reqs.end()) return QUOIN_ERR_NULL;
         r = it->second;
-        S->cv_done.wait(lk, [&] { return r->done; });
+        // the waiter claims the ticket before it sleeps: a second wait on it
+        // (or after it) gets QUOIN_ERR_NULL, and no iterator is held across
+        // the wait, during which submits may rehash the map
         S->reqs.erase(it);
+        S->cv_done.wait(lk, [&] { return r->done; });
     }
     if (out_len) *out_len = r->out_len;
     if (latency_ms)

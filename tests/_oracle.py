@@ -100,6 +100,7 @@ def _load():
     for f in ("orc_pack32", "orc_unpack32", "orc_pack64", "orc_unpack64"):
         getattr(lib, f).argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
     lib.orc_mode_wins.argtypes = [ctypes.c_void_p]
+    lib.orc_block_feats.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
     return lib
 
 
@@ -331,6 +332,43 @@ def decode_residuals(payload, max_len):
     if rc != 0:
         raise ValueError(f"decode_residuals rc={rc}", rc)
     return _take(out, n.value)
+
+
+def block_feats(lane, dtype):
+    """(exp_range, distinct_low, repeats, incompressible) of one block
+    (encoder.rs:655-750)."""
+    lane = np.ascontiguousarray(lane, dtype=np.uint64)
+    out = np.zeros(4, dtype=np.uint32)
+    lib.orc_block_feats(_ptr(lane) if len(lane) else None, len(lane), dtype, _ptr(out))
+    return tuple(int(x) for x in out)
+
+
+def encode_mode_lv(mode, lane, dtype, level, plog2=16):
+    """One mode's payload at a level (encoder.rs:543-581), None when it
+    declines."""
+    lane = np.ascontiguousarray(lane, dtype=np.uint64)
+    n = ctypes.c_size_t()
+    p = lib.orc_encode_mode_lv(MODE[mode] if isinstance(mode, str) else mode,
+                               _ptr(lane) if len(lane) else None, len(lane), plog2, dtype, level,
+                               ctypes.byref(n))
+    if not p:
+        return None
+    b = ctypes.string_at(p, n.value)
+    lib.orc_free(p)
+    return b
+
+
+def lane_words(values):
+    """The u64 lane words of a typed column (src/lib.rs:389-403): f32 widened
+    to f64 bits, i32 sign-extended, u32 zero-extended."""
+    v = np.asarray(values)
+    if v.dtype == np.float32:
+        return v.astype(np.float64).view(np.uint64)
+    if v.dtype == np.int32:
+        return v.astype(np.int64).view(np.uint64)
+    if v.dtype == np.uint32:
+        return v.astype(np.uint64)
+    return np.ascontiguousarray(v).view(np.uint64)
 
 
 def linear_select_order(lane):

@@ -92,13 +92,23 @@ def test_no_gpu_fails_loudly(qa):
         qa.Gpu(0)
 
 
-def test_levels_above_fast_are_unsupported_not_approximated(qa):
-    # Balanced/High/Max need the host range coder, LZ and pco: rejected, not
-    # silently downgraded (checked before any device work)
+def test_every_level_needs_the_device(qa):
+    # Balanced/High/Max plan and size their blocks on the device before the
+    # host coders run: without a GPU they fail loudly like Fastest and Fast
     import torch
     if torch.cuda.is_available():
-        pytest.skip("checked on the GPU by test_gpu_parity")
+        pytest.skip("a GPU is present")
     v = np.arange(100, dtype=np.int64)
     for lvl in (qa.Level.BALANCED, qa.Level.HIGH, qa.Level.MAX):
-        with pytest.raises(qa.QuoinError):
+        with pytest.raises(qa.QuoinError) as e:
             qa.compress_column(v, None, qa.Config(lvl))
+        assert e.value.code == qa.ERR_DEVICE
+
+
+def test_sample_selection_is_refused_not_approximated(qa):
+    # Selection::Sample is not on the device path: refused before any device
+    # work, never silently run as Full
+    v = np.arange(100, dtype=np.int64)
+    with pytest.raises(qa.QuoinError) as e:
+        qa.compress_column(v, None, qa.Config(qa.Level.MAX, selection=qa.Selection.SAMPLE))
+    assert e.value.code == qa.ERR_UNSUPPORTED
