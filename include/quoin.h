@@ -403,6 +403,20 @@ int quoin_gpu_pred_decode(QuoinGpu *g, int32_t kind, uint32_t predictor_log2,
                           const QuoinLinearStream *streams, size_t n_blocks, int32_t *status,
                           void *stream);
 
+/* ---- addition with no counterpart in the reference: no device allocation - */
+/* quoin_gpu_linear_encode for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): same arguments, results and errors, but the call makes no
+ * hipMalloc / hipFree. Its scratch is one fixed 2 MiB device array per GPU,
+ * held until the process ends, which limits a call to 1024 blocks and 65536
+ * tiles of 2048 values (134217728 values); larger calls return
+ * QUOIN_ERR_BUFFER_TOO_SMALL before any device work. Calls share their GPU's
+ * array: calls that overlap on one GPU race. */
+int quoin_gpu_linear_encode_noalloc(QuoinGpu *g, int32_t kind, int32_t order,
+                                    const QuoinLinearInput *in, size_t n_blocks,
+                                    uint64_t *out_lens, int32_t *orders, int32_t *status,
+                                    void *stream);
+
 
 #ifdef __cplusplus
 } /* extern "C" */

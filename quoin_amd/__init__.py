@@ -218,6 +218,7 @@ def _load():
                                             ctypes.POINTER(ctypes.c_uint64),
                                             ctypes.POINTER(ctypes.c_int32),
                                             ctypes.POINTER(ctypes.c_int32), vp]
+    lib.quoin_gpu_linear_encode_noalloc.argtypes = lib.quoin_gpu_linear_encode.argtypes
     lib.quoin_gpu_linear_decode.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(LinearStream), sz,
                                             ctypes.POINTER(ctypes.c_int32), vp]
     return lib
@@ -239,7 +240,7 @@ EXPORTS = [
     "quoin_gpu_rans_decompress", "quoin_linear_bound", "quoin_gpu_linear_encode",
     "quoin_gpu_linear_decode", "quoin_residual_bound", "quoin_gpu_code_residuals",
     "quoin_gpu_decode_residuals", "quoin_gpu_pred_encode", "quoin_gpu_pred_decode",
-    "quoin_gpu_set_autotune", "quoin_gpu_tuned_shapes",
+    "quoin_gpu_set_autotune", "quoin_gpu_tuned_shapes", "quoin_gpu_linear_encode_noalloc",
 ]
 
 
@@ -479,6 +480,21 @@ class Gpu:
         ords = (ctypes.c_int32 * k)()
         st = (ctypes.c_int32 * k)()
         rc = lib.quoin_gpu_linear_encode(self.h, kind, order, arr, k, lens, ords, st, stream)
+        if rc != OK:
+            raise QuoinError(rc)
+        return [int(x) for x in lens], [int(x) for x in ords], [int(x) for x in st]
+
+    def linear_encode_noalloc(self, kind, inputs, order=0, stream=None):
+        """linear_encode with no device allocation: the scratch is one fixed
+        device array per GPU (at most 1024 blocks and 65536 tiles of 2048
+        values per call)."""
+        k = len(inputs)
+        arr = (LinearInput * k)(*inputs)
+        lens = (ctypes.c_uint64 * k)()
+        ords = (ctypes.c_int32 * k)()
+        st = (ctypes.c_int32 * k)()
+        rc = lib.quoin_gpu_linear_encode_noalloc(self.h, kind, order, arr, k, lens, ords, st,
+                                                 stream)
         if rc != OK:
             raise QuoinError(rc)
         return [int(x) for x in lens], [int(x) for x in ords], [int(x) for x in st]

@@ -176,6 +176,7 @@ extern "C" int quoin_gpu_tuned_shapes(const QuoinGpu *g, uint32_t out[3]) {
 }
 
 int qa_gpu_device(const QuoinGpu *g) { return g->device; }
+std::recursive_mutex &qa_gpu_mutex(QuoinGpu *g) { return g->mu; }
 hipStream_t qa_gpu_stream(const QuoinGpu *g, void *stream) {
     return stream ? (hipStream_t)stream : g->stream;
 }

@@ -2,6 +2,9 @@
 // host engine.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <mutex>
+
 #include "qa_common.h"
 
 size_t qa_scan_tmp_words(uint64_t n);
@@ -317,6 +320,8 @@ void qa_put_bytes_launch(const BytePuts &b, hipStream_t s);
 struct QuoinGpu;
 int qa_gpu_device(const QuoinGpu *g);
 hipStream_t qa_gpu_stream(const QuoinGpu *g, void *stream);
+// the lock that serializes a handle's calls
+std::recursive_mutex &qa_gpu_mutex(QuoinGpu *g);
 
 void qa_rans_decode_launch(const RansDecD *d, RansDecR *r, uint32_t ns, hipStream_t s);
 void qa_rans_encode_launch(const RansEncD *d, RansEncR *r, uint64_t *tabs, uint8_t *scr, uint32_t ns,
