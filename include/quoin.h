@@ -6,15 +6,17 @@
  * and buffer ownership. The second block adds the device-resident batched and
  * served entry points the GPU build exists for.
  *
- * Levels. The reference's Config::default is Level::Max, whose competition also
- * runs pco, the order-1 range coder and the LZ cascade. Those host coders are
- * not part of this build yet, so the reference-named compress entry points run
- * at QUOIN_DEFAULT_LEVEL (Level::Fast: the largest level whose whole codec pool
- * runs on the GPU). Their streams are valid quoin streams, bit-identical to the
- * reference encoder at Level::Fast, not at Level::Max. Levels above Fast return
- * QUOIN_ERR_UNSUPPORTED from the *_cfg entry points. Decoding accepts every
- * frame the Fast and Fastest pools emit; a stream holding a frame of a host-only
- * mode (pco, range coder, LZ, predictors) returns QUOIN_ERR_UNSUPPORTED.
+ * Levels. The reference-named compress entry points run at Config::default,
+ * Level::Max (capi/src/lib.rs:29-34, capi/src/typed.rs:71-73), as the reference
+ * does; the *_cfg and quoin_gpu_compress entry points take any Level. Fastest
+ * and Fast run wholly on the GPU. Balanced, High and Max plan blocks, size the
+ * device-emitted candidates and write the predictor residual streams on the
+ * GPU, and run the sequential coders of the competition (range coder, rANS,
+ * LZ, pco, the entropy forms and the LZ cascade) on host cores; the device
+ * then lays out and emits every frame. Every stream is bit-identical to the
+ * reference encoder at the same Config. Decoding accepts every frame mode:
+ * host-coded frames are first rewritten on host cores (and the device's
+ * recurrence kernels) into device-decodable frames.
  *
  * Threading: the library owns one HIP device context per process (device 0
  * unless quoin_gpu_create picks another). Calls are serialized per handle.
@@ -116,7 +118,7 @@ int quoin_typed_decompress(int32_t dtype, size_t n, const uint8_t *src, size_t s
 #define QUOIN_LEVEL_BALANCED 2
 #define QUOIN_LEVEL_HIGH     3
 #define QUOIN_LEVEL_MAX      4
-#define QUOIN_DEFAULT_LEVEL  QUOIN_LEVEL_FAST
+#define QUOIN_DEFAULT_LEVEL  QUOIN_LEVEL_MAX
 
 /* Selection (src/lib.rs:150-161). */
 #define QUOIN_SELECTION_FULL   0

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libquoin_amd.so")
 SOURCES = ["qa_scan.hip", "qa_encode.hip", "qa_fast.hip", "qa_fast_emit.hip", "qa_decode.hip",
            "qa_fast_dec.hip", "qa_dec_tiles.hip", "qa_rans.hip", "qa_resid.hip", "qa_host.cpp", "qa_server.cpp",
-           "qa_entropy.cpp", "qa_decimal.hip", "qa_pco.cpp", "qa_hybrid.cpp", "qa_noalloc.hip"]
+           "qa_entropy.cpp", "qa_decimal.hip", "qa_pco.cpp", "qa_hybrid.cpp", "qa_noalloc.hip", "qa_pool.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

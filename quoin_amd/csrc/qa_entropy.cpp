@@ -18,6 +18,7 @@
 
 #include "../../include/quoin.h"
 #include "qa_kernels.h"
+#include "qa_pool.h"
 
 namespace {
 
@@ -456,26 +457,13 @@ int residuals_host(const uint8_t *p, size_t len, uint64_t max_len, Bytes &out) {
 // QUOIN_ERR_PANIC for its item
 template <class F>
 void host_for(size_t n, std::vector<int> &err, F f) {
-    const unsigned hc = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const size_t nt = std::min<size_t>(n, hc);
-    std::atomic<size_t> next{0};
-    auto work = [&]() {
-        for (size_t i; (i = next.fetch_add(1)) < n;) {
-            try {
-                f(i);
-            } catch (...) {
-                err[i] = QUOIN_ERR_PANIC;
-            }
+    qapool::run(n, [&](size_t i) {
+        try {
+            f(i);
+        } catch (...) {
+            err[i] = QUOIN_ERR_PANIC;
         }
-    };
-    if (nt <= 1) {
-        work();
-        return;
-    }
-    std::vector<std::thread> th;
-    for (size_t t = 1; t < nt; t++) th.emplace_back(work);
-    work();
-    for (auto &t : th) t.join();
+    });
 }
 
 // one byte at each of the given device addresses, in kernel-argument chunks
@@ -672,3 +660,27 @@ void parallel_for(size_t n, std::vector<int> &err, const std::function<void(size
 }
 
 }  // namespace qahost
+
+// Test hooks of the host coders (no device work): code_residuals with every
+// coder on host cores, and decode_residuals. 0 = ok (*len bytes written),
+// -2 = cap below the result (*len = its size), QUOIN_ERR_CORRUPT for a stream
+// decode_residuals refuses.
+extern "C" int qa_host_code_residuals(const uint8_t *d, size_t n, uint64_t lambda, int allow_lz,
+                                      uint8_t *out, size_t cap, size_t *len) {
+    std::vector<uint8_t> o;
+    qahost::code_residuals(d, n, lambda, allow_lz != 0, o);
+    *len = o.size();
+    if (o.size() > cap) return -2;
+    if (!o.empty()) memcpy(out, o.data(), o.size());
+    return 0;
+}
+extern "C" int qa_host_decode_residuals(const uint8_t *p, size_t plen, uint64_t max_len, uint8_t *out,
+                                        size_t cap, size_t *len) {
+    std::vector<uint8_t> o;
+    const int rc = qahost::decode_residuals(p, plen, max_len, o);
+    if (rc != QUOIN_OK) return rc;
+    *len = o.size();
+    if (o.size() > cap) return -2;
+    if (!o.empty()) memcpy(out, o.data(), o.size());
+    return 0;
+}

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "qa_hostcoders.h"
+#include "qa_pool.h"
 #include "qa_hybrid.h"
 
 #include "../../include/quoin.h"
@@ -1734,7 +1735,10 @@ static int host_decompress_decimal(int32_t dtype, size_t n, const uint8_t *src, 
 extern "C" size_t quoin_typed_compress_decimal(int32_t dtype, size_t n, const void *values,
                                                const uint8_t *validity, int8_t scale,
                                                uint8_t precision, uint8_t *dst, size_t dst_cap) {
-    if (!values || !dst) return 0;
+    if (!values || !dst) {
+        t_last_err = QUOIN_ERR_NULL;
+        return 0;
+    }
     try {
         long long r = host_compress_decimal(dtype, n, values, validity, scale, precision, dst, dst_cap);
         t_last_err = r > 0 ? QUOIN_OK : r <= -1000 ? QUOIN_ERR_BUFFER_TOO_SMALL : (int)r;
@@ -1820,22 +1824,41 @@ extern "C" int quoin_decompress(const uint8_t *src, size_t src_len, double *dst,
     }
 }
 
+// capi/src/lib.rs:161-224: a context owns a persistent pool of `threads`
+// workers for the host coders of the calls made through it (0 = the
+// library's default pool)
 struct QuoinCtx {
-    size_t threads;
+    size_t threads = 0;
+    qapool::Pool *pool = nullptr;
 };
 
 extern "C" QuoinCtx *quoin_ctx_create(size_t threads) {
     QuoinCtx *c = new (std::nothrow) QuoinCtx();
-    if (c) c->threads = threads;
+    if (!c) return nullptr;
+    c->threads = threads;
+    if (threads) {
+        try {
+            c->pool = qapool::create(threads);
+        } catch (...) {
+            delete c;
+            return nullptr;
+        }
+    }
     return c;
 }
-extern "C" void quoin_ctx_free(QuoinCtx *ctx) { delete ctx; }
-extern "C" int quoin_compress_ctx(const QuoinCtx *, const double *src, size_t n, uint8_t *dst,
+extern "C" void quoin_ctx_free(QuoinCtx *ctx) {
+    if (!ctx) return;
+    qapool::destroy(ctx->pool);
+    delete ctx;
+}
+extern "C" int quoin_compress_ctx(const QuoinCtx *ctx, const double *src, size_t n, uint8_t *dst,
                                   size_t dst_cap, size_t *out_len) {
+    qapool::Use use(ctx ? ctx->pool : nullptr);
     return quoin_compress(src, n, dst, dst_cap, out_len);
 }
-extern "C" int quoin_decompress_ctx(const QuoinCtx *, const uint8_t *src, size_t src_len, double *dst,
+extern "C" int quoin_decompress_ctx(const QuoinCtx *ctx, const uint8_t *src, size_t src_len, double *dst,
                                     size_t dst_cap_values, size_t *out_values) {
+    qapool::Use use(ctx ? ctx->pool : nullptr);
     return quoin_decompress(src, src_len, dst, dst_cap_values, out_values);
 }
 

@@ -79,6 +79,32 @@ def test_null_arguments(qa):
     assert lib.quoin_typed_compress_decimal(13, 1, None, None, 2, 10, None, 0) == 0
 
 
+def test_last_error_names_the_null_argument(qa):
+    # quoin_last_error is the code behind the last 0 returned on this thread:
+    # a null argument must replace whatever an earlier call left
+    lib = qa.lib
+    v = np.arange(4, dtype=np.int64)
+    dst = np.zeros(64, dtype=np.uint8)
+    vp, dp = ctypes.c_void_p(v.ctypes.data), ctypes.c_void_p(dst.ctypes.data)
+    assert lib.quoin_typed_compress(99, 4, vp, None, dp, 64) == 0  # unknown dtype first
+    assert lib.quoin_last_error() == qa.ERR_UNSUPPORTED
+    assert lib.quoin_typed_compress(1, 4, None, None, dp, 64) == 0
+    assert lib.quoin_last_error() == qa.ERR_NULL
+    assert lib.quoin_typed_compress(99, 4, vp, None, dp, 64) == 0
+    assert lib.quoin_typed_compress_decimal(13, 4, None, None, 2, 10, dp, 64) == 0
+    assert lib.quoin_last_error() == qa.ERR_NULL
+
+
+def test_decompress_leaves_out_values_on_error(qa):
+    # capi/src/lib.rs:80-110: *out_values is written only after a good decode
+    got = ctypes.c_size_t(777)
+    junk = np.frombuffer(b"not a quoin stream", dtype=np.uint8)
+    out = np.zeros(4, dtype=np.float64)
+    rc = qa.lib.quoin_decompress(ctypes.c_void_p(junk.ctypes.data), len(junk),
+                                 ctypes.c_void_p(out.ctypes.data), 4, ctypes.byref(got))
+    assert rc == qa.ERR_CORRUPT and got.value == 777
+
+
 def test_no_gpu_fails_loudly(qa):
     import torch
     if torch.cuda.is_available():

@@ -41,7 +41,8 @@ def _check(qa, name, vals, dtype, wire, ow, validity=None, scale=2, precision=30
     print(f"case decimal {name} dtype={int(dtype)} n={n} nulls={validity is not None}", flush=True)
     got = qa.compress_decimal(raw, dtype, validity, scale, precision)
     wide = _le(vals, 16 if wire == 6 else 32)
-    want = orc.compress_decimal(wide, wire, validity, scale, precision)
+    # the typed ABI compresses the limbs at Config::default (capi/src/typed.rs:71-73)
+    want = orc.compress_decimal(wide, wire, validity, scale, precision, level=orc.MAX)
     assert got == want, f"{name}: container differs ({len(got)} vs {len(want)} B)"
     back, vb = qa.decompress_decimal(got, dtype, n, with_validity=True)
     expect = list(vals)

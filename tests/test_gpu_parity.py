@@ -182,3 +182,54 @@ def test_validity_blob_damage_matches_oracle(qa):
             ok += 1
         assert ok and bad, (ok, bad)
     assert tags == {0, 1}, tags
+
+
+def test_empty_column_validity_blob_is_checked(qa):
+    """n = 0 with the validity flag: validity::decode still reads the blob's
+    tag (src/validity.rs:73-81), so an empty blob or an unknown tag is
+    corrupt, as the oracle says, and a raw or runs tag decodes."""
+    import struct
+    hdr = b"FCR1" + bytes([2, 1, 16, orc.U32]) + struct.pack("<Q", 0)
+    for blob, ok in ((b"", False), (b"\x02", False), (b"\x81", False), (b"\x00", True),
+                     (b"\x01", True)):
+        s = hdr + bytes([len(blob)]) + blob
+        print(f"case empty-column blob {blob.hex()}", flush=True)
+        try:
+            orc.decompress(s)
+            want_ok = True
+        except ValueError:
+            want_ok = False
+        assert want_ok == ok, blob
+        if ok:
+            out, vout = qa.decompress_column(s, qa.DType.U32, 0, True)
+            assert len(out) == 0
+        else:
+            with pytest.raises(qa.QuoinError) as e:
+                qa.decompress_column(s, qa.DType.U32, 0, True)
+            assert e.value.code == qa.ERR_CORRUPT
+
+
+def test_too_small_decompress_still_decodes_first(qa):
+    """quoin_decompress with a small dst decodes first (capi/src/lib.rs:80-110):
+    a corrupt payload is CORRUPT, not BUFFER_TOO_SMALL, and *out_values is
+    written only after a good decode."""
+    import ctypes
+    v = _data.float_columns(5000, seed=3)["walk_f64"]
+    s = bytearray(qa.compress_column(v, None, qa.Config(qa.Level.FAST)))
+    out = np.zeros(10, dtype=np.float64)
+    for damage, want in ((None, qa.ERR_BUFFER_TOO_SMALL), (len(s) - 20, qa.ERR_CORRUPT)):
+        b = bytearray(s)
+        if damage is not None:
+            b = b[:damage] + bytes(20)  # zeroed tail: the last frame no longer decodes
+            try:
+                orc.decompress(bytes(b))
+                pytest.skip("damage happened to decode")
+            except ValueError:
+                pass
+        buf = np.frombuffer(bytes(b), dtype=np.uint8)
+        got = ctypes.c_size_t(99)
+        rc = qa.lib.quoin_decompress(ctypes.c_void_p(buf.ctypes.data), len(buf),
+                                     ctypes.c_void_p(out.ctypes.data), 10, ctypes.byref(got))
+        print(f"case too-small damage={damage} rc={rc}", flush=True)
+        assert rc == want
+        assert got.value == (len(v) if want == qa.ERR_BUFFER_TOO_SMALL else 99)

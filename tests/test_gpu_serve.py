@@ -145,3 +145,58 @@ def test_job_over_budget_is_refused(qa):
         assert e.value.code == qa.ERR_BUFFER_TOO_SMALL
     finally:
         srv.close()
+
+
+def test_submit_and_wait_on_two_threads_match_the_oracle(qa):
+    """One thread submits while another waits (ctypes releases the GIL for
+    both), with Fast and Max jobs mixed; every served stream is the oracle's
+    and every served decode is the oracle's decode. A second wait on a ticket
+    already waited for is refused."""
+    import queue
+    import threading
+
+    import torch
+
+    import _oracle as orc
+    odt = {np.dtype(np.float64): orc.F64, np.dtype(np.int64): orc.I64,
+           np.dtype(np.uint64): orc.U64, np.dtype(np.int32): orc.I32,
+           np.dtype(np.uint32): orc.U32, np.dtype(np.float32): orc.F32}
+    cols = columns(23)[:10]
+    levels = [qa.Level.MAX if i % 2 else qa.Level.FAST for i in range(len(cols))]
+    print(f"case serve two threads jobs={len(cols)}", flush=True)
+    bufs = [Bufs(qa, v, bm) for v, bm in cols]
+    srv = qa.Server(0, 0, 2)
+    q, results, errors = queue.Queue(), {}, []
+
+    def waiter():
+        try:
+            while True:
+                item = q.get()
+                if item is None:
+                    return
+                i, t = item
+                results[i] = srv.wait(t)
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    try:
+        tickets = []
+        for i, b in enumerate(bufs):
+            t = srv.submit_compress(b.column(qa), qa.Config(levels[i]))
+            tickets.append(t)
+            q.put((i, t))
+        q.put(None)
+        th.join(timeout=240)
+        assert not th.is_alive() and not errors, errors
+        torch.cuda.synchronize()
+        assert srv.wait(tickets[0])[0] == qa.ERR_NULL  # claimed by the waiter already
+    finally:
+        srv.close()
+    for i, (v, bm) in enumerate(cols):
+        rc, n, _ = results[i]
+        assert rc == qa.OK, (i, rc)
+        got = bufs[i].s[:n].cpu().numpy().tobytes()
+        want = orc.compress(v, odt[v.dtype], bm, int(levels[i]), threads=16)
+        assert got == want, f"job {i} (level {int(levels[i])}): stream differs from the oracle"
