@@ -1666,7 +1666,11 @@ static int host_decompress_decimal(int32_t dtype, size_t n, const uint8_t *src, 
         if (!rd_varint(src, len, &pos, &sl) || sl > len - pos) return QUOIN_ERR_CORRUPT;
         const uint8_t *sub = src + pos;
         ParsedStream ps;
-        if (parse_stream(sub, (size_t)sl, ps) != QUOIN_OK || ps.has_validity || sub[7] > 2)
+        // UNSUPPORTED here only means host-coded frames, which the decode
+        // below rewrites for the device: the limb stream's framing is sound
+        const int prc = parse_stream(sub, (size_t)sl, ps);
+        if ((prc != QUOIN_OK && prc != QUOIN_ERR_UNSUPPORTED) || (sub[5] & 0x02) || ps.has_validity ||
+            sub[7] > 2)
             return QUOIN_ERR_CORRUPT;
         std::vector<uint8_t> &y = syn[l];
         put_header(y, has_v ? 0x01 : 0x00, sub[6], 2, ln);
