@@ -4,9 +4,10 @@ Workload (one GPU): 16 Mi i64 monotone timestamps (start 1.7e18, deltas uniform
 [1, 1000]) plus three 16 Mi u32 columns uniform in [0, 2^k), k in {3, 11, 23},
 each with a Bernoulli 1 % null bitmap. All blocks of all four columns go through
 one batched launch sequence. A step = compress the four columns + decompress the
-four streams, device-resident (inputs already in HBM). The first step is checked
-bit for bit (stream bytes against the CPU oracle on the first step's sample,
-round trip in full).
+four streams, device-resident (inputs already in HBM). The first call is checked
+bit for bit: every stream byte against the CPU oracle at the same level, and
+every round trip in full. Further legs: the plain form, serving, rANS decode,
+the residual coders, the mixed-float workload and Config::default (Level::Max).
 
 Prints one JSON line: the metric, its value and the configuration, the
 roofline of the dominant kernels and the CPU baseline. --gpus > 1 exits: the
@@ -57,21 +58,71 @@ def measured_traffic(stage, args):
     return st.get("hbm_bytes_per_call")
 
 
-def cpu_baseline(cols, level, sample_n):
-    """Oracle (single-threaded C restatement) over a bounded sample."""
+def host_cores():
+    """Threads the CPU baseline uses on this host (the box's CPU share is 16
+    for one GPU) and the CPU model."""
+    n = os.cpu_count() or 1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return min(n, 16), n, model
+
+
+def oracle_round_trip(cols, level, threads):
+    """Seconds of one oracle compress + decompress of every column, input in
+    memory to output in memory (the C calls only: no Python copies of the
+    decoded values). cols: [(values, engine dtype, bitmap or None)]."""
+    import ctypes
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as orc
-    dmap = {np.dtype(np.int64): orc.I64, np.dtype(np.uint32): orc.U32}
-    raw = 0
+    lib = orc.lib
     t0 = time.perf_counter()
-    for name, v, bm in cols:
-        vs = v[:sample_n]
-        bms = bm[: (sample_n + 7) // 8] if bm is not None else None
-        s = orc.compress(vs, dmap[v.dtype], bms, level)
-        orc.decompress(s)
-        raw += vs.nbytes
-    dt = time.perf_counter() - t0
-    return raw / dt / 1e9, raw
+    for v, dt, bm in cols:
+        out, ln = ctypes.POINTER(ctypes.c_uint8)(), ctypes.c_size_t()
+        rc = lib.orc_compress_column_cfg(dt, len(v), orc._ptr(v), orc._ptr(bm), int(level), 0, 0, 16,
+                                         threads, ctypes.byref(out), ctypes.byref(ln))
+        assert rc == 0, rc
+        dtv, n, vals = ctypes.c_int(), ctypes.c_size_t(), ctypes.c_void_p()
+        vb = ctypes.POINTER(ctypes.c_uint8)()
+        rc = lib.orc_decompress_column_mt(out, ln.value, threads, ctypes.byref(dtv), ctypes.byref(n),
+                                          ctypes.byref(vals), ctypes.byref(vb))
+        assert rc == 0, rc
+        lib.orc_free(out)
+        lib.orc_free(vals)
+        if vb:
+            lib.orc_free(vb)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(cols, level, runs=5, one_thread_values=4 << 20, label=""):
+    """The oracle (the C restatement of the reference encoder and decoder,
+    blocks fanned out over threads like the reference's rayon pool) timed on
+    this host: all cores (up to 16) over the whole workload, median of `runs`;
+    and one thread over the first `one_thread_values` values of each column,
+    median of 3. cols: [(values, engine dtype, bitmap or None)]."""
+    threads, ncpu, model = host_cores()
+    raw = sum(v.nbytes for v, _, _ in cols)
+    oracle_round_trip([(v[:4096], dt, bm[:512] if bm is not None else None) for v, dt, bm in cols],
+                      level, threads)  # warm-up
+    times = sorted(oracle_round_trip(cols, level, threads) for _ in range(runs))
+    sample = [(v[:one_thread_values], dt, bm[:(one_thread_values + 7) // 8] if bm is not None else None)
+              for v, dt, bm in cols]
+    raw1 = sum(v.nbytes for v, _, _ in sample)
+    t1 = sorted(oracle_round_trip(sample, level, 1) for _ in range(3))
+    med, med1 = times[len(times) // 2], t1[1]
+    return {"value": round(raw / med / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+            "one_thread_gbs": round(raw1 / med1 / 1e9, 4), "nproc": ncpu, "cpu_model": model,
+            "runs": runs, "median_s": round(med, 4),
+            "sample": f"{label}the whole workload ({raw / 1e6:.1f} MB), compress+decompress "
+                      f"at level {int(level)}, oracle C over {threads} threads, median of {runs} "
+                      f"runs; one_thread_gbs: first {one_thread_values} values of each column "
+                      f"({raw1 / 1e6:.1f} MB), 1 thread, median of 3"}
 
 
 def serve_pool(qa, gpu, cfg, dev, count, seed=1):
@@ -481,8 +532,13 @@ def rans_cpu_baseline(sample, sample_len):
                                       "rANS decode, oracle C, 1 thread"}
 
 
-MODE_NAMES = {1: "const", 2: "stride", 3: "xorz", 5: "raw", 9: "alp", 17: "float_mult",
-              19: "dict", 50: "for_bitpack", 51: "delta_bitpack", 52: "alp_rd", 53: "rle"}
+MODE_NAMES = {0: "pred", 1: "const", 2: "stride", 3: "xorz", 4: "lz", 5: "raw",
+              7: "ordered_delta", 9: "alp", 11: "byte_transpose", 17: "float_mult", 19: "dict",
+              20: "delta2", 25: "pred2", 29: "pred_rc", 45: "delta_dp", 50: "for_bitpack",
+              51: "delta_bitpack", 52: "alp_rd", 53: "rle", 54: "pco"}
+# winners whose payload the device emits at the entropy levels; the others are
+# written by the host coders (qa_hybrid.cpp)
+DEVICE_EMITTED = {1, 2, 3, 5, 9, 50, 51, 53}
 
 
 def float_columns(total=64 << 20, ncols=40, seed=4):
@@ -558,9 +614,9 @@ def float_bench(qa, gpu, dev, steps, warmup, check_oracle=True):
     if check_oracle:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import _oracle as orc
-        for kind in ("decimal", "full", "dict"):
-            i = min((j for j in range(len(cols)) if cols[j][0] == kind), key=lambda j: len(cols[j][1]))
-            exp = orc.compress(cols[i][1], orc.F64, None, orc.FAST)
+        threads = host_cores()[0]
+        for i, (_, v) in enumerate(cols):
+            exp = orc.compress(v, orc.F64, None, orc.FAST, threads=threads)
             assert host[i].tobytes() == exp, f"mixed-float stream differs from the oracle on column {i}"
             checked.append(i)
     dcols = (qa.DeviceColumn * len(dcols))(*dcols)
@@ -610,23 +666,117 @@ def float_bench(qa, gpu, dev, steps, warmup, check_oracle=True):
     }, cols
 
 
-def float_cpu_baseline(cols, budget_values=1 << 20):
-    """Oracle compress + decompress at Level::Fast (single-threaded C) over the
-    first `budget_values` values of each kind's shortest column."""
+def float_cpu_baseline(cols):
+    """The oracle at Level::Fast over the 40 mixed-float columns."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as orc
-    sample = []
-    for kind in ("decimal", "full", "dict"):
-        i = min((j for j in range(len(cols)) if cols[j][0] == kind), key=lambda j: len(cols[j][1]))
-        sample.append(cols[i][1][:budget_values])
+    return cpu_baseline([(v, orc.F64, None) for _, v in cols], orc.FAST, runs=3,
+                        one_thread_values=256 << 10, label="mixed float: ")
+
+
+def default_workload(seed=0):
+    """Columns of the Config::default leg: the plain-method column (1 Mi f64
+    2-decimal random walk, seed 0), and reduced-size columns of the integer
+    lanes (1 Mi per column) and entropy (2 Mi each) workloads."""
+    r = np.random.default_rng(seed)
+    n = 1 << 20
+    walk = np.round(100 * (1000 + np.cumsum(r.normal(size=n)))) / 100
+    cols = [("walk_f64 (1Mi, plain method)", walk, None)]
+    for name, v, bm in make_workload(1 << 20, seed=1):
+        cols.append((name + " (1Mi)", v, bm))
+    m = 2 << 20
+    i = np.arange(m, dtype=np.float64)
+    cols.append(("sine_f64 (2Mi)", np.sin(2 * np.pi * i / 1000) * 1e3 + r.normal(0, 1e-3, m), None))
+    cols.append(("unif_f32 (2Mi)", r.uniform(-1, 1, m).astype(np.float32), None))
+    return cols
+
+
+def default_bench(qa, gpu, dev, steps, check_oracle=True):
+    """Config::default (Level::Max) over default_workload(): the device plans
+    and sizes, writes the residual streams and emits its own modes; host cores
+    run the rest of the competition. One batched compress call and one batched
+    decompress call per step, device-resident. The first call's streams are
+    checked against the oracle at Level::Max byte for byte, and every round
+    trip in full. Reports GB/s, the per-mode wins, and the split of blocks
+    between device-emitted and host-written payloads."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as orc
+    cols = default_workload()
+    cfg = qa.Config()  # Config::default
+    odt = {np.dtype(np.float64): orc.F64, np.dtype(np.int64): orc.I64,
+           np.dtype(np.uint32): orc.U32, np.dtype(np.float32): orc.F32}
+    dv, ds, do, dvo, dcols = [], [], [], [], []
+    raw = 0
+    for name, v, bm in cols:
+        dt = qa.FROM_NP[v.dtype]
+        t = torch.from_numpy(v.view(np.uint8).copy()).to(dev)
+        b = torch.from_numpy(bm.copy()).to(dev) if bm is not None else None
+        cap = qa.lib.quoin_typed_compress_bound(int(dt), len(v))
+        st = torch.empty(cap, dtype=torch.uint8, device=dev)
+        dv.append((t, b))
+        ds.append(st)
+        do.append(torch.empty(v.nbytes, dtype=torch.uint8, device=dev))
+        dvo.append(torch.empty(max((len(v) + 7) // 8, 1), dtype=torch.uint8, device=dev)
+                   if bm is not None else None)
+        dcols.append(qa.DeviceColumn(int(dt), 0, len(v), t.data_ptr(),
+                                     b.data_ptr() if b is not None else None, st.data_ptr(), cap))
+        raw += v.nbytes
+    torch.cuda.synchronize()
+    qa.reset_mode_win_counts()
     t0 = time.perf_counter()
-    for v in sample:
-        orc.decompress(orc.compress(v, orc.F64, None, orc.FAST))
-    dt = time.perf_counter() - t0
-    rawb = sum(v.nbytes for v in sample)
-    return {"value": round(rawb / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"the shortest column of each kind ({rawb / 1e6:.1f} MB), compress+decompress "
-                      "at Level::Fast, oracle C, 1 thread"}
+    lens = gpu.compress(dcols, cfg)
+    first_c = time.perf_counter() - t0
+    wins = qa.mode_win_counts()
+    host = [ds[i][:lens[i]].cpu().numpy() for i in range(len(cols))]
+    dsts = [qa.DeviceStream(int(qa.FROM_NP[v.dtype]), 0, len(v), ds[i].data_ptr(), host[i].ctypes.data,
+                            lens[i], do[i].data_ptr(), dvo[i].data_ptr() if dvo[i] is not None else None)
+            for i, (_, v, _) in enumerate(cols)]
+    gpu.decompress(dsts)
+    torch.cuda.synchronize()
+    for i, (name, v, bm) in enumerate(cols):
+        assert do[i].cpu().numpy().tobytes() == v.view(np.uint8).tobytes(), f"round trip: {name}"
+    checked = []
+    if check_oracle:
+        threads = host_cores()[0]
+        for i, (name, v, bm) in enumerate(cols):
+            exp = orc.compress(v, odt[v.dtype], bm, orc.MAX, threads=threads)
+            assert host[i].tobytes() == exp, f"Config::default stream differs from the oracle: {name}"
+            checked.append(name)
+    dcols = (qa.DeviceColumn * len(dcols))(*dcols)
+    dsts = (qa.DeviceStream * len(dsts))(*dsts)
+    tc = td = 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        gpu.compress(dcols, cfg)
+        t1 = time.perf_counter()
+        gpu.decompress(dsts)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tc += t1 - t0
+        td += t2 - t1
+    comp = int(sum(lens))
+    blocks = int(sum(wins))
+    dev_blocks = int(sum(wins[m] for m in DEVICE_EMITTED))
+    per_col = []
+    for i, (name, v, bm) in enumerate(cols):
+        per_col.append({"column": name, "values": len(v), "bytes": int(v.nbytes),
+                        "compressed": int(lens[i]), "ratio": round(v.nbytes / max(lens[i], 1), 3)})
+    return {
+        "workload": "Config::default (Level::Max, adaptive blocks): 1Mi f64 2-decimal walk (plain "
+                    "method), the integer-lanes columns at 1Mi each (1% nulls on the u32 columns), "
+                    "2Mi f64 sine+noise and 2Mi f32 uniform; one batched call per direction",
+        "columns": per_col, "uncompressed_bytes": raw, "compressed_bytes": comp,
+        "ratio": round(raw / comp, 3),
+        "compress_gbs": round(raw * steps / tc / 1e9, 4),
+        "decompress_gbs": round(raw * steps / td / 1e9, 4),
+        "round_trip_gbs": round(raw * steps / (tc + td) / 1e9, 4),
+        "first_compress_s": round(first_c, 3),
+        "wins": {MODE_NAMES.get(m, str(m)): int(wins[m]) for m in range(64) if wins[m]},
+        "routing": {"blocks": blocks, "device_emitted": dev_blocks, "host_written": blocks - dev_blocks,
+                    "host_share": round((blocks - dev_blocks) / max(blocks, 1), 4)},
+        "oracle_checked_columns": checked,
+    }, [(v, odt[v.dtype], bm) for _, v, bm in cols]
 
 
 def main():
@@ -636,7 +786,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=16 << 20)
     ap.add_argument("--level", default="fastest")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 20)
+    ap.add_argument("--cpu-sample", type=int, default=4 << 20,
+                    help="values per column of the 1-thread CPU figure")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="steps of the plain form (one launch sequence per block); 0 skips it")
@@ -646,6 +797,9 @@ def main():
                     help="values per column of the rANS decode leg")
     ap.add_argument("--no-resid", action="store_true", help="skip the residual coder leg")
     ap.add_argument("--no-float", action="store_true", help="skip the mixed-float leg")
+    ap.add_argument("--no-default", action="store_true",
+                    help="skip the Config::default (Level::Max) leg")
+    ap.add_argument("--default-steps", type=int, default=2)
     ap.add_argument("--resid-n", type=int, default=32 << 20,
                     help="f64 values of the residual coder leg")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -660,7 +814,8 @@ def main():
     import torch
     import quoin_amd as qa
 
-    level = {"fastest": qa.Level.FASTEST, "fast": qa.Level.FAST}[args.level]
+    level = {"fastest": qa.Level.FASTEST, "fast": qa.Level.FAST, "balanced": qa.Level.BALANCED,
+             "high": qa.Level.HIGH, "max": qa.Level.MAX}[args.level]
     cfg = qa.Config(level=level)
     dev = torch.device("cuda:0")
     cols = make_workload(args.n)
@@ -695,7 +850,13 @@ def main():
                                     dvouts[i].data_ptr() if dvouts[i] is not None else None))
     gpu.decompress(dsts)
     torch.cuda.synchronize()
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as orc
+    odt = {np.dtype(np.int64): orc.I64, np.dtype(np.uint32): orc.U32}
     for i, (name, v, bm) in enumerate(cols):
+        # every stream byte for byte against the oracle at the same level
+        exp = orc.compress(v, odt[v.dtype], bm, int(level), threads=host_cores()[0])
+        assert host_streams[i].tobytes() == exp, f"stream differs from the oracle on {name}"
         got = douts[i].cpu().numpy()
         assert got.tobytes() == v.view(np.uint8).tobytes(), f"round trip mismatch on {name}"
         if bm is not None:
@@ -808,12 +969,18 @@ def main():
     if not args.no_float:
         floatmix, float_cols = float_bench(qa, gpu, dev, 5, 1)
 
+    default = default_cols = None
+    if not args.no_default:
+        default, default_cols = default_bench(qa, gpu, dev, args.default_steps)
+
     cpu = None
     if not args.no_cpu:
-        cpu_gbs, sample_raw = cpu_baseline(cols, int(level), args.cpu_sample)
-        cpu = {"value": round(cpu_gbs, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-               "sample": f"first {args.cpu_sample} values of each of the 4 columns "
-                         f"({sample_raw / 1e6:.1f} MB), compress+decompress, oracle C, 1 thread"}
+        cpu = cpu_baseline([(v, odt[v.dtype], bm) for _, v, bm in cols], level, runs=5,
+                           one_thread_values=args.cpu_sample, label="integer lanes: ")
+        if default is not None:
+            default["cpu_baseline"] = cpu_baseline(default_cols, qa.Level.MAX, runs=3,
+                                                   one_thread_values=256 << 10,
+                                                   label="Config::default leg: ")
         if entropy is not None:
             entropy["cpu_baseline"] = rans_cpu_baseline(*ent_sample)
         if resid is not None:
@@ -843,6 +1010,7 @@ def main():
         "entropy_decode": entropy,
         "residual_coders": resid,
         "mixed_float": floatmix,
+        "config_default": default,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

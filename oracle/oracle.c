@@ -103,14 +103,6 @@ static void varint_write(buf_t *b, uint64_t v) {
         }
     }
 }
-static size_t varint_len(uint64_t v) {
-    size_t n = 1;
-    while (v >= 0x80) {
-        v >>= 7;
-        n++;
-    }
-    return n;
-}
 /* varint.rs:24-39 */
 static int varint_read(const uint8_t *in, size_t len, size_t *pos, uint64_t *out) {
     uint64_t result = 0;
@@ -2863,6 +2855,34 @@ static int decode_frame(int mode, const uint8_t *p, size_t len, size_t n, int dt
 }
 
 /* decoder.rs:43-118 */
+/* phase 2 of the decoder over a pool of threads (the rayon fan-out of
+ * decoder.rs:120-131): frames claimed in order, each decoded into its slice */
+typedef struct {
+    const uint8_t *src;
+    const size_t *off, *cnt, *at;
+    size_t nf, next;
+    int dt, plog2, rc;
+    uint64_t *lane;
+    pthread_mutex_t mu;
+} frame_job_t;
+static void *frame_worker(void *arg) {
+    frame_job_t *j = (frame_job_t *)arg;
+    for (;;) {
+        pthread_mutex_lock(&j->mu);
+        size_t f = j->next++;
+        pthread_mutex_unlock(&j->mu);
+        if (f >= j->nf) return NULL;
+        const uint8_t *h = j->src + j->off[f];
+        int rc = decode_frame(h[0], h + 9, rd_u32le(h + 5), j->cnt[f], j->dt, j->plog2, j->lane + j->at[f]);
+        if (rc) {
+            pthread_mutex_lock(&j->mu);
+            if (!j->rc) j->rc = rc;
+            pthread_mutex_unlock(&j->mu);
+        }
+    }
+}
+static __thread int t_dec_threads = 0;
+
 int orc_decompress_column(const uint8_t *src, size_t len, int *dtype, size_t *nout,
                           void **values, uint8_t **validity) {
     if (len < 16) return ORC_TRUNCATED;
@@ -2932,6 +2952,49 @@ int orc_decompress_column(const uint8_t *src, size_t len, int *dtype, size_t *no
         }
     }
     if (rc == ORC_OK && cnt != nt) rc = ORC_LENGTH_MISMATCH;
+    int nthreads = t_dec_threads > 1 ? t_dec_threads : 1;
+    if (rc == ORC_OK && nthreads > 1) {
+        size_t nf = 0, cap = 64, p2 = pos, c2 = 0;
+        size_t *off = (size_t *)malloc(cap * sizeof(size_t)), *fc = (size_t *)malloc(cap * sizeof(size_t)),
+               *at = (size_t *)malloc(cap * sizeof(size_t));
+        while (c2 < nt) {
+            if (nf == cap) {
+                cap *= 2;
+                off = (size_t *)realloc(off, cap * sizeof(size_t));
+                fc = (size_t *)realloc(fc, cap * sizeof(size_t));
+                at = (size_t *)realloc(at, cap * sizeof(size_t));
+            }
+            off[nf] = p2;
+            fc[nf] = rd_u32le(src + p2 + 1);
+            at[nf] = c2;
+            c2 += fc[nf];
+            p2 += 9 + rd_u32le(src + p2 + 5);
+            nf++;
+        }
+        frame_job_t job;
+        memset(&job, 0, sizeof job);
+        job.src = src;
+        job.off = off;
+        job.cnt = fc;
+        job.at = at;
+        job.nf = nf;
+        job.dt = dt;
+        job.plog2 = src[6];
+        job.rc = ORC_OK;
+        job.lane = lane;
+        pthread_mutex_init(&job.mu, NULL);
+        int nth = nthreads < (int)nf ? nthreads : (int)nf;
+        pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (nth ? nth : 1));
+        for (int t = 0; t < nth; t++) pthread_create(&th[t], NULL, frame_worker, &job);
+        for (int t = 0; t < nth; t++) pthread_join(th[t], NULL);
+        free(th);
+        pthread_mutex_destroy(&job.mu);
+        rc = job.rc;
+        counted = nt;
+        free(off);
+        free(fc);
+        free(at);
+    }
     while (rc == ORC_OK && counted < nt) {
         int mode = src[pos];
         size_t fn = rd_u32le(src + pos + 1), pl = rd_u32le(src + pos + 5);
@@ -2967,6 +3030,15 @@ int orc_decompress_column(const uint8_t *src, size_t len, int *dtype, size_t *no
     *values = vals;
     *validity = bm;
     return ORC_OK;
+}
+
+/* orc_decompress_column with its frame decode over `threads` threads */
+int orc_decompress_column_mt(const uint8_t *src, size_t len, int threads, int *dtype, size_t *nout,
+                             void **values, uint8_t **validity) {
+    t_dec_threads = threads;
+    int rc = orc_decompress_column(src, len, dtype, nout, values, validity);
+    t_dec_threads = 0;
+    return rc;
 }
 
 void orc_free(void *p) { free(p); }

@@ -100,6 +100,9 @@ def _load():
     for f in ("orc_pack32", "orc_unpack32", "orc_pack64", "orc_unpack64"):
         getattr(lib, f).argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
     lib.orc_mode_wins.argtypes = [ctypes.c_void_p]
+    lib.orc_decompress_column_mt.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p]
     lib.orc_block_feats.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
     return lib
 
@@ -129,15 +132,22 @@ def compress(values, dtype, validity=None, level=FAST, block_size=0, selection=0
     return b
 
 
-def decompress(stream):
-    """Returns (dtype, values ndarray, validity ndarray|None) or raises."""
+def decompress(stream, threads=1):
+    """Returns (dtype, values ndarray, validity ndarray|None) or raises.
+    threads > 1 decodes the frames over that many threads."""
     buf = np.frombuffer(stream, dtype=np.uint8)
     dt = ctypes.c_int()
     n = ctypes.c_size_t()
     vals = ctypes.c_void_p()
     vb = ctypes.POINTER(ctypes.c_uint8)()
-    rc = lib.orc_decompress_column(_ptr(buf) if len(buf) else None, len(buf), ctypes.byref(dt),
-                                   ctypes.byref(n), ctypes.byref(vals), ctypes.byref(vb))
+    if threads > 1:
+        rc = lib.orc_decompress_column_mt(_ptr(buf) if len(buf) else None, len(buf), threads,
+                                          ctypes.byref(dt), ctypes.byref(n), ctypes.byref(vals),
+                                          ctypes.byref(vb))
+    else:
+        rc = lib.orc_decompress_column(_ptr(buf) if len(buf) else None, len(buf),
+                                       ctypes.byref(dt), ctypes.byref(n), ctypes.byref(vals),
+                                       ctypes.byref(vb))
     if rc != 0:
         raise ValueError(f"oracle decompress rc={rc}")
     npdt = NP_DTYPE[dt.value]

@@ -1457,3 +1457,31 @@ def test_lz_cascade_applies_to_the_top_two_at_max_only():
                                       "LZ", "BYTE_TRANSPOSE")}
         return any(m in ent and p[:1] == b"\x03" for m, _, p in orc.frames(st))
     assert not lz_tagged(hi)
+
+
+def test_threaded_frame_decode_matches_the_sequential_one():
+    """The oracle's multi-threaded frame decode (the baseline's decoder) gives
+    the single-threaded decode's values, validity and errors."""
+    r = np.random.default_rng(2)
+    n = 300000
+    walk = np.round(100 * (1000 + np.cumsum(r.normal(size=n)))) / 100
+    u = r.integers(0, 1 << 11, n).astype(np.uint32)
+    valid = r.random(n) >= 0.01
+    u[~valid] = 0
+    bm = np.packbits(valid.astype(np.uint8), bitorder="little")
+    for vals, dt, v in ((walk, orc.F64, None), (u, orc.U32, bm)):
+        for lv in (orc.FASTEST, orc.MAX):
+            s = orc.compress(vals, dt, v, lv, threads=8)
+            a = orc.decompress(s)
+            b = orc.decompress(s, threads=8)
+            assert a[0] == b[0] and a[1].tobytes() == b[1].tobytes()
+            assert (a[2] is None and b[2] is None) or a[2].tobytes() == b[2].tobytes()
+            bad = bytearray(s)
+            bad[len(bad) // 2] ^= 0xFF
+            outs = []
+            for th in (1, 8):
+                try:
+                    outs.append(orc.decompress(bytes(bad), threads=th)[1].tobytes())
+                except ValueError:
+                    outs.append(None)
+            assert outs[0] == outs[1]
