@@ -739,7 +739,8 @@ __global__ void __launch_bounds__(256) k_block_decide(EncArgs a) {
     uint64_t sz[C_N];
     for (int k = 0; k < C_N; k++) sz[k] = ~0ull;
     sz[C_RAW] = raw;
-    if (!(!fl && b.incompressible)) {
+    // Selection::Sample (hybrid 2) sizes every block whatever its features
+    if (a.hybrid == 2 || !(!fl && b.incompressible)) {
         sz[C_CONST] = cst ? 8 : ~0ull;
         sz[C_STRIDE] = (n >= 2 && str) ? 16 : ~0ull;
         sz[C_XORZ] = reduced ? ~0ull : xz;
@@ -757,8 +758,9 @@ __global__ void __launch_bounds__(256) k_block_decide(EncArgs a) {
         b.v1 = v1;
         for (int k = 0; k < C_N; k++) b.sz[k] = sz[k];
         // entropy levels: the host decides the gate after its own candidates
-        b.compressible = a.hybrid ? !(!fl && b.incompressible)
-                                  : best_len * 20 < (n * 8) * 19 && !(!fl && b.incompressible);
+        b.compressible = a.hybrid == 2 ? 1
+                         : a.hybrid ? !(!fl && b.incompressible)
+                                    : best_len * 20 < (n * 8) * 19 && !(!fl && b.incompressible);
         b.mode = mode;
         b.payload_len = best_len;
         b.score = best_score;

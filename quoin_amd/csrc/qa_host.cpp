@@ -117,6 +117,9 @@ struct QuoinGpu {
     // Levels Balanced / High / Max: lane words, residual streams, the forced
     // modes, the host-written frames and their payload bytes
     DevBuf hlane, hres, hforce, hscat, hsrc;
+    // Selection::Sample's sample pass runs on a second handle of the same
+    // device (its own arenas), created on first use
+    QuoinGpu *aux = nullptr;
     // decimal container: partial minima, vmin and limbs
     DevBuf dm;
     // autotuned launch shapes of the tile kernels (QA_TUNE_*, 0 = untuned)
@@ -374,6 +377,7 @@ extern "C" QuoinGpu *quoin_gpu_create(int device, uint64_t hbm_budget_bytes) {
 
 extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
     if (!g) return;
+    quoin_gpu_destroy(g->aux);
     hipSetDevice(g->device);
     hipStreamSynchronize(g->stream);
     if (g->hstage.p) hipHostFree(g->hstage.p);
@@ -419,52 +423,63 @@ extern "C" int quoin_gpu_last_timing(const QuoinGpu *g, double *total_ms, double
     return QUOIN_OK;
 }
 
-// Levels Balanced / High / Max (src/encoder.rs:275-469). The device plans the
-// blocks and sizes the candidates it emits itself; at High / Max it also
-// writes the FCM / DFCM / Delta2 / DeltaDp / OrderedDelta residual streams.
-// Host cores run the rest of the competition per block (entropy forms, range
-// coder and rANS, LZ, ByteTranspose, pco, the LZ cascade). The device then
-// lays out every frame, emits the winners of its own modes, and
-// k_host_frames copies the host-coded payloads into their frames.
-static int hybrid_encode(QuoinGpu *g, EncArgs &a, const std::vector<DCol> &cols, int level,
-                         uint64_t lane_total, uint64_t *out_lens, hipStream_t s) {
+// The host side of one hybrid encode pass: the device's per-block results
+// read back, the lane words, the residual streams, and the per-block inputs
+// of the host competition (pointing into the vectors above them).
+struct HybridPrep {
+    std::vector<DBlock> blk;
+    std::vector<DBlockX> bx;
+    std::vector<DColState> st;
+    std::vector<uint64_t> lanes;
+    std::vector<uint32_t> slots;  // region slot of each block
+    std::vector<uint8_t> hres[5];
+    std::vector<qahyb::EncBlock> eb;
+};
+
+// Levels Balanced / High / Max and Selection::Sample (src/encoder.rs:275-641),
+// part 1. The device plans the blocks and sizes the candidates it emits
+// itself; at High / Max it also writes the FCM / DFCM / Delta2 / DeltaDp /
+// OrderedDelta residual streams (predictor tables of 2^plog2 entries).
+// a.hybrid: 1 = the Full competition's gates, 2 = sizes for every block
+// whatever its features (Selection::Sample).
+static int hybrid_prepare(QuoinGpu *g, EncArgs &a, const std::vector<DCol> &cols, int level, uint32_t plog2,
+                          uint64_t lane_total, hipStream_t s, HybridPrep &P) {
     const size_t ncols = cols.size();
     const uint32_t nr = a.nregions;
-    a.hybrid = 1;
     a.force = nullptr;
     qa_encode_plan_launch(a, s);
     qa_encode_blocks_launch(a, s);
     if (!ensure(g->hlane, lane_total * 8)) return QUOIN_ERR_DEVICE;
     uint64_t *dlane = dp<uint64_t>(g->hlane);
     qa_lane_words_launch(a, lane_total, dlane, s);
-    std::vector<DBlock> blk(nr);
-    std::vector<DBlockX> bx(nr);
-    std::vector<DColState> st(ncols);
-    std::vector<uint64_t> lanes(lane_total);
+    P.blk.resize(nr);
+    P.bx.resize(nr);
+    P.st.resize(ncols);
+    P.lanes.resize(lane_total);
     bool ok = true;
     if (nr) {
-        ok &= hipMemcpyAsync(blk.data(), g->blocks.p, nr * sizeof(DBlock), hipMemcpyDeviceToHost, s) ==
+        ok &= hipMemcpyAsync(P.blk.data(), g->blocks.p, nr * sizeof(DBlock), hipMemcpyDeviceToHost, s) ==
               hipSuccess;
-        ok &= hipMemcpyAsync(bx.data(), g->blocksx.p, nr * sizeof(DBlockX), hipMemcpyDeviceToHost, s) ==
+        ok &= hipMemcpyAsync(P.bx.data(), g->blocksx.p, nr * sizeof(DBlockX), hipMemcpyDeviceToHost, s) ==
               hipSuccess;
     }
-    ok &= hipMemcpyAsync(st.data(), g->cst.p, ncols * sizeof(DColState), hipMemcpyDeviceToHost, s) ==
+    ok &= hipMemcpyAsync(P.st.data(), g->cst.p, ncols * sizeof(DColState), hipMemcpyDeviceToHost, s) ==
           hipSuccess;
     if (lane_total)
-        ok &= hipMemcpyAsync(lanes.data(), dlane, lane_total * 8, hipMemcpyDeviceToHost, s) == hipSuccess;
+        ok &= hipMemcpyAsync(P.lanes.data(), dlane, lane_total * 8, hipMemcpyDeviceToHost, s) == hipSuccess;
     if (!ok || hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
         return QUOIN_ERR_DEVICE;
-    std::vector<uint32_t> slots;
+    P.slots.clear();
     for (uint32_t r = 0; r < nr; r++)
-        if (blk[r].len && !st[blk[r].col].err) slots.push_back(r);
-    const size_t nb = slots.size();
-    std::vector<qahyb::EncBlock> eb(nb);
+        if (P.blk[r].len && !P.st[P.blk[r].col].err) P.slots.push_back(r);
+    const size_t nb = P.slots.size();
+    P.eb.assign(nb, qahyb::EncBlock{});
     for (size_t k = 0; k < nb; k++) {
-        const DBlock &b = blk[slots[k]];
-        const DBlockX &x = bx[slots[k]];
+        const DBlock &b = P.blk[P.slots[k]];
+        const DBlockX &x = P.bx[P.slots[k]];
         const DCol &c = cols[b.col];
-        qahyb::EncBlock &e = eb[k];
-        e.v = lanes.data() + c.lane_off + b.start;
+        qahyb::EncBlock &e = P.eb[k];
+        e.v = P.lanes.data() + c.lane_off + b.start;
         e.n = b.len;
         e.dt = c.dtype;
         for (int j = 0; j < C_N; j++) e.sz[j] = b.sz[j];
@@ -480,25 +495,23 @@ static int hybrid_encode(QuoinGpu *g, EncArgs &a, const std::vector<DCol> &cols,
         }
     }
     // High / Max: the residual streams of the five predictor-family modes
-    std::vector<uint8_t> hres[5];
     if (level >= QUOIN_LEVEL_HIGH && nb) {
         std::vector<uint64_t> roff(nb + 1, 0);
-        for (size_t k = 0; k < nb; k++) roff[k + 1] = roff[k] + quoin_linear_bound(eb[k].n);
+        for (size_t k = 0; k < nb; k++) roff[k + 1] = roff[k] + quoin_linear_bound(P.eb[k].n);
         if (!ensure(g->hres, roff[nb])) return QUOIN_ERR_DEVICE;
         uint8_t *dres = dp<uint8_t>(g->hres);
-        const uint32_t plog2 = cols[0].plog2;
         for (int j = 0; j < 5; j++) {
             const bool float_only = j == 2 || j == 3;
             std::vector<size_t> idx;
             for (size_t k = 0; k < nb; k++)
-                if (!float_only || eb[k].dt == QA_F64 || eb[k].dt == QA_F32) idx.push_back(k);
+                if (!float_only || P.eb[k].dt == QA_F64 || P.eb[k].dt == QA_F32) idx.push_back(k);
             if (idx.empty()) continue;
             std::vector<QuoinLinearInput> in(idx.size());
             for (size_t i = 0; i < idx.size(); i++) {
                 const size_t k = idx[i];
-                const DBlock &b = blk[slots[k]];
-                in[i] = QuoinLinearInput{dlane + cols[b.col].lane_off + b.start, eb[k].n, dres + roff[k],
-                                         quoin_linear_bound(eb[k].n)};
+                const DBlock &b = P.blk[P.slots[k]];
+                in[i] = QuoinLinearInput{dlane + cols[b.col].lane_off + b.start, P.eb[k].n, dres + roff[k],
+                                         quoin_linear_bound(P.eb[k].n)};
             }
             std::vector<uint64_t> lens(idx.size(), 0);
             std::vector<int32_t> status(idx.size(), 0);
@@ -521,48 +534,53 @@ static int hybrid_encode(QuoinGpu *g, EncArgs &a, const std::vector<DCol> &cols,
                 if (rc != QUOIN_OK) return rc;
                 i0 = i1;
             }
-            hres[j].resize(roff[nb]);
-            if (hipMemcpyAsync(hres[j].data(), dres, roff[nb], hipMemcpyDeviceToHost, s) != hipSuccess ||
+            P.hres[j].resize(roff[nb]);
+            if (hipMemcpyAsync(P.hres[j].data(), dres, roff[nb], hipMemcpyDeviceToHost, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess)
                 return QUOIN_ERR_DEVICE;
             for (size_t i = 0; i < idx.size(); i++) {
-                qahyb::EncBlock &e = eb[idx[i]];
+                qahyb::EncBlock &e = P.eb[idx[i]];
                 if (status[i]) continue;  // dp_encode's None
-                e.res[j] = hres[j].data() + roff[idx[i]];
+                e.res[j] = P.hres[j].data() + roff[idx[i]];
                 e.rlen[j] = lens[i];
                 e.has_res[j] = true;
             }
         }
     }
-    std::vector<int> err(nb, QUOIN_OK);
-    qahost::parallel_for(nb, err, [&](size_t k) { qahyb::compete(eb[k], level); });
-    for (int e : err)
-        if (e != QUOIN_OK) return e;
-    // the winners: forced modes for the layout, host payloads for the scatter
+    return QUOIN_OK;
+}
+
+// Part 3, once the host has decided every block (P.eb[k].mode / host /
+// payload / len): the device lays out every frame, emits the winners of its
+// own modes, and k_host_frames copies the host-coded payloads into theirs.
+static int hybrid_emit(QuoinGpu *g, EncArgs &a, size_t ncols, HybridPrep &P, uint64_t *out_lens,
+                       hipStream_t s) {
+    const uint32_t nr = a.nregions;
+    const size_t nb = P.slots.size();
     std::vector<HForce> force(nr ? nr : 1, HForce{QA_M_RAW, 0, 0});
     std::vector<HScatter> sc;
     uint64_t hbytes = 0;
     for (size_t k = 0; k < nb; k++) {
-        const qahyb::EncBlock &e = eb[k];
+        const qahyb::EncBlock &e = P.eb[k];
         if (e.len > 0xFFFFFFFFull) return QUOIN_ERR_PANIC;  // the frame's u32 payload length
-        force[slots[k]] = HForce{(uint32_t)e.mode, e.host ? 1u : 0u, e.len};
+        force[P.slots[k]] = HForce{(uint32_t)e.mode, e.host ? 1u : 0u, e.len};
         if (e.host) {
-            sc.push_back(HScatter{slots[k], (uint32_t)e.mode, hbytes, e.len});
+            sc.push_back(HScatter{P.slots[k], (uint32_t)e.mode, hbytes, e.len});
             hbytes += e.len;
         }
         g_wins[e.mode & 63]++;
     }
     std::vector<uint8_t> pay(hbytes ? hbytes : 1);
     for (size_t k = 0, o = 0; k < nb; k++)
-        if (eb[k].host) {
-            memcpy(pay.data() + o, eb[k].payload.data(), eb[k].len);
-            o += eb[k].len;
+        if (P.eb[k].host) {
+            memcpy(pay.data() + o, P.eb[k].payload.data(), P.eb[k].len);
+            o += P.eb[k].len;
         }
     if (!ensure(g->hforce, force.size() * sizeof(HForce)) ||
         !ensure(g->hscat, (sc.size() + 1) * sizeof(HScatter)) || !ensure(g->hsrc, pay.size()))
         return QUOIN_ERR_DEVICE;
-    ok = hipMemcpyAsync(g->hforce.p, force.data(), force.size() * sizeof(HForce), hipMemcpyHostToDevice, s) ==
-         hipSuccess;
+    bool ok = hipMemcpyAsync(g->hforce.p, force.data(), force.size() * sizeof(HForce), hipMemcpyHostToDevice,
+                             s) == hipSuccess;
     if (!sc.empty())
         ok &= hipMemcpyAsync(g->hscat.p, sc.data(), sc.size() * sizeof(HScatter), hipMemcpyHostToDevice, s) ==
               hipSuccess;
@@ -574,7 +592,7 @@ static int hybrid_encode(QuoinGpu *g, EncArgs &a, const std::vector<DCol> &cols,
     qa_encode_emit_launch(a, s);
     qa_host_frames_launch(a, dp<HScatter>(g->hscat), (uint32_t)sc.size(), dp<uint8_t>(g->hsrc), s);
     hipEventRecord(g->ev1, s);
-    if (hipMemcpyAsync(st.data(), g->cst.p, ncols * sizeof(DColState), hipMemcpyDeviceToHost, s) !=
+    if (hipMemcpyAsync(P.st.data(), g->cst.p, ncols * sizeof(DColState), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
         return QUOIN_ERR_DEVICE;
@@ -585,20 +603,52 @@ static int hybrid_encode(QuoinGpu *g, EncArgs &a, const std::vector<DCol> &cols,
     g->last_kernel = "";
     int rc = QUOIN_OK;
     for (size_t c = 0; c < ncols; c++) {
-        out_lens[c] = st[c].stream_len;
-        if (st[c].err & QA_ERR_OVERFLOW) rc = QUOIN_ERR_BUFFER_TOO_SMALL;
-        else if (st[c].err) rc = QUOIN_ERR_PANIC;
+        out_lens[c] = P.st[c].stream_len;
+        if (P.st[c].err & QA_ERR_OVERFLOW) rc = QUOIN_ERR_BUFFER_TOO_SMALL;
+        else if (P.st[c].err) rc = QUOIN_ERR_PANIC;
     }
     return rc;
 }
 
+static int sample_pass(QuoinGpu *g, int level, const HybridPrep &P, const std::vector<DCol> &cols,
+                       HybridPrep &S, hipStream_t s);
+
+// The whole hybrid encode: prepare, the host decisions (the Full competition,
+// or Selection::Sample's sampled choice), emission.
+static int hybrid_encode(QuoinGpu *g, EncArgs &a, const std::vector<DCol> &cols, int level, uint32_t plog2,
+                         bool sample, uint64_t lane_total, uint64_t *out_lens, hipStream_t s) {
+    HybridPrep P;
+    a.hybrid = sample ? 2 : 1;
+    int rc = hybrid_prepare(g, a, cols, level, plog2, lane_total, s, P);
+    if (rc != QUOIN_OK) return rc;
+    const size_t nb = P.slots.size();
+    std::vector<int> err(nb, QUOIN_OK);
+    if (!sample) {
+        qahost::parallel_for(nb, err, [&](size_t k) { qahyb::compete(P.eb[k], level); });
+    } else {
+        HybridPrep S;
+        rc = sample_pass(g, level, P, cols, S, s);
+        if (rc != QUOIN_OK) return rc;
+        qahost::parallel_for(nb, err, [&](size_t k) { qahyb::compete_sampled(P.eb[k], S.eb[k], level); });
+    }
+    for (int e : err)
+        if (e != QUOIN_OK) return e;
+    return hybrid_emit(g, a, cols.size(), P, out_lens, s);
+}
+
+// prep_only: run the hybrid pass's device part and read back its per-block
+// inputs into *prep_only, emitting nothing (the sample pass of
+// Selection::Sample: every block sized whatever its features)
 static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinDeviceColumn *in,
-                               size_t ncols, uint64_t *out_lens, hipStream_t s) {
+                               size_t ncols, uint64_t *out_lens, hipStream_t s,
+                               HybridPrep *prep_only = nullptr) {
     int level = cfg ? cfg->level : QUOIN_DEFAULT_LEVEL;
     if (level < 0 || level > QUOIN_LEVEL_MAX) return QUOIN_ERR_UNSUPPORTED;
     if (!level_supported(level)) return QUOIN_ERR_UNSUPPORTED;
-    if (cfg && cfg->selection != QUOIN_SELECTION_FULL) return QUOIN_ERR_UNSUPPORTED;
-    const bool hybrid = level >= QUOIN_LEVEL_BALANCED;
+    const int selection = cfg ? cfg->selection : QUOIN_SELECTION_FULL;
+    if (selection != QUOIN_SELECTION_FULL && selection != QUOIN_SELECTION_SAMPLE) return QUOIN_ERR_UNSUPPORTED;
+    const bool sample = selection == QUOIN_SELECTION_SAMPLE;
+    const bool hybrid = level >= QUOIN_LEVEL_BALANCED || sample || prep_only;
     // Config::predictor_log2 clamps to [10, 16] (src/lib.rs:236-238)
     uint32_t plog2 = cfg && cfg->predictor_log2 ? cfg->predictor_log2 : 16;
     plog2 = plog2 < 10 ? 10 : plog2 > 16 ? 16 : plog2;
@@ -716,7 +766,11 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     hipEventRecord(g->ev0, s);
     zero_async(g->cst.p, ncols * sizeof(DColState), s);
     zero_async(g->wins.p, 64 * 8, s);
-    if (hybrid) return hybrid_encode(g, a, cols, level, lane_off, out_lens, s);
+    if (prep_only) {
+        a.hybrid = 2;
+        return hybrid_prepare(g, a, cols, level, plog2, lane_off, s, *prep_only);
+    }
+    if (hybrid) return hybrid_encode(g, a, cols, level, plog2, sample, lane_off, out_lens, s);
     if (g->form == 0) {
         a.kev0 = g->kev0;
         a.kev1 = g->kev1;
@@ -783,6 +837,85 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         else if (st[c].err) rc = QUOIN_ERR_PANIC;
     }
     return rc;
+}
+
+// Selection::Sample's sample of each block (encoder.rs:483-641, build_sample):
+// the block itself up to 512 values, else 8 runs of 64 values starting at
+// r (n - 64) / 7. The samples go through the device as a batch of one-block
+// columns on the handle's auxiliary handle (its own arenas: the main pass's
+// results stay in place for the emission), at the call's level with
+// predictor tables of 2^10 entries.
+static int sample_pass(QuoinGpu *g, int level, const HybridPrep &P, const std::vector<DCol> &cols,
+                       HybridPrep &S, hipStream_t) {
+    (void)cols;
+    const size_t nb = P.eb.size();
+    if (nb == 0) return QUOIN_OK;
+    if (!g->aux) {
+        g->aux = quoin_gpu_create(g->device, g->budget);
+        if (!g->aux) return QUOIN_ERR_DEVICE;
+        if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    }
+    QuoinGpu *x = g->aux;
+    std::lock_guard<std::recursive_mutex> lk(x->mu);
+    static const int32_t kTyped[6] = {QUOIN_DTYPE_F64, QUOIN_DTYPE_I64, QUOIN_DTYPE_U64,
+                                      QUOIN_DTYPE_I32, QUOIN_DTYPE_U32, QUOIN_DTYPE_F32};
+    std::vector<size_t> off(nb + 1, 0);
+    std::vector<uint32_t> sn(nb);
+    for (size_t k = 0; k < nb; k++) {
+        const qahyb::EncBlock &e = P.eb[k];
+        sn[k] = e.n <= 512 ? e.n : 512;
+        off[k + 1] = off[k] + align256((size_t)sn[k] * 8);
+    }
+    std::vector<uint8_t> host(off[nb]);
+    for (size_t k = 0; k < nb; k++) {
+        const qahyb::EncBlock &e = P.eb[k];
+        const bool w4 = e.dt == QA_I32 || e.dt == QA_U32 || e.dt == QA_F32;
+        uint8_t *dst = host.data() + off[k];
+        auto put = [&](size_t j, uint64_t v) {
+            if (e.dt == QA_F32) {
+                double d;
+                memcpy(&d, &v, 8);
+                const float f = (float)d;  // the lane is the f32 widened: exact
+                memcpy(dst + 4 * j, &f, 4);
+            } else if (w4) {
+                const uint32_t u = (uint32_t)v;
+                memcpy(dst + 4 * j, &u, 4);
+            } else {
+                memcpy(dst + 8 * j, &v, 8);
+            }
+        };
+        if (e.n <= 512) {
+            for (uint32_t j = 0; j < e.n; j++) put(j, e.v[j]);
+        } else {
+            for (uint32_t r = 0; r < 8; r++) {
+                const size_t start = (size_t)r * (e.n - 64) / 7;
+                for (uint32_t j = 0; j < 64; j++) put(64 * r + j, e.v[start + j]);
+            }
+        }
+    }
+    if (!ensure(x->st_in, host.size()) || !ensure(x->st_out, 4096)) return QUOIN_ERR_DEVICE;
+    hipStream_t xs = x->stream;
+    if (hipMemcpyAsync(x->st_in.p, host.data(), host.size(), hipMemcpyHostToDevice, xs) != hipSuccess)
+        return QUOIN_ERR_DEVICE;
+    std::vector<QuoinDeviceColumn> sc(nb);
+    for (size_t k = 0; k < nb; k++) {
+        QuoinDeviceColumn &c = sc[k];
+        memset(&c, 0, sizeof c);
+        c.dtype = kTyped[P.eb[k].dt];
+        c.n = sn[k];
+        c.values = (const uint8_t *)x->st_in.p + off[k];
+        c.dst = (uint8_t *)x->st_out.p;  // the sample pass emits nothing
+        c.dst_cap = 4096;
+    }
+    const QuoinConfig cfg{level, 512, QUOIN_SELECTION_FULL, 10};
+    std::vector<uint64_t> lens(nb);
+    int rc = gpu_compress_locked(x, &cfg, sc.data(), nb, lens.data(), xs, &S);
+    if (hipSetDevice(g->device) != hipSuccess) return QUOIN_ERR_DEVICE;
+    if (rc != QUOIN_OK) return rc;
+    if (S.eb.size() != nb) return QUOIN_ERR_PANIC;
+    for (size_t k = 0; k < nb; k++)
+        if (S.blk[S.slots[k]].col != k) return QUOIN_ERR_PANIC;
+    return QUOIN_OK;
 }
 
 extern "C" int quoin_gpu_compress(QuoinGpu *g, const QuoinConfig *cfg, const QuoinDeviceColumn *cols,
@@ -1422,7 +1555,7 @@ static long long host_compress(const QuoinConfig *cfg, int32_t dtype, size_t n,
     if (!map_dtype(dtype, &dt, &kind, &width)) return QUOIN_ERR_UNSUPPORTED;
     // a config the engine does not run is refused before any device work
     if (cfg && (cfg->level < QUOIN_LEVEL_FASTEST || cfg->level > QUOIN_LEVEL_MAX ||
-                cfg->selection != QUOIN_SELECTION_FULL))
+                (cfg->selection != QUOIN_SELECTION_FULL && cfg->selection != QUOIN_SELECTION_SAMPLE)))
         return QUOIN_ERR_UNSUPPORTED;
     QuoinGpu *g = default_gpu();
     if (!g) return QUOIN_ERR_DEVICE;

@@ -816,7 +816,7 @@ void transpose(const uint64_t *v, size_t n, Bytes &o) {  // transpose.rs:10-19
 }
 
 // float_mult.rs:66-98 at an entropy level, with the scale the device found
-bool fm_payload(const EncBlock &e, uint64_t lambda, bool allow_lz, Bytes &out) {
+bool fm_payload(const EncBlock &e, uint64_t lambda, bool allow_lz, bool entropy, Bytes &out) {
     if (e.fm_scale >= 7 || !e.n) return false;
     const double s = FM_SCALES[e.fm_scale];
     std::vector<uint64_t> ks(e.n);
@@ -829,13 +829,16 @@ bool fm_payload(const EncBlock &e, uint64_t lambda, bool allow_lz, Bytes &out) {
     Bytes bp;
     for_encode(ks.data(), e.n, true, bp);
     Bytes d, coded;
-    uint64_t prev = 0;
-    for (uint32_t i = 0; i < e.n; i++) {
-        put_varint(d, zigzag(ks[i] - prev));
-        prev = ks[i];
+    bool ent = false;
+    if (entropy) {
+        uint64_t prev = 0;
+        for (uint32_t i = 0; i < e.n; i++) {
+            put_varint(d, zigzag(ks[i] - prev));
+            prev = ks[i];
+        }
+        qahost::code_residuals(d.data(), d.size(), lambda, allow_lz, coded);
+        ent = coded.size() < bp.size();
     }
-    qahost::code_residuals(d.data(), d.size(), lambda, allow_lz, coded);
-    const bool ent = coded.size() < bp.size();
     out.clear();
     out.push_back(ent ? 1 : 0);
     out.push_back((uint8_t)e.fm_scale);
@@ -845,7 +848,7 @@ bool fm_payload(const EncBlock &e, uint64_t lambda, bool allow_lz, Bytes &out) {
 }
 
 // alp_rd.rs:132-206 at an entropy level, with the cut and dictionary the device chose
-bool alprd_payload(const EncBlock &e, uint64_t lambda, bool allow_lz, Bytes &out) {
+bool alprd_payload(const EncBlock &e, uint64_t lambda, bool allow_lz, bool entropy, Bytes &out) {
     if (e.rd_bail || !e.n) return false;
     const uint32_t rbw = 64 - e.rd_left;
     const uint64_t mask = rbw >= 64 ? ~0ull : ((1ull << rbw) - 1);
@@ -875,18 +878,24 @@ bool alprd_payload(const EncBlock &e, uint64_t lambda, bool allow_lz, Bytes &out
     }
     Bytes cbp, cent;
     for_encode(codes.data(), e.n, false, cbp);
-    Bytes cb(e.n);
-    for (uint32_t i = 0; i < e.n; i++) cb[i] = (uint8_t)codes[i];
-    qahost::code_residuals(cb.data(), cb.size(), lambda, allow_lz, cent);
-    const bool ce = cent.size() < cbp.size();
+    bool ce = false;
+    if (entropy) {
+        Bytes cb(e.n);
+        for (uint32_t i = 0; i < e.n; i++) cb[i] = (uint8_t)codes[i];
+        qahost::code_residuals(cb.data(), cb.size(), lambda, allow_lz, cent);
+        ce = cent.size() < cbp.size();
+    }
     out.push_back(ce ? 1 : 0);
     put_varint(out, ce ? cent.size() : cbp.size());
     const Bytes &c = ce ? cent : cbp;
     out.insert(out.end(), c.begin(), c.end());
     Bytes rbp, rent;
     for_encode(rights.data(), e.n, false, rbp);
-    qahost::code_residuals(rbp.data(), rbp.size(), lambda, allow_lz, rent);
-    const bool re = rent.size() < rbp.size();
+    bool re = false;
+    if (entropy) {
+        qahost::code_residuals(rbp.data(), rbp.size(), lambda, allow_lz, rent);
+        re = rent.size() < rbp.size();
+    }
     out.push_back(re ? 1 : 0);
     const Bytes &r = re ? rent : rbp;
     out.insert(out.end(), r.begin(), r.end());
@@ -894,7 +903,7 @@ bool alprd_payload(const EncBlock &e, uint64_t lambda, bool allow_lz, Bytes &out
 }
 
 // dict.rs:87-152 at an entropy level
-bool dict_payload(const EncBlock &e, uint64_t lambda, bool allow_lz, Bytes &out) {
+bool dict_payload(const EncBlock &e, uint64_t lambda, bool allow_lz, bool entropy, Bytes &out) {
     if (!e.n) return false;
     const size_t max_card = e.n / 2 + 1;
     std::unordered_map<uint64_t, uint32_t> map;
@@ -933,17 +942,19 @@ bool dict_payload(const EncBlock &e, uint64_t lambda, bool allow_lz, Bytes &out)
         vtag = 1;
         vb.swap(alt);
     }
-    Bytes tr;
-    transpose(sorted.data(), card, tr);
-    qahost::code_residuals(tr.data(), tr.size(), lambda, allow_lz, alt);
-    if (alt.size() < vb.size()) {
-        vtag = 2;
-        vb.swap(alt);
+    if (entropy) {
+        Bytes tr;
+        transpose(sorted.data(), card, tr);
+        qahost::code_residuals(tr.data(), tr.size(), lambda, allow_lz, alt);
+        if (alt.size() < vb.size()) {
+            vtag = 2;
+            vb.swap(alt);
+        }
     }
     uint8_t ctag = 0;
     Bytes cb;
     for_encode(codes.data(), e.n, false, cb);
-    if (card <= (1u << 16)) {
+    if (entropy && card <= (1u << 16)) {
         const int np = card <= 256 ? 1 : 2;
         Bytes blob, plane(e.n), coded;
         blob.push_back((uint8_t)np);
@@ -979,7 +990,7 @@ bool coded_if_competitive(const uint8_t *r, size_t n, uint64_t lambda, bool allo
 // encoder.rs:543-581: one host-coded mode's payload at lambda, with the LZ
 // cascade inside code_residuals when allow_lz
 bool encode_host_mode(const EncBlock &e, int m, uint64_t lambda, bool allow_lz, unsigned pco_level,
-                      Bytes &out) {
+                      bool entropy, Bytes &out) {
     const int k = m == M_PRED_RC ? 0 : m == M_PRED2 ? 1 : m == M_DELTA2 ? 2 : m == M_DELTA_DP ? 3 :
                   m == M_ORDERED_DELTA ? 4 : -1;
     if (k >= 0) {
@@ -993,9 +1004,9 @@ bool encode_host_mode(const EncBlock &e, int m, uint64_t lambda, bool allow_lz, 
         if (!e.has_res[0]) return false;
         out.assign(e.res[0], e.res[0] + e.rlen[0]);
         return true;
-    case QA_M_FLOAT_MULT: return fm_payload(e, lambda, allow_lz, out);
-    case QA_M_ALP_RD: return alprd_payload(e, lambda, allow_lz, out);
-    case QA_M_DICT: return dict_payload(e, lambda, allow_lz, out);
+    case QA_M_FLOAT_MULT: return fm_payload(e, lambda, allow_lz, entropy, out);
+    case QA_M_ALP_RD: return alprd_payload(e, lambda, allow_lz, entropy, out);
+    case QA_M_DICT: return dict_payload(e, lambda, allow_lz, entropy, out);
     case M_LZ:
         le_bytes(e.v, e.n, tmp);
         {
@@ -1047,9 +1058,9 @@ void compete(EncBlock &e, int level) {
     gpu(QA_M_FOR, C_FOR);
     gpu(QA_M_DELTA, C_DELTA);
     if (isf) {
-        if (fm_payload(e, lambda, false, p)) b.consider(QA_M_FLOAT_MULT, p.size(), true, &p);
+        if (fm_payload(e, lambda, false, true, p)) b.consider(QA_M_FLOAT_MULT, p.size(), true, &p);
         gpu(QA_M_ALP, C_ALP);
-        if (alprd_payload(e, lambda, false, p)) b.consider(QA_M_ALP_RD, p.size(), true, &p);
+        if (alprd_payload(e, lambda, false, true, p)) b.consider(QA_M_ALP_RD, p.size(), true, &p);
     }
     if (predictors) {
         if (e.has_res[0]) {
@@ -1095,7 +1106,7 @@ void compete(EncBlock &e, int level) {
             if (coded_if_competitive(t.data(), t.size(), lambda, false, b.score, p))
                 b.consider(M_BYTE_TRANSPOSE, p.size(), true, &p);
         }
-        if (dict_payload(e, lambda, false, p)) b.consider(QA_M_DICT, p.size(), true, &p);
+        if (dict_payload(e, lambda, false, true, p)) b.consider(QA_M_DICT, p.size(), true, &p);
         gpu(QA_M_RLE, C_RLE);
     }
     {
@@ -1107,10 +1118,103 @@ void compete(EncBlock &e, int level) {
         const int ms[2] = {b.mode, b.runner};
         for (int m : ms) {
             if (m < 0 || !cascades_lz(m)) continue;
-            if (encode_host_mode(e, m, lambda, true, 12, p)) b.consider(m, p.size(), true, &p);
+            if (encode_host_mode(e, m, lambda, true, 12, true, p)) b.consider(m, p.size(), true, &p);
         }
     }
     emit();
+}
+
+
+namespace {
+// encoder.rs:24-73: which modes may compete for a family at a level
+bool is_entropy_mode(int m) {
+    return m == M_PRED_RC || m == M_PRED2 || m == M_DELTA2 || m == M_DELTA_DP || m == M_ORDERED_DELTA ||
+           m == QA_M_FLOAT_MULT || m == M_LZ || m == M_BYTE_TRANSPOSE;
+}
+bool is_predictor_mode(int m) {
+    return m == M_PRED || m == M_PRED_RC || m == M_PRED2 || m == M_DELTA2 || m == M_DELTA_DP ||
+           m == M_ORDERED_DELTA;
+}
+bool mode_runs(int m, bool isf, int level) {
+    const bool fam_float = m == M_DELTA2 || m == M_DELTA_DP || m == QA_M_FLOAT_MULT || m == QA_M_ALP ||
+                           m == QA_M_ALP_RD;
+    if (fam_float && !isf) return false;
+    if (is_predictor_mode(m)) return level >= 3;
+    return level >= 2 || !is_entropy_mode(m);
+}
+uint64_t level_lambda(int level) {  // src/lib.rs:124-139
+    static const uint64_t kLambda[5] = {16, 4, 2, 0, 0};
+    return kLambda[level];
+}
+// the device candidate slot of a mode the device sizes and emits, else -1
+int device_slot(int m) {
+    switch (m) {
+    case QA_M_CONST: return C_CONST;
+    case QA_M_STRIDE: return C_STRIDE;
+    case QA_M_XORZ: return C_XORZ;
+    case QA_M_FOR: return C_FOR;
+    case QA_M_DELTA: return C_DELTA;
+    case QA_M_ALP: return C_ALP;
+    case QA_M_RLE: return C_RLE;
+    default: return -1;
+    }
+}
+// encode_mode at a level (encoder.rs:543-581): a device mode's size, or a
+// host-coded mode's payload (the level's lambda, its LZ cascade at Max)
+bool mode_at_level(const EncBlock &e, int m, int level, uint64_t &len, bool &host, Bytes &out) {
+    const int slot = device_slot(m);
+    if (slot >= 0) {
+        if (e.sz[slot] == UINT64_MAX) return false;
+        len = e.sz[slot];
+        host = false;
+        return true;
+    }
+    if (!encode_host_mode(e, m, level_lambda(level), level == 4, level == 4 ? 12 : 8, level >= 2, out))
+        return false;
+    len = out.size();
+    host = true;
+    return true;
+}
+// encoder.rs:483-641, the sampled modes in the reference's order
+const int kSampleModes[15] = {QA_M_XORZ, M_PRED, M_PRED_RC, M_PRED2, M_DELTA2, M_DELTA_DP, M_ORDERED_DELTA,
+                              QA_M_FLOAT_MULT, M_BYTE_TRANSPOSE, QA_M_FOR, QA_M_DELTA, QA_M_ALP,
+                              QA_M_ALP_RD, QA_M_DICT, QA_M_RLE};
+}  // namespace
+
+void compete_sampled(EncBlock &e, const EncBlock &smp, int level) {
+    const bool isf = e.dt == QA_F64 || e.dt == QA_F32;
+    const uint64_t n = e.n;
+    Best b;
+    b.lambda = level_lambda(level);
+    b.dec = n * lane_bytes(e.dt);
+    b.mode = QA_M_RAW;
+    b.len = e.sz[C_RAW];
+    b.score = b.len;  // RAW's decode weight is 0
+    uint64_t len;
+    bool host;
+    Bytes p;
+    for (int m : {QA_M_CONST, QA_M_STRIDE})
+        if (mode_at_level(e, m, level, len, host, p)) b.consider(m, len, host, &p);
+    if (mode_runs(M_LZ, isf, level) && (e.distinct_low || e.repeats) && mode_at_level(e, M_LZ, level, len, host, p))
+        b.consider(M_LZ, len, host, &p);
+    if (level >= 2 && encode_host_mode(e, M_PCO, 0, false, level == 4 ? 12 : 8, true, p))
+        b.consider(M_PCO, p.size(), true, &p);
+    // the winner on the sample (smallest payload, the earlier on ties), then
+    // that mode on the whole block
+    int win = -1;
+    uint64_t win_len = UINT64_MAX;
+    for (int m : kSampleModes) {
+        if (!mode_runs(m, isf, level)) continue;
+        if (mode_at_level(smp, m, level, len, host, p) && len < win_len) {
+            win_len = len;
+            win = m;
+        }
+    }
+    if (win >= 0 && mode_at_level(e, win, level, len, host, p)) b.consider(win, len, host, &p);
+    e.mode = b.mode;
+    e.host = b.host;
+    e.len = b.len;
+    if (b.host) e.payload.swap(b.payload);
 }
 
 }  // namespace qahyb
@@ -1177,7 +1281,7 @@ extern "C" int qa_hybrid_encode_mode(int mode, uint64_t lambda, int allow_lz, un
         e.rlen[j] = e.has_res[j] ? rlen[j] : 0;
     }
     qahyb::Bytes out;
-    if (!qahyb::encode_host_mode(e, mode, lambda, allow_lz != 0, pco_level, out)) return 0;
+    if (!qahyb::encode_host_mode(e, mode, lambda, allow_lz != 0, pco_level, true, out)) return 0;
     *len = out.size();
     if (cap < out.size()) return -2;
     if (!out.empty()) memcpy(payload, out.data(), out.size());

@@ -71,5 +71,10 @@ struct EncBlock {
     uint64_t len = 0;
 };
 void compete(EncBlock &e, int level);
+// Selection::Sample (src/encoder.rs:483-641) at any level: RAW, CONST, STRIDE,
+// LZ (gated) and pco on the whole block, and the one mode that wins on the
+// block's sample `smp` (8 runs of 64 values, predictor tables of 2^10), again
+// on the whole block.
+void compete_sampled(EncBlock &e, const EncBlock &smp, int level);
 
 }  // namespace qahyb

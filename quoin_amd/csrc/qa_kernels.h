@@ -71,9 +71,10 @@ struct EncArgs {
     unsigned long long *wins;  // 64 per-mode win counters
     // Level::Fast codecs
     uint32_t fast;         // some column runs at Level::Fast
-    // Levels Balanced / High / Max: the device sizes its own candidates for
-    // every block (Dict is the host's), counts no wins, and with `force` set
-    // lays out the modes the host competition chose
+    // Levels Balanced / High / Max and Selection::Sample: the device sizes
+    // its own candidates (1: for the blocks the Full competition's gates let
+    // through; 2: for every block), leaves Dict to the host, counts no wins,
+    // and with `force` set lays out the modes the host chose
     uint32_t hybrid;
     DTileF *tilesf;        // per tile slot
     DBlockX *blocksx;      // per region slot

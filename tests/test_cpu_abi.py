@@ -131,10 +131,15 @@ def test_every_level_needs_the_device(qa):
         assert e.value.code == qa.ERR_DEVICE
 
 
-def test_sample_selection_is_refused_not_approximated(qa):
-    # Selection::Sample is not on the device path: refused before any device
-    # work, never silently run as Full
+def test_unknown_selection_is_refused(qa):
+    # a selection other than Full (0) and Sample (1) is refused before any
+    # device work; Sample itself needs the device like every level
+    import torch
     v = np.arange(100, dtype=np.int64)
     with pytest.raises(qa.QuoinError) as e:
-        qa.compress_column(v, None, qa.Config(qa.Level.MAX, selection=qa.Selection.SAMPLE))
+        qa.compress_column(v, None, qa.Config(qa.Level.MAX, selection=7))
     assert e.value.code == qa.ERR_UNSUPPORTED
+    if not torch.cuda.is_available():
+        with pytest.raises(qa.QuoinError) as e:
+            qa.compress_column(v, None, qa.Config(qa.Level.MAX, selection=qa.Selection.SAMPLE))
+        assert e.value.code == qa.ERR_DEVICE

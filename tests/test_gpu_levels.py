@@ -28,13 +28,14 @@ def engine_dtype(arr):
             np.dtype(np.uint32): orc.U32, np.dtype(np.float32): orc.F32}[arr.dtype]
 
 
-def check(qa, name, vals, level, validity=None, block_size=0, plog2=16):
+def check(qa, name, vals, level, validity=None, block_size=0, plog2=16, selection=0):
     print(f"case {name} n={len(vals)} dtype={vals.dtype} nulls={validity is not None} "
-          f"level={level} bs={block_size} plog2={plog2}", flush=True)
-    cfg = qa.Config(level=qa.Level(level), block_size=block_size, predictor_log2=plog2)
+          f"level={level} bs={block_size} plog2={plog2} selection={selection}", flush=True)
+    cfg = qa.Config(level=qa.Level(level), block_size=block_size, predictor_log2=plog2,
+                    selection=qa.Selection(selection))
     got = qa.compress_column(vals, validity, cfg)
     want = orc.compress(vals, engine_dtype(vals), validity, level, block_size,
-                        predictor_log2=plog2, threads=16)
+                        selection=selection, predictor_log2=plog2, threads=16)
     if got != want:
         gf, wf = orc.frames(got) if validity is None else [], \
             orc.frames(want) if validity is None else []
@@ -170,3 +171,16 @@ def test_reference_default_config_streams_decode(qa):
                                          with_validity=bm is not None)
         _, want, _ = orc.decompress(s)
         assert out.view(np.uint8).tobytes() == want.view(np.uint8).tobytes(), name
+
+
+@pytest.mark.parametrize("level", [orc.FASTEST, orc.FAST, orc.BALANCED, orc.HIGH, orc.MAX])
+def test_sample_selection(qa, level):
+    """Selection::Sample (src/encoder.rs:483-641): the winner on each block's
+    sample of 8 runs of 64 values, then on the whole block, next to RAW,
+    CONST, STRIDE, LZ and pco; at every level, with small and grown blocks."""
+    cols = {**_data.int_columns(40000, seed=51), **_data.float_columns(40000, seed=52)}
+    for name, v in cols.items():
+        check(qa, f"sample {name}", v, level, selection=1)
+    check(qa, "sample small", _data.float_columns(300, seed=3)["walk_f64"], level, selection=1)
+    vv, bm = _data.with_nulls(_data.int_columns(20000, seed=6)["u32_k11"], 0.2, seed=4)
+    check(qa, "sample nulls", vv, level, validity=bm, selection=1)
