@@ -10,10 +10,15 @@
 // table (sym | f | slot - cum, one 32-bit read) and reads the 12 payload bytes
 // at the stream position from an LDS window (issued together, so one LDS
 // latency per step); the byte counts follow from the new state alone (0, 1 or
-// 2 bytes for every state a valid stream reaches), so a 4-lane DPP scan places
-// each lane's bytes. The window is a 1 KiB ring per stream, refilled from
-// registers loaded a round (64 steps) ahead. States that need 3 or more bytes
-// (only a corrupt stream reaches them) take an exact sequential path.
+// 2 bytes for every state a valid stream reaches, from its leading zeros), so
+// a 4-lane DPP scan places each lane's bytes. The window is a 1 KiB ring per
+// stream, refilled from registers loaded a round (64 steps) ahead. A round
+// runs branch-free; a state that needs 3 or more bytes (only a corrupt stream
+// reaches one) is found at the round's end, and the round is replayed on an
+// exact sequential path. The chain of one step (table read, state update,
+// byte count, quad scan, byte extraction) is the kernel's bound: every stream
+// of the batch is resident at once, so the time is the steps of the longest
+// stream times that chain's latency.
 //
 // Encode: a workgroup per stream builds the histogram, normalizes it
 // (rans.rs:25-50), writes the header and the per-symbol encode table; then one
@@ -316,6 +321,8 @@ __global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ de
         if (di == 0) *(uint4 *)&L.ring[gl][RING / 4] = v;
     };
 
+    // a wavefront with an irregular table runs every round on the exact loop
+    const bool any_irr = __builtin_amdgcn_ballot_w64(irr != 0) != 0;
     bool first = true;
     uint32_t k = 0;
     while (k < kmax) {
@@ -365,6 +372,55 @@ __global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ de
         const uint32_t rend = min(kmax, (k | 63u) + 1u);
         uint32_t sp = (uint32_t)og * 256 + ((k & 63) << 2) + (uint32_t)idx;  // staging byte
         bool slow = false;
+        if (!any_irr) {
+            // Fast round: no per-step branch. A state below 128 (3+ bytes: only a
+            // corrupt stream reaches one) and a read past the payload end are
+            // found at the round's end; the first replays the round on the exact
+            // loop below from its start, the second fails the stream as that
+            // loop would. Every step consumes at most 2 bytes per state (the
+            // shift is clamped), so the round stays inside the window even on
+            // a corrupt stream.
+            const uint32_t k0 = k, xs0 = xs, pos0 = pos, sp0 = sp;
+            uint32_t minx = 0xFFFFFFFFu;
+            for (; k < rend; k++, sp += 4) {
+                const bool act = k < klim;
+                const uint32_t amask = act ? 0x18u : 0u;
+                const uint32_t e = L.tab[g][xs & (RM - 1)];
+                const uint32_t *wp = &L.ring[g][(pos >> 2) & (RING / 4 - 1)];
+                const uint32_t w0 = wp[0], w1 = wp[1], w2 = wp[2];
+                asm volatile("" ::"v"(w0), "v"(w1), "v"(w2), "v"(e));
+                const uint32_t xhi = xs >> 12;
+                // x' = f (x >> 12) + slot - cum = (f - 1) xhi + (xhi + slot - cum)
+                const uint32_t xn = __umul24((e >> 8) & 4095, xhi) + (xhi + (e >> 20));
+                minx = min(minx, act ? xn : 0xFFFFFFFFu);
+                ((uint8_t *)L.outs)[sp] = (uint8_t)e;
+                // renormalization shift in bits: 0 (xn >= 2^23), 8 or 16 (xn >= 2^7)
+                const uint32_t lz = (uint32_t)__builtin_clz(xn | 1u);
+                const uint32_t sh = (min(max(lz, 8u), 24u) - 1u) & amask;
+                uint32_t w = sh << sh8;
+                w |= (uint32_t)__builtin_amdgcn_mov_dpp((int)w, QA_DPP_QP_1032, 0xF, 0xF, false);
+                w |= (uint32_t)__builtin_amdgcn_mov_dpp((int)w, QA_DPP_QP_2301, 0xF, 0xF, false);
+                const uint32_t exb = __builtin_amdgcn_sad_u8(w & below, 0u, 0u);
+                const uint32_t totb = __builtin_amdgcn_sad_u8(w, 0u, 0u);
+                const uint32_t off = (pos & 3) + (exb >> 3);
+                const uint32_t lo = off < 4 ? w0 : (off < 8 ? w1 : w2);
+                const uint32_t hi = off < 4 ? w1 : w2;
+                const uint32_t d = __builtin_amdgcn_alignbyte(hi, lo, off);
+                const uint32_t bs = __builtin_amdgcn_perm(0u, d, 0x0c0c0001u);  // b0 << 8 | b1
+                xs = (xn << sh) | (bs >> (16 - sh));
+                pos += totb >> 3;
+            }
+            if (__builtin_amdgcn_ballot_w64(minx < 128u)) {
+                k = k0;
+                xs = xs0;
+                pos = pos0;
+                sp = sp0;
+            } else {
+                const bool over = pos > qe32;
+                err |= over ? 1u : 0u;
+                klim = over ? 0u : klim;
+            }
+        }
         for (; k < rend; k++, sp += 4) {
             const bool act = k < klim;
             const uint32_t e = L.tab[g][xs & (RM - 1)];
