@@ -19,6 +19,17 @@
  *    tests/ratio_regression.rs:40-56 (size within ~3%).
  *  - rANS: inverse on all 1-3 symbol alphabets; normalize's tie rule against a
  *    brute-force restatement.
+ *  - range coder / LZ / tag layer: hand-derived streams (rc of one byte, the
+ *    empty stream, short LZSS parses), the 64-candidate chain limit, the
+ *    order-1 model-size bound, the reference's own entropy tests.
+ *  - Balanced / High / Max and Selection::Sample: the reference's ratio floors
+ *    at Balanced and Max, ties under the decode penalty, the order-1
+ *    estimate's bounds, the cascade's level gate, Sample's construction.
+ *  - validity blob: hand-derived known answers.
+ *  - pco: the engine's own host restatement (quoin_amd/csrc/qa_pco.cpp) is
+ *    linked in; it is pinned by pco's unit-test known answers, round trips and
+ *    the size guarantee (tests/test_pco_pins.py) and is otherwise parity
+ *    unpinned.
  */
 #include "oracle.h"
 
