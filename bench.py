@@ -196,7 +196,7 @@ def serve_run(qa, srv, pool, cfg, rate, count, seed=2):
             rc, _, ms = srv.wait(t)
             if rc != qa.OK:
                 raise RuntimeError(f"served job failed: {rc}")
-            lat.append(ms)
+            lat.append((ms, req["bytes"]))
             req["busy"] = False
 
     th = threading.Thread(target=collect)
@@ -233,7 +233,7 @@ def serve_run(qa, srv, pool, cfg, rate, count, seed=2):
     return done_bytes / wall / 1e9, lat, wall
 
 
-def serve_bench(qa, dev, budget_gb=64, slots=2, pool_size=48, per_point=800):
+def serve_bench(qa, dev, budget_gb=64, slots=3, pool_size=48, per_point=800):
     cfg = qa.Config(level=qa.Level.FAST)
     gpu = qa.Gpu(0)
     pool = serve_pool(qa, gpu, cfg, dev, pool_size)
@@ -248,9 +248,16 @@ def serve_bench(qa, dev, budget_gb=64, slots=2, pool_size=48, per_point=800):
             j0, b0 = srv.stats()
             gbs, lat, wall = serve_run(qa, srv, pool, cfg, frac * sat_rps, per_point, seed=int(frac * 100))
             j1, b1 = srv.stats()
+            ms = [m for m, _ in lat]
+            sm = [m for m, b in lat if b <= 1 << 20] or [0.0]
+            lg = [m for m, b in lat if b > 1 << 20] or [0.0]
             points.append({"offered_rps": round(frac * sat_rps, 1), "achieved_gbs": round(gbs, 3),
-                           "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                           "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                           "p50_ms": round(float(np.percentile(ms, 50)), 3),
+                           "p99_ms": round(float(np.percentile(ms, 99)), 3),
+                           "small_le_1mb": {"p50_ms": round(float(np.percentile(sm, 50)), 3),
+                                            "p99_ms": round(float(np.percentile(sm, 99)), 3)},
+                           "large": {"p50_ms": round(float(np.percentile(lg, 50)), 3),
+                                     "p99_ms": round(float(np.percentile(lg, 99)), 3)},
                            "jobs": j1 - j0, "batches": b1 - b0})
     finally:
         srv.close()
@@ -260,6 +267,9 @@ def serve_bench(qa, dev, budget_gb=64, slots=2, pool_size=48, per_point=800):
             "hbm_budget_gb": budget_gb, "slots": slots,
             "saturation_gbs": round(sat_gbs, 3), "saturation_rps": round(sat_rps, 1),
             "p50_ms": hi["p50_ms"], "p99_ms": hi["p99_ms"], "at_offered_rps": hi["offered_rps"],
+            "p99_ms_small_le_1mb": hi["small_le_1mb"]["p99_ms"],
+            "scheduling": "slot 0 serves requests of up to 1 MiB only; each slot's arenas are "
+                          "capped at budget / slots",
             "points": points}
 
 

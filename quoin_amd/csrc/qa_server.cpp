@@ -3,10 +3,12 @@
 // HBM byte budget.
 //
 // Worker slots each own a QuoinGpu handle (its own HIP stream and scratch
-// arenas). A slot that is free takes the oldest waiting job and every other
-// waiting job of the same group (compress with the same config, or
-// decompress) whose bytes fit the budget next to everything already admitted,
-// and runs them as one batched launch sequence. Batches of different slots
+// arenas, capped at budget / slots). A slot that is free takes the oldest
+// waiting job of its class and every other waiting job of the same group
+// (compress with the same config, or decompress) whose bytes fit the budget
+// next to everything already admitted, and runs them as one batched launch
+// sequence. With two or more slots, slot 0 serves only small requests (up to
+// 1 MiB of values), so they never queue behind large ones. Batches of different slots
 // run concurrently on their streams, so work of jobs that arrived while
 // others were on the device overlaps with it. Columns are independent in a
 // batch (every stage keys its state by column), so each job's bytes are the
@@ -35,7 +37,12 @@ struct Req {
     int rc = 0;
     uint64_t out_len = 0;
     bool done = false;
+    bool small = false;  // uncompressed bytes <= kSmallBytes
 };
+
+// size class of a job: requests up to 1 MiB of values have a slot of their own
+// (slot 0 when there are two or more), so they never wait behind large ones
+constexpr uint64_t kSmallBytes = 1ull << 20;
 
 size_t width_of(int32_t dtype) { return quoin_dtype_width(dtype); }
 
@@ -105,22 +112,35 @@ void run_batch(QuoinGpu *g, std::vector<Req *> &batch) {
 void worker(QuoinServer *S, size_t slot) {
     hipSetDevice(S->device);
     QuoinGpu *g = S->gpus[slot];
+    const bool small_only = slot == 0 && S->gpus.size() >= 2;
     std::vector<Req *> batch;
+    // this slot's next job: the oldest of its class (FIFO within the class),
+    // provided it fits the budget beside everything admitted
+    auto head = [&]() -> Req * {
+        for (Req *r : S->pending) {
+            if (small_only && !r->small) continue;
+            return S->inflight + r->bytes <= S->budget ? r : nullptr;
+        }
+        return nullptr;
+    };
+    auto none_for_me = [&]() {
+        for (Req *r : S->pending)
+            if (!small_only || r->small) return false;
+        return true;
+    };
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_work.wait(lk, [&] {
-                return (S->stop && S->pending.empty()) ||
-                       (!S->pending.empty() &&
-                        S->inflight + S->pending.front()->bytes <= S->budget);
-            });
-            if (S->pending.empty()) return;  // stopping with nothing queued
+            S->cv_work.wait(lk, [&] { return (S->stop && none_for_me()) || head() != nullptr; });
+            Req *h = head();
+            if (!h) return;  // stopping with nothing of this slot's class queued
             batch.clear();
-            const QuoinJob head = S->pending.front()->job;
+            const QuoinJob first = h->job;
             for (auto it = S->pending.begin();
                  it != S->pending.end() && batch.size() < S->max_batch;) {
                 Req *r = *it;
-                if (same_group(r->job, head) && S->inflight + r->bytes <= S->budget) {
+                if ((!small_only || r->small) && same_group(r->job, first) &&
+                    S->inflight + r->bytes <= S->budget) {
                     S->inflight += r->bytes;
                     batch.push_back(r);
                     it = S->pending.erase(it);
@@ -157,7 +177,8 @@ extern "C" QuoinServer *quoin_server_create(int device, uint64_t hbm_budget_byte
     S->budget = hbm_budget_bytes ? hbm_budget_bytes : (64ull << 30);
     S->max_batch = max_batch ? max_batch : 256;
     for (uint32_t i = 0; i < slots; i++) {
-        QuoinGpu *g = quoin_gpu_create(S->device, S->budget);
+        // the budget is shared: each slot's arenas are capped at its share
+        QuoinGpu *g = quoin_gpu_create(S->device, S->budget / slots);
         if (!g) {
             for (QuoinGpu *h : S->gpus) quoin_gpu_destroy(h);
             delete S;
@@ -177,7 +198,11 @@ extern "C" int quoin_server_submit(QuoinServer *S, const QuoinJob *job, uint64_t
     if (!r) return QUOIN_ERR_PANIC;
     r->job = *job;
     r->bytes = charge(*job);
-    if (r->bytes > S->budget) {
+    r->small = (job->kind == QUOIN_JOB_COMPRESS ? job->column.n * width_of(job->column.dtype)
+                                                 : job->stream.n * width_of(job->stream.dtype)) <=
+               kSmallBytes;
+    // every slot's handle holds at most budget / slots of scratch
+    if (r->bytes > S->budget / S->gpus.size()) {
         delete r;
         return QUOIN_ERR_BUFFER_TOO_SMALL;
     }
@@ -193,7 +218,7 @@ extern "C" int quoin_server_submit(QuoinServer *S, const QuoinJob *job, uint64_t
         S->pending.push_back(r);
         *ticket = r->ticket;
     }
-    S->cv_work.notify_one();
+    S->cv_work.notify_all();  // the job's class decides which slots may take it
     return QUOIN_OK;
 }
 

@@ -127,11 +127,11 @@ def test_tight_budget_serializes_without_changing_results(qa):
     cols = columns(11)[:6]
     cfgs = [qa.Config(qa.Level.FAST)] * len(cols)
     ref_s, ref_d = alone(qa, cols, cfgs)
-    # room for the largest single job only
+    # each slot's share (budget / slots) has room for the largest single job only
     big = max(len(v) * v.dtype.itemsize * 2 + 1024 + len(v) * (v.dtype.itemsize + 100)
               for v, _ in cols)
-    print(f"case serve budget={big}", flush=True)
-    got_s, got_d, jobs, batches = served(qa, cols, cfgs, ref_s, budget=big, slots=3)
+    print(f"case serve budget={3 * big}", flush=True)
+    got_s, got_d, jobs, batches = served(qa, cols, cfgs, ref_s, budget=3 * big, slots=3)
     compare(cols, ref_s, ref_d, got_s, got_d)
 
 
@@ -200,3 +200,28 @@ def test_submit_and_wait_on_two_threads_match_the_oracle(qa):
         got = bufs[i].s[:n].cpu().numpy().tobytes()
         want = orc.compress(v, odt[v.dtype], bm, int(levels[i]), threads=16)
         assert got == want, f"job {i} (level {int(levels[i])}): stream differs from the oracle"
+
+
+def test_small_requests_do_not_wait_behind_large_ones(qa):
+    """With two or more slots, slot 0 takes only requests of up to 1 MiB of
+    values: a small job submitted behind a burst of large ones completes
+    first, and every result is still the same as alone."""
+    r = np.random.default_rng(8)
+    large = [(r.integers(0, 1 << 40, 4 << 20).astype(np.int64), None) for _ in range(6)]
+    small = [(r.integers(0, 1000, 2000).astype(np.int64), None)]
+    cols = large + small
+    cfgs = [qa.Config(qa.Level.FAST)] * len(cols)
+    print("case serve size classes", flush=True)
+    ref_s, _ = alone(qa, cols, cfgs)
+    bufs = [Bufs(qa, v, bm) for v, bm in cols]
+    srv = qa.Server(0, 0, 2)
+    try:
+        tickets = [srv.submit_compress(b.column(qa), cfg) for b, cfg in zip(bufs, cfgs)]
+        res = [srv.wait(t) for t in tickets]
+    finally:
+        srv.close()
+    for i, (rc, n, ms) in enumerate(res):
+        assert rc == qa.OK, (i, rc)
+        assert bufs[i].s[:n].cpu().numpy().tobytes() == ref_s[i], f"job {i} differs from alone"
+    small_ms = res[-1][2]
+    assert small_ms < max(ms for _, _, ms in res[:-1]), (small_ms, [ms for _, _, ms in res])
