@@ -1062,10 +1062,123 @@ template <class L> struct Info {
     uint32_t symbol;
 };
 
+// One row of the bin optimizer's dynamic program: over j = i .. 0, the cost
+// best_cost[j] + bin_cost(bins j..i) with bin_cost's f32 operations in their
+// order, and the first minimum in that order (the largest j among equal
+// costs). ob[j] holds the offset bits of bins j..i as a float.
+struct DpRow {
+    float cost;
+    size_t j;
+};
+DpRow dp_row_scalar(const float *best_cost, const uint32_t *cc, const float *ob, size_t i, uint32_t ci,
+                    float total_log2, float meta) {
+    float bc = 3.40282347e+38f;
+    size_t bj = (size_t)-1;
+    for (size_t j = i + 1; j-- > 0;) {
+        const float c = (float)(ci - cc[j]);
+        const float ans_cost = total_log2 - log2_approx(c);
+        const float sv = ans_cost + ob[j];
+        const float cost = best_cost[j] + (meta + sv * c);
+        if (cost < bc) {
+            bc = cost;
+            bj = j;
+        }
+    }
+    return DpRow{bc, bj};
+}
+
+#if defined(__x86_64__)
+typedef float v8f __attribute__((vector_size(32)));
+typedef int32_t v8i __attribute__((vector_size(32)));
+typedef uint32_t v8u __attribute__((vector_size(32)));
+
+// The same row eight j at a time (AVX2): every lane runs log2_approx's and
+// bin_cost's f32 operations in the same order, so each cost is bit-identical
+// to the scalar one; the minimum keeps the largest j among equal costs.
+__attribute__((target("avx2"))) DpRow dp_row_avx2(const float *best_cost, const uint32_t *cc, const float *ob,
+                                                  size_t i, uint32_t ci, float total_log2, float meta) {
+    const float Z = 0.674f;
+    uint32_t zb;
+    memcpy(&zb, &Z, 4);
+    const uint32_t ZS = zb & 0x7FFFFF;
+    const float B = 2.0f / Z, C = -B / (6.0f * Z), A = -B - C;
+    const size_t n = i + 1;
+    // an empty bin i (count 0 somewhere in the row) converts differently as a
+    // signed lane: the scalar row takes it
+    if (ci == cc[i]) return dp_row_scalar(best_cost, cc, ob, i, ci, total_log2, meta);
+    v8f best;
+    v8i bestj;
+    for (int l = 0; l < 8; l++) {
+        best[l] = 3.40282347e+38f;
+        bestj[l] = -1;
+    }
+    const v8i lanes = {0, 1, 2, 3, 4, 5, 6, 7};
+    size_t j0 = 0;
+    for (; j0 + 8 <= n; j0 += 8) {
+        v8u cj, cnt;
+        v8f bcj, obj;
+        memcpy(&cj, cc + j0, 32);
+        memcpy(&bcj, best_cost + j0, 32);
+        memcpy(&obj, ob + j0, 32);
+        cnt = (v8u){ci, ci, ci, ci, ci, ci, ci, ci} - cj;
+        const v8f c = __builtin_convertvector((v8i)cnt, v8f);
+        const v8u bits = (v8u)c;
+        const v8u e = bits >> 23, signif = bits & 0x7FFFFF;
+        const v8u high = (v8u)(signif > ZS) & 1u;
+        const v8u log_int = e + high - 127u;
+        const v8u nbits = ((0x7Fu ^ high) << 23) | signif;
+        const v8f normalized = (v8f)nbits;
+        const v8f t = C * normalized;
+        const v8f u = B + t;
+        const v8f v = normalized * u;
+        const v8f w = __builtin_convertvector((v8i)log_int, v8f) + A;
+        const v8f lg = w + v;
+        const v8f ans_cost = total_log2 - lg;
+        const v8f sv = ans_cost + obj;
+        const v8f cost = bcj + (meta + sv * c);
+        const v8i jj = (v8i){(int)j0, (int)j0, (int)j0, (int)j0, (int)j0, (int)j0, (int)j0, (int)j0} + lanes;
+        // later chunks hold larger j: they win ties
+        const v8i take = (cost < best) | (cost == best);
+        best = take ? cost : best;
+        bestj = take ? jj : bestj;
+    }
+    float bc = 3.40282347e+38f;
+    long bj = -1;
+    for (int l = 0; l < 8; l++) {
+        if (bestj[l] < 0) continue;
+        if (best[l] < bc || (best[l] == bc && bestj[l] > bj)) {
+            bc = best[l];
+            bj = bestj[l];
+        }
+    }
+    // the j past the last full chunk, scalar (each larger than every j above)
+    for (size_t j = j0; j < n; j++) {
+        const float c = (float)(ci - cc[j]);
+        const float ans_cost = total_log2 - log2_approx(c);
+        const float sv = ans_cost + ob[j];
+        const float cost = best_cost[j] + (meta + sv * c);
+        if (cost < bc || (cost == bc && (long)j > bj)) {
+            bc = cost;
+            bj = (long)j;
+        }
+    }
+    return DpRow{bc, (size_t)bj};
+}
+#endif
+
+DpRow (*pick_dp_row())(const float *, const uint32_t *, const float *, size_t, uint32_t, float, float) {
+#if defined(__x86_64__)
+    if (__builtin_cpu_supports("avx2")) return dp_row_avx2;
+#endif
+    return dp_row_scalar;
+}
+
 template <class L> std::vector<Info<L>> optimize_bins(const std::vector<HBin<L>> &bins, Bitlen asl) {
+    static DpRow (*const dp_row)(const float *, const uint32_t *, const float *, size_t, uint32_t, float,
+                                 float) = pick_dp_row();
     const size_t nb = bins.size();
     std::vector<uint32_t> cc(nb + 1, 0);
-    std::vector<float> best_cost(nb + 1, 0.0f);
+    std::vector<float> best_cost(nb + 1, 0.0f), ob(nb + 8, 0.0f);
     uint32_t c = 0;
     for (size_t i = 0; i < nb; i++) {
         c += (uint32_t)bins[i].count;
@@ -1075,20 +1188,33 @@ template <class L> std::vector<Info<L>> optimize_bins(const std::vector<HBin<L>>
     const float total_log2 = log2_approx((float)c);
     std::vector<size_t> best_js(nb);
     const float meta = (float)bin_exact_bits<L>(asl);
+    std::vector<L> lowers(nb);
+    for (size_t i = 0; i < nb; i++) lowers[i] = bins[i].lower;
+    // pk[k]: the first j with upper_i - lower_j < 2^k. The uppers ascend with
+    // i, so each pointer only moves forward over the rows.
+    size_t pk[BITS<L> + 1];
+    for (Bitlen k = 0; k <= BITS<L>; k++) pk[k] = 0;
     for (size_t i = 0; i < nb; i++) {
-        float bc = 3.40282347e+38f;
-        size_t bj = (size_t)-1;
         const L upper = bins[i].upper;
         const uint32_t ci = cc[i + 1];
-        for (size_t j = i + 1; j-- > 0;) {
-            const float cost = best_cost[j] + bin_cost<L>(meta, bins[j].lower, upper, ci - cc[j], total_log2);
-            if (cost < bc) {
-                bc = cost;
-                bj = j;
+        // offset bits of bins j..i, bte_offset(upper - lower_j): the lowers
+        // ascend with j, so the bits are k on the run [pk[k], pk[k - 1]) of j
+        size_t hi = i + 1;
+        for (Bitlen k = bte_offset<L>((L)(upper - lowers[i])); hi > 0; k++) {
+            size_t p = 0;
+            if (k < BITS<L> && (upper >> k) != 0) {
+                const L t = (L)(upper - ((L)1 << k));  // lower_j > t <=> upper - lower_j < 2^k
+                size_t q = pk[k];
+                while (q < hi && lowers[q] <= t) q++;
+                pk[k] = q;
+                p = q;
             }
+            std::fill(ob.begin() + p, ob.begin() + hi, (float)k);
+            hi = p;
         }
-        best_cost[i + 1] = bc;
-        best_js[i] = bj;
+        const DpRow r = dp_row(best_cost.data(), cc.data(), ob.data(), i, ci, total_log2, meta);
+        best_cost[i + 1] = r.cost;
+        best_js[i] = r.j;
     }
     const float bcost = best_cost[nb];
     std::vector<std::pair<size_t, size_t>> part;
