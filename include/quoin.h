@@ -200,8 +200,12 @@ int quoin_gpu_decompress(QuoinGpu *g, const QuoinDeviceStream *streams, size_t n
                          void *stream);
 
 /* Launch form of the two calls above: 0 = batched (default: every block of every
- * column in one launch per kernel stage), 1 = plain (one launch sequence per
- * block, the per-block-launch baseline). */
+ * column in one launch per kernel stage; a batch shape met on two calls in a
+ * row is captured into a hipGraph on the second and replayed while it repeats,
+ * at Level::Fastest and Fast and for streams of device-coded frames),
+ * 1 = plain (one launch sequence per block, the per-block-launch baseline),
+ * 2 = batched, always launched kernel by kernel. Streams and decodes do not
+ * depend on the form. */
 int quoin_gpu_set_launch_form(QuoinGpu *g, int form);
 
 /* No reference counterpart: the autotuner of the tile kernels' launch shape
@@ -214,6 +218,10 @@ int quoin_gpu_set_autotune(QuoinGpu *g, int on);
 /* The tuned shapes: out[0] k_dec_fl<4>, out[1] k_dec_fl<8>, out[2] k_emit_fl
  * (0 = not tuned yet). */
 int quoin_gpu_tuned_shapes(const QuoinGpu *g, uint32_t out[3]);
+/* No reference counterpart: launch form 0's graphs so far, out[0] compress
+ * sequences captured, out[1] compress graph launches, out[2] / out[3] the
+ * same for decompress. */
+int quoin_gpu_graph_counts(const QuoinGpu *g, uint64_t out[4]);
 
 /* Device time (ms, HIP events on the handle's stream) of the last compress /
  * decompress call, and of its dominant kernel with its name. */

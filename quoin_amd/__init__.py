@@ -203,6 +203,7 @@ def _load():
                                           ctypes.POINTER(ctypes.c_int32), vp]
     lib.quoin_gpu_set_autotune.argtypes = [vp, ctypes.c_int]
     lib.quoin_gpu_tuned_shapes.argtypes = [vp, ctypes.c_void_p]
+    lib.quoin_gpu_graph_counts.argtypes = [vp, ctypes.c_void_p]
     lib.quoin_residual_bound.restype = sz
     lib.quoin_residual_bound.argtypes = [sz]
     lib.quoin_gpu_code_residuals.argtypes = [vp, ctypes.c_uint64, ctypes.c_int,
@@ -241,6 +242,7 @@ EXPORTS = [
     "quoin_gpu_linear_decode", "quoin_residual_bound", "quoin_gpu_code_residuals",
     "quoin_gpu_decode_residuals", "quoin_gpu_pred_encode", "quoin_gpu_pred_decode",
     "quoin_gpu_set_autotune", "quoin_gpu_tuned_shapes", "quoin_gpu_linear_encode_noalloc",
+    "quoin_gpu_graph_counts",
 ]
 
 
@@ -521,6 +523,14 @@ class Gpu:
         out = (ctypes.c_uint32 * 3)()
         lib.quoin_gpu_tuned_shapes(self.h, out)
         return {"k_dec_fl<4>": out[0], "k_dec_fl<8>": out[1], "k_emit_fl": out[2]}
+
+    def graph_counts(self):
+        """Launch form 0's hipGraphs so far: captures and graph launches of the
+        compress and decompress sequences."""
+        out = (ctypes.c_uint64 * 4)()
+        lib.quoin_gpu_graph_counts(self.h, out)
+        return {"compress_captures": out[0], "compress_launches": out[1],
+                "decompress_captures": out[2], "decompress_launches": out[3]}
 
     def last_timing(self):
         t, k = ctypes.c_double(), ctypes.c_double()

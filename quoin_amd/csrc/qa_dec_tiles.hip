@@ -552,17 +552,17 @@ __global__ void __launch_bounds__(64 * WPB) k_dec_fl(DecArgs a) {
 
 void qa_dec_desc_launch(const DecArgs &a, hipStream_t s) {
     if (a.t1 <= a.t0) return;
-    hipLaunchKernelGGL(k_dec_desc, dim3(nbt(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
+    qa_launch(k_dec_desc, dim3(nbt(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
 }
 
 // k_dec_fl<OW> over n tiles at WPB wavefronts per workgroup
 template <int OW>
 static void launch_fl(const DecArgs &b, uint64_t n, uint32_t wpb, hipStream_t s) {
     switch (wpb) {
-    case 1: hipLaunchKernelGGL((k_dec_fl<OW, 1>), dim3(nbt(n, 1)), dim3(64), 0, s, b); break;
-    case 2: hipLaunchKernelGGL((k_dec_fl<OW, 2>), dim3(nbt(n, 2)), dim3(128), 0, s, b); break;
-    case 8: hipLaunchKernelGGL((k_dec_fl<OW, 8>), dim3(nbt(n, 8)), dim3(512), 0, s, b); break;
-    default: hipLaunchKernelGGL((k_dec_fl<OW, 4>), dim3(nbt(n, 4)), dim3(256), 0, s, b);
+    case 1: qa_launch((k_dec_fl<OW, 1>), dim3(nbt(n, 1)), dim3(64), 0, s, b); break;
+    case 2: qa_launch((k_dec_fl<OW, 2>), dim3(nbt(n, 2)), dim3(128), 0, s, b); break;
+    case 8: qa_launch((k_dec_fl<OW, 8>), dim3(nbt(n, 8)), dim3(512), 0, s, b); break;
+    default: qa_launch((k_dec_fl<OW, 4>), dim3(nbt(n, 4)), dim3(256), 0, s, b);
     }
 }
 
@@ -594,7 +594,7 @@ void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
         if (tiles < QA_TUNE_MIN_TILES) continue;
         tune[ti] = qa_tune_pick([&](uint32_t w) { fl(kind, b, tiles, w); }, s);
     }
-    if (a.kev0) hipEventRecord((hipEvent_t)a.kev0, s);
+    qa_mark(a.kev0, s);
     const uint32_t w4 = tune && tune[QA_TUNE_DEC4] ? tune[QA_TUNE_DEC4] : 4;
     const uint32_t w8 = tune && tune[QA_TUNE_DEC8] ? tune[QA_TUNE_DEC8] : 4;
     auto launch = [&](uint32_t kind, uint64_t t0, uint64_t t1) {
@@ -603,7 +603,7 @@ void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
         b.t1 = t1;
         if (kind == 4) fl(4, b, t1 - t0, w4);
         else if (kind == 8) fl(8, b, t1 - t0, w8);
-        else hipLaunchKernelGGL(k_dec_tiles, dim3(nbt(t1 - t0, 4)), dim3(256), 0, s, b);
+        else qa_launch(k_dec_tiles, dim3(nbt(t1 - t0, 4)), dim3(256), 0, s, b);
     };
     if (a.krun_all) {
         const uint32_t kinds[3] = {4, 8, 0};
@@ -613,7 +613,7 @@ void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
             b.krun = a.krun_all + a.kr_off[i];
             b.nkrun = a.kr_n[i];
             b.ktotal = a.kr_tot[i];
-            if (kinds[i] == 0) hipLaunchKernelGGL(k_dec_tiles, dim3(nbt(b.ktotal, 4)), dim3(256), 0, s, b);
+            if (kinds[i] == 0) qa_launch(k_dec_tiles, dim3(nbt(b.ktotal, 4)), dim3(256), 0, s, b);
             else fl(kinds[i], b, b.ktotal, kinds[i] == 4 ? w4 : w8);
         }
     } else if (a.runs) {
@@ -627,5 +627,5 @@ void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s) {
         launch(8, a.t0, a.t1);
         launch(0, a.t0, a.t1);
     }
-    if (a.kev1) hipEventRecord((hipEvent_t)a.kev1, s);
+    qa_mark(a.kev1, s);
 }

@@ -128,17 +128,17 @@ __global__ void __launch_bounds__(DM_WG) k_decm_join(DecmArgs a) {
 
 void qa_decm_limbs_launch(const DecmArgs &a, hipStream_t s) {
     if (!a.n) {
-        hipLaunchKernelGGL(k_decm_min_final, dim3(1), dim3(64), 0, s, a, 0u);
+        qa_launch(k_decm_min_final, dim3(1), dim3(64), 0, s, a, 0u);
         return;
     }
     uint64_t g = (a.n + DM_WG - 1) / DM_WG;
     const uint32_t parts = (uint32_t)(g < QA_DECM_PARTS ? g : QA_DECM_PARTS);
-    hipLaunchKernelGGL(k_decm_min, dim3(parts), dim3(DM_WG), 0, s, a);
-    hipLaunchKernelGGL(k_decm_min_final, dim3(1), dim3(64), 0, s, a, parts);
-    hipLaunchKernelGGL(k_decm_limbs, dim3((unsigned)g), dim3(DM_WG), 0, s, a);
+    qa_launch(k_decm_min, dim3(parts), dim3(DM_WG), 0, s, a);
+    qa_launch(k_decm_min_final, dim3(1), dim3(64), 0, s, a, parts);
+    qa_launch(k_decm_limbs, dim3((unsigned)g), dim3(DM_WG), 0, s, a);
 }
 
 void qa_decm_join_launch(const DecmArgs &a, hipStream_t s) {
     if (!a.n) return;
-    hipLaunchKernelGGL(k_decm_join, dim3((unsigned)((a.n + DM_WG - 1) / DM_WG)), dim3(DM_WG), 0, s, a);
+    qa_launch(k_decm_join, dim3((unsigned)((a.n + DM_WG - 1) / DM_WG)), dim3(DM_WG), 0, s, a);
 }

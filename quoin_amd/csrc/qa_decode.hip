@@ -404,7 +404,7 @@ static inline unsigned nbk(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 #define QA_LAUNCH(k, grid, ...)                                   \
     do {                                                          \
         dim3 g_ = (grid);                                         \
-        if (g_.x) hipLaunchKernelGGL(k, g_, __VA_ARGS__);         \
+        if (g_.x) qa_launch(k, g_, __VA_ARGS__);         \
     } while (0)
 
 // column-level: validity blob -> bitmap (+ validity_out), valid counts, the

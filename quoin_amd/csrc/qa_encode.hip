@@ -1416,7 +1416,7 @@ static inline unsigned nb(uint64_t n, unsigned b) { return (unsigned)((n + b - 1
 #define QA_LAUNCH(k, grid, ...)                                   \
     do {                                                          \
         dim3 g_ = (grid);                                         \
-        if (g_.x) hipLaunchKernelGGL(k, g_, __VA_ARGS__);         \
+        if (g_.x) qa_launch(k, g_, __VA_ARGS__);         \
     } while (0)
 
 // column-level stages: validity counts, compaction, region statistics, plan
@@ -1479,7 +1479,7 @@ void qa_encode_layout_launch(const EncArgs &a, hipStream_t s) {
 // per-block stages over [r0, r1) / [t0, t1): frame headers and payloads
 void qa_encode_emit_launch(const EncArgs &a, hipStream_t s) {
     QA_LAUNCH(k_emit_frames, dim3(nb(a.r1 - a.r0, 64)), dim3(64), 0, s, a);
-    hipLaunchKernelGGL(k_tlist_clear, dim3(1), dim3(64), 0, s, a);
+    qa_launch(k_tlist_clear, dim3(1), dim3(64), 0, s, a);
     QA_LAUNCH(k_tile_desc, dim3(nb(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
     // k_emit_fl at the handle's tuned wavefronts per workgroup (tuned here on
     // the first batch with enough tiles: the stage is idempotent)
@@ -1494,14 +1494,14 @@ void qa_encode_emit_launch(const EncArgs &a, hipStream_t s) {
     };
     if (a.tune && !a.tune[QA_TUNE_EMIT] && a.t1 - a.t0 >= QA_TUNE_MIN_TILES)
         a.tune[QA_TUNE_EMIT] = qa_tune_pick(emit_fl, s);
-    if (a.kev0) hipEventRecord((hipEvent_t)a.kev0, s);
+    qa_mark(a.kev0, s);
     emit_fl(a.tune && a.tune[QA_TUNE_EMIT] ? a.tune[QA_TUNE_EMIT] : 4);
     {
         // persistent over the list: at most four groups per CU
         const unsigned g = nb(a.t1 - a.t0, 4);
         QA_LAUNCH(k_emit_tiles, dim3(g < 1024 ? g : 1024), dim3(256), 0, s, a);
     }
-    if (a.kev1) hipEventRecord((hipEvent_t)a.kev1, s);
+    qa_mark(a.kev1, s);
     qa_fast_emit_launch(a, s);
 }
 

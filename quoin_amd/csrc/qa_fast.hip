@@ -1508,7 +1508,7 @@ static inline unsigned nbf(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 #define QA_LAUNCH(k, grid, ...)                                   \
     do {                                                          \
         dim3 g_ = (grid);                                         \
-        if (g_.x) hipLaunchKernelGGL(k, g_, __VA_ARGS__);         \
+        if (g_.x) qa_launch(k, g_, __VA_ARGS__);         \
     } while (0)
 
 // before k_block_decide: the float-value codecs' statistics; then the decision

@@ -260,6 +260,6 @@ static inline unsigned nbd(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 
 void qa_fast_dec_index_launch(const DecArgs &a, hipStream_t s) {
     if (!a.fast || a.f1 <= a.f0) return;
-    hipLaunchKernelGGL(k_dec_index_fast, dim3(nbd(a.f1 - a.f0, 4)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_dec_frame_wg, dim3(a.f1 - a.f0), dim3(1024), 0, s, a);
+    qa_launch(k_dec_index_fast, dim3(nbd(a.f1 - a.f0, 4)), dim3(256), 0, s, a);
+    qa_launch(k_dec_frame_wg, dim3(a.f1 - a.f0), dim3(1024), 0, s, a);
 }

@@ -400,10 +400,10 @@ static inline unsigned nbe(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 void qa_fast_emit_launch(const EncArgs &a, hipStream_t s) {
     if (!a.fast) return;
     if (a.r1 > a.r0)
-        hipLaunchKernelGGL(k_emit_fast_frames, dim3(nbe(a.r1 - a.r0, 64)), dim3(64), 0, s, a);
+        qa_launch(k_emit_fast_frames, dim3(nbe(a.r1 - a.r0, 64)), dim3(64), 0, s, a);
     if (a.t1 > a.t0) {
-        hipLaunchKernelGGL(k_emit_fast_tiles, dim3((unsigned)(a.t1 - a.t0)), dim3(64), 0, s, a);
-        hipLaunchKernelGGL(k_emit_fast_chunks, dim3((unsigned)(a.t1 - a.t0)), dim3(64), 0, s, a);
+        qa_launch(k_emit_fast_tiles, dim3((unsigned)(a.t1 - a.t0)), dim3(64), 0, s, a);
+        qa_launch(k_emit_fast_chunks, dim3((unsigned)(a.t1 - a.t0)), dim3(64), 0, s, a);
     }
 }
 
@@ -430,7 +430,7 @@ __global__ void __launch_bounds__(256) k_lane_words(EncArgs a, uint64_t total, u
 void qa_lane_words_launch(const EncArgs &a, uint64_t total, uint64_t *out, hipStream_t s) {
     if (!total || !a.ncols) return;
     const uint64_t g = nbe(total, 256);
-    hipLaunchKernelGGL(k_lane_words, dim3((unsigned)(g < 8192 ? g : 8192)), dim3(256), 0, s, a,
+    qa_launch(k_lane_words, dim3((unsigned)(g < 8192 ? g : 8192)), dim3(256), 0, s, a,
                        total, out);
 }
 
@@ -449,5 +449,5 @@ __global__ void __launch_bounds__(256) k_host_frames(EncArgs a, const HScatter *
 
 void qa_host_frames_launch(const EncArgs &a, const HScatter *sc, uint32_t ns, const uint8_t *src,
                            hipStream_t s) {
-    if (ns) hipLaunchKernelGGL(k_host_frames, dim3(ns), dim3(256), 0, s, a, sc, src);
+    if (ns) qa_launch(k_host_frames, dim3(ns), dim3(256), 0, s, a, sc, src);
 }

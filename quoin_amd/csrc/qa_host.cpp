@@ -91,6 +91,18 @@ thread_local int t_last_err = 0;
 
 }  // namespace
 
+// A launch sequence replayed from a hipGraph: the sequence of a key (its kernel
+// arguments and launch shapes) met on two calls in a row is recorded on the
+// second and replayed while the key repeats, so the host enqueues one graph
+// instead of every kernel of the sequence.
+struct SeqGraph {
+    std::vector<uint8_t> key;   // the captured sequence's key
+    std::vector<uint8_t> seen;  // the key of the last eager run
+    hipGraphExec_t exec = nullptr;
+    std::vector<QaRecStep> steps;  // the recorded steps (their argument values) behind exec
+    uint64_t captures = 0, launches = 0;
+};
+
 struct QuoinGpu {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -131,12 +143,15 @@ struct QuoinGpu {
     hipEvent_t efork = nullptr, ejoin = nullptr;
     double last_total_ms = 0, last_kernel_ms = 0;
     const char *last_kernel = "";
+    // the batched compress and decompress sequences (launch form 0)
+    SeqGraph genc, gdec;
 };
 
 // The autotuner's measurement: one warm launch, then each candidate shape
 // twice, timed with HIP events on the stream; the fastest shape's best time wins
 // (ties to the smaller shape).
 uint32_t qa_tune_pick(const std::function<void(uint32_t)> &launch, hipStream_t s) {
+    if (qa_recorder()) return 0;  // nothing runs while a sequence is recorded: stays untuned
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return 4;
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -166,10 +181,131 @@ uint32_t qa_tune_pick(const std::function<void(uint32_t)> &launch, hipStream_t s
     return pick;
 }
 
+std::vector<QaRecStep> *&qa_recorder() {
+    static thread_local std::vector<QaRecStep> *r = nullptr;
+    return r;
+}
+
+// A timing event on the stream, or a step of the sequence being recorded.
+void qa_mark(void *ev, hipStream_t s) {
+    if (!ev) return;
+    if (std::vector<QaRecStep> *r = qa_recorder()) {
+        r->push_back(QaRecStep{nullptr, dim3(), dim3(), 0, nullptr, {}, ev});
+        return;
+    }
+    (void)hipEventRecord((hipEvent_t)ev, s);
+}
+
+// The key of a launch sequence: the bytes of its arguments, then the extras.
+template <typename A>
+static std::vector<uint8_t> seq_key(const A &a, std::initializer_list<uint64_t> extra,
+                                    const void *more = nullptr, size_t more_len = 0) {
+    std::vector<uint8_t> k(sizeof(A) + extra.size() * 8 + more_len);
+    memcpy(k.data(), &a, sizeof(A));
+    size_t o = sizeof(A);
+    for (uint64_t x : extra) {
+        memcpy(k.data() + o, &x, 8);
+        o += 8;
+    }
+    if (more_len) memcpy(k.data() + o, more, more_len);
+    return k;
+}
+
+static bool graph_trace() {
+    static const bool on = getenv("QUOIN_GRAPH_TRACE") != nullptr;
+    return on;
+}
+
+static bool graph_ok(hipError_t e, const char *what) {
+    if (e != hipSuccess && graph_trace())
+        fprintf(stderr, "quoin graph: %s: %s\n", what, hipGetErrorString(e));
+    return e == hipSuccess;
+}
+
+// A recorded sequence as a graph: its steps as a chain of kernel and event
+// record nodes. Null if the runtime refuses a node.
+static hipGraphExec_t seq_graph(const std::vector<QaRecStep> &steps) {
+    hipGraph_t gr = nullptr;
+    if (!graph_ok(hipGraphCreate(&gr, 0), "create")) return nullptr;
+    hipGraphNode_t prev = nullptr;
+    bool ok = true;
+    for (const QaRecStep &st : steps) {
+        hipGraphNode_t node = nullptr;
+        if (st.ev) {
+            ok = graph_ok(hipGraphAddEventRecordNode(&node, gr, prev ? &prev : nullptr, prev ? 1 : 0,
+                                                     (hipEvent_t)st.ev),
+                          "event record node");
+        } else {
+            hipKernelNodeParams p;
+            memset(&p, 0, sizeof p);
+            p.func = const_cast<void *>(st.func);
+            p.gridDim = st.grid;
+            p.blockDim = st.block;
+            p.sharedMemBytes = st.shm;
+            p.kernelParams = const_cast<void **>(st.params.data());
+            ok = graph_ok(hipGraphAddKernelNode(&node, gr, prev ? &prev : nullptr, prev ? 1 : 0, &p),
+                          "kernel node");
+        }
+        if (!ok) break;
+        prev = node;
+    }
+    hipGraphExec_t ex = nullptr;
+    if (ok && !graph_ok(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0), "instantiate"))
+        ex = nullptr;
+    hipGraphDestroy(gr);
+    return ok ? ex : nullptr;
+}
+
+// Run a batched launch sequence (launch(recording) enqueues it on s, or
+// records it). In launch form 0, a key met on the previous call is recorded
+// now, built into a graph (explicit nodes: no stream capture, which would
+// restrict what other threads may do meanwhile) and replayed from then on
+// while the key repeats. The call that tuned a launch shape changes the key,
+// so a recording never runs the tuner. A graph the runtime refuses leaves the
+// call eager; nothing runs while a sequence is recorded. Form 2 is always eager.
+static bool seq_run(QuoinGpu *g, SeqGraph &sg, std::vector<uint8_t> &&key, hipStream_t s,
+                    const std::function<void(bool)> &launch) {
+    if (g->form == 0) {
+        if (sg.exec && sg.key == key) {
+            sg.launches++;
+            return graph_ok(hipGraphLaunch(sg.exec, s), "replay");
+        }
+        if (sg.seen == key) {
+            std::vector<QaRecStep> steps;
+            qa_recorder() = &steps;
+            launch(true);
+            qa_recorder() = nullptr;
+            if (hipGraphExec_t ex = seq_graph(steps)) {
+                if (sg.exec) hipGraphExecDestroy(sg.exec);
+                sg.exec = ex;
+                sg.steps = std::move(steps);
+                sg.key = std::move(key);
+                sg.seen.clear();
+                sg.captures++;
+                sg.launches++;
+                return graph_ok(hipGraphLaunch(sg.exec, s), "first launch");
+            }
+            (void)hipGetLastError();  // a refused graph: the sequence runs eagerly
+        }
+        sg.seen = std::move(key);
+    }
+    launch(false);
+    return true;
+}
+
 extern "C" int quoin_gpu_set_autotune(QuoinGpu *g, int on) {
     if (!g) return QUOIN_ERR_NULL;
     std::lock_guard<std::recursive_mutex> lk(g->mu);
     g->autotune = on != 0;
+    return QUOIN_OK;
+}
+
+extern "C" int quoin_gpu_graph_counts(const QuoinGpu *g, uint64_t out[4]) {
+    if (!g || !out) return QUOIN_ERR_NULL;
+    out[0] = g->genc.captures;
+    out[1] = g->genc.launches;
+    out[2] = g->gdec.captures;
+    out[3] = g->gdec.launches;
     return QUOIN_OK;
 }
 
@@ -193,7 +329,7 @@ __global__ void k_zero_u64(uint64_t *p, uint64_t n) {
 }
 void zero_async(void *p, size_t bytes, hipStream_t s) {
     uint64_t n = (bytes + 7) / 8;
-    if (n) hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+    if (n) qa_launch(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                               (uint64_t *)p, n);
 }
 
@@ -395,6 +531,8 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
                      &g->hscat, &g->hsrc};
     for (DevBuf *b : all)
         if (b->p) hipFree(b->p);
+    for (SeqGraph *q : {&g->genc, &g->gdec})
+        if (q->exec) hipGraphExecDestroy(q->exec);
     hipEventDestroy(g->ev0);
     hipEventDestroy(g->ev1);
     hipEventDestroy(g->kev0);
@@ -409,7 +547,8 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
 
 extern "C" int quoin_gpu_set_launch_form(QuoinGpu *g, int form) {
     if (!g) return QUOIN_ERR_NULL;
-    if (form != 0 && form != 1) return QUOIN_ERR_UNSUPPORTED;
+    if (form < 0 || form > 2) return QUOIN_ERR_UNSUPPORTED;
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
     g->form = form;
     return QUOIN_OK;
 }
@@ -709,7 +848,8 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         !ensure(g->gsets, region_base * 2048ull) || !ensure(g->temit, tile_base * sizeof(TileEmit)) ||
         !ensure(g->tlist, (tile_base + 16) * 4))
         return QUOIN_ERR_DEVICE;
-    EncArgs a{};
+    EncArgs a;
+    memset(&a, 0, sizeof a);  // padding too: the bytes key the sequence's graph
     a.cols = dp<DCol>(g->cols);
     a.cst = dp<DColState>(g->cst);
     a.ncols = (uint32_t)ncols;
@@ -764,21 +904,37 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     if (hipMemcpyAsync(g->cols.p, hs, ncols * sizeof(DCol), hipMemcpyHostToDevice, s) != hipSuccess)
         return QUOIN_ERR_DEVICE;
     hipEventRecord(g->ev0, s);
-    zero_async(g->cst.p, ncols * sizeof(DColState), s);
-    zero_async(g->wins.p, 64 * 8, s);
+    auto zero_state = [&] {
+        zero_async(g->cst.p, ncols * sizeof(DColState), s);
+        zero_async(g->wins.p, 64 * 8, s);
+    };
     if (prep_only) {
+        zero_state();
         a.hybrid = 2;
         return hybrid_prepare(g, a, cols, level, plog2, lane_off, s, *prep_only);
     }
-    if (hybrid) return hybrid_encode(g, a, cols, level, plog2, sample, lane_off, out_lens, s);
-    if (g->form == 0) {
+    if (hybrid) {
+        zero_state();
+        return hybrid_encode(g, a, cols, level, plog2, sample, lane_off, out_lens, s);
+    }
+    if (g->form != 1) {
         a.kev0 = g->kev0;
         a.kev1 = g->kev1;
         a.tune = g->autotune ? g->tune : nullptr;
-        qa_encode_launch(a, s, level, g->side, g->efork, g->ejoin);
+        // a recorded sequence is one chain of nodes (no side-stream fork)
+        auto launch = [&](bool recording) {
+            zero_state();
+            qa_encode_launch(a, s, level, recording ? nullptr : g->side, g->efork, g->ejoin);
+        };
+        if (!seq_run(g, g->genc,
+                     seq_key(a, {(uint64_t)level, g->tune[0], g->tune[1], g->tune[2],
+                                 (uint64_t)g->autotune}),
+                     s, launch))
+            return QUOIN_ERR_DEVICE;
     } else {
         // plain form: column-level planning, then one launch sequence per
         // block for its competition and again for its payload
+        zero_state();
         qa_encode_plan_launch(a, s);
         std::vector<uint32_t> owner(region_base);
         if (region_base &&
@@ -822,7 +978,7 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
     g->last_total_ms = ms;
     g->last_kernel_ms = 0;
     g->last_kernel = "";
-    if (g->form == 0 && tile_base) {
+    if (g->form != 1 && tile_base) {
         float kms = 0;
         if (hipEventElapsedTime(&kms, g->kev0, g->kev1) == hipSuccess) {
             g->last_kernel_ms = kms;
@@ -1037,7 +1193,8 @@ static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t
         return QUOIN_ERR_DEVICE;
     for (size_t c = 0; c < ns; c++)
         if (cols[c].nvwords) cols[c].bitmap = dp<uint64_t>(g->dbitmaps) + cols[c].vword_base;
-    DecArgs a{};
+    DecArgs a;
+    memset(&a, 0, sizeof a);  // padding too: the bytes key the sequence's graph
     a.dst = dp<DDecState>(g->dstate);
     a.ncols = (uint32_t)ns;
     a.nframes = (uint32_t)frames.size();
@@ -1112,14 +1269,26 @@ static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t
     a.frames = (DFrame *)(dm + o_frames);
     if (!kr.empty()) a.krun_all = (KindRun *)(dm + o_kr);
     hipEventRecord(g->ev0, s);
-    zero_async(g->derr.p, 8, s);
-    zero_async(g->dstate.p, ns * sizeof(DDecState), s);
-    if (g->form == 0) {
+    if (g->form != 1) {
         a.kev0 = g->kev0;
         a.kev1 = g->kev1;
         a.tune = g->autotune ? g->tune : nullptr;
-        qa_decode_launch(a, s);
+        auto launch = [&](bool) {
+            zero_async(g->derr.p, 8, s);
+            zero_async(g->dstate.p, ns * sizeof(DDecState), s);
+            qa_decode_launch(a, s);
+        };
+        // the key holds the host run table's contents, not its address
+        DecArgs k = a;
+        k.runs = nullptr;
+        if (!seq_run(g, g->gdec,
+                     seq_key(k, {g->tune[0], g->tune[1], g->tune[2], (uint64_t)g->autotune},
+                             a.runs, a.runs ? a.nruns * sizeof(TileRun) : 0),
+                     s, launch))
+            return QUOIN_ERR_DEVICE;
     } else {
+        zero_async(g->derr.p, 8, s);
+        zero_async(g->dstate.p, ns * sizeof(DDecState), s);
         // plain form: one launch sequence per frame (the tile starts of
         // streams with nulls are a column-level select, run once)
         qa_decode_prep_launch(a, s);
@@ -1147,7 +1316,7 @@ static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t
     g->last_total_ms = ms;
     g->last_kernel_ms = 0;
     g->last_kernel = "";
-    if (g->form == 0 && tile_base && !frames.empty()) {
+    if (g->form != 1 && tile_base && !frames.empty()) {
         float kms = 0;
         if (hipEventElapsedTime(&kms, g->kev0, g->kev1) == hipSuccess) {
             g->last_kernel_ms = kms;

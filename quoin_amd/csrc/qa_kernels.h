@@ -3,9 +3,41 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <mutex>
+#include <tuple>
+#include <utility>
+#include <vector>
 
 #include "qa_common.h"
+
+// One step of a recorded launch sequence: a kernel launch with its argument
+// values, or (ev set) a timing event recorded after the steps before it.
+struct QaRecStep {
+    const void *func;
+    dim3 grid, block;
+    uint32_t shm;
+    std::shared_ptr<void> hold;  // the argument values
+    std::vector<void *> params;  // a pointer to each of them
+    void *ev;
+};
+// The calling thread's recorder: while it is set, qa_launch and qa_mark append
+// steps to it instead of enqueueing work (the steps become a hipGraph's nodes).
+std::vector<QaRecStep> *&qa_recorder();
+
+// Every kernel launch of the engine: enqueued on s, or recorded.
+template <typename... P, typename... A>
+inline void qa_launch(void (*k)(P...), dim3 grid, dim3 block, uint32_t shm, hipStream_t s,
+                      A &&...args) {
+    auto vals = std::make_shared<std::tuple<P...>>(std::forward<A>(args)...);
+    std::vector<void *> ps =
+        std::apply([](auto &...x) { return std::vector<void *>{(void *)&x...}; }, *vals);
+    if (std::vector<QaRecStep> *r = qa_recorder()) {
+        r->push_back(QaRecStep{(const void *)k, grid, block, shm, vals, std::move(ps), nullptr});
+        return;
+    }
+    (void)hipLaunchKernel((const void *)k, grid, block, ps.data(), shm, s);
+}
 
 size_t qa_scan_tmp_words(uint64_t n);
 void qa_exclusive_scan(const uint64_t *in, uint64_t n, uint64_t *out, uint64_t *tmp,
@@ -45,6 +77,8 @@ struct TileEmit {
 #define QA_TUNE_MIN_TILES 4096
 #include <functional>
 uint32_t qa_tune_pick(const std::function<void(uint32_t)> &launch, hipStream_t s);
+// a timing event (null: none) recorded on s, as an event node when s is capturing
+void qa_mark(void *ev, hipStream_t s);
 
 // Device pointers of one encode batch (all arrays sized by the host from upper
 // bounds; counts that depend on the data live in DColState).

@@ -100,7 +100,7 @@ extern "C" int quoin_gpu_linear_encode_noalloc(QuoinGpu *g, int32_t kind, int32_
     uint64_t *tmp = (uint64_t *)(base + o_s);
     if (hipMemcpyAsync(dd, d, nb * sizeof(LinD), hipMemcpyHostToDevice, s) != hipSuccess)
         return QUOIN_ERR_DEVICE;
-    hipLaunchKernelGGL(k_na_zero, dim3(1), dim3(64), 0, s, tmp, 2u);  // the scan tickets
+    qa_launch(k_na_zero, dim3(1), dim3(64), 0, s, tmp, 2u);  // the scan tickets
     qa_lin_encode_launch((uint32_t)kind, dd, dr, (uint32_t)nb, tiles, tsz, tpre, tmp, s);
     if (hipMemcpyAsync(r, dr, nb * sizeof(LinR), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)

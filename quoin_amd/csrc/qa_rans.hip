@@ -731,18 +731,18 @@ __global__ void k_put_bytes(BytePuts b) {
     if (i < b.n) b.p[i][0] = b.v[i];
 }
 void qa_put_bytes_launch(const BytePuts &b, hipStream_t s) {
-    if (b.n) hipLaunchKernelGGL(k_put_bytes, dim3(1), dim3(128), 0, s, b);
+    if (b.n) qa_launch(k_put_bytes, dim3(1), dim3(128), 0, s, b);
 }
 
 void qa_rans_decode_launch(const RansDecD *d, RansDecR *r, uint32_t ns, hipStream_t s) {
     if (!ns) return;
-    hipLaunchKernelGGL(k_rans_dec, dim3((ns + DG - 1) / DG), dim3(64), 0, s, d, r, ns);
+    qa_launch(k_rans_dec, dim3((ns + DG - 1) / DG), dim3(64), 0, s, d, r, ns);
 }
 
 void qa_rans_encode_launch(const RansEncD *d, RansEncR *r, uint64_t *tabs, uint8_t *scr, uint32_t ns,
                            hipStream_t s) {
     if (!ns) return;
-    hipLaunchKernelGGL(k_rans_hist, dim3(ns), dim3(256), 0, s, d, r, tabs);
-    hipLaunchKernelGGL(k_rans_enc, dim3((ns + EG - 1) / EG), dim3(64), 0, s, d, r, tabs, scr, ns);
-    hipLaunchKernelGGL(k_rans_copy, dim3(ns), dim3(256), 0, s, d, r, scr);
+    qa_launch(k_rans_hist, dim3(ns), dim3(256), 0, s, d, r, tabs);
+    qa_launch(k_rans_enc, dim3((ns + EG - 1) / EG), dim3(64), 0, s, d, r, tabs, scr, ns);
+    qa_launch(k_rans_copy, dim3(ns), dim3(256), 0, s, d, r, scr);
 }

@@ -522,12 +522,12 @@ __global__ void __launch_bounds__(64) k_lin_seq(const LinD *d, const LinR *r, ui
 template <uint32_t KIND>
 void enc_launch(const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles, uint64_t *tsz,
                 uint64_t *tpre, uint64_t *tmp, hipStream_t s) {
-    hipLaunchKernelGGL(k_lin_order<KIND>, dim3(nblk), dim3(256), 0, s, d, r);
+    qa_launch(k_lin_order<KIND>, dim3(nblk), dim3(256), 0, s, d, r);
     if (!ntiles) return;
-    hipLaunchKernelGGL((k_lin_enc<KIND, false>), dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, r,
+    qa_launch((k_lin_enc<KIND, false>), dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, r,
                        nblk, tsz);
     qa_exclusive_scan(tsz, ntiles, tpre, tmp, s);
-    hipLaunchKernelGGL((k_lin_enc<KIND, true>), dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, r,
+    qa_launch((k_lin_enc<KIND, true>), dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, r,
                        nblk, tpre);
 }
 
@@ -535,19 +535,19 @@ template <uint32_t KIND>
 void dec_launch(const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles, uint64_t *cnt,
                 uint64_t *pre, uint64_t *tmp, hipStream_t s, void *kev0, void *kev1) {
     if (ntiles)
-        hipLaunchKernelGGL(k_lin_vcount<KIND>, dim3((unsigned)ntiles), dim3(LD_WG), 0, s, d, nblk,
+        qa_launch(k_lin_vcount<KIND>, dim3((unsigned)ntiles), dim3(LD_WG), 0, s, d, nblk,
                            cnt);
     qa_exclusive_scan(cnt, ntiles, pre, tmp, s);
     if (ntiles)
-        hipLaunchKernelGGL(k_lin_vparse<KIND>, dim3((unsigned)ntiles), dim3(LD_WG), 0, s, d, r,
+        qa_launch(k_lin_vparse<KIND>, dim3((unsigned)ntiles), dim3(LD_WG), 0, s, d, r,
                            nblk, pre);
     const unsigned g64 = (nblk + 63) / 64;
-    hipLaunchKernelGGL(k_lin_dcheck<KIND>, dim3(g64), dim3(64), 0, s, d, r, nblk, pre);
+    qa_launch(k_lin_dcheck<KIND>, dim3(g64), dim3(64), 0, s, d, r, nblk, pre);
     if (kev0) (void)hipEventRecord((hipEvent_t)kev0, s);
     if constexpr (KIND == LIN_OD)
-        hipLaunchKernelGGL(k_lin_od, dim3(nblk), dim3(LE_WG), 0, s, d, r);
+        qa_launch(k_lin_od, dim3(nblk), dim3(LE_WG), 0, s, d, r);
     else
-        hipLaunchKernelGGL(k_lin_seq<KIND>, dim3(g64), dim3(64), 0, s, d, r, nblk);
+        qa_launch(k_lin_seq<KIND>, dim3(g64), dim3(64), 0, s, d, r, nblk);
     if (kev1) (void)hipEventRecord((hipEvent_t)kev1, s);
 }
 
@@ -717,16 +717,16 @@ void pred_enc_launch(const LinD *d, LinR *r, uint32_t nblk, uint64_t ntiles, uin
                      uint64_t *tmp, hipStream_t s) {
     if (!ntiles) return;
     const uint64_t total = ntiles * LE_TILE;
-    hipLaunchKernelGGL(k_pred_keys<PK>, dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, nblk,
+    qa_launch(k_pred_keys<PK>, dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, nblk,
                        (1u << log2) - 1, keys, idx);
     size_t bytes = sort_bytes;
     (void)rocprim::radix_sort_pairs(sort_tmp, bytes, keys, keys2, idx, idx2, (size_t)total, 0, 32, s);
-    hipLaunchKernelGGL(k_pred_res<PK>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, d,
+    qa_launch(k_pred_res<PK>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, d,
                        keys2, idx2, total, res);
-    hipLaunchKernelGGL((k_pred_enc<false>), dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, r, nblk, tsz,
+    qa_launch((k_pred_enc<false>), dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, r, nblk, tsz,
                        res);
     qa_exclusive_scan(tsz, ntiles, tpre, tmp, s);
-    hipLaunchKernelGGL((k_pred_enc<true>), dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, r, nblk, tpre,
+    qa_launch((k_pred_enc<true>), dim3((unsigned)ntiles), dim3(LE_WG), 0, s, d, r, nblk, tpre,
                        res);
 }
 
@@ -756,23 +756,23 @@ void qa_pred_decode_launch(uint32_t kind, const LinD *d, LinR *r, uint32_t nblk,
                            uint64_t *tmp, hipStream_t s, void *kev0, void *kev1) {
     // the varint parse and its checks are OrderedDelta's (no header byte)
     if (ntiles)
-        hipLaunchKernelGGL(k_lin_vcount<LIN_OD>, dim3((unsigned)ntiles), dim3(LD_WG), 0, s, d, nblk,
+        qa_launch(k_lin_vcount<LIN_OD>, dim3((unsigned)ntiles), dim3(LD_WG), 0, s, d, nblk,
                            cnt);
     qa_exclusive_scan(cnt, ntiles, pre, tmp, s);
     if (ntiles)
-        hipLaunchKernelGGL(k_lin_vparse<LIN_OD>, dim3((unsigned)ntiles), dim3(LD_WG), 0, s, d, r,
+        qa_launch(k_lin_vparse<LIN_OD>, dim3((unsigned)ntiles), dim3(LD_WG), 0, s, d, r,
                            nblk, pre);
     const unsigned g64 = (nblk + 63) / 64;
-    hipLaunchKernelGGL(k_lin_dcheck<LIN_OD>, dim3(g64), dim3(64), 0, s, d, r, nblk, pre);
+    qa_launch(k_lin_dcheck<LIN_OD>, dim3(g64), dim3(64), 0, s, d, r, nblk, pre);
     const uint64_t words = (uint64_t)nblk << log2;
     const uint64_t zg = (words + 255) / 256;
-    hipLaunchKernelGGL(k_pred_zero, dim3((unsigned)(zg < 65536 ? zg : 65536)), dim3(256), 0, s,
+    qa_launch(k_pred_zero, dim3((unsigned)(zg < 65536 ? zg : 65536)), dim3(256), 0, s,
                        tables, words);
     if (kev0) (void)hipEventRecord((hipEvent_t)kev0, s);
     if (kind == PRED_FCM)
-        hipLaunchKernelGGL(k_pred_seq<PRED_FCM>, dim3(g64), dim3(64), 0, s, d, r, nblk, log2, tables);
+        qa_launch(k_pred_seq<PRED_FCM>, dim3(g64), dim3(64), 0, s, d, r, nblk, log2, tables);
     else
-        hipLaunchKernelGGL(k_pred_seq<PRED_DFCM>, dim3(g64), dim3(64), 0, s, d, r, nblk, log2, tables);
+        qa_launch(k_pred_seq<PRED_DFCM>, dim3(g64), dim3(64), 0, s, d, r, nblk, log2, tables);
     if (kev1) (void)hipEventRecord((hipEvent_t)kev1, s);
 }
 

@@ -154,8 +154,8 @@ void qa_exclusive_scan_dn(const uint64_t *in, uint64_t nmax, const uint64_t *nde
                           uint64_t *tmp, hipStream_t s) {
     const unsigned grid = (unsigned)(nmax / SCAN_CHUNK + 1);
     const ScanJobs j{{in, nullptr}, {out, nullptr}};
-    hipLaunchKernelGGL(k_scan_reduce, dim3(grid), dim3(SCAN_WG), 0, s, j, nmax, ndev, tmp);
-    hipLaunchKernelGGL(k_scan_apply, dim3(grid), dim3(SCAN_WG), 0, s, j, nmax, ndev, tmp);
+    qa_launch(k_scan_reduce, dim3(grid), dim3(SCAN_WG), 0, s, j, nmax, ndev, tmp);
+    qa_launch(k_scan_apply, dim3(grid), dim3(SCAN_WG), 0, s, j, nmax, ndev, tmp);
 }
 
 // Two exclusive scans of n elements each in one launch pair (twice the
@@ -165,8 +165,8 @@ void qa_exclusive_scan2(const uint64_t *in0, uint64_t *out0, const uint64_t *in1
                         uint64_t n, uint64_t *tmp, hipStream_t s) {
     const unsigned grid = (unsigned)(n / SCAN_CHUNK + 1);
     const ScanJobs j{{in0, in1}, {out0, out1}};
-    hipLaunchKernelGGL(k_scan_reduce, dim3(grid, 2), dim3(SCAN_WG), 0, s, j, n, nullptr, tmp);
-    hipLaunchKernelGGL(k_scan_apply, dim3(grid, 2), dim3(SCAN_WG), 0, s, j, n, nullptr, tmp);
+    qa_launch(k_scan_reduce, dim3(grid, 2), dim3(SCAN_WG), 0, s, j, n, nullptr, tmp);
+    qa_launch(k_scan_apply, dim3(grid, 2), dim3(SCAN_WG), 0, s, j, n, nullptr, tmp);
 }
 
 void qa_exclusive_scan(const uint64_t *in, uint64_t n, uint64_t *out, uint64_t *tmp,

@@ -112,10 +112,67 @@ def test_many_columns_alternating_kinds(qa):
 def test_launch_form_rejects_unknown(qa):
     gpu = qa.Gpu(0)
     try:
-        with pytest.raises(qa.QuoinError):
-            gpu.set_launch_form(2)
+        for form in (3, -1):
+            with pytest.raises(qa.QuoinError):
+                gpu.set_launch_form(form)
     finally:
         gpu.close()
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_graph_replays_equal_eager(qa, level):
+    """Launch form 0 captures a batch shape met on two calls in a row into a
+    hipGraph and replays it while the shape repeats. Five compress calls of
+    one shape with new values each time, with calls of another shape, host
+    reads and synchronizations between them, give the streams of a form-2
+    handle (always kernel by kernel) and the oracle's; decodes replayed the
+    same way return the oracle's values and bitmaps, and the tile stage's
+    timing events are recorded on replays too."""
+    cfg = qa.Config(level=qa.Level(level))
+    n = 70000
+    graph, eager = qa.Gpu(0), qa.Gpu(0)
+    try:
+        eager.set_launch_form(2)
+        other = batch(3000, seed=7 + level)[:3]
+        for rep in range(5):
+            cols = batch(n, seed=300 + 10 * level + rep)  # one shape, new values
+            print(f"case graph compress level={level} rep={rep}", flush=True)
+            got = _gpu.device_compress(qa, graph, cols, cfg)
+            assert got == _gpu.device_compress(qa, eager, cols, cfg), f"rep {rep}: eager"
+            if rep >= 2:
+                _, kms, _ = graph.last_timing()
+                assert kms > 0, "tile-stage events of a replay"
+            if rep in (2, 4):
+                for i, ((v, bm), st) in enumerate(zip(cols, got)):
+                    assert st == orc.compress(v, ODT[v.dtype], bm, level), f"rep {rep} column {i}"
+            if rep == 3:  # another shape between two replays
+                small = _gpu.device_compress(qa, graph, other, cfg)
+                for i, ((v, bm), st) in enumerate(zip(other, small)):
+                    assert st == orc.compress(v, ODT[v.dtype], bm, level), f"other column {i}"
+        counts = graph.graph_counts()
+        print("graph counts after compress:", counts, flush=True)
+        assert counts["compress_captures"] >= 1 and counts["compress_launches"] >= 4, counts
+        dts = [qa.FROM_NP[v.dtype] for v, _ in cols]
+        ns = [len(v) for v, _ in cols]
+        want = [orc.decompress(st) for st in got]
+        for rep in range(4):
+            print(f"case graph decompress level={level} rep={rep}", flush=True)
+            outs = _gpu.device_decompress(qa, graph, got, dts, ns, True)
+            for i, ((v, bm), (o, vo)) in enumerate(zip(cols, outs)):
+                assert o.view(np.uint8).tobytes() == want[i][1].view(np.uint8).tobytes(), \
+                    f"rep {rep} column {i}"
+                assert o.tobytes() == v.tobytes(), f"rep {rep} column {i}: input"
+                if bm is not None:
+                    assert vo == bm.tobytes(), f"rep {rep} column {i}: bitmap"
+            if rep == 1:  # another shape between two replays
+                _gpu.device_decompress(qa, graph, small, [qa.FROM_NP[v.dtype] for v, _ in other],
+                                       [len(v) for v, _ in other], False)
+        counts = graph.graph_counts()
+        print("graph counts after decompress:", counts, flush=True)
+        assert counts["decompress_captures"] >= 1 and counts["decompress_launches"] >= 2, counts
+    finally:
+        graph.close()
+        eager.close()
 
 
 def test_autotuned_shapes_give_the_same_bytes(qa):
