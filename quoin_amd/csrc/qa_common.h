@@ -35,7 +35,11 @@ enum {
 };
 
 // error flags raised by kernels (host maps them to QUOIN_ERR_*)
-enum { QA_ERR_CORRUPT = 1, QA_ERR_UNSUPPORTED = 2, QA_ERR_OVERFLOW = 4 };
+// QA_ERR_STALLED: a look-back wait passed its bound (a device fault, not a
+// property of the stream)
+enum { QA_ERR_CORRUPT = 1, QA_ERR_UNSUPPORTED = 2, QA_ERR_OVERFLOW = 4, QA_ERR_STALLED = 8 };
+// spin bound of the look-back waits (each round sleeps and reloads)
+#define QA_SPIN_MAX (1u << 22)
 
 struct DCol {
     const void *values;       // typed input, device

@@ -157,6 +157,39 @@ __device__ __forceinline__ uint32_t select64(uint64_t w, uint32_t r) {
     return pos;
 }
 
+// Word w of a stream's bitmap, holding valid values [P, P + popcount): if the
+// start of a tile of an aligned stream is among them, its logical position.
+__device__ __forceinline__ void tile_start_in_word(const DecArgs &a, const DDecCol &col,
+                                                   uint64_t w, uint64_t P, uint64_t word) {
+    if (!col.tl_aligned || !word) return;
+    const uint64_t c = (P + QA_TILE - 1) & ~(uint64_t)(QA_TILE - 1);
+    const uint32_t cnt = (uint32_t)__popcll(word);
+    if (c == 0 || c >= P + cnt) return;
+    const uint64_t t = col.tile0 + c / QA_TILE;
+    if (t < col.tile1) a.tlpos[t] = w * 64 + select64(word, (uint32_t)(c - P));
+}
+
+// logical position of the c-th valid value of a stream with nulls (c < nv),
+// by a binary search of the words' valid prefix and a select in the word;
+// n (and a flagged error) when the bitmap does not hold it
+__device__ __forceinline__ uint64_t tile_lpos(const DecArgs &a, const DDecCol &col, uint64_t c) {
+    if (c == 0) return 0;  // the first tile also takes the leading nulls
+    if (col.tl_aligned) return a.tlpos[col.tile0 + c / QA_TILE];
+    const uint64_t *vp = a.vpre + col.vword_base;
+    uint64_t lo = 0, hi = col.nvwords - 1;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi + 1) >> 1;
+        if (vp[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    const uint64_t r = c - vp[lo], w = col.bitmap[lo];
+    if (r >= (uint64_t)__popcll(w)) {
+        atomicOr(a.err, QA_ERR_CORRUPT);
+        return col.n;
+    }
+    return lo * 64 + select64(w, (uint32_t)r);
+}
+
+
 // Output element of the column (its narrow Arrow type, src/lib.rs:492-506).
 __device__ __forceinline__ int out_form(const DDecCol &col) {
     return (int)qa_out_form(col.dtype, col.out_kind);

@@ -92,13 +92,10 @@ __global__ void k_dec_desc(DecArgs a) {
         d.nulls = st.nv != col.n;
         d.fl = tile_kind(d.mode, d.oform);
         if (d.nulls) {
-            // logical positions [lpos, next tile's lpos in the same stream, or n)
-            d.L0 = ts.lpos;
-            d.L1 = col.n;
-            if (t + 1 < a.ntiles) {
-                const DTileD &nx = a.tiles[t + 1];
-                if (nx.frame < st.fused_end && a.frames[nx.frame].col == fr.col) d.L1 = nx.lpos;
-            }
+            // logical positions [this tile's start, the next tile's, or n):
+            // the starts of its first value and of the first value after it
+            d.L0 = tile_lpos(a, col, d.base);
+            d.L1 = d.base + d.cnt < st.nv ? tile_lpos(a, col, d.base + d.cnt) : col.n;
         }
     }
     a.tdec[t] = d;

@@ -115,7 +115,7 @@ struct QuoinGpu {
     DevBuf cols, cst, lane, vpre, tpre, tpos, rsz, colbase, scan_tmp, rfeat, owner, blocks, tiles,
         wins, tilesf, blocksx, dsorted, rstart, hkeys, hcnt, rgrown, gsets, temit, tlist, dpart, dparts;
     // decode arenas
-    DevBuf dstate, dtiles, dtdec, dvpre, derr, dbitmaps, dvbpre, dvbval, dvruns, dscan, dfx, dscr,
+    DevBuf dstate, dtiles, dtdec, dvpre, dtlpos, dbitmaps, dvbval, dvruns, dscan, dfx, dscr,
         dscr2;
     // host-API staging
     DevBuf st_in, st_valid, st_out, st_vout;
@@ -419,7 +419,7 @@ struct ParsedStream {
 // counted. With a bitmap the valid count comes out of the blob, which the
 // device expands; the host then reads frame headers until the stream ends or a
 // header stops parsing, and the device keeps the frames the reference would
-// have read (k_dec_colfin), reporting the same failures.
+// have read (k_vb_scan), reporting the same failures.
 int parse_stream(const uint8_t *src, size_t len, ParsedStream &ps) {
     if (len < 16) return QUOIN_ERR_CORRUPT;
     if (memcmp(src, "FCR1", 4) != 0 || src[4] != 2) return QUOIN_ERR_CORRUPT;
@@ -454,6 +454,10 @@ int parse_stream(const uint8_t *src, size_t len, ParsedStream &ps) {
         // is truncated, a tag other than raw (0) / runs (1) is corrupt
         // (src/validity.rs:73-81)
         if (vlen == 0 || src[pos] > 1) return QUOIN_ERR_CORRUPT;
+        // a raw bitmap shorter than ceil(n / 8), or no run varint where runs
+        // must reach n > 0: truncated (src/validity.rs:83-113)
+        if (src[pos] == 0 && vlen - 1 < (ps.n + 7) / 8) return QUOIN_ERR_CORRUPT;
+        if (src[pos] == 1 && vlen == 1 && ps.n > 0) return QUOIN_ERR_CORRUPT;
         ps.vblob_off = pos;
         ps.vblob_len = vlen;
         pos += vlen;
@@ -521,8 +525,8 @@ extern "C" void quoin_gpu_destroy(QuoinGpu *g) {
     DevBuf *all[] = {&g->cols,  &g->cst,      &g->lane,   &g->vpre,    &g->tpre,    &g->tpos,
                      &g->rsz,   &g->colbase,  &g->scan_tmp, &g->rfeat, &g->owner,   &g->blocks,
                      &g->tiles, &g->wins,     &g->dtiles,  &g->dtdec,
-                     &g->dvpre, &g->derr,     &g->dbitmaps, &g->dscan, &g->st_in,   &g->st_valid,
-                     &g->dstate, &g->dvbpre, &g->dvbval, &g->dvruns, &g->tilesf, &g->blocksx, &g->dsorted,
+                     &g->dvpre, &g->dtlpos, &g->dbitmaps, &g->dscan, &g->st_in,   &g->st_valid,
+                     &g->dstate, &g->dvbval, &g->dvruns, &g->tilesf, &g->blocksx, &g->dsorted,
                      &g->rstart, &g->hkeys, &g->hcnt, &g->rgrown, &g->gsets, &g->temit, &g->tlist, &g->dpart, &g->dparts, &g->dfx,
                      &g->dscr, &g->dscr2, &g->dmeta,
                      &g->st_out, &g->st_vout, &g->rdesc, &g->rres, &g->rtabs, &g->rscr,
@@ -1133,6 +1137,7 @@ static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t
     std::vector<DDecCol> cols(ns);
     std::vector<DFrame> frames;
     uint64_t tile_base = 0, vword_base = 0, val_base = 0, vbyte_base = 0, avbyte_base = 0;
+    uint64_t vchunk_base = 0;
     uint64_t scr_base = 0;
     uint32_t any_fast = 0;
     for (size_t c = 0; c < ns; c++) {
@@ -1156,18 +1161,27 @@ static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t
         d.vword_base = vword_base;
         d.vbyte_base = vbyte_base;
         d.avbyte_base = avbyte_base;
+        d.vchunk_base = (uint32_t)vchunk_base;
         if (ps[c].has_validity) {
             d.nvwords = (uint32_t)((q.n + 63) / 64);
             d.vblob_off = ps[c].vblob_off;
             d.vblob_len = ps[c].vblob_len;
             if (d.vblob_len > 1) vbyte_base += d.vblob_len - 1;
+            // k_vb_scan's chunks: run bytes of an RLE blob, words of a raw one
+            if (d.nvwords)
+                vchunk_base += q.host_src[d.vblob_off] == 1
+                                   ? (d.vblob_len - 1 + QA_VB_CHUNK - 1) / QA_VB_CHUNK
+                                   : (d.nvwords + QA_VB_WORDS - 1) / QA_VB_WORDS;
         } else if (q.validity_out) {
             avbyte_base += (q.n + 7) / 8;  // all valid (src/lib.rs:503-506)
         }
         vword_base += d.nvwords;
         val_base += q.n;
         d.fbegin = (uint32_t)frames.size();
+        d.tile0 = tile_base;
+        d.tl_aligned = 1;
         for (DFrame f : ps[c].frames) {
+            if (f.out_off % QA_TILE) d.tl_aligned = 0;
             f.col = (uint32_t)c;
             f.tile_base = tile_base;
             f.scr = scr_base;
@@ -1179,14 +1193,20 @@ static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t
             frames.push_back(f);
         }
         d.fend = (uint32_t)frames.size();
+        d.tile1 = tile_base;
     }
     uint64_t nscan = std::max<uint64_t>(vword_base, vbyte_base) + 1;
-    if (!ensure(g->dstate, ns * sizeof(DDecState)) ||
+    if (vchunk_base > 0xFFFFFFFFull) return QUOIN_ERR_UNSUPPORTED;
+    // per-call device state, zeroed by one fill: the error flags, the chunk
+    // ticket, the column states and the chunks' look-back state
+    const size_t o_dst = 128, o_vch = o_dst + align256(ns * sizeof(DDecState));
+    const size_t state_bytes = o_vch + vchunk_base * sizeof(VbChunk);
+    if (!ensure(g->dstate, state_bytes) ||
         !ensure(g->dtiles, tile_base * sizeof(DTileD)) ||
         !ensure(g->dtdec, tile_base * sizeof(TileDec)) ||
-        !ensure(g->dvpre, (vword_base + 1) * 8) || !ensure(g->derr, 8) ||
-        !ensure(g->dbitmaps, (vword_base + 1) * 8) || !ensure(g->dvbpre, (vbyte_base + 1) * 8) ||
-        !ensure(g->dvbval, (vbyte_base + 1) * 8) || !ensure(g->dvruns, (vbyte_base + 1) * 8) ||
+        !ensure(g->dvpre, (vword_base + 1) * 8) || !ensure(g->dtlpos, (tile_base + 1) * 8) ||
+        !ensure(g->dbitmaps, (vword_base + 1) * 8) ||
+        !ensure(g->dvbval, (vbyte_base + ns + 1) * 8) || !ensure(g->dvruns, (vbyte_base + ns + 1) * 8) ||
         !ensure_scan(g->dscan, 2 * qa_scan_tmp_words(nscan) * 8, s) ||
         (any_fast && (!ensure(g->dfx, frames.size() * sizeof(DFrameX)) ||
                       !ensure(g->dscr, scr_base * 8) || !ensure(g->dscr2, scr_base * 8))))
@@ -1195,7 +1215,10 @@ static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t
         if (cols[c].nvwords) cols[c].bitmap = dp<uint64_t>(g->dbitmaps) + cols[c].vword_base;
     DecArgs a;
     memset(&a, 0, sizeof a);  // padding too: the bytes key the sequence's graph
-    a.dst = dp<DDecState>(g->dstate);
+    a.dst = (DDecState *)((uint8_t *)g->dstate.p + o_dst);
+    a.vchunks = (VbChunk *)((uint8_t *)g->dstate.p + o_vch);
+    a.vticket = (uint32_t *)((uint8_t *)g->dstate.p + 64);
+    a.nvchunks = (uint32_t)vchunk_base;
     a.ncols = (uint32_t)ns;
     a.nframes = (uint32_t)frames.size();
     a.ntiles = tile_base;
@@ -1204,12 +1227,12 @@ static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t
     a.nvbytes = vbyte_base;
     a.navbytes = avbyte_base;
     a.vpre = dp<uint64_t>(g->dvpre);
-    a.vbpre = dp<uint64_t>(g->dvbpre);
+    a.tlpos = dp<uint64_t>(g->dtlpos);
     a.vbval = dp<uint64_t>(g->dvbval);
     a.vruns = dp<uint64_t>(g->dvruns);
     a.tiles = dp<DTileD>(g->dtiles);
     a.tdec = dp<TileDec>(g->dtdec);
-    a.err = dp<uint32_t>(g->derr);
+    a.err = (uint32_t *)g->dstate.p;
     a.scan_tmp = dp<uint64_t>(g->dscan);
     if (any_fast) {
         a.fast = 1;
@@ -1274,21 +1297,20 @@ static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t
         a.kev1 = g->kev1;
         a.tune = g->autotune ? g->tune : nullptr;
         auto launch = [&](bool) {
-            zero_async(g->derr.p, 8, s);
-            zero_async(g->dstate.p, ns * sizeof(DDecState), s);
+            zero_async(g->dstate.p, state_bytes, s);
             qa_decode_launch(a, s);
         };
         // the key holds the host run table's contents, not its address
         DecArgs k = a;
         k.runs = nullptr;
         if (!seq_run(g, g->gdec,
-                     seq_key(k, {g->tune[0], g->tune[1], g->tune[2], (uint64_t)g->autotune},
+                     seq_key(k, {g->tune[0], g->tune[1], g->tune[2], (uint64_t)g->autotune,
+                                 (uint64_t)state_bytes},
                              a.runs, a.runs ? a.nruns * sizeof(TileRun) : 0),
                      s, launch))
             return QUOIN_ERR_DEVICE;
     } else {
-        zero_async(g->derr.p, 8, s);
-        zero_async(g->dstate.p, ns * sizeof(DDecState), s);
+        zero_async(g->dstate.p, state_bytes, s);
         // plain form: one launch sequence per frame (the tile starts of
         // streams with nulls are a column-level select, run once)
         qa_decode_prep_launch(a, s);
@@ -1302,12 +1324,11 @@ static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t
             return b;
         };
         for (size_t f = 0; f < frames.size(); f++) qa_decode_index_launch(one(f), s);
-        qa_decode_lpos_launch(a, s);
         for (size_t f = 0; f < frames.size(); f++) qa_decode_values_launch(one(f), s);
         qa_decode_final_launch(a, s);
     }
     hipEventRecord(g->ev1, s);
-    hipMemcpyAsync(hs + o_err, g->derr.p, 4, hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(hs + o_err, g->dstate.p, 4, hipMemcpyDeviceToHost, s);
     if (hipStreamSynchronize(s) != hipSuccess) return QUOIN_ERR_DEVICE;
     const uint32_t err = *(const uint32_t *)(hs + o_err);
     if (hipGetLastError() != hipSuccess) return QUOIN_ERR_DEVICE;
@@ -1323,6 +1344,7 @@ static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t
             g->last_kernel = "tile decode (k_dec_fl, k_dec_tiles)";
         }
     }
+    if (err & QA_ERR_STALLED) return QUOIN_ERR_DEVICE;
     if (err & QA_ERR_UNSUPPORTED) return QUOIN_ERR_UNSUPPORTED;
     if (err) return QUOIN_ERR_CORRUPT;
     return QUOIN_OK;

@@ -230,7 +230,26 @@ struct DDecCol {
     uint32_t dtype, nvwords;
     uint32_t out_kind;      // QA_IN_* narrow output element type
     uint32_t fbegin, fend;  // this column's frames
-    uint32_t pad;
+    uint32_t vchunk_base;   // first validity chunk of this column (k_vb_scan)
+    uint32_t tl_aligned;    // every frame starts at a multiple of 1024 values: the
+                            // column's tile m starts at valid value 1024 m
+    uint64_t tile0, tile1;  // the column's decode tiles [tile0, tile1)
+};
+
+// Validity blob decode: a column's blob is cut into chunks (QA_VB_CHUNK run
+// bytes, or QA_VB_WORDS bitmap words of a raw blob), one workgroup each,
+// chained by a decoupled look-back. A chunk's aggregate: runs (c), bits (s),
+// valid bits when its first run has an even index (v0) or an odd one (v1);
+// its inclusive prefix from the column's start: runs, bits, valid bits.
+#define QA_VB_CHUNK 4096u
+#define QA_VB_WORDS 512u
+// Every word carries bit 63 once written (values saturate below it), so a
+// reader takes a set only when each of its words is marked: no ordering
+// between the words is needed, and none is asked of the memory system.
+struct VbChunk {
+    uint64_t a[4];          // aggregate: c, s, v0, v1
+    uint64_t p[3];          // inclusive prefix: c, s, v
+    uint64_t pad;
 };
 
 // device-computed per-column decode state
@@ -252,10 +271,12 @@ struct DecArgs {
     uint64_t nvwords;
     uint64_t nvbytes;      // total validity blob bytes (after tags)
     uint64_t navbytes;     // validity_out bytes of streams without a bitmap
-    uint64_t *vpre;        // rank prefix (nvwords + 1)
-    uint64_t *vbpre;       // varint-end prefix over blob bytes (nvbytes + 1)
-    uint64_t *vbval;       // per blob byte: its varint's value if it ends one, then the prefix (nvbytes + 1)
-    uint64_t *vruns;       // run starts, by run index (nvbytes + 1)
+    uint64_t *vpre;        // per bitmap word: the column's valid values before it (nvwords + 1)
+    uint64_t *vbval;       // per run: the column's valid values before it (nvbytes + ncols + 1)
+    uint64_t *tlpos;       // per tile of an aligned stream with nulls: the logical
+                           // position of its first value (written with the bitmap)
+    uint64_t *vruns;       // per run: its start (nvbytes + ncols + 1); a column's runs
+                           // start at slot vbyte_base + column index
     DTileD *tiles;         // per decode tile
     TileDec *tdec;         // per decode tile
     uint32_t *err;         // error flags
@@ -263,7 +284,9 @@ struct DecArgs {
     DFrameX *fx;           // per frame
     uint64_t *scr, *scr2;  // per frame value slots (Dict values, RLE starts, XORZ, ALP-RD exceptions)
     uint32_t fast;         // some frame needs the Level::Fast decoders
-    uint32_t pad;
+    uint32_t nvchunks;     // validity chunks (k_vb_scan workgroups)
+    VbChunk *vchunks;      // per chunk look-back state, zeroed per call
+    uint32_t *vticket;     // chunk ticket counter, zeroed per call
     // launch range of the per-frame stages: frames [f0, f1), tiles [t0, t1)
     // (the whole batch, or one frame in the plain form)
     uint32_t f0, f1;
@@ -308,7 +331,6 @@ struct KindRun {
 void qa_decode_launch(const DecArgs &a, hipStream_t s);
 void qa_decode_prep_launch(const DecArgs &a, hipStream_t s);
 void qa_decode_index_launch(const DecArgs &a, hipStream_t s);
-void qa_decode_lpos_launch(const DecArgs &a, hipStream_t s);
 void qa_decode_values_launch(const DecArgs &a, hipStream_t s);
 void qa_decode_final_launch(const DecArgs &a, hipStream_t s);
 void qa_fast_dec_index_launch(const DecArgs &a, hipStream_t s);
