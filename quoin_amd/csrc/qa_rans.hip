@@ -307,6 +307,9 @@ __global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ de
     auto mx = [](uint64_t a, uint64_t b) { return a > b ? a : b; };
     const uint32_t kmax = (uint32_t)qa_wave_reduce64(klim, mx, 0);
     const uint32_t sh8 = 8u * (uint32_t)idx, below = (1u << sh8) - 1u;
+    // a lane's shift placed in its byte of the quad word, and the byte mask
+    // of the lower lanes (for the dot product that sums their shifts)
+    const uint32_t q8 = 8u << sh8, q16 = 16u << sh8, lowb = 0x01010101u & below;
     const int og = lane < 4 * DG ? g : DG;
     // slow-path threshold: states below 128 need 3+ bytes; irregular tables
     // always take the exact path
@@ -322,7 +325,9 @@ __global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ de
     };
 
     // a wavefront with an irregular table runs every round on the exact loop
-    const bool any_irr = __builtin_amdgcn_ballot_w64(irr != 0) != 0;
+    // (the fast rounds keep positions in bits, so payloads of 2^29 bytes or
+    // more take the exact loop)
+    const bool any_irr = __builtin_amdgcn_ballot_w64(irr != 0 || gqe >= (1ull << 29)) != 0;
     bool first = true;
     uint32_t k = 0;
     while (k < kmax) {
@@ -382,34 +387,78 @@ __global__ void __launch_bounds__(64) k_rans_dec(const RansDecD *__restrict__ de
             // a corrupt stream.
             const uint32_t k0 = k, xs0 = xs, pos0 = pos, sp0 = sp;
             uint32_t minx = 0xFFFFFFFFu;
-            for (; k < rend; k++, sp += 4) {
-                const bool act = k < klim;
-                const uint32_t amask = act ? 0x18u : 0u;
+            // One step of every lane. The payload position is kept in bits
+            // (posb = 8 pos: a funnel shift by posb takes the byte offset pos & 3).
+            uint32_t posb = pos << 3;
+            // the payload words at posb, read one step ahead of their use
+            uint32_t w0, w1, w2;
+            auto window = [&]() {
+                const uint32_t *wp = &L.ring[g][__builtin_amdgcn_ubfe(posb, 5, 8)];
+                w0 = wp[0];
+                w1 = wp[1];
+                w2 = wp[2];
+            };
+            window();
+            uint8_t *const outs = (uint8_t *)L.outs;
+            auto step = [&](uint32_t thr1, uint32_t thr2, uint32_t dead, uint32_t spo) {
                 const uint32_t e = L.tab[g][xs & (RM - 1)];
-                const uint32_t *wp = &L.ring[g][(pos >> 2) & (RING / 4 - 1)];
-                const uint32_t w0 = wp[0], w1 = wp[1], w2 = wp[2];
-                asm volatile("" ::"v"(w0), "v"(w1), "v"(w2), "v"(e));
                 const uint32_t xhi = xs >> 12;
                 // x' = f (x >> 12) + slot - cum = (f - 1) xhi + (xhi + slot - cum)
                 const uint32_t xn = __umul24((e >> 8) & 4095, xhi) + (xhi + (e >> 20));
-                minx = min(minx, act ? xn : 0xFFFFFFFFu);
-                ((uint8_t *)L.outs)[sp] = (uint8_t)e;
-                // renormalization shift in bits: 0 (xn >= 2^23), 8 or 16 (xn >= 2^7)
-                const uint32_t lz = (uint32_t)__builtin_clz(xn | 1u);
-                const uint32_t sh = (min(max(lz, 8u), 24u) - 1u) & amask;
+                // renormalization bytes: 1 below 2^23, 2 below 2^15 (3 or more,
+                // below 2^7, only on a corrupt stream: replayed). The shift in
+                // bits, then the same in this lane's byte of the quad word.
+                const bool b1 = xn < thr1, b2 = xn < thr2;
+                const uint32_t sh = b2 ? 16u : (b1 ? 8u : 0u);
                 uint32_t w = sh << sh8;
-                w |= (uint32_t)__builtin_amdgcn_mov_dpp((int)w, QA_DPP_QP_1032, 0xF, 0xF, false);
-                w |= (uint32_t)__builtin_amdgcn_mov_dpp((int)w, QA_DPP_QP_2301, 0xF, 0xF, false);
-                const uint32_t exb = __builtin_amdgcn_sad_u8(w & below, 0u, 0u);
-                const uint32_t totb = __builtin_amdgcn_sad_u8(w, 0u, 0u);
-                const uint32_t off = (pos & 3) + (exb >> 3);
-                const uint32_t lo = off < 4 ? w0 : (off < 8 ? w1 : w2);
-                const uint32_t hi = off < 4 ? w1 : w2;
-                const uint32_t d = __builtin_amdgcn_alignbyte(hi, lo, off);
-                const uint32_t bs = __builtin_amdgcn_perm(0u, d, 0x0c0c0001u);  // b0 << 8 | b1
-                xs = (xn << sh) | (bs >> (16 - sh));
-                pos += totb >> 3;
+                w |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)w, QA_DPP_QP_1032, 0xF, 0xF, true);
+                w |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)w, QA_DPP_QP_2301, 0xF, 0xF, true);
+                // bit offset of this lane's bytes: the lower lanes' shifts
+                const uint32_t exb = __builtin_amdgcn_udot4(w, lowb, 0u, false);
+                // (the state's chain above is issued first; what follows fills its
+                // waits)
+                __builtin_amdgcn_sched_barrier(0);
+                // the 8 payload bytes from pos, big-endian (byte pos in bits 56-63)
+                const uint64_t win =
+                    ((uint64_t)__builtin_amdgcn_perm(0u, __builtin_amdgcn_alignbit(w1, w0, posb), 0x00010203u)
+                     << 32) |
+                    __builtin_amdgcn_perm(0u, __builtin_amdgcn_alignbit(w2, w1, posb), 0x00010203u);
+                // the quad's bits, summed onto the position
+                posb = __builtin_amdgcn_udot4(w, 0x01010101u, posb, false);
+                minx = min(minx, xn | dead);
+                outs[spo] = (uint8_t)e;
+                // this lane's bytes lead the shifted window; the new state is
+                // x' followed by them, shifted left by sh (x' < 2^(32 - sh) when
+                // sh > 0)
+                const uint64_t xw = ((uint64_t)xn << 32) | (uint32_t)((win << exb) >> 32);
+                xs = (uint32_t)((xw << sh) >> 32);
+                window();
+            };
+            if (__builtin_amdgcn_ballot_w64(klim > k && klim < rend)) {
+                for (; k < rend; k++, sp += 4) {
+                    const bool act = k < klim;
+                    uint32_t thr1 = act ? RL : 0u, thr2 = act ? (1u << 15) : 0u;
+                    // kept as per-lane values, so each compare feeds its select
+                    // directly (no scalar mask arithmetic on the state's chain)
+                    asm volatile("" : "+v"(thr1), "+v"(thr2));
+                    step(thr1, thr2, act ? 0u : 0xFFFFFFFFu, sp);
+                }
+            } else {
+                const bool act = k < klim;
+                uint32_t thr1 = act ? RL : 0u, thr2 = act ? (1u << 15) : 0u;
+                const uint32_t dead = act ? 0u : 0xFFFFFFFFu;
+                asm volatile("" : "+v"(thr1), "+v"(thr2));
+                // four steps a trip (a round's steps end at a multiple of 64, or
+                // at kmax)
+                for (; k + 4 <= rend; k += 4, sp += 16) {
+                    step(thr1, thr2, dead, sp);
+                    step(thr1, thr2, dead, sp + 4);
+                    step(thr1, thr2, dead, sp + 8);
+                    step(thr1, thr2, dead, sp + 12);
+                }
+                for (; k < rend; k++, sp += 4) step(thr1, thr2, dead, sp);
             }
+            pos = posb >> 3;
             if (__builtin_amdgcn_ballot_w64(minx < 128u)) {
                 k = k0;
                 xs = xs0;
