@@ -320,23 +320,31 @@ __device__ __forceinline__ uint32_t rd_code_bits(uint32_t dl) {  // alp_rd.rs:72
     return b > 1 ? b : 1;
 }
 
-__device__ void lds_bitonic_sort(uint64_t *s, uint32_t n_pow2) {
+// Bitonic sort of n_pow2 (a power of two >= 2) keys in LDS, ascending, by the
+// whole workgroup (its size a multiple of 64). Thread t takes compare pairs
+// t, t + blockDim, ...: pair p joins elements lo and lo + j. For j <= 64 the
+// 64 pairs of a wavefront lie in 128 consecutive elements, so those stages
+// need only the wavefront's own LDS order; wider ones take barriers.
+template <typename T>
+__device__ void lds_bitonic_sort(T *s, uint32_t n_pow2) {
     for (uint32_t k = 2; k <= n_pow2; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < n_pow2; i += blockDim.x) {
-                uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    bool up = (i & k) == 0;
-                    uint64_t x = s[i], y = s[ixj];
-                    if ((x > y) == up) {
-                        s[i] = y;
-                        s[ixj] = x;
-                    }
+            const bool wide = j >= 128;
+            if (wide) __syncthreads();
+            for (uint32_t p = threadIdx.x; p < n_pow2 / 2; p += blockDim.x) {
+                const uint32_t lo = ((p & ~(j - 1)) << 1) | (p & (j - 1)), hi = lo + j;
+                const bool up = (lo & k) == 0;
+                const T x = s[lo], y = s[hi];
+                if ((x > y) == up) {
+                    s[lo] = y;
+                    s[hi] = x;
                 }
             }
-            __syncthreads();
+            if (wide) __syncthreads();
+            else wave_lds_sync();
         }
     }
+    __syncthreads();
 }
 
 __device__ __forceinline__ bool fast_float_block(const EncArgs &a, const DBlock &b) {
@@ -345,7 +353,9 @@ __device__ __forceinline__ bool fast_float_block(const EncArgs &a, const DBlock 
 }
 
 __global__ void __launch_bounds__(1024) k_alprd_cut(EncArgs a) {
-    __shared__ uint64_t s[4096];
+    // the samples' top 32 bits: sorted, they order every cut's left part
+    // (at most 24 bits)
+    __shared__ uint32_t s[4096];
     __shared__ double bits_of[24];
     const uint32_t slot = a.r0 + blockIdx.x;
     if (slot >= a.r1) return;
@@ -356,63 +366,60 @@ __global__ void __launch_bounds__(1024) k_alprd_cut(EncArgs a) {
     const uint64_t n = b.len;
     const uint64_t step = n / 2048 > 1 ? n / 2048 : 1;
     const uint32_t sn = (uint32_t)((n + step - 1) / step);
-    for (uint32_t i = threadIdx.x; i < 4096; i += blockDim.x)
-        s[i] = i < sn ? lane_at(st, b.start + i * step) : ~0ull;
-    __syncthreads();
-    uint32_t np2 = 1;
+    uint32_t np2 = 2;
     while (np2 < sn) np2 <<= 1;
-    lds_bitonic_sort(s, np2 < 2 ? 2 : np2);
+    for (uint32_t i = threadIdx.x; i < np2; i += blockDim.x)
+        s[i] = i < sn ? (uint32_t)(lane_at(st, b.start + i * step) >> 32) : ~0u;
+    __syncthreads();
+    lds_bitonic_sort(s, np2);
     // Every cut's left parts are sorted too, so its histogram is the runs of
     // equal keys: wavefront w takes cuts w + 1 and w + 17. Run starts come
     // from a ballot per 64 samples; the cost needs only the distinct count and
-    // the sum of the 8 largest counts (ties do not change the sum), kept as a
-    // wavefront-uniform sorted top 8.
+    // the sum of the 8 largest counts (ties do not change the sum). Each lane
+    // keeps its own top 8 (descending, in registers) of the runs it closes;
+    // the wavefront's top 8 are then taken by 8 rounds of a wavefront maximum.
     const int lane = threadIdx.x & 63, wv = qa_wave_id();
+    auto keep = [](uint32_t (&top)[8], uint32_t v) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const uint32_t hi = v > top[q] ? v : top[q];
+            v = v > top[q] ? top[q] : v;
+            top[q] = hi;
+        }
+    };
     for (uint32_t left = (uint32_t)wv + 1; left <= 24; left += 16) {
-        const uint32_t rbw = 64 - left;
+        const uint32_t rbw = 64 - left, ksh = rbw - 32;
         uint32_t top[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         uint32_t distinct = 0, open = 0;  // start of the run still open
         for (uint32_t c = 0; c < sn; c += WAVE) {
             const uint32_t i = c + (uint32_t)lane;
-            const bool st = i < sn && (i == 0 || (s[i] >> rbw) != (s[i - 1] >> rbw));
+            const bool st = i < sn && (i == 0 || (s[i] >> ksh) != (s[i - 1] >> ksh));
             const uint64_t m = __ballot(st);
             distinct += (uint32_t)__builtin_popcountll(m);
-            // runs completed in this chunk: from each start to the next one
+            // a run starting here and ending in this chunk; lane 63 (whose run,
+            // if any, stays open) takes the open run that this chunk's first
+            // start closes
             uint32_t len = 0;
-            if (st) {
-                const uint64_t later = lane == 63 ? 0 : m & (~0ull << (lane + 1));
+            if (st && lane < 63) {
+                const uint64_t later = m & (~0ull << (lane + 1));
                 if (later) len = (uint32_t)__builtin_ctzll(later) - (uint32_t)lane;
             }
-            uint32_t closed = 0;  // the open run, ended by this chunk's first start
-            if (m && c > 0) closed = c + (uint32_t)__builtin_ctzll(m) - open;
+            if (lane == 63 && m && c > 0) len = c + (uint32_t)__builtin_ctzll(m) - open;
             if (m) open = c + 63 - (uint32_t)__builtin_clzll(m);
-            for (int pass = 0; pass < 2; pass++) {
-                uint64_t cand = pass == 0 ? (closed > top[7] ? 1ull : 0ull) : __ballot(len > top[7]);
-                while (cand) {
-                    const int src = __builtin_ctzll(cand);
-                    cand &= cand - 1;
-                    uint32_t v = pass == 0 ? closed : (uint32_t)__builtin_amdgcn_readlane((int)len, src);
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        const uint32_t hi = v > top[q] ? v : top[q];
-                        v = v > top[q] ? top[q] : v;
-                        top[q] = hi;
-                    }
-                }
-            }
+            keep(top, len);
         }
-        {  // the last run ends at the sample's end
-            uint32_t v = sn - open;
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const uint32_t hi = v > top[q] ? v : top[q];
-                v = v > top[q] ? top[q] : v;
-                top[q] = hi;
-            }
-        }
+        keep(top, lane == 63 ? sn - open : 0u);  // the last run ends at the sample's end
         uint64_t covered = 0;
+        for (int r = 0; r < 8; r++) {
+            const uint32_t mx = (uint32_t)wave_reduce((uint64_t)top[0], OpMaxU());
+            covered += mx;
+            const uint64_t who = __ballot(top[0] == mx);
+            if (lane == __builtin_ctzll(who)) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) covered += top[q];
+                for (int q = 0; q < 7; q++) top[q] = top[q + 1];
+                top[7] = 0;
+            }
+        }
         const uint32_t dl = distinct < 8 ? distinct : 8;
         const uint64_t exc = sn - covered;
         const uint64_t num = (uint64_t)sn * (rbw + rd_code_bits(dl)) + exc * 96;
@@ -448,9 +455,10 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {  // SplitMix64 finalizer
     return x;
 }
 
-// top-8 by (count desc, key asc) over a (keys, counts) table; keys < 2^24
-__device__ void top8_select(uint64_t *keys, uint32_t *cnt, uint32_t cap, uint64_t *out,
-                            uint32_t *outn) {
+// top-8 by (count desc, key asc) over a (keys, counts) table; keys < 2^24,
+// empty slots hold all-ones
+template <typename K>
+__device__ void top8_select(K *keys, uint32_t *cnt, uint32_t cap, uint64_t *out, uint32_t *outn) {
     __shared__ unsigned long long best;
     uint32_t got = 0;
     for (int r = 0; r < 8; r++) {
@@ -458,7 +466,7 @@ __device__ void top8_select(uint64_t *keys, uint32_t *cnt, uint32_t cap, uint64_
         __syncthreads();
         unsigned long long my = 0;
         for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
-            if (keys[i] == HEMPTY || cnt[i] == 0) continue;
+            if (keys[i] == (K)HEMPTY || cnt[i] == 0) continue;
             unsigned long long p = ((unsigned long long)cnt[i] << 24) | (0xFFFFFFull - keys[i]);
             my = p > my ? p : my;
         }
