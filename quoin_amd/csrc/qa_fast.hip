@@ -841,8 +841,9 @@ __device__ void dict_values_size(const EncArgs &a, uint32_t slot, const uint64_t
 // the eight of a batch are in flight together
 template <int LK>
 __device__ __forceinline__ void bucket_loop(const void *lp, uint32_t lk, uint64_t s0, uint64_t n,
-                                            uint64_t *bits) {
+                                            uint64_t *bits, uint32_t *has_max) {
     const uint64_t bd = blockDim.x;
+    bool hm = false;
     for (uint64_t i0 = threadIdx.x; i0 < n; i0 += 8 * bd) {
         uint64_t v[8];
 #pragma unroll
@@ -853,10 +854,12 @@ __device__ __forceinline__ void bucket_loop(const void *lp, uint32_t lk, uint64_
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             if (i0 + u * bd >= n) break;
+            hm |= v[u] == HEMPTY;
             const uint32_t bk = (uint32_t)(mix64(v[u]) >> 45);
             atomicOr((unsigned long long *)&bits[bk >> 6], 1ull << (bk & 63));
         }
     }
+    if (hm) *has_max = 1;
 }
 
 // slot of value v in the 8192-slot LDS table (any function of v gives the
@@ -882,6 +885,9 @@ __device__ __forceinline__ void dh_insert(uint64_t *keys, uint64_t v, uint32_t *
             }
             if (old == v) return;
         }
+        // once the set has overflowed it is discarded: stop probing (a full
+        // table would otherwise be probed through for every later value)
+        if (*(volatile uint32_t *)overflow) return;
         h = (h + 1) & (DH_CAP - 1);
     }
 }
@@ -915,7 +921,10 @@ __device__ __forceinline__ void insert_loop(const void *lp, uint32_t lk, uint64_
 // Lower bound on the distinct count of values [s0, s0 + n): the number of
 // occupied buckets among 2^19 hash buckets (distinct buckets need distinct
 // values). bits: 8192 LDS words. Every thread returns the bound.
-__device__ uint64_t dict_bucket_bound(const DColState &st, uint64_t s0, uint64_t n, uint64_t *bits) {
+// (it reads every value, so it also reports whether the all-ones value occurs:
+// *has_max, raised once per block by any thread)
+__device__ uint64_t dict_bucket_bound(const DColState &st, uint64_t s0, uint64_t n, uint64_t *bits,
+                                      uint32_t *has_max) {
     __shared__ unsigned long long lb;
     for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x) bits[i] = 0;
     if (threadIdx.x == 0) lb = 0;
@@ -923,11 +932,11 @@ __device__ uint64_t dict_bucket_bound(const DColState &st, uint64_t s0, uint64_t
     const void *lp = st.lanep;
     const uint32_t lk = st.lkind;
     switch (lk) {
-    case QA_LK_U64: bucket_loop<QA_LK_U64>(lp, lk, s0, n, bits); break;
-    case QA_LK_U32: bucket_loop<QA_LK_U32>(lp, lk, s0, n, bits); break;
-    case QA_LK_I32: bucket_loop<QA_LK_I32>(lp, lk, s0, n, bits); break;
-    case QA_LK_F32: bucket_loop<QA_LK_F32>(lp, lk, s0, n, bits); break;
-    default: bucket_loop<QA_LK_ANY>(lp, lk, s0, n, bits);
+    case QA_LK_U64: bucket_loop<QA_LK_U64>(lp, lk, s0, n, bits, has_max); break;
+    case QA_LK_U32: bucket_loop<QA_LK_U32>(lp, lk, s0, n, bits, has_max); break;
+    case QA_LK_I32: bucket_loop<QA_LK_I32>(lp, lk, s0, n, bits, has_max); break;
+    case QA_LK_F32: bucket_loop<QA_LK_F32>(lp, lk, s0, n, bits, has_max); break;
+    default: bucket_loop<QA_LK_ANY>(lp, lk, s0, n, bits, has_max);
     }
     __syncthreads();
     uint64_t pc = 0;
@@ -1096,6 +1105,7 @@ __device__ int dict_ranges(const DColState &st, uint64_t s0, uint64_t n, uint64_
 __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
     __shared__ uint64_t keys[DH_CAP];
     __shared__ uint32_t card_l, overflow, has_max;
+    __shared__ uint32_t hmax_b;  // the all-ones value occurs (seen by a full bucket pass)
     const uint32_t slot = a.r0 + blockIdx.x;
     if (slot >= a.r1) return;
     const DBlock &b = a.blocks[slot];
@@ -1114,8 +1124,9 @@ __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
     // a block whose probe saw many distinct values first tries the cheap
     // certificate that its cardinality exceeds n / 2 + 1 (dict.rs bails then)
     bool lb_known = false;
+    if (threadIdx.x == 0) hmax_b = 0;  // ordered by the bound's first barrier
     if (!b.distinct_low && !parts_ok) {
-        const uint64_t lb = dict_bucket_bound(st, b.start, n, keys);
+        const uint64_t lb = dict_bucket_bound(st, b.start, n, keys, &hmax_b);
         if (lb > max_card) {
             if (threadIdx.x == 0) {
                 x.dict_state = 1;
@@ -1163,14 +1174,18 @@ __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
     } else {
         // phase 2: a lower bound on the cardinality from 2^19 hash buckets
         // (already known to be within the limit when it ran first)
-        const uint64_t lb = lb_known ? 0 : dict_bucket_bound(st, b.start, n, keys);
+        const uint64_t lb = lb_known ? 0 : dict_bucket_bound(st, b.start, n, keys, &hmax_b);
+        // phase 1 stopped at the overflow, so its flag may have missed the
+        // all-ones value; the bucket pass saw every value
+        __syncthreads();
+        const uint32_t hm = has_max | hmax_b;
         int ranged = 2;
         if (lb > max_card) {
             bail = true;
         } else {
             // phase 3: exact sets of value ranges in LDS
             uint64_t rc = 0;
-            ranged = dict_ranges(st, b.start, n, max_card, has_max, keys, sorted, &rc);
+            ranged = dict_ranges(st, b.start, n, max_card, hm, keys, sorted, &rc);
             if (ranged == 0) card = rc;
             if (ranged == 1) bail = true;
         }
@@ -1195,7 +1210,7 @@ __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
                     unsigned long long old =
                         atomicCAS((unsigned long long *)&gk[h], HEMPTY, (unsigned long long)v);
                     if (old == HEMPTY) {
-                        if (atomicAdd(&card_l, 1u) + 1 + has_max > max_card) overflow = 1;
+                        if (atomicAdd(&card_l, 1u) + 1 + hm > max_card) overflow = 1;
                         break;
                     }
                     if (old == v) break;
@@ -1204,7 +1219,7 @@ __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
             }
             __threadfence_block();
             __syncthreads();
-            card = card_l + has_max;
+            card = card_l + hm;
             bail = overflow || card > max_card;
             if (!bail) {
                 // compact, pad to a power of two with the all-ones value, sort

@@ -203,3 +203,22 @@ def test_fixed_block_sizes_fast(qa):
     v = _data.float_columns(20000, seed=2)["walk_f64"]
     for bs in (1, 100, 1000, 1024, 5000, 131072):
         enc(qa, f"walk bs{bs}", v, block_size=bs, decode=DECODE)
+
+
+def test_dict_above_lds_limit_keeps_the_all_ones_value(qa):
+    """A block of about 9,600 distinct 40-bit values (above the LDS set's
+    limit, below n / 2 + 1, so Dict is sized by value ranges) holding the
+    all-ones value (i64 -1) only late in the block: the dictionary must count
+    it. Dict wins here; the stream must equal the oracle's."""
+    r = np.random.default_rng(11)
+    n = 32768
+    words = r.integers(0, 2**40, 10_000, dtype=np.int64)
+    v = words[r.integers(0, len(words), n)]
+    v[30_000] = -1
+    s = enc(qa, "dict_allones_late", v)
+    assert "DICT" in modes_of(s), modes_of(s)
+    # the same values with the all-ones value early (found before the LDS set overflows)
+    v2 = v.copy()
+    v2[30_000] = v[0]
+    v2[5] = -1
+    enc(qa, "dict_allones_early", v2)
