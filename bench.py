@@ -27,33 +27,96 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+GENERATOR = ("xoshiro256** seeded by SplitMix64, Box-Muller normals, Lemire bounded integers "
+             "(bench_gen.c); checksums: 64-bit FNV-1a over little-endian words")
+_G = None
 
-def make_workload(n, seed=0):
-    r = np.random.default_rng(seed)
-    ts = (1_700_000_000_000_000_000 + np.cumsum(r.integers(1, 1001, n))).astype(np.int64)
-    cols = [("ts_i64", ts, None)]
-    for k in (3, 11, 23):
-        v = (r.integers(0, 2**32, n, dtype=np.uint64) >> np.uint64(32 - k)).astype(np.uint32)
-        valid = r.random(n) >= 0.01
-        v[~valid] = 0
-        bm = np.packbits(valid.astype(np.uint8), bitorder="little")
+
+def _gen():
+    """The column generator (bench_gen.so, built by __graft_entry__.build())."""
+    global _G
+    if _G is None:
+        path = os.path.join(ROOT, "bench_gen.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built: run __graft_entry__.build()")
+        g = ctypes.CDLL(path)
+        u64, dbl, vp = ctypes.c_uint64, ctypes.c_double, ctypes.c_void_p
+        for name in ("walk_f64", "timestamps_i64", "decimal_f64", "full_f64", "dict_runs_f64",
+                     "sine_f64", "unif_f32", "raw"):
+            getattr(g, "qg_" + name).argtypes = [u64, u64, vp]
+        g.qg_kbits_u32.argtypes = [u64, u64, ctypes.c_uint32, vp]
+        g.qg_validity.argtypes = [u64, u64, dbl, vp, vp]
+        g.qg_loguniform.argtypes = [u64, u64, dbl, dbl, vp]
+        g.qg_checksum.argtypes = [vp, u64]
+        g.qg_checksum.restype = u64
+        _G = g
+    return _G
+
+
+def gen(name, seed, n, dtype):
+    """n values of bench_gen.c's qg_<name> column from `seed`."""
+    out = np.empty(n, dtype=dtype)
+    getattr(_gen(), "qg_" + name)(seed, n, out.ctypes.data)
+    return out
+
+
+def gen_kbits(seed, n, k):
+    out = np.empty(n, dtype=np.uint32)
+    _gen().qg_kbits_u32(seed, n, k, out.ctypes.data)
+    return out
+
+
+def gen_validity(seed, n, p_valid, values=None):
+    """A Bernoulli(p_valid) bitmap (LSB first); null slots of `values` (4-byte
+    elements) are zeroed."""
+    bm = np.empty((n + 7) // 8, dtype=np.uint8)
+    _gen().qg_validity(seed, n, p_valid, bm.ctypes.data, values.ctypes.data if values is not None else None)
+    return bm
+
+
+def checksum(a):
+    a = np.ascontiguousarray(a)
+    return "%016x" % _gen().qg_checksum(a.ctypes.data, a.nbytes)
+
+
+def column_checksums(cols):
+    """name -> checksum of the values (and of the bitmap, when there is one)."""
+    out = {}
+    for c in cols:
+        name, v = c[0], c[1]
+        bm = c[2] if len(c) > 2 else None
+        out[name] = checksum(v) + ("/" + checksum(bm) if bm is not None else "")
+    return out
+
+
+def make_workload(n, seed=100):
+    """The integer-lanes columns: n i64 timestamps (t_0 = 1.7e18, steps uniform
+    in [1, 1000]) and three n u32 columns of uniform k-bit values, k in
+    {3, 11, 23}, each with a Bernoulli(0.99) validity bitmap (null slots 0).
+    Column j's values come from seed + j, its bitmap from seed + 10 + j."""
+    cols = [("ts_i64", gen("timestamps_i64", seed, n, np.int64), None)]
+    for j, k in enumerate((3, 11, 23), 1):
+        v = gen_kbits(seed + j, n, k)
+        bm = gen_validity(seed + 10 + j, n, 0.99, v)
         cols.append((f"u32_k{k}", v, bm))
     return cols
 
 
-def measured_traffic(stage, args):
-    """HBM bytes per call of a tile stage on this workload, from the committed
-    counter summary (profiles/traffic_<workload>.json: rocprofv3 FETCH_SIZE
-    and WRITE_SIZE passes, scaled by the same counters on a plain copy of
-    known size), or None when no summary matches this configuration."""
-    path = os.path.join(ROOT, "profiles", f"traffic_intlanes_{args.level}.json")
+def measured_traffic(stage, args=None, name=None):
+    """HBM bytes per call of a stage, from the committed counter summary
+    (profiles/traffic_<name>.json: rocprofv3 FETCH_SIZE and WRITE_SIZE passes,
+    scaled by the same counters on a plain copy of known size), or None when
+    no summary matches this configuration. The integer-lanes summaries are
+    per level (name from args) and must match the column length."""
+    fname = name or f"intlanes_{args.level}"
+    path = os.path.join(ROOT, "profiles", f"traffic_{fname}.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
     st = t.get("stages", {}).get(stage)
-    if t.get("values_per_column") != args.n or not st:
+    if not st or (name is None and t.get("values_per_column") != args.n):
         return None
     return st.get("hbm_bytes_per_call")
 
@@ -116,7 +179,9 @@ def cpu_baseline(cols, level, runs=5, one_thread_values=4 << 20, label=""):
     raw1 = sum(v.nbytes for v, _, _ in sample)
     t1 = sorted(oracle_round_trip(sample, level, 1) for _ in range(3))
     med, med1 = times[len(times) // 2], t1[1]
+    sums = [checksum(v) + ("/" + checksum(bm) if bm is not None else "") for v, _, bm in cols]
     return {"value": round(raw / med / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+            "input_checksums": sums,
             "one_thread_gbs": round(raw1 / med1 / 1e9, 4), "nproc": ncpu, "cpu_model": model,
             "runs": runs, "median_s": round(med, 4),
             "sample": f"{label}the whole workload ({raw / 1e6:.1f} MB), compress+decompress "
@@ -125,38 +190,31 @@ def cpu_baseline(cols, level, runs=5, one_thread_values=4 << 20, label=""):
                       f"({raw1 / 1e6:.1f} MB), 1 thread, median of 3"}
 
 
-def serve_pool(qa, gpu, cfg, dev, count, seed=1):
+def serve_pool(qa, gpu, cfg, dev, count, seed=400):
     """Requests of the serving workload: uncompressed sizes log-uniform in
     [4 KB, 64 MB], types f64/f32/i64/i32 in turn with the generators of the
     other workloads (f64: 2-decimal walk, uniform doubles, 256-value runs;
     f32 uniform [-1, 1); i64 timestamps; i32 uniform k-bit), half compress and
-    half decompress (of a stream made beforehand). Device buffers per request."""
+    half decompress (of a stream made beforehand). Device buffers per request.
+    Sizes come from `seed`, request i's values from seed + 1 + i."""
     import torch
-    r = np.random.default_rng(seed)
+    sizes = np.empty(count, dtype=np.float64)
+    _gen().qg_loguniform(seed, count, float(4 << 10), float(64 << 20), sizes.ctypes.data)
     pool = []
     for i in range(count):
-        nbytes = int(np.exp(r.uniform(np.log(4 << 10), np.log(64 << 20))))
+        nbytes = int(sizes[i])
         kind = ["f64", "f32", "i64", "i32"][i % 4]
         w = 4 if kind in ("f32", "i32") else 8
         n = max(nbytes // w, 1)
+        rs = seed + 1 + i  # request i's values
         if kind == "f64":
-            g = i // 4 % 3
-            if g == 0:
-                v = np.round(100 * (1000 + np.cumsum(r.normal(size=n)))) / 100
-            elif g == 1:
-                v = r.integers(0, 2**53, n, dtype=np.uint64) * 2.0**-53
-            else:
-                runs = r.geometric(1 / 8, n // 4 + 2)
-                v = np.repeat(r.integers(0, 256, len(runs)) * 0.5, runs)[:n]
-                if len(v) < n:
-                    v = np.resize(v, n)
+            v = gen(("walk_f64", "full_f64", "dict_runs_f64")[i // 4 % 3], rs, n, np.float64)
         elif kind == "f32":
-            v = ((r.integers(0, 2**24, n) * 2.0**-23) - 1).astype(np.float32)
+            v = gen("unif_f32", rs, n, np.float32)
         elif kind == "i64":
-            v = (1_700_000_000_000_000_000 + np.cumsum(r.integers(1, 1001, n))).astype(np.int64)
+            v = gen("timestamps_i64", rs, n, np.int64)
         else:
-            k = [3, 11, 23][i // 4 % 3]
-            v = (r.integers(0, 2**32, n, dtype=np.uint64) >> np.uint64(32 - k)).astype(np.int32)
+            v = gen_kbits(rs, n, [3, 11, 23][i // 4 % 3]).view(np.int32)
         v = np.ascontiguousarray(v)
         dt = qa.FROM_NP[v.dtype]
         t = torch.from_numpy(v.view(np.uint8).copy()).to(dev)
@@ -183,7 +241,11 @@ def serve_run(qa, srv, pool, cfg, rate, count, seed=2):
     Returns (GB/s, latencies in ms, wall seconds)."""
     import threading
     import queue
-    r = np.random.default_rng(seed)
+    # exponential gaps -ln(u) / rate, u uniform in (0, 1] from the generator
+    gaps = None
+    if rate:
+        raw = gen("raw", seed, count, np.uint64)
+        gaps = -np.log(((raw >> np.uint64(11)).astype(np.float64) + 1.0) * 2.0**-53) / rate
     q = queue.Queue()
     lat = []
 
@@ -205,9 +267,9 @@ def serve_run(qa, srv, pool, cfg, rate, count, seed=2):
     t0 = time.perf_counter()
     next_t = t0
     k = 0
-    for _ in range(count):
+    for j in range(count):
         if rate:
-            next_t += r.exponential(1.0 / rate)
+            next_t += gaps[j]
             d = next_t - time.perf_counter()
             if d > 0:
                 time.sleep(d)
@@ -233,7 +295,7 @@ def serve_run(qa, srv, pool, cfg, rate, count, seed=2):
     return done_bytes / wall / 1e9, lat, wall
 
 
-def serve_bench(qa, dev, budget_gb=64, slots=3, pool_size=48, per_point=800):
+def serve_bench(qa, dev, budget_gb=64, slots=3, pool_size=192, per_point=800):
     cfg = qa.Config(level=qa.Level.FAST)
     gpu = qa.Gpu(0)
     pool = serve_pool(qa, gpu, cfg, dev, pool_size)
@@ -246,7 +308,8 @@ def serve_bench(qa, dev, budget_gb=64, slots=3, pool_size=48, per_point=800):
         points = []
         for frac in (0.25, 0.5, 0.75, 0.9):
             j0, b0 = srv.stats()
-            gbs, lat, wall = serve_run(qa, srv, pool, cfg, frac * sat_rps, per_point, seed=int(frac * 100))
+            gbs, lat, wall = serve_run(qa, srv, pool, cfg, frac * sat_rps, per_point,
+                                      seed=500 + int(frac * 100))
             j1, b1 = srv.stats()
             ms = [m for m, _ in lat]
             sm = [m for m, b in lat if b <= 1 << 20] or [0.0]
@@ -273,15 +336,12 @@ def serve_bench(qa, dev, budget_gb=64, slots=3, pool_size=48, per_point=800):
             "points": points}
 
 
-def entropy_lanes(n, seed=3):
+def entropy_lanes(n, seed=300):
     """The entropy workload's two columns as f64 lanes: n f64 values
-    sin(2 pi i / 1000) * 1e3 + N(0, 1e-3) and n f32 values ((u32 >> 8) 2^-23) - 1
-    widened to f64."""
-    r = np.random.default_rng(seed)
-    i = np.arange(n, dtype=np.float64)
-    f64 = np.sin(2 * np.pi * i / 1000) * 1000 + r.normal(0, 1e-3, n)
-    u = r.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
-    f32 = (u >> 8).astype(np.float32) * np.float32(2.0**-23) - np.float32(1)
+    sin(2 pi i / 1000) * 1e3 + N(0, 1e-3) (from seed) and n f32 values
+    ((u32 >> 8) 2^-23) - 1 (from seed + 1) widened to f64."""
+    f64 = gen("sine_f64", seed, n, np.float64)
+    f32 = gen("unif_f32", seed + 1, n, np.float32)
     return f64, f32.astype(np.float64)
 
 
@@ -299,7 +359,7 @@ def linear_blocks(qa, gpu, torch, V, block, kind, order=0):
     return out, ins, spans
 
 
-def resid_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=3):
+def resid_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=300):
     """The linear residual coders (src/codecs/linear.rs) and the FCM / DFCM
     predictors (src/codecs/pred.rs) on n f64 values
     sin(2 pi i / 1000) * 1e3 + N(0, 1e-3) in 32768-value blocks, every block of
@@ -415,14 +475,16 @@ def resid_cpu_baseline(samples):
                       "oracle C, 1 thread"}
 
 
-def entropy_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=3):
+def entropy_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=300):
     """rANS decode on the entropy workload (entropy_lanes): each 32768-value
     block's Delta2 residual bytes, from the device residual coder, are rANS-coded
     by the GPU coder, then every block stream is decoded in one launch; decoded
     bytes are checked against the residuals."""
     import torch
     res, bspans = [], []
-    for lane in entropy_lanes(n, seed):
+    lanes = entropy_lanes(n, seed)
+    input_sums = {"sine_f64": checksum(lanes[0]), "unif_f32 (as f32)": checksum(lanes[1].astype(np.float32))}
+    for lane in lanes:
         V = torch.from_numpy(lane.view(np.int64)).to(dev)
         out, _, sp = linear_blocks(qa, gpu, torch, V, block, qa.LINEAR_DELTA2)
         res.append(torch.cat([out[o:o + ln] for o, ln in sp]))
@@ -473,11 +535,13 @@ def entropy_bench(qa, gpu, dev, n, steps, warmup, block=32768, seed=3):
         "entropy_layer": layer,
         "workload": "entropy decode: 32Mi f64 sine+noise and 32Mi f32 uniform[-1,1), Delta2 "
                     "residual bytes of each 32768-value block, rANS-coded (one stream per block)",
+        "inputs": {"generator": GENERATOR, "checksums": input_sums},
         "streams": len(spans), "symbols": nsym, "coded_bytes": cbytes,
         "decode_gbs": round(nsym / (k_ms * 1e-3) / 1e9, 2),
         "decode_wall_gbs": round(nsym * steps / wall / 1e9, 2),
         "encode_gbs": round(nsym / enc_s / 1e9, 2),
         "roofline": {"bound": "hbm", "kernel": "k_rans_dec", "kernel_ms": round(k_ms, 4),
+                     "traffic": measured_traffic("rans_decode", name="entropy_rans"),
                      "algorithmic_bytes": alg,
                      "achieved": round(alg / (k_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
@@ -551,17 +615,17 @@ MODE_NAMES = {0: "pred", 1: "const", 2: "stride", 3: "xorz", 4: "lz", 5: "raw",
 DEVICE_EMITTED = {1, 2, 3, 5, 9, 50, 51, 53}
 
 
-def float_columns(total=64 << 20, ncols=40, seed=4):
+def float_columns(total=64 << 20, ncols=40, seed=200):
     """The mixed-float workload: `ncols` f64 columns whose lengths are
     exp(U[ln 65536, ln 8388608]) rescaled to sum to `total` and kept in that
     range. Column i is, by i % 4: 0, 1 decimals k / 1000 with k uniform in
     [0, 10^9) (scale 10^3 in [0, 1e6)); 2 full-precision uniform [0, 1) doubles
     (u64 >> 11) 2^-53; 3 a 256-value dictionary of such decimals, written in
     runs of Geometric(1/8) length (mean 8), each run's value uniform over the
-    256."""
-    r = np.random.default_rng(seed)
+    256. Lengths come from `seed`, column i's values from seed + 1 + i."""
     lo, hi = 65536, 8388608
-    L = np.exp(r.uniform(np.log(lo), np.log(hi), ncols))
+    L = np.empty(ncols, dtype=np.float64)
+    _gen().qg_loguniform(seed, ncols, float(lo), float(hi), L.ctypes.data)
     for _ in range(50):
         L = np.clip(L * total / L.sum(), lo, hi)
     L = np.floor(L).astype(np.int64)
@@ -573,17 +637,8 @@ def float_columns(total=64 << 20, ncols=40, seed=4):
     cols = []
     for i, n in enumerate(L.tolist()):
         kind = ("decimal", "decimal", "full", "dict")[i % 4]
-        if kind == "decimal":
-            v = r.integers(0, 10**9, n) / 1000.0
-        elif kind == "full":
-            v = (r.integers(0, 2**64, n, dtype=np.uint64) >> np.uint64(11)) * 2.0**-53
-        else:
-            vals = r.integers(0, 10**9, 256) / 1000.0
-            runs = r.geometric(1 / 8, n // 4 + 64)
-            while runs.sum() < n:
-                runs = np.concatenate([runs, r.geometric(1 / 8, n // 8 + 64)])
-            v = np.repeat(vals[r.integers(0, 256, len(runs))], runs)[:n]
-        cols.append((kind, np.ascontiguousarray(v, dtype=np.float64)))
+        fn = {"decimal": "decimal_f64", "full": "full_f64", "dict": "dict_runs_f64"}[kind]
+        cols.append((kind, gen(fn, seed + 1 + i, n, np.float64)))
     return cols
 
 
@@ -662,15 +717,19 @@ def float_bench(qa, gpu, dev, steps, warmup, check_oracle=True):
         "decompress_gbs": round(raw / (dd * 1e-3) / 1e9, 2),
         "device_ms_compress": round(dc, 3), "device_ms_decompress": round(dd, 3),
         "wins": {MODE_NAMES.get(m, str(m)): int(wins[m]) for m in range(64) if wins[m]},
+        "inputs": {"generator": GENERATOR,
+                   "checksums": {f"{i}:{kind}": checksum(v) for i, (kind, v) in enumerate(cols)}},
         "oracle_checked_columns": checked,
         "roofline": {"bound": "hbm", "kernel": "tile decode: k_dec_tiles + k_dec_fl",
                      "kernel_ms": round(kd, 4), "algorithmic_bytes": alg,
+                     "traffic": measured_traffic("tile_decode", name="mixed_float_fast"),
                      "achieved": round(alg / (kd * 1e-3) / 1e9, 1) if kd > 0 else 0.0,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(alg / (kd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if kd > 0 else 0.0,
                      "decode_sequence_gbs": round(alg / (dd * 1e-3) / 1e9, 1)},
         "encode_roofline": {"bound": "hbm", "kernel": "tile encode: k_emit_fl + k_emit_tiles",
                             "kernel_ms": round(kc, 4),
+                            "traffic": measured_traffic("tile_encode", name="mixed_float_fast"),
                             "achieved": round(alg / (kc * 1e-3) / 1e9, 1) if kc > 0 else 0.0,
                             "encode_sequence_gbs": round(alg / (dc * 1e-3) / 1e9, 1)},
     }, cols
@@ -686,18 +745,17 @@ def float_cpu_baseline(cols):
 
 def default_workload(seed=0):
     """Columns of the Config::default leg: the plain-method column (1 Mi f64
-    2-decimal random walk, seed 0), and reduced-size columns of the integer
-    lanes (1 Mi per column) and entropy (2 Mi each) workloads."""
-    r = np.random.default_rng(seed)
+    2-decimal random walk, seed 0), and the first 1 Mi values of each
+    integer-lanes column and 2 Mi of each entropy column (the same
+    generators and seeds as those legs)."""
     n = 1 << 20
-    walk = np.round(100 * (1000 + np.cumsum(r.normal(size=n)))) / 100
-    cols = [("walk_f64 (1Mi, plain method)", walk, None)]
-    for name, v, bm in make_workload(1 << 20, seed=1):
+    cols = [("walk_f64 (1Mi, plain method)", gen("walk_f64", seed, n, np.float64), None)]
+    for name, v, bm in make_workload(1 << 20):
         cols.append((name + " (1Mi)", v, bm))
     m = 2 << 20
-    i = np.arange(m, dtype=np.float64)
-    cols.append(("sine_f64 (2Mi)", np.sin(2 * np.pi * i / 1000) * 1e3 + r.normal(0, 1e-3, m), None))
-    cols.append(("unif_f32 (2Mi)", r.uniform(-1, 1, m).astype(np.float32), None))
+    f64, f32 = entropy_lanes(m)
+    cols.append(("sine_f64 (2Mi)", f64, None))
+    cols.append(("unif_f32 (2Mi)", f32.astype(np.float32), None))
     return cols
 
 
@@ -785,6 +843,7 @@ def default_bench(qa, gpu, dev, steps, check_oracle=True):
         "wins": {MODE_NAMES.get(m, str(m)): int(wins[m]) for m in range(64) if wins[m]},
         "routing": {"blocks": blocks, "device_emitted": dev_blocks, "host_written": blocks - dev_blocks,
                     "host_share": round((blocks - dev_blocks) / max(blocks, 1), 4)},
+        "inputs": {"generator": GENERATOR, "checksums": column_checksums(cols)},
         "oracle_checked_columns": checked,
     }, [(v, odt[v.dtype], bm) for _, v, bm in cols]
 
@@ -1013,7 +1072,8 @@ def main():
                    "device_ms_decompress": round(dev_d_ms, 3),
                    "plain_form_gbs": round(plain, 3) if plain else None,
                    "batched_over_plain": round(value / plain, 2) if plain else None,
-                   "waves_per_workgroup": gpu.tuned_shapes()},
+                   "waves_per_workgroup": gpu.tuned_shapes(),
+                   "inputs": {"generator": GENERATOR, "checksums": column_checksums(cols)}},
         "roofline": roofline,
         "encode_roofline": enc_roofline,
         "serving": serving,
