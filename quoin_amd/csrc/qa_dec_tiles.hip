@@ -79,7 +79,7 @@ __global__ void k_dec_desc(DecArgs a) {
         d.out = col.out;
         d.bitmap = col.bitmap;
         d.off = ts.off;
-        d.tail = ts.tail;  // DeltaBitpack: replaced by the carry (k_dec_delta_sums/scan)
+        d.tail = ts.tail;  // DeltaBitpack: the carry (k_dec_delta_sums, k_dec_delta_scan)
         d.payload = fr.payload_off;
         d.p0 = (uint64_t)lt * QA_TILE;
         d.base = fr.out_off + d.p0;

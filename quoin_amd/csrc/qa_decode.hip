@@ -419,10 +419,9 @@ __device__ __forceinline__ uint64_t vb_word_from(const uint64_t *rs, uint64_t ba
 // every lane the last run starting at or before its word, so long runs cost
 // nothing extra. A wavefront spanning two streams searches per word. Raw
 // blobs were written by k_vb_scan.
-__global__ void __launch_bounds__(256) k_vb_words(DecArgs a) {
-    __shared__ uint32_t marks[4][WAVE];
-    const int lane = threadIdx.x & 63, wv = qa_wave_id();
-    const uint64_t j0 = ((uint64_t)blockIdx.x * 4 + wv) * WAVE;
+__device__ __forceinline__ void vb_words_wave(const DecArgs &a, uint64_t wi, uint32_t *mk) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t j0 = wi * WAVE;
     if (j0 >= a.nvwords) return;
     const uint64_t j = j0 + lane;
     if (*a.err) {
@@ -454,7 +453,6 @@ __global__ void __launch_bounds__(256) k_vb_words(DecArgs a) {
                 lo = plo;
             }
         }
-        uint32_t *mk = marks[wv];
         mk[lane] = 0;
         wave_lds_sync();
         const uint64_t Bend = B0 + 64 * WAVE;
@@ -579,23 +577,23 @@ __device__ __forceinline__ T wsum(T v) {
 }
 
 // per DeltaBitpack tile: the wrapping sum of its unzigzagged deltas, into the
-// tile descriptor's tail (k_dec_delta_scan turns the sums into carries)
-__global__ void __launch_bounds__(256) k_dec_delta_sums(DecArgs a) {
-    // staged words of a w <= 32 sub-block (32 w, plus the row-word read past the
-    // last one); wider sub-blocks (rare) are read directly
-    __shared__ __align__(16) uint32_t wbuf[4][1088];
-    const int lane = threadIdx.x & 63, wv = qa_wave_id();
-    uint64_t t = a.t0 + (uint64_t)blockIdx.x * 4 + wv;
-    if (t >= a.t1) return;
-    TileDec &td = a.tdec[t];
-    if (td.mode != QA_M_DELTA) return;  // also every tile once an error is flagged
-    const uint32_t cnt = td.cnt;
-    const uint8_t *h = td.src + td.off;
-    const uint32_t w = td.w;
+// tile map's tail (k_dec_delta_scan turns the sums into carries). Reads only
+// what k_dec_index wrote: the tile's frame, sub-block offset and width. wd
+// stages the words of a w <= 32 sub-block (32 w, plus the row word read past
+// the last one); wider sub-blocks (rare) are read directly.
+__device__ __forceinline__ void delta_sum_tile(const DecArgs &a, uint64_t t, uint32_t *wd) {
+    const int lane = threadIdx.x & 63;
+    DTileD &ts = a.tiles[t];
+    const uint32_t f = ts.frame;
+    const DFrame &fr = a.frames[f];
+    if (fr.mode != QA_M_DELTA || *a.err || !frame_used(a, f)) return;
+    const uint32_t lt = (uint32_t)(t - fr.tile_base);
+    const uint32_t cnt = min((uint32_t)QA_TILE, fr.n - lt * QA_TILE);
+    const uint8_t *h = a.cols[fr.col].src + ts.off;
+    const uint32_t w = ts.w;
     uint64_t mn = qa_ld64u(h + 1);
     uint64_t s = 0;
     if (w <= 32) {
-        uint32_t *wd = wbuf[wv];
         stage_words(h + 9, 128 * w, wd, lane);
         wave_lds_sync();
         uint32_t v[16];
@@ -607,22 +605,62 @@ __global__ void __launch_bounds__(256) k_dec_delta_sums(DecArgs a) {
         for (uint32_t k = lane; k < cnt; k += WAVE) s += qa_unzigzag(for_value_direct(h, k));
     }
     s = wsum(s);
-    if (lane == 0) td.tail = s;
+    if (lane == 0) ts.tail = s;
 }
 
-// one wavefront per frame: exclusive scan of the tile sums, seeded with the base
+// the Delta tiles [t0, t1) of the launch range, one wavefront per tile
+__device__ __forceinline__ bool delta_span(const DecArgs &a, uint64_t *t0, uint64_t *t1) {
+    *t0 = a.t0 > a.dt0 ? a.t0 : a.dt0;
+    *t1 = a.t1 < a.dt1 ? a.t1 : a.dt1;
+    return *t1 > *t0;
+}
+
+__global__ void __launch_bounds__(256) k_dec_delta_sums(DecArgs a) {
+    __shared__ __align__(16) uint32_t wbuf[4][1088];
+    uint64_t t0, t1;
+    if (!delta_span(a, &t0, &t1)) return;
+    const uint64_t t = t0 + (uint64_t)blockIdx.x * 4 + qa_wave_id();
+    if (t < t1) delta_sum_tile(a, t, wbuf[qa_wave_id()]);
+}
+
+// The bitmap words and the Delta tile sums in one launch: workgroups
+// [0, nbw) take 256 bitmap words each (k_vb_words), the rest four Delta tiles
+// each. The two read disjoint inputs and write disjoint outputs, and both
+// need only k_vb_scan and k_dec_index before them.
+__global__ void __launch_bounds__(256) k_vb_words_dsums(DecArgs a, uint32_t nbw) {
+    __shared__ __align__(16) uint32_t lds[4][1088];
+    const int wv = qa_wave_id();
+    if (blockIdx.x < nbw) {
+        vb_words_wave(a, (uint64_t)blockIdx.x * 4 + wv, lds[wv]);
+        return;
+    }
+    uint64_t t0, t1;
+    if (!delta_span(a, &t0, &t1)) return;
+    const uint64_t t = t0 + (uint64_t)(blockIdx.x - nbw) * 4 + wv;
+    if (t < t1) delta_sum_tile(a, t, lds[wv]);
+}
+
+__global__ void __launch_bounds__(256) k_vb_words(DecArgs a) {
+    __shared__ uint32_t marks[4][WAVE];
+    vb_words_wave(a, (uint64_t)blockIdx.x * 4 + qa_wave_id(), marks[qa_wave_id()]);
+}
+
+// one wavefront per Delta frame: exclusive scan of its tile sums, seeded with
+// the base, into the tile map (k_dec_desc copies each carry into its tile's
+// descriptor)
 __global__ void __launch_bounds__(256) k_dec_delta_scan(DecArgs a) {
     const int lane = threadIdx.x & 63;
-    uint32_t f = a.f0 + blockIdx.x * 4 + qa_wave_id();
-    if (f >= a.f1 || *a.err) return;
+    const uint32_t lo = a.f0 > a.df0 ? a.f0 : a.df0;
+    uint32_t f = lo + blockIdx.x * 4 + qa_wave_id();
+    if (f >= a.f1 || f >= a.df1 || *a.err) return;
     const DFrame &fr = a.frames[f];
     if (fr.mode != QA_M_DELTA || !frame_used(a, f)) return;
     uint64_t carry = qa_ld64u(a.cols[fr.col].src + fr.payload_off);  // base
     for (uint32_t t0 = 0; t0 < fr.ntiles; t0 += WAVE) {
         uint32_t t = t0 + lane;
-        uint64_t s = t < fr.ntiles ? a.tdec[fr.tile_base + t].tail : 0;
+        uint64_t s = t < fr.ntiles ? a.tiles[fr.tile_base + t].tail : 0;
         const uint64_t inc = qa_wave_scan_add64(s);
-        if (t < fr.ntiles) a.tdec[fr.tile_base + t].tail = carry + inc - s;
+        if (t < fr.ntiles) a.tiles[fr.tile_base + t].tail = carry + inc - s;
         carry += __shfl(inc, 63, 64);
     }
 }
@@ -670,12 +708,16 @@ void qa_decode_index_launch(const DecArgs &a, hipStream_t s) {
     qa_fast_dec_index_launch(a, s);
 }
 
-// per frame [f0, f1) / tile [t0, t1): Delta carries, tile decode
+// per frame [f0, f1) / tile [t0, t1): Delta sums and carries, tile decode
 void qa_decode_values_launch(const DecArgs &a, hipStream_t s) {
     if (a.t1 <= a.t0) return;
+    const uint64_t t0 = a.t0 > a.dt0 ? a.t0 : a.dt0, t1 = a.t1 < a.dt1 ? a.t1 : a.dt1;
+    if (t1 > t0) {
+        QA_LAUNCH(k_dec_delta_sums, dim3(nbk(t1 - t0, 4)), dim3(256), 0, s, a);
+        const uint32_t f0 = a.f0 > a.df0 ? a.f0 : a.df0, f1 = a.f1 < a.df1 ? a.f1 : a.df1;
+        if (f1 > f0) QA_LAUNCH(k_dec_delta_scan, dim3(nbk(f1 - f0, 4)), dim3(256), 0, s, a);
+    }
     qa_dec_desc_launch(a, s);
-    QA_LAUNCH(k_dec_delta_sums, dim3(nbk(a.t1 - a.t0, 4)), dim3(256), 0, s, a);
-    QA_LAUNCH(k_dec_delta_scan, dim3(nbk(a.f1 - a.f0, 4)), dim3(256), 0, s, a);
     qa_dec_tiles_launch(a, s);
 }
 
@@ -685,10 +727,16 @@ void qa_decode_final_launch(const DecArgs &a, hipStream_t s) {
     QA_LAUNCH(k_dec_allvalid, dim3(nbk(a.navbytes, 256)), dim3(256), 0, s, a);
 }
 
-// the batched form: every frame of every stream in one launch per stage
+// the batched form: every frame of every stream in one launch per stage; the
+// bitmap words and the Delta tile sums share one launch after the index
 void qa_decode_launch(const DecArgs &a, hipStream_t s) {
-    qa_decode_prep_launch(a, s);
+    QA_LAUNCH(k_vb_scan, dim3(a.nvchunks ? a.nvchunks : 1), dim3(256), 0, s, a);
     qa_decode_index_launch(a, s);
-    qa_decode_values_launch(a, s);
+    const uint32_t nbw = a.nvbytes ? nbk(a.nvwords, 4 * WAVE) : 0;
+    const uint32_t nbs = a.dt1 > a.dt0 ? nbk(a.dt1 - a.dt0, 4) : 0;
+    QA_LAUNCH(k_vb_words_dsums, dim3(nbw + nbs), dim3(256), 0, s, a, nbw);
+    if (a.df1 > a.df0) QA_LAUNCH(k_dec_delta_scan, dim3(nbk(a.df1 - a.df0, 4)), dim3(256), 0, s, a);
+    qa_dec_desc_launch(a, s);
+    qa_dec_tiles_launch(a, s);
     qa_decode_final_launch(a, s);
 }

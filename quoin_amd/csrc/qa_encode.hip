@@ -49,11 +49,15 @@ __device__ __forceinline__ uint32_t lkind_of(const DCol &col) {
     }
 }
 
+// One thread per column: the column's state, written whole (every field a
+// later stage accumulates into starts at zero), and the batch's mode-win
+// counters cleared; no kernel before this one touches either.
 __global__ void k_col_counts(EncArgs a) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < 64) a.wins[c] = 0;
     if (c >= (int)a.ncols) return;
     const DCol &col = a.cols[c];
-    DColState &st = a.cst[c];
+    DColState st{};
     if (col.nvwords) {
         uint64_t nv = a.vpre[col.vword_base + col.nvwords] - a.vpre[col.vword_base];
         st.nv = nv;
@@ -69,6 +73,7 @@ __global__ void k_col_counts(EncArgs a) {
     // without nulls the kernels read the caller's values in place
     st.lanep = st.has_nulls ? (const void *)((const uint8_t *)a.lane + col.lane_off * 8)
                             : col.values;
+    a.cst[c] = st;
     if (c == (int)a.ncols - 1) {
         uint64_t T = a.nvwords ? a.tpre[a.nvwords] : 0;
         a.colbase[a.ncols] = T;
@@ -940,6 +945,7 @@ __global__ void __launch_bounds__(256) k_col_layout(EncArgs a) {
     const int c = blockIdx.x;
     const DCol &col = a.cols[c];
     DColState &st = a.cst[c];
+    if (c == 0 && threadIdx.x == 0) a.tlist[0] = 0;  // k_emit_tiles' list, filled after the layout
     __shared__ uint64_t frames_off_s;
     __shared__ uint64_t buf[256];
     if (threadIdx.x == 0) {
@@ -987,9 +993,9 @@ __global__ void __launch_bounds__(256) k_col_layout(EncArgs a) {
 // ---------------------------------------------------------------------------
 // stage 7: emission
 // ---------------------------------------------------------------------------
-__global__ void k_emit_header(EncArgs a) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (int)a.ncols) return;
+__device__ __forceinline__ void emit_header_one(const EncArgs &a, uint64_t ci) {
+    if (ci >= a.ncols) return;
+    const int c = (int)ci;
     const DCol &col = a.cols[c];
     const DColState &st = a.cst[c];
     if (st.err) return;
@@ -1008,10 +1014,10 @@ __global__ void k_emit_header(EncArgs a) {
     }
 }
 
-__global__ void k_emit_vruns(EncArgs a) {
+// run slots s0, s0 + stride, ... of the device-side run count
+__device__ __forceinline__ void emit_vruns_from(const EncArgs &a, uint64_t s0, uint64_t stride) {
     uint64_t total = a.colbase[a.ncols + 1];
-    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total;
-         s += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t s = s0; s < total; s += stride) {
         int c = col_by_run(a, s);
         const DCol &col = a.cols[c];
         const DColState &st = a.cst[c];
@@ -1030,8 +1036,7 @@ __global__ void k_emit_vruns(EncArgs a) {
 // raw validity blob: the bitmap bytes verbatim (src/validity.rs:63-68)
 // the raw bitmap blob [0][ceil(n / 8) bytes] of columns whose runs blob is not
 // smaller; one thread per bitmap word (8 bytes)
-__global__ void k_emit_vraw(EncArgs a) {
-    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void emit_vraw_one(const EncArgs &a, uint64_t j) {
     if (j >= a.nvwords) return;
     int c = col_by_vword(a.cols, a.ncols, j);
     const DCol &col = a.cols[c];
@@ -1044,9 +1049,9 @@ __global__ void k_emit_vraw(EncArgs a) {
     for (uint64_t bi = b0; bi < b0 + 8 && bi < raw; bi++) d[bi] = col.validity[bi];
 }
 
-__global__ void k_emit_frames(EncArgs a) {
-    uint32_t slot = a.r0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= a.r1) return;
+__device__ __forceinline__ void emit_frame_one(const EncArgs &a, uint64_t i) {
+    if (i >= (uint64_t)(a.r1 - a.r0)) return;
+    const uint32_t slot = a.r0 + (uint32_t)i;
     const DBlock &b = a.blocks[slot];
     if (b.len == 0) return;
     const DCol &col = a.cols[b.col];
@@ -1163,9 +1168,10 @@ __device__ __forceinline__ void emit_tile_t(const TileEmit &e, uint64_t *o64w, i
     wave_store_bytes(q + 9, outw, 128 * w, lane);
 }
 
-// one thread per tile: the emit descriptor (block, frame offset, minimum)
-__global__ void k_tile_desc(EncArgs a) {
-    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// one thread per tile: the emit descriptor (block, frame offset, minimum);
+// called by whole wavefronts (the list append takes one atomic per wavefront)
+__device__ __forceinline__ void tile_desc_one(const EncArgs &a, uint64_t i) {
+    const uint64_t t = a.t0 + i;
     if (t >= a.t1) return;
     TileEmit e{};
     const DTile &ts = a.tiles[t];
@@ -1230,6 +1236,39 @@ __global__ void k_tile_desc(EncArgs a) {
                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
         if (gen) a.tlist[16 + base + r] = (uint32_t)t;
     }
+}
+
+static __device__ __forceinline__ uint64_t flat_tid() {
+    return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+}
+__global__ void k_emit_header(EncArgs a) { emit_header_one(a, flat_tid()); }
+__global__ void k_emit_vruns(EncArgs a) {
+    emit_vruns_from(a, flat_tid(), (uint64_t)gridDim.x * blockDim.x);
+}
+__global__ void k_emit_vraw(EncArgs a) { emit_vraw_one(a, flat_tid()); }
+__global__ void k_emit_frames(EncArgs a) { emit_frame_one(a, flat_tid()); }
+__global__ void k_tile_desc(EncArgs a) { tile_desc_one(a, flat_tid()); }
+
+// Once the frame layout is known, the stream header, the validity blob, the
+// frame headers and the tile emit descriptors depend on nothing of each
+// other, so they share one launch of 256-thread workgroups, split by range:
+// column headers, validity runs (grid-stride over the device-side run count),
+// raw validity words, frame headers, tile descriptors.
+struct MetaSplit {
+    uint32_t h, r, w, f;  // workgroups of the first four parts; the rest: tiles
+};
+__global__ void __launch_bounds__(256) k_emit_meta(EncArgs a, MetaSplit m) {
+    uint32_t b = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    if (b < m.h) return emit_header_one(a, (uint64_t)b * 256 + tid);
+    b -= m.h;
+    if (b < m.r) return emit_vruns_from(a, (uint64_t)b * 256 + tid, (uint64_t)m.r * 256);
+    b -= m.r;
+    if (b < m.w) return emit_vraw_one(a, (uint64_t)b * 256 + tid);
+    b -= m.w;
+    if (b < m.f) return emit_frame_one(a, (uint64_t)b * 256 + tid);
+    b -= m.f;
+    tile_desc_one(a, (uint64_t)b * 256 + tid);
 }
 
 __constant__ double c_emit_fm_scales[7] = {10.0, 100.0, 1000.0, 10000.0, 100000.0, 1000000.0,
@@ -1425,7 +1464,7 @@ static void encode_counts_launch(const EncArgs &a, hipStream_t s) {
         QA_LAUNCH(k_vcount, dim3(nb(a.nvwords, 256)), dim3(256), 0, s, a);
         qa_exclusive_scan2(a.vpre, a.vpre, a.tpre, a.tpre, a.nvwords, a.scan_tmp, s);
     }
-    QA_LAUNCH(k_col_counts, dim3(nb(a.ncols, 64)), dim3(64), 0, s, a);
+    QA_LAUNCH(k_col_counts, dim3(nb(a.ncols > 64 ? a.ncols : 64, 64)), dim3(64), 0, s, a);
 }
 
 static void encode_stats_launch(const EncArgs &a, hipStream_t s) {
@@ -1476,11 +1515,8 @@ void qa_encode_layout_launch(const EncArgs &a, hipStream_t s) {
     encode_layout_rest_launch(a, s);
 }
 
-// per-block stages over [r0, r1) / [t0, t1): frame headers and payloads
-void qa_encode_emit_launch(const EncArgs &a, hipStream_t s) {
-    QA_LAUNCH(k_emit_frames, dim3(nb(a.r1 - a.r0, 64)), dim3(64), 0, s, a);
-    qa_launch(k_tlist_clear, dim3(1), dim3(64), 0, s, a);
-    QA_LAUNCH(k_tile_desc, dim3(nb(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
+// the payloads of tiles [t0, t1): k_emit_fl, k_emit_tiles, the Fast pool's
+static void encode_payload_launch(const EncArgs &a, hipStream_t s) {
     // k_emit_fl at the handle's tuned wavefronts per workgroup (tuned here on
     // the first batch with enough tiles: the stage is idempotent)
     auto emit_fl = [&](uint32_t wpb) {
@@ -1503,6 +1539,14 @@ void qa_encode_emit_launch(const EncArgs &a, hipStream_t s) {
     }
     qa_mark(a.kev1, s);
     qa_fast_emit_launch(a, s);
+}
+
+// per-block stages over [r0, r1) / [t0, t1): frame headers and payloads
+void qa_encode_emit_launch(const EncArgs &a, hipStream_t s) {
+    QA_LAUNCH(k_emit_frames, dim3(nb(a.r1 - a.r0, 64)), dim3(64), 0, s, a);
+    qa_launch(k_tlist_clear, dim3(1), dim3(64), 0, s, a);
+    QA_LAUNCH(k_tile_desc, dim3(nb(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
+    encode_payload_launch(a, s);
 }
 
 // the batched form: every block of every column in one launch per stage
@@ -1529,6 +1573,18 @@ void qa_encode_launch(const EncArgs &a, hipStream_t s, int, hipStream_t side, hi
     } else {
         encode_vruns_launch(a, s);
     }
-    encode_layout_rest_launch(a, s);
-    qa_encode_emit_launch(a, s);
+    // the frame layout (it also clears k_emit_tiles' list), then the header,
+    // validity blob, frame headers and tile descriptors in one launch
+    QA_LAUNCH(k_col_layout, dim3(a.ncols), dim3(256), 0, s, a);
+    MetaSplit m{};
+    m.h = nb(a.ncols, 256);
+    if (a.nvwords) {
+        const uint64_t nruns_max = a.ntrans_max + a.ncols;
+        m.r = nb(nruns_max, 256) < 4096 ? nb(nruns_max, 256) : 4096;
+        m.w = nb(a.nvwords, 256);
+    }
+    m.f = nb(a.r1 - a.r0, 256);
+    QA_LAUNCH(k_emit_meta, dim3(m.h + m.r + m.w + m.f + nb(a.t1 - a.t0, 256)), dim3(256), 0, s, a,
+              m);
+    encode_payload_launch(a, s);
 }

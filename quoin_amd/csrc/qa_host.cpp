@@ -503,9 +503,15 @@ extern "C" QuoinGpu *quoin_gpu_create(int device, uint64_t hbm_budget_bytes) {
     if (!g) return nullptr;
     g->device = device;
     g->budget = hbm_budget_bytes ? hbm_budget_bytes : kDefaultBudget;
+    // the timing events only time: no system-scope fence when they are recorded
+    // (such a fence writes back and invalidates every L2 in the middle of the
+    // sequence); results reach the host through copies and stream waits
+    const unsigned tf = hipEventDisableSystemFence;
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess ||
-        hipEventCreate(&g->kev0) != hipSuccess || hipEventCreate(&g->kev1) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev0, tf) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev1, tf) != hipSuccess ||
+        hipEventCreateWithFlags(&g->kev0, tf) != hipSuccess ||
+        hipEventCreateWithFlags(&g->kev1, tf) != hipSuccess ||
         hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&g->efork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&g->ejoin, hipEventDisableTiming) != hipSuccess) {
@@ -926,8 +932,8 @@ static int gpu_compress_locked(QuoinGpu *g, const QuoinConfig *cfg, const QuoinD
         a.kev1 = g->kev1;
         a.tune = g->autotune ? g->tune : nullptr;
         // a recorded sequence is one chain of nodes (no side-stream fork)
+        // (k_col_counts writes the column states and clears the win counters)
         auto launch = [&](bool recording) {
-            zero_state();
             qa_encode_launch(a, s, level, recording ? nullptr : g->side, g->efork, g->ejoin);
         };
         if (!seq_run(g, g->genc,
@@ -1244,6 +1250,18 @@ static int gpu_decompress_tiles(QuoinGpu *g, const QuoinDeviceStream *in, size_t
     a.f1 = (uint32_t)frames.size();
     a.t0 = 0;
     a.t1 = tile_base;
+    // the span of the DeltaBitpack frames and their tiles (their carries)
+    a.df0 = a.df1 = 0;
+    a.dt0 = a.dt1 = 0;
+    for (size_t f = 0; f < frames.size(); f++) {
+        if (frames[f].mode != QA_M_DELTA || !frames[f].ntiles) continue;
+        if (a.df1 == 0) {
+            a.df0 = (uint32_t)f;
+            a.dt0 = frames[f].tile_base;
+        }
+        a.df1 = (uint32_t)f + 1;
+        a.dt1 = frames[f].tile_base + frames[f].ntiles;
+    }
     // runs of consecutive tiles taken by one tile decoder, so that each is
     // launched over its own tiles only (beyond 32 runs every decoder is
     // launched over all tiles and takes its own)

@@ -306,6 +306,10 @@ struct DecArgs {
     const struct KindRun *krun;
     uint32_t nkrun, pad3;
     uint64_t ktotal;
+    // the DeltaBitpack frames [df0, df1) and their tiles [dt0, dt1) (spans
+    // that hold every Delta frame; empty when there is none)
+    uint32_t df0, df1;
+    uint64_t dt0, dt1;
 };
 
 // Which tile decoder takes a tile (from its frame's mode and the output
