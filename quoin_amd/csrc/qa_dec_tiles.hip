@@ -301,22 +301,9 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
         // is the last run with start <= p (rle.rs:45-75).
         const uint64_t R = a.fx[f].card, aux = a.fx[f].aux_off;
         const uint64_t *rs = a.scr + a.frames[f].scr;
-        // r0 = the last run with start <= p0, by a 64-ary search (rs[0] = 0)
-        uint64_t lo = 0, hi = R;
-        while (hi - lo > 1) {
-            const uint64_t span = hi - lo;
-            const uint64_t pi = lo + 1 + (span - 1) * (uint64_t)lane / WAVE;
-            const uint64_t le = __ballot(pi < hi && rs[pi] <= p0);
-            const uint32_t c = (uint32_t)__builtin_popcountll(le);
-            if (!c) {
-                hi = lo + 1;
-            } else {
-                const uint64_t plo = lo + 1 + (span - 1) * (uint64_t)(c - 1) / WAVE;
-                hi = c < WAVE ? lo + 1 + (span - 1) * (uint64_t)c / WAVE : hi;
-                lo = plo;
-            }
-        }
-        const uint64_t r0 = lo;
+        // r0 = the last run with start <= p0 (the run covering p0, recorded
+        // while the starts were summed; zero-length runs cover nothing)
+        const uint64_t r0 = a.tiles[t].run0;
         // each later run starting inside the tile marks its first position
         // with its index - r0; a prefix maximum then gives every position its run
         uint32_t *mark = wd;

@@ -523,7 +523,7 @@ __global__ void __launch_bounds__(256) k_dec_index(DecArgs a) {
         DTileD d;
         d.off = 0;
         d.tail = 0;
-        d.lpos = 0;
+        d.run0 = 0;
         d.frame = f;
         d.w = 0;
         a.tiles[fr.tile_base + t] = d;

@@ -5,7 +5,7 @@
 //                     (alp.rs:168-248), ALP-RD (alp_rd.rs:208-285), Dict
 //                     (dict.rs:154-207) and RLE (rle.rs:45-75), every length
 //                     bounds-checked against the frame
-//   k_dec_frame_wg    one workgroup per frame: XORZ varints + prefix XOR
+//   k_dec_frame_wg    one workgroup (512 threads) per frame: XORZ varints + prefix XOR
 //                     (xorz.rs:18-31) into the frame's value slots, Dict
 //                     values (raw or delta-bitpacked), RLE run lengths -> run
 //                     starts
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(256) k_dec_index_fast(DecArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// workgroup-wide scans (1024 threads)
+// workgroup-wide scans (up to 1024 threads)
 // ---------------------------------------------------------------------------
 template <typename T, typename Op>
 __device__ __forceinline__ T wg_incl_scan(T v, Op op, T *tot) {
@@ -166,7 +166,7 @@ struct OpAdd64 { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { 
 
 // FoR sub-block value k at stream sub-block header h
 
-__global__ void __launch_bounds__(1024) k_dec_frame_wg(DecArgs a) {
+__global__ void __launch_bounds__(512) k_dec_frame_wg(DecArgs a) {
     const uint32_t f = a.f0 + blockIdx.x;
     if (*a.err || f >= a.f1 || !frame_used(a, f)) return;
     const DFrame &fr = a.frames[f];
@@ -178,6 +178,7 @@ __global__ void __launch_bounds__(1024) k_dec_frame_wg(DecArgs a) {
     const uint64_t n = fr.n;
     __shared__ uint64_t carry_s;
     __shared__ uint32_t bad;
+    __shared__ __align__(16) uint32_t wst[8][1088];  // staged sub-block words, per wavefront
     if (threadIdx.x == 0) {
         carry_s = 0;
         bad = 0;
@@ -235,21 +236,70 @@ __global__ void __launch_bounds__(1024) k_dec_frame_wg(DecArgs a) {
         return;
     }
     const uint64_t m = x.card;  // dictionary size or number of runs
-    const DTileD *tl = a.tiles + fr.tile_base;
+    DTileD *tl = a.tiles + fr.tile_base;
     uint64_t acc = dict ? qa_ld64u(s + x.aux_off) : 0;  // delta base / first run start
-    for (uint64_t b0 = 0; b0 < m; b0 += blockDim.x) {
-        uint64_t i = b0 + threadIdx.x, v = 0;
-        if (i < m) {
-            uint64_t r = for_value_direct(s + tl[i / QA_TILE].tail, (uint32_t)(i % QA_TILE));
-            if (dict) {
-                v = qa_unzigzag(r);
+    // RLE: run k covers [st, st + len); the value tiles whose first position it
+    // covers record k, the last run starting at or before that position
+    auto mark_tiles = [&](uint64_t k, uint64_t st, uint64_t len) {
+        if (dict || !len) return;
+        const uint64_t e = st + len < n ? st + len : n;
+        for (uint64_t t = (st + QA_TILE - 1) / QA_TILE; t * QA_TILE < e; t++) tl[t].run0 = k;
+    };
+    // (512 threads: the 16 entries a thread holds fit its registers without
+    // scratch; a 1024-thread form spilled and was not kept)
+    // Wavefront q of the workgroup unpacks sub-block q of each group of 8
+    // (1024 entries) through LDS, so lane L holds entries 16 L .. 16 L + 15 of
+    // it: every thread then holds 16 consecutive entries, in entry order, and
+    // one workgroup scan of the thread sums places them.
+    const int lane = threadIdx.x & 63, wv = qa_wave_id();
+    uint32_t *wd = wst[wv];
+    uint64_t tot = 0;
+    for (uint64_t g0 = 0; g0 < m; g0 += 8ull * QA_TILE) {
+        const uint64_t sb = g0 / QA_TILE + wv, i0 = sb * QA_TILE + 16ull * lane;
+        uint64_t v[16], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = 0;
+        if (sb * QA_TILE < m) {
+            const uint8_t *h = s + tl[sb].tail;
+            const uint32_t w = h[0];
+            const uint64_t mn = qa_ld64u(h + 1);
+            if (w <= 32) {
+                wave_lds_sync();  // the group before is done with the staged words
+                stage_words(h + 9, 128 * w, wd, lane);
+                wave_lds_sync();
+                uint32_t u[16];
+                unpack16_u32(wd, w, lane, u);
+#pragma unroll
+                for (int k = 0; k < 16; k++) v[k] = mn + u[k];
             } else {
-                v = r > n ? n + 1 : r;  // a run past n is an error below
+#pragma unroll
+                for (int k = 0; k < 16; k++) v[k] = for_value_direct(h, 16u * lane + k);
+            }
+            // an entry is a dictionary delta, or a run length (a run past n
+            // reads as n + 1, an error below); entries past m are 0
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const uint64_t r = v[k];
+                v[k] = i0 + k >= m ? 0 : dict ? qa_unzigzag(r) : (r > n ? n + 1 : r);
+                sum += v[k];
             }
         }
-        uint64_t tot;
-        uint64_t inc = wg_incl_scan(v, OpAdd64(), &tot);
-        if (i < m) scr[i] = dict ? acc + inc : acc + inc - v;  // values / run starts
+        const uint64_t inc = wg_incl_scan(sum, OpAdd64(), &tot);
+        uint64_t run = acc + inc - sum;  // before this thread's first entry
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint64_t i = i0 + k;
+            if (i < m && sb * QA_TILE < m) {
+                if (dict) {
+                    run += v[k];
+                    scr[i] = run;  // values
+                } else {
+                    scr[i] = run;  // run starts
+                    mark_tiles(i, run, v[k]);
+                    run += v[k];
+                }
+            }
+        }
         acc += tot;
     }
     // rle.rs:60-74: the runs must cover exactly n values
@@ -261,5 +311,5 @@ static inline unsigned nbd(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 void qa_fast_dec_index_launch(const DecArgs &a, hipStream_t s) {
     if (!a.fast || a.f1 <= a.f0) return;
     qa_launch(k_dec_index_fast, dim3(nbd(a.f1 - a.f0, 4)), dim3(256), 0, s, a);
-    qa_launch(k_dec_frame_wg, dim3(a.f1 - a.f0), dim3(1024), 0, s, a);
+    qa_launch(k_dec_frame_wg, dim3(a.f1 - a.f0), dim3(512), 0, s, a);
 }

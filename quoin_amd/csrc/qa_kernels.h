@@ -192,7 +192,7 @@ struct DTileD {
     uint64_t off;    // stream offset of the tile's sub-block header (FoR-family modes, ALP)
     uint64_t tail;   // DeltaBitpack: tile sum, then carry; ALP-RD rights / Dict values /
                      // RLE lengths: sub-block header of the second for_bitpack walk
-    uint64_t lpos;   // streams with nulls: logical position of the tile's first value
+    uint64_t run0;   // RLE: the run holding the tile's first position (k_dec_frame_wg)
     uint32_t frame;  // owning frame
     uint32_t w;      // FastLanes width of the sub-block at off
 };
