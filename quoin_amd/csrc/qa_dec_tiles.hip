@@ -22,6 +22,9 @@
 // logical range (src/validity.rs:130-146).
 #include "qa_dec.h"
 
+typedef uint64_t dec_u64x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t dec_u32x4 __attribute__((ext_vector_type(4)));
+
 __constant__ double d_exp10[19] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,  1e8, 1e9,
                                    1e10, 1e11, 1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18};
 __constant__ double d_frac10[19] = {1e0,   1e-1,  1e-2,  1e-3,  1e-4,  1e-5,  1e-6,
@@ -501,7 +504,26 @@ __global__ void __launch_bounds__(64 * WPB) k_dec_fl(DecArgs a) {
     wave_lds_sync();
     const bool o32 = OW == 4 || td.oform == QA_OUT_U32;
     if (!td.nulls) {
-        if (OW == 4) {
+        // full tiles: 16 bytes a lane per store, consecutive lanes on
+        // consecutive 16 bytes (a store covers 1 KiB)
+        const bool full = cnt == QA_TILE && ((uintptr_t)((const uint8_t *)td.out +
+                                                         td.base * (o32 ? 4 : 8)) & 15) == 0;
+        if (OW == 4 && full) {
+            dec_u32x4 *o = (dec_u32x4 *)((uint32_t *)td.out + td.base);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t k = 4u * (uint32_t)lane + 256u * q;
+                __builtin_nontemporal_store(*(const dec_u32x4 *)(wd + lds_pos<4>(k)), o + (k >> 2));
+            }
+        } else if (OW == 8 && !o32 && full) {
+            dec_u64x2 *o = (dec_u64x2 *)((uint64_t *)td.out + td.base);
+            const uint64_t *vv = (const uint64_t *)wd;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const uint32_t k = 2u * (uint32_t)lane + 128u * q;
+                __builtin_nontemporal_store(*(const dec_u64x2 *)(vv + lds_pos<8>(k)), o + (k >> 1));
+            }
+        } else if (OW == 4) {
             uint32_t *o = (uint32_t *)td.out + td.base + lane;
 #pragma unroll
             for (int q = 0; q < QV; q++)
