@@ -308,7 +308,10 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
         // while the starts were summed; zero-length runs cover nothing)
         const uint64_t r0 = a.tiles[t].run0;
         // each later run starting inside the tile marks its first position
-        // with its index - r0; a prefix maximum then gives every position its run
+        // with its index - r0; a prefix maximum then gives every position its
+        // run. Marks sit at lds_pos<4> and values at lds_pos<8> (16-byte chunks
+        // swizzled per lane), so the lane-contiguous reads and writes below hit
+        // distinct banks.
         uint32_t *mark = wd;
 #pragma unroll
         for (int q = 0; q < QV; q++) mark[lane + WAVE * q] = 0;
@@ -317,26 +320,43 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
             const uint64_t j = c0 + lane;
             const uint64_t st = j < R ? rs[j] : ~0ull;
             const bool in = st < p0 + cnt;  // and st > p0: r0 is the last start <= p0
-            if (in) atomicMax(&mark[st - p0], (uint32_t)(j - r0));
+            if (in) atomicMax(&mark[lds_pos<4>((uint32_t)(st - p0))], (uint32_t)(j - r0));
             if (!(__ballot(in) >> 63)) break;
         }
         wave_lds_sync();
         uint32_t run[16], m = 0;
+        {
+            const uint32_t swz = ((uint32_t)lane >> 1) & 3;  // lds_pos<4> of 16 lane + 4 c
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            m = max(m, mark[16 * lane + i]);
-            run[i] = m;
+            for (int c = 0; c < 4; c++) {
+                const uint4 q4 = ((const uint4 *)(mark + 16 * lane))[c ^ swz];
+                const uint32_t e[4] = {q4.x, q4.y, q4.z, q4.w};
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    m = max(m, e[u]);
+                    run[4 * c + u] = m;
+                }
+            }
         }
         const uint32_t before = qa_dpp32<QA_DPP_WAVE_SHR1>(0, qa_wave_scan_max32(m));
-        wave_lds_sync();  // every lane has read its marks
+        uint64_t y[16];
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const uint32_t k = 16 * lane + i;
-            vv[k] = k < cnt && R ? qa_ld64u(s + aux + 8 * (r0 + max(run[i], before))) : 0;
+            y[i] = k < cnt && R ? qa_ld64u(s + aux + 8 * (r0 + max(run[i], before))) : 0;
+        }
+        wave_lds_sync();  // every lane has read its marks
+        {
+            const uint32_t swz = (uint32_t)lane & 7;  // lds_pos<8> of 16 lane + 2 c
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                ((uint4 *)(vv + 16 * lane))[c ^ swz] =
+                    make_uint4((uint32_t)y[2 * c], (uint32_t)(y[2 * c] >> 32), (uint32_t)y[2 * c + 1],
+                               (uint32_t)(y[2 * c + 1] >> 32));
         }
         wave_lds_sync();
 #pragma unroll
-        for (int q = 0; q < QV; q++) x[q] = vv[lane + WAVE * q];
+        for (int q = 0; q < QV; q++) x[q] = vv[lds_pos<8>(lane + WAVE * q)];
         break;
     }
     case QA_M_XORZ: {
