@@ -338,14 +338,31 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
                 }
             }
         }
-        const uint32_t before = qa_dpp32<QA_DPP_WAVE_SHR1>(0, qa_wave_scan_max32(m));
+        const uint32_t incl = qa_wave_scan_max32(m);
+        const uint32_t before = qa_dpp32<QA_DPP_WAVE_SHR1>(0, incl);
+        // the tile's runs are r0 .. r0 + nr - 1: their values are staged in LDS
+        // by coalesced loads (zero-length runs can make more than fit: then
+        // each value is read directly)
+        const uint32_t nr = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63) + 1;
+        const bool staged = R && nr <= QA_TILE && r0 + nr <= R;
         uint64_t y[16];
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint32_t k = 16 * lane + i;
-            y[i] = k < cnt && R ? qa_ld64u(s + aux + 8 * (r0 + max(run[i], before))) : 0;
-        }
         wave_lds_sync();  // every lane has read its marks
+        if (staged) {
+            stage_words(s + aux + 8 * r0, 8 * nr, wd, lane);
+            wave_lds_sync();
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t k = 16 * lane + i;
+                y[i] = k < cnt ? vv[max(run[i], before)] : 0;
+            }
+            wave_lds_sync();  // every lane has read the staged values
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t k = 16 * lane + i;
+                y[i] = k < cnt && R ? qa_ld64u(s + aux + 8 * (r0 + max(run[i], before))) : 0;
+            }
+        }
         {
             const uint32_t swz = (uint32_t)lane & 7;  // lds_pos<8> of 16 lane + 2 c
 #pragma unroll
