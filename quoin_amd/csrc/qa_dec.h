@@ -357,6 +357,78 @@ __device__ __forceinline__ void unpack16_u64(const uint32_t *wd, uint32_t w, int
     }
 }
 
+// The same unpacks from a swizzled staging (stage_words_fl): a 32-bit row
+// (w <= 32) keeps its eight 16-byte chunks at chunk ^ (row & 7), a 64-bit row
+// (w > 32) its sixteen 8-byte words at word ^ (row & 15). The lanes of one
+// read hold different rows, so they then reach different LDS banks (read in
+// place, each lane's row-words sit 128 bytes apart and share banks).
+__device__ __forceinline__ uint32_t fl_swz32(uint32_t i) {
+    return (i & ~31u) | ((((i >> 2) & 7u) ^ ((i >> 5) & 7u)) << 2) | (i & 3u);
+}
+__device__ __forceinline__ uint32_t fl_swz64(uint32_t i) {  // i: 32-bit word index
+    const uint32_t q = i >> 1, j = q >> 4;
+    return (((j << 4) | ((q & 15u) ^ (j & 15u))) << 1) | (i & 1u);
+}
+// stage_words with each word at its swizzled position (W64: the 64-bit layout)
+template <bool W64>
+__device__ __forceinline__ void stage_words_fl(const uint8_t *src, uint32_t nbytes, uint32_t *lds,
+                                               int lane) {
+    uintptr_t ad = (uintptr_t)src;
+    const uint32_t *al = (const uint32_t *)(ad & ~(uintptr_t)3);
+    const uint32_t m = (uint32_t)(ad & 3);
+    const uint32_t nwords = (nbytes + 3) / 4;
+    const uint32_t nal = (m + nbytes + 3) / 4;
+    for (uint32_t i0 = 0; i0 < nwords; i0 += 8 * WAVE) {
+        uint32_t lo[8], hi[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t i = i0 + u * WAVE + lane;
+            lo[u] = i < nwords ? al[i] : 0;
+            hi[u] = (m && i + 1 < nal) ? al[i + 1] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t i = i0 + u * WAVE + lane;
+            if (i < nwords)
+                lds[W64 ? fl_swz64(i) : fl_swz32(i)] =
+                    m ? (uint32_t)((((uint64_t)hi[u] << 32) | lo[u]) >> (8 * m)) : lo[u];
+        }
+    }
+}
+__device__ __forceinline__ void unpack16_u32_fl(const uint32_t *wd, uint32_t w, int lane,
+                                                uint32_t (&v)[16]) {
+    if (w == 0) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = 0;
+        return;
+    }
+    const uint32_t bit = (uint32_t)(lane >> 1) * w, j = bit >> 5, sh = bit & 31;
+    const uint32_t mask = w >= 32 ? 0xFFFFFFFFu : ((1u << w) - 1);
+    const uint32_t h4 = ((uint32_t)lane & 1) * 4;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const uint4 a = *(const uint4 *)(wd + j * 32 + (((h4 + c) ^ (j & 7)) << 2));
+        const uint4 b = *(const uint4 *)(wd + (j + 1) * 32 + (((h4 + c) ^ ((j + 1) & 7)) << 2));
+        v[4 * c + 0] = __builtin_amdgcn_alignbit(b.x, a.x, sh) & mask;
+        v[4 * c + 1] = __builtin_amdgcn_alignbit(b.y, a.y, sh) & mask;
+        v[4 * c + 2] = __builtin_amdgcn_alignbit(b.z, a.z, sh) & mask;
+        v[4 * c + 3] = __builtin_amdgcn_alignbit(b.w, a.w, sh) & mask;
+    }
+}
+__device__ __forceinline__ void unpack16_u64_fl(const uint32_t *wd, uint32_t w, int lane,
+                                                uint64_t (&v)[16]) {
+    const uint64_t *w64 = (const uint64_t *)wd;
+    const uint32_t bit = (uint32_t)lane * w, j = bit >> 6, sh = bit & 63;
+    const uint64_t mask = w >= 64 ? ~0ull : ((1ull << w) - 1);
+    const bool straddle = sh + w > 64;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint64_t lo = w64[j * 16 + ((uint32_t)i ^ (j & 15))];
+        const uint64_t hi = straddle ? w64[(j + 1) * 16 + ((uint32_t)i ^ ((j + 1) & 15))] : 0;
+        v[i] = (sh ? (lo >> sh) | (hi << (64 - sh)) : lo) & mask;
+    }
+}
+
 // Values y[i] (value k = 16 lane + i) of a stream without nulls to
 // out[base + k]: 16-byte stores when the lane's 16 values are all in the tile
 // and the destination is aligned, element stores otherwise.

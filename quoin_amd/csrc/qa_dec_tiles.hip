@@ -443,10 +443,10 @@ __global__ void __launch_bounds__(64 * WPB) k_dec_fl(DecArgs a) {
     if (OW == 4) {
         uint32_t y[16];
         if (w <= 32) {
-            stage_words(h + 9, 128 * w, wd, lane);
+            stage_words_fl<false>(h + 9, 128 * w, wd, lane);
             wave_lds_sync();
             uint32_t v[16];
-            unpack16_u32(wd, w, lane, v);
+            unpack16_u32_fl(wd, w, lane, v);
 #pragma unroll
             for (int i = 0; i < 16; i++) y[i] = (uint32_t)mn + v[i];
         } else {
@@ -462,17 +462,18 @@ __global__ void __launch_bounds__(64 * WPB) k_dec_fl(DecArgs a) {
         // FoR-family sub-blocks: [w][min u64][128 w B]; ALP: [e][f][exc u16][min][w][128 w B]
         const bool alp = td.mode == QA_M_ALP;
         const uint64_t base = alp ? qa_ld64u(h + 4) : mn;
-        stage_words(h + (alp ? 13 : 9), 128 * w, wd, lane);
+        if (w <= 32) stage_words_fl<false>(h + (alp ? 13 : 9), 128 * w, wd, lane);
+        else stage_words_fl<true>(h + (alp ? 13 : 9), 128 * w, wd, lane);
         wave_lds_sync();
         uint64_t y[16];
         if (w <= 32) {
             uint32_t v[16];
-            unpack16_u32(wd, w, lane, v);
+            unpack16_u32_fl(wd, w, lane, v);
 #pragma unroll
             for (int i = 0; i < 16; i++) y[i] = base + v[i];
         } else {
             uint64_t v[16];
-            unpack16_u64(wd, w, lane, v);
+            unpack16_u64_fl(wd, w, lane, v);
 #pragma unroll
             for (int i = 0; i < 16; i++) y[i] = base + v[i];
         }
