@@ -583,6 +583,26 @@ __device__ __forceinline__ uint32_t crc32c_u64(const uint32_t *t, uint32_t crc, 
     return crc;
 }
 
+// The same CRC32C, eight bytes at a time (slicing by 8): t[k][i] is the CRC
+// of byte i followed by k zero bytes, so the eight lookups are independent
+// (the decode recurrence waits on one step's CRC before its next table read).
+__device__ __forceinline__ void crc_tables8(uint32_t (*t)[256]) {
+    crc_table(t[0]);
+    __syncthreads();
+    for (int k = 1; k < 8; k++) {
+        for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
+            const uint32_t c = t[k - 1][i];
+            t[k][i] = (c >> 8) ^ t[0][c & 0xFFu];
+        }
+        __syncthreads();
+    }
+}
+__device__ __forceinline__ uint32_t crc32c_u64_s8(const uint32_t (*t)[256], uint32_t crc, uint64_t v) {
+    const uint32_t lo = (uint32_t)v ^ crc, hi = (uint32_t)(v >> 32);
+    return (t[7][lo & 0xFFu] ^ t[6][(lo >> 8) & 0xFFu]) ^ (t[5][(lo >> 16) & 0xFFu] ^ t[4][lo >> 24]) ^
+           (t[3][hi & 0xFFu] ^ t[2][(hi >> 8) & 0xFFu]) ^ (t[1][(hi >> 16) & 0xFFu] ^ t[0][hi >> 24]);
+}
+
 // block index of a flat position (tile t, slot k) of the encode tiles
 __device__ __forceinline__ uint64_t pred_local(const LinD &db, uint64_t f) {
     return (f / LE_TILE - db.tile0) * LE_TILE + f % LE_TILE;
@@ -667,9 +687,8 @@ __global__ void k_pred_zero(uint64_t *p, uint64_t n) {
 template <uint32_t PK>
 __global__ void __launch_bounds__(64) k_pred_seq(const LinD *d, const LinR *r, uint32_t nblk,
                                                  uint32_t log2, uint64_t *tables) {
-    __shared__ uint32_t ct[256];
-    crc_table(ct);
-    __syncthreads();
+    __shared__ uint32_t ct[8][256];
+    crc_tables8(ct);
     const uint32_t b = blockIdx.x * 64 + threadIdx.x;
     if (b >= nblk || r[b].status) return;
     const uint32_t mask = (1u << log2) - 1;
@@ -682,13 +701,13 @@ __global__ void __launch_bounds__(64) k_pred_seq(const LinD *d, const LinR *r, u
         if (PK == PRED_FCM) {
             const uint64_t v = x ^ tab[ctx];
             tab[ctx] = v;
-            ctx = crc32c_u64(ct, PRED_SEED, v) & mask;
+            ctx = crc32c_u64_s8(ct, PRED_SEED, v) & mask;
             return v;
         } else {
             const uint64_t v = x ^ (last + tab[ctx]);
             const uint64_t delta = v - last;
             tab[ctx] = delta;
-            ctx = crc32c_u64(ct, PRED_SEED, delta) & mask;
+            ctx = crc32c_u64_s8(ct, PRED_SEED, delta) & mask;
             last = v;
             return v;
         }
