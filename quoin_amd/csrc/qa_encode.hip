@@ -948,6 +948,11 @@ __global__ void __launch_bounds__(256) k_col_layout(EncArgs a) {
     if (c == 0 && threadIdx.x == 0) a.tlist[0] = 0;  // k_emit_tiles' list, filled after the layout
     __shared__ uint64_t frames_off_s;
     __shared__ uint64_t buf[256];
+    // the column's blocks per winning mode, added to the batch's win counters
+    // once per mode (one counter atomic per block would serialize on a few
+    // addresses); the host stages count no wins
+    __shared__ uint32_t mcount[64];
+    if (threadIdx.x < 64) mcount[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
         uint64_t vlen = 0;
         if (st.has_nulls) {
@@ -968,7 +973,10 @@ __global__ void __launch_bounds__(256) k_col_layout(EncArgs a) {
         uint64_t v = 0;
         if (lr < col.nregions) {
             const DBlock &b = a.blocks[col.region_base + lr];
-            if (b.len) v = 9 + b.payload_len;
+            if (b.len) {
+                v = 9 + b.payload_len;
+                if (!a.hybrid) atomicAdd(&mcount[b.mode & 63], 1u);
+            }
         }
         buf[threadIdx.x] = v;
         __syncthreads();
@@ -988,6 +996,9 @@ __global__ void __launch_bounds__(256) k_col_layout(EncArgs a) {
         st.stream_len = carry;
         if (carry > col.dst_cap) st.err |= QA_ERR_OVERFLOW;
     }
+    __syncthreads();
+    if (threadIdx.x < 64 && mcount[threadIdx.x])
+        atomicAdd(&a.wins[threadIdx.x], (unsigned long long)mcount[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------

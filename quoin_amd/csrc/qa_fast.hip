@@ -1449,7 +1449,6 @@ __global__ void __launch_bounds__(256) k_block_final(EncArgs a) {
     if (lane == 0) {
         b.mode = mode;
         b.payload_len = best_len;
-        if (!a.hybrid) atomicAdd(&a.wins[mode & 63], 1ull);
     }
     uint64_t tot;
     const uint64_t vn = qa_varint_len(n);
