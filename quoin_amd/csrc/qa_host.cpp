@@ -1437,8 +1437,9 @@ extern "C" int quoin_gpu_rans_compress(QuoinGpu *g, const QuoinRansInput *in, si
         return QUOIN_ERR_DEVICE;
     hipEventRecord(g->ev0, s);
     hipMemcpyAsync(g->rdesc.p, d.data(), m * sizeof(RansEncD), hipMemcpyHostToDevice, s);
+    // the 64 bytes past the last stream's region take the encoder's unused byte stores
     qa_rans_encode_launch(dp<RansEncD>(g->rdesc), dp<RansEncR>(g->rres), dp<uint64_t>(g->rtabs),
-                          dp<uint8_t>(g->rscr), (uint32_t)m, s);
+                          dp<uint8_t>(g->rscr), (uint32_t)m, dp<uint8_t>(g->rscr) + scr, s);
     hipEventRecord(g->ev1, s);
     std::vector<RansEncR> r(m);
     hipMemcpyAsync(r.data(), g->rres.p, m * sizeof(RansEncR), hipMemcpyDeviceToHost, s);

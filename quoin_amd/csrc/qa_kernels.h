@@ -385,8 +385,9 @@ hipStream_t qa_gpu_stream(const QuoinGpu *g, void *stream);
 std::recursive_mutex &qa_gpu_mutex(QuoinGpu *g);
 
 void qa_rans_decode_launch(const RansDecD *d, RansDecR *r, uint32_t ns, hipStream_t s);
+// trash: 64 device bytes outside every stream's scratch region
 void qa_rans_encode_launch(const RansEncD *d, RansEncR *r, uint64_t *tabs, uint8_t *scr, uint32_t ns,
-                           hipStream_t s);
+                           uint8_t *trash, hipStream_t s);
 
 // linear residual coders (src/codecs/linear.rs): one block of a batch
 struct LinD {

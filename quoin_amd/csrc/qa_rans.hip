@@ -682,10 +682,14 @@ __global__ void __launch_bounds__(256) k_rans_hist(const RansEncD *__restrict__ 
 
 // backward encode (rans.rs:73-97): one stream per 4 lanes, state = lane & 3;
 // the bytes go backward from the stream's scratch end, then the 4 states
+// trash: 64 bytes past every stream's scratch region. Steps that emit fewer
+// than two bytes (and steps past a stream's end) store the unused byte there,
+// so the byte stores and the symbol loads need no branch.
 __global__ void __launch_bounds__(64) k_rans_enc(const RansEncD *__restrict__ descs,
                                                  RansEncR *__restrict__ res,
                                                  const uint64_t *__restrict__ tabs,
-                                                 uint8_t *__restrict__ scr, uint32_t ns) {
+                                                 uint8_t *__restrict__ scr, uint32_t ns,
+                                                 uint8_t *__restrict__ trash) {
     __shared__ uint64_t T[EG][256];
     const int lane = threadIdx.x;
     const int g = lane >> 2, idx = lane & 3;
@@ -706,12 +710,17 @@ __global__ void __launch_bounds__(64) k_rans_enc(const RansEncD *__restrict__ de
     uint64_t P = D.scr_end;
     constexpr int U = 16;
     uint32_t cur[U], nxt[U];
+    // symbol loads at a clamped index (a stream with no symbols reads trash)
+    typedef const __attribute__((address_space(1))) uint8_t *gsrc_t;
+    const gsrc_t src = (gsrc_t)(n ? D.src : trash);
     auto fetch = [&](uint64_t tb, uint32_t *buf) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint64_t t = tb + u;
             const uint64_t i = ((nk - 1 - t) << 2) + idx;
-            buf[u] = t < nk && i < n ? D.src[i] : 0u;
+            const bool ok = t < nk && i < n;
+            const uint32_t v = src[ok ? i : 0];
+            buf[u] = ok ? v : 0u;
         }
     };
     fetch(0, cur);
@@ -734,8 +743,10 @@ __global__ void __launch_bounds__(64) k_rans_enc(const RansEncD *__restrict__ de
             const uint32_t incl = quad_scan(cnt, idx);
             const uint32_t tot = quad_bcast<3>(incl);
             const uint64_t at = P - (tot - incl);  // one past this lane's bytes
-            if (cnt >= 1) scr[at - 1] = (uint8_t)x;
-            if (cnt == 2) scr[at - 2] = (uint8_t)(x >> 8);
+            uint8_t *const b1 = cnt >= 1 ? scr + (at - 1) : trash + lane;
+            uint8_t *const b2 = cnt == 2 ? scr + (at - 2) : trash + lane;
+            *b1 = (uint8_t)x;
+            *b2 = (uint8_t)(x >> 8);
             P -= tot;
             if (act) x = xn;
         }
@@ -789,9 +800,9 @@ void qa_rans_decode_launch(const RansDecD *d, RansDecR *r, uint32_t ns, hipStrea
 }
 
 void qa_rans_encode_launch(const RansEncD *d, RansEncR *r, uint64_t *tabs, uint8_t *scr, uint32_t ns,
-                           hipStream_t s) {
+                           uint8_t *trash, hipStream_t s) {
     if (!ns) return;
     qa_launch(k_rans_hist, dim3(ns), dim3(256), 0, s, d, r, tabs);
-    qa_launch(k_rans_enc, dim3((ns + EG - 1) / EG), dim3(64), 0, s, d, r, tabs, scr, ns);
+    qa_launch(k_rans_enc, dim3((ns + EG - 1) / EG), dim3(64), 0, s, d, r, tabs, scr, ns, trash);
     qa_launch(k_rans_copy, dim3(ns), dim3(256), 0, s, d, r, scr);
 }
