@@ -86,13 +86,19 @@ __device__ __noinline__ double pred_sse(uint32_t eff, double h0, double h1, doub
 // predict_f64: pred = 0.0; pred += c_k h_k for k < eff, h0 the latest value.
 // Without a NaN anywhere the hardware steps are the SSE2 steps; NaN is
 // absorbing, so a non-NaN result proves no step needed the rule.
+// Each term is computed and then selected (eff differs between the lanes'
+// blocks), so the recurrence's step has no branch; a term that is not
+// selected cannot reach the result.
 __device__ __forceinline__ double lin_pred(uint32_t eff, double h0, double h1, double h2,
                                            double h3) {
-    if (eff == 0) return 0.0;
     double p = 0.0 + (double)eff * h0;
-    if (eff >= 2) p = p + lin_coef(eff, 1) * h1;
-    if (eff >= 3) p = p + lin_coef(eff, 2) * h2;
-    if (eff >= 4) p = p - h3;
+    const double p2 = p + lin_coef(eff, 1) * h1;
+    p = eff >= 2 ? p2 : p;
+    const double p3 = p + lin_coef(eff, 2) * h2;
+    p = eff >= 3 ? p3 : p;
+    const double p4 = p - h3;
+    p = eff >= 4 ? p4 : p;
+    p = eff == 0 ? 0.0 : p;
     if (__builtin_expect(is_nan(p), 0)) p = pred_sse(eff, h0, h1, h2, h3);
     return p;
 }
@@ -506,13 +512,16 @@ __global__ void __launch_bounds__(64) k_lin_seq(const LinD *d, const LinR *r, ui
     constexpr int SEQ_AHEAD = 32;
     uint64_t i = 4;
     uint64_t x[SEQ_AHEAD];
+    // loads at a clamped index (n > 4 here whenever the loop runs), no branch
 #pragma unroll
     for (int k = 0; k < SEQ_AHEAD; k++) x[k] = i + k < n ? out[i + k] : 0;
     for (; i + SEQ_AHEAD <= n; i += SEQ_AHEAD) {
 #pragma unroll
         for (int k = 0; k < SEQ_AHEAD; k++) {
             out[i + k] = step(x[k], order);
-            x[k] = i + SEQ_AHEAD + k < n ? out[i + SEQ_AHEAD + k] : 0;
+            const uint64_t j = i + SEQ_AHEAD + k;
+            const uint64_t v = out[j < n ? j : n - 1];
+            x[k] = j < n ? v : 0;
         }
     }
     for (int k = 0; k < SEQ_AHEAD; k++)
