@@ -30,6 +30,8 @@
 
 #include "qa_kernels.h"
 
+#include <type_traits>
+
 #define LIN_OD 7u
 #define LIN_D2 20u
 #define LIN_DP 45u
@@ -101,6 +103,22 @@ __device__ __forceinline__ double lin_pred(uint32_t eff, double h0, double h1, d
     p = eff == 0 ? 0.0 : p;
     if (__builtin_expect(is_nan(p), 0)) p = pred_sse(eff, h0, h1, h2, h3);
     return p;
+}
+
+// lin_pred at a constant effective order E: the same operations in the same
+// order, with the coefficients as constants
+template <int E>
+__device__ __forceinline__ double lin_pred_c(double h0, double h1, double h2, double h3) {
+    if constexpr (E == 0) {
+        return 0.0;
+    } else {
+        double p = 0.0 + (double)E * h0;
+        if constexpr (E >= 2) p = p + (E == 2 ? -1.0 : E == 3 ? -3.0 : -6.0) * h1;
+        if constexpr (E >= 3) p = p + (E == 3 ? 1.0 : 4.0) * h2;
+        if constexpr (E >= 4) p = p - h3;
+        if (__builtin_expect(is_nan(p), 0)) p = pred_sse(E, h0, h1, h2, h3);
+        return p;
+    }
 }
 
 // the block of a batch owning global tile t: the last b with tile0 <= t
@@ -486,8 +504,8 @@ __global__ void __launch_bounds__(64) k_lin_seq(const LinD *d, const LinR *r, ui
     uint64_t *out = d[b].out;
     double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
     uint64_t prev = 0;
-    auto step = [&](uint64_t x, uint32_t eff) -> uint64_t {
-        const double p = lin_pred(eff, h0, h1, h2, h3);
+    // one step with the prediction p
+    auto step_p = [&](uint64_t x, double p) -> uint64_t {
         uint64_t bits;
         double v;
         if (KIND == LIN_D2) {
@@ -506,26 +524,49 @@ __global__ void __launch_bounds__(64) k_lin_seq(const LinD *d, const LinR *r, ui
         return bits;
     };
     const uint64_t nw = n < 4 ? n : 4;
-    for (uint64_t i = 0; i < nw; i++) out[i] = step(out[i], min(order, (uint32_t)i));
-    // the residual words are loaded SEQ_AHEAD values before they are used, so
-    // memory latency overlaps the f64 dependency chain
+    for (uint64_t i = 0; i < nw; i++)
+        out[i] = step_p(out[i], lin_pred(min(order, (uint32_t)i), h0, h1, h2, h3));
+    // the rest at the block's order: a constant when every block of the
+    // wavefront has the same one. The residual words are loaded SEQ_AHEAD
+    // values before they are used, so memory latency overlaps the f64
+    // dependency chain; loads at a clamped index (n > 4 whenever the loop
+    // runs), so the step has no branch.
     constexpr int SEQ_AHEAD = 32;
-    uint64_t i = 4;
-    uint64_t x[SEQ_AHEAD];
-    // loads at a clamped index (n > 4 here whenever the loop runs), no branch
+    auto run = [&](auto E) {
+        constexpr int e = decltype(E)::value;
+        auto pred = [&]() -> double {
+            if constexpr (e < 0) return lin_pred(order, h0, h1, h2, h3);
+            else return lin_pred_c<e>(h0, h1, h2, h3);
+        };
+        uint64_t i = 4;
+        uint64_t x[SEQ_AHEAD];
 #pragma unroll
-    for (int k = 0; k < SEQ_AHEAD; k++) x[k] = i + k < n ? out[i + k] : 0;
-    for (; i + SEQ_AHEAD <= n; i += SEQ_AHEAD) {
+        for (int k = 0; k < SEQ_AHEAD; k++) x[k] = i + k < n ? out[i + k] : 0;
+        for (; i + SEQ_AHEAD <= n; i += SEQ_AHEAD) {
 #pragma unroll
-        for (int k = 0; k < SEQ_AHEAD; k++) {
-            out[i + k] = step(x[k], order);
-            const uint64_t j = i + SEQ_AHEAD + k;
-            const uint64_t v = out[j < n ? j : n - 1];
-            x[k] = j < n ? v : 0;
+            for (int k = 0; k < SEQ_AHEAD; k++) {
+                out[i + k] = step_p(x[k], pred());
+                const uint64_t j = i + SEQ_AHEAD + k;
+                const uint64_t v = out[j < n ? j : n - 1];
+                x[k] = j < n ? v : 0;
+            }
         }
+        for (int k = 0; k < SEQ_AHEAD; k++)
+            if (i + k < n) out[i + k] = step_p(x[k], pred());
+    };
+    const uint32_t o0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)order);
+    if (__ballot(order != o0) == 0) {
+        switch (o0) {
+        case 0: run(std::integral_constant<int, 0>{}); break;
+        case 1: run(std::integral_constant<int, 1>{}); break;
+        case 2: run(std::integral_constant<int, 2>{}); break;
+        case 3: run(std::integral_constant<int, 3>{}); break;
+        case 4: run(std::integral_constant<int, 4>{}); break;
+        default: run(std::integral_constant<int, -1>{});
+        }
+    } else {
+        run(std::integral_constant<int, -1>{});
     }
-    for (int k = 0; k < SEQ_AHEAD; k++)
-        if (i + k < n) out[i + k] = step(x[k], order);
 }
 
 template <uint32_t KIND>
