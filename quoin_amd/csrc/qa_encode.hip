@@ -851,9 +851,15 @@ __global__ void __launch_bounds__(256) k_grown_feats(EncArgs a) {
     } else {
         QA_LANE_DISPATCH(grown_samples, lk, false, false, lp, lk, s0, n, seen, &mn, &mx, &dz);
     }
-    atomicMin((unsigned long long *)&smin, mn);
-    atomicMax((unsigned long long *)&smax, mx);
-    atomicMax((unsigned long long *)&sdz, dz);
+    // one LDS atomic per wavefront (all 256 threads on one address serialize)
+    mn = wave_reduce(mn, OpMinU());
+    mx = wave_reduce(mx, OpMaxU());
+    dz = wave_reduce(dz, OpMaxU());
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin((unsigned long long *)&smin, mn);
+        atomicMax((unsigned long long *)&smax, mx);
+        atomicMax((unsigned long long *)&sdz, dz);
+    }
     // the consecutive-equal count combines from the tiles
     const DTile *tl = a.tiles + col.tile_base + (uint64_t)(slot - col.region_base) * col.tpr;
     uint32_t eq = 0;
@@ -867,7 +873,8 @@ __global__ void __launch_bounds__(256) k_grown_feats(EncArgs a) {
     __syncthreads();
     uint32_t d = 0;
     for (int k = threadIdx.x; k < 256; k += blockDim.x) d += (uint32_t)__popcll(seen[k]);
-    atomicAdd(&dsum, d);
+    d = (uint32_t)wave_reduce((uint64_t)d, OpAdd());
+    if ((threadIdx.x & 63) == 0) atomicAdd(&dsum, d);
     __syncthreads();
     if (threadIdx.x == 0) {
         bool dl = dsum < 2048;

@@ -960,6 +960,21 @@ __device__ uint64_t dict_bucket_bound(const DColState &st, uint64_t s0, uint64_t
         }                                                            \
     } while (0)
 
+// out[*pos++] = v for the lanes that keep theirs, in any order: one LDS atomic
+// per wavefront (one per lane serializes on the counter)
+template <typename T>
+__device__ __forceinline__ void wave_append(bool keep, T v, T *out, unsigned int *pos) {
+    const uint64_t m = __ballot(keep);
+    if (!m) return;
+    const int lane = threadIdx.x & 63, lead = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if (lane == lead) base = atomicAdd(pos, (uint32_t)__popcll(m));
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, lead);
+    const uint32_t r =
+        __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    if (keep) out[base + r] = v;
+}
+
 // A block grown past one region (131072 values) is too much for one
 // workgroup: each of its region slots builds the exact distinct set of its
 // part in LDS and leaves it in dparts (dpart[slot] = count | has-max << 31, or
@@ -995,7 +1010,7 @@ __global__ void __launch_bounds__(1024) k_dict_part(EncArgs a) {
     }
     uint64_t *out = a.dparts + (uint64_t)slot * DH_LIMIT;
     for (uint32_t i = threadIdx.x; i < DH_CAP; i += blockDim.x)
-        if (keys[i] != HEMPTY) out[atomicAdd(&wpos, 1u)] = keys[i];
+        wave_append(keys[i] != HEMPTY, keys[i], out, &wpos);
     if (threadIdx.x == 0) a.dpart[slot] = card_l | (has_max << 31);
 }
 
@@ -1232,7 +1247,7 @@ __global__ void __launch_bounds__(1024) k_dict_set(EncArgs a) {
                 uint32_t np2 = 1;
                 while (np2 < card) np2 <<= 1;
                 for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x)
-                    if (gk[i] != HEMPTY) buf[atomicAdd(&wpos, 1u)] = gk[i];
+                    wave_append(gk[i] != HEMPTY, gk[i], buf, &wpos);
                 __syncthreads();
                 for (uint32_t i = threadIdx.x + wpos; i < np2; i += blockDim.x) buf[i] = HEMPTY;
                 __threadfence_block();
