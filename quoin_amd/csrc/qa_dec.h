@@ -141,6 +141,25 @@ __device__ __forceinline__ void store_typed(const DDecCol &col, uint64_t idx, ui
     }
 }
 
+// bitmap word j of an all-null stream: its 64 values written as zeros (no
+// tile writes them)
+__device__ __forceinline__ void dec_nullfill_one(const DecArgs &a, uint64_t j) {
+    if (j >= a.nvwords || *a.err) return;
+    const int ci = dcol_by_vword(a, j);
+    const DDecCol &c = a.cols[ci];
+    if (a.dst[ci].nv != 0) return;
+    uint64_t i0 = (j - c.vword_base) * 64;
+    for (uint64_t i = i0; i < i0 + 64 && i < c.n; i++) store_typed(c, i, 0);
+}
+// validity_out byte g of the streams without a bitmap: all valid, trailing
+// bits cleared (src/lib.rs:503-506)
+__device__ __forceinline__ void dec_allvalid_one(const DecArgs &a, uint64_t g) {
+    if (g >= a.navbytes || *a.err) return;
+    const DDecCol &col = a.cols[dcol_by_avbyte(a, g)];
+    uint64_t b = g - col.avbyte_base, rem = col.n - b * 8;
+    col.validity_out[b] = rem >= 8 ? 0xFF : (uint8_t)((1u << rem) - 1);
+}
+
 // position of the r-th set bit (0-based) of w; r < popcount(w)
 __device__ __forceinline__ uint32_t select64(uint64_t w, uint32_t r) {
     uint32_t pos = 0, c = (uint32_t)__popc((uint32_t)w);

@@ -67,8 +67,7 @@ __device__ __forceinline__ bool dec_tile_at(const DecArgs &a, uint64_t g, uint64
 }
 
 // one thread per tile: its decode descriptor
-__global__ void k_dec_desc(DecArgs a) {
-    const uint64_t t = a.t0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void dec_desc_one(const DecArgs &a, uint64_t t) {
     if (t >= a.t1) return;
     TileDec d{};
     const DTileD &ts = a.tiles[t];
@@ -102,6 +101,20 @@ __global__ void k_dec_desc(DecArgs a) {
         }
     }
     a.tdec[t] = d;
+}
+
+__global__ void k_dec_desc(DecArgs a) {
+    dec_desc_one(a, a.t0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The descriptors, the zeros of all-null streams and the validity_out of
+// streams without a bitmap in one launch: workgroups [0, nd) take 256 tiles,
+// [nd, nd + nn) 256 bitmap words, the rest 256 validity_out bytes.
+__global__ void k_dec_desc_fill(DecArgs a, uint32_t nd, uint32_t nn) {
+    const uint32_t b = blockIdx.x;
+    if (b < nd) return dec_desc_one(a, a.t0 + (uint64_t)b * 256 + threadIdx.x);
+    if (b < nd + nn) return dec_nullfill_one(a, (uint64_t)(b - nd) * 256 + threadIdx.x);
+    dec_allvalid_one(a, (uint64_t)(b - nd - nn) * 256 + threadIdx.x);
 }
 
 __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
@@ -597,6 +610,13 @@ __global__ void __launch_bounds__(64 * WPB) k_dec_fl(DecArgs a) {
 void qa_dec_desc_launch(const DecArgs &a, hipStream_t s) {
     if (a.t1 <= a.t0) return;
     qa_launch(k_dec_desc, dim3(nbt(a.t1 - a.t0, 256)), dim3(256), 0, s, a);
+}
+
+void qa_dec_desc_fill_launch(const DecArgs &a, hipStream_t s) {
+    const uint32_t nd = a.t1 > a.t0 ? nbt(a.t1 - a.t0, 256) : 0;
+    const uint32_t nn = a.nvwords ? nbt(a.nvwords, 256) : 0;
+    const uint32_t nv = a.navbytes ? nbt(a.navbytes, 256) : 0;
+    if (nd + nn + nv) qa_launch(k_dec_desc_fill, dim3(nd + nn + nv), dim3(256), 0, s, a, nd, nn);
 }
 
 // k_dec_fl<OW> over n tiles at WPB wavefronts per workgroup

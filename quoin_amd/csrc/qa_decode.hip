@@ -669,22 +669,12 @@ __global__ void __launch_bounds__(256) k_dec_delta_scan(DecArgs a) {
 // stage 7: all-null streams (no tile writes them), validity_out without bitmap
 // ---------------------------------------------------------------------------
 __global__ void k_dec_nullfill(DecArgs a) {
-    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= a.nvwords || *a.err) return;
-    const int ci = dcol_by_vword(a, j);
-    const DDecCol &c = a.cols[ci];
-    if (a.dst[ci].nv != 0) return;
-    uint64_t i0 = (j - c.vword_base) * 64;
-    for (uint64_t i = i0; i < i0 + 64 && i < c.n; i++) store_typed(c, i, 0);
+    dec_nullfill_one(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // validity_out of streams without a bitmap: all valid, trailing bits cleared
 __global__ void k_dec_allvalid(DecArgs a) {
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.navbytes || *a.err) return;
-    const DDecCol &col = a.cols[dcol_by_avbyte(a, g)];
-    uint64_t b = g - col.avbyte_base, rem = col.n - b * 8;
-    col.validity_out[b] = rem >= 8 ? 0xFF : (uint8_t)((1u << rem) - 1);
+    dec_allvalid_one(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 static inline unsigned nbk(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
@@ -736,7 +726,8 @@ void qa_decode_launch(const DecArgs &a, hipStream_t s) {
     const uint32_t nbs = a.dt1 > a.dt0 ? nbk(a.dt1 - a.dt0, 4) : 0;
     QA_LAUNCH(k_vb_words_dsums, dim3(nbw + nbs), dim3(256), 0, s, a, nbw);
     if (a.df1 > a.df0) QA_LAUNCH(k_dec_delta_scan, dim3(nbk(a.df1 - a.df0, 4)), dim3(256), 0, s, a);
-    qa_dec_desc_launch(a, s);
+    // the tile descriptors, with the zeros of all-null streams and the
+    // validity_out of streams without a bitmap (nothing after reads them)
+    qa_dec_desc_fill_launch(a, s);
     qa_dec_tiles_launch(a, s);
-    qa_decode_final_launch(a, s);
 }

@@ -339,6 +339,7 @@ void qa_decode_values_launch(const DecArgs &a, hipStream_t s);
 void qa_decode_final_launch(const DecArgs &a, hipStream_t s);
 void qa_fast_dec_index_launch(const DecArgs &a, hipStream_t s);
 void qa_dec_desc_launch(const DecArgs &a, hipStream_t s);
+void qa_dec_desc_fill_launch(const DecArgs &a, hipStream_t s);
 void qa_dec_tiles_launch(const DecArgs &a, hipStream_t s);
 
 // rANS (src/entropy/rans.rs): per-stream descriptors of a batch
