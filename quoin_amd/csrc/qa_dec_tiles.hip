@@ -82,6 +82,9 @@ __device__ __forceinline__ void dec_desc_one(const DecArgs &a, uint64_t t) {
         d.bitmap = col.bitmap;
         d.off = ts.off;
         d.tail = ts.tail;  // DeltaBitpack: the carry (k_dec_delta_sums, k_dec_delta_scan)
+        // FloatMult: the divisor's bits, so the tile decoders read no frame state
+        if (fr.mode == QA_M_FLOAT_MULT && a.fx)
+            d.tail = (uint64_t)__double_as_longlong(d_fm_scales[a.fx[f].scale]);
         d.payload = fr.payload_off;
         d.p0 = (uint64_t)lt * QA_TILE;
         d.base = fr.out_off + d.p0;
@@ -198,7 +201,7 @@ __global__ void __launch_bounds__(256) k_dec_tiles(DecArgs a) {
                 carry += qa_readlane64(inc, 63);
             }
         } else if (td.mode == QA_M_FLOAT_MULT) {
-            const double sc = d_fm_scales[a.fx[f].scale];
+            const double sc = __longlong_as_double((long long)td.tail);  // k_dec_desc
 #pragma unroll
             for (int q = 0; q < QV; q++)
                 x[q] = (uint64_t)__double_as_longlong(__ddiv_rn((double)(int64_t)x[q], sc));
@@ -506,7 +509,7 @@ __global__ void __launch_bounds__(64 * WPB) k_dec_fl(DecArgs a) {
             for (int i = 0; i < 16; i++) y[i] += off;
         } else if (td.mode == QA_M_FLOAT_MULT) {
             // k / scale in IEEE f64 (float_mult.rs:100-133)
-            const double sc = d_fm_scales[a.fx[td.frame].scale];
+            const double sc = __longlong_as_double((long long)td.tail);  // k_dec_desc
 #pragma unroll
             for (int i = 0; i < 16; i++)
                 y[i] = (uint64_t)__double_as_longlong(__ddiv_rn((double)(int64_t)y[i], sc));

@@ -204,7 +204,7 @@ struct TileDec {
     const uint8_t *src;       // the stream
     void *out;                // typed output
     const uint64_t *bitmap;   // decoded validity words (streams with nulls)
-    uint64_t off, tail;       // DTileD.off / .tail
+    uint64_t off, tail;       // DTileD.off / .tail (FloatMult: the scale, as f64 bits)
     uint64_t payload;         // stream offset of the frame's payload
     uint64_t base;            // compact index of the tile's first value
     uint64_t p0;              // index of the tile's first value within its frame

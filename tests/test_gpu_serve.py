@@ -218,10 +218,16 @@ def test_small_requests_do_not_wait_behind_large_ones(qa):
     try:
         tickets = [srv.submit_compress(b.column(qa), cfg) for b, cfg in zip(bufs, cfgs)]
         res = [srv.wait(t) for t in tickets]
+        # the small job alone on the same (warm) server, best of three
+        alone_ms = min(srv.wait(srv.submit_compress(bufs[-1].column(qa), cfgs[-1]))[2]
+                       for _ in range(3))
     finally:
         srv.close()
     for i, (rc, n, ms) in enumerate(res):
         assert rc == qa.OK, (i, rc)
         assert bufs[i].s[:n].cpu().numpy().tobytes() == ref_s[i], f"job {i} differs from alone"
-    small_ms = res[-1][2]
-    assert small_ms < max(ms for _, _, ms in res[:-1]), (small_ms, [ms for _, _, ms in res])
+    # Queued behind the burst, the small job would finish about its own time
+    # after the large ones; on its own slot it runs beside them (their kernels
+    # share the GPU, so it may still end a little after them).
+    small_ms, large_ms = res[-1][2], max(ms for _, _, ms in res[:-1])
+    assert small_ms < large_ms + 0.5 * alone_ms, (small_ms, alone_ms, [ms for _, _, ms in res])
